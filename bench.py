@@ -1,0 +1,280 @@
+"""Benchmark of the assemble-and-solve path on one MI355X.
+
+Workload (the precision configuration): P1 Poisson on 128^3 unit-cube
+hexahedra (Q1, 2,097,152 cells, 2,146,689 dofs, 57,066,625 nonzeros),
+kappa = exp(G), G ~ N(0,1) i.i.d. per cell, f = 1, u = 0 on the boundary.
+
+A step is one full value assembly of the system on the GPU: A (FP64) with
+BC rows/columns zeroed and unit diagonal, b with lifting and bc.set. The
+sparsity pattern and the slot map are built once beforehand.
+`value` = scalar nonzeros assembled per second (GNNZ/s) over the timed
+steps. Alongside: CG time-to-1e-8 (||b - A x||/||b|| in FP64) for plain FP64
+Jacobi-PCG and for FP32 assembly + FP32 inner PCG with FP64 refinement, the
+roofline of the assembly kernel and of the FP64 SpMV, and the CPU oracle
+timed on the host.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def ensure_built():
+    if not os.path.exists(os.path.join(ROOT, "dolfinx_amd", "libdfx.so")):
+        from dolfinx_amd import build
+
+        build.build()
+
+
+def event_ms(a, b):
+    import ctypes as C
+
+    from dolfinx_amd import _lib
+
+    ms = C.c_float()
+    _lib.call("dfx_event_elapsed", a, b, C.byref(ms))
+    return ms.value
+
+
+def record(slot):
+    from dolfinx_amd import _lib
+
+    _lib.call("dfx_event_record", slot)
+
+
+def sync():
+    from dolfinx_amd import _lib
+
+    _lib.call("dfx_synchronize")
+
+
+def cpu_baseline(n, kappa_input, seed):
+    """Oracle (CPU restatement of the reference loops) on the host: one full
+    FP64 matrix assembly of the same mesh with the binary-search insert."""
+    import oracle as O
+
+    om = O.Mesh((n, n, n), O.HEX)
+    dm, nd = om.dofmap(1)
+    rp, cols = O.sparsity(dm, dm, nd)
+    kap = kappa_input[om.original_cell_index].reshape(-1, 1)
+    bn = om.boundary_nodes(1, dm)
+    mark = np.zeros(nd, np.int8)
+    mark[bn] = 1
+    t0 = time.perf_counter()
+    A = O.assemble_matrix(O.K_Q1HEX_A, om.x, om.cells, dm, 1, kap, np.zeros(1), mark, mark,
+                          rp, cols)
+    O.set_diagonal(A, bn, 1, 1.0, rp, cols)
+    t1 = time.perf_counter()
+    # a few CG iterations to price the CPU solve per iteration
+    b = O.assemble_vector(O.K_Q1HEX_L, om.x, om.cells, dm, 1, np.ones((om.num_cells, 8)),
+                          None, nd)
+    O.bc_set(b, bn, np.zeros(nd))
+    k = 10
+    t2 = time.perf_counter()
+    O.cg(rp, cols, A, 1, b, kmax=k, rtol=1e-30)
+    t3 = time.perf_counter()
+    return {
+        "value": round(len(cols) / (t1 - t0) / 1e9, 4),
+        "unit": "GNNZ/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"one FP64 Q1 matrix assembly of the {n}^3 hex mesh ({om.num_cells} cells, "
+                  f"binary-search CSR insert) = {t1 - t0:.2f} s; Jacobi-PCG "
+                  f"{(t3 - t2) / k * 1e3:.1f} ms/iteration ({k} iterations timed)",
+        "cpu_pcg_ms_per_iter": round((t3 - t2) / k * 1e3, 2),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    ensure_built()
+    from dolfinx_amd import _lib, fem, la, problems, solvers
+    from dolfinx_amd.mesh import CellType
+
+    n = args.n
+    t_setup = time.perf_counter()
+    p64 = problems.scalar_problem((n, n, n), CellType.hexahedron, kappa="exp_normal", c=0.0,
+                                  f="one", seed=args.seed, dtype=np.float64)
+    p32 = problems.scalar_problem((n, n, n), CellType.hexahedron, kappa=p64.kappa, c=0.0,
+                                  f=p64.f, msh=p64.mesh, V=p64.V, dtype=np.float32)
+    sp = fem.create_sparsity_pattern(p64.a)
+    A64 = la.MatrixCSR.from_sparsity(sp, np.float64)
+    A32 = la.MatrixCSR.from_sparsity(sp, np.float32)
+    b64 = la.Vector(p64.V.num_dofs, np.float64)
+    b32 = la.Vector(p64.V.num_dofs, np.float32)
+    setup_s = time.perf_counter() - t_setup
+    nnz = A64.nnz
+    N = p64.V.num_dofs
+    C = p64.mesh.topology.cells.shape[0]
+    Nv = p64.mesh.geometry.x.shape[0]
+    print(f"# setup {setup_s:.1f} s: cells {C} dofs {N} nnz {nnz}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        problems.assemble_system(p64, A64, b64)
+    sync()
+
+    # ---- timed region: K full FP64 assemblies ----
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        problems.assemble_system(p64, A64, b64)
+    sync()
+    t1 = time.perf_counter()
+    step_s = (t1 - t0) / args.steps
+    gnnz = nnz / step_s / 1e9
+
+    # ---- assembly kernel duration (events bracket the single launch) ----
+    reps = 5
+    kms = []
+    for _ in range(reps):
+        A64.set_value(0.0)
+        record(0)
+        _lib.call("dfx_assemble_matrix", A64.handle, p64.a.handle, p64.bc.handle)
+        record(1)
+        kms.append(event_ms(0, 1))
+    asm_ms = float(np.median(kms))
+    # algorithmic bytes of the matrix kernel: dofmap + coords + kappa + values
+    asm_bytes = C * 8 * 4 + Nv * 3 * 8 + C * 8 + nnz * 8
+    asm_gbs = asm_bytes / (asm_ms * 1e-3) / 1e9
+
+    # ---- FP32 assembly ----
+    for _ in range(2):
+        problems.assemble_system(p32, A32, b32)
+    sync()
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        problems.assemble_system(p32, A32, b32)
+    sync()
+    asm32_s = (time.perf_counter() - t2) / args.steps
+
+    # ---- SpMV roofline (FP64) ----
+    x = la.Vector.from_array(np.random.default_rng(1).standard_normal(N))
+    y = la.Vector(N)
+    for _ in range(3):
+        A64.mult(x, y)
+    record(2)
+    nsp = 20
+    for _ in range(nsp):
+        A64.mult(x, y)
+    record(3)
+    spmv_ms = event_ms(2, 3) / nsp
+    spmv_bytes = nnz * (8 + 4) + (N + 1) * 4 + 2 * N * 8 + N * 8  # y is read and written
+    spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    x32 = la.Vector.from_array(np.random.default_rng(1).standard_normal(N).astype(np.float32))
+    y32 = la.Vector(N, np.float32)
+    for _ in range(3):
+        A32.mult(x32, y32)
+    record(4)
+    for _ in range(nsp):
+        A32.mult(x32, y32)
+    record(5)
+    spmv32_ms = event_ms(4, 5) / nsp
+    spmv32_gbs = (nnz * 8 + (N + 1) * 4 + 3 * N * 4) / (spmv32_ms * 1e-3) / 1e9
+
+    out_solve = {}
+    if not args.no_solve:
+        problems.assemble_system(p64, A64, b64)
+        problems.assemble_system(p32, A32, b32)
+        sync()
+        xs = la.Vector(N)
+        r64 = solvers.cg(A64, b64, xs, rtol=1e-8, max_it=20000, jacobi=True, check_every=64)
+        x64 = xs.array
+        xm = la.Vector(N)
+        rm = solvers.mixed_refinement(A64, A32, b64, xm, rtol=1e-8, inner_rtol=1e-4,
+                                      check_every=64)
+        xmix = xm.array
+        # reference-accuracy solution for the error figures
+        xt = la.Vector(N)
+        solvers.cg(A64, b64, xt, rtol=1e-11, max_it=40000, jacobi=True, check_every=128)
+        xref = xt.array
+        nref = np.linalg.norm(xref)
+        out_solve = {
+            "cg_fp64_s": round(r64.seconds, 4),
+            "cg_fp64_iterations": r64.iterations,
+            "cg_fp64_rel_residual": r64.true_rel_residual,
+            "mixed_s": round(rm.seconds, 4),
+            "mixed_outer": rm.outer,
+            "mixed_inner_iterations": rm.inner_total,
+            "mixed_rel_residual": rm.true_rel_residual,
+            "mixed_speedup_vs_fp64": round(r64.seconds / rm.seconds, 3),
+            "err_fp64_vs_tight": float(np.linalg.norm(x64 - xref) / nref),
+            "err_mixed_vs_tight": float(np.linalg.norm(xmix - xref) / nref),
+            "err_mixed_vs_fp64": float(np.linalg.norm(xmix - x64) / np.linalg.norm(x64)),
+        }
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(n, p64.kappa, args.seed)
+
+    line = {
+        "metric": "assembly GNNZ/s + CG time-to-1e-8, mixed vs FP64 on GPU; serve p99 latency",
+        "value": round(gnnz, 4),
+        "unit": "GNNZ/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_s * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"Q1 Poisson on {n}^3 unit-cube hexahedra, kappa=exp(N(0,1)) per cell, "
+                        "f=1, u=0 on the boundary (precision configuration)",
+            "cells": C, "dofs": N, "nnz": nnz,
+            "step": "FP64 assemble_matrix (BC rows/cols zeroed, diag 1) + assemble_vector "
+                    "+ apply_lifting + set_bc",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "q1hex_a_affine_k (FP64 matrix assembly)",
+            "achieved": round(asm_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(asm_gbs / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes": asm_bytes,
+            "kernel_ms": round(asm_ms, 4),
+        },
+        "assembly_f32_gnnz_s": round(nnz / asm32_s / 1e9, 4),
+        "spmv": {
+            "f64_ms": round(spmv_ms, 4), "f64_gbs": round(spmv_gbs, 1),
+            "f64_frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
+            "f32_ms": round(spmv32_ms, 4), "f32_gbs": round(spmv32_gbs, 1),
+            "f32_frac": round(spmv32_gbs / HBM_PEAK_GBS, 4),
+        },
+        **out_solve,
+        "cpu_baseline": cpu,
+        "setup_s": round(setup_s, 2),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,980 @@
+// Cell-loop assembly on the device: hand-written element kernels for the
+// config forms, batched over all cells, scattering into a device CSR with
+// FP atomics through a precomputed cell -> CSR-slot map. Also Dirichlet
+// lifting, set_diagonal and DirichletBC::set. gfx950 (MI355X) only.
+//
+// Replaces fem::impl::assemble_cells_matrix (fem/assemble_matrix_impl.h:
+// 92-200), fem::impl::assemble_cells (fem/assemble_vector_impl.h:72-116),
+// fem::impl::lift_bc (fem/assemble_vector_impl.h:361-414) and the
+// MatrixCSR::add binary-search insert (la/matrix_csr_impl.h:67-109).
+#include "device.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace dfx
+{
+std::uint64_t next_pattern_id();
+
+namespace
+{
+
+constexpr int kBlock = 256;
+
+struct KernelInfo
+{
+  int cell;         // DFX_TET / DFX_HEX
+  int npc;          // geometry nodes per cell
+  int k;            // dofs (nodes) per cell
+  int bs;           // block size
+  bool bilinear;    // matrix (true) or vector (false) form
+};
+
+bool kernel_info(int id, KernelInfo& ki)
+{
+  switch (id)
+  {
+  case DFX_K_P1TET_A:
+    ki = {DFX_TET, 4, 4, 1, true};
+    return true;
+  case DFX_K_P1TET_L:
+    ki = {DFX_TET, 4, 4, 1, false};
+    return true;
+  case DFX_K_Q1HEX_A:
+    ki = {DFX_HEX, 8, 8, 1, true};
+    return true;
+  case DFX_K_Q1HEX_L:
+    ki = {DFX_HEX, 8, 8, 1, false};
+    return true;
+  case DFX_K_P2TET_ELAST_A:
+    ki = {DFX_TET, 4, 10, 3, true};
+    return true;
+  case DFX_K_P2TET_ELAST_L:
+    ki = {DFX_TET, 4, 10, 3, false};
+    return true;
+  case DFX_K_P3TET_A:
+    ki = {DFX_TET, 4, 20, 1, true};
+    return true;
+  case DFX_K_P3TET_L:
+    ki = {DFX_TET, 4, 20, 1, false};
+    return true;
+  default:
+    return false;
+  }
+}
+
+// ---------------------------------------------------------------------
+// Geometry helpers
+// ---------------------------------------------------------------------
+
+/// adj = det(J) J^{-1} (row-major, adj[3a + r]) for J[3r + a] = dx_r/dxi_a;
+/// returns det(J).
+template <typename T>
+__device__ inline T adjugate(const T J[9], T adj[9])
+{
+  adj[0] = J[4] * J[8] - J[5] * J[7];
+  adj[1] = J[2] * J[7] - J[1] * J[8];
+  adj[2] = J[1] * J[5] - J[2] * J[4];
+  adj[3] = J[5] * J[6] - J[3] * J[8];
+  adj[4] = J[0] * J[8] - J[2] * J[6];
+  adj[5] = J[2] * J[3] - J[0] * J[5];
+  adj[6] = J[3] * J[7] - J[4] * J[6];
+  adj[7] = J[1] * J[6] - J[0] * J[7];
+  adj[8] = J[0] * J[4] - J[1] * J[3];
+  return J[0] * adj[0] + J[1] * adj[3] + J[2] * adj[6];
+}
+
+/// Slot of (row dof i, column dof j) of each cell as an offset inside row i.
+template <typename O>
+__global__ void __launch_bounds__(kBlock)
+    slot_map_k(std::int32_t nc, int k, const std::int32_t* __restrict__ dm,
+               const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
+               O* __restrict__ slots, int* err)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  const std::int64_t kk = std::int64_t(k) * k;
+  if (t >= std::int64_t(nc) * kk)
+    return;
+  const std::int64_t c = t / kk;
+  const int e = static_cast<int>(t % kk);
+  const std::int32_t row = dm[c * k + e / k];
+  const std::int32_t col = dm[c * k + e % k];
+  std::int32_t lo = rp[row], hi = rp[row + 1];
+  const std::int32_t r0 = lo, r1 = hi;
+  while (lo < hi)
+  {
+    const std::int32_t mid = (lo + hi) >> 1;
+    if (ci[mid] < col)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if (lo >= r1 || ci[lo] != col)
+  {
+    atomicOr(err, 1);
+    slots[t] = 0;
+    return;
+  }
+  slots[t] = static_cast<O>(lo - r0);
+}
+
+// ---------------------------------------------------------------------
+// P1 tetrahedra: a = kappa grad u . grad v + c u v, 16 lanes per cell,
+// lane (i, j) computes one entry. MODE 0: A += Ae with BC rows/cols
+// zeroed; MODE 1: b -= alpha Ae[:, bc] (g - x0) (lifting).
+// ---------------------------------------------------------------------
+template <typename T, typename O, int MODE>
+__global__ void __launch_bounds__(kBlock)
+    p1tet_a_k(std::int32_t nc, const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
+              const std::int32_t* __restrict__ dm, const T* __restrict__ coeffs, int cstride,
+              T creac, const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
+              const O* __restrict__ slots, T* __restrict__ vals, const double* __restrict__ g,
+              const T* __restrict__ x0, double alpha, T* __restrict__ b)
+{
+  const std::int64_t gid = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  const std::int64_t c = gid >> 4;
+  const int e = static_cast<int>(gid & 15), i = e >> 2, j = e & 3;
+  if (c >= nc)
+    return;
+  const int4 gv = *reinterpret_cast<const int4*>(xdm + 4 * c);
+  const int4 dv = *reinterpret_cast<const int4*>(dm + 4 * c);
+  const std::int32_t di = (&dv.x)[i], dj = (&dv.x)[j];
+  if constexpr (MODE == 0)
+  {
+    if (bc && (bc[di] | bc[dj]))
+      return;
+  }
+  else
+  {
+    if (!bc[dj])
+      return;
+  }
+  const T* p0 = x + 3 * std::int64_t(gv.x);
+  const T* p1 = x + 3 * std::int64_t(gv.y);
+  const T* p2 = x + 3 * std::int64_t(gv.z);
+  const T* p3 = x + 3 * std::int64_t(gv.w);
+  T J[9], adj[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+  {
+    J[3 * r + 0] = p1[r] - p0[r];
+    J[3 * r + 1] = p2[r] - p0[r];
+    J[3 * r + 2] = p3[r] - p0[r];
+  }
+  const T det = adjugate(J, adj);
+  const T adet = det < T(0) ? -det : det;
+  // h_i = det * grad phi_i: rows of adj (phi_0: minus their sum)
+  T hi[3], hj[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+  {
+    const T s = adj[r] + adj[3 + r] + adj[6 + r];
+    hi[r] = i == 0 ? -s : adj[3 * (i - 1) + r];
+    hj[r] = j == 0 ? -s : adj[3 * (j - 1) + r];
+  }
+  const T kap = coeffs[c * cstride];
+  T a = kap * (hi[0] * hj[0] + hi[1] * hj[1] + hi[2] * hj[2]) / (T(6) * adet);
+  a += creac * adet * (i == j ? T(2) : T(1)) / T(120);
+  if constexpr (MODE == 0)
+  {
+    atomicAdd(vals + rp[di] + slots[gid], a);
+  }
+  else
+  {
+    const double xb = g ? g[dj] : 0.0;
+    const double xo = x0 ? double(x0[dj]) : 0.0;
+    atomicAdd(b + di, static_cast<T>(-double(a) * alpha * (xb - xo)));
+  }
+}
+
+/// P1 tetrahedra: L = f v with a P1 coefficient f (w = f at the vertices):
+/// b_i += |K|/20 (f_i + sum_j f_j). 4 lanes per cell.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    p1tet_l_k(std::int32_t nc, const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
+              const std::int32_t* __restrict__ dm, const T* __restrict__ coeffs, int cstride,
+              T* __restrict__ b)
+{
+  const std::int64_t gid = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  const std::int64_t c = gid >> 2;
+  const int i = static_cast<int>(gid & 3);
+  if (c >= nc)
+    return;
+  const int4 gv = *reinterpret_cast<const int4*>(xdm + 4 * c);
+  const T* p0 = x + 3 * std::int64_t(gv.x);
+  const T* p1 = x + 3 * std::int64_t(gv.y);
+  const T* p2 = x + 3 * std::int64_t(gv.z);
+  const T* p3 = x + 3 * std::int64_t(gv.w);
+  T J[9], adj[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+  {
+    J[3 * r + 0] = p1[r] - p0[r];
+    J[3 * r + 1] = p2[r] - p0[r];
+    J[3 * r + 2] = p3[r] - p0[r];
+  }
+  const T det = adjugate(J, adj);
+  const T adet = det < T(0) ? -det : det;
+  const T* f = coeffs + c * cstride;
+  const T v = adet / T(120) * (f[0] + f[1] + f[2] + f[3] + f[i]);
+  atomicAdd(b + dm[4 * c + i], v);
+}
+
+// ---------------------------------------------------------------------
+// Q1 hexahedra. Reference cube [0,1]^3, vertex i at bits (i&1, i>>1&1,
+// i>>2&1). One-dimensional integrals of the linear shape functions give
+// every reference tensor exactly.
+// ---------------------------------------------------------------------
+
+/// Symmetrised reference stiffness S6 (order 00, 11, 22, 01+10, 02+20,
+/// 12+21) and reference mass of entry (i, j).
+template <typename T>
+__device__ inline void q1_reference(int i, int j, T S6[6], T& M)
+{
+  T m[3], k[3], d[3], e[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+  {
+    const int p = (i >> a) & 1, q = (j >> a) & 1;
+    m[a] = p == q ? T(1) / T(3) : T(1) / T(6);
+    k[a] = p == q ? T(1) : T(-1);
+    d[a] = p ? T(0.5) : T(-0.5); // int f_p' f_q
+    e[a] = q ? T(0.5) : T(-0.5); // int f_p f_q'
+  }
+  M = m[0] * m[1] * m[2];
+  S6[0] = k[0] * m[1] * m[2];
+  S6[1] = m[0] * k[1] * m[2];
+  S6[2] = m[0] * m[1] * k[2];
+  S6[3] = (d[0] * e[1] + e[0] * d[1]) * m[2];
+  S6[4] = (d[0] * e[2] + e[0] * d[2]) * m[1];
+  S6[5] = (d[1] * e[2] + e[1] * d[2]) * m[0];
+}
+
+/// Affine hexahedra, a = kappa grad u.grad v + c u v. Each wavefront
+/// takes 64 cells: phase 1 (lane = cell) forms the 6 geometric factors
+/// G = kappa adj adj^T / |det|, the mass scale c|det|, the dofs, their row
+/// starts and BC flags in LDS; phase 2 (lane = entry (i, j)) evaluates
+/// A_ij = sum G S6 + c|det| M for each of the 64 cells and scatters.
+template <typename T, typename O, int MODE>
+__global__ void __launch_bounds__(kBlock)
+    q1hex_a_affine_k(std::int32_t nc, const T* __restrict__ x,
+                     const std::int32_t* __restrict__ xdm, const std::int32_t* __restrict__ dm,
+                     const T* __restrict__ coeffs, int cstride, T creac,
+                     const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
+                     const O* __restrict__ slots, T* __restrict__ vals,
+                     const double* __restrict__ g, const T* __restrict__ x0, double alpha,
+                     T* __restrict__ b)
+{
+  __shared__ T sG[4][64][7];
+  __shared__ std::int32_t sDof[4][64][8];
+  __shared__ std::int32_t sRow[4][64][8];
+  __shared__ unsigned sBC[4][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const std::int64_t base = (std::int64_t(blockIdx.x) * 4 + wave) * 64;
+  {
+    const std::int64_t c = base + lane;
+    if (c < nc)
+    {
+      const int4 g0 = *reinterpret_cast<const int4*>(xdm + 8 * c);
+      const int4 g1 = *reinterpret_cast<const int4*>(xdm + 8 * c + 4);
+      const T* q0 = x + 3 * std::int64_t(g0.x);
+      const T* q1 = x + 3 * std::int64_t(g0.y);
+      const T* q2 = x + 3 * std::int64_t(g0.z);
+      const T* q4 = x + 3 * std::int64_t(g1.x);
+      T J[9], adj[9];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+      {
+        J[3 * r + 0] = q1[r] - q0[r];
+        J[3 * r + 1] = q2[r] - q0[r];
+        J[3 * r + 2] = q4[r] - q0[r];
+      }
+      const T det = adjugate(J, adj);
+      const T adet = det < T(0) ? -det : det;
+      const T s = coeffs[c * cstride] / adet;
+      T* G = sG[wave][lane];
+      G[0] = s * (adj[0] * adj[0] + adj[1] * adj[1] + adj[2] * adj[2]);
+      G[1] = s * (adj[3] * adj[3] + adj[4] * adj[4] + adj[5] * adj[5]);
+      G[2] = s * (adj[6] * adj[6] + adj[7] * adj[7] + adj[8] * adj[8]);
+      G[3] = s * (adj[0] * adj[3] + adj[1] * adj[4] + adj[2] * adj[5]);
+      G[4] = s * (adj[0] * adj[6] + adj[1] * adj[7] + adj[2] * adj[8]);
+      G[5] = s * (adj[3] * adj[6] + adj[4] * adj[7] + adj[5] * adj[8]);
+      G[6] = creac * adet;
+      const int4 d0 = *reinterpret_cast<const int4*>(dm + 8 * c);
+      const int4 d1 = *reinterpret_cast<const int4*>(dm + 8 * c + 4);
+      const std::int32_t dd[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+      unsigned mask = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+      {
+        sDof[wave][lane][k] = dd[k];
+        if constexpr (MODE == 0)
+          sRow[wave][lane][k] = rp[dd[k]];
+        if (bc && bc[dd[k]])
+          mask |= 1u << k;
+      }
+      sBC[wave][lane] = mask;
+    }
+  }
+  __syncthreads();
+  const int i = lane >> 3, j = lane & 7;
+  T S6[6], M;
+  q1_reference<T>(i, j, S6, M);
+  for (int t = 0; t < 64; ++t)
+  {
+    const std::int64_t c = base + t;
+    if (c >= nc)
+      break;
+    const unsigned mask = sBC[wave][t];
+    if constexpr (MODE == 0)
+    {
+      if ((mask >> i | mask >> j) & 1u)
+        continue;
+    }
+    else
+    {
+      if (!((mask >> j) & 1u))
+        continue;
+    }
+    const T* G = sG[wave][t];
+    const T a = G[0] * S6[0] + G[1] * S6[1] + G[2] * S6[2] + G[3] * S6[3] + G[4] * S6[4]
+                + G[5] * S6[5] + G[6] * M;
+    if constexpr (MODE == 0)
+    {
+      atomicAdd(vals + sRow[wave][t][i] + slots[c * 64 + lane], a);
+    }
+    else
+    {
+      const std::int32_t dj = sDof[wave][t][j];
+      const double xb = g ? g[dj] : 0.0;
+      const double xo = x0 ? double(x0[dj]) : 0.0;
+      atomicAdd(b + sDof[wave][t][i], static_cast<T>(-double(a) * alpha * (xb - xo)));
+    }
+  }
+}
+
+/// Q1 shape functions and reference gradients at point xi.
+template <typename T>
+__device__ inline void q1_tab(const T xi[3], T phi[8], T dphi[8][3])
+{
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+  {
+    T f[3], df[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+      const int bit = (i >> a) & 1;
+      f[a] = bit ? xi[a] : T(1) - xi[a];
+      df[a] = bit ? T(1) : T(-1);
+    }
+    phi[i] = f[0] * f[1] * f[2];
+    dphi[i][0] = df[0] * f[1] * f[2];
+    dphi[i][1] = f[0] * df[1] * f[2];
+    dphi[i][2] = f[0] * f[1] * df[2];
+  }
+}
+
+/// General (non-affine) hexahedra: one thread per cell, 2x2x2 Gauss, the
+/// full 8x8 element matrix in registers, then the scatter.
+template <typename T, typename O, int MODE>
+__global__ void __launch_bounds__(kBlock)
+    q1hex_a_general_k(std::int32_t nc, const T* __restrict__ x,
+                      const std::int32_t* __restrict__ xdm, const std::int32_t* __restrict__ dm,
+                      const T* __restrict__ coeffs, int cstride, T creac,
+                      const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
+                      const O* __restrict__ slots, T* __restrict__ vals,
+                      const double* __restrict__ g, const T* __restrict__ x0, double alpha,
+                      T* __restrict__ b)
+{
+  const std::int64_t c = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (c >= nc)
+    return;
+  T cd[8][3];
+  for (int v = 0; v < 8; ++v)
+    for (int r = 0; r < 3; ++r)
+      cd[v][r] = x[3 * std::int64_t(xdm[8 * c + v]) + r];
+  const T kap = coeffs[c * cstride];
+  T A[8][8];
+  for (int p = 0; p < 8; ++p)
+    for (int q = 0; q < 8; ++q)
+      A[p][q] = 0;
+  const T ga = T(0.5) - T(0.5) / sqrt(T(3)), gb = T(0.5) + T(0.5) / sqrt(T(3));
+  for (int qp = 0; qp < 8; ++qp)
+  {
+    const T xi[3] = {(qp & 1) ? gb : ga, ((qp >> 1) & 1) ? gb : ga, ((qp >> 2) & 1) ? gb : ga};
+    T phi[8], dphi[8][3], J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, adj[9];
+    q1_tab(xi, phi, dphi);
+    for (int v = 0; v < 8; ++v)
+      for (int r = 0; r < 3; ++r)
+        for (int a = 0; a < 3; ++a)
+          J[3 * r + a] += cd[v][r] * dphi[v][a];
+    const T det = adjugate(J, adj);
+    const T adet = det < T(0) ? -det : det;
+    T h[8][3];
+    for (int v = 0; v < 8; ++v)
+      for (int r = 0; r < 3; ++r)
+        h[v][r] = dphi[v][0] * adj[r] + dphi[v][1] * adj[3 + r] + dphi[v][2] * adj[6 + r];
+    const T ws = kap / (T(8) * adet), wm = creac * adet / T(8);
+    for (int p = 0; p < 8; ++p)
+      for (int q = 0; q < 8; ++q)
+        A[p][q] += ws * (h[p][0] * h[q][0] + h[p][1] * h[q][1] + h[p][2] * h[q][2])
+                   + wm * phi[p] * phi[q];
+  }
+  std::int32_t d[8];
+  for (int v = 0; v < 8; ++v)
+    d[v] = dm[8 * c + v];
+  for (int p = 0; p < 8; ++p)
+    for (int q = 0; q < 8; ++q)
+    {
+      if constexpr (MODE == 0)
+      {
+        if (bc && (bc[d[p]] | bc[d[q]]))
+          continue;
+        atomicAdd(vals + rp[d[p]] + slots[c * 64 + 8 * p + q], A[p][q]);
+      }
+      else
+      {
+        if (!bc[d[q]])
+          continue;
+        const double xb = g ? g[d[q]] : 0.0;
+        const double xo = x0 ? double(x0[d[q]]) : 0.0;
+        atomicAdd(b + d[p], static_cast<T>(-double(A[p][q]) * alpha * (xb - xo)));
+      }
+    }
+}
+
+/// Q1 hexahedra: L = f v with a Q1 coefficient (w = f at the vertices).
+/// 8 lanes per cell; affine cells use b_i = |det| sum_j M_ij f_j, others
+/// 2x2x2 Gauss.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    q1hex_l_k(std::int32_t nc, const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
+              const std::int32_t* __restrict__ dm, const T* __restrict__ coeffs, int cstride,
+              int affine, T* __restrict__ b)
+{
+  const std::int64_t gid = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  const std::int64_t c = gid >> 3;
+  const int i = static_cast<int>(gid & 7);
+  if (c >= nc)
+    return;
+  const T* f = coeffs + c * cstride;
+  T v = 0;
+  if (affine)
+  {
+    const T* q0 = x + 3 * std::int64_t(xdm[8 * c + 0]);
+    const T* q1 = x + 3 * std::int64_t(xdm[8 * c + 1]);
+    const T* q2 = x + 3 * std::int64_t(xdm[8 * c + 2]);
+    const T* q4 = x + 3 * std::int64_t(xdm[8 * c + 4]);
+    T J[9], adj[9];
+    for (int r = 0; r < 3; ++r)
+    {
+      J[3 * r + 0] = q1[r] - q0[r];
+      J[3 * r + 1] = q2[r] - q0[r];
+      J[3 * r + 2] = q4[r] - q0[r];
+    }
+    const T det = adjugate(J, adj);
+    const T adet = det < T(0) ? -det : det;
+    for (int j = 0; j < 8; ++j)
+    {
+      T m = 1;
+      for (int a = 0; a < 3; ++a)
+        m *= (((i >> a) & 1) == ((j >> a) & 1)) ? T(1) / T(3) : T(1) / T(6);
+      v += m * f[j];
+    }
+    v *= adet;
+  }
+  else
+  {
+    T cd[8][3];
+    for (int w = 0; w < 8; ++w)
+      for (int r = 0; r < 3; ++r)
+        cd[w][r] = x[3 * std::int64_t(xdm[8 * c + w]) + r];
+    const T ga = T(0.5) - T(0.5) / sqrt(T(3)), gb = T(0.5) + T(0.5) / sqrt(T(3));
+    for (int qp = 0; qp < 8; ++qp)
+    {
+      const T xi[3] = {(qp & 1) ? gb : ga, ((qp >> 1) & 1) ? gb : ga, ((qp >> 2) & 1) ? gb : ga};
+      T phi[8], dphi[8][3], J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, adj[9];
+      q1_tab(xi, phi, dphi);
+      for (int w = 0; w < 8; ++w)
+        for (int r = 0; r < 3; ++r)
+          for (int a = 0; a < 3; ++a)
+            J[3 * r + a] += cd[w][r] * dphi[w][a];
+      const T det = adjugate(J, adj);
+      const T adet = det < T(0) ? -det : det;
+      T fq = 0;
+      for (int w = 0; w < 8; ++w)
+        fq += f[w] * phi[w];
+      v += adet / T(8) * fq * phi[i];
+    }
+  }
+  atomicAdd(b + dm[8 * c + i], v);
+}
+
+// ---------------------------------------------------------------------
+// Dirichlet helpers
+// ---------------------------------------------------------------------
+
+/// A[d, d] = diag for each BC dof d (unrolled index; block row d / bs).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    set_diagonal_k(std::int32_t n, const std::int32_t* dofs, int bs, const std::int32_t* diag,
+                   T* vals, T value, int* err)
+{
+  const std::int32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n)
+    return;
+  const std::int32_t d = dofs[t];
+  const std::int32_t p = diag[d / bs];
+  if (p < 0)
+  {
+    atomicOr(err, 1);
+    return;
+  }
+  const int k = d % bs;
+  vals[(std::int64_t(p) * bs + k) * bs + k] = value;
+}
+
+/// b[d] = alpha (g[d] - x0[d]) for each BC dof d.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    bc_set_k(std::int32_t n, const std::int32_t* dofs, const double* g, const T* x0,
+             double alpha, T* b)
+{
+  const std::int32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n)
+    return;
+  const std::int32_t d = dofs[t];
+  const double gv = g ? g[d] : 0.0;
+  const double xv = x0 ? double(x0[d]) : 0.0;
+  b[d] = static_cast<T>(alpha * (gv - xv));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) to_dtype_k(T* dst, const double* src, std::int64_t n)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t < n)
+    dst[t] = static_cast<T>(src[t]);
+}
+
+int blocks_for(std::int64_t n) { return static_cast<int>((n + kBlock - 1) / kBlock); }
+
+/// Device copy of a host FP64 array converted to dtype.
+void* upload_as(const double* src, std::int64_t n, int dtype)
+{
+  void* d = dmalloc(dtype_size(dtype) * std::max<std::int64_t>(n, 1));
+  if (n == 0)
+    return d;
+  if (dtype == DFX_F64)
+  {
+    h2d(d, src, sizeof(double) * n);
+    return d;
+  }
+  double* tmp = static_cast<double*>(dmalloc(sizeof(double) * n));
+  h2d(tmp, src, sizeof(double) * n);
+  to_dtype_k<float><<<blocks_for(n), kBlock, 0, stream()>>>(static_cast<float*>(d), tmp, n);
+  check_launch("upload_as");
+  DFX_HIP(hipStreamSynchronize(stream()));
+  dfree(tmp);
+  return d;
+}
+
+/// Builds (or reuses) the cell -> CSR slot map of form f for matrix A.
+void ensure_slots(dfx_form& f, const dfx_csr& A)
+{
+  if (f.slots && f.slot_pattern == A.pattern_id)
+    return;
+  dfree(f.slots);
+  f.slots = nullptr;
+  const int k = f.d.dofs_per_cell;
+  const std::int64_t n = std::int64_t(f.d.num_cells) * k * k;
+  f.slot_bytes = A.max_row <= 256 ? 1 : 2;
+  require(A.max_row <= 65536, "row too long for the slot map");
+  f.slots = dmalloc(std::max<std::int64_t>(n, 1) * f.slot_bytes);
+  int* err = static_cast<int*>(dmalloc(sizeof(int)));
+  DFX_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
+  if (n > 0)
+  {
+    if (f.slot_bytes == 1)
+      slot_map_k<std::uint8_t><<<blocks_for(n), kBlock, 0, stream()>>>(
+          f.d.num_cells, k, f.dofmap, A.row_ptr, A.cols, static_cast<std::uint8_t*>(f.slots), err);
+    else
+      slot_map_k<std::uint16_t><<<blocks_for(n), kBlock, 0, stream()>>>(
+          f.d.num_cells, k, f.dofmap, A.row_ptr, A.cols, static_cast<std::uint16_t*>(f.slots), err);
+    check_launch("slot_map");
+  }
+  int h = 0;
+  d2h(&h, err, sizeof(int));
+  dfree(err);
+  if (h)
+  {
+    dfree(f.slots);
+    f.slots = nullptr;
+    throw Error(DFX_ERR_PATTERN, "Entry not in sparsity");
+  }
+  f.slot_pattern = A.pattern_id;
+}
+
+template <typename T, typename O, int MODE>
+void launch_bilinear(const dfx_form& f, const dfx_csr* A, const dfx_bc* bc, const T* x0,
+                     double alpha, T* b)
+{
+  hipStream_t s = stream();
+  const std::int32_t nc = f.d.num_cells;
+  if (nc == 0)
+    return;
+  const T* x = static_cast<const T*>(f.x);
+  const T* co = static_cast<const T*>(f.coeffs);
+  const T creac = static_cast<T>(f.constants[0]);
+  const std::int8_t* mark = bc ? bc->marker : nullptr;
+  const double* g = bc ? bc->g : nullptr;
+  const std::int32_t* rp = A ? A->row_ptr : nullptr;
+  const O* sl = static_cast<const O*>(f.slots);
+  T* vals = A ? static_cast<T*>(A->values) : nullptr;
+  switch (f.d.kernel)
+  {
+  case DFX_K_P1TET_A:
+    p1tet_a_k<T, O, MODE><<<blocks_for(std::int64_t(nc) * 16), kBlock, 0, s>>>(
+        nc, x, f.x_dofmap, f.dofmap, co, f.d.cstride, creac, mark, rp, sl, vals, g, x0, alpha, b);
+    break;
+  case DFX_K_Q1HEX_A:
+    if (f.affine)
+      q1hex_a_affine_k<T, O, MODE><<<static_cast<int>((nc + 255) / 256), kBlock, 0, s>>>(
+          nc, x, f.x_dofmap, f.dofmap, co, f.d.cstride, creac, mark, rp, sl, vals, g, x0, alpha, b);
+    else
+      q1hex_a_general_k<T, O, MODE><<<blocks_for(nc), kBlock, 0, s>>>(
+          nc, x, f.x_dofmap, f.dofmap, co, f.d.cstride, creac, mark, rp, sl, vals, g, x0, alpha, b);
+    break;
+  default:
+    throw Error(DFX_ERR_ARG, "no device kernel for this bilinear form yet");
+  }
+  check_launch("assemble_matrix");
+}
+
+template <typename T>
+void launch_linear(const dfx_form& f, T* b)
+{
+  hipStream_t s = stream();
+  const std::int32_t nc = f.d.num_cells;
+  if (nc == 0)
+    return;
+  const T* x = static_cast<const T*>(f.x);
+  const T* co = static_cast<const T*>(f.coeffs);
+  switch (f.d.kernel)
+  {
+  case DFX_K_P1TET_L:
+    p1tet_l_k<T><<<blocks_for(std::int64_t(nc) * 4), kBlock, 0, s>>>(nc, x, f.x_dofmap, f.dofmap,
+                                                                      co, f.d.cstride, b);
+    break;
+  case DFX_K_Q1HEX_L:
+    q1hex_l_k<T><<<blocks_for(std::int64_t(nc) * 8), kBlock, 0, s>>>(
+        nc, x, f.x_dofmap, f.dofmap, co, f.d.cstride, f.affine ? 1 : 0, b);
+    break;
+  default:
+    throw Error(DFX_ERR_ARG, "no device kernel for this linear form yet");
+  }
+  check_launch("assemble_vector");
+}
+
+/// True if every hexahedron is a parallelepiped (x3 = x1 + x2 - x0 and
+/// likewise for x5, x6, x7), so that its Jacobian is constant.
+bool hex_all_affine(const dfx_form_desc& d)
+{
+  for (std::int64_t c = 0; c < d.num_cells; ++c)
+  {
+    const std::int32_t* v = d.x_dofmap + 8 * c;
+    const double* X[8];
+    for (int k = 0; k < 8; ++k)
+      X[k] = d.x + 3 * std::int64_t(v[k]);
+    for (int r = 0; r < 3; ++r)
+    {
+      const double h = std::abs(X[1][r] - X[0][r]) + std::abs(X[2][r] - X[0][r])
+                       + std::abs(X[4][r] - X[0][r]) + 1e-300;
+      const double e3 = X[3][r] - (X[1][r] + X[2][r] - X[0][r]);
+      const double e5 = X[5][r] - (X[1][r] + X[4][r] - X[0][r]);
+      const double e6 = X[6][r] - (X[2][r] + X[4][r] - X[0][r]);
+      const double e7 = X[7][r] - (X[1][r] + X[2][r] + X[4][r] - 2 * X[0][r]);
+      const double tol = 1e-13 * h;
+      if (std::abs(e3) > tol || std::abs(e5) > tol || std::abs(e6) > tol || std::abs(e7) > tol)
+        return false;
+    }
+  }
+  return true;
+}
+
+} // namespace
+} // namespace dfx
+
+using namespace dfx;
+
+extern "C" {
+
+dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out)
+{
+  return guard(
+      [&]
+      {
+        require(desc && out, "null argument");
+        const dfx_form_desc& d = *desc;
+        KernelInfo ki;
+        require(kernel_info(d.kernel, ki), "unknown kernel id");
+        require(d.dtype == DFX_F32 || d.dtype == DFX_F64, "bad dtype");
+        require(d.num_cells >= 0 && d.num_nodes >= 0, "negative size");
+        require(d.nodes_per_cell == ki.npc, "geometry nodes per cell do not match the kernel");
+        require(d.dofs_per_cell == ki.k && d.bs == ki.bs, "dofmap does not match the kernel");
+        require(d.x && d.x_dofmap && d.dofmap, "null mesh data");
+        require(d.num_constants >= 0 && d.num_constants <= 8, "at most 8 constants");
+        require(d.cstride >= 0 && (d.cstride == 0 || d.coeffs), "missing coefficients");
+        // Required coefficient layouts of the kernels
+        int need = 0;
+        if (d.kernel == DFX_K_P1TET_A || d.kernel == DFX_K_Q1HEX_A || d.kernel == DFX_K_P3TET_A)
+          need = 1;
+        else if (d.kernel == DFX_K_P1TET_L)
+          need = 4;
+        else if (d.kernel == DFX_K_Q1HEX_L)
+          need = 8;
+        else if (d.kernel == DFX_K_P2TET_ELAST_A)
+          need = 2;
+        require(d.cstride >= need, "too few packed coefficients for this kernel");
+        const std::int64_t nc = d.num_cells;
+        for (std::int64_t i = 0; i < nc * d.nodes_per_cell; ++i)
+          require(d.x_dofmap[i] >= 0 && d.x_dofmap[i] < d.num_nodes, "geometry dofmap out of range");
+        std::int32_t max_dof = -1;
+        for (std::int64_t i = 0; i < nc * d.dofs_per_cell; ++i)
+        {
+          require(d.dofmap[i] >= 0, "negative dof");
+          max_dof = std::max(max_dof, d.dofmap[i]);
+        }
+        auto f = std::make_unique<dfx_form>();
+        f->max_dof = max_dof;
+        f->d = d;
+        f->d.x = nullptr;
+        f->d.x_dofmap = nullptr;
+        f->d.dofmap = nullptr;
+        f->d.coeffs = nullptr;
+        f->d.constants = nullptr;
+        for (int i = 0; i < d.num_constants; ++i)
+          f->constants[i] = d.constants[i];
+        f->affine = ki.cell == DFX_TET ? true : hex_all_affine(d);
+        f->x = upload_as(d.x, std::int64_t(d.num_nodes) * 3, d.dtype);
+        f->x_dofmap = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * nc * d.nodes_per_cell));
+        h2d(f->x_dofmap, d.x_dofmap, sizeof(std::int32_t) * nc * d.nodes_per_cell);
+        if (d.dofmap == d.x_dofmap
+            || (d.dofs_per_cell == d.nodes_per_cell
+                && std::memcmp(d.dofmap, d.x_dofmap, sizeof(std::int32_t) * nc * d.nodes_per_cell) == 0))
+          f->dofmap = f->x_dofmap;
+        else
+        {
+          f->dofmap = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * nc * d.dofs_per_cell));
+          h2d(f->dofmap, d.dofmap, sizeof(std::int32_t) * nc * d.dofs_per_cell);
+        }
+        f->coeffs = upload_as(d.coeffs, nc * d.cstride, d.dtype);
+        *out = f.release();
+      });
+}
+
+dfx_status dfx_form_destroy(dfx_form* f)
+{
+  if (f)
+  {
+    if (f->dofmap != f->x_dofmap)
+      dfree(f->dofmap);
+    dfree(f->x_dofmap);
+    dfree(f->x);
+    dfree(f->coeffs);
+    dfree(f->slots);
+    delete f;
+  }
+  return DFX_OK;
+}
+
+dfx_status dfx_form_set_coefficients(dfx_form* f, const double* coeffs, const double* constants)
+{
+  return guard(
+      [&]
+      {
+        require(f, "null form");
+        if (coeffs && f->d.cstride > 0)
+        {
+          void* nc = upload_as(coeffs, std::int64_t(f->d.num_cells) * f->d.cstride, f->d.dtype);
+          dfree(f->coeffs);
+          f->coeffs = nc;
+        }
+        if (constants)
+          for (int i = 0; i < f->d.num_constants; ++i)
+            f->constants[i] = constants[i];
+      });
+}
+
+dfx_status dfx_bc_create(const int32_t* dofs, int32_t ndofs, int32_t n_total, const double* g,
+                         dfx_bc** out)
+{
+  return guard(
+      [&]
+      {
+        require(out && ndofs >= 0 && n_total >= 0 && (ndofs == 0 || dofs), "bad argument");
+        std::vector<std::int8_t> mark(std::max(n_total, 1), 0);
+        for (std::int32_t i = 0; i < ndofs; ++i)
+        {
+          require(dofs[i] >= 0 && dofs[i] < n_total, "BC dof out of range");
+          mark[dofs[i]] = 1;
+        }
+        auto bc = std::make_unique<dfx_bc>();
+        bc->n_total = n_total;
+        bc->ndofs = ndofs;
+        bc->dofs = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * std::max(ndofs, 1)));
+        h2d(bc->dofs, dofs, sizeof(std::int32_t) * ndofs);
+        bc->marker = static_cast<std::int8_t*>(dmalloc(mark.size()));
+        h2d(bc->marker, mark.data(), mark.size());
+        if (g)
+        {
+          bc->g = static_cast<double*>(dmalloc(sizeof(double) * std::max(n_total, 1)));
+          h2d(bc->g, g, sizeof(double) * n_total);
+        }
+        *out = bc.release();
+      });
+}
+
+dfx_status dfx_bc_destroy(dfx_bc* bc)
+{
+  if (bc)
+  {
+    dfree(bc->dofs);
+    dfree(bc->marker);
+    dfree(bc->g);
+    delete bc;
+  }
+  return DFX_OK;
+}
+
+dfx_status dfx_assemble_matrix(dfx_csr* A, const dfx_form* a, const dfx_bc* bc)
+{
+  return guard(
+      [&]
+      {
+        require(A && a, "null argument");
+        KernelInfo ki;
+        kernel_info(a->d.kernel, ki);
+        require(ki.bilinear, "assemble_matrix needs a bilinear form");
+        require(A->dtype == a->d.dtype, "matrix and form dtypes differ");
+        require(A->bs == a->d.bs, "matrix and form block sizes differ");
+        require(!bc || bc->n_total == A->nrows * A->bs, "BC size does not match the matrix");
+        require(a->max_dof < A->nrows && a->max_dof < A->ncols, "dofmap exceeds the matrix size");
+        dfx_form& f = const_cast<dfx_form&>(*a);
+        ensure_slots(f, *A);
+        if (A->dtype == DFX_F64)
+        {
+          if (f.slot_bytes == 1)
+            launch_bilinear<double, std::uint8_t, 0>(f, A, bc, nullptr, 0.0, nullptr);
+          else
+            launch_bilinear<double, std::uint16_t, 0>(f, A, bc, nullptr, 0.0, nullptr);
+        }
+        else
+        {
+          if (f.slot_bytes == 1)
+            launch_bilinear<float, std::uint8_t, 0>(f, A, bc, nullptr, 0.0, nullptr);
+          else
+            launch_bilinear<float, std::uint16_t, 0>(f, A, bc, nullptr, 0.0, nullptr);
+        }
+      });
+}
+
+dfx_status dfx_set_diagonal(dfx_csr* A, const dfx_bc* bc, double diag)
+{
+  return guard(
+      [&]
+      {
+        require(A && bc, "null argument");
+        require(bc->n_total == A->nrows * A->bs, "BC size does not match the matrix");
+        if (bc->ndofs == 0)
+          return;
+        int* err = static_cast<int*>(dmalloc(sizeof(int)));
+        DFX_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
+        if (A->dtype == DFX_F64)
+          set_diagonal_k<double><<<blocks_for(bc->ndofs), kBlock, 0, stream()>>>(
+              bc->ndofs, bc->dofs, A->bs, A->diag, static_cast<double*>(A->values), diag, err);
+        else
+          set_diagonal_k<float><<<blocks_for(bc->ndofs), kBlock, 0, stream()>>>(
+              bc->ndofs, bc->dofs, A->bs, A->diag, static_cast<float*>(A->values),
+              static_cast<float>(diag), err);
+        check_launch("set_diagonal");
+        int h = 0;
+        d2h(&h, err, sizeof(int));
+        dfree(err);
+        if (h)
+          throw Error(DFX_ERR_PATTERN, "Entry not in sparsity");
+      });
+}
+
+dfx_status dfx_assemble_vector(dfx_vec* b, const dfx_form* L)
+{
+  return guard(
+      [&]
+      {
+        require(b && L, "null argument");
+        KernelInfo ki;
+        kernel_info(L->d.kernel, ki);
+        require(!ki.bilinear, "assemble_vector needs a linear form");
+        require(b->dtype == L->d.dtype, "vector and form dtypes differ");
+        require(std::int64_t(L->max_dof + 1) * L->d.bs <= b->n, "dofmap exceeds the vector size");
+        if (b->dtype == DFX_F64)
+          launch_linear<double>(*L, static_cast<double*>(b->data));
+        else
+          launch_linear<float>(*L, static_cast<float*>(b->data));
+      });
+}
+
+dfx_status dfx_apply_lifting(dfx_vec* b, const dfx_form* a, const dfx_bc* bc, const dfx_vec* x0,
+                             double alpha)
+{
+  return guard(
+      [&]
+      {
+        require(b && a, "null argument");
+        KernelInfo ki;
+        kernel_info(a->d.kernel, ki);
+        require(ki.bilinear, "apply_lifting needs a bilinear form");
+        require(b->dtype == a->d.dtype, "vector and form dtypes differ");
+        if (!bc || bc->ndofs == 0)
+          return;
+        require(bc->n_total == b->n, "BC size does not match the vector");
+        require(!x0 || (x0->n == b->n && x0->dtype == b->dtype), "x0 does not match b");
+        require(std::int64_t(a->max_dof + 1) * a->d.bs <= b->n, "dofmap exceeds the vector size");
+        const dfx_form& f = *a;
+        if (b->dtype == DFX_F64)
+          launch_bilinear<double, std::uint8_t, 1>(
+              f, nullptr, bc, x0 ? static_cast<const double*>(x0->data) : nullptr, alpha,
+              static_cast<double*>(b->data));
+        else
+          launch_bilinear<float, std::uint8_t, 1>(
+              f, nullptr, bc, x0 ? static_cast<const float*>(x0->data) : nullptr, alpha,
+              static_cast<float*>(b->data));
+      });
+}
+
+dfx_status dfx_bc_set(dfx_vec* b, const dfx_bc* bc, const dfx_vec* x0, double alpha)
+{
+  return guard(
+      [&]
+      {
+        require(b && bc, "null argument");
+        require(bc->n_total == b->n, "BC size does not match the vector");
+        require(!x0 || (x0->n == b->n && x0->dtype == b->dtype), "x0 does not match b");
+        if (bc->ndofs == 0)
+          return;
+        if (b->dtype == DFX_F64)
+          bc_set_k<double><<<blocks_for(bc->ndofs), kBlock, 0, stream()>>>(
+              bc->ndofs, bc->dofs, bc->g, x0 ? static_cast<const double*>(x0->data) : nullptr,
+              alpha, static_cast<double*>(b->data));
+        else
+          bc_set_k<float><<<blocks_for(bc->ndofs), kBlock, 0, stream()>>>(
+              bc->ndofs, bc->dofs, bc->g, x0 ? static_cast<const float*>(x0->data) : nullptr,
+              alpha, static_cast<float*>(b->data));
+        check_launch("bc_set");
+      });
+}
+
+} // extern "C"

@@ -1,0 +1,166 @@
+// Device-side internals of libdfx: HIP context, device handle layouts and
+// the reduction helpers shared by the kernels. gfx950 (MI355X) only.
+#pragma once
+
+#include "common.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+
+namespace dfx
+{
+
+#define DFX_HIP(call)                                                          \
+  do                                                                           \
+  {                                                                            \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      throw ::dfx::Error(DFX_ERR_HIP, std::string(#call) + ": "                \
+                                          + hipGetErrorString(e_));            \
+  } while (0)
+
+/// Library-wide device state: one stream at default priority.
+struct Context
+{
+  bool ready = false;
+  int device = 0;
+  int num_cu = 256;
+  hipStream_t stream = nullptr;
+  hipEvent_t events[64] = {};
+};
+
+Context& ctx();
+void ensure_init();
+hipStream_t stream();
+void check_launch(const char* what);
+
+inline std::size_t dtype_size(int dtype) { return dtype == DFX_F64 ? 8 : 4; }
+
+/// Device allocation helpers (hipMalloc on the library's device).
+void* dmalloc(std::size_t bytes);
+void dfree(void* p);
+void h2d(void* dst, const void* src, std::size_t bytes);
+void d2h(void* dst, const void* src, std::size_t bytes);
+
+/// Blocks dispatch round-robin over the 8 XCDs of an MI355X
+/// (block b runs on XCD b % 8). Map b to a logical index so that each XCD
+/// receives one contiguous range of logical blocks, keeping gathers of
+/// neighbouring rows inside one XCD's L2.
+__device__ inline int xcd_contiguous(int b, int nb)
+{
+  const int xcd = b & 7, idx = b >> 3;
+  const int q = nb >> 3, r = nb & 7;
+  return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+/// Sum over a 256-thread block (4 wavefronts of 64); result valid in
+/// thread 0. Fixed order, so deterministic.
+__device__ inline double block_sum_256(double v, double* lds /*[4]*/)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    lds[wave] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0)
+    s = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+  __syncthreads();
+  return s;
+}
+
+/// Grid-wide reduction of NV values with a last-block finish: every block
+/// stores its partial sums, the block that takes the final ticket sums all
+/// partials in block order (deterministic) and runs fin(sums) in thread 0.
+/// The ticket is reset by that block for the next launch.
+template <int NV, typename Fin>
+__device__ inline void grid_reduce(double (&v)[NV], double* partials,
+                                   unsigned* ticket, Fin&& fin)
+{
+  __shared__ double lds[4];
+  __shared__ bool last;
+  double s[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    s[k] = block_sum_256(v[k], lds);
+  const int nb = gridDim.x;
+  if (threadIdx.x == 0)
+  {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      partials[k * nb + blockIdx.x] = s[k];
+    __threadfence();
+    const unsigned t = atomicAdd(ticket, 1u);
+    last = (t == static_cast<unsigned>(nb) - 1);
+  }
+  __syncthreads();
+  if (!last)
+    return;
+  __threadfence();
+  double tot[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+  {
+    double a = 0;
+    for (int i = threadIdx.x; i < nb; i += 256)
+      a += __hip_atomic_load(partials + k * nb + i, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    tot[k] = block_sum_256(a, lds);
+  }
+  if (threadIdx.x == 0)
+  {
+    fin(tot);
+    *ticket = 0;
+  }
+}
+
+} // namespace dfx
+
+/// la::MatrixCSR on the device (compact blocks, row-major inside blocks).
+struct dfx_csr
+{
+  std::int32_t nrows = 0, ncols = 0, bs = 1, dtype = DFX_F64;
+  std::int64_t nnz = 0;          // block entries
+  std::int32_t max_row = 0;      // longest row (blocks)
+  std::int32_t* row_ptr = nullptr; // device, nrows+1 (int32: nnz < 2^31)
+  std::int32_t* cols = nullptr;    // device, nnz
+  std::int32_t* diag = nullptr;    // device, position of (i,i) or -1
+  void* values = nullptr;          // device, nnz*bs*bs of dtype
+  std::uint64_t pattern_id = 0;    // identifies the pattern for slot caches
+};
+
+struct dfx_vec
+{
+  std::int32_t n = 0, dtype = DFX_F64;
+  void* data = nullptr;
+};
+
+struct dfx_bc
+{
+  std::int32_t n_total = 0, ndofs = 0;
+  std::int32_t* dofs = nullptr;   // device, unrolled dof indices
+  std::int8_t* marker = nullptr;  // device, n_total
+  double* g = nullptr;            // device, n_total (FP64), nullptr if g = 0
+};
+
+struct dfx_form
+{
+  dfx_form_desc d{};          // sizes and kernel id (host pointers cleared)
+  double constants[8] = {0};
+  bool affine = true;         // every cell an affine image of the reference
+  std::int32_t max_dof = -1;  // largest node index in the dofmap
+  void* x = nullptr;          // device, num_nodes x 3 of dtype
+  std::int32_t* x_dofmap = nullptr;
+  std::int32_t* dofmap = nullptr; // may alias x_dofmap
+  void* coeffs = nullptr;     // device, num_cells x cstride of dtype
+  // cell -> CSR slot offsets (position inside the row), built on first
+  // assembly into a pattern and reused while the pattern is unchanged
+  std::uint64_t slot_pattern = 0;
+  void* slots = nullptr;
+  int slot_bytes = 0;         // 1 (uint8) or 2 (uint16)
+};

@@ -1,0 +1,1103 @@
+// Device linear algebra for the assemble-and-solve path: la::MatrixCSR and
+// la::Vector on HBM, CSR SpMV, FP64-accumulated inner products, and the
+// device-resident (Jacobi-preconditioned) CG and mixed-precision
+// refinement solvers. gfx950 (MI355X) only.
+#include "device.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace dfx
+{
+
+namespace
+{
+thread_local std::string g_last_error;
+Context g_ctx;
+std::atomic<std::uint64_t> g_pattern_counter{1};
+} // namespace
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+Context& ctx() { return g_ctx; }
+
+void ensure_init()
+{
+  if (g_ctx.ready)
+    return;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+    throw Error(DFX_ERR_NODEV, "no HIP device available");
+  DFX_HIP(hipSetDevice(0));
+  hipDeviceProp_t prop;
+  DFX_HIP(hipGetDeviceProperties(&prop, 0));
+  g_ctx.num_cu = prop.multiProcessorCount;
+  DFX_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+  for (auto& e : g_ctx.events)
+    DFX_HIP(hipEventCreate(&e));
+  g_ctx.ready = true;
+}
+
+hipStream_t stream()
+{
+  ensure_init();
+  return g_ctx.stream;
+}
+
+void check_launch(const char* what)
+{
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    throw Error(DFX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void* dmalloc(std::size_t bytes)
+{
+  ensure_init();
+  void* p = nullptr;
+  if (bytes == 0)
+    bytes = 256;
+  DFX_HIP(hipMalloc(&p, bytes));
+  return p;
+}
+
+void dfree(void* p)
+{
+  if (p)
+    (void)hipFree(p);
+}
+
+void h2d(void* dst, const void* src, std::size_t bytes)
+{
+  if (bytes)
+    DFX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
+  DFX_HIP(hipStreamSynchronize(stream()));
+}
+
+void d2h(void* dst, const void* src, std::size_t bytes)
+{
+  if (bytes)
+    DFX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+  DFX_HIP(hipStreamSynchronize(stream()));
+}
+
+std::uint64_t next_pattern_id() { return g_pattern_counter++; }
+
+// ---------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------
+
+constexpr int kBlock = 256;
+
+/// Blocks for a grid-stride elementwise/reduction kernel over n items.
+int reduce_blocks(std::int64_t n)
+{
+  std::int64_t b = (n + kBlock * 4 - 1) / (kBlock * 4);
+  return static_cast<int>(std::clamp<std::int64_t>(b, 1, 2048));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) fill_k(T* v, std::int64_t n, T a)
+{
+  for (std::int64_t i = blockIdx.x * std::int64_t(kBlock) + threadIdx.x; i < n;
+       i += std::int64_t(gridDim.x) * kBlock)
+    v[i] = a;
+}
+
+template <typename T, typename S>
+__global__ void __launch_bounds__(kBlock) convert_k(T* dst, const S* src, std::int64_t n)
+{
+  for (std::int64_t i = blockIdx.x * std::int64_t(kBlock) + threadIdx.x; i < n;
+       i += std::int64_t(gridDim.x) * kBlock)
+    dst[i] = static_cast<T>(src[i]);
+}
+
+/// r = alpha x + y
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    axpy_k(T* r, double alpha, const T* x, const T* y, std::int64_t n)
+{
+  for (std::int64_t i = blockIdx.x * std::int64_t(kBlock) + threadIdx.x; i < n;
+       i += std::int64_t(gridDim.x) * kBlock)
+    r[i] = static_cast<T>(alpha) * x[i] + y[i];
+}
+
+/// out = sum a_i b_i, accumulated in FP64
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    dot_k(const T* a, const T* b, std::int64_t n, double* partials,
+          unsigned* ticket, double* out)
+{
+  double s[1] = {0.0};
+  for (std::int64_t i = blockIdx.x * std::int64_t(kBlock) + threadIdx.x; i < n;
+       i += std::int64_t(gridDim.x) * kBlock)
+    s[0] += double(a[i]) * double(b[i]);
+  grid_reduce<1>(s, partials, ticket, [out](double* t) { *out = t[0]; });
+}
+
+/// Position of the diagonal entry of each block row.
+__global__ void __launch_bounds__(kBlock)
+    diag_pos_k(std::int32_t nrows, const std::int32_t* rp, const std::int32_t* ci,
+               std::int32_t* diag)
+{
+  const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nrows)
+    return;
+  std::int32_t lo = rp[r], hi = rp[r + 1];
+  while (lo < hi)
+  {
+    const std::int32_t mid = (lo + hi) >> 1;
+    if (ci[mid] < r)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  diag[r] = (lo < rp[r + 1] && ci[lo] == r) ? lo : -1;
+}
+
+/// y (+)= A x for scalar CSR: G lanes per row, each lane strides the row,
+/// partial sums combined with wavefront shuffles. Rows are handed out in
+/// XCD-contiguous chunks of kBlock/G rows. Accumulation in T.
+template <typename T, int G>
+__global__ void __launch_bounds__(kBlock)
+    spmv_k(std::int32_t nrows, const std::int32_t* __restrict__ rp,
+           const std::int32_t* __restrict__ ci, const T* __restrict__ v,
+           const T* __restrict__ x, T* __restrict__ y, int accumulate)
+{
+  constexpr int RPB = kBlock / G;
+  const int nchunk = (nrows + RPB - 1) / RPB;
+  const int lane = threadIdx.x % G;
+  for (int chunk = xcd_contiguous(blockIdx.x, gridDim.x); chunk < nchunk;
+       chunk += gridDim.x)
+  {
+    const std::int32_t row = chunk * RPB + threadIdx.x / G;
+    T s = 0;
+    if (row < nrows)
+    {
+      const std::int32_t e = rp[row + 1];
+      for (std::int32_t j = rp[row] + lane; j < e; j += G)
+        s += v[j] * x[ci[j]];
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+      s += __shfl_down(s, off, G);
+    if (lane == 0 && row < nrows)
+      y[row] = accumulate ? y[row] + s : s;
+  }
+}
+
+/// Block (bs x bs) CSR: one lane per (block row, component k0), looping
+/// over the row's blocks; y[bs*i + k0] (+)= sum_j sum_k1 A_j[k0,k1] x[...].
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock)
+    bsr_spmv_k(std::int32_t nrows, const std::int32_t* __restrict__ rp,
+               const std::int32_t* __restrict__ ci, const T* __restrict__ v,
+               const T* __restrict__ x, T* __restrict__ y, int accumulate)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t >= std::int64_t(nrows) * BS)
+    return;
+  const std::int32_t row = static_cast<std::int32_t>(t / BS);
+  const int k0 = static_cast<int>(t % BS);
+  T s = 0;
+  for (std::int32_t j = rp[row]; j < rp[row + 1]; ++j)
+  {
+    const T* a = v + (std::int64_t(j) * BS + k0) * BS;
+    const T* xx = x + std::int64_t(ci[j]) * BS;
+#pragma unroll
+    for (int k1 = 0; k1 < BS; ++k1)
+      s += a[k1] * xx[k1];
+  }
+  y[t] = accumulate ? y[t] + s : s;
+}
+
+template <typename T>
+void launch_spmv(const dfx_csr& A, const T* x, T* y, bool accumulate)
+{
+  hipStream_t s = stream();
+  if (A.bs == 1)
+  {
+    // lanes per row from the mean row length (power of two, 2..64)
+    const double mean = A.nrows ? double(A.nnz) / A.nrows : 1.0;
+    int G = 2;
+    while (G < 64 && G * 3 < mean * 2)
+      G *= 2;
+    const int rpb = kBlock / G;
+    const int nchunk = (A.nrows + rpb - 1) / rpb;
+    const int nb = std::max(1, std::min(nchunk, ctx().num_cu * 16));
+    const T* v = static_cast<const T*>(A.values);
+    switch (G)
+    {
+#define DFX_SPMV_CASE(GG)                                                      \
+  case GG:                                                                     \
+    spmv_k<T, GG><<<nb, kBlock, 0, s>>>(A.nrows, A.row_ptr, A.cols, v, x, y,  \
+                                        accumulate);                           \
+    break;
+      DFX_SPMV_CASE(2)
+      DFX_SPMV_CASE(4)
+      DFX_SPMV_CASE(8)
+      DFX_SPMV_CASE(16)
+      DFX_SPMV_CASE(32)
+      DFX_SPMV_CASE(64)
+#undef DFX_SPMV_CASE
+    }
+  }
+  else
+  {
+    const std::int64_t n = std::int64_t(A.nrows) * A.bs;
+    const int nb = static_cast<int>((n + kBlock - 1) / kBlock);
+    const T* v = static_cast<const T*>(A.values);
+    if (A.bs == 2)
+      bsr_spmv_k<T, 2><<<nb, kBlock, 0, s>>>(A.nrows, A.row_ptr, A.cols, v, x, y, accumulate);
+    else if (A.bs == 3)
+      bsr_spmv_k<T, 3><<<nb, kBlock, 0, s>>>(A.nrows, A.row_ptr, A.cols, v, x, y, accumulate);
+    else
+      throw Error(DFX_ERR_ARG, "block size must be 1..3");
+  }
+  check_launch("spmv");
+}
+
+// ---------------------------------------------------------------------
+// Conjugate gradients, device resident. Scalars live in CGState; each
+// kernel returns at once when the solve has converged, so a batch of
+// queued iterations costs only launch latency after convergence.
+// ---------------------------------------------------------------------
+
+constexpr int kMaxPartials = 4096;
+
+struct CGState
+{
+  double rz, rr, rr0, pq, alpha, beta, tol2;
+  std::int32_t iter, done, max_it, pad;
+  unsigned ticket[4];
+  double partials[3 * kMaxPartials];
+};
+
+/// Jacobi inverse diagonal of a (block) CSR matrix: 1 / A[i,i].
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    inv_diag_k(std::int32_t nrows, int bs, const std::int32_t* diag, const T* v, T* dinv)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t >= std::int64_t(nrows) * bs)
+    return;
+  const std::int32_t r = static_cast<std::int32_t>(t / bs);
+  const int k = static_cast<int>(t % bs);
+  const std::int32_t p = diag[r];
+  const T d = p >= 0 ? v[(std::int64_t(p) * bs + k) * bs + k] : T(0);
+  dinv[t] = d != T(0) ? T(1) / d : T(1);
+}
+
+/// r = b - y; z = dinv r; p = z; reduce (r.z, r.r); starts the solve.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    cg_start_k(std::int64_t n, const T* b, const T* y, const T* dinv, T* r, T* p,
+               CGState* st)
+{
+  double s[2] = {0.0, 0.0};
+  for (std::int64_t i = blockIdx.x * std::int64_t(kBlock) + threadIdx.x; i < n;
+       i += std::int64_t(gridDim.x) * kBlock)
+  {
+    const T ri = b[i] - y[i];
+    const T zi = dinv ? dinv[i] * ri : ri;
+    r[i] = ri;
+    p[i] = zi;
+    s[0] += double(ri) * double(zi);
+    s[1] += double(ri) * double(ri);
+  }
+  grid_reduce<2>(s, st->partials, st->ticket,
+                 [st](double* t)
+                 {
+                   st->rz = t[0];
+                   st->rr = t[1];
+                   st->rr0 = t[1];
+                   st->iter = 0;
+                   st->done = (t[1] == 0.0) ? 1 : 0;
+                 });
+}
+
+/// q = A p and p.q for scalar CSR (G lanes per row); alpha = rz / pq.
+template <typename T, int G>
+__global__ void __launch_bounds__(kBlock)
+    cg_spmv_k(std::int32_t nrows, const std::int32_t* __restrict__ rp,
+              const std::int32_t* __restrict__ ci, const T* __restrict__ v,
+              const T* __restrict__ p, T* __restrict__ q, CGState* st)
+{
+  if (st->done)
+    return;
+  constexpr int RPB = kBlock / G;
+  const int nchunk = (nrows + RPB - 1) / RPB;
+  const int lane = threadIdx.x % G;
+  double d[1] = {0.0};
+  for (int chunk = xcd_contiguous(blockIdx.x, gridDim.x); chunk < nchunk;
+       chunk += gridDim.x)
+  {
+    const std::int32_t row = chunk * RPB + threadIdx.x / G;
+    T s = 0;
+    if (row < nrows)
+    {
+      const std::int32_t e = rp[row + 1];
+      for (std::int32_t j = rp[row] + lane; j < e; j += G)
+        s += v[j] * p[ci[j]];
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+      s += __shfl_down(s, off, G);
+    if (lane == 0 && row < nrows)
+    {
+      q[row] = s;
+      d[0] += double(s) * double(p[row]);
+    }
+  }
+  grid_reduce<1>(d, st->partials, st->ticket + 1,
+                 [st](double* t)
+                 {
+                   st->pq = t[0];
+                   st->alpha = st->rz / t[0];
+                 });
+}
+
+/// Block CSR variant of cg_spmv_k (one lane per (block row, component)).
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock)
+    cg_bsr_spmv_k(std::int32_t nrows, const std::int32_t* __restrict__ rp,
+                  const std::int32_t* __restrict__ ci, const T* __restrict__ v,
+                  const T* __restrict__ p, T* __restrict__ q, CGState* st)
+{
+  if (st->done)
+    return;
+  const std::int64_t n = std::int64_t(nrows) * BS;
+  double d[1] = {0.0};
+  for (std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x; t < n;
+       t += std::int64_t(gridDim.x) * kBlock)
+  {
+    const std::int32_t row = static_cast<std::int32_t>(t / BS);
+    const int k0 = static_cast<int>(t % BS);
+    T s = 0;
+    for (std::int32_t j = rp[row]; j < rp[row + 1]; ++j)
+    {
+      const T* a = v + (std::int64_t(j) * BS + k0) * BS;
+      const T* xx = p + std::int64_t(ci[j]) * BS;
+#pragma unroll
+      for (int k1 = 0; k1 < BS; ++k1)
+        s += a[k1] * xx[k1];
+    }
+    q[t] = s;
+    d[0] += double(s) * double(p[t]);
+  }
+  grid_reduce<1>(d, st->partials, st->ticket + 1,
+                 [st](double* t)
+                 {
+                   st->pq = t[0];
+                   st->alpha = st->rz / t[0];
+                 });
+}
+
+/// x += alpha p; r -= alpha q; reduce (r.z, r.r) with z = dinv r;
+/// convergence test ||r||^2/||r0||^2 < rtol^2; beta = rz_new / rz.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    cg_update_k(std::int64_t n, T* x, T* r, const T* p, const T* q, const T* dinv,
+                CGState* st)
+{
+  if (st->done)
+    return;
+  const T alpha = static_cast<T>(st->alpha);
+  double s[2] = {0.0, 0.0};
+  for (std::int64_t i = blockIdx.x * std::int64_t(kBlock) + threadIdx.x; i < n;
+       i += std::int64_t(gridDim.x) * kBlock)
+  {
+    x[i] += alpha * p[i];
+    const T ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    const T zi = dinv ? dinv[i] * ri : ri;
+    s[0] += double(ri) * double(zi);
+    s[1] += double(ri) * double(ri);
+  }
+  grid_reduce<2>(s, st->partials, st->ticket + 2,
+                 [st](double* t)
+                 {
+                   st->iter += 1;
+                   st->rr = t[1];
+                   st->beta = t[0] / st->rz;
+                   st->rz = t[0];
+                   if (t[1] / st->rr0 < st->tol2 || st->iter >= st->max_it)
+                     st->done = 1;
+                 });
+}
+
+/// p = dinv r + beta p
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    cg_direction_k(std::int64_t n, const T* r, const T* dinv, T* p, const CGState* st)
+{
+  if (st->done)
+    return;
+  const T beta = static_cast<T>(st->beta);
+  for (std::int64_t i = blockIdx.x * std::int64_t(kBlock) + threadIdx.x; i < n;
+       i += std::int64_t(gridDim.x) * kBlock)
+    p[i] = (dinv ? dinv[i] * r[i] : r[i]) + beta * p[i];
+}
+
+/// Per-dtype scratch of a CG solve (device).
+template <typename T>
+struct CGWork
+{
+  std::int64_t n = 0;
+  T *r = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr;
+  void alloc(std::int64_t nn, bool jacobi)
+  {
+    n = nn;
+    r = static_cast<T*>(dmalloc(sizeof(T) * n));
+    p = static_cast<T*>(dmalloc(sizeof(T) * n));
+    q = static_cast<T*>(dmalloc(sizeof(T) * n));
+    if (jacobi)
+      dinv = static_cast<T*>(dmalloc(sizeof(T) * n));
+  }
+  ~CGWork()
+  {
+    dfree(r);
+    dfree(p);
+    dfree(q);
+    dfree(dinv);
+  }
+};
+
+struct StateHolder
+{
+  CGState* st = nullptr;
+  StateHolder() { st = static_cast<CGState*>(dmalloc(sizeof(CGState))); }
+  ~StateHolder() { dfree(st); }
+};
+
+template <typename T>
+void launch_cg_spmv(const dfx_csr& A, const T* p, T* q, CGState* st)
+{
+  hipStream_t s = stream();
+  const T* v = static_cast<const T*>(A.values);
+  if (A.bs == 1)
+  {
+    const double mean = A.nrows ? double(A.nnz) / A.nrows : 1.0;
+    int G = 2;
+    while (G < 64 && G * 3 < mean * 2)
+      G *= 2;
+    const int rpb = kBlock / G;
+    const int nchunk = (A.nrows + rpb - 1) / rpb;
+    const int nb = std::max(1, std::min({nchunk, ctx().num_cu * 8, kMaxPartials}));
+    switch (G)
+    {
+#define DFX_CG_CASE(GG)                                                        \
+  case GG:                                                                     \
+    cg_spmv_k<T, GG><<<nb, kBlock, 0, s>>>(A.nrows, A.row_ptr, A.cols, v, p, q, st); \
+    break;
+      DFX_CG_CASE(2)
+      DFX_CG_CASE(4)
+      DFX_CG_CASE(8)
+      DFX_CG_CASE(16)
+      DFX_CG_CASE(32)
+      DFX_CG_CASE(64)
+#undef DFX_CG_CASE
+    }
+  }
+  else
+  {
+    const int nb = std::min(reduce_blocks(std::int64_t(A.nrows) * A.bs), kMaxPartials);
+    if (A.bs == 3)
+      cg_bsr_spmv_k<T, 3><<<nb, kBlock, 0, s>>>(A.nrows, A.row_ptr, A.cols, v, p, q, st);
+    else if (A.bs == 2)
+      cg_bsr_spmv_k<T, 2><<<nb, kBlock, 0, s>>>(A.nrows, A.row_ptr, A.cols, v, p, q, st);
+    else
+      throw Error(DFX_ERR_ARG, "block size must be 1..3");
+  }
+}
+
+/// Runs (preconditioned) CG on A x = b. All arrays are device arrays of T.
+/// Returns iterations; sets *converged.
+template <typename T>
+int cg_run(const dfx_csr& A, const T* b, T* x, int max_it, double rtol, bool jacobi,
+           int check_every, CGWork<T>& w, CGState* st, bool* converged, double* rr0,
+           double* rr)
+{
+  hipStream_t s = stream();
+  const std::int64_t n = std::int64_t(A.nrows) * A.bs;
+  const int nbv = reduce_blocks(n);
+  if (jacobi)
+  {
+    const int nb = static_cast<int>((n + kBlock - 1) / kBlock);
+    inv_diag_k<T><<<nb, kBlock, 0, s>>>(A.nrows, A.bs, A.diag,
+                                        static_cast<const T*>(A.values), w.dinv);
+  }
+  // q = A x (initial residual), then r, p and the first reductions
+  launch_spmv<T>(A, x, w.q, false);
+  CGState init{};
+  init.tol2 = rtol * rtol;
+  init.max_it = max_it;
+  DFX_HIP(hipMemcpyAsync(st, &init, offsetof(CGState, partials), hipMemcpyHostToDevice, s));
+  cg_start_k<T><<<nbv, kBlock, 0, s>>>(n, b, w.q, jacobi ? w.dinv : nullptr, w.r, w.p, st);
+  check_launch("cg_start");
+  struct
+  {
+    std::int32_t iter, done;
+  } h{0, 0};
+  const int batch = std::max(1, check_every);
+  int queued = 0;
+  while (true)
+  {
+    for (int k = 0; k < batch && queued < max_it; ++k, ++queued)
+    {
+      launch_cg_spmv<T>(A, w.p, w.q, st);
+      cg_update_k<T><<<nbv, kBlock, 0, s>>>(n, x, w.r, w.p, w.q, jacobi ? w.dinv : nullptr, st);
+      cg_direction_k<T><<<nbv, kBlock, 0, s>>>(n, w.r, jacobi ? w.dinv : nullptr, w.p, st);
+    }
+    check_launch("cg iteration");
+    DFX_HIP(hipMemcpyAsync(&h, &st->iter, sizeof(h), hipMemcpyDeviceToHost, s));
+    DFX_HIP(hipStreamSynchronize(s));
+    if (h.done || queued >= max_it)
+      break;
+  }
+  CGState fin;
+  DFX_HIP(hipMemcpy(&fin, st, offsetof(CGState, ticket), hipMemcpyDeviceToHost));
+  *rr0 = fin.rr0;
+  *rr = fin.rr;
+  *converged = fin.rr0 == 0.0 || fin.rr / fin.rr0 < fin.tol2;
+  return fin.iter;
+}
+
+/// FP64 true residual norm: ||b - A x||_2, with y scratch of T.
+template <typename T>
+double true_residual(const dfx_csr& A, const T* b, const T* x, T* y, double* partials,
+                     unsigned* ticket, double* out)
+{
+  hipStream_t s = stream();
+  const std::int64_t n = std::int64_t(A.nrows) * A.bs;
+  launch_spmv<T>(A, x, y, false);
+  const int nb = reduce_blocks(n);
+  axpy_k<T><<<nb, kBlock, 0, s>>>(y, -1.0, y, b, n);
+  dot_k<T><<<nb, kBlock, 0, s>>>(y, y, n, partials, ticket, out);
+  check_launch("residual");
+  double h = 0;
+  d2h(&h, out, sizeof(double));
+  return std::sqrt(h);
+}
+
+struct Reducer
+{
+  double* partials = nullptr;
+  unsigned* ticket = nullptr;
+  double* out = nullptr;
+  Reducer()
+  {
+    char* p = static_cast<char*>(dmalloc(8 * 2048 + 256));
+    partials = reinterpret_cast<double*>(p);
+    ticket = reinterpret_cast<unsigned*>(p + 8 * 2048);
+    out = reinterpret_cast<double*>(p + 8 * 2048 + 64);
+    DFX_HIP(hipMemsetAsync(p, 0, 8 * 2048 + 256, stream()));
+  }
+  ~Reducer() { dfree(partials); }
+  template <typename T>
+  double dot(const T* a, const T* b, std::int64_t n)
+  {
+    dot_k<T><<<reduce_blocks(n), kBlock, 0, stream()>>>(a, b, n, partials, ticket, out);
+    check_launch("dot");
+    double h = 0;
+    d2h(&h, out, sizeof(double));
+    return h;
+  }
+};
+
+Reducer& reducer()
+{
+  static Reducer* r = new Reducer();
+  return *r;
+}
+
+} // namespace dfx
+
+namespace dfx
+{
+template <typename T>
+void cg_solve_t(const dfx_csr& A, const dfx_vec& b, dfx_vec& x,
+                       const dfx_cg_opts& o, dfx_cg_result& res)
+{
+  const std::int64_t n = std::int64_t(A.nrows) * A.bs;
+  CGWork<T> w;
+  w.alloc(n, o.jacobi != 0);
+  StateHolder sh;
+  DFX_HIP(hipMemsetAsync(sh.st, 0, sizeof(CGState), stream()));
+  DFX_HIP(hipStreamSynchronize(stream()));
+  const auto t0 = std::chrono::steady_clock::now();
+  bool conv = false;
+  double rr0 = 0, rr = 0;
+  const int it = cg_run<T>(A, static_cast<const T*>(b.data), static_cast<T*>(x.data), o.max_it,
+                           o.rtol, o.jacobi != 0, o.check_every, w, sh.st, &conv, &rr0, &rr);
+  DFX_HIP(hipStreamSynchronize(stream()));
+  const auto t1 = std::chrono::steady_clock::now();
+  res.iterations = it;
+  res.converged = conv ? 1 : 0;
+  res.rnorm0 = std::sqrt(rr0);
+  res.rnorm = std::sqrt(rr);
+  res.seconds = std::chrono::duration<double>(t1 - t0).count();
+  Reducer& red = reducer();
+  const double bn = std::sqrt(red.dot(static_cast<const T*>(b.data), static_cast<const T*>(b.data), n));
+  const double tr = true_residual<T>(A, static_cast<const T*>(b.data), static_cast<const T*>(x.data),
+                                     w.q, red.partials, red.ticket, red.out);
+  res.true_rel_residual = bn > 0 ? tr / bn : tr;
+}
+
+/// x += d (FP64 update from an FP32 correction)
+__global__ void __launch_bounds__(kBlock) add_correction_k(double* x, const float* d, std::int64_t n)
+{
+  for (std::int64_t i = blockIdx.x * std::int64_t(kBlock) + threadIdx.x; i < n;
+       i += std::int64_t(gridDim.x) * kBlock)
+    x[i] += static_cast<double>(d[i]);
+}
+
+/// r = b - y (FP64), r32 = (float) r, reduce ||r||^2
+__global__ void __launch_bounds__(kBlock)
+    residual_split_k(const double* b, const double* y, float* r32, std::int64_t n,
+                     double* partials, unsigned* ticket, double* out)
+{
+  double s[1] = {0.0};
+  for (std::int64_t i = blockIdx.x * std::int64_t(kBlock) + threadIdx.x; i < n;
+       i += std::int64_t(gridDim.x) * kBlock)
+  {
+    const double ri = b[i] - y[i];
+    r32[i] = static_cast<float>(ri);
+    s[0] += ri * ri;
+  }
+  grid_reduce<1>(s, partials, ticket, [out](double* t) { *out = t[0]; });
+}
+
+} // namespace dfx
+
+using namespace dfx;
+
+extern "C" {
+
+const char* dfx_last_error(void) { return dfx::g_last_error.c_str(); }
+
+const char* dfx_version(void) { return "dolfinx_amd 0.1 (gfx950)"; }
+
+dfx_status dfx_init(void)
+{
+  return guard([] { ensure_init(); });
+}
+
+dfx_status dfx_synchronize(void)
+{
+  return guard([] { DFX_HIP(hipStreamSynchronize(stream())); });
+}
+
+void* dfx_stream(void)
+{
+  void* s = nullptr;
+  guard([&] { s = static_cast<void*>(stream()); });
+  return s;
+}
+
+dfx_status dfx_event_record(int slot)
+{
+  return guard(
+      [&]
+      {
+        require(slot >= 0 && slot < 64, "event slot out of range");
+        DFX_HIP(hipEventRecord(ctx().events[slot], stream()));
+      });
+}
+
+dfx_status dfx_event_elapsed(int a, int b, float* ms)
+{
+  return guard(
+      [&]
+      {
+        require(a >= 0 && a < 64 && b >= 0 && b < 64 && ms, "bad event arguments");
+        ensure_init();
+        DFX_HIP(hipEventSynchronize(ctx().events[b]));
+        DFX_HIP(hipEventElapsedTime(ms, ctx().events[a], ctx().events[b]));
+      });
+}
+
+// ---- MatrixCSR ----
+
+dfx_status dfx_csr_create(const int64_t* row_ptr, const int32_t* cols, int32_t nrows,
+                          int32_t ncols, int bs, int dtype, dfx_csr** out)
+{
+  return guard(
+      [&]
+      {
+        require(row_ptr && cols && out, "null argument");
+        require(nrows >= 0 && ncols >= 0, "negative size");
+        require(bs >= 1 && bs <= 3, "block size must be 1..3");
+        require(dtype == DFX_F32 || dtype == DFX_F64, "bad dtype");
+        const std::int64_t nnz = row_ptr[nrows];
+        require(row_ptr[0] == 0 && nnz < INT32_MAX, "row_ptr out of range");
+        std::vector<std::int32_t> rp32(std::size_t(nrows) + 1);
+        std::int32_t maxrow = 0;
+        for (std::int32_t i = 0; i <= nrows; ++i)
+        {
+          rp32[i] = static_cast<std::int32_t>(row_ptr[i]);
+          if (i > 0)
+          {
+            require(row_ptr[i] >= row_ptr[i - 1], "row_ptr not monotone");
+            maxrow = std::max<std::int32_t>(maxrow, rp32[i] - rp32[i - 1]);
+          }
+        }
+        for (std::int64_t j = 0; j < nnz; ++j)
+          require(cols[j] >= 0 && cols[j] < ncols, "column index out of range");
+        auto A = std::make_unique<dfx_csr>();
+        A->nrows = nrows;
+        A->ncols = ncols;
+        A->bs = bs;
+        A->dtype = dtype;
+        A->nnz = nnz;
+        A->max_row = maxrow;
+        A->row_ptr = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * (nrows + 1)));
+        A->cols = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * std::max<std::int64_t>(nnz, 1)));
+        A->diag = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * std::max(nrows, 1)));
+        const std::size_t vb = dtype_size(dtype) * std::size_t(nnz) * bs * bs;
+        A->values = dmalloc(std::max<std::size_t>(vb, 8));
+        h2d(A->row_ptr, rp32.data(), sizeof(std::int32_t) * (nrows + 1));
+        h2d(A->cols, cols, sizeof(std::int32_t) * nnz);
+        DFX_HIP(hipMemsetAsync(A->values, 0, vb, stream()));
+        if (nrows > 0)
+          diag_pos_k<<<(nrows + kBlock - 1) / kBlock, kBlock, 0, stream()>>>(nrows, A->row_ptr, A->cols, A->diag);
+        check_launch("csr_create");
+        DFX_HIP(hipStreamSynchronize(stream()));
+        A->pattern_id = next_pattern_id();
+        *out = A.release();
+      });
+}
+
+dfx_status dfx_csr_destroy(dfx_csr* A)
+{
+  if (A)
+  {
+    dfree(A->row_ptr);
+    dfree(A->cols);
+    dfree(A->diag);
+    dfree(A->values);
+    delete A;
+  }
+  return DFX_OK;
+}
+
+dfx_status dfx_csr_sizes(const dfx_csr* A, int32_t* nrows, int64_t* nnz, int32_t* bs,
+                         int32_t* dtype)
+{
+  return guard(
+      [&]
+      {
+        require(A, "null matrix");
+        if (nrows)
+          *nrows = A->nrows;
+        if (nnz)
+          *nnz = A->nnz;
+        if (bs)
+          *bs = A->bs;
+        if (dtype)
+          *dtype = A->dtype;
+      });
+}
+
+dfx_status dfx_csr_zero(dfx_csr* A)
+{
+  return guard(
+      [&]
+      {
+        require(A, "null matrix");
+        const std::int64_t n = A->nnz * A->bs * A->bs;
+        const int nb = reduce_blocks(n);
+        if (A->dtype == DFX_F64)
+          fill_k<double><<<nb, kBlock, 0, stream()>>>(static_cast<double*>(A->values), n, 0.0);
+        else
+          fill_k<float><<<nb, kBlock, 0, stream()>>>(static_cast<float*>(A->values), n, 0.0f);
+        check_launch("csr_zero");
+      });
+}
+
+dfx_status dfx_csr_get_values(const dfx_csr* A, void* values)
+{
+  return guard(
+      [&]
+      {
+        require(A && values, "null argument");
+        d2h(values, A->values, dtype_size(A->dtype) * A->nnz * A->bs * A->bs);
+      });
+}
+
+dfx_status dfx_csr_set_values(dfx_csr* A, const void* values)
+{
+  return guard(
+      [&]
+      {
+        require(A && values, "null argument");
+        h2d(A->values, values, dtype_size(A->dtype) * A->nnz * A->bs * A->bs);
+      });
+}
+
+dfx_status dfx_csr_get_pattern(const dfx_csr* A, int64_t* row_ptr, int32_t* cols)
+{
+  return guard(
+      [&]
+      {
+        require(A, "null matrix");
+        if (row_ptr)
+        {
+          std::vector<std::int32_t> rp(std::size_t(A->nrows) + 1);
+          d2h(rp.data(), A->row_ptr, sizeof(std::int32_t) * rp.size());
+          for (std::size_t i = 0; i < rp.size(); ++i)
+            row_ptr[i] = rp[i];
+        }
+        if (cols)
+          d2h(cols, A->cols, sizeof(std::int32_t) * A->nnz);
+      });
+}
+
+// ---- Vector ----
+
+dfx_status dfx_vec_create(int32_t n, int dtype, dfx_vec** out)
+{
+  return guard(
+      [&]
+      {
+        require(out && n >= 0, "bad argument");
+        require(dtype == DFX_F32 || dtype == DFX_F64, "bad dtype");
+        auto v = std::make_unique<dfx_vec>();
+        v->n = n;
+        v->dtype = dtype;
+        v->data = dmalloc(dtype_size(dtype) * std::max(n, 1));
+        DFX_HIP(hipMemsetAsync(v->data, 0, dtype_size(dtype) * std::max(n, 1), stream()));
+        DFX_HIP(hipStreamSynchronize(stream()));
+        *out = v.release();
+      });
+}
+
+dfx_status dfx_vec_destroy(dfx_vec* v)
+{
+  if (v)
+  {
+    dfree(v->data);
+    delete v;
+  }
+  return DFX_OK;
+}
+
+dfx_status dfx_vec_size(const dfx_vec* v, int32_t* n, int32_t* dtype)
+{
+  return guard(
+      [&]
+      {
+        require(v, "null vector");
+        if (n)
+          *n = v->n;
+        if (dtype)
+          *dtype = v->dtype;
+      });
+}
+
+dfx_status dfx_vec_upload(dfx_vec* v, const void* host)
+{
+  return guard(
+      [&]
+      {
+        require(v && host, "null argument");
+        h2d(v->data, host, dtype_size(v->dtype) * v->n);
+      });
+}
+
+dfx_status dfx_vec_download(const dfx_vec* v, void* host)
+{
+  return guard(
+      [&]
+      {
+        require(v && host, "null argument");
+        d2h(host, v->data, dtype_size(v->dtype) * v->n);
+      });
+}
+
+dfx_status dfx_vec_set(dfx_vec* v, double value)
+{
+  return guard(
+      [&]
+      {
+        require(v, "null vector");
+        const int nb = reduce_blocks(v->n);
+        if (v->dtype == DFX_F64)
+          fill_k<double><<<nb, kBlock, 0, stream()>>>(static_cast<double*>(v->data), v->n, value);
+        else
+          fill_k<float><<<nb, kBlock, 0, stream()>>>(static_cast<float*>(v->data), v->n,
+                                                      static_cast<float>(value));
+        check_launch("vec_set");
+      });
+}
+
+dfx_status dfx_vec_copy(dfx_vec* dst, const dfx_vec* src)
+{
+  return guard(
+      [&]
+      {
+        require(dst && src && dst->n == src->n, "size mismatch");
+        const int nb = reduce_blocks(src->n);
+        hipStream_t s = stream();
+        if (dst->dtype == DFX_F64 && src->dtype == DFX_F64)
+          convert_k<double, double><<<nb, kBlock, 0, s>>>(static_cast<double*>(dst->data), static_cast<const double*>(src->data), src->n);
+        else if (dst->dtype == DFX_F64)
+          convert_k<double, float><<<nb, kBlock, 0, s>>>(static_cast<double*>(dst->data), static_cast<const float*>(src->data), src->n);
+        else if (src->dtype == DFX_F64)
+          convert_k<float, double><<<nb, kBlock, 0, s>>>(static_cast<float*>(dst->data), static_cast<const double*>(src->data), src->n);
+        else
+          convert_k<float, float><<<nb, kBlock, 0, s>>>(static_cast<float*>(dst->data), static_cast<const float*>(src->data), src->n);
+        check_launch("vec_copy");
+      });
+}
+
+dfx_status dfx_spmv(const dfx_csr* A, const dfx_vec* x, dfx_vec* y)
+{
+  return guard(
+      [&]
+      {
+        require(A && x && y, "null argument");
+        require(x->dtype == A->dtype && y->dtype == A->dtype, "dtype mismatch");
+        require(x->n == A->ncols * A->bs && y->n == A->nrows * A->bs, "size mismatch");
+        if (A->dtype == DFX_F64)
+          launch_spmv<double>(*A, static_cast<const double*>(x->data), static_cast<double*>(y->data), true);
+        else
+          launch_spmv<float>(*A, static_cast<const float*>(x->data), static_cast<float*>(y->data), true);
+      });
+}
+
+dfx_status dfx_inner_product(const dfx_vec* a, const dfx_vec* b, double* out)
+{
+  return guard(
+      [&]
+      {
+        require(a && b && out && a->n == b->n && a->dtype == b->dtype, "mismatch");
+        if (a->dtype == DFX_F64)
+          *out = reducer().dot(static_cast<const double*>(a->data), static_cast<const double*>(b->data), a->n);
+        else
+          *out = reducer().dot(static_cast<const float*>(a->data), static_cast<const float*>(b->data), a->n);
+      });
+}
+
+dfx_status dfx_squared_norm(const dfx_vec* a, double* out)
+{
+  return dfx_inner_product(a, a, out);
+}
+
+dfx_status dfx_axpy(dfx_vec* r, double alpha, const dfx_vec* x, const dfx_vec* y)
+{
+  return guard(
+      [&]
+      {
+        require(r && x && y && r->n == x->n && x->n == y->n, "size mismatch");
+        require(r->dtype == x->dtype && x->dtype == y->dtype, "dtype mismatch");
+        const int nb = reduce_blocks(r->n);
+        if (r->dtype == DFX_F64)
+          axpy_k<double><<<nb, kBlock, 0, stream()>>>(static_cast<double*>(r->data), alpha, static_cast<const double*>(x->data), static_cast<const double*>(y->data), r->n);
+        else
+          axpy_k<float><<<nb, kBlock, 0, stream()>>>(static_cast<float*>(r->data), alpha, static_cast<const float*>(x->data), static_cast<const float*>(y->data), r->n);
+        check_launch("axpy");
+      });
+}
+
+// ---- Solvers ----
+
+dfx_status dfx_cg_solve(const dfx_csr* A, const dfx_vec* b, dfx_vec* x,
+                        const dfx_cg_opts* opts, dfx_cg_result* res)
+{
+  return guard(
+      [&]
+      {
+        require(A && b && x && opts && res, "null argument");
+        require(b->dtype == A->dtype && x->dtype == A->dtype, "dtype mismatch");
+        require(A->nrows == A->ncols, "matrix not square");
+        require(b->n == A->nrows * A->bs && x->n == b->n, "size mismatch");
+        require(opts->max_it >= 0 && opts->rtol >= 0, "bad options");
+        *res = dfx_cg_result{};
+        if (A->dtype == DFX_F64)
+          cg_solve_t<double>(*A, *b, *x, *opts, *res);
+        else
+          cg_solve_t<float>(*A, *b, *x, *opts, *res);
+        if (!res->converged)
+          throw Error(DFX_ERR_NOCONV, "CG exceeded max iterations ("
+                                          + std::to_string(opts->max_it)
+                                          + "). Relative residual="
+                                          + std::to_string(res->rnorm / std::max(res->rnorm0, 1e-300)));
+      });
+}
+
+dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec* b,
+                           dfx_vec* x, const dfx_mixed_opts* o, dfx_mixed_result* res)
+{
+  return guard(
+      [&]
+      {
+        require(A64 && A32 && b && x && o && res, "null argument");
+        require(A64->dtype == DFX_F64 && A32->dtype == DFX_F32, "need FP64 and FP32 matrices");
+        require(b->dtype == DFX_F64 && x->dtype == DFX_F64, "b and x must be FP64");
+        require(A64->nrows == A32->nrows && A64->nnz == A32->nnz && A64->bs == A32->bs,
+                "matrices differ in shape");
+        require(b->n == A64->nrows * A64->bs && x->n == b->n, "size mismatch");
+        *res = dfx_mixed_result{};
+        hipStream_t s = stream();
+        const std::int64_t n = b->n;
+        const double* bd = static_cast<const double*>(b->data);
+        double* xd = static_cast<double*>(x->data);
+        CGWork<float> w;
+        w.alloc(n, true);
+        StateHolder sh;
+        float* r32 = static_cast<float*>(dmalloc(sizeof(float) * n));
+        float* d32 = static_cast<float*>(dmalloc(sizeof(float) * n));
+        double* y = static_cast<double*>(dmalloc(sizeof(double) * n));
+        Reducer& red = reducer();
+        DFX_HIP(hipMemsetAsync(sh.st, 0, sizeof(CGState), s));
+        DFX_HIP(hipStreamSynchronize(s));
+        const int nb = reduce_blocks(n);
+        const auto t0 = std::chrono::steady_clock::now();
+        const double bn2 = red.dot(bd, bd, n);
+        const double tol2 = o->rtol * o->rtol * bn2;
+        int outer = 0, inner = 0;
+        double rr = 0;
+        bool ok = false;
+        for (;;)
+        {
+          launch_spmv<double>(*A64, xd, y, false);
+          residual_split_k<<<nb, kBlock, 0, s>>>(bd, y, r32, n, red.partials, red.ticket, red.out);
+          check_launch("residual");
+          d2h(&rr, red.out, sizeof(double));
+          if (rr <= tol2)
+          {
+            ok = true;
+            break;
+          }
+          if (outer >= o->max_outer)
+            break;
+          fill_k<float><<<nb, kBlock, 0, s>>>(d32, n, 0.0f);
+          bool conv = false;
+          double a0 = 0, a1 = 0;
+          inner += cg_run<float>(*A32, r32, d32, o->max_inner, o->inner_rtol, true,
+                                 o->check_every, w, sh.st, &conv, &a0, &a1);
+          add_correction_k<<<nb, kBlock, 0, s>>>(xd, d32, n);
+          check_launch("correction");
+          ++outer;
+        }
+        DFX_HIP(hipStreamSynchronize(s));
+        const auto t1 = std::chrono::steady_clock::now();
+        dfree(r32);
+        dfree(d32);
+        dfree(y);
+        res->outer = outer;
+        res->inner_total = inner;
+        res->converged = ok ? 1 : 0;
+        res->true_rel_residual = bn2 > 0 ? std::sqrt(rr / bn2) : std::sqrt(rr);
+        res->seconds = std::chrono::duration<double>(t1 - t0).count();
+        if (!ok)
+          throw Error(DFX_ERR_NOCONV, "mixed-precision refinement did not converge");
+      });
+}
+
+} // extern "C"

@@ -1,0 +1,255 @@
+"""Finite element assembly (mirror of dolfinx.fem for this path).
+
+The UFL/FFCx front end is replaced by a fixed set of hand-written device
+kernels (one per config form); ``form`` selects one by name and packs its
+coefficients like fem::pack_coefficients (cpp/dolfinx/fem/pack.h:265-388).
+Assembly, lifting and BC application run on the GPU through include/dfx.h.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .la import MatrixCSR, SparsityPattern, Vector
+from .mesh import CellType, Mesh
+
+
+class DofMap:
+    def __init__(self, dofmap, bs, num_nodes):
+        self.list = dofmap            # (num_cells, dofs_per_cell) node indices
+        self.bs = bs
+        self.index_map_size = num_nodes
+
+    def cell_dofs(self, c):
+        return self.list[c]
+
+
+class FunctionSpace:
+    """Lagrange space (fem::create_functionspace, fem/utils.h:925-950)."""
+
+    def __init__(self, mesh: Mesh, degree: int, bs: int = 1):
+        self.mesh = mesh
+        self.degree = degree
+        self._h = C.c_void_p()
+        _lib.call("dfx_space_create", mesh.handle, degree, bs, C.byref(self._h))
+        nn, k, b = C.c_int32(), C.c_int32(), C.c_int32()
+        _lib.call("dfx_space_sizes", self._h, C.byref(nn), C.byref(k), C.byref(b))
+        dm = np.zeros((mesh.topology.cells.shape[0], k.value), np.int32)
+        _lib.call("dfx_space_dofmap", self._h, _lib.ptr(dm))
+        self.dofmap = DofMap(dm, b.value, nn.value)
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def num_dofs(self):
+        """Unrolled size (nodes * block size)."""
+        return self.dofmap.index_map_size * self.dofmap.bs
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            try:
+                _lib.lib().dfx_space_destroy(h)
+            except Exception:
+                pass
+
+
+def functionspace(mesh: Mesh, element) -> FunctionSpace:
+    """functionspace(mesh, ("Lagrange", degree[, (3,)]))."""
+    family, degree = element[0], element[1]
+    if family not in ("Lagrange", "P", "Q"):
+        raise ValueError(f"unsupported family {family}")
+    bs = 1
+    if len(element) > 2 and element[2]:
+        bs = int(np.prod(element[2]))
+    return FunctionSpace(mesh, int(degree), bs)
+
+
+def locate_boundary_dofs(V: FunctionSpace, axis=None, value=0.0) -> np.ndarray:
+    """Unrolled dofs on exterior facets (all of them, or those with every
+    vertex at x[axis] == value): locate_dofs_topological on the facets of
+    exterior_facet_indices / locate_entities_boundary."""
+    marker = 0 if axis is None else 1
+    n = C.c_int32()
+    _lib.call("dfx_locate_boundary_dofs", V.handle, marker, 0 if axis is None else axis,
+              float(value), C.byref(n), None)
+    out = np.zeros(n.value, np.int32)
+    _lib.call("dfx_locate_boundary_dofs", V.handle, marker, 0 if axis is None else axis,
+              float(value), C.byref(n), _lib.ptr(out))
+    return out
+
+
+class DirichletBC:
+    """fem::DirichletBC on unrolled dofs with values g (fem/DirichletBC.h)."""
+
+    def __init__(self, V: FunctionSpace, dofs, g=None):
+        self.V = V
+        self.dofs = np.ascontiguousarray(dofs, np.int32)
+        n = V.num_dofs
+        self.g = None if g is None else np.ascontiguousarray(g, np.float64)
+        if self.g is not None and self.g.size != n:
+            raise ValueError("g must have one value per dof of V")
+        self._h = C.c_void_p()
+        _lib.call("dfx_bc_create", _lib.ptr(self.dofs), len(self.dofs), n, _lib.ptr(self.g),
+                  C.byref(self._h))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def markers(self):
+        m = np.zeros(self.V.num_dofs, np.int8)
+        m[self.dofs] = 1
+        return m
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            try:
+                _lib.lib().dfx_bc_destroy(h)
+            except Exception:
+                pass
+
+
+def dirichletbc(V: FunctionSpace, dofs, g=None) -> DirichletBC:
+    return DirichletBC(V, dofs, g)
+
+
+# name -> (kernel id, cell type, degree, block size, rank)
+KERNELS = {
+    "p1_stiffness_reaction": (_lib.K_P1TET_A, CellType.tetrahedron, 1, 1, 2),
+    "p1_source": (_lib.K_P1TET_L, CellType.tetrahedron, 1, 1, 1),
+    "q1_stiffness_reaction": (_lib.K_Q1HEX_A, CellType.hexahedron, 1, 1, 2),
+    "q1_source": (_lib.K_Q1HEX_L, CellType.hexahedron, 1, 1, 1),
+    "p2_elasticity": (_lib.K_P2TET_ELAST_A, CellType.tetrahedron, 2, 3, 2),
+    "p2_body_force": (_lib.K_P2TET_ELAST_L, CellType.tetrahedron, 2, 3, 1),
+    "p3_stiffness_reaction": (_lib.K_P3TET_A, CellType.tetrahedron, 3, 1, 2),
+    "p3_source": (_lib.K_P3TET_L, CellType.tetrahedron, 3, 1, 1),
+}
+
+
+def pack_cellwise(mesh: Mesh, values_by_input_cell) -> np.ndarray:
+    """DG0 coefficient(s) given per input (lexicographic) cell, packed in the
+    mesh's cell order: (num_cells, ncomp)."""
+    v = np.asarray(values_by_input_cell, np.float64)
+    v = v.reshape(v.shape[0], -1)
+    return np.ascontiguousarray(v[mesh.topology.original_cell_index])
+
+
+def pack_vertexwise(mesh: Mesh, values_by_input_vertex) -> np.ndarray:
+    """P1/Q1 coefficient given per input vertex, packed per cell vertex:
+    (num_cells, vertices_per_cell)."""
+    f = np.asarray(values_by_input_vertex, np.float64)
+    fv = f[mesh.geometry.input_global_indices]  # by local vertex
+    return np.ascontiguousarray(fv[mesh.topology.cells])
+
+
+class Form:
+    """fem::Form holding device copies of geometry, dofmap and packed
+    coefficients (fem/Form.h:116-)."""
+
+    def __init__(self, name, V: FunctionSpace, coeffs=None, constants=None, dtype=np.float64):
+        if name not in KERNELS:
+            raise ValueError(f"unknown form {name}")
+        kid, ct, deg, bs, rank = KERNELS[name]
+        mesh = V.mesh
+        if mesh.topology.cell_type != ct or V.degree != deg or V.dofmap.bs != bs:
+            raise ValueError(f"form {name} does not match the space")
+        self.name, self.V, self.rank, self.dtype = name, V, rank, np.dtype(dtype)
+        self.coeffs = None if coeffs is None else np.ascontiguousarray(coeffs, np.float64)
+        nc = mesh.topology.cells.shape[0]
+        if self.coeffs is not None:
+            self.coeffs = self.coeffs.reshape(nc, -1)
+        self.constants = np.ascontiguousarray(
+            np.zeros(1) if constants is None else np.atleast_1d(constants), np.float64)
+        x = np.ascontiguousarray(mesh.geometry.x)
+        xd = np.ascontiguousarray(mesh.geometry.dofmap)
+        dm = np.ascontiguousarray(V.dofmap.list)
+        d = _lib.FormDesc()
+        d.kernel, d.dtype = kid, _lib.code_of(dtype)
+        d.num_cells, d.num_nodes, d.nodes_per_cell = nc, x.shape[0], xd.shape[1]
+        d.x, d.x_dofmap = _lib.ptr(x), _lib.ptr(xd)
+        d.dofs_per_cell, d.bs, d.dofmap = dm.shape[1], bs, _lib.ptr(dm)
+        d.cstride = 0 if self.coeffs is None else self.coeffs.shape[1]
+        d.coeffs = _lib.ptr(self.coeffs)
+        d.num_constants = len(self.constants)
+        d.constants = _lib.ptr(self.constants)
+        self._h = C.c_void_p()
+        _lib.call("dfx_form_create", C.byref(d), C.byref(self._h))
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def function_spaces(self):
+        return [self.V] * self.rank
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            try:
+                _lib.lib().dfx_form_destroy(h)
+            except Exception:
+                pass
+
+
+def form(name, V, coeffs=None, constants=None, dtype=np.float64) -> Form:
+    return Form(name, V, coeffs, constants, dtype)
+
+
+def create_sparsity_pattern(a: Form) -> SparsityPattern:
+    """fem::create_sparsity_pattern (fem/utils.h:197-218)."""
+    h = C.c_void_p()
+    _lib.call("dfx_sparsity_create", a.V.handle, a.V.handle, C.byref(h))
+    return SparsityPattern(h)
+
+
+def create_matrix(a: Form, dtype=None) -> MatrixCSR:
+    """python/dolfinx/fem/assemble.py:127-143 (block_mode compact)."""
+    sp = create_sparsity_pattern(a)
+    return MatrixCSR.from_sparsity(sp, a.dtype if dtype is None else dtype)
+
+
+def assemble_matrix(a: Form, bcs=(), diag=1.0, A: MatrixCSR | None = None) -> MatrixCSR:
+    """fem.assemble_matrix (python/dolfinx/fem/assemble.py:283-376): assemble
+    with BC rows/columns zeroed, then insert ``diag`` on BC rows."""
+    if A is None:
+        A = create_matrix(a)
+    for bc in bcs or [None]:
+        _lib.call("dfx_assemble_matrix", A.handle, a.handle, None if bc is None else bc.handle)
+        break
+    if len(bcs) > 1:
+        raise ValueError("pass the union of Dirichlet dofs as one DirichletBC")
+    for bc in bcs:
+        _lib.call("dfx_set_diagonal", A.handle, bc.handle, float(diag))
+    return A
+
+
+def assemble_vector(L: Form, b: Vector | None = None) -> Vector:
+    """fem.assemble_vector (fem/assembler.h:230-257): b += L."""
+    if b is None:
+        b = Vector(L.V.num_dofs, L.dtype)
+    _lib.call("dfx_assemble_vector", b.handle, L.handle)
+    return b
+
+
+def apply_lifting(b: Vector, a_list, bcs_list, x0=None, alpha=1.0) -> None:
+    """fem.apply_lifting (python/dolfinx/fem/assemble.py:378-493)."""
+    for i, (a, bcs) in enumerate(zip(a_list, bcs_list)):
+        xx = None if not x0 else x0[i]
+        for bc in bcs:
+            _lib.call("dfx_apply_lifting", b.handle, a.handle, bc.handle,
+                      None if xx is None else xx.handle, float(alpha))
+
+
+def set_bc(b: Vector, bcs, x0: Vector | None = None, alpha=1.0) -> None:
+    """DirichletBC::set (fem/DirichletBC.h:495-578): b[bc] = alpha (g - x0)."""
+    for bc in bcs:
+        _lib.call("dfx_bc_set", b.handle, bc.handle, None if x0 is None else x0.handle,
+                  float(alpha))

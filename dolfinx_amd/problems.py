@@ -1,0 +1,89 @@
+"""Builders for the benchmark problems of this path (seeded, synthetic).
+
+Random fields use numpy's PCG64 generator with a fixed seed per problem and
+are indexed by the *input* (lexicographic) cell or vertex of create_box, so
+they do not depend on the cell reordering.
+
+  poisson_tet   -div(kappa grad u) + c u = f on create_box tetrahedra, P1
+  poisson_hex   the same on hexahedra, Q1
+  u = 0 on the whole boundary (DirichletBC on all exterior facet dofs)
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import fem, la, mesh
+
+
+@dataclass
+class ScalarProblem:
+    mesh: mesh.Mesh
+    V: fem.FunctionSpace
+    a: fem.Form
+    L: fem.Form
+    bc: fem.DirichletBC
+    kappa: np.ndarray       # per input cell
+    c: float
+    f: np.ndarray           # per input vertex
+    dtype: np.dtype
+
+
+def kappa_field(kind, num_cells, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    if kind == "one":
+        return np.ones(num_cells)
+    if kind == "exp_normal":          # kappa = exp(G), G ~ N(0, 1)
+        return np.exp(rng.standard_normal(num_cells))
+    if kind == "lognormal_0.5":       # log-normal(0, 0.5)
+        return np.exp(0.5 * rng.standard_normal(num_cells))
+    if kind == "uniform_1_10":
+        return rng.uniform(1.0, 10.0, num_cells)
+    raise ValueError(kind)
+
+
+def scalar_problem(n, cell, kappa="one", c=0.0, f="one", seed=0, dtype=np.float64,
+                   bc=True, msh=None, V=None) -> ScalarProblem:
+    """Build mesh, space, forms and the homogeneous Dirichlet BC."""
+    ct = mesh.CellType(cell)
+    if msh is None:
+        msh = mesh.create_box([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], n, ct)
+    if V is None:
+        V = fem.functionspace(msh, ("Lagrange", 1))
+    nc = msh.topology.cells.shape[0]
+    nv = msh.geometry.x.shape[0]
+    kap = kappa_field(kappa, nc, seed) if isinstance(kappa, str) else np.asarray(kappa)
+    if isinstance(f, str):
+        rng = np.random.Generator(np.random.PCG64(seed + 1))
+        fv = np.ones(nv) if f == "one" else rng.standard_normal(nv)
+    else:
+        fv = np.asarray(f)
+    tet = ct == mesh.CellType.tetrahedron
+    a = fem.form("p1_stiffness_reaction" if tet else "q1_stiffness_reaction", V,
+                 fem.pack_cellwise(msh, kap), [c], dtype)
+    L = fem.form("p1_source" if tet else "q1_source", V, fem.pack_vertexwise(msh, fv),
+                 None, dtype)
+    dofs = fem.locate_boundary_dofs(V) if bc else np.zeros(0, np.int32)
+    bcs = fem.dirichletbc(V, dofs)
+    return ScalarProblem(msh, V, a, L, bcs, kap, c, fv, np.dtype(dtype))
+
+
+def assemble_system(p: ScalarProblem, A: la.MatrixCSR | None = None, b: la.Vector | None = None):
+    """The reference's Poisson pipeline: A = assemble_matrix(a, [bc]) (BC
+    rows/cols zeroed, diagonal 1); b = assemble_vector(L); apply_lifting;
+    set_bc."""
+    if A is None:
+        A = fem.create_matrix(p.a)
+    else:
+        A.set_value(0.0)
+    fem.assemble_matrix(p.a, [p.bc], 1.0, A)
+    if b is None:
+        b = la.Vector(p.V.num_dofs, p.dtype)
+    else:
+        b.set(0.0)
+    fem.assemble_vector(p.L, b)
+    fem.apply_lifting(b, [p.a], [[p.bc]])
+    fem.set_bc(b, [p.bc])
+    return A, b

@@ -1,0 +1,282 @@
+/*
+ * dfx.h - C ABI of the MI355X-native DOLFINx assemble-and-solve path.
+ *
+ * This is the only boundary between host code and the HIP kernels. It takes
+ * plain pointers and sizes, returns integer status codes (0 = success), and
+ * keeps the message of the last failure in a per-thread buffer
+ * (dfx_last_error). The C++ drop-in headers in include/dolfinx_amd/ wrap it
+ * with the reference's fem:: and la:: names and rethrow failures as
+ * std::runtime_error, as the reference does.
+ *
+ * Every entry point names the reference interface it replaces
+ * (paths relative to the DOLFINx tree cpp/dolfinx/ unless stated).
+ *
+ * Ownership: host arrays passed in are only read during the call. Handles
+ * (dfx_mesh, dfx_space, dfx_sparsity, dfx_csr, dfx_vec, dfx_form, dfx_bc)
+ * own their memory and are released by the matching *_destroy.
+ * Threading: one host thread per library instance; device work is ordered
+ * on the library's HIP stream (dfx_stream).
+ */
+#ifndef DFX_H
+#define DFX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int dfx_status;
+enum
+{
+  DFX_OK = 0,
+  DFX_ERR_ARG = 1,     /* invalid argument (size mismatch, null handle) */
+  DFX_ERR_PATTERN = 2, /* entry not in sparsity (la/matrix_csr_impl.h:93) */
+  DFX_ERR_HIP = 3,     /* HIP runtime failure */
+  DFX_ERR_NOCONV = 4,  /* iterative solver did not converge */
+  DFX_ERR_NODEV = 5    /* no GPU device available */
+};
+
+enum { DFX_F32 = 0, DFX_F64 = 1 };
+enum { DFX_TET = 0, DFX_HEX = 1 };
+
+/* Hand-written element kernels. Each replaces one FFCx tabulate_tensor
+ * (fem/kernel.h:18-60) for a config form, batched over all cells. */
+enum
+{
+  DFX_K_P1TET_A = 0,       /* kappa grad u.grad v + c u v; w=[kappa] c=[c] */
+  DFX_K_P1TET_L = 1,       /* f v; w = [f at the 4 vertices] (P1 coeff.) */
+  DFX_K_Q1HEX_A = 2,       /* kappa grad u.grad v + c u v; w=[kappa] c=[c] */
+  DFX_K_Q1HEX_L = 3,       /* f v; w = [f at the 8 vertices] (Q1 coeff.) */
+  DFX_K_P2TET_ELAST_A = 4, /* sigma(u):eps(v), w = [lambda, mu] (DG0) */
+  DFX_K_P2TET_ELAST_L = 5, /* f.v, c = [fx, fy, fz] */
+  DFX_K_P3TET_A = 6,       /* kappa grad u.grad v + c u v; w=[kappa] c=[c] */
+  DFX_K_P3TET_L = 7        /* f v; c = [f] */
+};
+
+const char* dfx_last_error(void);
+/* Library version string. */
+const char* dfx_version(void);
+/* Initialise the device (select device 0, create the stream). Idempotent. */
+dfx_status dfx_init(void);
+/* Block until all queued device work has finished. */
+dfx_status dfx_synchronize(void);
+/* The library's HIP stream as an opaque pointer (hipStream_t). */
+void* dfx_stream(void);
+
+/* ------------------------------------------------------------------ */
+/* Host setup: mesh, dofmaps, Dirichlet dofs, sparsity (bit-exact).   */
+/* ------------------------------------------------------------------ */
+typedef struct dfx_mesh dfx_mesh;
+typedef struct dfx_space dfx_space;
+typedef struct dfx_sparsity dfx_sparsity;
+
+/* mesh::create_box (mesh/generation.h:104-134) on one rank, with the
+ * default graph::reorder_rcm cell reordering (graph/ordering.cpp:148). */
+dfx_status dfx_mesh_create_box(const double p0[3], const double p1[3],
+                               const int64_t n[3], int cell, dfx_mesh** out);
+dfx_status dfx_mesh_destroy(dfx_mesh* m);
+dfx_status dfx_mesh_sizes(const dfx_mesh* m, int32_t* num_cells,
+                          int32_t* num_vertices, int32_t* vertices_per_cell);
+/* Copies of Geometry::x (nodes x 3), the cell->vertex map, the original
+ * (input) cell index (Topology::original_cell_index, mesh/Topology.h:218)
+ * and the input vertex index of each vertex. Any pointer may be NULL. */
+dfx_status dfx_mesh_copy(const dfx_mesh* m, double* x, int32_t* cells,
+                         int64_t* original_cell_index,
+                         int64_t* input_vertex_index);
+
+/* fem::create_functionspace -> fem::create_dofmap (fem/utils.cpp:32-76)
+ * for Lagrange degree 1-3 (tetrahedra) or 1 (hexahedra), block size bs. */
+dfx_status dfx_space_create(dfx_mesh* m, int degree, int bs, dfx_space** out);
+dfx_status dfx_space_destroy(dfx_space* V);
+dfx_status dfx_space_sizes(const dfx_space* V, int32_t* num_nodes,
+                           int32_t* dofs_per_cell, int32_t* bs);
+dfx_status dfx_space_dofmap(const dfx_space* V, int32_t* dofmap);
+/* fem::locate_dofs_topological on exterior facets (fem/DirichletBC.cpp:
+ * 201-310, mesh/utils.cpp:61-89), unrolled to bs*node+comp as a DirichletBC
+ * on the whole space does (fem/DirichletBC.h:273-281). marker 0: all
+ * exterior facets; marker 1: exterior facets with every vertex at
+ * x[axis] == value (mesh::locate_entities_boundary). Call with dofs == NULL
+ * to query the count. */
+dfx_status dfx_locate_boundary_dofs(const dfx_space* V, int marker, int axis,
+                                    double value, int32_t* count,
+                                    int32_t* dofs);
+/* fem::create_sparsity_pattern + SparsityPattern::finalize (fem/utils.h:
+ * 197-314, la/SparsityPattern.cpp:264-491). Block pattern over nodes. */
+dfx_status dfx_sparsity_create(const dfx_space* V0, const dfx_space* V1,
+                               dfx_sparsity** out);
+dfx_status dfx_sparsity_destroy(dfx_sparsity* sp);
+dfx_status dfx_sparsity_sizes(const dfx_sparsity* sp, int32_t* nrows,
+                              int64_t* nnz, int32_t* bs);
+dfx_status dfx_sparsity_copy(const dfx_sparsity* sp, int64_t* row_ptr,
+                             int32_t* cols);
+
+/* ------------------------------------------------------------------ */
+/* Device linear algebra: la::MatrixCSR and la::Vector on HBM.         */
+/* ------------------------------------------------------------------ */
+typedef struct dfx_csr dfx_csr;
+typedef struct dfx_vec dfx_vec;
+
+/* la::MatrixCSR(sp, BlockMode::compact) (la/MatrixCSR.h:628-850) with a
+ * device container (la/MatrixCSR.h:67-70,210-237): values zeroed. */
+dfx_status dfx_csr_create(const int64_t* row_ptr, const int32_t* cols,
+                          int32_t nrows, int32_t ncols, int bs, int dtype,
+                          dfx_csr** out);
+dfx_status dfx_csr_destroy(dfx_csr* A);
+dfx_status dfx_csr_sizes(const dfx_csr* A, int32_t* nrows, int64_t* nnz,
+                         int32_t* bs, int32_t* dtype);
+/* MatrixCSR::set(0) */
+dfx_status dfx_csr_zero(dfx_csr* A);
+/* Copy values (nnz*bs*bs of the matrix dtype) to / from the host. */
+dfx_status dfx_csr_get_values(const dfx_csr* A, void* values);
+dfx_status dfx_csr_set_values(dfx_csr* A, const void* values);
+/* Copy the device pattern back (row_ptr as int64 like the reference). */
+dfx_status dfx_csr_get_pattern(const dfx_csr* A, int64_t* row_ptr,
+                               int32_t* cols);
+
+/* la::Vector (la/Vector.h:47-422) with a device container. */
+dfx_status dfx_vec_create(int32_t n, int dtype, dfx_vec** out);
+dfx_status dfx_vec_destroy(dfx_vec* v);
+dfx_status dfx_vec_size(const dfx_vec* v, int32_t* n, int32_t* dtype);
+dfx_status dfx_vec_upload(dfx_vec* v, const void* host);
+dfx_status dfx_vec_download(const dfx_vec* v, void* host);
+dfx_status dfx_vec_set(dfx_vec* v, double value);
+/* Copy-convert (la/Vector.h:168-187): dst = (dst dtype) src. */
+dfx_status dfx_vec_copy(dfx_vec* dst, const dfx_vec* src);
+
+/* MatrixCSR::mult (la/MatrixCSR.h:877-946 -> impl::spmv,
+ * la/matrix_csr_impl.h:259-286): y += A x. x, y share A's dtype. */
+dfx_status dfx_spmv(const dfx_csr* A, const dfx_vec* x, dfx_vec* y);
+/* la::inner_product (la/Vector.h:434-460); accumulated in FP64. */
+dfx_status dfx_inner_product(const dfx_vec* a, const dfx_vec* b,
+                             double* out);
+/* la::squared_norm (la/Vector.h:462-471). */
+dfx_status dfx_squared_norm(const dfx_vec* a, double* out);
+/* r = alpha x + y (cpp/demo/poisson_matrix_free/main.cpp:65-69). */
+dfx_status dfx_axpy(dfx_vec* r, double alpha, const dfx_vec* x,
+                    const dfx_vec* y);
+
+/* ------------------------------------------------------------------ */
+/* Forms, Dirichlet BCs and assembly.                                  */
+/* ------------------------------------------------------------------ */
+typedef struct dfx_form dfx_form;
+typedef struct dfx_bc dfx_bc;
+
+/* The data fem::impl::assemble_cells_matrix receives
+ * (fem/assemble_matrix_impl.h:92-118): geometry dofmap and x, the
+ * (test = trial) dofmap with its block size, packed coefficients
+ * (fem/pack.h:265-388, cells x cstride) and constants
+ * (fem/pack.h:578-611), plus the element kernel. */
+typedef struct
+{
+  int32_t kernel;         /* DFX_K_* */
+  int32_t dtype;          /* DFX_F32 / DFX_F64: scalar type of the form */
+  int32_t num_cells;
+  int32_t num_nodes;      /* geometry nodes */
+  int32_t nodes_per_cell; /* geometry nodes per cell */
+  const double* x;        /* num_nodes x 3 */
+  const int32_t* x_dofmap;/* num_cells x nodes_per_cell */
+  int32_t dofs_per_cell;  /* nodes per cell of the (block) dofmap */
+  int32_t bs;             /* block size of the dofmap */
+  const int32_t* dofmap;  /* num_cells x dofs_per_cell */
+  int32_t cstride;        /* coefficients per cell */
+  const double* coeffs;   /* num_cells x cstride (may be NULL if 0) */
+  int32_t num_constants;  /* <= 8 */
+  const double* constants;
+} dfx_form_desc;
+
+/* fem::Form (fem/Form.h:116-) holding device copies of the form data. */
+dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out);
+dfx_status dfx_form_destroy(dfx_form* f);
+/* Replace packed coefficients / constants (same sizes as at creation). */
+dfx_status dfx_form_set_coefficients(dfx_form* f, const double* coeffs,
+                                     const double* constants);
+
+/* fem::DirichletBC (fem/DirichletBC.h:55-618) on unrolled dof indices with
+ * values g (NULL: g = 0). n_total = size of the space (nodes*bs). */
+dfx_status dfx_bc_create(const int32_t* dofs, int32_t ndofs, int32_t n_total,
+                         const double* g, dfx_bc** out);
+dfx_status dfx_bc_destroy(dfx_bc* bc);
+
+/* fem::assemble_matrix(mat_add, a, bcs) (fem/assembler.h:542-602):
+ * A += a with rows and columns of the BC dofs zeroed (bc may be NULL). */
+dfx_status dfx_assemble_matrix(dfx_csr* A, const dfx_form* a,
+                               const dfx_bc* bc);
+/* fem::set_diagonal (fem/assembler.h:644-686): A[d,d] = diag for BC dofs */
+dfx_status dfx_set_diagonal(dfx_csr* A, const dfx_bc* bc, double diag);
+/* fem::assemble_vector(b, L) (fem/assembler.h:230-257): b += L */
+dfx_status dfx_assemble_vector(dfx_vec* b, const dfx_form* L);
+/* fem::apply_lifting(b, {a}, {{bc}}, {x0}, alpha) (fem/assembler.h:
+ * 336-493 -> impl::lift_bc, fem/assemble_vector_impl.h:361-414):
+ * b -= alpha A[:, bc] (g - x0) over cells with a BC column dof. */
+dfx_status dfx_apply_lifting(dfx_vec* b, const dfx_form* a, const dfx_bc* bc,
+                             const dfx_vec* x0, double alpha);
+/* DirichletBC::set (fem/DirichletBC.h:495-578): b[bc] = alpha (g - x0) */
+dfx_status dfx_bc_set(dfx_vec* b, const dfx_bc* bc, const dfx_vec* x0,
+                      double alpha);
+
+/* ------------------------------------------------------------------ */
+/* Krylov solvers.                                                     */
+/* ------------------------------------------------------------------ */
+typedef struct
+{
+  int32_t max_it;      /* maximum iterations */
+  double rtol;         /* stop when ||r||^2 / ||r0||^2 < rtol^2 */
+  int32_t jacobi;      /* 1: Jacobi preconditioner, 0: plain CG */
+  int32_t check_every; /* host convergence poll interval (iterations) */
+} dfx_cg_opts;
+
+typedef struct
+{
+  int32_t iterations;
+  int32_t converged;
+  double rnorm0;       /* ||r0||_2 (recursive, FP64) */
+  double rnorm;        /* ||r_k||_2 (recursive, FP64) */
+  double true_rel_residual; /* ||b - A x||_2 / ||b||_2 evaluated in FP64 */
+  double seconds;      /* wall time of the solve */
+} dfx_cg_result;
+
+/* CG of the reference (cpp/demo/poisson_matrix_free/main.cpp:84-132,
+ * python/test/conftest.py:161-205), optionally Jacobi-preconditioned,
+ * device resident. x holds the initial guess and receives the solution.
+ * A, b, x share one dtype; reductions are FP64. */
+dfx_status dfx_cg_solve(const dfx_csr* A, const dfx_vec* b, dfx_vec* x,
+                        const dfx_cg_opts* opts, dfx_cg_result* res);
+
+typedef struct
+{
+  int32_t max_outer;   /* refinement steps */
+  int32_t max_inner;   /* iterations per inner FP32 solve */
+  double rtol;         /* target ||b - A x||_2/||b||_2 (FP64 residual) */
+  double inner_rtol;   /* inner FP32 CG relative tolerance */
+  int32_t check_every;
+} dfx_mixed_opts;
+
+typedef struct
+{
+  int32_t outer;
+  int32_t inner_total;
+  int32_t converged;
+  double true_rel_residual;
+  double seconds;
+} dfx_mixed_result;
+
+/* Mixed-precision iterative refinement: FP64 residual r = b - A64 x and
+ * FP64 update x += d, with d from Jacobi-PCG on A32 in FP32 (FP64
+ * reductions). No counterpart in the reference; it wraps the same CG. */
+dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32,
+                           const dfx_vec* b, dfx_vec* x,
+                           const dfx_mixed_opts* opts, dfx_mixed_result* res);
+
+/* ------------------------------------------------------------------ */
+/* Timing on the library stream (hipEvent).                            */
+/* ------------------------------------------------------------------ */
+/* Record event slot (0..63) on the library stream. */
+dfx_status dfx_event_record(int slot);
+/* Milliseconds between two recorded slots (synchronises on 'b'). */
+dfx_status dfx_event_elapsed(int a, int b, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

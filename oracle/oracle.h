@@ -1,0 +1,167 @@
+/*
+ * CPU oracle for the DOLFINx assemble-and-solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of the reference
+ * algorithms, used by tests/, by __graft_entry__.smoke() and by the
+ * cpu_baseline leg of bench.py as the checker. The product library
+ * (dolfinx_amd/) never links, loads or calls anything in this directory.
+ *
+ * Every function names the reference file:line it restates (paths are
+ * relative to the DOLFINx source tree, cpp/dolfinx/... unless stated).
+ *
+ * Pinning (what checks each function against something other than itself):
+ *   orc_box_*            closed form of generation.h; tests check counts,
+ *                        sum |K| = 1, |det J| = h^3/6, lexicographic x.
+ *   orc_dual_graph       brute-force facet matching on tiny meshes (tests).
+ *   orc_rcm              permutation property; brute-force George-Liu
+ *                        restatement in Python on tiny graphs (tests).
+ *   orc_mesh_create      first-appearance vertex numbering invariant;
+ *                        geometry/topology consistency.
+ *   orc_entities         edges sorted by (min,max) vertex; Euler counts.
+ *   orc_dofmap           P1 dofmap == cell->vertex; Q1 likewise; P2/P3
+ *                        traversal-order invariant, entity counts.
+ *   orc_sparsity         brute-force dense outer-product pattern (tests).
+ *   kernels              closed forms (P1), A.1 = 0, sum of mass = |Omega|,
+ *                        exact integration of polynomials (tests).
+ *   orc_assemble_*       closed-form global matrices on 1-cell meshes;
+ *                        linearity; known answers from the reference tests.
+ *   orc_cg/orc_pcg       manufactured exact solutions; scipy.sparse.linalg.
+ * Functions without such a check are marked "parity unpinned" in their
+ * comment.
+ */
+#ifndef DFX_ORACLE_H
+#define DFX_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_TET = 0, ORC_HEX = 1 };
+
+typedef struct orc_mesh
+{
+  int celltype;                 /* ORC_TET / ORC_HEX */
+  int nvpc;                     /* vertices per cell (4 / 8) */
+  int32_t num_cells;
+  int32_t num_vertices;
+  double* x;                    /* (num_vertices x 3), geometry order */
+  int32_t* cells;               /* (num_cells x nvpc), local vertex indices */
+  int64_t* original_cell_index; /* input (lexicographic) cell of each cell */
+  int64_t* input_vertex_index;  /* input (lexicographic) vertex of each vertex */
+} orc_mesh;
+
+/* ---- mesh (mesh/generation.h, mesh/utils.h, graphbuild.cpp, ordering.cpp,
+ *      Topology.cpp, topologycomputation.cpp) ---- */
+int64_t orc_box_num_vertices(const int64_t n[3]);
+int64_t orc_box_num_cells(const int64_t n[3], int celltype);
+void orc_box_geometry(const double p0[3], const double p1[3],
+                      const int64_t n[3], double* x);
+void orc_box_cells(const int64_t n[3], int celltype, int64_t* cells);
+/* Local dual graph; returns number of edges (links = 2x). Caller frees. */
+int orc_dual_graph(int celltype, const int64_t* cells, int32_t num_cells,
+                   int32_t** offsets, int32_t** data);
+void orc_rcm(const int32_t* offsets, const int32_t* data, int32_t n,
+             int32_t* remap);
+orc_mesh* orc_mesh_create_box(const double p0[3], const double p1[3],
+                              const int64_t n[3], int celltype);
+void orc_mesh_free(orc_mesh* m);
+int orc_cell_num_entities(int celltype, int dim);
+/* Entities of dimension dim (1 or 2): c2e (num_cells x ne), e2v (n x nvpe).
+ * Returns the number of entities. Caller frees both. */
+int32_t orc_entities(const orc_mesh* m, int dim, int32_t** c2e,
+                     int32_t** e2v);
+/* Per-cell edge reflection bits (bit e set iff edge e is reflected). */
+void orc_edge_reflections(const orc_mesh* m, const int32_t* c2e,
+                          const int32_t* e2v, uint32_t* bits);
+/* Exterior facets as (cell, local facet) pairs; returns count. Caller frees */
+int32_t orc_exterior_facets(const orc_mesh* m, int32_t** cell_facet);
+
+/* ---- dofmaps, BCs, sparsity (fem/dofmapbuilder.cpp, fem/utils.cpp,
+ *      fem/DirichletBC.cpp, la/SparsityPattern.cpp) ---- */
+int orc_element_ndofs(int celltype, int degree);
+/* Scalar (node) dofmap for Lagrange degree 1..3 (tet) / 1 (hex).
+ * Returns number of nodes; dofmap is (num_cells x ndofs). */
+int32_t orc_dofmap(const orc_mesh* m, int degree, int32_t* dofmap);
+/* Sorted unique node indices in the closure of the given exterior facets.
+ * marker: 0 = all exterior facets, 1 = facets with all vertices at
+ * x[axis] == value. Returns count, caller frees. */
+int32_t orc_boundary_nodes(const orc_mesh* m, int degree,
+                           const int32_t* dofmap, int marker, int axis,
+                           double value, int32_t** nodes);
+/* Sparsity of the (block) dofmap outer products: row_ptr (nrows+1) int64,
+ * cols sorted unique. Returns nnz, caller frees cols. */
+int64_t orc_sparsity(const int32_t* dofmap0, int k0, const int32_t* dofmap1,
+                     int k1, int32_t num_cells, int32_t nrows,
+                     int64_t* row_ptr, int32_t** cols);
+
+/* ---- element kernels (FFCx tabulate_tensor ABI, fem/kernel.h:18-27) ---- */
+typedef void (*orc_kernel_fn)(double* A, const double* w, const double* c,
+                              const double* coordinate_dofs,
+                              const int* entity_local_index,
+                              const uint8_t* quadrature_permutation,
+                              void* custom_data);
+enum
+{
+  ORC_K_P1TET_A = 0,   /* a = kappa grad u.grad v + c u v; w=[kappa], c=[c] */
+  ORC_K_P1TET_L = 1,   /* L = f v; w = [f at 4 vertices] */
+  ORC_K_Q1HEX_A = 2,   /* a = kappa grad u.grad v + c u v; w=[kappa], c=[c] */
+  ORC_K_Q1HEX_L = 3,   /* L = f v; w = [f at 8 vertices] */
+  ORC_K_P2TET_ELAST_A = 4, /* sigma(u):eps(v); w = [lambda, mu] */
+  ORC_K_P2TET_ELAST_L = 5, /* f.v; c = [fx, fy, fz] */
+  ORC_K_P3TET_A = 6,   /* kappa grad u.grad v + c u v; w=[kappa], c=[c] */
+  ORC_K_P3TET_L = 7,   /* f v; c = [f] */
+  ORC_K_COUNT = 8
+};
+orc_kernel_fn orc_get_kernel(int id);
+void orc_tabulate(int id, double* A, const double* w, const double* c,
+                  const double* coordinate_dofs);
+
+/* ---- assembly (fem/assemble_matrix_impl.h, assemble_vector_impl.h,
+ *      assembler.h, la/matrix_csr_impl.h) ---- */
+/* Returns 0, or -1 if an entry is not in the sparsity pattern. */
+int orc_assemble_matrix(int kernel, const double* x, const int32_t* x_dofmap,
+                        int nodes_per_cell, int32_t num_cells,
+                        const int32_t* dofmap, int k, int bs,
+                        const double* coeffs, int cstride,
+                        const double* constants, const int8_t* bc0,
+                        const int8_t* bc1, const int64_t* row_ptr,
+                        const int32_t* cols, double* values);
+void orc_assemble_vector(int kernel, const double* x, const int32_t* x_dofmap,
+                         int nodes_per_cell, int32_t num_cells,
+                         const int32_t* dofmap, int k, int bs,
+                         const double* coeffs, int cstride,
+                         const double* constants, double* b);
+void orc_lift_bc(int kernel, const double* x, const int32_t* x_dofmap,
+                 int nodes_per_cell, int32_t num_cells, const int32_t* dofmap,
+                 int k, int bs, const double* coeffs, int cstride,
+                 const double* constants, const double* bc_values,
+                 const int8_t* bc_markers, const double* x0, double alpha,
+                 double* b);
+void orc_bc_set(double* b, const int32_t* dofs, int32_t ndofs,
+                const double* g, const double* x0, double alpha);
+int orc_set_diagonal(const int32_t* rows, int32_t nrows, int bs, double diag,
+                     const int64_t* row_ptr, const int32_t* cols,
+                     double* values);
+
+/* ---- linear algebra (la/matrix_csr_impl.h, la/Vector.h, demo CG) ---- */
+void orc_spmv(int32_t nrows, int bs, const int64_t* row_ptr,
+              const int32_t* cols, const double* values, const double* x,
+              double* y);
+double orc_dot(int32_t n, const double* a, const double* b);
+/* Reference CG (no preconditioner). Returns iterations, or -1 if kmax hit. */
+int orc_cg(int32_t nrows, int bs, const int64_t* row_ptr, const int32_t* cols,
+           const double* values, const double* b, double* x, int kmax,
+           double rtol);
+/* Jacobi-preconditioned CG with the reference's stopping test. */
+int orc_pcg_jacobi(int32_t nrows, int bs, const int64_t* row_ptr,
+                   const int32_t* cols, const double* values, const double* b,
+                   double* x, int kmax, double rtol);
+
+void orc_free(void* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,106 @@
+"""GPU parity of assembly (P1 tetrahedra, Q1 hexahedra) against the oracle.
+
+Tolerances: FP64 values within 1e-12 relative (norm-wise and entry-wise
+against max |A|); FP32 within 2e-6 relative (a few FP32 ulps of the
+element computation plus atomics reassociation).
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from dolfinx_amd import fem, la, problems
+from dolfinx_amd._lib import DfxError
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(O.TET, (3, 4, 2)), (O.HEX, (4, 3, 5)), (O.TET, (12, 12, 12)), (O.HEX, (20, 20, 20))]
+TOL = {np.float64: 1e-12, np.float32: 2e-6}
+
+
+def oracle_system(p, n, with_bc=True, g=None):
+    """Assemble the same problem with the oracle (on the oracle's own mesh)."""
+    om = O.Mesh(n, int(p.mesh.topology.cell_type))
+    dm, nd = om.dofmap(1)
+    rp, cols = O.sparsity(dm, dm, nd)
+    kap = p.kappa[om.original_cell_index].reshape(-1, 1)
+    fv = p.f[om.input_vertex_index][om.cells]
+    kA = O.K_P1TET_A if om.celltype == O.TET else O.K_Q1HEX_A
+    kL = O.K_P1TET_L if om.celltype == O.TET else O.K_Q1HEX_L
+    bnodes = om.boundary_nodes(1, dm) if with_bc else np.zeros(0, np.int32)
+    mark = np.zeros(nd, np.int8)
+    mark[bnodes] = 1
+    A = O.assemble_matrix(kA, om.x, om.cells, dm, 1, kap, np.array([p.c]), mark, mark, rp, cols)
+    O.set_diagonal(A, bnodes, 1, 1.0, rp, cols)
+    b = O.assemble_vector(kL, om.x, om.cells, dm, 1, fv, None, nd)
+    gg = np.zeros(nd) if g is None else g
+    O.lift_bc(b, kA, om.x, om.cells, dm, 1, kap, np.array([p.c]), gg, mark)
+    O.bc_set(b, bnodes, gg)
+    return dict(mesh=om, rp=rp, cols=cols, A=A, b=b, bnodes=bnodes, nd=nd)
+
+
+def build(cell, n, dtype, kappa="exp_normal", c=1.5, f="normal"):
+    return problems.scalar_problem(n, cell, kappa=kappa, c=c, f=f, seed=7, dtype=dtype)
+
+
+@pytest.mark.parametrize("cell,n", CASES)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_assemble_system_matches_oracle(cell, n, dtype):
+    p = build(cell, n, dtype)
+    A, b = problems.assemble_system(p)
+    ref = oracle_system(p, n)
+    rp, ci = A.pattern()
+    assert np.array_equal(rp, ref["rp"]) and np.array_equal(ci, ref["cols"])
+    tol = TOL[dtype]
+    Av = A.data.astype(np.float64)
+    assert rel_err(Av, ref["A"]) <= tol
+    assert np.max(np.abs(Av - ref["A"])) <= tol * np.max(np.abs(ref["A"]))
+    assert rel_err(b.array, ref["b"]) <= tol
+
+
+@pytest.mark.parametrize("cell,n", CASES[:2])
+def test_lifting_nonzero_g_matches_oracle(cell, n):
+    """Manufactured nonzero Dirichlet data pins apply_lifting + set_bc."""
+    p = build(cell, n, np.float64)
+    x = p.mesh.geometry.x
+    g = 1.0 + x[:, 0] ** 2 + 0.5 * x[:, 1] * x[:, 2]
+    bcg = fem.dirichletbc(p.V, p.bc.dofs, g)
+    b = fem.assemble_vector(p.L)
+    fem.apply_lifting(b, [p.a], [[bcg]])
+    fem.set_bc(b, [bcg])
+    ref = oracle_system(p, n, g=g)
+    assert rel_err(b.array, ref["b"]) <= 1e-12
+
+
+def test_entry_not_in_sparsity_raises():
+    """A pattern missing entries must raise, like insert_csr
+    (la/matrix_csr_impl.h:93-94)."""
+    p = build(O.TET, (2, 2, 2), np.float64)
+    sp = fem.create_sparsity_pattern(p.a)
+    n = sp.num_rows
+    diag = la.MatrixCSR(np.arange(n + 1, dtype=np.int64), np.arange(n, dtype=np.int32))
+    with pytest.raises(DfxError, match="Entry not in sparsity"):
+        fem.assemble_matrix(p.a, [], 1.0, diag)
+
+
+def test_reassembly_is_linear_in_kappa():
+    """Assembling with 2*kappa doubles the operator (test_assembler.py
+    re-assembly linearity)."""
+    p1 = build(O.HEX, (6, 5, 4), np.float64, c=0.0)
+    p2 = problems.scalar_problem((6, 5, 4), O.HEX, kappa=2 * p1.kappa, c=0.0, f=p1.f,
+                                 msh=p1.mesh, V=p1.V)
+    A1 = fem.assemble_matrix(p1.a, [])
+    A2 = fem.assemble_matrix(p2.a, [])
+    assert rel_err(A2.data, 2 * A1.data) <= 1e-14
+
+
+def test_stiffness_annihilates_constants():
+    """A 1 = 0 for the pure stiffness without BCs (cpp/test/matrix.cpp:116-119)."""
+    for cell, n in CASES[:2]:
+        p = build(cell, n, np.float64, c=0.0)
+        A = fem.assemble_matrix(p.a, [])
+        y = la.Vector(p.V.num_dofs)
+        A.mult(la.Vector.from_array(np.ones(p.V.num_dofs)), y)
+        assert np.max(np.abs(y.array)) <= 1e-13 * np.max(np.abs(A.data))

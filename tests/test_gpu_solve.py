@@ -1,0 +1,99 @@
+"""GPU parity of SpMV, reductions and the CG solvers against the oracle.
+
+Tolerances: SpMV FP64 within 1e-14 relative (summation order differs from
+the row-serial reference); FP32 SpMV within 1e-6; solutions within 1e-8
+relative L2 of the oracle's FP64 Jacobi-PCG solved to 1e-12.
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from dolfinx_amd import la, problems, solvers
+from dolfinx_amd._lib import DfxError
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def system(cell, n, dtype=np.float64, **kw):
+    p = problems.scalar_problem(n, cell, dtype=dtype, seed=11, **kw)
+    A, b = problems.assemble_system(p)
+    return p, A, b
+
+
+@pytest.mark.parametrize("cell,n", [(O.TET, (10, 9, 8)), (O.HEX, (24, 20, 16))])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-14), (np.float32, 1e-6)])
+def test_spmv_matches_oracle(cell, n, dtype, tol, rng):
+    p, A, b = system(cell, n, dtype, kappa="exp_normal")
+    x = rng.standard_normal(p.V.num_dofs)
+    y0 = rng.standard_normal(p.V.num_dofs)
+    xv = la.Vector.from_array(x.astype(dtype))
+    yv = la.Vector.from_array(y0.astype(dtype))
+    A.mult(xv, yv)  # y += A x
+    Av = A.data.astype(np.float64)
+    ref = O.spmv(A.indptr, A.indices, Av, 1, x.astype(dtype).astype(np.float64),
+                 y0.astype(dtype).astype(np.float64))
+    assert rel_err(yv.array, ref) <= tol
+
+
+def test_inner_product_and_axpy(rng):
+    n = 1_000_003
+    a, b = rng.standard_normal(n), rng.standard_normal(n)
+    va, vb = la.Vector.from_array(a), la.Vector.from_array(b)
+    assert abs(la.inner_product(va, vb) - O.lib().orc_dot(n, O._p(a), O._p(b))) <= 1e-12 * n ** 0.5
+    assert abs(la.squared_norm(va) - float(a @ a)) <= 1e-12 * float(a @ a)
+    r = la.Vector(n)
+    la.axpy(r, -2.5, va, vb)
+    assert rel_err(r.array, -2.5 * a + b) <= 1e-15
+
+
+@pytest.mark.parametrize("cell,n", [(O.TET, (12, 12, 12)), (O.HEX, (20, 20, 20))])
+@pytest.mark.parametrize("jacobi", [True, False])
+def test_cg_matches_oracle(cell, n, jacobi):
+    p, A, b = system(cell, n, kappa="exp_normal", c=0.0, f="one")
+    x = la.Vector(p.V.num_dofs)
+    res = solvers.cg(A, b, x, rtol=1e-12, max_it=20000, jacobi=jacobi, check_every=16)
+    assert res.converged and res.true_rel_residual <= 1e-11
+    xr, it = O.cg(A.indptr, A.indices, A.data, 1, b.array, kmax=20000, rtol=1e-12, jacobi=jacobi)
+    assert it > 0
+    assert rel_err(x.array, xr) <= 1e-8
+    # same recurrence: iteration counts agree up to round-off effects
+    assert abs(res.iterations - it) <= max(2, it // 50)
+
+
+def test_cg_raises_when_not_converged():
+    """conftest.py cg_solver raises after maxit (python/test/conftest.py:200-202)."""
+    p, A, b = system(O.TET, (8, 8, 8))
+    x = la.Vector(p.V.num_dofs)
+    with pytest.raises(DfxError, match="exceeded max iterations"):
+        solvers.cg(A, b, x, rtol=1e-14, max_it=3)
+
+
+@pytest.mark.parametrize("cell,n", [(O.TET, (12, 12, 12)), (O.HEX, (24, 24, 24))])
+def test_mixed_refinement_matches_fp64(cell, n):
+    kw = dict(kappa="exp_normal", c=0.0, f="one")
+    p64 = problems.scalar_problem(n, cell, dtype=np.float64, seed=3, **kw)
+    p32 = problems.scalar_problem(n, cell, dtype=np.float32, seed=3, msh=p64.mesh, V=p64.V, **kw)
+    A64, b = problems.assemble_system(p64)
+    A32, _ = problems.assemble_system(p32)
+    x = la.Vector(p64.V.num_dofs)
+    res = solvers.mixed_refinement(A64, A32, b, x, rtol=1e-10, inner_rtol=1e-4)
+    assert res.converged and res.true_rel_residual <= 1e-10
+    xr, _ = O.cg(A64.indptr, A64.indices, A64.data, 1, b.array, kmax=50000, rtol=1e-13)
+    assert rel_err(x.array, xr) <= 1e-8
+
+
+def test_poisson_tet_32_end_to_end():
+    """The CPU-runnable config: P1 Poisson on create_box 32^3 tets, f = 1,
+    u = 0 on the boundary, Jacobi-CG to 1e-10."""
+    p, A, b = system(O.TET, (32, 32, 32), kappa="one", c=0.0, f="one")
+    assert A.nnz == 513_313 and len(p.bc.dofs) == 6_146 and p.V.num_dofs == 35_937
+    x = la.Vector(p.V.num_dofs)
+    res = solvers.cg(A, b, x, rtol=1e-10)
+    assert res.converged
+    xr, _ = O.cg(A.indptr, A.indices, A.data, 1, b.array, rtol=1e-12)
+    assert rel_err(x.array, xr) <= 1e-8
+    # maximum of the solution of -lap u = 1 on the unit cube (P1, h = 1/32)
+    assert 0.0555 < x.array.max() < 0.0565
