@@ -172,6 +172,10 @@ def main():
     sync()
     asm32_s = (time.perf_counter() - t2) / args.steps
 
+    # ---- SpMV lanes per row picked by the autotuner ----
+    lanes64, _ = A64.autotune()
+    lanes32, _ = A32.autotune()
+
     # ---- SpMV roofline (FP64) ----
     x = la.Vector.from_array(np.random.default_rng(1).standard_normal(N))
     y = la.Vector(N)
@@ -264,6 +268,7 @@ def main():
         },
         "assembly_f32_gnnz_s": round(nnz / asm32_s / 1e9, 4),
         "spmv": {
+            "lanes_per_row": {"f64": lanes64, "f32": lanes32},
             "f64_ms": round(spmv_ms, 4), "f64_gbs": round(spmv_gbs, 1),
             "f64_frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
             "f32_ms": round(spmv32_ms, 4), "f32_gbs": round(spmv32_gbs, 1),

@@ -80,6 +80,8 @@ SIGNATURES = {
     "dfx_csr_get_values": (C.c_int, [P, P]),
     "dfx_csr_set_values": (C.c_int, [P, P]),
     "dfx_csr_get_pattern": (C.c_int, [P, P, P]),
+    "dfx_csr_autotune": (C.c_int, [P, P, P]),
+    "dfx_csr_set_lanes": (C.c_int, [P, I32]),
     "dfx_vec_create": (C.c_int, [I32, C.c_int, PP]),
     "dfx_vec_destroy": (C.c_int, [P]),
     "dfx_vec_size": (C.c_int, [P, P, P]),

@@ -618,6 +618,532 @@ void ensure_slots(dfx_form& f, const dfx_csr& A)
   f.slot_pattern = A.pattern_id;
 }
 
+// ---------------------------------------------------------------------
+// Row-owner assembly of affine cells (no atomics, deterministic).
+//
+// Pass 1, cell_factors_k (one thread per cell): the geometric factors
+//   G = kappa |det J| J^{-1} J^{-T}   (6 values, symmetric)
+// and the reaction scale c |det J|, plus the bit mask of the cell's dofs
+// that carry a Dirichlet condition.
+// Pass 2, row_assemble_k (one thread per matrix row): loop over the row's
+// incidences (cell c, local index i), in cell order, and add
+//   A_ij = sum_g G_g(c) S_g(i, j) + c|det J| M(i, j)
+// for every column dof j of c (S: symmetrised reference stiffness
+// integrals, M: reference mass) into an LDS copy of the block's slice of
+// the CSR values; the slice is then stored with coalesced writes. Every
+// entry is summed in the same order on every run.
+// ---------------------------------------------------------------------
+
+template <typename T, int NV>
+__global__ void __launch_bounds__(kBlock)
+    cell_factors_k(std::int32_t nc, const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
+                   const std::int32_t* __restrict__ dm, int k, const T* __restrict__ coeffs,
+                   int cstride, T creac, const std::int8_t* __restrict__ bc, T* __restrict__ cd,
+                   std::uint32_t* __restrict__ mask)
+{
+  const std::int64_t c = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (c >= nc)
+    return;
+  const std::int32_t* v = xdm + NV * c;
+  // Jacobian columns: x1-x0, x2-x0 and x3-x0 (tetrahedron) or x4-x0
+  // (hexahedron, vertices in lexicographic order)
+  const T* q0 = x + 3 * std::int64_t(v[0]);
+  const T* q1 = x + 3 * std::int64_t(v[1]);
+  const T* q2 = x + 3 * std::int64_t(v[2]);
+  const T* q3 = x + 3 * std::int64_t(v[NV == 4 ? 3 : 4]);
+  T J[9], adj[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+  {
+    J[3 * r + 0] = q1[r] - q0[r];
+    J[3 * r + 1] = q2[r] - q0[r];
+    J[3 * r + 2] = q3[r] - q0[r];
+  }
+  const T det = adjugate(J, adj);
+  const T adet = det < T(0) ? -det : det;
+  const T s = coeffs[c * cstride] / adet;
+  T* G = cd + 8 * c;
+  G[0] = s * (adj[0] * adj[0] + adj[1] * adj[1] + adj[2] * adj[2]);
+  G[1] = s * (adj[3] * adj[3] + adj[4] * adj[4] + adj[5] * adj[5]);
+  G[2] = s * (adj[6] * adj[6] + adj[7] * adj[7] + adj[8] * adj[8]);
+  G[3] = s * (adj[0] * adj[3] + adj[1] * adj[4] + adj[2] * adj[5]);
+  G[4] = s * (adj[0] * adj[6] + adj[1] * adj[7] + adj[2] * adj[8]);
+  G[5] = s * (adj[3] * adj[6] + adj[4] * adj[7] + adj[5] * adj[8]);
+  G[6] = creac * adet;
+  G[7] = T(0);
+  if (mask)
+  {
+    std::uint32_t m = 0;
+    for (int j = 0; j < k; ++j)
+      if (bc[dm[c * k + j]])
+        m |= 1u << j;
+    mask[c] = m;
+  }
+}
+
+/// Incidence slot offsets: for incidence p = (cell c, local row i) and
+/// column j, the position of dof(c, j) inside row dof(c, i).
+template <typename O>
+__global__ void __launch_bounds__(kBlock)
+    incidence_slots_k(std::int64_t ninc, int k, const std::int32_t* __restrict__ inc,
+                      const std::int32_t* __restrict__ dm, const std::int32_t* __restrict__ rp,
+                      const std::int32_t* __restrict__ ci, O* __restrict__ islots, int* err)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t >= ninc * k)
+    return;
+  const std::int64_t p = t / k;
+  const int j = static_cast<int>(t % k);
+  const std::int32_t cik = inc[p];
+  const std::int32_t c = cik / k;
+  const std::int32_t row = dm[cik];
+  const std::int32_t col = dm[std::int64_t(c) * k + j];
+  std::int32_t lo = rp[row], hi = rp[row + 1];
+  const std::int32_t r0 = lo, r1 = hi;
+  while (lo < hi)
+  {
+    const std::int32_t mid = (lo + hi) >> 1;
+    if (ci[mid] < col)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if (lo >= r1 || ci[lo] != col)
+  {
+    atomicOr(err, 1);
+    islots[t] = 0;
+    return;
+  }
+  islots[t] = static_cast<O>(lo - r0);
+}
+
+template <typename T, typename O, int K>
+__global__ void __launch_bounds__(kBlock)
+    row_assemble_k(std::int32_t nrows, std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
+                   const std::int32_t* __restrict__ inc, const O* __restrict__ islots,
+                   const T* __restrict__ cd, const std::uint32_t* __restrict__ mask,
+                   const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
+                   const T* __restrict__ ref, T* __restrict__ vals, int accumulate)
+{
+  extern __shared__ unsigned char smem_raw[];
+  T* S = reinterpret_cast<T*>(smem_raw); // [7][K][K]
+  T* acc = S + 7 * K * K;
+  const int R = blockDim.x;
+  const std::int32_t r0 = blockIdx.x * R;
+  const std::int32_t r1 = min(r0 + R, nrows);
+  const std::int32_t base = rp[r0];
+  const std::int32_t len = rp[r1] - base;
+  for (int t = threadIdx.x; t < 7 * K * K; t += R)
+    S[t] = ref[t];
+  for (int t = threadIdx.x; t < len; t += R)
+    acc[t] = T(0);
+  __syncthreads();
+  const std::int32_t r = r0 + threadIdx.x;
+  if (r < r1 && r < nplan && !(bc && bc[r]))
+  {
+    T* a = acc + (rp[r] - base);
+    const std::int32_t pe = inc_off[r + 1];
+    // Incidences in groups of four with every global load of a group
+    // issued before any use, to keep several loads in flight per thread.
+    for (std::int32_t pb = inc_off[r]; pb < pe; pb += 4)
+    {
+      std::int32_t cik[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        cik[u] = pb + u < pe ? inc[pb + u] : -1;
+      T g[4][7];
+      std::uint32_t m[4];
+      O sl[4][K];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+      {
+        if (cik[u] < 0)
+          continue;
+        const std::int64_t c = cik[u] / K;
+        const T* G = cd + 8 * c;
+#pragma unroll
+        for (int q = 0; q < 7; ++q)
+          g[u][q] = G[q];
+        m[u] = mask ? mask[c] : 0u;
+        const O* src = islots + std::int64_t(pb + u) * K;
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+          sl[u][j] = src[j];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+      {
+        if (cik[u] < 0)
+          continue;
+        const int i = cik[u] % K;
+        const T* Si = S + i * K;
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+        {
+          if ((m[u] >> j) & 1u)
+            continue;
+          const T v = g[u][0] * Si[j] + g[u][1] * Si[K * K + j] + g[u][2] * Si[2 * K * K + j]
+                      + g[u][3] * Si[3 * K * K + j] + g[u][4] * Si[4 * K * K + j]
+                      + g[u][5] * Si[5 * K * K + j] + g[u][6] * Si[6 * K * K + j];
+          a[sl[u][j]] += v;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (accumulate)
+    for (int t = threadIdx.x; t < len; t += R)
+      vals[base + t] += acc[t];
+  else
+    for (int t = threadIdx.x; t < len; t += R)
+      vals[base + t] = acc[t];
+}
+
+/// Lifting b[r] -= alpha sum_{(c,i) at r} sum_{j in BC} A_ij (g_j - x0_j),
+/// one thread per row, over cells whose mask marks a BC column.
+template <typename T, int K>
+__global__ void __launch_bounds__(kBlock)
+    row_lift_k(std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
+               const std::int32_t* __restrict__ inc, const std::int32_t* __restrict__ dm,
+               const T* __restrict__ cd, const std::uint32_t* __restrict__ mask,
+               const T* __restrict__ ref, const double* __restrict__ g,
+               const T* __restrict__ x0, double alpha, T* __restrict__ b)
+{
+  const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nplan)
+    return;
+  double s = 0.0;
+  for (std::int32_t p = inc_off[r]; p < inc_off[r + 1]; ++p)
+  {
+    const std::int32_t cik = inc[p];
+    const std::int32_t c = cik / K;
+    const std::uint32_t m = mask[c];
+    if (!m)
+      continue;
+    const int i = cik - c * K;
+    const T* G = cd + 8 * std::int64_t(c);
+    const T* Si = ref + i * K;
+    for (int j = 0; j < K; ++j)
+    {
+      if (!((m >> j) & 1u))
+        continue;
+      const T v = G[0] * Si[j] + G[1] * Si[K * K + j] + G[2] * Si[2 * K * K + j]
+                  + G[3] * Si[3 * K * K + j] + G[4] * Si[4 * K * K + j]
+                  + G[5] * Si[5 * K * K + j] + G[6] * Si[6 * K * K + j];
+      const std::int32_t col = dm[std::int64_t(c) * K + j];
+      const double gv = g ? g[col] : 0.0;
+      const double xv = x0 ? double(x0[col]) : 0.0;
+      s += double(v) * (gv - xv);
+    }
+  }
+  if (s != 0.0)
+    b[r] = static_cast<T>(double(b[r]) - alpha * s);
+}
+
+/// Element vectors of the source forms, one thread per cell:
+/// P1 tetrahedra b_i = |K|/20 (f_i + sum f); affine Q1 hexahedra
+/// b_i = |det J| sum_j M_ij f_j; other hexahedra 2x2x2 Gauss.
+template <typename T, int NV>
+__global__ void __launch_bounds__(kBlock)
+    cell_source_k(std::int32_t nc, const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
+                  const T* __restrict__ coeffs, int cstride, int affine,
+                  const std::int32_t* __restrict__ inc_pos, T* __restrict__ be)
+{
+  const std::int64_t c = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (c >= nc)
+    return;
+  const std::int32_t* v = xdm + NV * c;
+  const T* f = coeffs + c * cstride;
+  T out[NV];
+  if (affine)
+  {
+    const T* q0 = x + 3 * std::int64_t(v[0]);
+    const T* q1 = x + 3 * std::int64_t(v[1]);
+    const T* q2 = x + 3 * std::int64_t(v[2]);
+    const T* q3 = x + 3 * std::int64_t(v[NV == 4 ? 3 : 4]);
+    T J[9], adj[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+    {
+      J[3 * r + 0] = q1[r] - q0[r];
+      J[3 * r + 1] = q2[r] - q0[r];
+      J[3 * r + 2] = q3[r] - q0[r];
+    }
+    const T det = adjugate(J, adj);
+    const T adet = det < T(0) ? -det : det;
+    if constexpr (NV == 4)
+    {
+      const T sum = f[0] + f[1] + f[2] + f[3];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        out[i] = adet / T(120) * (sum + f[i]);
+    }
+    else
+    {
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+      {
+        T a = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+        {
+          T m = 1;
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            m *= (((i >> d) & 1) == ((j >> d) & 1)) ? T(1) / T(3) : T(1) / T(6);
+          a += m * f[j];
+        }
+        out[i] = adet * a;
+      }
+    }
+  }
+  else
+  {
+    T cdv[8][3];
+    for (int w = 0; w < 8; ++w)
+      for (int r = 0; r < 3; ++r)
+        cdv[w][r] = x[3 * std::int64_t(v[w]) + r];
+    for (int i = 0; i < NV; ++i)
+      out[i] = 0;
+    const T ga = T(0.5) - T(0.5) / sqrt(T(3)), gb = T(0.5) + T(0.5) / sqrt(T(3));
+    for (int qp = 0; qp < 8; ++qp)
+    {
+      const T xi[3] = {(qp & 1) ? gb : ga, ((qp >> 1) & 1) ? gb : ga, ((qp >> 2) & 1) ? gb : ga};
+      T phi[8], dphi[8][3], J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, adj[9];
+      q1_tab(xi, phi, dphi);
+      for (int w = 0; w < 8; ++w)
+        for (int r = 0; r < 3; ++r)
+          for (int a = 0; a < 3; ++a)
+            J[3 * r + a] += cdv[w][r] * dphi[w][a];
+      const T det = adjugate(J, adj);
+      const T adet = det < T(0) ? -det : det;
+      T fq = 0;
+      for (int w = 0; w < 8; ++w)
+        fq += f[w] * phi[w];
+      for (int i = 0; i < NV; ++i)
+        out[i] += adet / T(8) * fq * phi[i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+    be[inc_pos[c * NV + i]] = out[i];
+}
+
+/// b[r] += sum of the element vector entries of the row's incidences
+/// (stored in incidence order, so each row reads a contiguous range).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    row_vector_k(std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
+                 const T* __restrict__ be, T* __restrict__ b)
+{
+  const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nplan)
+    return;
+  T s = 0;
+  for (std::int32_t p = inc_off[r]; p < inc_off[r + 1]; ++p)
+    s += be[p];
+  b[r] += s;
+}
+
+/// Reference tensors of the affine stiffness + reaction forms, as
+/// [7][k][k]: the symmetrised stiffness integrals S00, S11, S22, S01+S10,
+/// S02+S20, S12+S21 (S_ab(i,j) = int d_a phi_i d_b phi_j over the reference
+/// cell) and the mass M(i,j) = int phi_i phi_j.
+std::vector<double> reference_tensors(int kernel)
+{
+  if (kernel == DFX_K_P1TET_A)
+  {
+    const double d[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    std::vector<double> t(7 * 16);
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j)
+      {
+        const int e = 4 * i + j;
+        auto S = [&](int a, int b) { return d[i][a] * d[j][b] / 6.0; };
+        t[0 * 16 + e] = S(0, 0);
+        t[1 * 16 + e] = S(1, 1);
+        t[2 * 16 + e] = S(2, 2);
+        t[3 * 16 + e] = S(0, 1) + S(1, 0);
+        t[4 * 16 + e] = S(0, 2) + S(2, 0);
+        t[5 * 16 + e] = S(1, 2) + S(2, 1);
+        t[6 * 16 + e] = (i == j ? 2.0 : 1.0) / 120.0;
+      }
+    return t;
+  }
+  if (kernel == DFX_K_Q1HEX_A)
+  {
+    std::vector<double> t(7 * 64);
+    for (int i = 0; i < 8; ++i)
+      for (int j = 0; j < 8; ++j)
+      {
+        double m[3], kk[3], dd[3], ee[3];
+        for (int a = 0; a < 3; ++a)
+        {
+          const int p = (i >> a) & 1, q = (j >> a) & 1;
+          m[a] = p == q ? 1.0 / 3.0 : 1.0 / 6.0;
+          kk[a] = p == q ? 1.0 : -1.0;
+          dd[a] = p ? 0.5 : -0.5;
+          ee[a] = q ? 0.5 : -0.5;
+        }
+        const int e = 8 * i + j;
+        t[0 * 64 + e] = kk[0] * m[1] * m[2];
+        t[1 * 64 + e] = m[0] * kk[1] * m[2];
+        t[2 * 64 + e] = m[0] * m[1] * kk[2];
+        t[3 * 64 + e] = (dd[0] * ee[1] + ee[0] * dd[1]) * m[2];
+        t[4 * 64 + e] = (dd[0] * ee[2] + ee[0] * dd[2]) * m[1];
+        t[5 * 64 + e] = (dd[1] * ee[2] + ee[1] * dd[2]) * m[0];
+        t[6 * 64 + e] = m[0] * m[1] * m[2];
+      }
+    return t;
+  }
+  return {};
+}
+
+bool row_path(const dfx_form& f)
+{
+  return f.affine && f.ref_tensors
+         && (f.d.kernel == DFX_K_P1TET_A || f.d.kernel == DFX_K_Q1HEX_A);
+}
+
+/// Incidence slot offsets of form f for the pattern of A (built once).
+void ensure_plan(dfx_form& f, const dfx_csr& A)
+{
+  if (f.islots && f.plan_pattern == A.pattern_id)
+    return;
+  dfree(f.islots);
+  f.islots = nullptr;
+  const int k = f.d.dofs_per_cell;
+  const std::int64_t ninc = std::int64_t(f.d.num_cells) * k;
+  require(A.max_row <= 65536, "row too long for the slot map");
+  f.islot_bytes = A.max_row <= 256 ? 1 : 2;
+  f.islots = dmalloc(std::max<std::int64_t>(ninc * k, 1) * f.islot_bytes);
+  int* err = static_cast<int*>(dmalloc(sizeof(int)));
+  DFX_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
+  if (ninc > 0)
+  {
+    if (f.islot_bytes == 1)
+      incidence_slots_k<std::uint8_t><<<blocks_for(ninc * k), kBlock, 0, stream()>>>(
+          ninc, k, f.inc, f.dofmap, A.row_ptr, A.cols, static_cast<std::uint8_t*>(f.islots), err);
+    else
+      incidence_slots_k<std::uint16_t><<<blocks_for(ninc * k), kBlock, 0, stream()>>>(
+          ninc, k, f.inc, f.dofmap, A.row_ptr, A.cols, static_cast<std::uint16_t*>(f.islots), err);
+    check_launch("incidence_slots");
+  }
+  int h = 0;
+  d2h(&h, err, sizeof(int));
+  dfree(err);
+  if (h)
+  {
+    dfree(f.islots);
+    f.islots = nullptr;
+    throw Error(DFX_ERR_PATTERN, "Entry not in sparsity");
+  }
+  f.plan_pattern = A.pattern_id;
+}
+
+template <typename T>
+void launch_cell_factors(const dfx_form& f, const dfx_bc* bc)
+{
+  const int nv = f.d.nodes_per_cell;
+  const std::int32_t nc = f.d.num_cells;
+  const T creac = static_cast<T>(f.constants[0]);
+  std::uint32_t* mask = bc ? f.cell_mask : nullptr;
+  if (nv == 4)
+    cell_factors_k<T, 4><<<blocks_for(nc), kBlock, 0, stream()>>>(
+        nc, static_cast<const T*>(f.x), f.x_dofmap, f.dofmap, f.d.dofs_per_cell,
+        static_cast<const T*>(f.coeffs), f.d.cstride, creac, bc ? bc->marker : nullptr,
+        static_cast<T*>(f.cell_data), mask);
+  else
+    cell_factors_k<T, 8><<<blocks_for(nc), kBlock, 0, stream()>>>(
+        nc, static_cast<const T*>(f.x), f.x_dofmap, f.dofmap, f.d.dofs_per_cell,
+        static_cast<const T*>(f.coeffs), f.d.cstride, creac, bc ? bc->marker : nullptr,
+        static_cast<T*>(f.cell_data), mask);
+  check_launch("cell_factors");
+}
+
+/// Rows per block of the row kernel: LDS holds the block's CSR slice.
+int row_block(const dfx_csr& A, int k, std::size_t tsize)
+{
+  const std::size_t budget = 64 * 1024 - 7 * k * k * tsize;
+  int R = 256;
+  while (R > 64 && std::size_t(R) * A.max_row * tsize > budget)
+    R /= 2;
+  return R;
+}
+
+template <typename T, typename O, int K>
+void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
+{
+  launch_cell_factors<T>(f, bc);
+  const int R = row_block(A, K, sizeof(T));
+  const std::size_t lds = (7 * K * K + std::size_t(R) * A.max_row) * sizeof(T);
+  require(lds <= 160 * 1024, "row too long for the LDS slice");
+  const int nb = (A.nrows + R - 1) / R;
+  if (nb > 0)
+    row_assemble_k<T, O, K><<<nb, R, lds, stream()>>>(
+        A.nrows, f.max_dof + 1, f.inc_off, f.inc, static_cast<const O*>(f.islots),
+        static_cast<const T*>(f.cell_data), bc ? f.cell_mask : nullptr, bc ? bc->marker : nullptr,
+        A.row_ptr, static_cast<const T*>(f.ref_tensors), static_cast<T*>(A.values),
+        A.zeroed ? 0 : 1);
+  check_launch("row_assemble");
+}
+
+template <typename T>
+void assemble_rows(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
+{
+  ensure_plan(f, A);
+  const int K = f.d.dofs_per_cell;
+  if (K == 4)
+  {
+    if (f.islot_bytes == 1)
+      launch_row_assemble<T, std::uint8_t, 4>(f, A, bc);
+    else
+      launch_row_assemble<T, std::uint16_t, 4>(f, A, bc);
+  }
+  else
+  {
+    if (f.islot_bytes == 1)
+      launch_row_assemble<T, std::uint8_t, 8>(f, A, bc);
+    else
+      launch_row_assemble<T, std::uint16_t, 8>(f, A, bc);
+  }
+}
+
+template <typename T>
+void lift_rows(dfx_form& f, const dfx_bc& bc, const T* x0, double alpha, T* b)
+{
+  launch_cell_factors<T>(f, &bc);
+  const std::int32_t nplan = f.max_dof + 1;
+  const T* cd = static_cast<const T*>(f.cell_data);
+  const T* ref = static_cast<const T*>(f.ref_tensors);
+  if (f.d.dofs_per_cell == 4)
+    row_lift_k<T, 4><<<blocks_for(nplan), kBlock, 0, stream()>>>(
+        nplan, f.inc_off, f.inc, f.dofmap, cd, f.cell_mask, ref, bc.g, x0, alpha, b);
+  else
+    row_lift_k<T, 8><<<blocks_for(nplan), kBlock, 0, stream()>>>(
+        nplan, f.inc_off, f.inc, f.dofmap, cd, f.cell_mask, ref, bc.g, x0, alpha, b);
+  check_launch("row_lift");
+}
+
+template <typename T>
+void vector_rows(const dfx_form& f, T* b)
+{
+  const std::int32_t nc = f.d.num_cells;
+  const int nv = f.d.nodes_per_cell;
+  T* be = static_cast<T*>(f.cell_vec);
+  if (nv == 4)
+    cell_source_k<T, 4><<<blocks_for(nc), kBlock, 0, stream()>>>(
+        nc, static_cast<const T*>(f.x), f.x_dofmap, static_cast<const T*>(f.coeffs),
+        f.d.cstride, 1, f.inc_pos, be);
+  else
+    cell_source_k<T, 8><<<blocks_for(nc), kBlock, 0, stream()>>>(
+        nc, static_cast<const T*>(f.x), f.x_dofmap, static_cast<const T*>(f.coeffs),
+        f.d.cstride, f.affine ? 1 : 0, f.inc_pos, be);
+  const std::int32_t nplan = f.max_dof + 1;
+  row_vector_k<T><<<blocks_for(nplan), kBlock, 0, stream()>>>(nplan, f.inc_off, be, b);
+  check_launch("assemble_vector");
+}
+
 template <typename T, typename O, int MODE>
 void launch_bilinear(const dfx_form& f, const dfx_csr* A, const dfx_bc* bc, const T* x0,
                      double alpha, T* b)
@@ -772,6 +1298,50 @@ dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out)
           h2d(f->dofmap, d.dofmap, sizeof(std::int32_t) * nc * d.dofs_per_cell);
         }
         f->coeffs = upload_as(d.coeffs, nc * d.cstride, d.dtype);
+        // Row-owner plan: incidences (cell, local index) of every row, in
+        // cell order (counting sort of the dofmap entries)
+        {
+          const int k = d.dofs_per_cell;
+          const std::int32_t nplan = max_dof + 1;
+          std::vector<std::int32_t> off(std::size_t(nplan) + 1, 0);
+          for (std::int64_t i = 0; i < nc * k; ++i)
+            ++off[d.dofmap[i] + 1];
+          std::int32_t mx = 0;
+          for (std::int32_t r = 0; r < nplan; ++r)
+          {
+            mx = std::max(mx, off[r + 1]);
+            off[r + 1] += off[r];
+          }
+          std::vector<std::int32_t> inc(std::max<std::int64_t>(nc * k, 1));
+          std::vector<std::int32_t> pos(off.begin(), off.end() - 1);
+          for (std::int64_t i = 0; i < nc * k; ++i)
+            inc[pos[d.dofmap[i]]++] = static_cast<std::int32_t>(i);
+          f->max_inc = mx;
+          f->inc_off = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * off.size()));
+          h2d(f->inc_off, off.data(), sizeof(std::int32_t) * off.size());
+          f->inc = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * inc.size()));
+          h2d(f->inc, inc.data(), sizeof(std::int32_t) * inc.size());
+          if (!ki.bilinear)
+          {
+            // position of each (cell, local index) in the incidence order
+            std::vector<std::int32_t> ipos(inc.size());
+            for (std::int64_t q = 0; q < nc * k; ++q)
+              ipos[inc[q]] = static_cast<std::int32_t>(q);
+            f->inc_pos = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * ipos.size()));
+            h2d(f->inc_pos, ipos.data(), sizeof(std::int32_t) * ipos.size());
+          }
+          const std::size_t ts = dtype_size(d.dtype);
+          if (ki.bilinear)
+          {
+            f->cell_data = dmalloc(ts * 8 * std::max<std::int64_t>(nc, 1));
+            f->cell_mask = static_cast<std::uint32_t*>(dmalloc(sizeof(std::uint32_t) * std::max<std::int64_t>(nc, 1)));
+            const std::vector<double> ref = reference_tensors(d.kernel);
+            if (!ref.empty())
+              f->ref_tensors = upload_as(ref.data(), static_cast<std::int64_t>(ref.size()), d.dtype);
+          }
+          else
+            f->cell_vec = dmalloc(ts * std::max<std::int64_t>(nc * k * d.bs, 1));
+        }
         *out = f.release();
       });
 }
@@ -786,6 +1356,14 @@ dfx_status dfx_form_destroy(dfx_form* f)
     dfree(f->x);
     dfree(f->coeffs);
     dfree(f->slots);
+    dfree(f->inc_off);
+    dfree(f->inc);
+    dfree(f->inc_pos);
+    dfree(f->islots);
+    dfree(f->cell_data);
+    dfree(f->cell_mask);
+    dfree(f->ref_tensors);
+    dfree(f->cell_vec);
     delete f;
   }
   return DFX_OK;
@@ -864,6 +1442,16 @@ dfx_status dfx_assemble_matrix(dfx_csr* A, const dfx_form* a, const dfx_bc* bc)
         require(!bc || bc->n_total == A->nrows * A->bs, "BC size does not match the matrix");
         require(a->max_dof < A->nrows && a->max_dof < A->ncols, "dofmap exceeds the matrix size");
         dfx_form& f = const_cast<dfx_form&>(*a);
+        if (row_path(f))
+        {
+          if (A->dtype == DFX_F64)
+            assemble_rows<double>(f, *A, bc);
+          else
+            assemble_rows<float>(f, *A, bc);
+          A->zeroed = false;
+          return;
+        }
+        A->zeroed = false;
         ensure_slots(f, *A);
         if (A->dtype == DFX_F64)
         {
@@ -901,6 +1489,7 @@ dfx_status dfx_set_diagonal(dfx_csr* A, const dfx_bc* bc, double diag)
               bc->ndofs, bc->dofs, A->bs, A->diag, static_cast<float*>(A->values),
               static_cast<float>(diag), err);
         check_launch("set_diagonal");
+        A->zeroed = false;
         int h = 0;
         d2h(&h, err, sizeof(int));
         dfree(err);
@@ -920,6 +1509,14 @@ dfx_status dfx_assemble_vector(dfx_vec* b, const dfx_form* L)
         require(!ki.bilinear, "assemble_vector needs a linear form");
         require(b->dtype == L->d.dtype, "vector and form dtypes differ");
         require(std::int64_t(L->max_dof + 1) * L->d.bs <= b->n, "dofmap exceeds the vector size");
+        if (L->d.kernel == DFX_K_P1TET_L || L->d.kernel == DFX_K_Q1HEX_L)
+        {
+          if (b->dtype == DFX_F64)
+            vector_rows<double>(*L, static_cast<double*>(b->data));
+          else
+            vector_rows<float>(*L, static_cast<float*>(b->data));
+          return;
+        }
         if (b->dtype == DFX_F64)
           launch_linear<double>(*L, static_cast<double*>(b->data));
         else
@@ -943,7 +1540,20 @@ dfx_status dfx_apply_lifting(dfx_vec* b, const dfx_form* a, const dfx_bc* bc, co
         require(bc->n_total == b->n, "BC size does not match the vector");
         require(!x0 || (x0->n == b->n && x0->dtype == b->dtype), "x0 does not match b");
         require(std::int64_t(a->max_dof + 1) * a->d.bs <= b->n, "dofmap exceeds the vector size");
-        const dfx_form& f = *a;
+        // g = 0 and x0 = 0: every term alpha A_ij (g_j - x0_j) is zero
+        if (!bc->g && !x0)
+          return;
+        dfx_form& f = const_cast<dfx_form&>(*a);
+        if (row_path(f))
+        {
+          if (b->dtype == DFX_F64)
+            lift_rows<double>(f, *bc, x0 ? static_cast<const double*>(x0->data) : nullptr, alpha,
+                              static_cast<double*>(b->data));
+          else
+            lift_rows<float>(f, *bc, x0 ? static_cast<const float*>(x0->data) : nullptr, alpha,
+                             static_cast<float*>(b->data));
+          return;
+        }
         if (b->dtype == DFX_F64)
           launch_bilinear<double, std::uint8_t, 1>(
               f, nullptr, bc, x0 ? static_cast<const double*>(x0->data) : nullptr, alpha,

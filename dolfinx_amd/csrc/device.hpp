@@ -74,49 +74,52 @@ __device__ inline double block_sum_256(double v, double* lds /*[4]*/)
   return s;
 }
 
-/// Grid-wide reduction of NV values with a last-block finish: every block
-/// stores its partial sums, the block that takes the final ticket sums all
-/// partials in block order (deterministic) and runs fin(sums) in thread 0.
-/// The ticket is reset by that block for the next launch.
-template <int NV, typename Fin>
-__device__ inline void grid_reduce(double (&v)[NV], double* partials,
-                                   unsigned* ticket, Fin&& fin)
+/// Reads a device flag once per block (thread 0) and broadcasts it, so
+/// that every wavefront of the block takes the same branch.
+__device__ inline int block_flag(const int* flag)
+{
+  __shared__ int f;
+  if (threadIdx.x == 0)
+    f = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const int v = f;
+  __syncthreads();
+  return v;
+}
+
+/// Per-block partial sums of NV values: block b stores its sums at
+/// partials[k * gridDim.x + b]. No atomics: a later kernel (or the one-block
+/// finish_sums_k) adds the partials in block order, so every reduction is
+/// deterministic.
+template <int NV>
+__device__ inline void block_partials(const double (&v)[NV], double* partials)
 {
   __shared__ double lds[4];
-  __shared__ bool last;
-  double s[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k)
-    s[k] = block_sum_256(v[k], lds);
-  const int nb = gridDim.x;
-  if (threadIdx.x == 0)
   {
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-      partials[k * nb + blockIdx.x] = s[k];
-    __threadfence();
-    const unsigned t = atomicAdd(ticket, 1u);
-    last = (t == static_cast<unsigned>(nb) - 1);
+    const double s = block_sum_256(v[k], lds);
+    if (threadIdx.x == 0)
+      partials[k * gridDim.x + blockIdx.x] = s;
   }
+}
+
+/// Sum of n partials, computed by the whole block in a fixed order; the
+/// result is returned to every thread.
+__device__ inline double sum_partials(const double* partials, int n)
+{
+  __shared__ double lds[4];
+  __shared__ double total;
+  double a = 0;
+  for (int i = threadIdx.x; i < n; i += 256)
+    a += partials[i];
+  const double s = block_sum_256(a, lds);
+  if (threadIdx.x == 0)
+    total = s;
   __syncthreads();
-  if (!last)
-    return;
-  __threadfence();
-  double tot[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k)
-  {
-    double a = 0;
-    for (int i = threadIdx.x; i < nb; i += 256)
-      a += __hip_atomic_load(partials + k * nb + i, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-    tot[k] = block_sum_256(a, lds);
-  }
-  if (threadIdx.x == 0)
-  {
-    fin(tot);
-    *ticket = 0;
-  }
+  const double t = total;
+  __syncthreads();
+  return t;
 }
 
 } // namespace dfx
@@ -132,6 +135,8 @@ struct dfx_csr
   std::int32_t* diag = nullptr;    // device, position of (i,i) or -1
   void* values = nullptr;          // device, nnz*bs*bs of dtype
   std::uint64_t pattern_id = 0;    // identifies the pattern for slot caches
+  bool zeroed = true;              // values known to be all zero
+  int lanes = 0;                   // lanes per row of SpMV (0: heuristic)
 };
 
 struct dfx_vec
@@ -163,4 +168,20 @@ struct dfx_form
   std::uint64_t slot_pattern = 0;
   void* slots = nullptr;
   int slot_bytes = 0;         // 1 (uint8) or 2 (uint16)
+  // Row-owner plan: incidences of each row (cell, local index), in cell
+  // order, and for each incidence the offsets inside the row of the
+  // cell's k column dofs. Built once per pattern.
+  std::int32_t* inc_off = nullptr; // device, (max_dof + 2)
+  std::int32_t* inc = nullptr;     // device, num_cells * k: c * k + i
+  std::int32_t* inc_pos = nullptr; // device, inverse of inc (linear forms)
+  void* islots = nullptr;          // device, num_cells * k * k
+  std::uint64_t plan_pattern = 0;
+  int islot_bytes = 0;
+  std::int32_t max_inc = 0;        // most incidences of one row
+  // Per-cell factors for the affine row kernel (8 values of dtype) and
+  // per-cell BC column masks; reference tensors (7 x k x k of dtype).
+  void* cell_data = nullptr;
+  std::uint32_t* cell_mask = nullptr;
+  void* ref_tensors = nullptr;
+  void* cell_vec = nullptr;        // per-cell element vectors (linear forms)
 };

@@ -126,17 +126,28 @@ __global__ void __launch_bounds__(kBlock)
     r[i] = static_cast<T>(alpha) * x[i] + y[i];
 }
 
-/// out = sum a_i b_i, accumulated in FP64
+/// Partial sums of a_i b_i (FP64) per block.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-    dot_k(const T* a, const T* b, std::int64_t n, double* partials,
-          unsigned* ticket, double* out)
+    dot_k(const T* a, const T* b, std::int64_t n, double* partials)
 {
   double s[1] = {0.0};
   for (std::int64_t i = blockIdx.x * std::int64_t(kBlock) + threadIdx.x; i < n;
        i += std::int64_t(gridDim.x) * kBlock)
     s[0] += double(a[i]) * double(b[i]);
-  grid_reduce<1>(s, partials, ticket, [out](double* t) { *out = t[0]; });
+  block_partials<1>(s, partials);
+}
+
+/// One block: out[k] = sum of the nb partials of value k.
+__global__ void __launch_bounds__(kBlock)
+    finish_sums_k(const double* partials, int nb, int nv, double* out)
+{
+  for (int k = 0; k < nv; ++k)
+  {
+    const double t = sum_partials(partials + std::int64_t(k) * nb, nb);
+    if (threadIdx.x == 0)
+      out[k] = t;
+  }
 }
 
 /// Position of the diagonal entry of each block row.
@@ -159,6 +170,36 @@ __global__ void __launch_bounds__(kBlock)
   diag[r] = (lo < rp[r + 1] && ci[lo] == r) ? lo : -1;
 }
 
+/// Partial dot product of one lane's share of a CSR row [b, e): the lane
+/// takes entries b + lane, b + lane + G, ... The first four are loaded
+/// together (values and columns streamed with nontemporal loads) before
+/// any x gather, so each lane keeps several loads in flight.
+template <typename T, int G>
+__device__ inline T row_share(std::int32_t b, std::int32_t e, int lane,
+                              const std::int32_t* __restrict__ ci, const T* __restrict__ v,
+                              const T* __restrict__ x)
+{
+  constexpr int U = 4;
+  const std::int32_t j0 = b + lane;
+  T vv[U];
+  std::int32_t cc[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+  {
+    const std::int32_t j = j0 + u * G;
+    vv[u] = j < e ? __builtin_nontemporal_load(v + j) : T(0);
+    cc[u] = j < e ? __builtin_nontemporal_load(ci + j) : 0;
+  }
+  T s = 0;
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (j0 + u * G < e)
+      s += vv[u] * x[cc[u]];
+  for (std::int32_t j = j0 + U * G; j < e; j += G)
+    s += v[j] * x[ci[j]];
+  return s;
+}
+
 /// y (+)= A x for scalar CSR: G lanes per row, each lane strides the row,
 /// partial sums combined with wavefront shuffles. Rows are handed out in
 /// XCD-contiguous chunks of kBlock/G rows. Accumulation in T.
@@ -177,11 +218,7 @@ __global__ void __launch_bounds__(kBlock)
     const std::int32_t row = chunk * RPB + threadIdx.x / G;
     T s = 0;
     if (row < nrows)
-    {
-      const std::int32_t e = rp[row + 1];
-      for (std::int32_t j = rp[row] + lane; j < e; j += G)
-        s += v[j] * x[ci[j]];
-    }
+      s = row_share<T, G>(rp[row], rp[row + 1], lane, ci, v, x);
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1)
       s += __shfl_down(s, off, G);
@@ -215,17 +252,25 @@ __global__ void __launch_bounds__(kBlock)
   y[t] = accumulate ? y[t] + s : s;
 }
 
+/// Lanes per row of the scalar CSR kernels for a mean row length.
+int lanes_per_row(const dfx_csr& A)
+{
+  if (A.lanes)
+    return A.lanes;
+  const double mean = A.nrows ? double(A.nnz) / A.nrows : 1.0;
+  int G = 2;
+  while (G < 64 && G * 6 < mean)
+    G *= 2;
+  return G;
+}
+
 template <typename T>
 void launch_spmv(const dfx_csr& A, const T* x, T* y, bool accumulate)
 {
   hipStream_t s = stream();
   if (A.bs == 1)
   {
-    // lanes per row from the mean row length (power of two, 2..64)
-    const double mean = A.nrows ? double(A.nnz) / A.nrows : 1.0;
-    int G = 2;
-    while (G < 64 && G * 3 < mean * 2)
-      G *= 2;
+    const int G = lanes_per_row(A);
     const int rpb = kBlock / G;
     const int nchunk = (A.nrows + rpb - 1) / rpb;
     const int nb = std::max(1, std::min(nchunk, ctx().num_cu * 16));
@@ -262,18 +307,25 @@ void launch_spmv(const dfx_csr& A, const T* x, T* y, bool accumulate)
 }
 
 // ---------------------------------------------------------------------
-// Conjugate gradients, device resident. Scalars live in CGState; each
-// kernel returns at once when the solve has converged, so a batch of
-// queued iterations costs only launch latency after convergence.
+// Conjugate gradients, device resident. Three kernels per iteration k,
+// no atomics and no host round trip:
+//   cg_spmv_k      q = A p, partial sums of p.q
+//   cg_update_k    alpha = rz_k / p.q (each block sums the partials);
+//                  x += alpha p, r -= alpha q; partials of r.z and r.r
+//   cg_direction_k beta = rz_{k+1} / rz_k; stop test ||r||^2/||r0||^2 <
+//                  rtol^2; p = z + beta p; block 0 records the scalars.
+// rz is kept in a two-slot ring indexed by the iteration parity, so the
+// slot being read in an iteration is never the one written. After the
+// stop test fires every later kernel returns at once.
 // ---------------------------------------------------------------------
 
-constexpr int kMaxPartials = 4096;
+constexpr int kMaxPartials = 1024;
 
 struct CGState
 {
-  double rz, rr, rr0, pq, alpha, beta, tol2;
+  double rz[2];
+  double rr0, rr, tol2;
   std::int32_t iter, done, max_it, pad;
-  unsigned ticket[4];
   double partials[3 * kMaxPartials];
 };
 
@@ -292,11 +344,10 @@ __global__ void __launch_bounds__(kBlock)
   dinv[t] = d != T(0) ? T(1) / d : T(1);
 }
 
-/// r = b - y; z = dinv r; p = z; reduce (r.z, r.r); starts the solve.
+/// r = b - y; p = z = dinv r; partial sums of (r.z, r.r)
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-    cg_start_k(std::int64_t n, const T* b, const T* y, const T* dinv, T* r, T* p,
-               CGState* st)
+    cg_start_k(std::int64_t n, const T* b, const T* y, const T* dinv, T* r, T* p, CGState* st)
 {
   double s[2] = {0.0, 0.0};
   for (std::int64_t i = blockIdx.x * std::int64_t(kBlock) + threadIdx.x; i < n;
@@ -309,25 +360,32 @@ __global__ void __launch_bounds__(kBlock)
     s[0] += double(ri) * double(zi);
     s[1] += double(ri) * double(ri);
   }
-  grid_reduce<2>(s, st->partials, st->ticket,
-                 [st](double* t)
-                 {
-                   st->rz = t[0];
-                   st->rr = t[1];
-                   st->rr0 = t[1];
-                   st->iter = 0;
-                   st->done = (t[1] == 0.0) ? 1 : 0;
-                 });
+  block_partials<2>(s, st->partials + kMaxPartials);
 }
 
-/// q = A p and p.q for scalar CSR (G lanes per row); alpha = rz / pq.
+/// One block: rz_0, rr0 from the start partials.
+__global__ void __launch_bounds__(kBlock) cg_start_finish_k(int nb, CGState* st)
+{
+  const double rz = sum_partials(st->partials + kMaxPartials, nb);
+  const double rr = sum_partials(st->partials + kMaxPartials + nb, nb);
+  if (threadIdx.x == 0)
+  {
+    st->rz[0] = rz;
+    st->rr0 = rr;
+    st->rr = rr;
+    st->iter = 0;
+    st->done = (rr == 0.0 || st->max_it <= 0) ? 1 : 0;
+  }
+}
+
+/// q = A p and partial sums of p.q, G lanes per row.
 template <typename T, int G>
 __global__ void __launch_bounds__(kBlock)
     cg_spmv_k(std::int32_t nrows, const std::int32_t* __restrict__ rp,
               const std::int32_t* __restrict__ ci, const T* __restrict__ v,
               const T* __restrict__ p, T* __restrict__ q, CGState* st)
 {
-  if (st->done)
+  if (block_flag(&st->done))
     return;
   constexpr int RPB = kBlock / G;
   const int nchunk = (nrows + RPB - 1) / RPB;
@@ -339,11 +397,7 @@ __global__ void __launch_bounds__(kBlock)
     const std::int32_t row = chunk * RPB + threadIdx.x / G;
     T s = 0;
     if (row < nrows)
-    {
-      const std::int32_t e = rp[row + 1];
-      for (std::int32_t j = rp[row] + lane; j < e; j += G)
-        s += v[j] * p[ci[j]];
-    }
+      s = row_share<T, G>(rp[row], rp[row + 1], lane, ci, v, p);
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1)
       s += __shfl_down(s, off, G);
@@ -353,12 +407,7 @@ __global__ void __launch_bounds__(kBlock)
       d[0] += double(s) * double(p[row]);
     }
   }
-  grid_reduce<1>(d, st->partials, st->ticket + 1,
-                 [st](double* t)
-                 {
-                   st->pq = t[0];
-                   st->alpha = st->rz / t[0];
-                 });
+  block_partials<1>(d, st->partials);
 }
 
 /// Block CSR variant of cg_spmv_k (one lane per (block row, component)).
@@ -368,7 +417,7 @@ __global__ void __launch_bounds__(kBlock)
                   const std::int32_t* __restrict__ ci, const T* __restrict__ v,
                   const T* __restrict__ p, T* __restrict__ q, CGState* st)
 {
-  if (st->done)
+  if (block_flag(&st->done))
     return;
   const std::int64_t n = std::int64_t(nrows) * BS;
   double d[1] = {0.0};
@@ -389,24 +438,19 @@ __global__ void __launch_bounds__(kBlock)
     q[t] = s;
     d[0] += double(s) * double(p[t]);
   }
-  grid_reduce<1>(d, st->partials, st->ticket + 1,
-                 [st](double* t)
-                 {
-                   st->pq = t[0];
-                   st->alpha = st->rz / t[0];
-                 });
+  block_partials<1>(d, st->partials);
 }
 
-/// x += alpha p; r -= alpha q; reduce (r.z, r.r) with z = dinv r;
-/// convergence test ||r||^2/||r0||^2 < rtol^2; beta = rz_new / rz.
+/// alpha = rz_k / p.q; x += alpha p; r -= alpha q; partials of r.z, r.r.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-    cg_update_k(std::int64_t n, T* x, T* r, const T* p, const T* q, const T* dinv,
-                CGState* st)
+    cg_update_k(std::int64_t n, int k, int nb_spmv, T* x, T* r, const T* p, const T* q,
+                const T* dinv, CGState* st)
 {
-  if (st->done)
+  if (block_flag(&st->done))
     return;
-  const T alpha = static_cast<T>(st->alpha);
+  const double pq = sum_partials(st->partials, nb_spmv);
+  const T alpha = static_cast<T>(st->rz[k & 1] / pq);
   double s[2] = {0.0, 0.0};
   for (std::int64_t i = blockIdx.x * std::int64_t(kBlock) + threadIdx.x; i < n;
        i += std::int64_t(gridDim.x) * kBlock)
@@ -418,29 +462,41 @@ __global__ void __launch_bounds__(kBlock)
     s[0] += double(ri) * double(zi);
     s[1] += double(ri) * double(ri);
   }
-  grid_reduce<2>(s, st->partials, st->ticket + 2,
-                 [st](double* t)
-                 {
-                   st->iter += 1;
-                   st->rr = t[1];
-                   st->beta = t[0] / st->rz;
-                   st->rz = t[0];
-                   if (t[1] / st->rr0 < st->tol2 || st->iter >= st->max_it)
-                     st->done = 1;
-                 });
+  block_partials<2>(s, st->partials + kMaxPartials);
 }
 
-/// p = dinv r + beta p
+/// beta = rz_{k+1} / rz_k; stop test; p = dinv r + beta p.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-    cg_direction_k(std::int64_t n, const T* r, const T* dinv, T* p, const CGState* st)
+    cg_direction_k(std::int64_t n, int k, int nb_upd, const T* r, const T* dinv, T* p,
+                   CGState* st)
 {
-  if (st->done)
+  if (block_flag(&st->done))
     return;
-  const T beta = static_cast<T>(st->beta);
+  const double rz = sum_partials(st->partials + kMaxPartials, nb_upd);
+  const double rr = sum_partials(st->partials + kMaxPartials + nb_upd, nb_upd);
+  const bool stop = rr / st->rr0 < st->tol2 || k + 1 >= st->max_it;
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+  {
+    st->rz[(k + 1) & 1] = rz;
+    st->rr = rr;
+    st->iter = k + 1;
+    if (stop)
+      st->done = 1;
+  }
+  if (stop)
+    return;
+  const T beta = static_cast<T>(rz / st->rz[k & 1]);
   for (std::int64_t i = blockIdx.x * std::int64_t(kBlock) + threadIdx.x; i < n;
        i += std::int64_t(gridDim.x) * kBlock)
     p[i] = (dinv ? dinv[i] * r[i] : r[i]) + beta * p[i];
+}
+
+/// Blocks of a reduction kernel: at most kMaxPartials.
+int partial_blocks(std::int64_t n)
+{
+  std::int64_t b = (n + kBlock * 8 - 1) / (kBlock * 8);
+  return static_cast<int>(std::clamp<std::int64_t>(b, 1, kMaxPartials));
 }
 
 /// Per-dtype scratch of a CG solve (device).
@@ -475,19 +531,16 @@ struct StateHolder
 };
 
 template <typename T>
-void launch_cg_spmv(const dfx_csr& A, const T* p, T* q, CGState* st)
+int launch_cg_spmv(const dfx_csr& A, const T* p, T* q, CGState* st)
 {
   hipStream_t s = stream();
   const T* v = static_cast<const T*>(A.values);
   if (A.bs == 1)
   {
-    const double mean = A.nrows ? double(A.nnz) / A.nrows : 1.0;
-    int G = 2;
-    while (G < 64 && G * 3 < mean * 2)
-      G *= 2;
+    const int G = lanes_per_row(A);
     const int rpb = kBlock / G;
     const int nchunk = (A.nrows + rpb - 1) / rpb;
-    const int nb = std::max(1, std::min({nchunk, ctx().num_cu * 8, kMaxPartials}));
+    const int nb = std::max(1, std::min(nchunk, kMaxPartials));
     switch (G)
     {
 #define DFX_CG_CASE(GG)                                                        \
@@ -502,17 +555,16 @@ void launch_cg_spmv(const dfx_csr& A, const T* p, T* q, CGState* st)
       DFX_CG_CASE(64)
 #undef DFX_CG_CASE
     }
+    return nb;
   }
+  const int nb = partial_blocks(std::int64_t(A.nrows) * A.bs);
+  if (A.bs == 3)
+    cg_bsr_spmv_k<T, 3><<<nb, kBlock, 0, s>>>(A.nrows, A.row_ptr, A.cols, v, p, q, st);
+  else if (A.bs == 2)
+    cg_bsr_spmv_k<T, 2><<<nb, kBlock, 0, s>>>(A.nrows, A.row_ptr, A.cols, v, p, q, st);
   else
-  {
-    const int nb = std::min(reduce_blocks(std::int64_t(A.nrows) * A.bs), kMaxPartials);
-    if (A.bs == 3)
-      cg_bsr_spmv_k<T, 3><<<nb, kBlock, 0, s>>>(A.nrows, A.row_ptr, A.cols, v, p, q, st);
-    else if (A.bs == 2)
-      cg_bsr_spmv_k<T, 2><<<nb, kBlock, 0, s>>>(A.nrows, A.row_ptr, A.cols, v, p, q, st);
-    else
-      throw Error(DFX_ERR_ARG, "block size must be 1..3");
-  }
+    throw Error(DFX_ERR_ARG, "block size must be 1..3");
+  return nb;
 }
 
 /// Runs (preconditioned) CG on A x = b. All arrays are device arrays of T.
@@ -520,11 +572,11 @@ void launch_cg_spmv(const dfx_csr& A, const T* p, T* q, CGState* st)
 template <typename T>
 int cg_run(const dfx_csr& A, const T* b, T* x, int max_it, double rtol, bool jacobi,
            int check_every, CGWork<T>& w, CGState* st, bool* converged, double* rr0,
-           double* rr)
+           double* rr, bool x_is_zero = false)
 {
   hipStream_t s = stream();
   const std::int64_t n = std::int64_t(A.nrows) * A.bs;
-  const int nbv = reduce_blocks(n);
+  const int nbv = partial_blocks(n);
   if (jacobi)
   {
     const int nb = static_cast<int>((n + kBlock - 1) / kBlock);
@@ -532,76 +584,63 @@ int cg_run(const dfx_csr& A, const T* b, T* x, int max_it, double rtol, bool jac
                                         static_cast<const T*>(A.values), w.dinv);
   }
   // q = A x (initial residual), then r, p and the first reductions
-  launch_spmv<T>(A, x, w.q, false);
+  if (x_is_zero)
+    fill_k<T><<<reduce_blocks(n), kBlock, 0, s>>>(w.q, n, T(0));
+  else
+    launch_spmv<T>(A, x, w.q, false);
   CGState init{};
   init.tol2 = rtol * rtol;
   init.max_it = max_it;
   DFX_HIP(hipMemcpyAsync(st, &init, offsetof(CGState, partials), hipMemcpyHostToDevice, s));
   cg_start_k<T><<<nbv, kBlock, 0, s>>>(n, b, w.q, jacobi ? w.dinv : nullptr, w.r, w.p, st);
+  cg_start_finish_k<<<1, kBlock, 0, s>>>(nbv, st);
   check_launch("cg_start");
   struct
   {
     std::int32_t iter, done;
   } h{0, 0};
   const int batch = std::max(1, check_every);
-  int queued = 0;
+  int k = 0;
   while (true)
   {
-    for (int k = 0; k < batch && queued < max_it; ++k, ++queued)
+    for (int j = 0; j < batch && k < max_it; ++j, ++k)
     {
-      launch_cg_spmv<T>(A, w.p, w.q, st);
-      cg_update_k<T><<<nbv, kBlock, 0, s>>>(n, x, w.r, w.p, w.q, jacobi ? w.dinv : nullptr, st);
-      cg_direction_k<T><<<nbv, kBlock, 0, s>>>(n, w.r, jacobi ? w.dinv : nullptr, w.p, st);
+      const int nbs = launch_cg_spmv<T>(A, w.p, w.q, st);
+      cg_update_k<T><<<nbv, kBlock, 0, s>>>(n, k, nbs, x, w.r, w.p, w.q,
+                                            jacobi ? w.dinv : nullptr, st);
+      cg_direction_k<T><<<nbv, kBlock, 0, s>>>(n, k, nbv, w.r, jacobi ? w.dinv : nullptr,
+                                               w.p, st);
     }
     check_launch("cg iteration");
     DFX_HIP(hipMemcpyAsync(&h, &st->iter, sizeof(h), hipMemcpyDeviceToHost, s));
     DFX_HIP(hipStreamSynchronize(s));
-    if (h.done || queued >= max_it)
+    if (h.done || k >= max_it)
       break;
   }
   CGState fin;
-  DFX_HIP(hipMemcpy(&fin, st, offsetof(CGState, ticket), hipMemcpyDeviceToHost));
+  DFX_HIP(hipMemcpy(&fin, st, offsetof(CGState, partials), hipMemcpyDeviceToHost));
   *rr0 = fin.rr0;
   *rr = fin.rr;
   *converged = fin.rr0 == 0.0 || fin.rr / fin.rr0 < fin.tol2;
   return fin.iter;
 }
 
-/// FP64 true residual norm: ||b - A x||_2, with y scratch of T.
-template <typename T>
-double true_residual(const dfx_csr& A, const T* b, const T* x, T* y, double* partials,
-                     unsigned* ticket, double* out)
-{
-  hipStream_t s = stream();
-  const std::int64_t n = std::int64_t(A.nrows) * A.bs;
-  launch_spmv<T>(A, x, y, false);
-  const int nb = reduce_blocks(n);
-  axpy_k<T><<<nb, kBlock, 0, s>>>(y, -1.0, y, b, n);
-  dot_k<T><<<nb, kBlock, 0, s>>>(y, y, n, partials, ticket, out);
-  check_launch("residual");
-  double h = 0;
-  d2h(&h, out, sizeof(double));
-  return std::sqrt(h);
-}
-
+/// Scratch for host-visible reductions.
 struct Reducer
 {
   double* partials = nullptr;
-  unsigned* ticket = nullptr;
   double* out = nullptr;
   Reducer()
   {
-    char* p = static_cast<char*>(dmalloc(8 * 2048 + 256));
-    partials = reinterpret_cast<double*>(p);
-    ticket = reinterpret_cast<unsigned*>(p + 8 * 2048);
-    out = reinterpret_cast<double*>(p + 8 * 2048 + 64);
-    DFX_HIP(hipMemsetAsync(p, 0, 8 * 2048 + 256, stream()));
+    partials = static_cast<double*>(dmalloc(8 * (kMaxPartials + 8)));
+    out = partials + kMaxPartials;
   }
-  ~Reducer() { dfree(partials); }
   template <typename T>
   double dot(const T* a, const T* b, std::int64_t n)
   {
-    dot_k<T><<<reduce_blocks(n), kBlock, 0, stream()>>>(a, b, n, partials, ticket, out);
+    const int nb = partial_blocks(n);
+    dot_k<T><<<nb, kBlock, 0, stream()>>>(a, b, n, partials);
+    finish_sums_k<<<1, kBlock, 0, stream()>>>(partials, nb, 1, out);
     check_launch("dot");
     double h = 0;
     d2h(&h, out, sizeof(double));
@@ -613,6 +652,18 @@ Reducer& reducer()
 {
   static Reducer* r = new Reducer();
   return *r;
+}
+
+/// FP64 true residual norm ||b - A x||_2, with y scratch of T.
+template <typename T>
+double true_residual(const dfx_csr& A, const T* b, const T* x, T* y)
+{
+  hipStream_t s = stream();
+  const std::int64_t n = std::int64_t(A.nrows) * A.bs;
+  launch_spmv<T>(A, x, y, false);
+  axpy_k<T><<<reduce_blocks(n), kBlock, 0, s>>>(y, -1.0, y, b, n);
+  check_launch("residual");
+  return std::sqrt(reducer().dot(y, y, n));
 }
 
 } // namespace dfx
@@ -644,7 +695,7 @@ void cg_solve_t(const dfx_csr& A, const dfx_vec& b, dfx_vec& x,
   Reducer& red = reducer();
   const double bn = std::sqrt(red.dot(static_cast<const T*>(b.data), static_cast<const T*>(b.data), n));
   const double tr = true_residual<T>(A, static_cast<const T*>(b.data), static_cast<const T*>(x.data),
-                                     w.q, red.partials, red.ticket, red.out);
+                                     w.q);
   res.true_rel_residual = bn > 0 ? tr / bn : tr;
 }
 
@@ -656,10 +707,10 @@ __global__ void __launch_bounds__(kBlock) add_correction_k(double* x, const floa
     x[i] += static_cast<double>(d[i]);
 }
 
-/// r = b - y (FP64), r32 = (float) r, reduce ||r||^2
+/// r = b - y (FP64), r32 = (float) r, partial sums of ||r||^2
 __global__ void __launch_bounds__(kBlock)
     residual_split_k(const double* b, const double* y, float* r32, std::int64_t n,
-                     double* partials, unsigned* ticket, double* out)
+                     double* partials)
 {
   double s[1] = {0.0};
   for (std::int64_t i = blockIdx.x * std::int64_t(kBlock) + threadIdx.x; i < n;
@@ -669,7 +720,49 @@ __global__ void __launch_bounds__(kBlock)
     r32[i] = static_cast<float>(ri);
     s[0] += ri * ri;
   }
-  grid_reduce<1>(s, partials, ticket, [out](double* t) { *out = t[0]; });
+  block_partials<1>(s, partials);
+}
+
+} // namespace dfx
+
+namespace dfx
+{
+template <typename T>
+int autotune_lanes(dfx_csr& A, int reps, float* best_ms)
+{
+  const std::int64_t nx = std::int64_t(A.ncols) * A.bs, ny = std::int64_t(A.nrows) * A.bs;
+  T* x = static_cast<T*>(dmalloc(sizeof(T) * std::max<std::int64_t>(nx, 1)));
+  T* y = static_cast<T*>(dmalloc(sizeof(T) * std::max<std::int64_t>(ny, 1)));
+  fill_k<T><<<reduce_blocks(nx), kBlock, 0, stream()>>>(x, nx, T(1));
+  hipEvent_t e0, e1;
+  DFX_HIP(hipEventCreate(&e0));
+  DFX_HIP(hipEventCreate(&e1));
+  int best = 0;
+  float bms = 1e30f;
+  for (int G = 2; G <= 64; G *= 2)
+  {
+    A.lanes = G;
+    launch_spmv<T>(A, x, y, false); // warm-up
+    DFX_HIP(hipEventRecord(e0, stream()));
+    for (int r = 0; r < reps; ++r)
+      launch_spmv<T>(A, x, y, false);
+    DFX_HIP(hipEventRecord(e1, stream()));
+    DFX_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    DFX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < bms)
+    {
+      bms = ms;
+      best = G;
+    }
+  }
+  A.lanes = best;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dfree(x);
+  dfree(y);
+  *best_ms = bms / reps;
+  return best;
 }
 
 } // namespace dfx
@@ -816,6 +909,7 @@ dfx_status dfx_csr_zero(dfx_csr* A)
         else
           fill_k<float><<<nb, kBlock, 0, stream()>>>(static_cast<float*>(A->values), n, 0.0f);
         check_launch("csr_zero");
+        A->zeroed = true;
       });
 }
 
@@ -836,6 +930,7 @@ dfx_status dfx_csr_set_values(dfx_csr* A, const void* values)
       {
         require(A && values, "null argument");
         h2d(A->values, values, dtype_size(A->dtype) * A->nnz * A->bs * A->bs);
+        A->zeroed = false;
       });
 }
 
@@ -970,6 +1065,40 @@ dfx_status dfx_spmv(const dfx_csr* A, const dfx_vec* x, dfx_vec* y)
       });
 }
 
+dfx_status dfx_csr_autotune(dfx_csr* A, int32_t* lanes, float* ms)
+{
+  return guard(
+      [&]
+      {
+        require(A, "null matrix");
+        float t = 0;
+        int g = 0;
+        if (A->bs != 1)
+          g = 0; // block rows: one lane per (row, component)
+        else if (A->dtype == DFX_F64)
+          g = autotune_lanes<double>(*A, 10, &t);
+        else
+          g = autotune_lanes<float>(*A, 10, &t);
+        if (lanes)
+          *lanes = g;
+        if (ms)
+          *ms = t;
+      });
+}
+
+dfx_status dfx_csr_set_lanes(dfx_csr* A, int32_t lanes)
+{
+  return guard(
+      [&]
+      {
+        require(A, "null matrix");
+        require(lanes == 0 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16
+                    || lanes == 32 || lanes == 64,
+                "lanes per row must be 0 or a power of two in [2, 64]");
+        A->lanes = lanes;
+      });
+}
+
 dfx_status dfx_inner_product(const dfx_vec* a, const dfx_vec* b, double* out)
 {
   return guard(
@@ -1066,7 +1195,9 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
         for (;;)
         {
           launch_spmv<double>(*A64, xd, y, false);
-          residual_split_k<<<nb, kBlock, 0, s>>>(bd, y, r32, n, red.partials, red.ticket, red.out);
+          const int nbr = partial_blocks(n);
+          residual_split_k<<<nbr, kBlock, 0, s>>>(bd, y, r32, n, red.partials);
+          finish_sums_k<<<1, kBlock, 0, s>>>(red.partials, nbr, 1, red.out);
           check_launch("residual");
           d2h(&rr, red.out, sizeof(double));
           if (rr <= tol2)
@@ -1080,7 +1211,7 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
           bool conv = false;
           double a0 = 0, a1 = 0;
           inner += cg_run<float>(*A32, r32, d32, o->max_inner, o->inner_rtol, true,
-                                 o->check_every, w, sh.st, &conv, &a0, &a1);
+                                 o->check_every, w, sh.st, &conv, &a0, &a1, true);
           add_correction_k<<<nb, kBlock, 0, s>>>(xd, d32, n);
           check_launch("correction");
           ++outer;

@@ -97,6 +97,16 @@ class MatrixCSR:
         _lib.call("dfx_csr_get_pattern", self._h, _lib.ptr(rp), _lib.ptr(ci))
         return rp, ci
 
+    def autotune(self):
+        """Pick the SpMV lanes per row for this matrix by timing (kept for
+        every later SpMV and CG on it). Returns (lanes, ms per SpMV)."""
+        lanes, ms = C.c_int32(), C.c_float()
+        _lib.call("dfx_csr_autotune", self._h, C.byref(lanes), C.byref(ms))
+        return lanes.value, ms.value
+
+    def set_lanes(self, lanes):
+        _lib.call("dfx_csr_set_lanes", self._h, int(lanes))
+
     def mult(self, x, y):
         """y += A x (MatrixCSR::mult, la/MatrixCSR.h:877-946)."""
         _lib.call("dfx_spmv", self._h, x.handle, y.handle)

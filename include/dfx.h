@@ -134,6 +134,13 @@ dfx_status dfx_csr_set_values(dfx_csr* A, const void* values);
 dfx_status dfx_csr_get_pattern(const dfx_csr* A, int64_t* row_ptr,
                                int32_t* cols);
 
+/* SpMV autotuner (no counterpart in the reference): times the scalar CSR
+ * kernel with 2..64 lanes per row on this matrix and keeps the fastest for
+ * every later SpMV/CG on it. Returns the choice and its time per SpMV. */
+dfx_status dfx_csr_autotune(dfx_csr* A, int32_t* lanes, float* ms);
+/* Fix the lanes per row (0 restores the built-in heuristic). */
+dfx_status dfx_csr_set_lanes(dfx_csr* A, int32_t lanes);
+
 /* la::Vector (la/Vector.h:47-422) with a device container. */
 dfx_status dfx_vec_create(int32_t n, int dtype, dfx_vec** out);
 dfx_status dfx_vec_destroy(dfx_vec* v);
