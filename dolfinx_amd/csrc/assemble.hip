@@ -717,7 +717,7 @@ __global__ void __launch_bounds__(kBlock)
   islots[t] = static_cast<O>(lo - r0);
 }
 
-template <typename T, typename O, int K>
+template <typename T, typename O, int K, int L>
 __global__ void __launch_bounds__(kBlock)
     row_assemble_k(std::int32_t nrows, std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
                    const std::int32_t* __restrict__ inc, const O* __restrict__ islots,
@@ -725,77 +725,63 @@ __global__ void __launch_bounds__(kBlock)
                    const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
                    const T* __restrict__ ref, T* __restrict__ vals, int accumulate)
 {
+  // L lanes per row, one incidence per lane: every lane issues the loads
+  // of its own incidence at once, and the lanes of a row add their
+  // entries into the row's LDS slice with LDS atomics.
   extern __shared__ unsigned char smem_raw[];
   T* S = reinterpret_cast<T*>(smem_raw); // [7][K][K]
   T* acc = S + 7 * K * K;
-  const int R = blockDim.x;
-  const std::int32_t r0 = blockIdx.x * R;
-  const std::int32_t r1 = min(r0 + R, nrows);
+  constexpr int RPB = kBlock / L;
+  const std::int32_t r0 = blockIdx.x * RPB;
+  const std::int32_t r1 = min(r0 + RPB, nrows);
   const std::int32_t base = rp[r0];
   const std::int32_t len = rp[r1] - base;
-  for (int t = threadIdx.x; t < 7 * K * K; t += R)
+  for (int t = threadIdx.x; t < 7 * K * K; t += kBlock)
     S[t] = ref[t];
-  for (int t = threadIdx.x; t < len; t += R)
+  for (int t = threadIdx.x; t < len; t += kBlock)
     acc[t] = T(0);
   __syncthreads();
-  const std::int32_t r = r0 + threadIdx.x;
+  const std::int32_t r = r0 + threadIdx.x / L;
+  const int lane = threadIdx.x % L;
   if (r < r1 && r < nplan && !(bc && bc[r]))
   {
     T* a = acc + (rp[r] - base);
     const std::int32_t pe = inc_off[r + 1];
-    // Incidences in groups of four with every global load of a group
-    // issued before any use, to keep several loads in flight per thread.
-    for (std::int32_t pb = inc_off[r]; pb < pe; pb += 4)
+    for (std::int32_t p = inc_off[r] + lane; p < pe; p += L)
     {
-      std::int32_t cik[4];
+      const std::int32_t cik = inc[p];
+      const std::int64_t c = cik / K;
+      const int i = cik - static_cast<int>(c) * K;
+      const T* G = cd + 8 * c;
+      T g[7];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        cik[u] = pb + u < pe ? inc[pb + u] : -1;
-      T g[4][7];
-      std::uint32_t m[4];
-      O sl[4][K];
+      for (int q = 0; q < 7; ++q)
+        g[q] = G[q];
+      const std::uint32_t m = mask ? mask[c] : 0u;
+      O sl[K];
+      const O* src = islots + std::int64_t(p) * K;
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+      for (int j = 0; j < K; ++j)
+        sl[j] = src[j];
+      const T* Si = S + i * K;
+#pragma unroll
+      for (int j = 0; j < K; ++j)
       {
-        if (cik[u] < 0)
+        if ((m >> j) & 1u)
           continue;
-        const std::int64_t c = cik[u] / K;
-        const T* G = cd + 8 * c;
-#pragma unroll
-        for (int q = 0; q < 7; ++q)
-          g[u][q] = G[q];
-        m[u] = mask ? mask[c] : 0u;
-        const O* src = islots + std::int64_t(pb + u) * K;
-#pragma unroll
-        for (int j = 0; j < K; ++j)
-          sl[u][j] = src[j];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-      {
-        if (cik[u] < 0)
-          continue;
-        const int i = cik[u] % K;
-        const T* Si = S + i * K;
-#pragma unroll
-        for (int j = 0; j < K; ++j)
-        {
-          if ((m[u] >> j) & 1u)
-            continue;
-          const T v = g[u][0] * Si[j] + g[u][1] * Si[K * K + j] + g[u][2] * Si[2 * K * K + j]
-                      + g[u][3] * Si[3 * K * K + j] + g[u][4] * Si[4 * K * K + j]
-                      + g[u][5] * Si[5 * K * K + j] + g[u][6] * Si[6 * K * K + j];
-          a[sl[u][j]] += v;
-        }
+        const T v = g[0] * Si[j] + g[1] * Si[K * K + j] + g[2] * Si[2 * K * K + j]
+                    + g[3] * Si[3 * K * K + j] + g[4] * Si[4 * K * K + j]
+                    + g[5] * Si[5 * K * K + j] + g[6] * Si[6 * K * K + j];
+        atomicAdd(a + sl[j], v);
       }
     }
   }
   __syncthreads();
   if (accumulate)
-    for (int t = threadIdx.x; t < len; t += R)
+    for (int t = threadIdx.x; t < len; t += kBlock)
       vals[base + t] += acc[t];
   else
-    for (int t = threadIdx.x; t < len; t += R)
+    for (int t = threadIdx.x; t < len; t += kBlock)
       vals[base + t] = acc[t];
 }
 
@@ -1061,31 +1047,50 @@ void launch_cell_factors(const dfx_form& f, const dfx_bc* bc)
   check_launch("cell_factors");
 }
 
-/// Rows per block of the row kernel: LDS holds the block's CSR slice.
-int row_block(const dfx_csr& A, int k, std::size_t tsize)
+/// Lanes per row of the row kernel: the power of two covering the most
+/// incidences of one row (8 for Q1 hexahedra, 32 for P1 tetrahedra).
+int incidence_lanes(const dfx_form& f)
 {
-  const std::size_t budget = 64 * 1024 - 7 * k * k * tsize;
-  int R = 256;
-  while (R > 64 && std::size_t(R) * A.max_row * tsize > budget)
-    R /= 2;
-  return R;
+  int L = 8;
+  while (L < 64 && L < f.max_inc)
+    L *= 2;
+  return L;
+}
+
+template <typename T, typename O, int K, int L>
+void launch_row_assemble_l(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
+{
+  constexpr int RPB = kBlock / L;
+  const std::size_t lds = (7 * K * K + std::size_t(RPB) * A.max_row) * sizeof(T);
+  require(lds <= 64 * 1024, "row too long for the LDS slice");
+  const int nb = (A.nrows + RPB - 1) / RPB;
+  if (nb > 0)
+    row_assemble_k<T, O, K, L><<<nb, kBlock, lds, stream()>>>(
+        A.nrows, f.max_dof + 1, f.inc_off, f.inc, static_cast<const O*>(f.islots),
+        static_cast<const T*>(f.cell_data), bc ? f.cell_mask : nullptr, bc ? bc->marker : nullptr,
+        A.row_ptr, static_cast<const T*>(f.ref_tensors), static_cast<T*>(A.values),
+        A.zeroed ? 0 : 1);
+  check_launch("row_assemble");
 }
 
 template <typename T, typename O, int K>
 void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
 {
   launch_cell_factors<T>(f, bc);
-  const int R = row_block(A, K, sizeof(T));
-  const std::size_t lds = (7 * K * K + std::size_t(R) * A.max_row) * sizeof(T);
-  require(lds <= 160 * 1024, "row too long for the LDS slice");
-  const int nb = (A.nrows + R - 1) / R;
-  if (nb > 0)
-    row_assemble_k<T, O, K><<<nb, R, lds, stream()>>>(
-        A.nrows, f.max_dof + 1, f.inc_off, f.inc, static_cast<const O*>(f.islots),
-        static_cast<const T*>(f.cell_data), bc ? f.cell_mask : nullptr, bc ? bc->marker : nullptr,
-        A.row_ptr, static_cast<const T*>(f.ref_tensors), static_cast<T*>(A.values),
-        A.zeroed ? 0 : 1);
-  check_launch("row_assemble");
+  switch (incidence_lanes(f))
+  {
+  case 8:
+    launch_row_assemble_l<T, O, K, 8>(f, A, bc);
+    break;
+  case 16:
+    launch_row_assemble_l<T, O, K, 16>(f, A, bc);
+    break;
+  case 32:
+    launch_row_assemble_l<T, O, K, 32>(f, A, bc);
+    break;
+  default:
+    launch_row_assemble_l<T, O, K, 64>(f, A, bc);
+  }
 }
 
 template <typename T>

@@ -136,7 +136,11 @@ struct dfx_csr
   void* values = nullptr;          // device, nnz*bs*bs of dtype
   std::uint64_t pattern_id = 0;    // identifies the pattern for slot caches
   bool zeroed = true;              // values known to be all zero
-  int lanes = 0;                   // lanes per row of SpMV (0: heuristic)
+  int lanes = 0;                   // SpMV variant: 0 default, 1 stream, G>1 vector
+  // Row blocks of the stream SpMV: consecutive rows whose entries fit one
+  // LDS chunk (or a single longer row); device, nblk + 1 row boundaries.
+  std::int32_t* rblk = nullptr;
+  std::int32_t nblk = 0;
 };
 
 struct dfx_vec

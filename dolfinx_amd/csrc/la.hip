@@ -170,34 +170,88 @@ __global__ void __launch_bounds__(kBlock)
   diag[r] = (lo < rp[r + 1] && ci[lo] == r) ? lo : -1;
 }
 
-/// Partial dot product of one lane's share of a CSR row [b, e): the lane
-/// takes entries b + lane, b + lane + G, ... The first four are loaded
-/// together (values and columns streamed with nontemporal loads) before
-/// any x gather, so each lane keeps several loads in flight.
-template <typename T, int G>
-__device__ inline T row_share(std::int32_t b, std::int32_t e, int lane,
-                              const std::int32_t* __restrict__ ci, const T* __restrict__ v,
-                              const T* __restrict__ x)
+/// Entries per LDS chunk of the stream SpMV (16 KB of products).
+template <typename T>
+constexpr int stream_chunk()
 {
-  constexpr int U = 4;
-  const std::int32_t j0 = b + lane;
-  T vv[U];
-  std::int32_t cc[U];
-#pragma unroll
-  for (int u = 0; u < U; ++u)
+  return 16384 / static_cast<int>(sizeof(T));
+}
+
+/// Stream CSR SpMV, y (+)= A x. Each row block (consecutive rows whose
+/// entries fit one chunk) is processed by one workgroup: the products
+/// v_j x[c_j] of the block's contiguous entry range are formed with fully
+/// coalesced loads (eight per thread in flight) into LDS, then one thread
+/// per row sums its products left to right, the order of the reference's
+/// row-serial loop. A block holding a single longer row is reduced with a
+/// strided sum. With dotv, partial sums of y.dotv go to partials.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    spmv_stream_k(std::int32_t nblk, const std::int32_t* __restrict__ rblk,
+                  const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
+                  const T* __restrict__ v, const T* __restrict__ x, T* __restrict__ y,
+                  int accumulate, const T* __restrict__ dotv, double* partials,
+                  const int* stop)
+{
+  if (stop && block_flag(stop))
+    return;
+  constexpr int NC = stream_chunk<T>();
+  constexpr int U = NC / kBlock;
+  __shared__ T prod[NC];
+  __shared__ double red[4];
+  double d[1] = {0.0};
+  for (int bi = xcd_contiguous(blockIdx.x, gridDim.x); bi < nblk; bi += gridDim.x)
   {
-    const std::int32_t j = j0 + u * G;
-    vv[u] = j < e ? __builtin_nontemporal_load(v + j) : T(0);
-    cc[u] = j < e ? __builtin_nontemporal_load(ci + j) : 0;
-  }
-  T s = 0;
+    const std::int32_t r0 = rblk[bi], r1 = rblk[bi + 1];
+    const std::int32_t j0 = rp[r0], j1 = rp[r1];
+    if (j1 - j0 <= NC)
+    {
+      T vv[U];
+      std::int32_t cc[U];
 #pragma unroll
-  for (int u = 0; u < U; ++u)
-    if (j0 + u * G < e)
-      s += vv[u] * x[cc[u]];
-  for (std::int32_t j = j0 + U * G; j < e; j += G)
-    s += v[j] * x[ci[j]];
-  return s;
+      for (int u = 0; u < U; ++u)
+      {
+        const std::int32_t j = j0 + threadIdx.x + u * kBlock;
+        vv[u] = j < j1 ? v[j] : T(0);
+        cc[u] = j < j1 ? ci[j] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+      {
+        const std::int32_t j = j0 + threadIdx.x + u * kBlock;
+        if (j < j1)
+          prod[j - j0] = vv[u] * x[cc[u]];
+      }
+      __syncthreads();
+      for (std::int32_t r = r0 + threadIdx.x; r < r1; r += kBlock)
+      {
+        T s = 0;
+        const std::int32_t e = rp[r + 1] - j0;
+        for (std::int32_t k = rp[r] - j0; k < e; ++k)
+          s += prod[k];
+        const T yr = accumulate ? y[r] + s : s;
+        y[r] = yr;
+        if (dotv)
+          d[0] += double(yr) * double(dotv[r]);
+      }
+      __syncthreads();
+    }
+    else
+    {
+      double s = 0;
+      for (std::int32_t j = j0 + threadIdx.x; j < j1; j += kBlock)
+        s += double(v[j] * x[ci[j]]);
+      const double t = block_sum_256(s, red);
+      if (threadIdx.x == 0)
+      {
+        const T yr = accumulate ? y[r0] + static_cast<T>(t) : static_cast<T>(t);
+        y[r0] = yr;
+        if (dotv)
+          d[0] += double(yr) * double(dotv[r0]);
+      }
+    }
+  }
+  if (partials)
+    block_partials<1>(d, partials);
 }
 
 /// y (+)= A x for scalar CSR: G lanes per row, each lane strides the row,
@@ -218,7 +272,11 @@ __global__ void __launch_bounds__(kBlock)
     const std::int32_t row = chunk * RPB + threadIdx.x / G;
     T s = 0;
     if (row < nrows)
-      s = row_share<T, G>(rp[row], rp[row + 1], lane, ci, v, x);
+    {
+      const std::int32_t e = rp[row + 1];
+      for (std::int32_t j = rp[row] + lane; j < e; j += G)
+        s += v[j] * x[ci[j]];
+    }
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1)
       s += __shfl_down(s, off, G);
@@ -257,6 +315,8 @@ int lanes_per_row(const dfx_csr& A)
 {
   if (A.lanes)
     return A.lanes;
+  if (A.nblk > 0)
+    return 1; // stream kernel
   const double mean = A.nrows ? double(A.nnz) / A.nrows : 1.0;
   int G = 2;
   while (G < 64 && G * 6 < mean)
@@ -268,7 +328,14 @@ template <typename T>
 void launch_spmv(const dfx_csr& A, const T* x, T* y, bool accumulate)
 {
   hipStream_t s = stream();
-  if (A.bs == 1)
+  if (A.bs == 1 && lanes_per_row(A) == 1)
+  {
+    const int nb = std::max(1, std::min(A.nblk, ctx().num_cu * 8));
+    spmv_stream_k<T><<<nb, kBlock, 0, s>>>(A.nblk, A.rblk, A.row_ptr, A.cols,
+                                           static_cast<const T*>(A.values), x, y,
+                                           accumulate, nullptr, nullptr, nullptr);
+  }
+  else if (A.bs == 1)
   {
     const int G = lanes_per_row(A);
     const int rpb = kBlock / G;
@@ -319,7 +386,7 @@ void launch_spmv(const dfx_csr& A, const T* x, T* y, bool accumulate)
 // stop test fires every later kernel returns at once.
 // ---------------------------------------------------------------------
 
-constexpr int kMaxPartials = 1024;
+constexpr int kMaxPartials = 2048;
 
 struct CGState
 {
@@ -397,7 +464,11 @@ __global__ void __launch_bounds__(kBlock)
     const std::int32_t row = chunk * RPB + threadIdx.x / G;
     T s = 0;
     if (row < nrows)
-      s = row_share<T, G>(rp[row], rp[row + 1], lane, ci, v, p);
+    {
+      const std::int32_t e = rp[row + 1];
+      for (std::int32_t j = rp[row] + lane; j < e; j += G)
+        s += v[j] * p[ci[j]];
+    }
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1)
       s += __shfl_down(s, off, G);
@@ -496,7 +567,7 @@ __global__ void __launch_bounds__(kBlock)
 int partial_blocks(std::int64_t n)
 {
   std::int64_t b = (n + kBlock * 8 - 1) / (kBlock * 8);
-  return static_cast<int>(std::clamp<std::int64_t>(b, 1, kMaxPartials));
+  return static_cast<int>(std::clamp<std::int64_t>(b, 1, 1024));
 }
 
 /// Per-dtype scratch of a CG solve (device).
@@ -535,6 +606,13 @@ int launch_cg_spmv(const dfx_csr& A, const T* p, T* q, CGState* st)
 {
   hipStream_t s = stream();
   const T* v = static_cast<const T*>(A.values);
+  if (A.bs == 1 && lanes_per_row(A) == 1)
+  {
+    const int nb = std::max(1, std::min({A.nblk, ctx().num_cu * 8, kMaxPartials}));
+    spmv_stream_k<T><<<nb, kBlock, 0, s>>>(A.nblk, A.rblk, A.row_ptr, A.cols, v, p, q, 0, p,
+                                           st->partials, &st->done);
+    return nb;
+  }
   if (A.bs == 1)
   {
     const int G = lanes_per_row(A);
@@ -739,8 +817,10 @@ int autotune_lanes(dfx_csr& A, int reps, float* best_ms)
   DFX_HIP(hipEventCreate(&e1));
   int best = 0;
   float bms = 1e30f;
-  for (int G = 2; G <= 64; G *= 2)
+  for (int G = 1; G <= 64; G *= 2)
   {
+    if (G == 1 && A.nblk == 0)
+      continue;
     A.lanes = G;
     launch_spmv<T>(A, x, y, false); // warm-up
     DFX_HIP(hipEventRecord(e0, stream()));
@@ -854,6 +934,25 @@ dfx_status dfx_csr_create(const int64_t* row_ptr, const int32_t* cols, int32_t n
         const std::size_t vb = dtype_size(dtype) * std::size_t(nnz) * bs * bs;
         A->values = dmalloc(std::max<std::size_t>(vb, 8));
         h2d(A->row_ptr, rp32.data(), sizeof(std::int32_t) * (nrows + 1));
+        if (bs == 1)
+        {
+          // row blocks of the stream SpMV: greedy, <= 256 rows and entries
+          // fitting the LDS chunk of either precision
+          const std::int32_t nc_max = stream_chunk<double>();
+          std::vector<std::int32_t> rb{0};
+          std::int32_t r = 0;
+          while (r < nrows)
+          {
+            std::int32_t e = r + 1;
+            while (e < nrows && e - r < 256 && rp32[e + 1] - rp32[r] <= nc_max)
+              ++e;
+            rb.push_back(e);
+            r = e;
+          }
+          A->nblk = static_cast<std::int32_t>(rb.size()) - 1;
+          A->rblk = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * rb.size()));
+          h2d(A->rblk, rb.data(), sizeof(std::int32_t) * rb.size());
+        }
         h2d(A->cols, cols, sizeof(std::int32_t) * nnz);
         DFX_HIP(hipMemsetAsync(A->values, 0, vb, stream()));
         if (nrows > 0)
@@ -870,6 +969,7 @@ dfx_status dfx_csr_destroy(dfx_csr* A)
   if (A)
   {
     dfree(A->row_ptr);
+    dfree(A->rblk);
     dfree(A->cols);
     dfree(A->diag);
     dfree(A->values);
@@ -1092,9 +1192,9 @@ dfx_status dfx_csr_set_lanes(dfx_csr* A, int32_t lanes)
       [&]
       {
         require(A, "null matrix");
-        require(lanes == 0 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16
-                    || lanes == 32 || lanes == 64,
-                "lanes per row must be 0 or a power of two in [2, 64]");
+        require(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8
+                    || lanes == 16 || lanes == 32 || lanes == 64,
+                "lanes must be 0 (default), 1 (stream) or a power of two in [2, 64]");
         A->lanes = lanes;
       });
 }
