@@ -134,6 +134,9 @@ def main():
     Nv = p64.mesh.geometry.x.shape[0]
     print(f"# setup {setup_s:.1f} s: cells {C} dofs {N} nnz {nnz}", file=sys.stderr, flush=True)
 
+    variant64 = p64.a.autotune(A64, p64.bc)
+    variant32 = p32.a.autotune(A32, p32.bc)
+    print(f"# assembly variants: f64 {variant64} f32 {variant32}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         problems.assemble_system(p64, A64, b64)
     sync()
@@ -212,6 +215,9 @@ def main():
         rm = solvers.mixed_refinement(A64, A32, b64, xm, rtol=1e-8, inner_rtol=1e-4,
                                       check_every=64)
         xmix = xm.array
+        xr_ = la.Vector(N)
+        rr_ = solvers.mixed_refinement(A64, A32, b64, xr_, rtol=1e-8, inner_rtol=1e-4,
+                                       check_every=64, restart=True)
         # reference-accuracy solution for the error figures
         xt = la.Vector(N)
         solvers.cg(A64, b64, xt, rtol=1e-11, max_it=40000, jacobi=True, check_every=128)
@@ -226,6 +232,8 @@ def main():
             "mixed_inner_iterations": rm.inner_total,
             "mixed_rel_residual": rm.true_rel_residual,
             "mixed_speedup_vs_fp64": round(r64.seconds / rm.seconds, 3),
+            "mixed_restarted_s": round(rr_.seconds, 4),
+            "mixed_restarted_inner_iterations": rr_.inner_total,
             "err_fp64_vs_tight": float(np.linalg.norm(x64 - xref) / nref),
             "err_mixed_vs_tight": float(np.linalg.norm(xmix - xref) / nref),
             "err_mixed_vs_fp64": float(np.linalg.norm(xmix - x64) / np.linalg.norm(x64)),
@@ -267,6 +275,7 @@ def main():
             "kernel_ms": round(asm_ms, 4),
         },
         "assembly_f32_gnnz_s": round(nnz / asm32_s / 1e9, 4),
+        "assembly_variant": {"f64": variant64[0], "f32": variant32[0]},
         "spmv": {
             "lanes_per_row": {"f64": lanes64, "f32": lanes32},
             "f64_ms": round(spmv_ms, 4), "f64_gbs": round(spmv_gbs, 1),

@@ -45,7 +45,7 @@ class CGResult(C.Structure):
 
 class MixedOpts(C.Structure):
     _fields_ = [("max_outer", I32), ("max_inner", I32), ("rtol", D), ("inner_rtol", D),
-                ("check_every", I32)]
+                ("check_every", I32), ("restart", I32)]
 
 
 class MixedResult(C.Structure):
@@ -100,6 +100,8 @@ SIGNATURES = {
     "dfx_bc_destroy": (C.c_int, [P]),
     "dfx_assemble_matrix": (C.c_int, [P, P, P]),
     "dfx_set_diagonal": (C.c_int, [P, P, D]),
+    "dfx_form_autotune": (C.c_int, [P, P, P, P, P]),
+    "dfx_form_set_variant": (C.c_int, [P, I32]),
     "dfx_assemble_vector": (C.c_int, [P, P]),
     "dfx_apply_lifting": (C.c_int, [P, P, P, P, D]),
     "dfx_bc_set": (C.c_int, [P, P, P, D]),

@@ -641,7 +641,8 @@ __global__ void __launch_bounds__(kBlock)
                    int cstride, T creac, const std::int8_t* __restrict__ bc, T* __restrict__ cd,
                    std::uint32_t* __restrict__ mask)
 {
-  const std::int64_t c = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  const std::int64_t c
+      = xcd_contiguous(blockIdx.x, gridDim.x) * std::int64_t(kBlock) + threadIdx.x;
   if (c >= nc)
     return;
   const std::int32_t* v = xdm + NV * c;
@@ -717,7 +718,21 @@ __global__ void __launch_bounds__(kBlock)
   islots[t] = static_cast<O>(lo - r0);
 }
 
-template <typename T, typename O, int K, int L>
+/// Entry (i, j) of the element matrix from the cell factors g and the
+/// reference tensors ref[7][K][K].
+template <typename T, int K>
+__device__ inline T element_entry(const T (&g)[7], const T* __restrict__ ref, int i, int j)
+{
+  const T* Si = ref + i * K + j;
+  return g[0] * Si[0] + g[1] * Si[K * K] + g[2] * Si[2 * K * K] + g[3] * Si[3 * K * K]
+         + g[4] * Si[4 * K * K] + g[5] * Si[5 * K * K] + g[6] * Si[6 * K * K];
+}
+
+/// Variant "rows": one thread per matrix row, R rows per workgroup. The
+/// thread walks the row's incidences four at a time (all global loads of a
+/// group issued before use) and accumulates into the workgroup's LDS copy
+/// of its CSR slice, which is then stored with coalesced writes.
+template <typename T, typename O, int K>
 __global__ void __launch_bounds__(kBlock)
     row_assemble_k(std::int32_t nrows, std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
                    const std::int32_t* __restrict__ inc, const O* __restrict__ islots,
@@ -725,19 +740,87 @@ __global__ void __launch_bounds__(kBlock)
                    const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
                    const T* __restrict__ ref, T* __restrict__ vals, int accumulate)
 {
-  // L lanes per row, one incidence per lane: every lane issues the loads
-  // of its own incidence at once, and the lanes of a row add their
-  // entries into the row's LDS slice with LDS atomics.
   extern __shared__ unsigned char smem_raw[];
-  T* S = reinterpret_cast<T*>(smem_raw); // [7][K][K]
-  T* acc = S + 7 * K * K;
+  T* acc = reinterpret_cast<T*>(smem_raw);
+  const int R = blockDim.x;
+  const std::int32_t r0 = xcd_contiguous(blockIdx.x, gridDim.x) * R;
+  const std::int32_t r1 = min(r0 + R, nrows);
+  const std::int32_t base = rp[r0];
+  const std::int32_t len = rp[r1] - base;
+  for (int t = threadIdx.x; t < len; t += R)
+    acc[t] = T(0);
+  __syncthreads();
+  const std::int32_t r = r0 + threadIdx.x;
+  if (r < r1 && r < nplan && !(bc && bc[r]))
+  {
+    T* a = acc + (rp[r] - base);
+    const std::int32_t pe = inc_off[r + 1];
+    for (std::int32_t pb = inc_off[r]; pb < pe; pb += 4)
+    {
+      std::int32_t cik[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        cik[u] = pb + u < pe ? inc[pb + u] : -1;
+      T g[4][7];
+      std::uint32_t m[4];
+      O sl[4][K];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+      {
+        if (cik[u] < 0)
+          continue;
+        const std::int64_t c = cik[u] / K;
+        const T* G = cd + 8 * c;
+#pragma unroll
+        for (int q = 0; q < 7; ++q)
+          g[u][q] = G[q];
+        m[u] = mask ? mask[c] : 0u;
+        const O* src = islots + std::int64_t(pb + u) * K;
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+          sl[u][j] = src[j];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+      {
+        if (cik[u] < 0)
+          continue;
+        const int i = cik[u] % K;
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+          if (!((m[u] >> j) & 1u))
+            a[sl[u][j]] += element_entry<T, K>(g[u], ref, i, j);
+      }
+    }
+  }
+  __syncthreads();
+  if (accumulate)
+    for (int t = threadIdx.x; t < len; t += R)
+      vals[base + t] += acc[t];
+  else
+    for (int t = threadIdx.x; t < len; t += R)
+      vals[base + t] = acc[t];
+}
+
+/// Variant "lanes": L lanes per row, one incidence per lane, so every lane
+/// has a single short chain of loads; the lanes of a row add their entries
+/// into the row's LDS slice with LDS atomics.
+template <typename T, typename O, int K, int L>
+__global__ void __launch_bounds__(kBlock)
+    row_assemble_lanes_k(std::int32_t nrows, std::int32_t nplan,
+                         const std::int32_t* __restrict__ inc_off,
+                         const std::int32_t* __restrict__ inc, const O* __restrict__ islots,
+                         const T* __restrict__ cd, const std::uint32_t* __restrict__ mask,
+                         const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
+                         const T* __restrict__ ref, T* __restrict__ vals, int accumulate)
+{
+  extern __shared__ unsigned char smem_raw[];
+  T* acc = reinterpret_cast<T*>(smem_raw);
   constexpr int RPB = kBlock / L;
-  const std::int32_t r0 = blockIdx.x * RPB;
+  const std::int32_t r0 = xcd_contiguous(blockIdx.x, gridDim.x) * RPB;
   const std::int32_t r1 = min(r0 + RPB, nrows);
   const std::int32_t base = rp[r0];
   const std::int32_t len = rp[r1] - base;
-  for (int t = threadIdx.x; t < 7 * K * K; t += kBlock)
-    S[t] = ref[t];
   for (int t = threadIdx.x; t < len; t += kBlock)
     acc[t] = T(0);
   __syncthreads();
@@ -763,17 +846,10 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
       for (int j = 0; j < K; ++j)
         sl[j] = src[j];
-      const T* Si = S + i * K;
 #pragma unroll
       for (int j = 0; j < K; ++j)
-      {
-        if ((m >> j) & 1u)
-          continue;
-        const T v = g[0] * Si[j] + g[1] * Si[K * K + j] + g[2] * Si[2 * K * K + j]
-                    + g[3] * Si[3 * K * K + j] + g[4] * Si[4 * K * K + j]
-                    + g[5] * Si[5 * K * K + j] + g[6] * Si[6 * K * K + j];
-        atomicAdd(a + sl[j], v);
-      }
+        if (!((m >> j) & 1u))
+          atomicAdd(a + sl[j], element_entry<T, K>(g, ref, i, j));
     }
   }
   __syncthreads();
@@ -1047,8 +1123,8 @@ void launch_cell_factors(const dfx_form& f, const dfx_bc* bc)
   check_launch("cell_factors");
 }
 
-/// Lanes per row of the row kernel: the power of two covering the most
-/// incidences of one row (8 for Q1 hexahedra, 32 for P1 tetrahedra).
+/// Lanes per row of the "lanes" variant: the power of two covering the
+/// most incidences of one row (8 for Q1 hexahedra, 32 for P1 tetrahedra).
 int incidence_lanes(const dfx_form& f)
 {
   int L = 8;
@@ -1057,40 +1133,61 @@ int incidence_lanes(const dfx_form& f)
   return L;
 }
 
+/// Row-kernel variants, picked per form by dfx_form_autotune:
+/// 0..2: "rows" with 64/128/256 rows per workgroup; 3: "lanes".
+constexpr int kRowVariants = 4;
+
 template <typename T, typename O, int K, int L>
-void launch_row_assemble_l(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
+void launch_lanes(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
 {
   constexpr int RPB = kBlock / L;
-  const std::size_t lds = (7 * K * K + std::size_t(RPB) * A.max_row) * sizeof(T);
+  const std::size_t lds = std::size_t(RPB) * A.max_row * sizeof(T);
   require(lds <= 64 * 1024, "row too long for the LDS slice");
   const int nb = (A.nrows + RPB - 1) / RPB;
   if (nb > 0)
-    row_assemble_k<T, O, K, L><<<nb, kBlock, lds, stream()>>>(
+    row_assemble_lanes_k<T, O, K, L><<<nb, kBlock, lds, stream()>>>(
         A.nrows, f.max_dof + 1, f.inc_off, f.inc, static_cast<const O*>(f.islots),
         static_cast<const T*>(f.cell_data), bc ? f.cell_mask : nullptr, bc ? bc->marker : nullptr,
         A.row_ptr, static_cast<const T*>(f.ref_tensors), static_cast<T*>(A.values),
         A.zeroed ? 0 : 1);
-  check_launch("row_assemble");
 }
 
 template <typename T, typename O, int K>
 void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
 {
   launch_cell_factors<T>(f, bc);
-  switch (incidence_lanes(f))
+  const int variant = f.row_variant;
+  if (variant < 3)
   {
-  case 8:
-    launch_row_assemble_l<T, O, K, 8>(f, A, bc);
-    break;
-  case 16:
-    launch_row_assemble_l<T, O, K, 16>(f, A, bc);
-    break;
-  case 32:
-    launch_row_assemble_l<T, O, K, 32>(f, A, bc);
-    break;
-  default:
-    launch_row_assemble_l<T, O, K, 64>(f, A, bc);
+    const int R = 64 << variant;
+    const std::size_t lds = std::size_t(R) * A.max_row * sizeof(T);
+    require(lds <= 64 * 1024, "row too long for the LDS slice");
+    const int nb = (A.nrows + R - 1) / R;
+    if (nb > 0)
+      row_assemble_k<T, O, K><<<nb, R, lds, stream()>>>(
+          A.nrows, f.max_dof + 1, f.inc_off, f.inc, static_cast<const O*>(f.islots),
+          static_cast<const T*>(f.cell_data), bc ? f.cell_mask : nullptr,
+          bc ? bc->marker : nullptr, A.row_ptr, static_cast<const T*>(f.ref_tensors),
+          static_cast<T*>(A.values), A.zeroed ? 0 : 1);
   }
+  else
+  {
+    switch (incidence_lanes(f))
+    {
+    case 8:
+      launch_lanes<T, O, K, 8>(f, A, bc);
+      break;
+    case 16:
+      launch_lanes<T, O, K, 16>(f, A, bc);
+      break;
+    case 32:
+      launch_lanes<T, O, K, 32>(f, A, bc);
+      break;
+    default:
+      launch_lanes<T, O, K, 64>(f, A, bc);
+    }
+  }
+  check_launch("row_assemble");
 }
 
 template <typename T>
@@ -1472,6 +1569,84 @@ dfx_status dfx_assemble_matrix(dfx_csr* A, const dfx_form* a, const dfx_bc* bc)
           else
             launch_bilinear<float, std::uint16_t, 0>(f, A, bc, nullptr, 0.0, nullptr);
         }
+      });
+}
+
+dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t* variant,
+                             float* ms)
+{
+  return guard(
+      [&]
+      {
+        require(a && A, "null argument");
+        KernelInfo ki;
+        kernel_info(a->d.kernel, ki);
+        require(ki.bilinear && A->dtype == a->d.dtype && A->bs == a->d.bs,
+                "form and matrix do not match");
+        require(a->max_dof < A->nrows, "dofmap exceeds the matrix size");
+        dfx_form& f = *a;
+        int best = f.row_variant;
+        float bms = -1;
+        if (row_path(f))
+        {
+          hipEvent_t e0, e1;
+          DFX_HIP(hipEventCreate(&e0));
+          DFX_HIP(hipEventCreate(&e1));
+          for (int v = 0; v < kRowVariants; ++v)
+          {
+            f.row_variant = v;
+            float t = 0;
+            try
+            {
+              for (int rep = 0; rep < 4; ++rep)
+              {
+                if (rep == 1)
+                  DFX_HIP(hipEventRecord(e0, stream()));
+                if (A->dtype == DFX_F64)
+                  assemble_rows<double>(f, *A, bc);
+                else
+                  assemble_rows<float>(f, *A, bc);
+                A->zeroed = false;
+              }
+              DFX_HIP(hipEventRecord(e1, stream()));
+              DFX_HIP(hipEventSynchronize(e1));
+              DFX_HIP(hipEventElapsedTime(&t, e0, e1));
+              t /= 3;
+            }
+            catch (const Error&)
+            {
+              continue; // variant not applicable (LDS slice too large)
+            }
+            if (bms < 0 || t < bms)
+            {
+              bms = t;
+              best = v;
+            }
+          }
+          (void)hipEventDestroy(e0);
+          (void)hipEventDestroy(e1);
+        }
+        f.row_variant = best;
+        // leave A zeroed
+        const std::int64_t n = A->nnz * A->bs * A->bs;
+        DFX_HIP(hipMemsetAsync(A->values, 0, dtype_size(A->dtype) * n, stream()));
+        DFX_HIP(hipStreamSynchronize(stream()));
+        A->zeroed = true;
+        if (variant)
+          *variant = best;
+        if (ms)
+          *ms = bms;
+      });
+}
+
+dfx_status dfx_form_set_variant(dfx_form* a, int32_t variant)
+{
+  return guard(
+      [&]
+      {
+        require(a, "null form");
+        require(variant >= 0 && variant < kRowVariants, "variant out of range");
+        a->row_variant = variant;
       });
 }
 

@@ -188,4 +188,5 @@ struct dfx_form
   std::uint32_t* cell_mask = nullptr;
   void* ref_tensors = nullptr;
   void* cell_vec = nullptr;        // per-cell element vectors (linear forms)
+  int row_variant = 1;             // row-kernel variant (dfx_form_autotune)
 };

@@ -563,6 +563,34 @@ __global__ void __launch_bounds__(kBlock)
     p[i] = (dinv ? dinv[i] * r[i] : r[i]) + beta * p[i];
 }
 
+/// Partial sums of (r.z, r.r) with z = dinv r, into the update partials
+/// (used after the residual has been replaced).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    cg_rz_partials_k(std::int64_t n, const T* r, const T* dinv, CGState* st)
+{
+  double s[2] = {0.0, 0.0};
+  for (std::int64_t i = blockIdx.x * std::int64_t(kBlock) + threadIdx.x; i < n;
+       i += std::int64_t(gridDim.x) * kBlock)
+  {
+    const T ri = r[i];
+    const T zi = dinv ? dinv[i] * ri : ri;
+    s[0] += double(ri) * double(zi);
+    s[1] += double(ri) * double(ri);
+  }
+  block_partials<2>(s, st->partials + kMaxPartials);
+}
+
+/// Re-arms a stopped CG after a residual replacement.
+__global__ void cg_resume_k(CGState* st, double rr0)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+  {
+    st->rr0 = rr0;
+    st->done = 0;
+  }
+}
+
 /// Blocks of a reduction kernel: at most kMaxPartials.
 int partial_blocks(std::int64_t n)
 {
@@ -1270,6 +1298,7 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
         require(b->dtype == DFX_F64 && x->dtype == DFX_F64, "b and x must be FP64");
         require(A64->nrows == A32->nrows && A64->nnz == A32->nnz && A64->bs == A32->bs,
                 "matrices differ in shape");
+        require(A64->nrows == A64->ncols, "matrix not square");
         require(b->n == A64->nrows * A64->bs && x->n == b->n, "size mismatch");
         *res = dfx_mixed_result{};
         hipStream_t s = stream();
@@ -1279,50 +1308,109 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
         CGWork<float> w;
         w.alloc(n, true);
         StateHolder sh;
-        float* r32 = static_cast<float*>(dmalloc(sizeof(float) * n));
+        CGState* st = sh.st;
         float* d32 = static_cast<float*>(dmalloc(sizeof(float) * n));
         double* y = static_cast<double*>(dmalloc(sizeof(double) * n));
         Reducer& red = reducer();
-        DFX_HIP(hipMemsetAsync(sh.st, 0, sizeof(CGState), s));
+        DFX_HIP(hipMemsetAsync(st, 0, sizeof(CGState), s));
         DFX_HIP(hipStreamSynchronize(s));
         const int nb = reduce_blocks(n);
+        const int nbv = partial_blocks(n);
         const auto t0 = std::chrono::steady_clock::now();
         const double bn2 = red.dot(bd, bd, n);
         const double tol2 = o->rtol * o->rtol * bn2;
-        int outer = 0, inner = 0;
-        double rr = 0;
-        bool ok = false;
-        for (;;)
+        // r64 = b - A64 x, r32 = (float) r64 (written into the CG residual)
+        auto true_residual_split = [&]() -> double
         {
           launch_spmv<double>(*A64, xd, y, false);
           const int nbr = partial_blocks(n);
-          residual_split_k<<<nbr, kBlock, 0, s>>>(bd, y, r32, n, red.partials);
+          residual_split_k<<<nbr, kBlock, 0, s>>>(bd, y, w.r, n, red.partials);
           finish_sums_k<<<1, kBlock, 0, s>>>(red.partials, nbr, 1, red.out);
           check_launch("residual");
-          d2h(&rr, red.out, sizeof(double));
+          double h = 0;
+          d2h(&h, red.out, sizeof(double));
+          return h;
+        };
+        int outer = 0, inner = 0, inner_base = 0;
+        double rr = true_residual_split();
+        bool ok = rr <= tol2;
+        const bool restart = o->restart != 0;
+        if (!ok)
+        {
+          inv_diag_k<float><<<static_cast<int>((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+              A32->nrows, A32->bs, A32->diag, static_cast<const float*>(A32->values), w.dinv);
+          fill_k<float><<<nb, kBlock, 0, s>>>(d32, n, 0.0f);
+          fill_k<float><<<nb, kBlock, 0, s>>>(w.q, n, 0.0f);
+          CGState init{};
+          init.tol2 = o->inner_rtol * o->inner_rtol;
+          init.max_it = o->max_inner;
+          DFX_HIP(hipMemcpyAsync(st, &init, offsetof(CGState, partials), hipMemcpyHostToDevice, s));
+          // r = r32 - A32 * 0, p = z: reuse cg_start with y = 0
+          cg_start_k<float><<<nbv, kBlock, 0, s>>>(n, w.r, w.q, w.dinv, w.r, w.p, st);
+          cg_start_finish_k<<<1, kBlock, 0, s>>>(nbv, st);
+          cg_resume_k<<<1, 1, 0, s>>>(st, rr);
+          check_launch("mixed start");
+        }
+        int k = 0;
+        struct
+        {
+          std::int32_t iter, done;
+        } h{0, 0};
+        while (!ok && k < o->max_inner)
+        {
+          for (int j = 0; j < std::max(1, o->check_every) && k < o->max_inner; ++j, ++k)
+          {
+            const int nbs = launch_cg_spmv<float>(*A32, w.p, w.q, st);
+            cg_update_k<float><<<nbv, kBlock, 0, s>>>(n, k, nbs, d32, w.r, w.p, w.q, w.dinv, st);
+            cg_direction_k<float><<<nbv, kBlock, 0, s>>>(n, k, nbv, w.r, w.dinv, w.p, st);
+          }
+          check_launch("mixed iteration");
+          DFX_HIP(hipMemcpyAsync(&h, &st->iter, sizeof(h), hipMemcpyDeviceToHost, s));
+          DFX_HIP(hipStreamSynchronize(s));
+          if (!h.done)
+            continue;
+          // FP32 residual dropped by inner_rtol (or the inner budget ran out):
+          // FP64 correction and true residual
+          inner = inner_base + h.iter;
+          add_correction_k<<<nb, kBlock, 0, s>>>(xd, d32, n);
+          fill_k<float><<<nb, kBlock, 0, s>>>(d32, n, 0.0f);
+          rr = true_residual_split();
+          ++outer;
           if (rr <= tol2)
           {
             ok = true;
             break;
           }
-          if (outer >= o->max_outer)
+          if (outer >= o->max_outer || h.iter >= o->max_inner)
             break;
-          fill_k<float><<<nb, kBlock, 0, s>>>(d32, n, 0.0f);
-          bool conv = false;
-          double a0 = 0, a1 = 0;
-          inner += cg_run<float>(*A32, r32, d32, o->max_inner, o->inner_rtol, true,
-                                 o->check_every, w, sh.st, &conv, &a0, &a1, true);
-          add_correction_k<<<nb, kBlock, 0, s>>>(xd, d32, n);
-          check_launch("correction");
-          ++outer;
+          if (restart)
+          {
+            // classic refinement: a new inner CG from d = 0 on the new r
+            inner_base += h.iter;
+            fill_k<float><<<nb, kBlock, 0, s>>>(w.q, n, 0.0f);
+            cg_start_k<float><<<nbv, kBlock, 0, s>>>(n, w.r, w.q, w.dinv, w.r, w.p, st);
+            cg_start_finish_k<<<1, kBlock, 0, s>>>(nbv, st);
+            cg_resume_k<<<1, 1, 0, s>>>(st, rr);
+            k = 0;
+          }
+          else
+          {
+            // reliable update: keep the search direction; beta is formed
+            // from the replaced residual, rz_new / rz_old
+            const int kk = h.iter - 1;
+            cg_resume_k<<<1, 1, 0, s>>>(st, rr);
+            cg_rz_partials_k<float><<<nbv, kBlock, 0, s>>>(n, w.r, w.dinv, st);
+            cg_direction_k<float><<<nbv, kBlock, 0, s>>>(n, kk, nbv, w.r, w.dinv, w.p, st);
+            k = kk + 1;
+          }
+          check_launch("mixed refinement");
         }
         DFX_HIP(hipStreamSynchronize(s));
         const auto t1 = std::chrono::steady_clock::now();
-        dfree(r32);
         dfree(d32);
         dfree(y);
         res->outer = outer;
-        res->inner_total = inner;
+        res->inner_total = std::max(inner, inner_base + h.iter);
         res->converged = ok ? 1 : 0;
         res->true_rel_residual = bn2 > 0 ? std::sqrt(rr / bn2) : std::sqrt(rr);
         res->seconds = std::chrono::duration<double>(t1 - t0).count();

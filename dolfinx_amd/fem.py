@@ -186,6 +186,20 @@ class Form:
     def handle(self):
         return self._h
 
+    def autotune(self, A: MatrixCSR, bc: DirichletBC | None = None):
+        """Pick the fastest assembly kernel variant for this form and the
+        pattern of A by timing (A's values are overwritten and left zero).
+        Returns (variant, ms per assembly)."""
+        v, ms = C.c_int32(), C.c_float()
+        _lib.call("dfx_form_autotune", self._h, A.handle, None if bc is None else bc.handle,
+                  C.byref(v), C.byref(ms))
+        return v.value, ms.value
+
+    def set_variant(self, variant: int):
+        """Fix the assembly kernel variant (0-2 thread per row with 64/128/256
+        rows per workgroup, 3 one incidence per lane)."""
+        _lib.call("dfx_form_set_variant", self._h, int(variant))
+
     @property
     def function_spaces(self):
         return [self.V] * self.rank

@@ -209,6 +209,15 @@ dfx_status dfx_bc_destroy(dfx_bc* bc);
  * A += a with rows and columns of the BC dofs zeroed (bc may be NULL). */
 dfx_status dfx_assemble_matrix(dfx_csr* A, const dfx_form* a,
                                const dfx_bc* bc);
+/* Assembly autotuner (no counterpart in the reference): times the
+ * row-kernel variants (rows per workgroup, lanes per row) of form a on
+ * matrix A and keeps the fastest for later assemblies of a. Overwrites A's
+ * values and leaves them zero. */
+dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc,
+                             int32_t* variant, float* ms);
+/* Fix the row-kernel variant of a form: 0-2 one thread per row with 64,
+ * 128, 256 rows per workgroup; 3 one incidence per lane. */
+dfx_status dfx_form_set_variant(dfx_form* a, int32_t variant);
 /* fem::set_diagonal (fem/assembler.h:644-686): A[d,d] = diag for BC dofs */
 dfx_status dfx_set_diagonal(dfx_csr* A, const dfx_bc* bc, double diag);
 /* fem::assemble_vector(b, L) (fem/assembler.h:230-257): b += L */
@@ -255,8 +264,10 @@ typedef struct
   int32_t max_outer;   /* refinement steps */
   int32_t max_inner;   /* iterations per inner FP32 solve */
   double rtol;         /* target ||b - A x||_2/||b||_2 (FP64 residual) */
-  double inner_rtol;   /* inner FP32 CG relative tolerance */
+  double inner_rtol;   /* refine when the FP32 residual fell by this */
   int32_t check_every;
+  int32_t restart;     /* 1: restart the FP32 CG at every refinement;
+                          0: keep its search direction (reliable updates) */
 } dfx_mixed_opts;
 
 typedef struct
@@ -268,9 +279,13 @@ typedef struct
   double seconds;
 } dfx_mixed_result;
 
-/* Mixed-precision iterative refinement: FP64 residual r = b - A64 x and
- * FP64 update x += d, with d from Jacobi-PCG on A32 in FP32 (FP64
- * reductions). No counterpart in the reference; it wraps the same CG. */
+/* Mixed-precision iterative refinement (no counterpart in the reference;
+ * it wraps the same CG): Jacobi-PCG on A32 in FP32 (FP64 reductions)
+ * accumulates a correction d; each time its residual has fallen by
+ * inner_rtol, x += d in FP64 and the residual is replaced by the FP64 true
+ * residual b - A64 x. With restart = 0 the FP32 CG keeps its search
+ * direction across replacements (reliable updates, van der Vorst & Ye
+ * 2000); with restart = 1 every refinement starts a new inner CG. */
 dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32,
                            const dfx_vec* b, dfx_vec* x,
                            const dfx_mixed_opts* opts, dfx_mixed_result* res);

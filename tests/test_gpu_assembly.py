@@ -60,6 +60,28 @@ def test_assemble_system_matches_oracle(cell, n, dtype):
     assert rel_err(b.array, ref["b"]) <= tol
 
 
+@pytest.mark.parametrize("cell,n", CASES)
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_every_row_kernel_variant_matches_oracle(cell, n, variant):
+    p = build(cell, n, np.float64)
+    p.a.set_variant(variant)
+    A, b = problems.assemble_system(p)
+    ref = oracle_system(p, n)
+    assert rel_err(A.data, ref["A"]) <= 1e-12
+    assert np.max(np.abs(A.data - ref["A"])) <= 1e-12 * np.max(np.abs(ref["A"]))
+
+
+def test_autotune_keeps_results():
+    p = build(O.HEX, (20, 20, 20), np.float64)
+    A = fem.create_matrix(p.a)
+    variant, ms = p.a.autotune(A, p.bc)
+    assert 0 <= variant <= 3 and ms > 0
+    assert np.all(A.data == 0)
+    fem.assemble_matrix(p.a, [p.bc], 1.0, A)
+    ref = oracle_system(p, (20, 20, 20))
+    assert rel_err(A.data, ref["A"]) <= 1e-12
+
+
 @pytest.mark.parametrize("cell,n", CASES[:2])
 def test_lifting_nonzero_g_matches_oracle(cell, n):
     """Manufactured nonzero Dirichlet data pins apply_lifting + set_bc."""

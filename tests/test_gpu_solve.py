@@ -72,14 +72,15 @@ def test_cg_raises_when_not_converged():
 
 
 @pytest.mark.parametrize("cell,n", [(O.TET, (12, 12, 12)), (O.HEX, (24, 24, 24))])
-def test_mixed_refinement_matches_fp64(cell, n):
+@pytest.mark.parametrize("restart", [False, True])
+def test_mixed_refinement_matches_fp64(cell, n, restart):
     kw = dict(kappa="exp_normal", c=0.0, f="one")
     p64 = problems.scalar_problem(n, cell, dtype=np.float64, seed=3, **kw)
     p32 = problems.scalar_problem(n, cell, dtype=np.float32, seed=3, msh=p64.mesh, V=p64.V, **kw)
     A64, b = problems.assemble_system(p64)
     A32, _ = problems.assemble_system(p32)
     x = la.Vector(p64.V.num_dofs)
-    res = solvers.mixed_refinement(A64, A32, b, x, rtol=1e-10, inner_rtol=1e-4)
+    res = solvers.mixed_refinement(A64, A32, b, x, rtol=1e-10, inner_rtol=1e-4, restart=restart)
     assert res.converged and res.true_rel_residual <= 1e-10
     xr, _ = O.cg(A64.indptr, A64.indices, A64.data, 1, b.array, kmax=50000, rtol=1e-13)
     assert rel_err(x.array, xr) <= 1e-8
