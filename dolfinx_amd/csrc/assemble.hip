@@ -741,12 +741,15 @@ __global__ void __launch_bounds__(kBlock)
                    const T* __restrict__ ref, T* __restrict__ vals, int accumulate)
 {
   extern __shared__ unsigned char smem_raw[];
-  T* acc = reinterpret_cast<T*>(smem_raw);
+  T* S = reinterpret_cast<T*>(smem_raw); // reference tensors [7][K][K]
+  T* acc = S + 7 * K * K;
   const int R = blockDim.x;
   const std::int32_t r0 = xcd_contiguous(blockIdx.x, gridDim.x) * R;
   const std::int32_t r1 = min(r0 + R, nrows);
   const std::int32_t base = rp[r0];
   const std::int32_t len = rp[r1] - base;
+  for (int t = threadIdx.x; t < 7 * K * K; t += R)
+    S[t] = ref[t];
   for (int t = threadIdx.x; t < len; t += R)
     acc[t] = T(0);
   __syncthreads();
@@ -789,7 +792,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
         for (int j = 0; j < K; ++j)
           if (!((m[u] >> j) & 1u))
-            a[sl[u][j]] += element_entry<T, K>(g[u], ref, i, j);
+            a[sl[u][j]] += element_entry<T, K>(g[u], S, i, j);
       }
     }
   }
@@ -815,12 +818,15 @@ __global__ void __launch_bounds__(kBlock)
                          const T* __restrict__ ref, T* __restrict__ vals, int accumulate)
 {
   extern __shared__ unsigned char smem_raw[];
-  T* acc = reinterpret_cast<T*>(smem_raw);
+  T* S = reinterpret_cast<T*>(smem_raw); // reference tensors [7][K][K]
+  T* acc = S + 7 * K * K;
   constexpr int RPB = kBlock / L;
   const std::int32_t r0 = xcd_contiguous(blockIdx.x, gridDim.x) * RPB;
   const std::int32_t r1 = min(r0 + RPB, nrows);
   const std::int32_t base = rp[r0];
   const std::int32_t len = rp[r1] - base;
+  for (int t = threadIdx.x; t < 7 * K * K; t += kBlock)
+    S[t] = ref[t];
   for (int t = threadIdx.x; t < len; t += kBlock)
     acc[t] = T(0);
   __syncthreads();
@@ -849,7 +855,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
       for (int j = 0; j < K; ++j)
         if (!((m >> j) & 1u))
-          atomicAdd(a + sl[j], element_entry<T, K>(g, ref, i, j));
+          atomicAdd(a + sl[j], element_entry<T, K>(g, S, i, j));
     }
   }
   __syncthreads();
@@ -1141,7 +1147,7 @@ template <typename T, typename O, int K, int L>
 void launch_lanes(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
 {
   constexpr int RPB = kBlock / L;
-  const std::size_t lds = std::size_t(RPB) * A.max_row * sizeof(T);
+  const std::size_t lds = (7 * K * K + std::size_t(RPB) * A.max_row) * sizeof(T);
   require(lds <= 64 * 1024, "row too long for the LDS slice");
   const int nb = (A.nrows + RPB - 1) / RPB;
   if (nb > 0)
@@ -1160,7 +1166,7 @@ void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
   if (variant < 3)
   {
     const int R = 64 << variant;
-    const std::size_t lds = std::size_t(R) * A.max_row * sizeof(T);
+    const std::size_t lds = (7 * K * K + std::size_t(R) * A.max_row) * sizeof(T);
     require(lds <= 64 * 1024, "row too long for the LDS slice");
     const int nb = (A.nrows + R - 1) / R;
     if (nb > 0)

@@ -170,11 +170,11 @@ __global__ void __launch_bounds__(kBlock)
   diag[r] = (lo < rp[r + 1] && ci[lo] == r) ? lo : -1;
 }
 
-/// Entries per LDS chunk of the stream SpMV (16 KB of products).
+/// Entries per LDS chunk of the stream SpMV (32 KB of products).
 template <typename T>
 constexpr int stream_chunk()
 {
-  return 16384 / static_cast<int>(sizeof(T));
+  return 32768 / static_cast<int>(sizeof(T));
 }
 
 /// Stream CSR SpMV, y (+)= A x. Each row block (consecutive rows whose
@@ -965,8 +965,9 @@ dfx_status dfx_csr_create(const int64_t* row_ptr, const int32_t* cols, int32_t n
         if (bs == 1)
         {
           // row blocks of the stream SpMV: greedy, <= 256 rows and entries
-          // fitting the LDS chunk of either precision
-          const std::int32_t nc_max = stream_chunk<double>();
+          // fitting the LDS chunk of the matrix precision
+          const std::int32_t nc_max
+              = dtype == DFX_F64 ? stream_chunk<double>() : stream_chunk<float>();
           std::vector<std::int32_t> rb{0};
           std::int32_t r = 0;
           while (r < nrows)

@@ -25,9 +25,11 @@ def cg(A: MatrixCSR, b: Vector, x: Vector, rtol=1e-8, max_it=10000, jacobi=True,
 
 def mixed_refinement(A64: MatrixCSR, A32: MatrixCSR, b: Vector, x: Vector, rtol=1e-8,
                      inner_rtol=1e-4, max_outer=50, max_inner=20000, check_every=32,
-                     restart=False):
-    """FP32 Jacobi-PCG inside FP64 iterative refinement. restart=False keeps
-    the FP32 search direction across refinements (reliable updates)."""
+                     restart=True):
+    """FP32 Jacobi-PCG inside FP64 iterative refinement: each inner solve
+    reduces the residual by inner_rtol, then x += d in FP64 and the FP64
+    true residual restarts the next inner solve. restart=False keeps the FP32
+    search direction across refinements instead (reliable updates)."""
     opts = _lib.MixedOpts(max_outer=int(max_outer), max_inner=int(max_inner), rtol=float(rtol),
                           inner_rtol=float(inner_rtol), check_every=int(check_every),
                           restart=1 if restart else 0)
