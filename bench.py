@@ -203,6 +203,20 @@ def main():
     spmv32_ms = event_ms(4, 5) / nsp
     spmv32_gbs = (nnz * 8 + (N + 1) * 4 + 3 * N * 4) / (spmv32_ms * 1e-3) / 1e9
 
+    # ---- practical streaming bandwidth: device copy of 2^27 doubles ----
+    ncopy = 1 << 27
+    src = la.Vector(ncopy)
+    dst = la.Vector(ncopy)
+    src.set(1.0)
+    dst.copy_from(src)
+    record(6)
+    for _ in range(5):
+        dst.copy_from(src)
+    record(7)
+    copy_ms = event_ms(6, 7) / 5
+    copy_gbs = 2 * 8 * ncopy / (copy_ms * 1e-3) / 1e9
+    del src, dst
+
     out_solve = {}
     if not args.no_solve:
         problems.assemble_system(p64, A64, b64)
@@ -274,6 +288,7 @@ def main():
             "algorithmic_bytes": asm_bytes,
             "kernel_ms": round(asm_ms, 4),
         },
+        "copy_gbs": round(copy_gbs, 1),
         "assembly_f32_gnnz_s": round(nnz / asm32_s / 1e9, 4),
         "assembly_variant": {"f64": variant64[0], "f32": variant32[0]},
         "spmv": {

@@ -72,7 +72,20 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([[HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-pthread"]])
     shutil.move(tmp, LIB)
+    build_examples()
     return LIB
+
+
+EXAMPLES = {"poisson_demo": "poisson.cpp"}
+
+
+def build_examples() -> None:
+    """C++ programs written against the drop-in headers, linked to libdfx."""
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    for exe, src in EXAMPLES.items():
+        _run([["g++", "-std=c++20", "-O2", "-Wall", f"-I{inc}", os.path.join(HERE, "examples", src),
+               "-o", os.path.join(OUT_DIR, exe), f"-L{HERE}", "-ldfx",
+               "-Wl,-rpath,$ORIGIN/..", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}"]])
 
 
 if __name__ == "__main__":

@@ -718,6 +718,60 @@ __global__ void __launch_bounds__(kBlock)
   islots[t] = static_cast<O>(lo - r0);
 }
 
+/// The 7 cell factors of cell c (stored as 8 values, 16-byte aligned).
+template <typename T>
+__device__ inline void load_factors(const T* __restrict__ cd, std::int64_t c, T (&g)[7])
+{
+  if constexpr (sizeof(T) == 8)
+  {
+    const double2* p = reinterpret_cast<const double2*>(cd + 8 * c);
+    const double2 a = p[0], b = p[1], d = p[2], e = p[3];
+    g[0] = a.x; g[1] = a.y; g[2] = b.x; g[3] = b.y; g[4] = d.x; g[5] = d.y; g[6] = e.x;
+  }
+  else
+  {
+    const float4* p = reinterpret_cast<const float4*>(cd + 8 * c);
+    const float4 a = p[0], b = p[1];
+    g[0] = a.x; g[1] = a.y; g[2] = a.z; g[3] = a.w; g[4] = b.x; g[5] = b.y; g[6] = b.z;
+  }
+}
+
+/// The K slot offsets of incidence p with one vector load where the width
+/// allows (K bytes or 2K bytes, naturally aligned).
+template <typename O, int K>
+__device__ inline void load_slots(const O* __restrict__ islots, std::int64_t p, O (&sl)[K])
+{
+  const O* src = islots + p * K;
+  if constexpr (sizeof(O) * K == 8)
+  {
+    const std::uint64_t w = *reinterpret_cast<const std::uint64_t*>(src);
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      sl[j] = static_cast<O>(w >> (8 * sizeof(O) * j));
+  }
+  else if constexpr (sizeof(O) * K == 4)
+  {
+    const std::uint32_t w = *reinterpret_cast<const std::uint32_t*>(src);
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      sl[j] = static_cast<O>(w >> (8 * sizeof(O) * j));
+  }
+  else if constexpr (sizeof(O) * K == 16)
+  {
+    const uint4 w = *reinterpret_cast<const uint4*>(src);
+    const std::uint32_t q[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      sl[j] = static_cast<O>(q[(j * sizeof(O)) / 4] >> (8 * ((j * sizeof(O)) % 4)));
+  }
+  else
+  {
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      sl[j] = src[j];
+  }
+}
+
 /// Entry (i, j) of the element matrix from the cell factors g and the
 /// reference tensors ref[7][K][K].
 template <typename T, int K>
@@ -773,15 +827,9 @@ __global__ void __launch_bounds__(kBlock)
         if (cik[u] < 0)
           continue;
         const std::int64_t c = cik[u] / K;
-        const T* G = cd + 8 * c;
-#pragma unroll
-        for (int q = 0; q < 7; ++q)
-          g[u][q] = G[q];
+        load_factors<T>(cd, c, g[u]);
         m[u] = mask ? mask[c] : 0u;
-        const O* src = islots + std::int64_t(pb + u) * K;
-#pragma unroll
-        for (int j = 0; j < K; ++j)
-          sl[u][j] = src[j];
+        load_slots<O, K>(islots, pb + u, sl[u]);
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u)
@@ -841,17 +889,11 @@ __global__ void __launch_bounds__(kBlock)
       const std::int32_t cik = inc[p];
       const std::int64_t c = cik / K;
       const int i = cik - static_cast<int>(c) * K;
-      const T* G = cd + 8 * c;
       T g[7];
-#pragma unroll
-      for (int q = 0; q < 7; ++q)
-        g[q] = G[q];
+      load_factors<T>(cd, c, g);
       const std::uint32_t m = mask ? mask[c] : 0u;
       O sl[K];
-      const O* src = islots + std::int64_t(p) * K;
-#pragma unroll
-      for (int j = 0; j < K; ++j)
-        sl[j] = src[j];
+      load_slots<O, K>(islots, p, sl);
 #pragma unroll
       for (int j = 0; j < K; ++j)
         if (!((m >> j) & 1u))

@@ -170,11 +170,11 @@ __global__ void __launch_bounds__(kBlock)
   diag[r] = (lo < rp[r + 1] && ci[lo] == r) ? lo : -1;
 }
 
-/// Entries per LDS chunk of the stream SpMV (32 KB of products).
+/// Entries per LDS chunk of the stream SpMV (16 KB of products).
 template <typename T>
 constexpr int stream_chunk()
 {
-  return 32768 / static_cast<int>(sizeof(T));
+  return 16384 / static_cast<int>(sizeof(T));
 }
 
 /// Stream CSR SpMV, y (+)= A x. Each row block (consecutive rows whose

@@ -1,0 +1,502 @@
+// C++ drop-in for the DOLFINx assemble-and-solve path on MI355X.
+//
+// Header-only wrappers over the C ABI of dfx.h that keep the reference's
+// call shapes and names, so that a caller of
+//   fem::assemble_matrix(A.mat_add_values(), *a, {bc});   (fem/assembler.h:542)
+//   fem::set_diagonal(A.mat_set_values(), *V, {bc}, 1.0); (fem/assembler.h:671)
+//   fem::assemble_vector(b, *L);                          (fem/assembler.h:247)
+//   fem::apply_lifting(b, {a}, {{bc}}, {}, 1.0);          (fem/assembler.h:451)
+//   bc->set(b, std::nullopt, 1.0);                        (fem/DirichletBC.h:495)
+//   A.mult(x, y);                                         (la/MatrixCSR.h:498)
+//   la::inner_product(x, y);                              (la/Vector.h:434)
+// switches namespace (dolfinx -> dolfinx_amd) and keeps its code. The data
+// lives in HBM: la::Vector and la::MatrixCSR are the reference's
+// container-templated types instantiated with a device container
+// (la/Vector.h:42-49, la/MatrixCSR.h:67-70). Failures of the C ABI are
+// rethrown as std::runtime_error, as the reference throws.
+#pragma once
+
+#include "../dfx.h"
+
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <optional>
+#include <span>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace dolfinx_amd
+{
+
+inline void check(dfx_status s)
+{
+  if (s != DFX_OK)
+    throw std::runtime_error(dfx_last_error());
+}
+
+template <typename T>
+constexpr int dtype_code()
+{
+  static_assert(std::is_same_v<T, double> || std::is_same_v<T, float>,
+                "scalar type must be float or double");
+  return std::is_same_v<T, double> ? DFX_F64 : DFX_F32;
+}
+
+namespace mesh
+{
+enum class CellType
+{
+  tetrahedron = DFX_TET,
+  hexahedron = DFX_HEX
+};
+
+/// mesh::Mesh after create_box (one rank, RCM-reordered cells).
+class Mesh
+{
+public:
+  explicit Mesh(dfx_mesh* h, CellType ct) : _h(h, dfx_mesh_destroy), _ct(ct)
+  {
+    check(dfx_mesh_sizes(h, &_nc, &_nv, &_npc));
+  }
+  dfx_mesh* handle() const { return _h.get(); }
+  CellType cell_type() const { return _ct; }
+  std::int32_t num_cells() const { return _nc; }
+  std::int32_t num_vertices() const { return _nv; }
+  std::int32_t nodes_per_cell() const { return _npc; }
+  /// Geometry::x (num_nodes x 3)
+  std::vector<double> x() const
+  {
+    std::vector<double> v(3 * std::size_t(_nv));
+    check(dfx_mesh_copy(_h.get(), v.data(), nullptr, nullptr, nullptr));
+    return v;
+  }
+  /// Geometry dofmap = cell-vertex map for P1 geometry
+  std::vector<std::int32_t> cells() const
+  {
+    std::vector<std::int32_t> v(std::size_t(_nc) * _npc);
+    check(dfx_mesh_copy(_h.get(), nullptr, v.data(), nullptr, nullptr));
+    return v;
+  }
+  /// Topology::original_cell_index (mesh/Topology.h:218)
+  std::vector<std::int64_t> original_cell_index() const
+  {
+    std::vector<std::int64_t> v(_nc);
+    check(dfx_mesh_copy(_h.get(), nullptr, nullptr, v.data(), nullptr));
+    return v;
+  }
+  /// Geometry::input_global_indices
+  std::vector<std::int64_t> input_global_indices() const
+  {
+    std::vector<std::int64_t> v(_nv);
+    check(dfx_mesh_copy(_h.get(), nullptr, nullptr, nullptr, v.data()));
+    return v;
+  }
+
+private:
+  std::shared_ptr<dfx_mesh> _h;
+  CellType _ct;
+  std::int32_t _nc = 0, _nv = 0, _npc = 0;
+};
+
+/// mesh::create_box (mesh/generation.h:157-163); comm and partitioner
+/// arguments of the reference are not needed on one rank.
+inline std::shared_ptr<Mesh> create_box(std::array<std::array<double, 3>, 2> p,
+                                        std::array<std::int64_t, 3> n, CellType celltype)
+{
+  dfx_mesh* h = nullptr;
+  check(dfx_mesh_create_box(p[0].data(), p[1].data(), n.data(), static_cast<int>(celltype), &h));
+  return std::make_shared<Mesh>(h, celltype);
+}
+} // namespace mesh
+
+namespace la
+{
+template <typename T>
+class Vector;
+
+/// la::SparsityPattern after finalize (block pattern over nodes).
+class SparsityPattern
+{
+public:
+  explicit SparsityPattern(dfx_sparsity* h) : _h(h, dfx_sparsity_destroy)
+  {
+    std::int64_t nnz = 0;
+    check(dfx_sparsity_sizes(h, &_nrows, &nnz, &_bs));
+    _row_ptr.resize(std::size_t(_nrows) + 1);
+    _cols.resize(nnz);
+    check(dfx_sparsity_copy(h, _row_ptr.data(), _cols.data()));
+  }
+  std::span<const std::int64_t> graph_offsets() const { return _row_ptr; }
+  std::span<const std::int32_t> graph_columns() const { return _cols; }
+  int block_size() const { return _bs; }
+  std::int32_t num_rows() const { return _nrows; }
+
+private:
+  std::shared_ptr<dfx_sparsity> _h;
+  std::int32_t _nrows = 0, _bs = 1;
+  std::vector<std::int64_t> _row_ptr;
+  std::vector<std::int32_t> _cols;
+};
+
+/// la::Vector<T, DeviceContainer>: values in HBM.
+template <typename T>
+class Vector
+{
+public:
+  using value_type = T;
+  explicit Vector(std::int32_t n)
+  {
+    dfx_vec* h = nullptr;
+    check(dfx_vec_create(n, dtype_code<T>(), &h));
+    _h.reset(h, dfx_vec_destroy);
+    _n = n;
+  }
+  /// Copy-convert constructor (la/Vector.h:168-187)
+  template <typename U>
+  explicit Vector(const Vector<U>& x) : Vector(x.size())
+  {
+    check(dfx_vec_copy(_h.get(), x.handle()));
+  }
+  dfx_vec* handle() const { return _h.get(); }
+  std::int32_t size() const { return _n; }
+  /// Host copy of the values
+  std::vector<T> array() const
+  {
+    std::vector<T> v(_n);
+    check(dfx_vec_download(_h.get(), v.data()));
+    return v;
+  }
+  void set_array(std::span<const T> v)
+  {
+    if (static_cast<std::int32_t>(v.size()) != _n)
+      throw std::runtime_error("size mismatch");
+    check(dfx_vec_upload(_h.get(), v.data()));
+  }
+  void set(T v) { check(dfx_vec_set(_h.get(), static_cast<double>(v))); }
+  /// No ghosts on one GPU: the scatters are no-ops (la/Vector.h:371-379)
+  void scatter_fwd() {}
+  void scatter_rev() {}
+
+private:
+  std::shared_ptr<dfx_vec> _h;
+  std::int32_t _n = 0;
+};
+
+/// Tag returned by mat_add_values(): routes fem::assemble_matrix to the
+/// device scatter (the reference's MatSet functor, la/utils.h:26-29).
+template <typename T>
+struct DeviceMatAdd
+{
+  dfx_csr* A;
+};
+template <typename T>
+struct DeviceMatSet
+{
+  dfx_csr* A;
+};
+
+/// la::MatrixCSR<T, DeviceContainer> in compact block mode.
+template <typename T>
+class MatrixCSR
+{
+public:
+  using value_type = T;
+  explicit MatrixCSR(const SparsityPattern& p)
+  {
+    dfx_csr* h = nullptr;
+    check(dfx_csr_create(p.graph_offsets().data(), p.graph_columns().data(), p.num_rows(),
+                         p.num_rows(), p.block_size(), dtype_code<T>(), &h));
+    _h.reset(h, dfx_csr_destroy);
+    _bs = p.block_size();
+    _nrows = p.num_rows();
+    _nnz = static_cast<std::int64_t>(p.graph_columns().size());
+  }
+  dfx_csr* handle() const { return _h.get(); }
+  DeviceMatAdd<T> mat_add_values() { return {_h.get()}; }
+  DeviceMatSet<T> mat_set_values() { return {_h.get()}; }
+  /// MatrixCSR::set(x): only zero is supported on the device
+  void set(T x)
+  {
+    if (x != T(0))
+      throw std::runtime_error("MatrixCSR::set: only zero is supported");
+    check(dfx_csr_zero(_h.get()));
+  }
+  /// y += A x (la/MatrixCSR.h:877-946)
+  void mult(Vector<T>& x, Vector<T>& y) const { check(dfx_spmv(_h.get(), x.handle(), y.handle())); }
+  /// Host copy of the values (row-major blocks)
+  std::vector<T> values() const
+  {
+    std::vector<T> v(_nnz * _bs * _bs);
+    check(dfx_csr_get_values(_h.get(), v.data()));
+    return v;
+  }
+  std::array<int, 2> block_size() const { return {_bs, _bs}; }
+  std::int32_t num_rows() const { return _nrows; }
+  /// Frobenius norm squared (la/MatrixCSR.h:470-490)
+  double squared_norm() const
+  {
+    double s = 0;
+    for (T v : values())
+      s += double(v) * double(v);
+    return s;
+  }
+  /// Pick the SpMV variant by timing on this matrix
+  std::pair<int, float> autotune()
+  {
+    std::int32_t l = 0;
+    float ms = 0;
+    check(dfx_csr_autotune(_h.get(), &l, &ms));
+    return {l, ms};
+  }
+
+private:
+  std::shared_ptr<dfx_csr> _h;
+  int _bs = 1;
+  std::int32_t _nrows = 0;
+  std::int64_t _nnz = 0;
+};
+
+/// la::inner_product (la/Vector.h:434-460); FP64 accumulation
+template <typename T>
+double inner_product(const Vector<T>& a, const Vector<T>& b)
+{
+  double r = 0;
+  check(dfx_inner_product(a.handle(), b.handle(), &r));
+  return r;
+}
+/// la::squared_norm (la/Vector.h:462-471)
+template <typename T>
+double squared_norm(const Vector<T>& a)
+{
+  double r = 0;
+  check(dfx_squared_norm(a.handle(), &r));
+  return r;
+}
+/// la::norm(x, Norm::l2) (la/Vector.h:478-514)
+template <typename T>
+double norm(const Vector<T>& a)
+{
+  return std::sqrt(squared_norm(a));
+}
+} // namespace la
+
+namespace fem
+{
+/// Lagrange function space with its (block) dofmap.
+class FunctionSpace
+{
+public:
+  FunctionSpace(std::shared_ptr<mesh::Mesh> m, int degree, int bs = 1) : _mesh(std::move(m))
+  {
+    dfx_space* h = nullptr;
+    check(dfx_space_create(_mesh->handle(), degree, bs, &h));
+    _h.reset(h, dfx_space_destroy);
+    check(dfx_space_sizes(h, &_nn, &_k, &_bs));
+    _degree = degree;
+  }
+  dfx_space* handle() const { return _h.get(); }
+  std::shared_ptr<mesh::Mesh> mesh() const { return _mesh; }
+  int degree() const { return _degree; }
+  int bs() const { return _bs; }
+  std::int32_t num_nodes() const { return _nn; }
+  std::int32_t dofs_per_cell() const { return _k; }
+  std::vector<std::int32_t> dofmap() const
+  {
+    std::vector<std::int32_t> d(std::size_t(_mesh->num_cells()) * _k);
+    check(dfx_space_dofmap(_h.get(), d.data()));
+    return d;
+  }
+
+private:
+  std::shared_ptr<mesh::Mesh> _mesh;
+  std::shared_ptr<dfx_space> _h;
+  std::int32_t _nn = 0, _k = 0, _bs = 1;
+  int _degree = 1;
+};
+
+/// fem::create_functionspace for ("Lagrange", degree[, shape])
+inline std::shared_ptr<FunctionSpace> create_functionspace(std::shared_ptr<mesh::Mesh> m,
+                                                           int degree, int bs = 1)
+{
+  return std::make_shared<FunctionSpace>(std::move(m), degree, bs);
+}
+
+/// locate_dofs_topological on the exterior facets (all, or those with every
+/// vertex at x[axis] == value), unrolled: returns dof indices.
+inline std::vector<std::int32_t> locate_boundary_dofs(const FunctionSpace& V,
+                                                      std::optional<int> axis = std::nullopt,
+                                                      double value = 0.0)
+{
+  std::int32_t n = 0;
+  const int marker = axis ? 1 : 0;
+  check(dfx_locate_boundary_dofs(V.handle(), marker, axis.value_or(0), value, &n, nullptr));
+  std::vector<std::int32_t> d(n);
+  check(dfx_locate_boundary_dofs(V.handle(), marker, axis.value_or(0), value, &n, d.data()));
+  return d;
+}
+
+/// fem::DirichletBC on unrolled dofs with values g (empty: g = 0).
+template <typename T>
+class DirichletBC
+{
+public:
+  DirichletBC(std::shared_ptr<const FunctionSpace> V, std::vector<std::int32_t> dofs,
+              std::vector<double> g = {})
+      : _V(std::move(V)), _dofs(std::move(dofs))
+  {
+    const std::int32_t n = _V->num_nodes() * _V->bs();
+    if (!g.empty() && static_cast<std::int32_t>(g.size()) != n)
+      throw std::runtime_error("g must have one value per dof");
+    dfx_bc* h = nullptr;
+    check(dfx_bc_create(_dofs.data(), static_cast<std::int32_t>(_dofs.size()), n,
+                        g.empty() ? nullptr : g.data(), &h));
+    _h.reset(h, dfx_bc_destroy);
+  }
+  dfx_bc* handle() const { return _h.get(); }
+  std::shared_ptr<const FunctionSpace> function_space() const { return _V; }
+  /// DirichletBC::set(x, x0, alpha): x[bc] = alpha (g - x0)
+  void set(la::Vector<T>& x, std::optional<std::reference_wrapper<const la::Vector<T>>> x0,
+           T alpha = 1) const
+  {
+    check(dfx_bc_set(x.handle(), _h.get(), x0 ? x0->get().handle() : nullptr, alpha));
+  }
+  std::span<const std::int32_t> dof_indices() const { return _dofs; }
+
+private:
+  std::shared_ptr<const FunctionSpace> _V;
+  std::vector<std::int32_t> _dofs;
+  std::shared_ptr<dfx_bc> _h;
+};
+
+/// fem::Form whose single cell integral is one of the hand-written device
+/// kernels (DFX_K_*), with packed coefficients (cells x cstride, in the
+/// mesh's cell order) and constants.
+template <typename T>
+class Form
+{
+public:
+  Form(int kernel, std::shared_ptr<const FunctionSpace> V, std::vector<double> coeffs,
+       int cstride, std::vector<double> constants)
+      : _V(std::move(V))
+  {
+    auto m = _V->mesh();
+    const std::vector<double> x = m->x();
+    const std::vector<std::int32_t> cells = m->cells();
+    const std::vector<std::int32_t> dm = _V->dofmap();
+    dfx_form_desc d{};
+    d.kernel = kernel;
+    d.dtype = dtype_code<T>();
+    d.num_cells = m->num_cells();
+    d.num_nodes = m->num_vertices();
+    d.nodes_per_cell = m->nodes_per_cell();
+    d.x = x.data();
+    d.x_dofmap = cells.data();
+    d.dofs_per_cell = _V->dofs_per_cell();
+    d.bs = _V->bs();
+    d.dofmap = dm.data();
+    d.cstride = cstride;
+    d.coeffs = coeffs.empty() ? nullptr : coeffs.data();
+    d.num_constants = static_cast<std::int32_t>(constants.size());
+    d.constants = constants.empty() ? nullptr : constants.data();
+    dfx_form* h = nullptr;
+    check(dfx_form_create(&d, &h));
+    _h.reset(h, dfx_form_destroy);
+    _rank = (kernel % 2 == 0) ? 2 : 1;
+  }
+  dfx_form* handle() const { return _h.get(); }
+  int rank() const { return _rank; }
+  std::vector<std::shared_ptr<const FunctionSpace>> function_spaces() const
+  {
+    return std::vector<std::shared_ptr<const FunctionSpace>>(_rank, _V);
+  }
+  /// Pick the assembly kernel variant for the pattern of A by timing
+  std::pair<int, float> autotune(la::MatrixCSR<T>& A, const DirichletBC<T>* bc = nullptr)
+  {
+    std::int32_t v = 0;
+    float ms = 0;
+    check(dfx_form_autotune(_h.get(), A.handle(), bc ? bc->handle() : nullptr, &v, &ms));
+    return {v, ms};
+  }
+
+private:
+  std::shared_ptr<const FunctionSpace> _V;
+  std::shared_ptr<dfx_form> _h;
+  int _rank = 2;
+};
+
+/// fem::create_sparsity_pattern (fem/utils.h:197-218)
+template <typename T>
+la::SparsityPattern create_sparsity_pattern(const Form<T>& a)
+{
+  auto V = a.function_spaces().at(0);
+  dfx_sparsity* h = nullptr;
+  check(dfx_sparsity_create(V->handle(), V->handle(), &h));
+  return la::SparsityPattern(h);
+}
+
+template <typename T>
+using BCs = std::vector<std::reference_wrapper<const DirichletBC<T>>>;
+
+/// fem::assemble_matrix(mat_add, a, bcs) (fem/assembler.h:542-602)
+template <typename T>
+void assemble_matrix(la::DeviceMatAdd<T> mat_add, const Form<T>& a, const BCs<T>& bcs = {})
+{
+  if (bcs.size() > 1)
+    throw std::runtime_error("pass the union of Dirichlet dofs as one DirichletBC");
+  check(dfx_assemble_matrix(mat_add.A, a.handle(), bcs.empty() ? nullptr : bcs[0].get().handle()));
+}
+
+/// fem::set_diagonal(set_fn, V, bcs, diagonal) (fem/assembler.h:671-686)
+template <typename T>
+void set_diagonal(la::DeviceMatSet<T> mat_set, const FunctionSpace&, const BCs<T>& bcs,
+                  T diagonal = 1)
+{
+  for (auto& bc : bcs)
+    check(dfx_set_diagonal(mat_set.A, bc.get().handle(), diagonal));
+}
+
+/// fem::assemble_vector(b, L) (fem/assembler.h:247-257): b += L
+template <typename T>
+void assemble_vector(la::Vector<T>& b, const Form<T>& L)
+{
+  check(dfx_assemble_vector(b.handle(), L.handle()));
+}
+
+/// fem::apply_lifting(b, a, bcs1, x0, alpha) (fem/assembler.h:451-493)
+template <typename T>
+void apply_lifting(la::Vector<T>& b, const std::vector<std::reference_wrapper<const Form<T>>>& a,
+                   const std::vector<BCs<T>>& bcs1,
+                   const std::vector<std::reference_wrapper<const la::Vector<T>>>& x0, T alpha)
+{
+  if (!x0.empty() && x0.size() != a.size())
+    throw std::runtime_error("Mismatch in size between x0 and bilinear form in assembler.");
+  if (a.size() != bcs1.size())
+    throw std::runtime_error("Mismatch in size between a and bcs in assembler.");
+  for (std::size_t i = 0; i < a.size(); ++i)
+    for (auto& bc : bcs1[i])
+      check(dfx_apply_lifting(b.handle(), a[i].get().handle(), bc.get().handle(),
+                              x0.empty() ? nullptr : x0[i].get().handle(), alpha));
+}
+} // namespace fem
+
+namespace linalg
+{
+/// CG of cpp/demo/poisson_matrix_free/main.cpp:84-132 on an assembled
+/// device matrix, optionally Jacobi-preconditioned; returns iterations.
+template <typename T>
+int cg(la::Vector<T>& x, const la::Vector<T>& b, const la::MatrixCSR<T>& A, int kmax = 50,
+       double rtol = 1e-8, bool jacobi = false)
+{
+  dfx_cg_opts o{kmax, rtol, jacobi ? 1 : 0, 32};
+  dfx_cg_result r{};
+  check(dfx_cg_solve(A.handle(), b.handle(), x.handle(), &o, &r));
+  return r.iterations;
+}
+} // namespace linalg
+
+} // namespace dolfinx_amd
