@@ -1055,6 +1055,393 @@ __global__ void __launch_bounds__(kBlock)
   b[r] += s;
 }
 
+// ---------------------------------------------------------------------
+// Lagrange elements of degree 2 and 3 on the reference tetrahedron, for the
+// reference tensors of the P2 elasticity and P3 forms. Nodes in the Basix
+// order (vertices; edge interiors from the edge's first vertex to its
+// second, at GLL points for degree 3, Basix's default gll_warped variant;
+// face centroids for degree 3); basis in monomial coefficients from the
+// Vandermonde matrix; integrals exact from int xi^a eta^b zeta^c =
+// a! b! c! / (a+b+c+3)! over the reference cell.
+// ---------------------------------------------------------------------
+
+struct TetBasis
+{
+  int n = 0;
+  std::vector<std::array<int, 3>> mono; // exponents of the n monomials
+  std::vector<double> C;                // C[m * n + i]: coefficient of mono m in phi_i
+};
+
+double factorial_d(int k)
+{
+  double f = 1;
+  for (int i = 2; i <= k; ++i)
+    f *= i;
+  return f;
+}
+
+double mono_integral(int a, int b, int c)
+{
+  return factorial_d(a) * factorial_d(b) * factorial_d(c) / factorial_d(a + b + c + 3);
+}
+
+TetBasis tet_basis(int degree)
+{
+  static constexpr int E[6][2] = {{2, 3}, {1, 3}, {1, 2}, {0, 3}, {0, 2}, {0, 1}};
+  static constexpr int F[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+  const double V[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  std::vector<std::array<double, 3>> pts;
+  for (auto& v : V)
+    pts.push_back({v[0], v[1], v[2]});
+  std::vector<double> t;
+  if (degree == 2)
+    t = {0.5};
+  if (degree == 3)
+    t = {0.5 - 0.5 / std::sqrt(5.0), 0.5 + 0.5 / std::sqrt(5.0)};
+  for (auto& e : E)
+    for (double s : t)
+      pts.push_back({V[e[0]][0] + s * (V[e[1]][0] - V[e[0]][0]),
+                     V[e[0]][1] + s * (V[e[1]][1] - V[e[0]][1]),
+                     V[e[0]][2] + s * (V[e[1]][2] - V[e[0]][2])});
+  if (degree == 3)
+    for (auto& f : F)
+      pts.push_back({(V[f[0]][0] + V[f[1]][0] + V[f[2]][0]) / 3,
+                     (V[f[0]][1] + V[f[1]][1] + V[f[2]][1]) / 3,
+                     (V[f[0]][2] + V[f[1]][2] + V[f[2]][2]) / 3});
+  TetBasis B;
+  B.n = static_cast<int>(pts.size());
+  for (int s = 0; s <= degree; ++s)
+    for (int a = s; a >= 0; --a)
+      for (int b = s - a; b >= 0; --b)
+        B.mono.push_back({a, b, s - a - b});
+  const int n = B.n;
+  // Solve V C = I with V[p][m] = mono_m(point p) (Gauss-Jordan, pivoting)
+  std::vector<double> A(n * n), X(n * n, 0.0);
+  for (int p = 0; p < n; ++p)
+    for (int m = 0; m < n; ++m)
+      A[p * n + m] = std::pow(pts[p][0], B.mono[m][0]) * std::pow(pts[p][1], B.mono[m][1])
+                     * std::pow(pts[p][2], B.mono[m][2]);
+  // we need C with sum_m C[m][i] mono_m(p) = delta_ip, i.e. V C = I
+  for (int i = 0; i < n; ++i)
+    X[i * n + i] = 1.0;
+  for (int col = 0; col < n; ++col)
+  {
+    int piv = col;
+    for (int r = col + 1; r < n; ++r)
+      if (std::abs(A[r * n + col]) > std::abs(A[piv * n + col]))
+        piv = r;
+    for (int k = 0; k < n; ++k)
+    {
+      std::swap(A[col * n + k], A[piv * n + k]);
+      std::swap(X[col * n + k], X[piv * n + k]);
+    }
+    const double d = A[col * n + col];
+    for (int k = 0; k < n; ++k)
+    {
+      A[col * n + k] /= d;
+      X[col * n + k] /= d;
+    }
+    for (int r = 0; r < n; ++r)
+      if (r != col && A[r * n + col] != 0.0)
+      {
+        const double f = A[r * n + col];
+        for (int k = 0; k < n; ++k)
+        {
+          A[r * n + k] -= f * A[col * n + k];
+          X[r * n + k] -= f * X[col * n + k];
+        }
+      }
+  }
+  B.C = X; // X = V^{-1}: X[m * n + i]
+  return B;
+}
+
+/// int (d_a phi_i)(d_b phi_j) (a, b = -1: no derivative) over the
+/// reference tetrahedron.
+double basis_product_integral(const TetBasis& B, int i, int a, int j, int b)
+{
+  const int n = B.n;
+  double s = 0;
+  for (int m = 0; m < n; ++m)
+  {
+    std::array<int, 3> e = B.mono[m];
+    double cm = B.C[m * n + i];
+    if (cm == 0.0)
+      continue;
+    if (a >= 0)
+    {
+      if (e[a] == 0)
+        continue;
+      cm *= e[a];
+      e[a] -= 1;
+    }
+    for (int q = 0; q < n; ++q)
+    {
+      std::array<int, 3> f = B.mono[q];
+      double cq = B.C[q * n + j];
+      if (cq == 0.0)
+        continue;
+      if (b >= 0)
+      {
+        if (f[b] == 0)
+          continue;
+        cq *= f[b];
+        f[b] -= 1;
+      }
+      s += cm * cq * mono_integral(e[0] + f[0], e[1] + f[1], e[2] + f[2]);
+    }
+  }
+  return s;
+}
+
+double basis_integral(const TetBasis& B, int i)
+{
+  double s = 0;
+  for (int m = 0; m < B.n; ++m)
+    s += B.C[m * B.n + i] * mono_integral(B.mono[m][0], B.mono[m][1], B.mono[m][2]);
+  return s;
+}
+
+/// [7][k][k] tensors of the affine stiffness + reaction form of a basis.
+std::vector<double> scalar_tensors(const TetBasis& B)
+{
+  const int k = B.n;
+  std::vector<double> t(7 * k * k);
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j)
+    {
+      const int e = i * k + j;
+      auto S = [&](int a, int b) { return basis_product_integral(B, i, a, j, b); };
+      t[0 * k * k + e] = S(0, 0);
+      t[1 * k * k + e] = S(1, 1);
+      t[2 * k * k + e] = S(2, 2);
+      t[3 * k * k + e] = S(0, 1) + S(1, 0);
+      t[4 * k * k + e] = S(0, 2) + S(2, 0);
+      t[5 * k * k + e] = S(1, 2) + S(2, 1);
+      t[6 * k * k + e] = basis_product_integral(B, i, -1, j, -1);
+    }
+  return t;
+}
+
+/// [3][3][k][k] tensors S_ab(i, j) = int d_a phi_i d_b phi_j (elasticity).
+std::vector<double> gradient_tensors(const TetBasis& B)
+{
+  const int k = B.n;
+  std::vector<double> t(9 * k * k);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j)
+          t[((a * 3 + b) * k + i) * k + j] = basis_product_integral(B, i, a, j, b);
+  return t;
+}
+
+// ---------------------------------------------------------------------
+// P2 vector elasticity, a = sigma(u):eps(v) with DG0 lambda, mu. Per cell:
+// J^{-1}, |det J|, lambda, mu (16 values). Entry block (i, j) (3 x 3, row
+// component c, column component d) with P = J^{-T}-transformed reference
+// gradient products, P_cd = sum_ab Kinv[a][c] S_ab(i,j) Kinv[b][d]:
+//   A_(i,c)(j,d) = |det J| (mu (delta_cd tr P + P_dc) + lambda P_cd).
+// ---------------------------------------------------------------------
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    elast_factors_k(std::int32_t nc, const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
+                    const std::int32_t* __restrict__ dm, const T* __restrict__ coeffs, int cstride,
+                    const std::int8_t* __restrict__ bc, T* __restrict__ cd,
+                    std::uint32_t* __restrict__ mask)
+{
+  const std::int64_t c
+      = xcd_contiguous(blockIdx.x, gridDim.x) * std::int64_t(kBlock) + threadIdx.x;
+  if (c >= nc)
+    return;
+  const std::int32_t* v = xdm + 4 * c;
+  const T* q0 = x + 3 * std::int64_t(v[0]);
+  const T* q1 = x + 3 * std::int64_t(v[1]);
+  const T* q2 = x + 3 * std::int64_t(v[2]);
+  const T* q3 = x + 3 * std::int64_t(v[3]);
+  T J[9], adj[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+  {
+    J[3 * r + 0] = q1[r] - q0[r];
+    J[3 * r + 1] = q2[r] - q0[r];
+    J[3 * r + 2] = q3[r] - q0[r];
+  }
+  const T det = adjugate(J, adj);
+  T* o = cd + 16 * c;
+#pragma unroll
+  for (int q = 0; q < 9; ++q)
+    o[q] = adj[q] / det;
+  o[9] = det < T(0) ? -det : det;
+  o[10] = coeffs[c * cstride];     // lambda
+  o[11] = coeffs[c * cstride + 1]; // mu
+  if (mask)
+  {
+    std::uint32_t m = 0;
+    for (int j = 0; j < 10; ++j)
+    {
+      const std::int32_t node = dm[c * 10 + j];
+      for (int b = 0; b < 3; ++b)
+        if (bc[3 * std::int64_t(node) + b])
+          m |= 1u << (3 * j + b);
+    }
+    mask[c] = m;
+  }
+}
+
+/// L lanes per block row (one incidence per lane); 3 x 3 blocks added into
+/// the row's LDS slice with LDS atomics.
+template <typename T, typename O, int L>
+__global__ void __launch_bounds__(kBlock)
+    elast_rows_k(std::int32_t nrows, std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
+                 const std::int32_t* __restrict__ inc, const O* __restrict__ islots,
+                 const T* __restrict__ cd, const std::uint32_t* __restrict__ mask,
+                 const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
+                 const T* __restrict__ ref, T* __restrict__ vals, int accumulate)
+{
+  constexpr int K = 10;
+  extern __shared__ unsigned char smem_raw[];
+  T* S = reinterpret_cast<T*>(smem_raw); // [9][K][K]
+  T* acc = S + 9 * K * K;
+  constexpr int RPB = kBlock / L;
+  const std::int32_t r0 = xcd_contiguous(blockIdx.x, gridDim.x) * RPB;
+  const std::int32_t r1 = min(r0 + RPB, nrows);
+  const std::int32_t base = rp[r0];
+  const std::int32_t len = (rp[r1] - base) * 9;
+  for (int t = threadIdx.x; t < 9 * K * K; t += kBlock)
+    S[t] = ref[t];
+  for (int t = threadIdx.x; t < len; t += kBlock)
+    acc[t] = T(0);
+  __syncthreads();
+  const std::int32_t r = r0 + threadIdx.x / L;
+  const int lane = threadIdx.x % L;
+  if (r < r1 && r < nplan)
+  {
+    std::uint32_t rowbc = 0;
+    if (bc)
+      for (int cc = 0; cc < 3; ++cc)
+        if (bc[3 * std::int64_t(r) + cc])
+          rowbc |= 1u << cc;
+    T* a = acc + (rp[r] - base) * 9;
+    const std::int32_t pe = inc_off[r + 1];
+    for (std::int32_t p = inc_off[r] + lane; p < pe && rowbc != 7u; p += L)
+    {
+      const std::int32_t cik = inc[p];
+      const std::int64_t c = cik / K;
+      const int i = cik - static_cast<int>(c) * K;
+      const T* o = cd + 16 * c;
+      T Ki[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q)
+        Ki[q] = o[q];
+      const T adet = o[9], lam = o[10], mu = o[11];
+      const std::uint32_t m = mask ? mask[c] : 0u;
+      O sl[K];
+      load_slots<O, K>(islots, p, sl);
+      for (int j = 0; j < K; ++j)
+      {
+        T Tm[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+          Tm[q] = S[(q * K + i) * K + j];
+        // W = T Kinv, P = Kinv^T W
+        T W[9], P[9];
+#pragma unroll
+        for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            W[aa * 3 + d] = Tm[aa * 3 + 0] * Ki[0 * 3 + d] + Tm[aa * 3 + 1] * Ki[1 * 3 + d]
+                            + Tm[aa * 3 + 2] * Ki[2 * 3 + d];
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            P[cc * 3 + d] = Ki[0 * 3 + cc] * W[0 * 3 + d] + Ki[1 * 3 + cc] * W[1 * 3 + d]
+                            + Ki[2 * 3 + cc] * W[2 * 3 + d];
+        const T tr = P[0] + P[4] + P[8];
+        T* blk = a + 9 * std::int32_t(sl[j]);
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+        {
+          if ((rowbc >> cc) & 1u)
+            continue;
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+          {
+            if ((m >> (3 * j + d)) & 1u)
+              continue;
+            const T v = adet * (mu * ((cc == d ? tr : T(0)) + P[d * 3 + cc]) + lam * P[cc * 3 + d]);
+            atomicAdd(blk + cc * 3 + d, v);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (accumulate)
+    for (int t = threadIdx.x; t < len; t += kBlock)
+      vals[std::int64_t(base) * 9 + t] += acc[t];
+  else
+    for (int t = threadIdx.x; t < len; t += kBlock)
+      vals[std::int64_t(base) * 9 + t] = acc[t];
+}
+
+/// Element vectors of the tetrahedral source forms with constant data,
+/// one thread per cell, stored in incidence order:
+/// P2 body force b_(i,c) = |det J| w_i f_c; P3 source b_i = |det J| w_i f
+/// (w_i = int phi_i over the reference cell).
+template <typename T, int K, int BS>
+__global__ void __launch_bounds__(kBlock)
+    cell_const_source_k(std::int32_t nc, const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
+                        const T* __restrict__ w, T f0, T f1, T f2,
+                        const std::int32_t* __restrict__ inc_pos, T* __restrict__ be)
+{
+  const std::int64_t c = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (c >= nc)
+    return;
+  const std::int32_t* v = xdm + 4 * c;
+  const T* q0 = x + 3 * std::int64_t(v[0]);
+  const T* q1 = x + 3 * std::int64_t(v[1]);
+  const T* q2 = x + 3 * std::int64_t(v[2]);
+  const T* q3 = x + 3 * std::int64_t(v[3]);
+  T J[9], adj[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+  {
+    J[3 * r + 0] = q1[r] - q0[r];
+    J[3 * r + 1] = q2[r] - q0[r];
+    J[3 * r + 2] = q3[r] - q0[r];
+  }
+  const T det = adjugate(J, adj);
+  const T adet = det < T(0) ? -det : det;
+  const T f[3] = {f0, f1, f2};
+  for (int i = 0; i < K; ++i)
+  {
+    const std::int64_t p = inc_pos[c * K + i];
+#pragma unroll
+    for (int b = 0; b < BS; ++b)
+      be[p * BS + b] = adet * w[i] * f[b];
+  }
+}
+
+/// b[BS r + comp] += sum of the row's incidence entries (incidence order).
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock)
+    row_vector_bs_k(std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
+                    const T* __restrict__ be, T* __restrict__ b)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t >= std::int64_t(nplan) * BS)
+    return;
+  const std::int32_t r = static_cast<std::int32_t>(t / BS);
+  const int comp = static_cast<int>(t % BS);
+  T s = 0;
+  for (std::int32_t p = inc_off[r]; p < inc_off[r + 1]; ++p)
+    s += be[std::int64_t(p) * BS + comp];
+  b[t] += s;
+}
+
 /// Reference tensors of the affine stiffness + reaction forms, as
 /// [7][k][k]: the symmetrised stiffness integrals S00, S11, S22, S01+S10,
 /// S02+S20, S12+S21 (S_ab(i,j) = int d_a phi_i d_b phi_j over the reference
@@ -1106,13 +1493,26 @@ std::vector<double> reference_tensors(int kernel)
       }
     return t;
   }
+  if (kernel == DFX_K_P3TET_A)
+    return scalar_tensors(tet_basis(3));
+  if (kernel == DFX_K_P2TET_ELAST_A)
+    return gradient_tensors(tet_basis(2));
+  if (kernel == DFX_K_P2TET_ELAST_L || kernel == DFX_K_P3TET_L)
+  {
+    const TetBasis B = tet_basis(kernel == DFX_K_P3TET_L ? 3 : 2);
+    std::vector<double> w(B.n);
+    for (int i = 0; i < B.n; ++i)
+      w[i] = basis_integral(B, i);
+    return w;
+  }
   return {};
 }
 
 bool row_path(const dfx_form& f)
 {
   return f.affine && f.ref_tensors
-         && (f.d.kernel == DFX_K_P1TET_A || f.d.kernel == DFX_K_Q1HEX_A);
+         && (f.d.kernel == DFX_K_P1TET_A || f.d.kernel == DFX_K_Q1HEX_A
+             || f.d.kernel == DFX_K_P3TET_A || f.d.kernel == DFX_K_P2TET_ELAST_A);
 }
 
 /// Incidence slot offsets of form f for the pattern of A (built once).
@@ -1200,11 +1600,33 @@ void launch_lanes(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
         A.zeroed ? 0 : 1);
 }
 
+template <typename T, typename O>
+void launch_elast(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
+{
+  const std::int32_t nc = f.d.num_cells;
+  elast_factors_k<T><<<blocks_for(nc), kBlock, 0, stream()>>>(
+      nc, static_cast<const T*>(f.x), f.x_dofmap, f.dofmap, static_cast<const T*>(f.coeffs),
+      f.d.cstride, bc ? bc->marker : nullptr, static_cast<T*>(f.cell_data),
+      bc ? f.cell_mask : nullptr);
+  constexpr int L = 32;
+  constexpr int RPB = kBlock / L;
+  const std::size_t lds = (9 * 100 + std::size_t(RPB) * A.max_row * 9) * sizeof(T);
+  require(lds <= 64 * 1024, "row too long for the LDS slice");
+  const int nb = (A.nrows + RPB - 1) / RPB;
+  if (nb > 0)
+    elast_rows_k<T, O, L><<<nb, kBlock, lds, stream()>>>(
+        A.nrows, f.max_dof + 1, f.inc_off, f.inc, static_cast<const O*>(f.islots),
+        static_cast<const T*>(f.cell_data), bc ? f.cell_mask : nullptr, bc ? bc->marker : nullptr,
+        A.row_ptr, static_cast<const T*>(f.ref_tensors), static_cast<T*>(A.values),
+        A.zeroed ? 0 : 1);
+  check_launch("elasticity assembly");
+}
+
 template <typename T, typename O, int K>
 void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
 {
   launch_cell_factors<T>(f, bc);
-  const int variant = f.row_variant;
+  const int variant = K > 8 ? 3 : f.row_variant; // wide elements: lanes only
   if (variant < 3)
   {
     const int R = 64 << variant;
@@ -1243,19 +1665,33 @@ void assemble_rows(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
 {
   ensure_plan(f, A);
   const int K = f.d.dofs_per_cell;
-  if (K == 4)
+  if (f.d.kernel == DFX_K_P2TET_ELAST_A)
+  {
+    if (f.islot_bytes == 1)
+      launch_elast<T, std::uint8_t>(f, A, bc);
+    else
+      launch_elast<T, std::uint16_t>(f, A, bc);
+  }
+  else if (K == 4)
   {
     if (f.islot_bytes == 1)
       launch_row_assemble<T, std::uint8_t, 4>(f, A, bc);
     else
       launch_row_assemble<T, std::uint16_t, 4>(f, A, bc);
   }
-  else
+  else if (K == 8)
   {
     if (f.islot_bytes == 1)
       launch_row_assemble<T, std::uint8_t, 8>(f, A, bc);
     else
       launch_row_assemble<T, std::uint16_t, 8>(f, A, bc);
+  }
+  else
+  {
+    if (f.islot_bytes == 1)
+      launch_row_assemble<T, std::uint8_t, 20>(f, A, bc);
+    else
+      launch_row_assemble<T, std::uint16_t, 20>(f, A, bc);
   }
 }
 
@@ -1269,8 +1705,11 @@ void lift_rows(dfx_form& f, const dfx_bc& bc, const T* x0, double alpha, T* b)
   if (f.d.dofs_per_cell == 4)
     row_lift_k<T, 4><<<blocks_for(nplan), kBlock, 0, stream()>>>(
         nplan, f.inc_off, f.inc, f.dofmap, cd, f.cell_mask, ref, bc.g, x0, alpha, b);
-  else
+  else if (f.d.dofs_per_cell == 8)
     row_lift_k<T, 8><<<blocks_for(nplan), kBlock, 0, stream()>>>(
+        nplan, f.inc_off, f.inc, f.dofmap, cd, f.cell_mask, ref, bc.g, x0, alpha, b);
+  else
+    row_lift_k<T, 20><<<blocks_for(nplan), kBlock, 0, stream()>>>(
         nplan, f.inc_off, f.inc, f.dofmap, cd, f.cell_mask, ref, bc.g, x0, alpha, b);
   check_launch("row_lift");
 }
@@ -1281,6 +1720,27 @@ void vector_rows(const dfx_form& f, T* b)
   const std::int32_t nc = f.d.num_cells;
   const int nv = f.d.nodes_per_cell;
   T* be = static_cast<T*>(f.cell_vec);
+  const std::int32_t np = f.max_dof + 1;
+  if (f.d.kernel == DFX_K_P2TET_ELAST_L)
+  {
+    cell_const_source_k<T, 10, 3><<<blocks_for(nc), kBlock, 0, stream()>>>(
+        nc, static_cast<const T*>(f.x), f.x_dofmap, static_cast<const T*>(f.ref_tensors),
+        static_cast<T>(f.constants[0]), static_cast<T>(f.constants[1]),
+        static_cast<T>(f.constants[2]), f.inc_pos, be);
+    row_vector_bs_k<T, 3><<<blocks_for(std::int64_t(np) * 3), kBlock, 0, stream()>>>(
+        np, f.inc_off, be, b);
+    check_launch("assemble_vector");
+    return;
+  }
+  if (f.d.kernel == DFX_K_P3TET_L)
+  {
+    cell_const_source_k<T, 20, 1><<<blocks_for(nc), kBlock, 0, stream()>>>(
+        nc, static_cast<const T*>(f.x), f.x_dofmap, static_cast<const T*>(f.ref_tensors),
+        static_cast<T>(f.constants[0]), T(0), T(0), f.inc_pos, be);
+    row_vector_bs_k<T, 1><<<blocks_for(np), kBlock, 0, stream()>>>(np, f.inc_off, be, b);
+    check_launch("assemble_vector");
+    return;
+  }
   if (nv == 4)
     cell_source_k<T, 4><<<blocks_for(nc), kBlock, 0, stream()>>>(
         nc, static_cast<const T*>(f.x), f.x_dofmap, static_cast<const T*>(f.coeffs),
@@ -1483,14 +1943,20 @@ dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out)
           const std::size_t ts = dtype_size(d.dtype);
           if (ki.bilinear)
           {
-            f->cell_data = dmalloc(ts * 8 * std::max<std::int64_t>(nc, 1));
+            const int per_cell = d.kernel == DFX_K_P2TET_ELAST_A ? 16 : 8;
+            f->cell_data = dmalloc(ts * per_cell * std::max<std::int64_t>(nc, 1));
             f->cell_mask = static_cast<std::uint32_t*>(dmalloc(sizeof(std::uint32_t) * std::max<std::int64_t>(nc, 1)));
             const std::vector<double> ref = reference_tensors(d.kernel);
             if (!ref.empty())
               f->ref_tensors = upload_as(ref.data(), static_cast<std::int64_t>(ref.size()), d.dtype);
           }
           else
+          {
             f->cell_vec = dmalloc(ts * std::max<std::int64_t>(nc * k * d.bs, 1));
+            const std::vector<double> w = reference_tensors(d.kernel);
+            if (!w.empty())
+              f->ref_tensors = upload_as(w.data(), static_cast<std::int64_t>(w.size()), d.dtype);
+          }
         }
         *out = f.release();
       });
@@ -1737,7 +2203,8 @@ dfx_status dfx_assemble_vector(dfx_vec* b, const dfx_form* L)
         require(!ki.bilinear, "assemble_vector needs a linear form");
         require(b->dtype == L->d.dtype, "vector and form dtypes differ");
         require(std::int64_t(L->max_dof + 1) * L->d.bs <= b->n, "dofmap exceeds the vector size");
-        if (L->d.kernel == DFX_K_P1TET_L || L->d.kernel == DFX_K_Q1HEX_L)
+        if (L->d.kernel == DFX_K_P1TET_L || L->d.kernel == DFX_K_Q1HEX_L
+            || L->d.kernel == DFX_K_P2TET_ELAST_L || L->d.kernel == DFX_K_P3TET_L)
         {
           if (b->dtype == DFX_F64)
             vector_rows<double>(*L, static_cast<double*>(b->data));
@@ -1772,6 +2239,9 @@ dfx_status dfx_apply_lifting(dfx_vec* b, const dfx_form* a, const dfx_bc* bc, co
         if (!bc->g && !x0)
           return;
         dfx_form& f = const_cast<dfx_form&>(*a);
+        require(f.d.kernel != DFX_K_P2TET_ELAST_A,
+                "apply_lifting with nonzero Dirichlet values is not implemented for the "
+                "P2 elasticity form (g = 0 and x0 = 0 are)");
         if (row_path(f))
         {
           if (b->dtype == DFX_F64)

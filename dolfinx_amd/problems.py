@@ -87,3 +87,54 @@ def assemble_system(p: ScalarProblem, A: la.MatrixCSR | None = None, b: la.Vecto
     fem.apply_lifting(b, [p.a], [[p.bc]])
     fem.set_bc(b, [p.bc])
     return A, b
+
+
+@dataclass
+class VectorProblem:
+    mesh: mesh.Mesh
+    V: fem.FunctionSpace
+    a: fem.Form
+    L: fem.Form
+    bc: fem.DirichletBC
+    E: np.ndarray           # Young's modulus per input cell
+    dtype: np.dtype
+
+
+def lame_from_E(E, nu=0.3):
+    """Lame parameters scaled by E (fixed Poisson ratio nu)."""
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    mu = E / (2.0 * (1.0 + nu))
+    return lam, mu
+
+
+def elasticity_problem(n, seed=0, dtype=np.float64, msh=None, V=None) -> VectorProblem:
+    """P2 vector elasticity on create_box tetrahedra: sigma(u):eps(v) with
+    DG0 lambda, mu from E ~ U[0.5, 2] per cell (nu = 0.3), body force
+    (0, 0, -1), the x = 0 face clamped."""
+    if msh is None:
+        msh = mesh.create_box([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], n, mesh.CellType.tetrahedron)
+    if V is None:
+        V = fem.functionspace(msh, ("Lagrange", 2, (3,)))
+    nc = msh.topology.cells.shape[0]
+    rng = np.random.Generator(np.random.PCG64(seed))
+    E = rng.uniform(0.5, 2.0, nc)
+    lam, mu = lame_from_E(E)
+    a = fem.form("p2_elasticity", V, fem.pack_cellwise(msh, np.stack([lam, mu], 1)), None, dtype)
+    L = fem.form("p2_body_force", V, None, [0.0, 0.0, -1.0], dtype)
+    bc = fem.dirichletbc(V, fem.locate_boundary_dofs(V, axis=0, value=0.0))
+    return VectorProblem(msh, V, a, L, bc, E, np.dtype(dtype))
+
+
+def p3_problem(n, seed=0, dtype=np.float64, c=1.0, msh=None, V=None) -> ScalarProblem:
+    """P3 reaction-diffusion on create_box tetrahedra: kappa grad u.grad v
+    + c u v with kappa ~ U[1, 10] per cell, f = 1, no Dirichlet condition."""
+    if msh is None:
+        msh = mesh.create_box([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], n, mesh.CellType.tetrahedron)
+    if V is None:
+        V = fem.functionspace(msh, ("Lagrange", 3))
+    nc = msh.topology.cells.shape[0]
+    kap = kappa_field("uniform_1_10", nc, seed)
+    a = fem.form("p3_stiffness_reaction", V, fem.pack_cellwise(msh, kap), [c], dtype)
+    L = fem.form("p3_source", V, None, [1.0], dtype)
+    bcs = fem.dirichletbc(V, np.zeros(0, np.int32))
+    return ScalarProblem(msh, V, a, L, bcs, kap, c, np.ones(1), np.dtype(dtype))
