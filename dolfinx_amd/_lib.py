@@ -74,6 +74,7 @@ SIGNATURES = {
     "dfx_sparsity_sizes": (C.c_int, [P, P, P, P]),
     "dfx_sparsity_copy": (C.c_int, [P, P, P]),
     "dfx_csr_create": (C.c_int, [P, P, I32, I32, C.c_int, C.c_int, PP]),
+    "dfx_csr_create_shared": (C.c_int, [P, C.c_int, PP]),
     "dfx_csr_destroy": (C.c_int, [P]),
     "dfx_csr_sizes": (C.c_int, [P, P, P, P, P]),
     "dfx_csr_zero": (C.c_int, [P]),
@@ -107,6 +108,12 @@ SIGNATURES = {
     "dfx_bc_set": (C.c_int, [P, P, P, D]),
     "dfx_cg_solve": (C.c_int, [P, P, P, C.POINTER(CGOpts), C.POINTER(CGResult)]),
     "dfx_mixed_solve": (C.c_int, [P, P, P, P, C.POINTER(MixedOpts), C.POINTER(MixedResult)]),
+    "dfx_batch_create": (C.c_int, [I32, I64, PP]),
+    "dfx_batch_destroy": (C.c_int, [P]),
+    "dfx_batch_add": (C.c_int, [P, P, P, P, D, I32, C.POINTER(I32)]),
+    "dfx_batch_remove": (C.c_int, [P, I32]),
+    "dfx_batch_step": (C.c_int, [P, I32, I32]),
+    "dfx_batch_status": (C.c_int, [P, I32, P, P, P, P, P]),
     "dfx_event_record": (C.c_int, [C.c_int]),
     "dfx_event_elapsed": (C.c_int, [C.c_int, C.c_int, C.POINTER(F)]),
 }

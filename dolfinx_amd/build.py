@@ -20,7 +20,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 HOST_SOURCES = ["setup.cpp"]
-HIP_SOURCES = ["la.hip", "assemble.hip"]
+HIP_SOURCES = ["la.hip", "assemble.hip", "batch.hip"]
 
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

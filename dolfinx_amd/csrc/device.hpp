@@ -141,6 +141,8 @@ struct dfx_csr
   // LDS chunk (or a single longer row); device, nblk + 1 row boundaries.
   std::int32_t* rblk = nullptr;
   std::int32_t nblk = 0;
+  // pattern arrays borrowed from another matrix (not freed by this one)
+  const dfx_csr* pattern_owner = nullptr;
 };
 
 struct dfx_vec

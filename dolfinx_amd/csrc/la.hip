@@ -993,14 +993,45 @@ dfx_status dfx_csr_create(const int64_t* row_ptr, const int32_t* cols, int32_t n
       });
 }
 
+dfx_status dfx_csr_create_shared(const dfx_csr* P, int dtype, dfx_csr** out)
+{
+  return guard(
+      [&]
+      {
+        require(P && out, "null argument");
+        require(dtype == DFX_F32 || dtype == DFX_F64, "bad dtype");
+        require(!P->pattern_owner, "share the pattern of its owner");
+        auto A = std::make_unique<dfx_csr>();
+        *A = *P;
+        A->dtype = dtype;
+        A->pattern_owner = P;
+        A->lanes = 0;
+        if (dtype != P->dtype)
+        {
+          // the stream SpMV row blocks depend on the value size
+          A->rblk = nullptr;
+          A->nblk = 0;
+        }
+        const std::size_t vb = dtype_size(dtype) * std::size_t(A->nnz) * A->bs * A->bs;
+        A->values = dmalloc(std::max<std::size_t>(vb, 8));
+        DFX_HIP(hipMemsetAsync(A->values, 0, vb, stream()));
+        DFX_HIP(hipStreamSynchronize(stream()));
+        A->zeroed = true;
+        *out = A.release();
+      });
+}
+
 dfx_status dfx_csr_destroy(dfx_csr* A)
 {
   if (A)
   {
-    dfree(A->row_ptr);
-    dfree(A->rblk);
-    dfree(A->cols);
-    dfree(A->diag);
+    if (!A->pattern_owner)
+    {
+      dfree(A->row_ptr);
+      dfree(A->rblk);
+      dfree(A->cols);
+      dfree(A->diag);
+    }
     dfree(A->values);
     delete A;
   }

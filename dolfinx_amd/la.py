@@ -60,6 +60,18 @@ class MatrixCSR:
     def from_sparsity(cls, sp: SparsityPattern, dtype=np.float64):
         return cls(sp.row_ptr, sp.cols, bs=sp.block_size, dtype=dtype)
 
+    @classmethod
+    def shared(cls, pattern: "MatrixCSR", dtype=np.float64):
+        """A matrix with its own values on the device pattern of `pattern`
+        (shared, not copied; `pattern` must outlive it)."""
+        A = cls.__new__(cls)
+        A._h = C.c_void_p()
+        _lib.call("dfx_csr_create_shared", pattern.handle, _lib.code_of(dtype), C.byref(A._h))
+        A._pattern = pattern
+        A.indptr, A.indices, A.bs = pattern.indptr, pattern.indices, pattern.bs
+        A.shape, A.dtype = pattern.shape, np.dtype(dtype)
+        return A
+
     @property
     def handle(self):
         return self._h

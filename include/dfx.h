@@ -122,6 +122,9 @@ typedef struct dfx_vec dfx_vec;
 dfx_status dfx_csr_create(const int64_t* row_ptr, const int32_t* cols,
                           int32_t nrows, int32_t ncols, int bs, int dtype,
                           dfx_csr** out);
+/* A matrix with its own values on the pattern of P (shared, not copied;
+ * P must outlive it): many systems on one mesh keep one pattern in HBM. */
+dfx_status dfx_csr_create_shared(const dfx_csr* P, int dtype, dfx_csr** out);
 dfx_status dfx_csr_destroy(dfx_csr* A);
 dfx_status dfx_csr_sizes(const dfx_csr* A, int32_t* nrows, int64_t* nnz,
                          int32_t* bs, int32_t* dtype);
@@ -289,6 +292,31 @@ typedef struct
 dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32,
                            const dfx_vec* b, dfx_vec* x,
                            const dfx_mixed_opts* opts, dfx_mixed_result* res);
+
+/* ------------------------------------------------------------------ */
+/* Batched solves (no counterpart in the reference).                   */
+/* ------------------------------------------------------------------ */
+typedef struct dfx_batch dfx_batch;
+
+/* Segmented FP64 Jacobi-CG over up to max_systems independent systems
+ * with at most max_rows rows in total. Every system is cut into chunks
+ * that depend only on itself: its iterates are bitwise the same alone and
+ * in any mix. */
+dfx_status dfx_batch_create(int32_t max_systems, int64_t max_rows, dfx_batch** out);
+dfx_status dfx_batch_destroy(dfx_batch* B);
+/* Join system A x = b (x is zeroed; A, b, x must outlive membership);
+ * returns its slot. */
+dfx_status dfx_batch_add(dfx_batch* B, const dfx_csr* A, const dfx_vec* b, dfx_vec* x,
+                         double rtol, int32_t max_it, int32_t* slot);
+dfx_status dfx_batch_remove(dfx_batch* B, int32_t slot);
+/* Queue `iterations` CG iterations for every active system; use_graph
+ * replays them as one captured hipGraph. Asynchronous. */
+dfx_status dfx_batch_step(dfx_batch* B, int32_t iterations, int32_t use_graph);
+/* Per-slot state (synchronises): done, converged, iterations and the
+ * recursive relative residual ||r||/||r0||. Output arrays may be NULL. */
+dfx_status dfx_batch_status(dfx_batch* B, int32_t n, const int32_t* slots, int32_t* done,
+                            int32_t* converged, int32_t* iterations,
+                            double* rel_residual);
 
 /* ------------------------------------------------------------------ */
 /* Timing on the library stream (hipEvent).                            */
