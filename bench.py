@@ -107,6 +107,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -253,6 +256,31 @@ def main():
             "err_mixed_vs_fp64": float(np.linalg.norm(xmix - x64) / np.linalg.norm(x64)),
         }
 
+    serve_out = None
+    if not args.no_serve:
+        from dolfinx_amd import serve
+
+        t_s = time.perf_counter()
+        svc = serve.SolveService(hbm_budget_bytes=16 << 30, max_slots=64, rtol=1e-8,
+                                 iters_per_step=16, sizes=range(8, 49))
+        setup_serve = time.perf_counter() - t_s
+        # warm the graph and every template once
+        svc.run([serve.make_request(10_000 + i, seed=99) for i in range(8)], rate=1000.0)
+        reqs = [serve.make_request(i, seed=2026) for i in range(args.serve_requests)]
+        keep = [0, 1, 2, 3, 4]
+        st = svc.run(reqs, rate=args.serve_rate, keep_solutions=keep)
+        diffs = [float(np.max(np.abs(svc.solve_alone(r) - r.x))) for r in reqs if r.rid in keep]
+        serve_out = st.summary(args.serve_rate)
+        serve_out.update({
+            "workload": "P1 tets n^3, n~U{8..48}, kappa~logN(0,0.5)/cell, c~U[0,10], "
+                        "f~N(0,1)/vertex, u=0 on the boundary, CG+Jacobi to 1e-8",
+            "all_converged": all(r.converged for r in reqs),
+            "mean_iterations": round(float(np.mean([r.iterations for r in reqs])), 1),
+            "alone_vs_mix_max_abs_diff": max(diffs),
+            "template_setup_s": round(setup_serve, 2),
+        })
+        del svc
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, p64.kappa, args.seed)
@@ -299,6 +327,7 @@ def main():
             "f32_frac": round(spmv32_gbs / HBM_PEAK_GBS, 4),
         },
         **out_solve,
+        "serve": serve_out,
         "cpu_baseline": cpu,
         "setup_s": round(setup_s, 2),
     }

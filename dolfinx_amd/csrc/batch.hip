@@ -103,98 +103,101 @@ __global__ void __launch_bounds__(kBlock)
     bcg_spmv_k(const int* nchunk, const ChunkDesc* chunks, const SysDesc* sys, SysState* st,
                double* p_pq)
 {
-  const int t = blockIdx.x;
-  if (t >= *nchunk)
-    return;
-  const ChunkDesc ch = chunks[t];
-  SysState& S = st[ch.sys];
-  if (ch.first && threadIdx.x == 0)
-    S.iter = S.iter_next;
-  if (block_skip(S))
-    return;
-  const SysDesc& D = sys[ch.sys];
-  const std::int32_t row = ch.row0 + threadIdx.x / kLanes;
-  const int lane = threadIdx.x % kLanes;
-  double s = 0;
-  if (row < ch.row1)
+  const int nch = *nchunk;
+  for (int t = blockIdx.x; t < nch; t += gridDim.x)
   {
-    const std::int32_t e = D.rp[row + 1];
-    for (std::int32_t j = D.rp[row] + lane; j < e; j += kLanes)
-      s += D.v[j] * D.p[D.ci[j]];
-  }
+    const ChunkDesc ch = chunks[t];
+    SysState& S = st[ch.sys];
+    if (ch.first && threadIdx.x == 0)
+      S.iter = S.iter_next;
+    if (block_skip(S))
+      continue;
+    const SysDesc& D = sys[ch.sys];
+    const std::int32_t row = ch.row0 + threadIdx.x / kLanes;
+    const int lane = threadIdx.x % kLanes;
+    double s = 0;
+    if (row < ch.row1)
+    {
+      const std::int32_t e = D.rp[row + 1];
+      for (std::int32_t j = D.rp[row] + lane; j < e; j += kLanes)
+        s += D.v[j] * D.p[D.ci[j]];
+    }
 #pragma unroll
-  for (int off = kLanes / 2; off > 0; off >>= 1)
-    s += __shfl_down(s, off, kLanes);
-  double d = 0;
-  if (lane == 0 && row < ch.row1)
-  {
-    D.q[row] = s;
-    d = s * D.p[row];
+    for (int off = kLanes / 2; off > 0; off >>= 1)
+      s += __shfl_down(s, off, kLanes);
+    double d = 0;
+    if (lane == 0 && row < ch.row1)
+    {
+      D.q[row] = s;
+      d = s * D.p[row];
+    }
+    store_block_sum(d, p_pq, t);
   }
-  store_block_sum(d, p_pq, t);
 }
 
 __global__ void __launch_bounds__(kBlock)
     bcg_update_k(const int* nchunk, const ChunkDesc* chunks, const SysDesc* sys,
                  const SysState* st, const double* p_pq, double* p_rz, double* p_rr)
 {
-  const int t = blockIdx.x;
-  if (t >= *nchunk)
-    return;
-  const ChunkDesc ch = chunks[t];
-  const SysState& S = st[ch.sys];
-  if (block_skip(S))
-    return;
-  const SysDesc& D = sys[ch.sys];
-  const double pq = sum_range(p_pq, D.chunk0, D.nchunks);
-  const double alpha = S.rz[S.iter & 1] / pq;
-  double srz = 0, srr = 0;
-  for (std::int32_t i = ch.row0 + threadIdx.x; i < ch.row1; i += kBlock)
+  const int nch = *nchunk;
+  for (int t = blockIdx.x; t < nch; t += gridDim.x)
   {
-    D.x[i] += alpha * D.p[i];
-    const double ri = D.r[i] - alpha * D.q[i];
-    D.r[i] = ri;
-    srz += ri * D.dinv[i] * ri;
-    srr += ri * ri;
+    const ChunkDesc ch = chunks[t];
+    const SysState& S = st[ch.sys];
+    if (block_skip(S))
+      continue;
+    const SysDesc& D = sys[ch.sys];
+    const double pq = sum_range(p_pq, D.chunk0, D.nchunks);
+    const double alpha = S.rz[S.iter & 1] / pq;
+    double srz = 0, srr = 0;
+    for (std::int32_t i = ch.row0 + threadIdx.x; i < ch.row1; i += kBlock)
+    {
+      D.x[i] += alpha * D.p[i];
+      const double ri = D.r[i] - alpha * D.q[i];
+      D.r[i] = ri;
+      srz += ri * D.dinv[i] * ri;
+      srr += ri * ri;
+    }
+    store_block_sum(srz, p_rz, t);
+    store_block_sum(srr, p_rr, t);
   }
-  store_block_sum(srz, p_rz, t);
-  store_block_sum(srr, p_rr, t);
 }
 
 __global__ void __launch_bounds__(kBlock)
     bcg_direction_k(const int* nchunk, const ChunkDesc* chunks, const SysDesc* sys,
                     SysState* st, const double* p_rz, const double* p_rr)
 {
-  const int t = blockIdx.x;
-  if (t >= *nchunk)
-    return;
-  const ChunkDesc ch = chunks[t];
-  SysState& S = st[ch.sys];
-  if (block_skip(S))
-    return;
-  const SysDesc& D = sys[ch.sys];
-  const double rz = sum_range(p_rz, D.chunk0, D.nchunks);
-  const double rr = sum_range(p_rr, D.chunk0, D.nchunks);
-  const int k = S.iter;
-  const bool conv = rr / S.rr0 < S.tol2;
-  const bool stop = conv || k + 1 >= S.max_it;
-  const double beta = rz / S.rz[k & 1];
-  __syncthreads(); // every thread has read the scalars before chunk 0 writes
-  if (ch.first && threadIdx.x == 0)
+  const int nch = *nchunk;
+  for (int t = blockIdx.x; t < nch; t += gridDim.x)
   {
-    S.rz[(k + 1) & 1] = rz;
-    S.rr = rr;
-    S.iter_next = k + 1;
-    if (stop)
+    const ChunkDesc ch = chunks[t];
+    SysState& S = st[ch.sys];
+    if (block_skip(S))
+      continue;
+    const SysDesc& D = sys[ch.sys];
+    const double rz = sum_range(p_rz, D.chunk0, D.nchunks);
+    const double rr = sum_range(p_rr, D.chunk0, D.nchunks);
+    const int k = S.iter;
+    const bool conv = rr / S.rr0 < S.tol2;
+    const bool stop = conv || k + 1 >= S.max_it;
+    const double beta = rz / S.rz[k & 1];
+    __syncthreads(); // every thread has read the scalars before the writes
+    if (ch.first && threadIdx.x == 0)
     {
-      S.converged = conv ? 1 : 0;
-      S.done = 1;
+      S.rz[(k + 1) & 1] = rz;
+      S.rr = rr;
+      S.iter_next = k + 1;
+      if (stop)
+      {
+        S.converged = conv ? 1 : 0;
+        S.done = 1;
+      }
     }
+    if (stop)
+      continue;
+    for (std::int32_t i = ch.row0 + threadIdx.x; i < ch.row1; i += kBlock)
+      D.p[i] = D.dinv[i] * D.r[i] + beta * D.p[i];
   }
-  if (stop)
-    return;
-  for (std::int32_t i = ch.row0 + threadIdx.x; i < ch.row1; i += kBlock)
-    D.p[i] = D.dinv[i] * D.r[i] + beta * D.p[i];
 }
 
 /// Start of a system's solve from x = 0: r = b, p = z = dinv b, partial
@@ -295,16 +298,22 @@ void batch_rebuild_chunks(dfx_batch& B)
   DFX_HIP(hipStreamSynchronize(stream()));
 }
 
+/// Workgroups of the phase kernels: a fixed grid (so that a captured graph
+/// stays valid) striding over the live chunks.
+int phase_grid(const dfx_batch& B)
+{
+  return std::max(1, std::min(B.max_chunks, ctx().num_cu * 4));
+}
+
 void launch_iterations(dfx_batch& B, int iters)
 {
   hipStream_t s = stream();
+  const int g = phase_grid(B);
   for (int i = 0; i < iters; ++i)
   {
-    bcg_spmv_k<<<B.max_chunks, kBlock, 0, s>>>(B.nchunk, B.chunks, B.sys, B.st, B.p_pq);
-    bcg_update_k<<<B.max_chunks, kBlock, 0, s>>>(B.nchunk, B.chunks, B.sys, B.st, B.p_pq, B.p_rz,
-                                                 B.p_rr);
-    bcg_direction_k<<<B.max_chunks, kBlock, 0, s>>>(B.nchunk, B.chunks, B.sys, B.st, B.p_rz,
-                                                    B.p_rr);
+    bcg_spmv_k<<<g, kBlock, 0, s>>>(B.nchunk, B.chunks, B.sys, B.st, B.p_pq);
+    bcg_update_k<<<g, kBlock, 0, s>>>(B.nchunk, B.chunks, B.sys, B.st, B.p_pq, B.p_rz, B.p_rr);
+    bcg_direction_k<<<g, kBlock, 0, s>>>(B.nchunk, B.chunks, B.sys, B.st, B.p_rz, B.p_rr);
   }
 }
 
