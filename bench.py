@@ -307,7 +307,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "q1hex_a_affine_k (FP64 matrix assembly)",
+            "kernel": "cell_factors_k + row_assemble_k (FP64 Q1 matrix assembly, one dfx_assemble_matrix call)",
             "achieved": round(asm_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -126,3 +126,17 @@ def test_stiffness_annihilates_constants():
         y = la.Vector(p.V.num_dofs)
         A.mult(la.Vector.from_array(np.ones(p.V.num_dofs)), y)
         assert np.max(np.abs(y.array)) <= 1e-13 * np.max(np.abs(A.data))
+
+
+@pytest.mark.parametrize("cell,n", CASES[:2])
+def test_functionals_of_the_source_form(cell, n):
+    """sum_i b_i = int f dx: 1 for f = 1 and 1/2 for f = x_0 on the unit cube
+    (python/test/unit/fem/test_assembler.py:67-78)."""
+    p = build(cell, n, np.float64, f="one")
+    b = fem.assemble_vector(p.L)
+    assert abs(b.array.sum() - 1.0) <= 1e-13
+    x0 = np.zeros(p.mesh.geometry.x.shape[0])
+    x0[p.mesh.geometry.input_global_indices] = p.mesh.geometry.x[:, 0]
+    p2 = problems.scalar_problem(n, cell, f=x0, msh=p.mesh, V=p.V)
+    b2 = fem.assemble_vector(p2.L)
+    assert abs(b2.array.sum() - 0.5) <= 1e-13

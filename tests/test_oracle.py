@@ -431,6 +431,24 @@ def test_mass_total_and_stiffness_nullspace():
     assert np.allclose(O.spmv(rp, cols, K, 1, np.ones(nd)), 0, atol=1e-13)
 
 
+@pytest.mark.parametrize("ct,kL", [(O.TET, O.K_P1TET_L), (O.HEX, O.K_Q1HEX_L)])
+def test_source_functionals(ct, kL):
+    """sum b = int f dx = 1 for f = 1 and 1/2 for f = x_0
+    (python/test/unit/fem/test_assembler.py:67-78); P3 source total = |Omega| c."""
+    m = O.Mesh((3, 4, 2), ct)
+    dm, nd = m.dofmap(1)
+    one = np.ones((m.num_cells, m.nvpc))
+    assert np.isclose(O.assemble_vector(kL, m.x, m.cells, dm, 1, one, None, nd).sum(), 1.0,
+                      rtol=1e-14)
+    fx = m.x[:, 0][m.cells]
+    assert np.isclose(O.assemble_vector(kL, m.x, m.cells, dm, 1, fx, None, nd).sum(), 0.5,
+                      rtol=1e-14)
+    if ct == O.TET:
+        dm3, nd3 = m.dofmap(3)
+        b3 = O.assemble_vector(O.K_P3TET_L, m.x, m.cells, dm3, 1, None, np.array([2.5]), nd3)
+        assert np.isclose(b3.sum(), 2.5, rtol=1e-13)
+
+
 def test_insert_outside_pattern_raises():
     m, dm, nd, rp, cols, A, _, _ = assemble_p1((2, 2, 2), with_bc=False)
     diag_rp = np.arange(nd + 1, dtype=np.int64)
