@@ -98,6 +98,75 @@ def cpu_baseline(n, kappa_input, seed):
     }
 
 
+def timed_assembly(assemble, steps):
+    """Seconds per call of `assemble` (device-synchronised, after one warm call)."""
+    assemble()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        assemble()
+    sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def other_configs(steps):
+    """The remaining workloads of the path, each assembled and solved on the
+    GPU: P1 Poisson on 32^3 tetrahedra (CG+Jacobi to 1e-10), P2 vector
+    elasticity on 48^3 tetrahedra (CG+Jacobi to 1e-8) and P3
+    reaction-diffusion on 24^3 tetrahedra (FP64 and FP32 assembly,
+    CG+Jacobi to 1e-8). Assembly rates count scalar nonzeros."""
+    from dolfinx_amd import la, problems, solvers
+    from dolfinx_amd.mesh import CellType
+
+    out = {}
+
+    def run(name, p, rtol, desc, fp32=None):
+        t0 = time.perf_counter()
+        A, b = problems.assemble_system(p)
+        sync()
+        setup = time.perf_counter() - t0
+        variant = p.a.autotune(A, p.bc)
+        s = timed_assembly(lambda: problems.assemble_system(p, A, b), steps)
+        scalar_nnz = A.nnz * A.bs * A.bs
+        x = la.Vector(p.V.num_dofs)
+        r = solvers.cg(A, b, x, rtol=rtol, max_it=50000, jacobi=True, check_every=64)
+        rec = {
+            "workload": desc,
+            "cells": int(p.mesh.topology.cells.shape[0]),
+            "dofs": int(p.V.num_dofs),
+            "scalar_nnz": int(scalar_nnz),
+            "assembly_ms": round(s * 1e3, 4),
+            "assembly_gnnz_s": round(scalar_nnz / s / 1e9, 3),
+            "assembly_variant": variant[0],
+            "cg_rtol": rtol,
+            "cg_s": round(r.seconds, 4),
+            "cg_iterations": r.iterations,
+            "cg_rel_residual": r.true_rel_residual,
+            "first_assembly_s": round(setup, 3),
+        }
+        if fp32 is not None:
+            A32, b32 = problems.assemble_system(fp32)
+            fp32.a.autotune(A32, fp32.bc)
+            s32 = timed_assembly(lambda: problems.assemble_system(fp32, A32, b32), steps)
+            rec["assembly_f32_gnnz_s"] = round(scalar_nnz / s32 / 1e9, 3)
+        out[name] = rec
+        print(f"# {name}: {rec}", file=sys.stderr, flush=True)
+
+    p = problems.scalar_problem((32, 32, 32), CellType.tetrahedron, kappa="one", c=0.0, f="one")
+    run("p1_tet_32", p, 1e-10, "P1 Poisson -lap u = 1 on 32^3 unit-cube tetrahedra, u=0 on the "
+        "boundary")
+    del p
+    p = problems.elasticity_problem((48, 48, 48), seed=2026)
+    run("p2_elasticity_tet_48", p, 1e-8, "P2 vector elasticity on 48^3 tetrahedra, DG0 lambda, "
+        "mu from E~U[0.5,2] (nu=0.3), x=0 clamped, body force (0,0,-1)")
+    del p
+    p = problems.p3_problem((24, 24, 24), seed=2026)
+    p32 = problems.p3_problem((24, 24, 24), seed=2026, msh=p.mesh, V=p.V, dtype=np.float32)
+    run("p3_tet_24", p, 1e-8, "P3 reaction-diffusion kappa grad u.grad v + u v on 24^3 "
+        "tetrahedra, kappa~U[1,10] per cell, f=1", fp32=p32)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +177,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     args = ap.parse_args()
@@ -281,6 +351,11 @@ def main():
         })
         del svc
 
+    configs_out = None
+    if not args.no_configs:
+        del A32, b32
+        configs_out = other_configs(args.steps)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, p64.kappa, args.seed)
@@ -328,6 +403,7 @@ def main():
         },
         **out_solve,
         "serve": serve_out,
+        "other_configs": configs_out,
         "cpu_baseline": cpu,
         "setup_s": round(setup_s, 2),
     }

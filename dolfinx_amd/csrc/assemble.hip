@@ -530,7 +530,8 @@ __global__ void __launch_bounds__(kBlock)
   const std::int32_t p = diag[d / bs];
   if (p < 0)
   {
-    atomicOr(err, 1);
+    if (err)
+      atomicOr(err, 1);
     return;
   }
   const int k = d % bs;
@@ -782,11 +783,49 @@ __device__ inline T element_entry(const T (&g)[7], const T* __restrict__ ref, in
          + g[4] * Si[4 * K * K] + g[5] * Si[5 * K * K] + g[6] * Si[6 * K * K];
 }
 
+/// Entry (i, j) of the affine Q1 element matrix in closed form, from the
+/// cell factors g (coefficients of S00, S11, S22, S01+S10, S02+S20,
+/// S12+S21 and M). On the reference cube, with vertex bits (b0, b1, b2) =
+/// lexicographic vertex index, the 1D factors are m = int phi_a phi_b (1/3
+/// if a = b, else 1/6), k = int phi_a' phi_b' (+1 if a = b, else -1) and
+/// int phi_a' phi_b = s_a / 2 (s_0 = -1, s_1 = +1), so with m_d, k_d taken
+/// on bit d of i and j:
+///   S00 = k0 m1 m2, S11 = m0 k1 m2, S22 = m0 m1 k2, M = m0 m1 m2,
+///   S01 + S10 = m2 s_i0 s_i1 (k0 + k1) / 4   (and cyclically).
+/// gs holds g[3..5] already multiplied by s_i s_i' / 4.
+template <typename T>
+__device__ inline T q1_entry(const T (&g)[7], const T (&gs)[3], int i, int j)
+{
+  const int e = i ^ j;
+  const T third = T(1) / T(3), sixth = T(1) / T(6);
+  const T m0 = (e & 1) ? sixth : third, k0 = (e & 1) ? T(-1) : T(1);
+  const T m1 = (e & 2) ? sixth : third, k1 = (e & 2) ? T(-1) : T(1);
+  const T m2 = (e & 4) ? sixth : third, k2 = (e & 4) ? T(-1) : T(1);
+  const T m01 = m0 * m1;
+  return g[0] * (k0 * m1 * m2) + g[1] * (m0 * k1 * m2) + g[2] * (m01 * k2)
+         + gs[0] * (m2 * (k0 + k1)) + gs[1] * (m1 * (k0 + k2)) + gs[2] * (m0 * (k1 + k2))
+         + g[6] * (m01 * m2);
+}
+
+/// The mixed-derivative factors of q1_entry for row vertex i.
+template <typename T>
+__device__ inline void q1_signed(const T (&g)[7], int i, T (&gs)[3])
+{
+  const T s0 = (i & 1) ? T(1) : T(-1);
+  const T s1 = (i & 2) ? T(1) : T(-1);
+  const T s2 = (i & 4) ? T(1) : T(-1);
+  gs[0] = T(0.25) * s0 * s1 * g[3];
+  gs[1] = T(0.25) * s0 * s2 * g[4];
+  gs[2] = T(0.25) * s1 * s2 * g[5];
+}
+
 /// Variant "rows": one thread per matrix row, R rows per workgroup. The
-/// thread walks the row's incidences four at a time (all global loads of a
+/// thread walks the row's incidences U at a time (all global loads of a
 /// group issued before use) and accumulates into the workgroup's LDS copy
-/// of its CSR slice, which is then stored with coalesced writes.
-template <typename T, typename O, int K>
+/// of its CSR slice, which is then stored with coalesced writes. With CF
+/// (affine Q1 hexahedra) the element entries come from q1_entry instead of the
+/// reference tensors, and LDS holds only the slice.
+template <typename T, typename O, int K, int U, bool CF>
 __global__ void __launch_bounds__(kBlock)
     row_assemble_k(std::int32_t nrows, std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
                    const std::int32_t* __restrict__ inc, const O* __restrict__ islots,
@@ -794,16 +833,18 @@ __global__ void __launch_bounds__(kBlock)
                    const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
                    const T* __restrict__ ref, T* __restrict__ vals, int accumulate)
 {
+  static_assert(!CF || K == 8, "closed form is the Q1 hexahedron");
   extern __shared__ unsigned char smem_raw[];
   T* S = reinterpret_cast<T*>(smem_raw); // reference tensors [7][K][K]
-  T* acc = S + 7 * K * K;
+  T* acc = CF ? S : S + 7 * K * K;
   const int R = blockDim.x;
   const std::int32_t r0 = xcd_contiguous(blockIdx.x, gridDim.x) * R;
   const std::int32_t r1 = min(r0 + R, nrows);
   const std::int32_t base = rp[r0];
   const std::int32_t len = rp[r1] - base;
-  for (int t = threadIdx.x; t < 7 * K * K; t += R)
-    S[t] = ref[t];
+  if constexpr (!CF)
+    for (int t = threadIdx.x; t < 7 * K * K; t += R)
+      S[t] = ref[t];
   for (int t = threadIdx.x; t < len; t += R)
     acc[t] = T(0);
   __syncthreads();
@@ -812,17 +853,17 @@ __global__ void __launch_bounds__(kBlock)
   {
     T* a = acc + (rp[r] - base);
     const std::int32_t pe = inc_off[r + 1];
-    for (std::int32_t pb = inc_off[r]; pb < pe; pb += 4)
+    for (std::int32_t pb = inc_off[r]; pb < pe; pb += U)
     {
-      std::int32_t cik[4];
+      std::int32_t cik[U];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+      for (int u = 0; u < U; ++u)
         cik[u] = pb + u < pe ? inc[pb + u] : -1;
-      T g[4][7];
-      std::uint32_t m[4];
-      O sl[4][K];
+      T g[U][7];
+      std::uint32_t m[U];
+      O sl[U][K];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+      for (int u = 0; u < U; ++u)
       {
         if (cik[u] < 0)
           continue;
@@ -832,15 +873,27 @@ __global__ void __launch_bounds__(kBlock)
         load_slots<O, K>(islots, pb + u, sl[u]);
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+      for (int u = 0; u < U; ++u)
       {
         if (cik[u] < 0)
           continue;
         const int i = cik[u] % K;
+        if constexpr (CF)
+        {
+          T gs[3];
+          q1_signed<T>(g[u], i, gs);
 #pragma unroll
-        for (int j = 0; j < K; ++j)
-          if (!((m[u] >> j) & 1u))
-            a[sl[u][j]] += element_entry<T, K>(g[u], S, i, j);
+          for (int j = 0; j < 8; ++j)
+            if (!((m[u] >> j) & 1u))
+              a[sl[u][j]] += q1_entry<T>(g[u], gs, i, j);
+        }
+        else
+        {
+#pragma unroll
+          for (int j = 0; j < K; ++j)
+            if (!((m[u] >> j) & 1u))
+              a[sl[u][j]] += element_entry<T, K>(g[u], S, i, j);
+        }
       }
     }
   }
@@ -950,14 +1003,169 @@ __global__ void __launch_bounds__(kBlock)
     b[r] = static_cast<T>(double(b[r]) - alpha * s);
 }
 
+typedef double dfx_double4 __attribute__((ext_vector_type(4)));
+typedef float dfx_float4 __attribute__((ext_vector_type(4)));
+
+/// One 16x16x4 matrix-core product-accumulate, acc += A B, in FP64 or
+/// FP32. Operand layout (one wavefront, lane l): a = A[l % 16][l / 16],
+/// b = B[l / 16][l % 16]; acc[r] = D[mfma16_row<T>(l / 16, r)][l % 16]:
+/// row 4 (l / 16) + r in FP32, row 4 r + l / 16 in FP64.
+__device__ inline dfx_double4 mfma16(double a, double b, dfx_double4 acc)
+{
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+}
+__device__ inline dfx_float4 mfma16(float a, float b, dfx_float4 acc)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+}
+template <typename T>
+__device__ inline int mfma16_row(int kq, int r)
+{
+  return sizeof(T) == 8 ? 4 * r + kq : 4 * kq + r;
+}
+
+/// Element matrices of all cells as one batched product on the matrix
+/// cores: E (cells x K*K) = F (cells x 8) * T (8 x K*K), where F holds the
+/// cell factors (G_0..G_5, c|det J|, 0) and T the reference tensors
+/// [7][K][K] with a zero eighth row. One wavefront forms 16 cells: two
+/// 16x16x4 products (k = 0..3, 4..7) per 16 columns of E.
+template <typename T, int K>
+__global__ void __launch_bounds__(kBlock)
+    elem_mfma_k(std::int32_t nc, const T* __restrict__ cd, const T* __restrict__ ref,
+                T* __restrict__ elem)
+{
+  constexpr int KK = K * K;
+  constexpr int NT = (KK + 15) / 16;
+  using V4 = decltype(mfma16(T(), T(), {}));
+  __shared__ T tab[8 * NT * 16];
+  for (int t = threadIdx.x; t < 8 * NT * 16; t += kBlock)
+  {
+    const int g = t / (NT * 16), e = t % (NT * 16);
+    tab[t] = (g < 7 && e < KK) ? ref[g * KK + e] : T(0);
+  }
+  __syncthreads();
+  const int l = threadIdx.x & 63;
+  const std::int64_t c0
+      = (std::int64_t(xcd_contiguous(blockIdx.x, gridDim.x)) * (kBlock / 64) + (threadIdx.x >> 6))
+        * 16;
+  if (c0 >= nc)
+    return;
+  const std::int64_t ca = c0 + (l & 15);
+  const int kq = l >> 4;
+  const T a0 = ca < nc ? cd[8 * ca + kq] : T(0);
+  const T a1 = ca < nc ? cd[8 * ca + 4 + kq] : T(0);
+#pragma unroll 5
+  for (int t = 0; t < NT; ++t)
+  {
+    const int e = 16 * t + (l & 15);
+    V4 acc = {0, 0, 0, 0};
+    acc = mfma16(a0, tab[kq * NT * 16 + e], acc);
+    acc = mfma16(a1, tab[(4 + kq) * NT * 16 + e], acc);
+    if (e < KK)
+    {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+      {
+        const std::int64_t c = c0 + mfma16_row<T>(kq, r);
+        if (c < nc)
+          elem[c * KK + e] = acc[r];
+      }
+    }
+  }
+}
+
+/// Incidence-parallel row assembly: a workgroup owns a block of consecutive
+/// rows holding at most one incidence per thread. Every thread loads its
+/// incidence's element row (from the element matrices E when MAT, else from
+/// the cell factors and the reference tensors in LDS), then the rows'
+/// incidences are added into the LDS slice in rounds by their rank in the
+/// row, so each entry is summed in cell order as in the other variants.
+template <typename T, typename O, int K, bool MAT>
+__global__ void __launch_bounds__(kBlock)
+    row_gather_k(const std::int32_t* __restrict__ gblk, const std::int32_t* __restrict__ inc_off,
+                 const std::int32_t* __restrict__ inc, const std::int32_t* __restrict__ inc_row,
+                 const O* __restrict__ islots, const T* __restrict__ cd,
+                 const T* __restrict__ elem, const std::uint32_t* __restrict__ mask,
+                 const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
+                 const T* __restrict__ ref, T* __restrict__ vals, int accumulate, int max_rank,
+                 std::int32_t nplan)
+{
+  extern __shared__ unsigned char smem_raw[];
+  T* S = reinterpret_cast<T*>(smem_raw); // reference tensors [7][K][K] unless MAT
+  T* acc = MAT ? S : S + 7 * K * K;
+  const int b = xcd_contiguous(blockIdx.x, gridDim.x);
+  const std::int32_t R0 = gblk[b], R1 = gblk[b + 1];
+  const std::int32_t base = rp[R0];
+  const std::int32_t len = rp[R1] - base;
+  if constexpr (!MAT)
+    for (int t = threadIdx.x; t < 7 * K * K; t += kBlock)
+      S[t] = ref[t];
+  for (int t = threadIdx.x; t < len; t += kBlock)
+    acc[t] = T(0);
+  // rows from nplan on have no incidences (inc_off has nplan + 1 entries)
+  const std::int32_t p = inc_off[min(R0, nplan)] + threadIdx.x;
+  bool live = p < inc_off[min(R1, nplan)];
+  std::int32_t r = 0, rank = -1;
+  if (live)
+  {
+    r = inc_row[p];
+    rank = p - inc_off[r];
+    live = !(bc && bc[r]);
+  }
+  T e[K];
+  O sl[K];
+  std::uint32_t m = 0;
+  __syncthreads();
+  if (live)
+  {
+    const std::int32_t cik = inc[p];
+    const std::int64_t c = cik / K;
+    m = mask ? mask[c] : 0u;
+    load_slots<O, K>(islots, p, sl);
+    if constexpr (MAT)
+    {
+      const T* er = elem + std::int64_t(cik) * K;
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+        e[j] = er[j];
+    }
+    else
+    {
+      T g[7];
+      load_factors<T>(cd, c, g);
+      const int i = cik - static_cast<int>(c) * K;
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+        e[j] = element_entry<T, K>(g, S, i, j);
+    }
+  }
+  T* a = acc + (live ? rp[r] - base : 0);
+  for (int k = 0; k < max_rank; ++k)
+  {
+    if (live && rank == k)
+    {
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+        if (!((m >> j) & 1u))
+          a[sl[j]] += e[j];
+    }
+    __syncthreads();
+  }
+  if (accumulate)
+    for (int t = threadIdx.x; t < len; t += kBlock)
+      vals[base + t] += acc[t];
+  else
+    for (int t = threadIdx.x; t < len; t += kBlock)
+      vals[base + t] = acc[t];
+}
+
 /// Element vectors of the source forms, one thread per cell:
 /// P1 tetrahedra b_i = |K|/20 (f_i + sum f); affine Q1 hexahedra
 /// b_i = |det J| sum_j M_ij f_j; other hexahedra 2x2x2 Gauss.
 template <typename T, int NV>
 __global__ void __launch_bounds__(kBlock)
     cell_source_k(std::int32_t nc, const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
-                  const T* __restrict__ coeffs, int cstride, int affine,
-                  const std::int32_t* __restrict__ inc_pos, T* __restrict__ be)
+                  const T* __restrict__ coeffs, int cstride, int affine, T* __restrict__ be)
 {
   const std::int64_t c = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
   if (c >= nc)
@@ -1034,24 +1242,27 @@ __global__ void __launch_bounds__(kBlock)
         out[i] += adet / T(8) * fq * phi[i];
     }
   }
+  // cell-major element vectors: a wavefront stores one contiguous range
 #pragma unroll
   for (int i = 0; i < NV; ++i)
-    be[inc_pos[c * NV + i]] = out[i];
+    be[c * NV + i] = out[i];
 }
 
-/// b[r] += sum of the element vector entries of the row's incidences
-/// (stored in incidence order, so each row reads a contiguous range).
+/// b[r] += sum of the element vector entries of the row's incidences, in
+/// cell order (inc holds c * k + i, so be is read where the cell pass
+/// stored it).
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     row_vector_k(std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
-                 const T* __restrict__ be, T* __restrict__ b)
+                 const std::int32_t* __restrict__ inc, const T* __restrict__ be,
+                 T* __restrict__ b)
 {
   const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
   if (r >= nplan)
     return;
   T s = 0;
   for (std::int32_t p = inc_off[r]; p < inc_off[r + 1]; ++p)
-    s += be[p];
+    s += be[inc[p]];
   b[r] += s;
 }
 
@@ -1394,8 +1605,7 @@ __global__ void __launch_bounds__(kBlock)
 template <typename T, int K, int BS>
 __global__ void __launch_bounds__(kBlock)
     cell_const_source_k(std::int32_t nc, const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
-                        const T* __restrict__ w, T f0, T f1, T f2,
-                        const std::int32_t* __restrict__ inc_pos, T* __restrict__ be)
+                        const T* __restrict__ w, T f0, T f1, T f2, T* __restrict__ be)
 {
   const std::int64_t c = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
   if (c >= nc)
@@ -1418,10 +1628,9 @@ __global__ void __launch_bounds__(kBlock)
   const T f[3] = {f0, f1, f2};
   for (int i = 0; i < K; ++i)
   {
-    const std::int64_t p = inc_pos[c * K + i];
 #pragma unroll
     for (int b = 0; b < BS; ++b)
-      be[p * BS + b] = adet * w[i] * f[b];
+      be[(c * K + i) * BS + b] = adet * w[i] * f[b];
   }
 }
 
@@ -1429,7 +1638,8 @@ __global__ void __launch_bounds__(kBlock)
 template <typename T, int BS>
 __global__ void __launch_bounds__(kBlock)
     row_vector_bs_k(std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
-                    const T* __restrict__ be, T* __restrict__ b)
+                    const std::int32_t* __restrict__ inc, const T* __restrict__ be,
+                    T* __restrict__ b)
 {
   const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
   if (t >= std::int64_t(nplan) * BS)
@@ -1438,7 +1648,7 @@ __global__ void __launch_bounds__(kBlock)
   const int comp = static_cast<int>(t % BS);
   T s = 0;
   for (std::int32_t p = inc_off[r]; p < inc_off[r + 1]; ++p)
-    s += be[std::int64_t(p) * BS + comp];
+    s += be[std::int64_t(inc[p]) * BS + comp];
   b[t] += s;
 }
 
@@ -1583,7 +1793,7 @@ int incidence_lanes(const dfx_form& f)
 
 /// Row-kernel variants, picked per form by dfx_form_autotune:
 /// 0..2: "rows" with 64/128/256 rows per workgroup; 3: "lanes".
-constexpr int kRowVariants = 4;
+constexpr int kRowVariants = 9;
 
 template <typename T, typename O, int K, int L>
 void launch_lanes(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
@@ -1622,23 +1832,143 @@ void launch_elast(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
   check_launch("elasticity assembly");
 }
 
+/// Row blocks of the incidence-parallel variants: consecutive rows with at
+/// most one incidence per thread and an LDS slice of at most 32 KB, plus the
+/// row of every incidence. Built once per pattern.
+void ensure_gather(dfx_form& f, const dfx_csr& A, std::size_t tsize)
+{
+  if (f.gblk && f.gather_pattern == A.pattern_id)
+    return;
+  dfree(f.gblk);
+  dfree(f.inc_row);
+  f.gblk = nullptr;
+  f.inc_row = nullptr;
+  const std::int32_t nplan = f.max_dof + 1;
+  std::vector<std::int32_t> off(nplan + 1), rp(A.nrows + 1);
+  d2h(off.data(), f.inc_off, sizeof(std::int32_t) * off.size());
+  d2h(rp.data(), A.row_ptr, sizeof(std::int32_t) * rp.size());
+  std::vector<std::int32_t> irow(std::max<std::int32_t>(off[nplan], 1));
+  for (std::int32_t r = 0; r < nplan; ++r)
+    for (std::int32_t q = off[r]; q < off[r + 1]; ++q)
+      irow[q] = r;
+  const std::int64_t cap_inc = kBlock;
+  const std::int64_t cap_slice = (32 * 1024) / static_cast<std::int64_t>(tsize);
+  std::vector<std::int32_t> gb{0};
+  std::int64_t slice = 0;
+  std::int32_t r = 0;
+  while (r < A.nrows)
+  {
+    std::int32_t e = r;
+    std::int64_t ninc = 0, nent = 0;
+    while (e < A.nrows)
+    {
+      const std::int64_t ci = e < nplan ? off[e + 1] - off[e] : 0;
+      const std::int64_t ce = rp[e + 1] - rp[e];
+      if (ninc + ci > cap_inc || nent + ce > cap_slice)
+      {
+        require(e > r, "row too long for the incidence-parallel variants");
+        break;
+      }
+      ninc += ci;
+      nent += ce;
+      ++e;
+    }
+    gb.push_back(e);
+    slice = std::max(slice, nent);
+    r = e;
+  }
+  f.ngblk = static_cast<std::int32_t>(gb.size()) - 1;
+  f.gblk_slice = static_cast<std::int32_t>(slice);
+  f.gblk = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * gb.size()));
+  h2d(f.gblk, gb.data(), sizeof(std::int32_t) * gb.size());
+  f.inc_row = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * irow.size()));
+  h2d(f.inc_row, irow.data(), sizeof(std::int32_t) * irow.size());
+  f.gather_pattern = A.pattern_id;
+}
+
+/// Variants 7 (element matrices on the matrix cores, then gathered by rows)
+/// and 8 (entries formed from the cell factors while gathering).
+template <typename T, typename O, int K>
+void launch_gather(dfx_form& f, dfx_csr& A, const dfx_bc* bc, bool mat)
+{
+  ensure_gather(f, A, sizeof(T));
+  const std::size_t lds = ((mat ? 0 : 7 * K * K) + std::size_t(f.gblk_slice)) * sizeof(T);
+  require(lds <= 64 * 1024, "row too long for the LDS slice");
+  launch_cell_factors<T>(f, bc);
+  const std::int32_t nc = f.d.num_cells;
+  if (mat)
+  {
+    if (!f.cell_elem)
+      f.cell_elem = dmalloc(sizeof(T) * std::max<std::int64_t>(std::int64_t(nc) * K * K, 1));
+    const int nb = static_cast<int>((std::int64_t(nc) + 63) / 64);
+    if (nb > 0)
+      elem_mfma_k<T, K><<<nb, kBlock, 0, stream()>>>(
+          nc, static_cast<const T*>(f.cell_data), static_cast<const T*>(f.ref_tensors),
+          static_cast<T*>(f.cell_elem));
+  }
+  if (f.ngblk == 0)
+    return;
+  auto go = [&](auto kern)
+  {
+    kern<<<f.ngblk, kBlock, lds, stream()>>>(
+        f.gblk, f.inc_off, f.inc, f.inc_row, static_cast<const O*>(f.islots),
+        static_cast<const T*>(f.cell_data), static_cast<const T*>(f.cell_elem),
+        bc ? f.cell_mask : nullptr, bc ? bc->marker : nullptr, A.row_ptr,
+        static_cast<const T*>(f.ref_tensors), static_cast<T*>(A.values), A.zeroed ? 0 : 1,
+        f.max_inc, f.max_dof + 1);
+  };
+  if (mat)
+    go(row_gather_k<T, O, K, true>);
+  else
+    go(row_gather_k<T, O, K, false>);
+}
+
 template <typename T, typename O, int K>
 void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
 {
-  launch_cell_factors<T>(f, bc);
-  const int variant = K > 8 ? 3 : f.row_variant; // wide elements: lanes only
-  if (variant < 3)
+  const int variant = f.row_variant;
+  if (variant == 7 || variant == 8)
   {
-    const int R = 64 << variant;
-    const std::size_t lds = (7 * K * K + std::size_t(R) * A.max_row) * sizeof(T);
+    launch_gather<T, O, K>(f, A, bc, variant == 7);
+    check_launch("row_assemble");
+    return;
+  }
+  // wide elements: the LDS slice of a thread-per-row block does not fit
+  require(K <= 8 || variant == 3, "variant not available for this element");
+  launch_cell_factors<T>(f, bc);
+  if (variant != 3)
+  {
+    // 0-2: four incidences per group, reference tensors in LDS;
+    // 4-6: eight per group, closed-form rows for Q1 hexahedra
+    const bool wide = variant >= 4;
+    const bool cf = wide && K == 8 && f.d.kernel == DFX_K_Q1HEX_A;
+    const int R = 64 << (wide ? variant - 4 : variant);
+    const std::size_t lds
+        = ((cf ? 0 : 7 * K * K) + std::size_t(R) * A.max_row) * sizeof(T);
     require(lds <= 64 * 1024, "row too long for the LDS slice");
     const int nb = (A.nrows + R - 1) / R;
-    if (nb > 0)
-      row_assemble_k<T, O, K><<<nb, R, lds, stream()>>>(
+    auto go = [&](auto kern)
+    {
+      kern<<<nb, R, lds, stream()>>>(
           A.nrows, f.max_dof + 1, f.inc_off, f.inc, static_cast<const O*>(f.islots),
           static_cast<const T*>(f.cell_data), bc ? f.cell_mask : nullptr,
           bc ? bc->marker : nullptr, A.row_ptr, static_cast<const T*>(f.ref_tensors),
           static_cast<T*>(A.values), A.zeroed ? 0 : 1);
+    };
+    if (nb > 0)
+    {
+      if (!wide)
+        go(row_assemble_k<T, O, K, 4, false>);
+      else if constexpr (K == 8)
+      {
+        if (cf)
+          go(row_assemble_k<T, O, K, 8, true>);
+        else
+          go(row_assemble_k<T, O, K, 8, false>);
+      }
+      else
+        go(row_assemble_k<T, O, K, 8, false>);
+    }
   }
   else
   {
@@ -1726,9 +2056,9 @@ void vector_rows(const dfx_form& f, T* b)
     cell_const_source_k<T, 10, 3><<<blocks_for(nc), kBlock, 0, stream()>>>(
         nc, static_cast<const T*>(f.x), f.x_dofmap, static_cast<const T*>(f.ref_tensors),
         static_cast<T>(f.constants[0]), static_cast<T>(f.constants[1]),
-        static_cast<T>(f.constants[2]), f.inc_pos, be);
+        static_cast<T>(f.constants[2]), be);
     row_vector_bs_k<T, 3><<<blocks_for(std::int64_t(np) * 3), kBlock, 0, stream()>>>(
-        np, f.inc_off, be, b);
+        np, f.inc_off, f.inc, be, b);
     check_launch("assemble_vector");
     return;
   }
@@ -1736,21 +2066,21 @@ void vector_rows(const dfx_form& f, T* b)
   {
     cell_const_source_k<T, 20, 1><<<blocks_for(nc), kBlock, 0, stream()>>>(
         nc, static_cast<const T*>(f.x), f.x_dofmap, static_cast<const T*>(f.ref_tensors),
-        static_cast<T>(f.constants[0]), T(0), T(0), f.inc_pos, be);
-    row_vector_bs_k<T, 1><<<blocks_for(np), kBlock, 0, stream()>>>(np, f.inc_off, be, b);
+        static_cast<T>(f.constants[0]), T(0), T(0), be);
+    row_vector_bs_k<T, 1><<<blocks_for(np), kBlock, 0, stream()>>>(np, f.inc_off, f.inc, be, b);
     check_launch("assemble_vector");
     return;
   }
   if (nv == 4)
     cell_source_k<T, 4><<<blocks_for(nc), kBlock, 0, stream()>>>(
         nc, static_cast<const T*>(f.x), f.x_dofmap, static_cast<const T*>(f.coeffs),
-        f.d.cstride, 1, f.inc_pos, be);
+        f.d.cstride, 1, be);
   else
     cell_source_k<T, 8><<<blocks_for(nc), kBlock, 0, stream()>>>(
         nc, static_cast<const T*>(f.x), f.x_dofmap, static_cast<const T*>(f.coeffs),
-        f.d.cstride, f.affine ? 1 : 0, f.inc_pos, be);
+        f.d.cstride, f.affine ? 1 : 0, be);
   const std::int32_t nplan = f.max_dof + 1;
-  row_vector_k<T><<<blocks_for(nplan), kBlock, 0, stream()>>>(nplan, f.inc_off, be, b);
+  row_vector_k<T><<<blocks_for(nplan), kBlock, 0, stream()>>>(nplan, f.inc_off, f.inc, be, b);
   check_launch("assemble_vector");
 }
 
@@ -1927,19 +2257,12 @@ dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out)
           for (std::int64_t i = 0; i < nc * k; ++i)
             inc[pos[d.dofmap[i]]++] = static_cast<std::int32_t>(i);
           f->max_inc = mx;
+          // thread-per-row slices do not fit LDS for wide elements
+          f->row_variant = k > 8 ? 3 : 1;
           f->inc_off = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * off.size()));
           h2d(f->inc_off, off.data(), sizeof(std::int32_t) * off.size());
           f->inc = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * inc.size()));
           h2d(f->inc, inc.data(), sizeof(std::int32_t) * inc.size());
-          if (!ki.bilinear)
-          {
-            // position of each (cell, local index) in the incidence order
-            std::vector<std::int32_t> ipos(inc.size());
-            for (std::int64_t q = 0; q < nc * k; ++q)
-              ipos[inc[q]] = static_cast<std::int32_t>(q);
-            f->inc_pos = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * ipos.size()));
-            h2d(f->inc_pos, ipos.data(), sizeof(std::int32_t) * ipos.size());
-          }
           const std::size_t ts = dtype_size(d.dtype);
           if (ki.bilinear)
           {
@@ -1974,12 +2297,14 @@ dfx_status dfx_form_destroy(dfx_form* f)
     dfree(f->slots);
     dfree(f->inc_off);
     dfree(f->inc);
-    dfree(f->inc_pos);
     dfree(f->islots);
     dfree(f->cell_data);
     dfree(f->cell_mask);
     dfree(f->ref_tensors);
     dfree(f->cell_vec);
+    dfree(f->inc_row);
+    dfree(f->gblk);
+    dfree(f->cell_elem);
     delete f;
   }
   return DFX_OK;
@@ -2060,13 +2385,17 @@ dfx_status dfx_assemble_matrix(dfx_csr* A, const dfx_form* a, const dfx_bc* bc)
         dfx_form& f = const_cast<dfx_form&>(*a);
         if (row_path(f))
         {
+          // store mode (A zeroed) overwrites every entry; add mode needs
+          // A's values, which are then already stored
           if (A->dtype == DFX_F64)
             assemble_rows<double>(f, *A, bc);
           else
             assemble_rows<float>(f, *A, bc);
           A->zeroed = false;
+          A->zero_pending = false;
           return;
         }
+        materialize_zero(*A);
         A->zeroed = false;
         ensure_slots(f, *A);
         if (A->dtype == DFX_F64)
@@ -2121,6 +2450,7 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t*
                 else
                   assemble_rows<float>(f, *A, bc);
                 A->zeroed = false;
+                A->zero_pending = false;
               }
               DFX_HIP(hipEventRecord(e1, stream()));
               DFX_HIP(hipEventSynchronize(e1));
@@ -2141,11 +2471,9 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t*
           (void)hipEventDestroy(e1);
         }
         f.row_variant = best;
-        // leave A zeroed
-        const std::int64_t n = A->nnz * A->bs * A->bs;
-        DFX_HIP(hipMemsetAsync(A->values, 0, dtype_size(A->dtype) * n, stream()));
-        DFX_HIP(hipStreamSynchronize(stream()));
+        // leave A zeroed (stored when next read)
         A->zeroed = true;
+        A->zero_pending = true;
         if (variant)
           *variant = best;
         if (ms)
@@ -2173,8 +2501,16 @@ dfx_status dfx_set_diagonal(dfx_csr* A, const dfx_bc* bc, double diag)
         require(bc->n_total == A->nrows * A->bs, "BC size does not match the matrix");
         if (bc->ndofs == 0)
           return;
-        int* err = static_cast<int*>(dmalloc(sizeof(int)));
-        DFX_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
+        materialize_zero(*A);
+        // the first call on a pattern checks that every BC dof has a
+        // diagonal slot (one host round trip); later calls only launch
+        const bool check = bc->diag_checked != A->pattern_id;
+        int* err = nullptr;
+        if (check)
+        {
+          err = static_cast<int*>(dmalloc(sizeof(int)));
+          DFX_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
+        }
         if (A->dtype == DFX_F64)
           set_diagonal_k<double><<<blocks_for(bc->ndofs), kBlock, 0, stream()>>>(
               bc->ndofs, bc->dofs, A->bs, A->diag, static_cast<double*>(A->values), diag, err);
@@ -2184,11 +2520,14 @@ dfx_status dfx_set_diagonal(dfx_csr* A, const dfx_bc* bc, double diag)
               static_cast<float>(diag), err);
         check_launch("set_diagonal");
         A->zeroed = false;
+        if (!check)
+          return;
         int h = 0;
         d2h(&h, err, sizeof(int));
         dfree(err);
         if (h)
           throw Error(DFX_ERR_PATTERN, "Entry not in sparsity");
+        bc->diag_checked = A->pattern_id;
       });
 }
 

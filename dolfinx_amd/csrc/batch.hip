@@ -393,6 +393,7 @@ dfx_status dfx_batch_add(dfx_batch* B, const dfx_csr* A, const dfx_vec* b, dfx_v
             break;
           }
         require(s >= 0, "batch is full (systems)");
+        materialize_zero(*A);
         SysDesc d{};
         d.rp = A->row_ptr;
         d.ci = A->cols;

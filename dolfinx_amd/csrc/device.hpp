@@ -136,6 +136,10 @@ struct dfx_csr
   void* values = nullptr;          // device, nnz*bs*bs of dtype
   std::uint64_t pattern_id = 0;    // identifies the pattern for slot caches
   bool zeroed = true;              // values known to be all zero
+  // zeroed but not yet written: the zeros are stored just before the
+  // values are first read (a store-mode assembly overwrites every entry
+  // and needs no zero fill)
+  mutable bool zero_pending = false;
   int lanes = 0;                   // SpMV variant: 0 default, 1 stream, G>1 vector
   // Row blocks of the stream SpMV: consecutive rows whose entries fit one
   // LDS chunk (or a single longer row); device, nblk + 1 row boundaries.
@@ -157,6 +161,8 @@ struct dfx_bc
   std::int32_t* dofs = nullptr;   // device, unrolled dof indices
   std::int8_t* marker = nullptr;  // device, n_total
   double* g = nullptr;            // device, n_total (FP64), nullptr if g = 0
+  // pattern whose diagonal slots were checked to hold every BC dof
+  mutable std::uint64_t diag_checked = 0;
 };
 
 struct dfx_form
@@ -179,7 +185,6 @@ struct dfx_form
   // cell's k column dofs. Built once per pattern.
   std::int32_t* inc_off = nullptr; // device, (max_dof + 2)
   std::int32_t* inc = nullptr;     // device, num_cells * k: c * k + i
-  std::int32_t* inc_pos = nullptr; // device, inverse of inc (linear forms)
   void* islots = nullptr;          // device, num_cells * k * k
   std::uint64_t plan_pattern = 0;
   int islot_bytes = 0;
@@ -190,5 +195,22 @@ struct dfx_form
   std::uint32_t* cell_mask = nullptr;
   void* ref_tensors = nullptr;
   void* cell_vec = nullptr;        // per-cell element vectors (linear forms)
+  // Incidence-parallel assembly: row of each incidence, row blocks of at
+  // most one incidence per thread (gblk, ngblk + 1 row boundaries), and the
+  // element matrices of all cells (num_cells x k x k of dtype) when they are
+  // formed as one batched product.
+  std::int32_t* inc_row = nullptr;
+  std::int32_t* gblk = nullptr;
+  std::int32_t ngblk = 0;
+  std::int32_t gblk_slice = 0;     // longest slice of a row block (entries)
+  std::uint64_t gather_pattern = 0;
+  void* cell_elem = nullptr;
   int row_variant = 1;             // row-kernel variant (dfx_form_autotune)
 };
+
+namespace dfx
+{
+/// Store the pending zeros of A's values (no-op unless zero_pending).
+/// Every reader of the values calls this first.
+void materialize_zero(const dfx_csr& A);
+} // namespace dfx

@@ -170,6 +170,15 @@ __global__ void __launch_bounds__(kBlock)
   diag[r] = (lo < rp[r + 1] && ci[lo] == r) ? lo : -1;
 }
 
+/// Workgroup barrier ordering LDS accesses only: waits for this wave's LDS
+/// operations, not for its outstanding global loads (which a full
+/// __syncthreads fence would drain).
+__device__ inline void lds_barrier()
+{
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+}
+
 /// Entries per LDS chunk of the stream SpMV (16 KB of products).
 template <typename T>
 constexpr int stream_chunk()
@@ -180,10 +189,12 @@ constexpr int stream_chunk()
 /// Stream CSR SpMV, y (+)= A x. Each row block (consecutive rows whose
 /// entries fit one chunk) is processed by one workgroup: the products
 /// v_j x[c_j] of the block's contiguous entry range are formed with fully
-/// coalesced loads (eight per thread in flight) into LDS, then one thread
-/// per row sums its products left to right, the order of the reference's
-/// row-serial loop. A block holding a single longer row is reduced with a
-/// strided sum. With dotv, partial sums of y.dotv go to partials.
+/// coalesced loads into LDS, then one thread per row sums its products left
+/// to right, the order of the reference's row-serial loop. The entries of
+/// the workgroup's next row block are loaded into registers while the rows
+/// of the current one are summed. A block holding a single longer row is
+/// reduced with a strided sum. With dotv, partial sums of y.dotv go to
+/// partials.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     spmv_stream_k(std::int32_t nblk, const std::int32_t* __restrict__ rblk,
@@ -199,41 +210,92 @@ __global__ void __launch_bounds__(kBlock)
   __shared__ T prod[NC];
   __shared__ double red[4];
   double d[1] = {0.0};
-  for (int bi = xcd_contiguous(blockIdx.x, gridDim.x); bi < nblk; bi += gridDim.x)
+  T vv[U];
+  std::int32_t cc[U];
+  std::int32_t r0 = 0, r1 = 0, j0 = 0, j1 = 0;
+  auto bounds = [&](int b)
   {
-    const std::int32_t r0 = rblk[bi], r1 = rblk[bi + 1];
-    const std::int32_t j0 = rp[r0], j1 = rp[r1];
+    r0 = rblk[b];
+    r1 = rblk[b + 1];
+    j0 = rp[r0];
+    j1 = rp[r1];
+  };
+  // unconditional loads at clamped positions (no branch around a load, so
+  // all of them are in flight together); entries past j1 are not used
+  auto stage = [&]()
+  {
+    const std::int32_t last = j1 - 1;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      const std::int32_t j = min(j0 + static_cast<std::int32_t>(threadIdx.x) + u * kBlock, last);
+      vv[u] = v[j];
+      cc[u] = ci[j];
+    }
+  };
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+  {
+    vv[u] = T(0);
+    cc[u] = 0;
+  }
+  int bi = xcd_contiguous(blockIdx.x, gridDim.x);
+  if (bi < nblk)
+  {
+    bounds(bi);
+    if (j1 > j0 && j1 - j0 <= NC)
+      stage();
+  }
+  while (bi < nblk)
+  {
+    const int nbi = bi + static_cast<int>(gridDim.x);
     if (j1 - j0 <= NC)
     {
-      T vv[U];
-      std::int32_t cc[U];
+      T xv[U];
 #pragma unroll
       for (int u = 0; u < U; ++u)
-      {
-        const std::int32_t j = j0 + threadIdx.x + u * kBlock;
-        vv[u] = j < j1 ? v[j] : T(0);
-        cc[u] = j < j1 ? ci[j] : 0;
-      }
+        xv[u] = x[cc[u]];
 #pragma unroll
       for (int u = 0; u < U; ++u)
       {
         const std::int32_t j = j0 + threadIdx.x + u * kBlock;
         if (j < j1)
-          prod[j - j0] = vv[u] * x[cc[u]];
+          prod[j - j0] = vv[u] * xv[u];
       }
-      __syncthreads();
-      for (std::int32_t r = r0 + threadIdx.x; r < r1; r += kBlock)
+      lds_barrier();
+      // a row block has at most kBlock rows: one row per thread. Its global
+      // reads come first, so that the next block's loads issued below are
+      // not waited for before the LDS sums.
+      const std::int32_t r = r0 + threadIdx.x;
+      const bool has = r < r1;
+      std::int32_t kb = 0, ke = 0;
+      T yold = T(0), dv = T(0);
+      if (has)
+      {
+        kb = rp[r] - j0;
+        ke = rp[r + 1] - j0;
+        if (accumulate)
+          yold = y[r];
+        if (dotv)
+          dv = dotv[r];
+      }
+      if (nbi < nblk)
+      {
+        bounds(nbi);
+        if (j1 > j0 && j1 - j0 <= NC)
+          stage();
+      }
+      if (has)
       {
         T s = 0;
-        const std::int32_t e = rp[r + 1] - j0;
-        for (std::int32_t k = rp[r] - j0; k < e; ++k)
+        for (std::int32_t k = kb; k < ke; ++k)
           s += prod[k];
-        const T yr = accumulate ? y[r] + s : s;
+        const T yr = accumulate ? yold + s : s;
         y[r] = yr;
         if (dotv)
-          d[0] += double(yr) * double(dotv[r]);
+          d[0] += double(yr) * double(dv);
       }
-      __syncthreads();
+      lds_barrier();
     }
     else
     {
@@ -248,7 +310,143 @@ __global__ void __launch_bounds__(kBlock)
         if (dotv)
           d[0] += double(yr) * double(dotv[r0]);
       }
+      if (nbi < nblk)
+      {
+        bounds(nbi);
+        if (j1 > j0 && j1 - j0 <= NC)
+          stage();
+      }
     }
+    bi = nbi;
+  }
+  if (partials)
+    block_partials<1>(d, partials);
+}
+
+/// Stream SpMV of a block-CSR matrix with BS x BS blocks (row-major inside
+/// a block), y (+)= A x. Row blocks hold at most kBlock / BS block rows and
+/// stream_chunk / BS blocks. Phase 1: task t = (block b, block-row component
+/// a) loads the BS contiguous values of that block row (consecutive tasks
+/// read consecutive addresses) and the BS entries of x, and stores their
+/// product sum in LDS; phase 2: one thread per (block row, component) sums
+/// its row's block products left to right. The next row block's entries
+/// are loaded while the current rows are summed.
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock)
+    bsr_stream_k(std::int32_t nblk, const std::int32_t* __restrict__ rblk,
+                 const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
+                 const T* __restrict__ v, const T* __restrict__ x, T* __restrict__ y,
+                 int accumulate, const T* __restrict__ dotv, double* partials,
+                 const int* stop)
+{
+  if (stop && block_flag(stop))
+    return;
+  constexpr int NC = stream_chunk<T>();        // products per chunk
+  constexpr int U = NC / kBlock;               // tasks per thread
+  __shared__ T prod[NC];
+  double d[1] = {0.0};
+  T vv[U][BS];
+  std::int32_t cc[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+  {
+#pragma unroll
+    for (int k = 0; k < BS; ++k)
+      vv[u][k] = T(0);
+    cc[u] = 0;
+  }
+  std::int32_t r0 = 0, r1 = 0, j0 = 0, j1 = 0;
+  auto bounds = [&](int b)
+  {
+    r0 = rblk[b];
+    r1 = rblk[b + 1];
+    j0 = rp[r0];
+    j1 = rp[r1];
+  };
+  // task t of the row block: block j0 + t / BS, component t % BS; loads at
+  // clamped positions so that no load sits behind a branch
+  auto stage = [&]()
+  {
+    const std::int32_t last = (j1 - j0) * BS - 1;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      const std::int32_t t = min(static_cast<std::int32_t>(threadIdx.x) + u * kBlock, last);
+      const std::int64_t j = j0 + t / BS;
+      const T* a = v + (j * BS + t % BS) * BS;
+#pragma unroll
+      for (int k = 0; k < BS; ++k)
+        vv[u][k] = a[k];
+      cc[u] = ci[j];
+    }
+  };
+  int bi = xcd_contiguous(blockIdx.x, gridDim.x);
+  if (bi < nblk)
+  {
+    bounds(bi);
+    if (j1 > j0)
+      stage();
+  }
+  while (bi < nblk)
+  {
+    const int nbi = bi + static_cast<int>(gridDim.x);
+    const std::int32_t ntask = (j1 - j0) * BS;
+    T xv[U][BS];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      const T* xx = x + std::int64_t(cc[u]) * BS;
+#pragma unroll
+      for (int k = 0; k < BS; ++k)
+        xv[u][k] = xx[k];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      const std::int32_t t = threadIdx.x + u * kBlock;
+      if (t < ntask)
+      {
+        T s = vv[u][0] * xv[u][0];
+#pragma unroll
+        for (int k = 1; k < BS; ++k)
+          s += vv[u][k] * xv[u][k];
+        prod[t] = s;
+      }
+    }
+    lds_barrier();
+    const std::int32_t br = r0 + static_cast<std::int32_t>(threadIdx.x) / BS;
+    const int comp = static_cast<int>(threadIdx.x) % BS;
+    const bool has = br < r1;
+    std::int32_t kb = 0, ke = 0;
+    const std::int64_t yi = std::int64_t(br) * BS + comp;
+    T yold = T(0), dv = T(0);
+    if (has)
+    {
+      kb = rp[br] - j0;
+      ke = rp[br + 1] - j0;
+      if (accumulate)
+        yold = y[yi];
+      if (dotv)
+        dv = dotv[yi];
+    }
+    if (nbi < nblk)
+    {
+      bounds(nbi);
+      if (j1 > j0)
+        stage();
+    }
+    if (has)
+    {
+      T s = 0;
+      for (std::int32_t k = kb; k < ke; ++k)
+        s += prod[k * BS + comp];
+      const T yr = accumulate ? yold + s : s;
+      y[yi] = yr;
+      if (dotv)
+        d[0] += double(yr) * double(dv);
+    }
+    lds_barrier();
+    bi = nbi;
   }
   if (partials)
     block_partials<1>(d, partials);
@@ -324,9 +522,24 @@ int lanes_per_row(const dfx_csr& A)
   return G;
 }
 
+void materialize_zero(const dfx_csr& A)
+{
+  if (!A.zero_pending)
+    return;
+  const std::int64_t n = A.nnz * A.bs * A.bs;
+  const int nb = reduce_blocks(n);
+  if (A.dtype == DFX_F64)
+    fill_k<double><<<nb, kBlock, 0, stream()>>>(static_cast<double*>(A.values), n, 0.0);
+  else
+    fill_k<float><<<nb, kBlock, 0, stream()>>>(static_cast<float*>(A.values), n, 0.0f);
+  check_launch("csr_zero");
+  A.zero_pending = false;
+}
+
 template <typename T>
 void launch_spmv(const dfx_csr& A, const T* x, T* y, bool accumulate)
 {
+  materialize_zero(A);
   hipStream_t s = stream();
   if (A.bs == 1 && lanes_per_row(A) == 1)
   {
@@ -360,9 +573,21 @@ void launch_spmv(const dfx_csr& A, const T* x, T* y, bool accumulate)
   }
   else
   {
+    const T* v = static_cast<const T*>(A.values);
+    if (A.nblk > 0 && A.lanes <= 1 && (A.bs == 2 || A.bs == 3))
+    {
+      const int nb = std::max(1, std::min(A.nblk, ctx().num_cu * 8));
+      if (A.bs == 3)
+        bsr_stream_k<T, 3><<<nb, kBlock, 0, s>>>(A.nblk, A.rblk, A.row_ptr, A.cols, v, x, y,
+                                                 accumulate, nullptr, nullptr, nullptr);
+      else
+        bsr_stream_k<T, 2><<<nb, kBlock, 0, s>>>(A.nblk, A.rblk, A.row_ptr, A.cols, v, x, y,
+                                                 accumulate, nullptr, nullptr, nullptr);
+      check_launch("spmv");
+      return;
+    }
     const std::int64_t n = std::int64_t(A.nrows) * A.bs;
     const int nb = static_cast<int>((n + kBlock - 1) / kBlock);
-    const T* v = static_cast<const T*>(A.values);
     if (A.bs == 2)
       bsr_spmv_k<T, 2><<<nb, kBlock, 0, s>>>(A.nrows, A.row_ptr, A.cols, v, x, y, accumulate);
     else if (A.bs == 3)
@@ -663,6 +888,17 @@ int launch_cg_spmv(const dfx_csr& A, const T* p, T* q, CGState* st)
     }
     return nb;
   }
+  if (A.nblk > 0 && A.lanes <= 1 && (A.bs == 2 || A.bs == 3))
+  {
+    const int nb = std::max(1, std::min({A.nblk, ctx().num_cu * 8, kMaxPartials}));
+    if (A.bs == 3)
+      bsr_stream_k<T, 3><<<nb, kBlock, 0, s>>>(A.nblk, A.rblk, A.row_ptr, A.cols, v, p, q, 0, p,
+                                               st->partials, &st->done);
+    else
+      bsr_stream_k<T, 2><<<nb, kBlock, 0, s>>>(A.nblk, A.rblk, A.row_ptr, A.cols, v, p, q, 0, p,
+                                               st->partials, &st->done);
+    return nb;
+  }
   const int nb = partial_blocks(std::int64_t(A.nrows) * A.bs);
   if (A.bs == 3)
     cg_bsr_spmv_k<T, 3><<<nb, kBlock, 0, s>>>(A.nrows, A.row_ptr, A.cols, v, p, q, st);
@@ -845,7 +1081,9 @@ int autotune_lanes(dfx_csr& A, int reps, float* best_ms)
   DFX_HIP(hipEventCreate(&e1));
   int best = 0;
   float bms = 1e30f;
-  for (int G = 1; G <= 64; G *= 2)
+  // block matrices: 1 stream, 2 one thread per (block row, component)
+  const int gmax = A.bs == 1 ? 64 : 2;
+  for (int G = 1; G <= gmax; G *= 2)
   {
     if (G == 1 && A.nblk == 0)
       continue;
@@ -962,18 +1200,20 @@ dfx_status dfx_csr_create(const int64_t* row_ptr, const int32_t* cols, int32_t n
         const std::size_t vb = dtype_size(dtype) * std::size_t(nnz) * bs * bs;
         A->values = dmalloc(std::max<std::size_t>(vb, 8));
         h2d(A->row_ptr, rp32.data(), sizeof(std::int32_t) * (nrows + 1));
-        if (bs == 1)
+        const std::int32_t chunk
+            = (dtype == DFX_F64 ? stream_chunk<double>() : stream_chunk<float>()) / bs;
+        if (bs == 1 || (bs <= 3 && maxrow <= chunk))
         {
-          // row blocks of the stream SpMV: greedy, <= 256 rows and entries
-          // fitting the LDS chunk of the matrix precision
-          const std::int32_t nc_max
-              = dtype == DFX_F64 ? stream_chunk<double>() : stream_chunk<float>();
+          // row blocks of the stream SpMV: greedy, at most kBlock / bs rows
+          // and entries fitting the LDS chunk of the matrix precision (a
+          // longer scalar row gets a block of its own)
+          const std::int32_t max_rows = kBlock / bs;
           std::vector<std::int32_t> rb{0};
           std::int32_t r = 0;
           while (r < nrows)
           {
             std::int32_t e = r + 1;
-            while (e < nrows && e - r < 256 && rp32[e + 1] - rp32[r] <= nc_max)
+            while (e < nrows && e - r < max_rows && rp32[e + 1] - rp32[r] <= chunk)
               ++e;
             rb.push_back(e);
             r = e;
@@ -1014,9 +1254,9 @@ dfx_status dfx_csr_create_shared(const dfx_csr* P, int dtype, dfx_csr** out)
         }
         const std::size_t vb = dtype_size(dtype) * std::size_t(A->nnz) * A->bs * A->bs;
         A->values = dmalloc(std::max<std::size_t>(vb, 8));
-        DFX_HIP(hipMemsetAsync(A->values, 0, vb, stream()));
-        DFX_HIP(hipStreamSynchronize(stream()));
+        // zeros stored when first read (an assembly overwrites them all)
         A->zeroed = true;
+        A->zero_pending = true;
         *out = A.release();
       });
 }
@@ -1062,14 +1302,10 @@ dfx_status dfx_csr_zero(dfx_csr* A)
       [&]
       {
         require(A, "null matrix");
-        const std::int64_t n = A->nnz * A->bs * A->bs;
-        const int nb = reduce_blocks(n);
-        if (A->dtype == DFX_F64)
-          fill_k<double><<<nb, kBlock, 0, stream()>>>(static_cast<double*>(A->values), n, 0.0);
-        else
-          fill_k<float><<<nb, kBlock, 0, stream()>>>(static_cast<float*>(A->values), n, 0.0f);
-        check_launch("csr_zero");
+        // MatrixCSR::set(0): the zeros are stored when the values are next
+        // read, unless an assembly overwrites every entry first
         A->zeroed = true;
+        A->zero_pending = true;
       });
 }
 
@@ -1079,6 +1315,7 @@ dfx_status dfx_csr_get_values(const dfx_csr* A, void* values)
       [&]
       {
         require(A && values, "null argument");
+        materialize_zero(*A);
         d2h(values, A->values, dtype_size(A->dtype) * A->nnz * A->bs * A->bs);
       });
 }
@@ -1091,6 +1328,7 @@ dfx_status dfx_csr_set_values(dfx_csr* A, const void* values)
         require(A && values, "null argument");
         h2d(A->values, values, dtype_size(A->dtype) * A->nnz * A->bs * A->bs);
         A->zeroed = false;
+        A->zero_pending = false;
       });
 }
 
@@ -1231,11 +1469,10 @@ dfx_status dfx_csr_autotune(dfx_csr* A, int32_t* lanes, float* ms)
       [&]
       {
         require(A, "null matrix");
+        materialize_zero(*A);
         float t = 0;
         int g = 0;
-        if (A->bs != 1)
-          g = 0; // block rows: one lane per (row, component)
-        else if (A->dtype == DFX_F64)
+        if (A->dtype == DFX_F64)
           g = autotune_lanes<double>(*A, 10, &t);
         else
           g = autotune_lanes<float>(*A, 10, &t);
@@ -1306,6 +1543,7 @@ dfx_status dfx_cg_solve(const dfx_csr* A, const dfx_vec* b, dfx_vec* x,
         require(A->nrows == A->ncols, "matrix not square");
         require(b->n == A->nrows * A->bs && x->n == b->n, "size mismatch");
         require(opts->max_it >= 0 && opts->rtol >= 0, "bad options");
+        materialize_zero(*A);
         *res = dfx_cg_result{};
         if (A->dtype == DFX_F64)
           cg_solve_t<double>(*A, *b, *x, *opts, *res);
@@ -1332,6 +1570,8 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
                 "matrices differ in shape");
         require(A64->nrows == A64->ncols, "matrix not square");
         require(b->n == A64->nrows * A64->bs && x->n == b->n, "size mismatch");
+        materialize_zero(*A64);
+        materialize_zero(*A32);
         *res = dfx_mixed_result{};
         hipStream_t s = stream();
         const std::int64_t n = b->n;

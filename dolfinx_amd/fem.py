@@ -197,7 +197,8 @@ class Form:
 
     def set_variant(self, variant: int):
         """Fix the assembly kernel variant (0-2 thread per row with 64/128/256
-        rows per workgroup, 3 one incidence per lane)."""
+        rows per workgroup, 3 one incidence per lane, 4-6 as 0-2 with eight
+        incidences in flight and closed-form Q1 element rows)."""
         _lib.call("dfx_form_set_variant", self._h, int(variant))
 
     @property

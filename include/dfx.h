@@ -137,11 +137,13 @@ dfx_status dfx_csr_set_values(dfx_csr* A, const void* values);
 dfx_status dfx_csr_get_pattern(const dfx_csr* A, int64_t* row_ptr,
                                int32_t* cols);
 
-/* SpMV autotuner (no counterpart in the reference): times the scalar CSR
- * kernel with 2..64 lanes per row on this matrix and keeps the fastest for
- * every later SpMV/CG on it. Returns the choice and its time per SpMV. */
+/* SpMV autotuner (no counterpart in the reference): times the SpMV
+ * variants on this matrix and keeps the fastest for every later SpMV/CG on
+ * it: 1 the row-block stream kernel, 2..64 lanes per row (scalar CSR), 2
+ * one thread per (block row, component) (block CSR). Returns the choice
+ * and its time per SpMV. */
 dfx_status dfx_csr_autotune(dfx_csr* A, int32_t* lanes, float* ms);
-/* Fix the lanes per row (0 restores the built-in heuristic). */
+/* Fix the SpMV variant as above (0 restores the built-in heuristic). */
 dfx_status dfx_csr_set_lanes(dfx_csr* A, int32_t lanes);
 
 /* la::Vector (la/Vector.h:47-422) with a device container. */
@@ -219,7 +221,9 @@ dfx_status dfx_assemble_matrix(dfx_csr* A, const dfx_form* a,
 dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc,
                              int32_t* variant, float* ms);
 /* Fix the row-kernel variant of a form: 0-2 one thread per row with 64,
- * 128, 256 rows per workgroup; 3 one incidence per lane. */
+ * 128, 256 rows per workgroup (four incidences in flight); 3 one incidence
+ * per lane; 4-6 as 0-2 with eight incidences in flight and, for Q1
+ * hexahedra, element rows in closed form instead of reference tensors. */
 dfx_status dfx_form_set_variant(dfx_form* a, int32_t variant);
 /* fem::set_diagonal (fem/assembler.h:644-686): A[d,d] = diag for BC dofs */
 dfx_status dfx_set_diagonal(dfx_csr* A, const dfx_bc* bc, double diag);

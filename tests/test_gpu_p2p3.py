@@ -55,11 +55,18 @@ def test_elasticity_solve_matches_oracle():
     xr, it = O.cg(rp, cols, Ar, 3, br, kmax=20000, rtol=1e-12)
     assert it > 0
     assert rel_err(x.array, xr) <= 1e-8
-    # the block SpMV against the oracle
+    # both block SpMV variants (1 row-block stream, 2 one thread per block
+    # row component) against the oracle
     v = np.random.default_rng(0).standard_normal(p.V.num_dofs)
-    y = la.Vector(p.V.num_dofs)
-    A.mult(la.Vector.from_array(v), y)
-    assert rel_err(y.array, O.spmv(rp, cols, Ar, 3, v)) <= 1e-13
+    yr = O.spmv(rp, cols, Ar, 3, v)
+    for lanes in (1, 2):
+        A.set_lanes(lanes)
+        y = la.Vector(p.V.num_dofs)
+        A.mult(la.Vector.from_array(v), y)
+        assert rel_err(y.array, yr) <= 1e-13, lanes
+        x2 = la.Vector(p.V.num_dofs)
+        assert solvers.cg(A, b, x2, rtol=1e-12, max_it=20000).converged
+        assert rel_err(x2.array, xr) <= 1e-8, lanes
 
 
 def oracle_p3(p, n, c):
@@ -74,9 +81,14 @@ def oracle_p3(p, n, c):
 
 
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 5e-6)])
-def test_p3_matches_oracle(dtype, tol):
+@pytest.mark.parametrize("variant", [3, 7, 8])
+def test_p3_matches_oracle(dtype, tol, variant):
+    """P3 variants: 3 one incidence per lane, 7 element matrices as one
+    batched product on the matrix cores then gathered by rows, 8
+    incidence-parallel gather from the cell factors."""
     n = (3, 2, 2)
     p = problems.p3_problem(n, seed=8, dtype=dtype)
+    p.a.set_variant(variant)
     A, b = problems.assemble_system(p)
     rp, cols, Ar, br = oracle_p3(p, n, 1.0)
     assert np.array_equal(A.indptr, rp) and np.array_equal(A.indices, cols)
@@ -85,9 +97,11 @@ def test_p3_matches_oracle(dtype, tol):
     assert rel_err(b.array, br) <= tol
 
 
-def test_p3_solve_matches_oracle():
+@pytest.mark.parametrize("variant", [3, 7, 8])
+def test_p3_solve_matches_oracle(variant):
     n = (4, 4, 3)
     p = problems.p3_problem(n, seed=9)
+    p.a.set_variant(variant)
     A, b = problems.assemble_system(p)
     x = la.Vector(p.V.num_dofs)
     res = solvers.cg(A, b, x, rtol=1e-12, max_it=20000)
