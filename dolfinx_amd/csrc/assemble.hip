@@ -1260,9 +1260,25 @@ __global__ void __launch_bounds__(kBlock)
   const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
   if (r >= nplan)
     return;
+  // eight incidences per group, loads at clamped positions so that all
+  // sixteen loads of a group are in flight together; summed in cell order
+  const std::int32_t p0 = inc_off[r], p1 = inc_off[r + 1];
   T s = 0;
-  for (std::int32_t p = inc_off[r]; p < inc_off[r + 1]; ++p)
-    s += be[inc[p]];
+  for (std::int32_t pb = p0; pb < p1; pb += 8)
+  {
+    std::int32_t q[8];
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      q[u] = inc[min(pb + u, p1 - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      v[u] = be[q[u]];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (pb + u < p1)
+        s += v[u];
+  }
   b[r] += s;
 }
 
@@ -1646,9 +1662,23 @@ __global__ void __launch_bounds__(kBlock)
     return;
   const std::int32_t r = static_cast<std::int32_t>(t / BS);
   const int comp = static_cast<int>(t % BS);
+  const std::int32_t p0 = inc_off[r], p1 = inc_off[r + 1];
   T s = 0;
-  for (std::int32_t p = inc_off[r]; p < inc_off[r + 1]; ++p)
-    s += be[std::int64_t(inc[p]) * BS + comp];
+  for (std::int32_t pb = p0; pb < p1; pb += 8)
+  {
+    std::int32_t q[8];
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      q[u] = inc[min(pb + u, p1 - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      v[u] = be[std::int64_t(q[u]) * BS + comp];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (pb + u < p1)
+        s += v[u];
+  }
   b[t] += s;
 }
 
@@ -1939,7 +1969,8 @@ void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
   if (variant != 3)
   {
     // 0-2: four incidences per group, reference tensors in LDS;
-    // 4-6: eight per group, closed-form rows for Q1 hexahedra
+    // 4-6: closed-form entries for Q1 hexahedra (four per group), else
+    // eight incidences per group
     const bool wide = variant >= 4;
     const bool cf = wide && K == 8 && f.d.kernel == DFX_K_Q1HEX_A;
     const int R = 64 << (wide ? variant - 4 : variant);
@@ -1962,7 +1993,7 @@ void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
       else if constexpr (K == 8)
       {
         if (cf)
-          go(row_assemble_k<T, O, K, 8, true>);
+          go(row_assemble_k<T, O, K, 4, true>);
         else
           go(row_assemble_k<T, O, K, 8, false>);
       }

@@ -197,8 +197,9 @@ class Form:
 
     def set_variant(self, variant: int):
         """Fix the assembly kernel variant (0-2 thread per row with 64/128/256
-        rows per workgroup, 3 one incidence per lane, 4-6 as 0-2 with eight
-        incidences in flight and closed-form Q1 element rows)."""
+        rows per workgroup, 3 one incidence per lane, 4-6 as 0-2 with
+        closed-form Q1 entries, 7 element matrices on the matrix cores then
+        gathered by rows, 8 one incidence per thread)."""
         _lib.call("dfx_form_set_variant", self._h, int(variant))
 
     @property

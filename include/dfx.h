@@ -222,8 +222,10 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc,
                              int32_t* variant, float* ms);
 /* Fix the row-kernel variant of a form: 0-2 one thread per row with 64,
  * 128, 256 rows per workgroup (four incidences in flight); 3 one incidence
- * per lane; 4-6 as 0-2 with eight incidences in flight and, for Q1
- * hexahedra, element rows in closed form instead of reference tensors. */
+ * per lane; 4-6 as 0-2 with, for Q1 hexahedra, element entries in closed
+ * form instead of reference tensors (other elements: eight incidences in
+ * flight); 7 element matrices of all cells as one matrix-core product,
+ * gathered by rows; 8 one incidence per thread, gathered by rows. */
 dfx_status dfx_form_set_variant(dfx_form* a, int32_t variant);
 /* fem::set_diagonal (fem/assembler.h:644-686): A[d,d] = diag for BC dofs */
 dfx_status dfx_set_diagonal(dfx_csr* A, const dfx_bc* bc, double diag);

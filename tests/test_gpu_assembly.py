@@ -64,9 +64,10 @@ def test_assemble_system_matches_oracle(cell, n, dtype):
 @pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_every_row_kernel_variant_matches_oracle(cell, n, variant, dtype):
-    """Variants 0-3 use the reference tensors, 4-6 eight incidences in flight
-    (closed-form element rows for Q1 hexahedra), 7 element matrices on the
-    matrix cores gathered by rows, 8 incidence-parallel gather."""
+    """Variants 0-3 use the reference tensors, 4-6 closed-form entries for
+    Q1 hexahedra (eight incidences in flight for tetrahedra), 7 element
+    matrices on the matrix cores gathered by rows, 8 incidence-parallel
+    gather."""
     p = build(cell, n, dtype)
     p.a.set_variant(variant)
     A, b = problems.assemble_system(p)
