@@ -97,6 +97,7 @@ SIGNATURES = {
     "dfx_form_create": (C.c_int, [C.POINTER(FormDesc), PP]),
     "dfx_form_destroy": (C.c_int, [P]),
     "dfx_form_set_coefficients": (C.c_int, [P, P, P]),
+    "dfx_form_pack_coefficients": (C.c_int, [P, P, C.c_int, P, I64]),
     "dfx_bc_create": (C.c_int, [P, I32, I32, P, PP]),
     "dfx_bc_destroy": (C.c_int, [P]),
     "dfx_assemble_matrix": (C.c_int, [P, P, P]),

@@ -564,6 +564,21 @@ __global__ void __launch_bounds__(kBlock) to_dtype_k(T* dst, const double* src, 
 int blocks_for(std::int64_t n) { return static_cast<int>((n + kBlock - 1) / kBlock); }
 
 /// Device copy of a host FP64 array converted to dtype.
+/// coeffs[t] = values[map[t] * ncomp + t % ncomp] (cell layout, map per
+/// cell) or values[map[t]] (vertex layout, map per position), in T.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    pack_gather_k(std::int64_t n, int ncomp, int per_cell, const std::int32_t* __restrict__ map,
+                  const double* __restrict__ values, T* __restrict__ coeffs)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t >= n)
+    return;
+  const std::int64_t src
+      = per_cell ? std::int64_t(map[t / ncomp]) * ncomp + t % ncomp : std::int64_t(map[t]);
+  coeffs[t] = static_cast<T>(values[src]);
+}
+
 void* upload_as(const double* src, std::int64_t n, int dtype)
 {
   void* d = dmalloc(dtype_size(dtype) * std::max<std::int64_t>(n, 1));
@@ -635,6 +650,16 @@ void ensure_slots(dfx_form& f, const dfx_csr& A)
 // entry is summed in the same order on every run.
 // ---------------------------------------------------------------------
 
+/// A 32-bit mask carried bit for bit in a value of T.
+template <typename T>
+__device__ inline T mask_bits(std::uint32_t m)
+{
+  if constexpr (sizeof(T) == 8)
+    return __longlong_as_double(static_cast<long long>(m));
+  else
+    return __uint_as_float(m);
+}
+
 template <typename T, int NV>
 __global__ void __launch_bounds__(kBlock)
     cell_factors_k(std::int32_t nc, const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
@@ -672,15 +697,17 @@ __global__ void __launch_bounds__(kBlock)
   G[4] = s * (adj[0] * adj[6] + adj[1] * adj[7] + adj[2] * adj[8]);
   G[5] = s * (adj[3] * adj[6] + adj[4] * adj[7] + adj[5] * adj[8]);
   G[6] = creac * adet;
-  G[7] = T(0);
+  std::uint32_t m = 0;
   if (mask)
   {
-    std::uint32_t m = 0;
     for (int j = 0; j < k; ++j)
       if (bc[dm[c * k + j]])
         m |= 1u << j;
     mask[c] = m;
   }
+  // the eighth slot carries the BC column mask bits (moved, never computed
+  // with), so a row kernel reads factors and mask with one load
+  G[7] = mask_bits<T>(m);
 }
 
 /// Incidence slot offsets: for incidence p = (cell c, local row i) and
@@ -719,21 +746,25 @@ __global__ void __launch_bounds__(kBlock)
   islots[t] = static_cast<O>(lo - r0);
 }
 
-/// The 7 cell factors of cell c (stored as 8 values, 16-byte aligned).
+/// The 7 cell factors of cell c and its BC column mask (stored as 8 values,
+/// 16-byte aligned, the mask bits in the eighth).
 template <typename T>
-__device__ inline void load_factors(const T* __restrict__ cd, std::int64_t c, T (&g)[7])
+__device__ inline void load_factors(const T* __restrict__ cd, std::int64_t c, T (&g)[7],
+                                    std::uint32_t& m)
 {
   if constexpr (sizeof(T) == 8)
   {
     const double2* p = reinterpret_cast<const double2*>(cd + 8 * c);
     const double2 a = p[0], b = p[1], d = p[2], e = p[3];
     g[0] = a.x; g[1] = a.y; g[2] = b.x; g[3] = b.y; g[4] = d.x; g[5] = d.y; g[6] = e.x;
+    m = static_cast<std::uint32_t>(__double_as_longlong(e.y));
   }
   else
   {
     const float4* p = reinterpret_cast<const float4*>(cd + 8 * c);
     const float4 a = p[0], b = p[1];
     g[0] = a.x; g[1] = a.y; g[2] = a.z; g[3] = a.w; g[4] = b.x; g[5] = b.y; g[6] = b.z;
+    m = __float_as_uint(b.w);
   }
 }
 
@@ -868,8 +899,7 @@ __global__ void __launch_bounds__(kBlock)
         if (cik[u] < 0)
           continue;
         const std::int64_t c = cik[u] / K;
-        load_factors<T>(cd, c, g[u]);
-        m[u] = mask ? mask[c] : 0u;
+        load_factors<T>(cd, c, g[u], m[u]);
         load_slots<O, K>(islots, pb + u, sl[u]);
       }
 #pragma unroll
@@ -943,8 +973,8 @@ __global__ void __launch_bounds__(kBlock)
       const std::int64_t c = cik / K;
       const int i = cik - static_cast<int>(c) * K;
       T g[7];
-      load_factors<T>(cd, c, g);
-      const std::uint32_t m = mask ? mask[c] : 0u;
+      std::uint32_t m;
+      load_factors<T>(cd, c, g, m);
       O sl[K];
       load_slots<O, K>(islots, p, sl);
 #pragma unroll
@@ -1052,8 +1082,9 @@ __global__ void __launch_bounds__(kBlock)
     return;
   const std::int64_t ca = c0 + (l & 15);
   const int kq = l >> 4;
+  // column 7 of F is zero (the factor data keeps the BC mask bits there)
   const T a0 = ca < nc ? cd[8 * ca + kq] : T(0);
-  const T a1 = ca < nc ? cd[8 * ca + 4 + kq] : T(0);
+  const T a1 = (ca < nc && kq < 3) ? cd[8 * ca + 4 + kq] : T(0);
 #pragma unroll 5
   for (int t = 0; t < NT; ++t)
   {
@@ -1132,7 +1163,8 @@ __global__ void __launch_bounds__(kBlock)
     else
     {
       T g[7];
-      load_factors<T>(cd, c, g);
+      std::uint32_t mf;
+      load_factors<T>(cd, c, g, mf);
       const int i = cik - static_cast<int>(c) * K;
 #pragma unroll
       for (int j = 0; j < K; ++j)
@@ -1157,6 +1189,173 @@ __global__ void __launch_bounds__(kBlock)
   else
     for (int t = threadIdx.x; t < len; t += kBlock)
       vals[base + t] = acc[t];
+}
+
+// ---------------------------------------------------------------------
+// Structured Q1 rows. On a box of hexahedra with lexicographic vertex order
+// a row vertex is corner I of at most one cell for each I = 0..7, and
+// column j of that cell is the neighbour at offset bit(j) - bit(I) in each
+// direction, one of the 27 neighbour slots q(I, j). With the corner cells
+// and the CSR position of each slot precomputed, a thread accumulates its
+// row in 27 registers with indices and reference values fixed at compile
+// time; entries are summed in corner order.
+// ---------------------------------------------------------------------
+
+__host__ __device__ constexpr int q1_slot(int i, int j)
+{
+  return (1 + ((j >> 0) & 1) - ((i >> 0) & 1)) + 3 * (1 + ((j >> 1) & 1) - ((i >> 1) & 1))
+         + 9 * (1 + ((j >> 2) & 1) - ((i >> 2) & 1));
+}
+
+__global__ void __launch_bounds__(kBlock) fill_i32_k(std::int32_t* v, std::int64_t n, std::int32_t a)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t < n)
+    v[t] = a;
+}
+
+/// corner_cell[row * 8 + i] = c for dof(c, i) = row; a second cell at the
+/// same corner of a row means the mesh is not a structured box.
+__global__ void __launch_bounds__(kBlock)
+    struct_corner_k(std::int32_t nc, const std::int32_t* __restrict__ dm,
+                    std::int32_t* __restrict__ corner, int* err)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t >= std::int64_t(nc) * 8)
+    return;
+  const std::int32_t c = static_cast<std::int32_t>(t / 8);
+  const int i = static_cast<int>(t % 8);
+  const std::int32_t row = dm[t];
+  if (atomicCAS(corner + std::int64_t(row) * 8 + i, -1, c) != -1)
+    atomicOr(err, 1);
+}
+
+/// tmp[row * 27 + q(i, j)] = position of dof(c, j) in row dof(c, i); all
+/// cells must agree on a slot's position.
+__global__ void __launch_bounds__(kBlock)
+    struct_slots_k(std::int32_t nc, const std::int32_t* __restrict__ dm,
+                   const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
+                   std::int32_t* __restrict__ tmp, int* err)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t >= std::int64_t(nc) * 64)
+    return;
+  const std::int64_t c = t / 64;
+  const int i = static_cast<int>((t / 8) % 8), j = static_cast<int>(t % 8);
+  const std::int32_t row = dm[c * 8 + i], col = dm[c * 8 + j];
+  std::int32_t lo = rp[row], hi = rp[row + 1];
+  const std::int32_t r0 = lo, r1 = hi;
+  while (lo < hi)
+  {
+    const std::int32_t mid = (lo + hi) >> 1;
+    if (ci[mid] < col)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if (lo >= r1 || ci[lo] != col)
+  {
+    atomicOr(err, 2);
+    return;
+  }
+  const std::int32_t pos = lo - r0;
+  const std::int32_t old = atomicCAS(tmp + std::int64_t(row) * 27 + q1_slot(i, j), -1, pos);
+  if (old != -1 && old != pos)
+    atomicOr(err, 4);
+}
+
+/// Pack the slot positions as bytes (255 absent) and check that the slots
+/// of each row cover every entry of the row exactly once.
+__global__ void __launch_bounds__(kBlock)
+    struct_pack_k(std::int32_t nrows, const std::int32_t* __restrict__ tmp,
+                  const std::int32_t* __restrict__ rp, std::uint8_t* __restrict__ slotmap,
+                  int* err)
+{
+  const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nrows)
+    return;
+  const std::int32_t len = rp[r + 1] - rp[r];
+  std::uint32_t seen = 0;
+  int count = 0;
+  for (int q = 0; q < 27; ++q)
+  {
+    const std::int32_t v = tmp[std::int64_t(r) * 27 + q];
+    if (v >= 0)
+    {
+      if (v >= 32 || ((seen >> v) & 1u))
+        atomicOr(err, 8);
+      else
+        seen |= 1u << v;
+      ++count;
+    }
+    slotmap[std::int64_t(r) * 28 + q] = v < 0 ? std::uint8_t(255) : static_cast<std::uint8_t>(v);
+  }
+  slotmap[std::int64_t(r) * 28 + 27] = 255;
+  if (count != len)
+    atomicOr(err, 8);
+}
+
+/// One thread per row of a structured Q1 matrix (see above). Factors of the
+/// row's corner cells are loaded four corners at a time at clamped indices
+/// (absent corners masked), so no load waits behind a branch.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    row_struct_q1_k(std::int32_t nrows, const std::int32_t* __restrict__ corner,
+                    const std::uint8_t* __restrict__ slotmap, const T* __restrict__ cd,
+                    const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
+                    T* __restrict__ vals, int accumulate)
+{
+  const std::int32_t r = xcd_contiguous(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (r >= nrows)
+    return;
+  T acc[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q)
+    acc[q] = T(0);
+  if (!(bc && bc[r]))
+  {
+    const int4 ca = *reinterpret_cast<const int4*>(corner + 8 * std::int64_t(r));
+    const int4 cb = *reinterpret_cast<const int4*>(corner + 8 * std::int64_t(r) + 4);
+    const std::int32_t cc[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+    {
+      T g[4][7];
+      std::uint32_t m[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+      {
+        const std::int32_t c = cc[4 * h + u];
+        load_factors<T>(cd, c < 0 ? 0 : c, g[u], m[u]);
+        if (c < 0)
+          m[u] = 0xffu; // absent corner: every column masked
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+      {
+        const int I = 4 * h + u;
+        T gs[3];
+        q1_signed<T>(g[u], I, gs);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (!((m[u] >> j) & 1u))
+            acc[q1_slot(I, j)] += q1_entry<T>(g[u], gs, I, j);
+      }
+    }
+  }
+  const std::uint32_t* sm = reinterpret_cast<const std::uint32_t*>(slotmap + 28 * std::int64_t(r));
+  std::uint32_t w[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k)
+    w[k] = sm[k];
+  T* out = vals + rp[r];
+#pragma unroll
+  for (int q = 0; q < 27; ++q)
+  {
+    const std::uint32_t s = (w[q / 4] >> (8 * (q % 4))) & 0xffu;
+    if (s != 0xffu)
+      out[s] = accumulate ? out[s] + acc[q] : acc[q];
+  }
 }
 
 /// Element vectors of the source forms, one thread per cell:
@@ -1823,7 +2022,7 @@ int incidence_lanes(const dfx_form& f)
 
 /// Row-kernel variants, picked per form by dfx_form_autotune:
 /// 0..2: "rows" with 64/128/256 rows per workgroup; 3: "lanes".
-constexpr int kRowVariants = 9;
+constexpr int kRowVariants = 10;
 
 template <typename T, typename O, int K, int L>
 void launch_lanes(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
@@ -1860,6 +2059,52 @@ void launch_elast(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
         A.row_ptr, static_cast<const T*>(f.ref_tensors), static_cast<T*>(A.values),
         A.zeroed ? 0 : 1);
   check_launch("elasticity assembly");
+}
+
+/// Structured-box plan of a Q1 form (corner cells and slot positions);
+/// returns false when the mesh is not a structured box of hexahedra.
+bool ensure_struct(dfx_form& f, const dfx_csr& A)
+{
+  if (f.struct_pattern == A.pattern_id && f.struct_ok >= 0)
+    return f.struct_ok == 1;
+  dfree(f.corner_cell);
+  dfree(f.slotmap);
+  f.corner_cell = nullptr;
+  f.slotmap = nullptr;
+  f.struct_pattern = A.pattern_id;
+  f.struct_ok = 0;
+  const std::int32_t nrows = A.nrows, nc = f.d.num_cells;
+  if (f.d.kernel != DFX_K_Q1HEX_A || !f.affine || f.max_dof + 1 != nrows || A.max_row > 27)
+    return false;
+  f.corner_cell = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * 8 * std::max(nrows, 1)));
+  f.slotmap = static_cast<std::uint8_t*>(dmalloc(28 * std::size_t(std::max(nrows, 1))));
+  std::int32_t* tmp = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * 27 * std::size_t(std::max(nrows, 1))));
+  int* err = static_cast<int*>(dmalloc(sizeof(int)));
+  DFX_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
+  fill_i32_k<<<blocks_for(std::int64_t(nrows) * 8), kBlock, 0, stream()>>>(f.corner_cell, std::int64_t(nrows) * 8, -1);
+  fill_i32_k<<<blocks_for(std::int64_t(nrows) * 27), kBlock, 0, stream()>>>(tmp, std::int64_t(nrows) * 27, -1);
+  if (nc > 0)
+  {
+    struct_corner_k<<<blocks_for(std::int64_t(nc) * 8), kBlock, 0, stream()>>>(nc, f.dofmap, f.corner_cell, err);
+    struct_slots_k<<<blocks_for(std::int64_t(nc) * 64), kBlock, 0, stream()>>>(nc, f.dofmap, A.row_ptr, A.cols, tmp, err);
+  }
+  if (nrows > 0)
+    struct_pack_k<<<blocks_for(nrows), kBlock, 0, stream()>>>(nrows, tmp, A.row_ptr, f.slotmap, err);
+  check_launch("struct_plan");
+  int h = 0;
+  d2h(&h, err, sizeof(int));
+  dfree(err);
+  dfree(tmp);
+  if (h)
+  {
+    dfree(f.corner_cell);
+    dfree(f.slotmap);
+    f.corner_cell = nullptr;
+    f.slotmap = nullptr;
+    return false;
+  }
+  f.struct_ok = 1;
+  return true;
 }
 
 /// Row blocks of the incidence-parallel variants: consecutive rows with at
@@ -1957,6 +2202,23 @@ template <typename T, typename O, int K>
 void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
 {
   const int variant = f.row_variant;
+  if (variant == 9)
+  {
+    if constexpr (K == 8)
+    {
+      require(ensure_struct(f, A), "variant not available for this mesh");
+      launch_cell_factors<T>(f, bc);
+      const int nb = (A.nrows + kBlock - 1) / kBlock;
+      if (nb > 0)
+        row_struct_q1_k<T><<<nb, kBlock, 0, stream()>>>(
+            A.nrows, f.corner_cell, f.slotmap, static_cast<const T*>(f.cell_data),
+            bc ? bc->marker : nullptr, A.row_ptr, static_cast<T*>(A.values), A.zeroed ? 0 : 1);
+      check_launch("row_assemble");
+      return;
+    }
+    else
+      throw Error(DFX_ERR_ARG, "variant not available for this element");
+  }
   if (variant == 7 || variant == 8)
   {
     launch_gather<T, O, K>(f, A, bc, variant == 7);
@@ -2336,6 +2598,10 @@ dfx_status dfx_form_destroy(dfx_form* f)
     dfree(f->inc_row);
     dfree(f->gblk);
     dfree(f->cell_elem);
+    dfree(f->corner_cell);
+    dfree(f->slotmap);
+    dfree(f->pack_map);
+    dfree(f->pack_stage);
     delete f;
   }
   return DFX_OK;
@@ -2347,15 +2613,88 @@ dfx_status dfx_form_set_coefficients(dfx_form* f, const double* coeffs, const do
       [&]
       {
         require(f, "null form");
-        if (coeffs && f->d.cstride > 0)
+        const std::int64_t n = std::int64_t(f->d.num_cells) * f->d.cstride;
+        if (coeffs && n > 0)
         {
-          void* nc = upload_as(coeffs, std::int64_t(f->d.num_cells) * f->d.cstride, f->d.dtype);
-          dfree(f->coeffs);
-          f->coeffs = nc;
+          if (f->d.dtype == DFX_F64 && f->coeffs)
+            h2d(f->coeffs, coeffs, sizeof(double) * n); // same size: reuse the buffer
+          else
+          {
+            void* nc = upload_as(coeffs, n, f->d.dtype);
+            dfree(f->coeffs);
+            f->coeffs = nc;
+          }
         }
         if (constants)
           for (int i = 0; i < f->d.num_constants; ++i)
             f->constants[i] = constants[i];
+      });
+}
+
+dfx_status dfx_form_pack_coefficients(dfx_form* f, const dfx_mesh* m, int layout,
+                                      const double* values, int64_t nvalues)
+{
+  return guard(
+      [&]
+      {
+        require(f && m && values, "null argument");
+        require(m->num_cells == f->d.num_cells, "mesh and form differ in cells");
+        const int cs = f->d.cstride;
+        require(cs > 0, "form has no coefficients");
+        std::int64_t need = 0;
+        if (layout == DFX_PACK_CELL)
+          need = std::int64_t(m->num_cells) * cs;
+        else if (layout == DFX_PACK_VERTEX)
+        {
+          require(cs == m->nvpc, "vertex layout needs one coefficient per cell vertex");
+          need = m->num_vertices;
+        }
+        else
+          throw Error(DFX_ERR_ARG, "unknown coefficient layout");
+        require(nvalues == need, "wrong number of values for the layout");
+        if (f->pack_mesh != m || f->pack_layout != layout || !f->pack_map)
+        {
+          dfree(f->pack_map);
+          f->pack_map = nullptr;
+          std::vector<std::int32_t> map;
+          if (layout == DFX_PACK_CELL)
+          {
+            map.resize(m->num_cells);
+            for (std::int32_t c = 0; c < m->num_cells; ++c)
+              map[c] = static_cast<std::int32_t>(m->original_cell[c]);
+          }
+          else
+          {
+            map.resize(std::size_t(m->num_cells) * m->nvpc);
+            for (std::size_t t = 0; t < map.size(); ++t)
+              map[t] = static_cast<std::int32_t>(m->input_vertex[m->cells[t]]);
+          }
+          f->pack_map = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * std::max<std::size_t>(map.size(), 1)));
+          h2d(f->pack_map, map.data(), sizeof(std::int32_t) * map.size());
+          f->pack_mesh = m;
+          f->pack_layout = layout;
+        }
+        if (f->pack_stage_n < nvalues)
+        {
+          dfree(f->pack_stage);
+          f->pack_stage = static_cast<double*>(dmalloc(sizeof(double) * std::max<std::int64_t>(nvalues, 1)));
+          f->pack_stage_n = nvalues;
+        }
+        h2d(f->pack_stage, values, sizeof(double) * nvalues);
+        const std::int64_t n = std::int64_t(f->d.num_cells) * cs;
+        if (!f->coeffs)
+          f->coeffs = dmalloc(dtype_size(f->d.dtype) * std::max<std::int64_t>(n, 1));
+        const int per_cell = layout == DFX_PACK_CELL ? 1 : 0;
+        if (n > 0)
+        {
+          if (f->d.dtype == DFX_F64)
+            pack_gather_k<double><<<blocks_for(n), kBlock, 0, stream()>>>(
+                n, cs, per_cell, f->pack_map, f->pack_stage, static_cast<double*>(f->coeffs));
+          else
+            pack_gather_k<float><<<blocks_for(n), kBlock, 0, stream()>>>(
+                n, cs, per_cell, f->pack_map, f->pack_stage, static_cast<float*>(f->coeffs));
+          check_launch("pack_coefficients");
+        }
       });
 }
 

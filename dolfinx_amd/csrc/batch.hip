@@ -270,6 +270,15 @@ struct dfx_batch
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_iters = 0;
+  // work vectors (r, p, q, diag^-1) of systems that left, kept by row count
+  // for the next system of that size (no device allocation or free, and so
+  // no device-wide synchronisation, while serving)
+  struct Work
+  {
+    std::int64_t n;
+    double* v[4];
+  };
+  std::vector<Work> pool;
 };
 
 namespace
@@ -362,6 +371,9 @@ dfx_status dfx_batch_destroy(dfx_batch* B)
         dfree(B->hsys[s].q);
         dfree(B->hsys[s].dinv);
       }
+    for (auto& w : B->pool)
+      for (double* v : w.v)
+        dfree(v);
     dfree(B->sys);
     dfree(B->st);
     dfree(B->chunks);
@@ -402,10 +414,23 @@ dfx_status dfx_batch_add(dfx_batch* B, const dfx_csr* A, const dfx_vec* b, dfx_v
         d.x = static_cast<double*>(x->data);
         d.nrows = A->nrows;
         const std::int64_t n = std::max(A->nrows, 1);
-        d.r = static_cast<double*>(dmalloc(sizeof(double) * n));
-        d.p = static_cast<double*>(dmalloc(sizeof(double) * n));
-        d.q = static_cast<double*>(dmalloc(sizeof(double) * n));
-        d.dinv = static_cast<double*>(dmalloc(sizeof(double) * n));
+        auto it = std::find_if(B->pool.begin(), B->pool.end(),
+                               [&](const dfx_batch::Work& w) { return w.n == n; });
+        if (it != B->pool.end())
+        {
+          d.r = it->v[0];
+          d.p = it->v[1];
+          d.q = it->v[2];
+          d.dinv = it->v[3];
+          B->pool.erase(it);
+        }
+        else
+        {
+          d.r = static_cast<double*>(dmalloc(sizeof(double) * n));
+          d.p = static_cast<double*>(dmalloc(sizeof(double) * n));
+          d.q = static_cast<double*>(dmalloc(sizeof(double) * n));
+          d.dinv = static_cast<double*>(dmalloc(sizeof(double) * n));
+        }
         B->hsys[s] = d;
         B->slot_used[s] = 1;
         try
@@ -438,12 +463,10 @@ dfx_status dfx_batch_remove(dfx_batch* B, int32_t slot)
       [&]
       {
         require(B && slot >= 0 && slot < B->max_sys && B->slot_used[slot], "bad slot");
-        DFX_HIP(hipStreamSynchronize(stream()));
         SysDesc& d = B->hsys[slot];
-        dfree(d.r);
-        dfree(d.p);
-        dfree(d.q);
-        dfree(d.dinv);
+        // queued work of this system precedes any later use of its vectors
+        // in stream order, so they go back to the pool at once
+        B->pool.push_back({std::max<std::int64_t>(d.nrows, 1), {d.r, d.p, d.q, d.dinv}});
         d = SysDesc{};
         B->slot_used[slot] = 0;
         SysState zero{};

@@ -56,6 +56,25 @@ __device__ inline int xcd_contiguous(int b, int nb)
   return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
 }
 
+/// Items of a persistent workgroup: XCD x (= blockIdx % 8) owns the item
+/// range [x n / 8, (x + 1) n / 8) and its workgroups stride through it, so
+/// each XCD sweeps one contiguous part (one L2 sees each neighbourhood
+/// once). Grids of fewer than 8 workgroups stride over all items.
+struct XcdRange
+{
+  int first, end, step;
+};
+__device__ inline XcdRange xcd_range(int n)
+{
+  const int nb = gridDim.x, b = blockIdx.x;
+  if (nb < 8)
+    return {b, n, nb};
+  const int x = b & 7;
+  const int lo = static_cast<int>(static_cast<long long>(n) * x / 8);
+  const int hi = static_cast<int>(static_cast<long long>(n) * (x + 1) / 8);
+  return {lo + (b >> 3), hi, (nb - x + 7) >> 3};
+}
+
 /// Sum over a 256-thread block (4 wavefronts of 64); result valid in
 /// thread 0. Fixed order, so deterministic.
 __device__ inline double block_sum_256(double v, double* lds /*[4]*/)
@@ -204,6 +223,22 @@ struct dfx_form
   std::int32_t ngblk = 0;
   std::int32_t gblk_slice = 0;     // longest slice of a row block (entries)
   std::uint64_t gather_pattern = 0;
+  // Structured Q1 rows: the cell at each of the 8 corners of a row vertex
+  // (-1 absent) and, per row, the CSR position of each of the 27 neighbour
+  // offsets (255 absent; 28 bytes per row). struct_ok: -1 not checked for
+  // struct_pattern, 0 the mesh is not a structured box, 1 usable.
+  std::int32_t* corner_cell = nullptr;
+  std::uint8_t* slotmap = nullptr;
+  std::uint64_t struct_pattern = 0;
+  int struct_ok = -1;
+  // Device packing of coefficients: for each packed position the index of
+  // its value in the caller's array (per input cell or input vertex), the
+  // mesh and layout it was built for, and a staging buffer for the values.
+  std::int32_t* pack_map = nullptr;
+  const dfx_mesh* pack_mesh = nullptr;
+  int pack_layout = -1;
+  double* pack_stage = nullptr;
+  std::int64_t pack_stage_n = 0;
   void* cell_elem = nullptr;
   int row_variant = 1;             // row-kernel variant (dfx_form_autotune)
 };

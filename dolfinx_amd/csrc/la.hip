@@ -239,16 +239,17 @@ __global__ void __launch_bounds__(kBlock)
     vv[u] = T(0);
     cc[u] = 0;
   }
-  int bi = xcd_contiguous(blockIdx.x, gridDim.x);
-  if (bi < nblk)
+  const XcdRange xr = xcd_range(nblk);
+  int bi = xr.first;
+  if (bi < xr.end)
   {
     bounds(bi);
     if (j1 > j0 && j1 - j0 <= NC)
       stage();
   }
-  while (bi < nblk)
+  while (bi < xr.end)
   {
-    const int nbi = bi + static_cast<int>(gridDim.x);
+    const int nbi = bi + xr.step;
     if (j1 - j0 <= NC)
     {
       T xv[U];
@@ -279,7 +280,7 @@ __global__ void __launch_bounds__(kBlock)
         if (dotv)
           dv = dotv[r];
       }
-      if (nbi < nblk)
+      if (nbi < xr.end)
       {
         bounds(nbi);
         if (j1 > j0 && j1 - j0 <= NC)
@@ -310,7 +311,7 @@ __global__ void __launch_bounds__(kBlock)
         if (dotv)
           d[0] += double(yr) * double(dotv[r0]);
       }
-      if (nbi < nblk)
+      if (nbi < xr.end)
       {
         bounds(nbi);
         if (j1 > j0 && j1 - j0 <= NC)
@@ -380,16 +381,17 @@ __global__ void __launch_bounds__(kBlock)
       cc[u] = ci[j];
     }
   };
-  int bi = xcd_contiguous(blockIdx.x, gridDim.x);
-  if (bi < nblk)
+  const XcdRange xr = xcd_range(nblk);
+  int bi = xr.first;
+  if (bi < xr.end)
   {
     bounds(bi);
     if (j1 > j0)
       stage();
   }
-  while (bi < nblk)
+  while (bi < xr.end)
   {
-    const int nbi = bi + static_cast<int>(gridDim.x);
+    const int nbi = bi + xr.step;
     const std::int32_t ntask = (j1 - j0) * BS;
     T xv[U][BS];
 #pragma unroll
@@ -429,7 +431,7 @@ __global__ void __launch_bounds__(kBlock)
       if (dotv)
         dv = dotv[yi];
     }
-    if (nbi < nblk)
+    if (nbi < xr.end)
     {
       bounds(nbi);
       if (j1 > j0)

@@ -195,11 +195,20 @@ class Form:
                   C.byref(v), C.byref(ms))
         return v.value, ms.value
 
+    def pack(self, mesh: Mesh, values, layout="cell"):
+        """fem::pack_coefficients (fem/pack.h:265-388) on the device: the
+        form's coefficients from values per input cell ("cell": DG0, cstride
+        values per cell) or per input vertex ("vertex": P1)."""
+        v = np.ascontiguousarray(values, np.float64).ravel()
+        code = {"cell": 0, "vertex": 1}[layout]
+        _lib.call("dfx_form_pack_coefficients", self._h, mesh.handle, code, _lib.ptr(v), v.size)
+
     def set_variant(self, variant: int):
         """Fix the assembly kernel variant (0-2 thread per row with 64/128/256
         rows per workgroup, 3 one incidence per lane, 4-6 as 0-2 with
         closed-form Q1 entries, 7 element matrices on the matrix cores then
-        gathered by rows, 8 one incidence per thread)."""
+        gathered by rows, 8 one incidence per thread, 9 structured-box Q1
+        rows in registers)."""
         _lib.call("dfx_form_set_variant", self._h, int(variant))
 
     @property

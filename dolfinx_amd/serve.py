@@ -122,11 +122,12 @@ class MeshTemplate:
     def assemble(self, req: Request):
         """A, b of the request on the device (reference pipeline: BC rows and
         columns zeroed with unit diagonal; b with lifting and bc.set)."""
-        fem_kappa = fem.pack_cellwise(self.mesh, req.kappa)
-        _lib.call("dfx_form_set_coefficients", self.a.handle, _lib.ptr(fem_kappa),
+        # coefficients packed on the device from the raw per-cell and
+        # per-vertex arrays (no host gather)
+        self.a.pack(self.mesh, req.kappa, "cell")
+        _lib.call("dfx_form_set_coefficients", self.a.handle, None,
                   _lib.ptr(np.array([req.c], np.float64)))
-        fpack = fem.pack_vertexwise(self.mesh, req.f)
-        _lib.call("dfx_form_set_coefficients", self.L.handle, _lib.ptr(fpack), None)
+        self.L.pack(self.mesh, req.f, "vertex")
         A = self.matrix()
         fem.assemble_matrix(self.a, [self.bc], 1.0, A)
         b = la.Vector(self.N)

@@ -203,6 +203,15 @@ dfx_status dfx_form_destroy(dfx_form* f);
 /* Replace packed coefficients / constants (same sizes as at creation). */
 dfx_status dfx_form_set_coefficients(dfx_form* f, const double* coeffs,
                                      const double* constants);
+/* fem::pack_coefficients (fem/pack.h:265-388) on the device: the form's
+ * packed coefficients (cells x cstride, in the mesh's cell order) gathered
+ * from values given per input cell of mesh m (DFX_PACK_CELL, cstride values
+ * per cell: DG0) or per input vertex (DFX_PACK_VERTEX, one value per
+ * vertex, cstride = vertices per cell: P1). nvalues must match. */
+#define DFX_PACK_CELL 0
+#define DFX_PACK_VERTEX 1
+dfx_status dfx_form_pack_coefficients(dfx_form* f, const dfx_mesh* m, int layout,
+                                      const double* values, int64_t nvalues);
 
 /* fem::DirichletBC (fem/DirichletBC.h:55-618) on unrolled dof indices with
  * values g (NULL: g = 0). n_total = size of the space (nodes*bs). */
@@ -225,7 +234,9 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc,
  * per lane; 4-6 as 0-2 with, for Q1 hexahedra, element entries in closed
  * form instead of reference tensors (other elements: eight incidences in
  * flight); 7 element matrices of all cells as one matrix-core product,
- * gathered by rows; 8 one incidence per thread, gathered by rows. */
+ * gathered by rows; 8 one incidence per thread, gathered by rows; 9 Q1 on a
+ * structured box of hexahedra: one thread per row accumulating its 27
+ * entries in registers (error if the mesh is not such a box). */
 dfx_status dfx_form_set_variant(dfx_form* a, int32_t variant);
 /* fem::set_diagonal (fem/assembler.h:644-686): A[d,d] = diag for BC dofs */
 dfx_status dfx_set_diagonal(dfx_csr* A, const dfx_bc* bc, double diag);

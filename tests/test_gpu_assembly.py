@@ -61,15 +61,19 @@ def test_assemble_system_matches_oracle(cell, n, dtype):
 
 
 @pytest.mark.parametrize("cell,n", CASES)
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_every_row_kernel_variant_matches_oracle(cell, n, variant, dtype):
     """Variants 0-3 use the reference tensors, 4-6 closed-form entries for
     Q1 hexahedra (eight incidences in flight for tetrahedra), 7 element
     matrices on the matrix cores gathered by rows, 8 incidence-parallel
-    gather."""
+    gather, 9 structured-box Q1 rows in registers (hexahedra only)."""
     p = build(cell, n, dtype)
     p.a.set_variant(variant)
+    if variant == 9 and cell == O.TET:
+        with pytest.raises(DfxError, match="not available"):
+            problems.assemble_system(p)
+        return
     A, b = problems.assemble_system(p)
     ref = oracle_system(p, n)
     tol = TOL[dtype]
@@ -175,3 +179,24 @@ def test_zeroing_and_accumulation_semantics():
     assert np.array_equal(d, expect) and A.to_scipy().nnz == A.nnz
     S = la.MatrixCSR.shared(A, np.float64)
     assert not np.any(S.data)
+
+
+@pytest.mark.parametrize("cell,n", CASES[:2])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_device_packing_equals_host_packing(cell, n, dtype):
+    """pack_coefficients on the device (values per input cell / vertex)
+    gives the same assembled A and b as packing on the host."""
+    p = build(cell, n, dtype)
+    A1, b1 = problems.assemble_system(p)
+    rng = np.random.default_rng(5)
+    kap = np.exp(rng.standard_normal(p.kappa.shape))
+    f = rng.standard_normal(p.f.shape)
+    p.a.pack(p.mesh, kap, "cell")
+    p.L.pack(p.mesh, f, "vertex")
+    A2, b2 = problems.assemble_system(p)
+    q = problems.scalar_problem(n, cell, kappa=kap, c=p.c, f=f, msh=p.mesh, V=p.V, dtype=dtype)
+    A3, b3 = problems.assemble_system(q)
+    assert np.array_equal(A2.data, A3.data) and np.array_equal(b2.array, b3.array)
+    assert not np.array_equal(A1.data, A2.data)
+    with pytest.raises(DfxError, match="wrong number of values"):
+        p.a.pack(p.mesh, kap[:-1], "cell")
