@@ -9,25 +9,34 @@
  * Every function names the reference file:line it restates (paths are
  * relative to the DOLFINx source tree, cpp/dolfinx/... unless stated).
  *
- * Pinning (what checks each function against something other than itself):
- *   orc_box_*            closed form of generation.h; tests check counts,
- *                        sum |K| = 1, |det J| = h^3/6, lexicographic x.
- *   orc_dual_graph       brute-force facet matching on tiny meshes (tests).
- *   orc_rcm              permutation property; brute-force George-Liu
- *                        restatement in Python on tiny graphs (tests).
- *   orc_mesh_create      first-appearance vertex numbering invariant;
- *                        geometry/topology consistency.
- *   orc_entities         edges sorted by (min,max) vertex; Euler counts.
- *   orc_dofmap           P1 dofmap == cell->vertex; Q1 likewise; P2/P3
- *                        traversal-order invariant, entity counts.
- *   orc_sparsity         brute-force dense outer-product pattern (tests).
- *   kernels              closed forms (P1), A.1 = 0, sum of mass = |Omega|,
- *                        exact integration of polynomials (tests).
- *   orc_assemble_*       closed-form global matrices on 1-cell meshes;
- *                        linearity; known answers from the reference tests.
- *   orc_cg/orc_pcg       manufactured exact solutions; scipy.sparse.linalg.
- * Functions without such a check are marked "parity unpinned" in their
- * comment.
+ * Pinning (what checks each function against something other than itself;
+ * tests/test_oracle.py unless stated):
+ *   orc_box_*            closed form of generation.h: counts, lexicographic
+ *                        coordinates, sum |K| = 1, |det J| = h^3/6.
+ *   orc_dual_graph       brute-force facet matching on tiny meshes.
+ *   orc_rcm              independent Python restatement of ordering.cpp on
+ *                        box dual graphs and random (disconnected) graphs.
+ *   orc_mesh_create_box  first-appearance vertex numbering invariant.
+ *   orc_entities         edges = lexicographic ranks of sorted vertex pairs;
+ *                        Euler characteristic V - E + F - C = 1.
+ *   orc_dofmap           independent Python restatement (P1, P2, P3); P3
+ *                        edge dofs agree between the cells of an edge.
+ *   orc_boundary_nodes   geometric brute force.
+ *   orc_sparsity         brute-force row sets.
+ *   orc_tabulate         P1 barycentric closed form; Q1 exact integration of
+ *                        trilinear fields on a parallelepiped; P3 exact
+ *                        quadratic forms of cubic fields; P2 elasticity rigid
+ *                        modes in the kernel and exact energy of linear
+ *                        fields; source totals int f (1 and 1/2 for f = 1,
+ *                        x_0, python/test/unit/fem/test_assembler.py:67-78).
+ *   orc_assemble_*       P1 mass total = |Omega|, A 1 = 0, "Entry not in
+ *                        sparsity", lifting = dense column product,
+ *                        manufactured linear solution exact at the nodes;
+ *                        frozen fixtures tests/golden/ (test_golden.py).
+ *   orc_cg, orc_pcg      scipy.sparse.linalg.spsolve; non-convergence flag.
+ * Parity unpinned (no reference run is possible here): the position of the
+ * P3 GLL edge nodes (the tests pin the P3 space, not its nodal basis), and
+ * whether the reference itself yields these dofmaps at the large sizes.
  */
 #ifndef DFX_ORACLE_H
 #define DFX_ORACLE_H
