@@ -98,6 +98,7 @@ SIGNATURES = {
     "dfx_form_destroy": (C.c_int, [P]),
     "dfx_form_set_coefficients": (C.c_int, [P, P, P]),
     "dfx_form_pack_coefficients": (C.c_int, [P, P, C.c_int, P, I64]),
+    "dfx_assemble_matrix_batched": (C.c_int, [I32, P, P, P]),
     "dfx_bc_create": (C.c_int, [P, I32, I32, P, PP]),
     "dfx_bc_destroy": (C.c_int, [P]),
     "dfx_assemble_matrix": (C.c_int, [P, P, P]),

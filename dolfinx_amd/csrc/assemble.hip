@@ -660,6 +660,34 @@ __device__ inline T mask_bits(std::uint32_t m)
     return __uint_as_float(m);
 }
 
+/// The affine factors of one cell from its Jacobian vertices q0..q3
+/// (x1-x0, x2-x0 and x3-x0 for a tetrahedron, x4-x0 for a hexahedron with
+/// lexicographic vertices): G = kappa |det J| J^-1 J^-T (6 values, through
+/// the adjugate) and c |det J|.
+template <typename T>
+__device__ inline void affine_factors(const T* q0, const T* q1, const T* q2, const T* q3,
+                                      T kappa, T creac, T (&G)[7])
+{
+  T J[9], adj[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+  {
+    J[3 * r + 0] = q1[r] - q0[r];
+    J[3 * r + 1] = q2[r] - q0[r];
+    J[3 * r + 2] = q3[r] - q0[r];
+  }
+  const T det = adjugate(J, adj);
+  const T adet = det < T(0) ? -det : det;
+  const T s = kappa / adet;
+  G[0] = s * (adj[0] * adj[0] + adj[1] * adj[1] + adj[2] * adj[2]);
+  G[1] = s * (adj[3] * adj[3] + adj[4] * adj[4] + adj[5] * adj[5]);
+  G[2] = s * (adj[6] * adj[6] + adj[7] * adj[7] + adj[8] * adj[8]);
+  G[3] = s * (adj[0] * adj[3] + adj[1] * adj[4] + adj[2] * adj[5]);
+  G[4] = s * (adj[0] * adj[6] + adj[1] * adj[7] + adj[2] * adj[8]);
+  G[5] = s * (adj[3] * adj[6] + adj[4] * adj[7] + adj[5] * adj[8]);
+  G[6] = creac * adet;
+}
+
 template <typename T, int NV>
 __global__ void __launch_bounds__(kBlock)
     cell_factors_k(std::int32_t nc, const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
@@ -672,31 +700,10 @@ __global__ void __launch_bounds__(kBlock)
   if (c >= nc)
     return;
   const std::int32_t* v = xdm + NV * c;
-  // Jacobian columns: x1-x0, x2-x0 and x3-x0 (tetrahedron) or x4-x0
-  // (hexahedron, vertices in lexicographic order)
-  const T* q0 = x + 3 * std::int64_t(v[0]);
-  const T* q1 = x + 3 * std::int64_t(v[1]);
-  const T* q2 = x + 3 * std::int64_t(v[2]);
-  const T* q3 = x + 3 * std::int64_t(v[NV == 4 ? 3 : 4]);
-  T J[9], adj[9];
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-  {
-    J[3 * r + 0] = q1[r] - q0[r];
-    J[3 * r + 1] = q2[r] - q0[r];
-    J[3 * r + 2] = q3[r] - q0[r];
-  }
-  const T det = adjugate(J, adj);
-  const T adet = det < T(0) ? -det : det;
-  const T s = coeffs[c * cstride] / adet;
-  T* G = cd + 8 * c;
-  G[0] = s * (adj[0] * adj[0] + adj[1] * adj[1] + adj[2] * adj[2]);
-  G[1] = s * (adj[3] * adj[3] + adj[4] * adj[4] + adj[5] * adj[5]);
-  G[2] = s * (adj[6] * adj[6] + adj[7] * adj[7] + adj[8] * adj[8]);
-  G[3] = s * (adj[0] * adj[3] + adj[1] * adj[4] + adj[2] * adj[5]);
-  G[4] = s * (adj[0] * adj[6] + adj[1] * adj[7] + adj[2] * adj[8]);
-  G[5] = s * (adj[3] * adj[6] + adj[4] * adj[7] + adj[5] * adj[8]);
-  G[6] = creac * adet;
+  T g[7];
+  affine_factors<T>(x + 3 * std::int64_t(v[0]), x + 3 * std::int64_t(v[1]),
+                    x + 3 * std::int64_t(v[2]), x + 3 * std::int64_t(v[NV == 4 ? 3 : 4]),
+                    coeffs[c * cstride], creac, g);
   std::uint32_t m = 0;
   if (mask)
   {
@@ -706,8 +713,23 @@ __global__ void __launch_bounds__(kBlock)
     mask[c] = m;
   }
   // the eighth slot carries the BC column mask bits (moved, never computed
-  // with), so a row kernel reads factors and mask with one load
-  G[7] = mask_bits<T>(m);
+  // with), so a row kernel reads factors and mask with one load; the record
+  // is stored with full-width vector stores
+  const T g7 = mask_bits<T>(m);
+  if constexpr (sizeof(T) == 8)
+  {
+    double2* G = reinterpret_cast<double2*>(cd + 8 * c);
+    G[0] = make_double2(g[0], g[1]);
+    G[1] = make_double2(g[2], g[3]);
+    G[2] = make_double2(g[4], g[5]);
+    G[3] = make_double2(g[6], g7);
+  }
+  else
+  {
+    float4* G = reinterpret_cast<float4*>(cd + 8 * c);
+    G[0] = make_float4(g[0], g[1], g[2], g[3]);
+    G[1] = make_float4(g[4], g[5], g[6], g7);
+  }
 }
 
 /// Incidence slot offsets: for incidence p = (cell c, local row i) and
@@ -855,21 +877,21 @@ __device__ inline void q1_signed(const T (&g)[7], int i, T (&gs)[3])
 /// group issued before use) and accumulates into the workgroup's LDS copy
 /// of its CSR slice, which is then stored with coalesced writes. With CF
 /// (affine Q1 hexahedra) the element entries come from q1_entry instead of the
-/// reference tensors, and LDS holds only the slice.
+/// reference tensors, and LDS holds only the slice. The body assembles rows
+/// [r0, r0 + blockDim.x) of one matrix; smem holds the tensors (unless CF)
+/// and the slice.
 template <typename T, typename O, int K, int U, bool CF>
-__global__ void __launch_bounds__(kBlock)
-    row_assemble_k(std::int32_t nrows, std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
-                   const std::int32_t* __restrict__ inc, const O* __restrict__ islots,
-                   const T* __restrict__ cd, const std::uint32_t* __restrict__ mask,
-                   const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
-                   const T* __restrict__ ref, T* __restrict__ vals, int accumulate)
+__device__ inline void row_block(std::int32_t r0, std::int32_t nrows, std::int32_t nplan,
+                                 const std::int32_t* __restrict__ inc_off,
+                                 const std::int32_t* __restrict__ inc, const O* __restrict__ islots,
+                                 const T* __restrict__ cd, const std::int8_t* __restrict__ bc,
+                                 const std::int32_t* __restrict__ rp, const T* __restrict__ ref,
+                                 T* __restrict__ vals, int accumulate, T* smem)
 {
   static_assert(!CF || K == 8, "closed form is the Q1 hexahedron");
-  extern __shared__ unsigned char smem_raw[];
-  T* S = reinterpret_cast<T*>(smem_raw); // reference tensors [7][K][K]
+  T* S = smem; // reference tensors [7][K][K]
   T* acc = CF ? S : S + 7 * K * K;
   const int R = blockDim.x;
-  const std::int32_t r0 = xcd_contiguous(blockIdx.x, gridDim.x) * R;
   const std::int32_t r1 = min(r0 + R, nrows);
   const std::int32_t base = rp[r0];
   const std::int32_t len = rp[r1] - base;
@@ -923,6 +945,220 @@ __global__ void __launch_bounds__(kBlock)
           for (int j = 0; j < K; ++j)
             if (!((m[u] >> j) & 1u))
               a[sl[u][j]] += element_entry<T, K>(g[u], S, i, j);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (accumulate)
+    for (int t = threadIdx.x; t < len; t += R)
+      vals[base + t] += acc[t];
+  else
+    for (int t = threadIdx.x; t < len; t += R)
+      vals[base + t] = acc[t];
+}
+
+template <typename T, typename O, int K, int U, bool CF>
+__global__ void __launch_bounds__(kBlock)
+    row_assemble_k(std::int32_t nrows, std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
+                   const std::int32_t* __restrict__ inc, const O* __restrict__ islots,
+                   const T* __restrict__ cd, const std::uint32_t* __restrict__ mask,
+                   const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
+                   const T* __restrict__ ref, T* __restrict__ vals, int accumulate)
+{
+  (void)mask; // carried in the factor records
+  extern __shared__ unsigned char smem_raw[];
+  row_block<T, O, K, U, CF>(xcd_contiguous(blockIdx.x, gridDim.x) * blockDim.x, nrows, nplan,
+                            inc_off, inc, islots, cd, bc, rp, ref, vals, accumulate,
+                            reinterpret_cast<T*>(smem_raw));
+}
+
+// ---------------------------------------------------------------------
+// Batched assembly of many independent systems of one affine kernel (no
+// counterpart in the reference): one launch forms the cell factors of all
+// systems and one assembles the rows of all systems, from device tables of
+// per-system pointers and of (system, first cell / first row) work items.
+// ---------------------------------------------------------------------
+
+template <typename T>
+struct BatchSys
+{
+  const T* x;
+  const std::int32_t* xdm;
+  const std::int32_t* dm;
+  const T* coeffs;
+  const std::int8_t* bc;     // markers or nullptr
+  T* cd;                     // cell factor records (8 per cell)
+  const std::int32_t* inc_off;
+  const std::int32_t* inc;
+  const std::uint8_t* islots;
+  const std::int32_t* rp;
+  T* vals;
+  T creac;
+  std::int32_t cstride, nc, nrows, nplan, accumulate;
+};
+
+template <typename T, int NV>
+__global__ void __launch_bounds__(kBlock)
+    batch_factors_k(const BatchSys<T>* __restrict__ sys, const int2* __restrict__ work)
+{
+  const int2 wi = work[blockIdx.x];
+  const BatchSys<T>& S = sys[wi.x];
+  const std::int64_t c = wi.y + static_cast<std::int64_t>(threadIdx.x);
+  if (c >= S.nc)
+    return;
+  const std::int32_t* v = S.xdm + NV * c;
+  T g[7];
+  affine_factors<T>(S.x + 3 * std::int64_t(v[0]), S.x + 3 * std::int64_t(v[1]),
+                    S.x + 3 * std::int64_t(v[2]), S.x + 3 * std::int64_t(v[NV == 4 ? 3 : 4]),
+                    S.coeffs[c * S.cstride], S.creac, g);
+  std::uint32_t m = 0;
+  if (S.bc)
+    for (int j = 0; j < NV; ++j)
+      if (S.bc[S.dm[c * NV + j]])
+        m |= 1u << j;
+  T* G = S.cd + 8 * c;
+#pragma unroll
+  for (int i = 0; i < 7; ++i)
+    G[i] = g[i];
+  G[7] = mask_bits<T>(m);
+}
+
+template <typename T, int K, bool CF>
+__global__ void __launch_bounds__(kBlock)
+    batch_rows_k(const BatchSys<T>* __restrict__ sys, const int2* __restrict__ work,
+                 const T* __restrict__ ref)
+{
+  extern __shared__ unsigned char smem_raw[];
+  const int2 wi = work[blockIdx.x];
+  const BatchSys<T>& S = sys[wi.x];
+  row_block<T, std::uint8_t, K, 4, CF>(wi.y, S.nrows, S.nplan, S.inc_off, S.inc, S.islots, S.cd,
+                                       S.bc, S.rp, ref, S.vals, S.accumulate,
+                                       reinterpret_cast<T*>(smem_raw));
+}
+
+/// Variant "fused geometry": as row_assemble_k, but every incidence forms
+/// its cell's affine factors from the cell's vertices, coefficient and BC
+/// markers (no cell pass and no factor array): per incidence the cell's
+/// vertex row and kappa are read, then its four Jacobian vertices
+/// (neighbouring rows share them in cache), instead of a 64-byte factor
+/// record written by a separate pass. Loads are staged for eight
+/// incidences at a time (indices, then vertex rows, then coordinates four
+/// cells at a time) at clamped positions, so each stage is one round trip.
+/// With SAME the dofmap is the vertex map (P1/Q1 on the mesh's own
+/// numbering) and the BC markers are read through the vertex row.
+template <typename T, typename O, int K, bool CF, bool SAME>
+__global__ void __launch_bounds__(kBlock)
+    row_fused_k(std::int32_t nrows, std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
+                const std::int32_t* __restrict__ inc, const O* __restrict__ islots,
+                const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
+                const std::int32_t* __restrict__ dm, const T* __restrict__ coeffs, int cstride,
+                T creac, const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
+                const T* __restrict__ ref, T* __restrict__ vals, int accumulate)
+{
+  static_assert(!CF || K == 8, "closed form is the Q1 hexahedron");
+  static_assert(K == 4 || K == 8, "affine P1 tetrahedra or Q1 hexahedra");
+  constexpr int NV = K;               // vertices per cell
+  constexpr int JV = K == 4 ? 3 : 4;  // third Jacobian vertex
+  extern __shared__ unsigned char smem_raw[];
+  T* S = reinterpret_cast<T*>(smem_raw); // reference tensors [7][K][K]
+  T* acc = CF ? S : S + 7 * K * K;
+  const int R = blockDim.x;
+  const std::int32_t r0 = xcd_contiguous(blockIdx.x, gridDim.x) * R;
+  const std::int32_t r1 = min(r0 + R, nrows);
+  const std::int32_t base = rp[r0];
+  const std::int32_t len = rp[r1] - base;
+  if constexpr (!CF)
+    for (int t = threadIdx.x; t < 7 * K * K; t += R)
+      S[t] = ref[t];
+  for (int t = threadIdx.x; t < len; t += R)
+    acc[t] = T(0);
+  __syncthreads();
+  const std::int32_t r = r0 + threadIdx.x;
+  if (r < r1 && r < nplan && !(bc && bc[r]))
+  {
+    T* a = acc + (rp[r] - base);
+    const std::int32_t p0 = inc_off[r], pe = inc_off[r + 1];
+    for (std::int32_t pb = p0; pb < pe; pb += 8)
+    {
+      // stage 1: incidences and slot offsets
+      std::int32_t cik[8];
+      O sl[8][K];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+      {
+        const std::int32_t p = min(pb + u, pe - 1);
+        cik[u] = inc[p];
+        load_slots<O, K>(islots, p, sl[u]);
+      }
+      // stage 2: vertex rows, kappa, BC column masks
+      std::int32_t vid[8][4];
+      T kap[8];
+      std::uint32_t m[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+      {
+        const std::int64_t c = cik[u] / K;
+        const int4 va = *reinterpret_cast<const int4*>(xdm + NV * c);
+        int row[K];
+        row[0] = va.x; row[1] = va.y; row[2] = va.z; row[3] = va.w;
+        if constexpr (K == 8)
+        {
+          const int4 vb = *reinterpret_cast<const int4*>(xdm + NV * c + 4);
+          row[4] = vb.x; row[5] = vb.y; row[6] = vb.z; row[7] = vb.w;
+        }
+        vid[u][0] = row[0];
+        vid[u][1] = row[1];
+        vid[u][2] = row[2];
+        vid[u][3] = row[JV];
+        kap[u] = coeffs[c * cstride];
+        std::uint32_t mm = 0;
+        if (bc)
+        {
+#pragma unroll
+          for (int j = 0; j < K; ++j)
+            if (bc[SAME ? row[j] : dm[c * K + j]])
+              mm |= 1u << j;
+        }
+        m[u] = mm;
+      }
+      // stage 3: coordinates and factors, four cells at a time
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+      {
+        T q[4][4][3];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int w = 0; w < 4; ++w)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+              q[u][w][d] = x[3 * std::int64_t(vid[4 * h + u][w]) + d];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+        {
+          const int uu = 4 * h + u;
+          if (pb + uu >= pe)
+            continue;
+          T g[7];
+          affine_factors<T>(q[u][0], q[u][1], q[u][2], q[u][3], kap[uu], creac, g);
+          const int i = cik[uu] % K;
+          if constexpr (CF)
+          {
+            T gs[3];
+            q1_signed<T>(g, i, gs);
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+              if (!((m[uu] >> j) & 1u))
+                a[sl[uu][j]] += q1_entry<T>(g, gs, i, j);
+          }
+          else
+          {
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+              if (!((m[uu] >> j) & 1u))
+                a[sl[uu][j]] += element_entry<T, K>(g, S, i, j);
+          }
         }
       }
     }
@@ -2022,7 +2258,7 @@ int incidence_lanes(const dfx_form& f)
 
 /// Row-kernel variants, picked per form by dfx_form_autotune:
 /// 0..2: "rows" with 64/128/256 rows per workgroup; 3: "lanes".
-constexpr int kRowVariants = 10;
+constexpr int kRowVariants = 11;
 
 template <typename T, typename O, int K, int L>
 void launch_lanes(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
@@ -2202,6 +2438,37 @@ template <typename T, typename O, int K>
 void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
 {
   const int variant = f.row_variant;
+  if (variant == 10)
+  {
+    // fused geometry: no cell pass; 64 rows per workgroup
+    require(f.affine && (K == 4 || K == 8) && f.d.kernel != DFX_K_P3TET_A,
+            "variant not available for this element");
+    if constexpr (K == 4 || K == 8)
+    {
+      constexpr bool cf = K == 8;
+      const int R = 64;
+      const std::size_t lds = ((cf ? 0 : 7 * K * K) + std::size_t(R) * A.max_row) * sizeof(T);
+      require(lds <= 64 * 1024, "row too long for the LDS slice");
+      const int nb = (A.nrows + R - 1) / R;
+      auto go = [&](auto kern)
+      {
+        kern<<<nb, R, lds, stream()>>>(
+            A.nrows, f.max_dof + 1, f.inc_off, f.inc, static_cast<const O*>(f.islots),
+            static_cast<const T*>(f.x), f.x_dofmap, f.dofmap, static_cast<const T*>(f.coeffs),
+            f.d.cstride, static_cast<T>(f.constants[0]), bc ? bc->marker : nullptr, A.row_ptr,
+            static_cast<const T*>(f.ref_tensors), static_cast<T*>(A.values), A.zeroed ? 0 : 1);
+      };
+      if (nb > 0)
+      {
+        if (f.x_dofmap == f.dofmap)
+          go(row_fused_k<T, O, K, cf, true>);
+        else
+          go(row_fused_k<T, O, K, cf, false>);
+      }
+      check_launch("row_assemble");
+    }
+    return;
+  }
   if (variant == 9)
   {
     if constexpr (K == 8)
@@ -2462,6 +2729,116 @@ bool hex_all_affine(const dfx_form_desc& d)
     }
   }
   return true;
+}
+
+/// Device copies of the batch tables, grown as needed and reused.
+struct BatchTables
+{
+  void* sys = nullptr;
+  std::size_t sys_bytes = 0;
+  int2* cwork = nullptr;
+  std::size_t cwork_n = 0;
+  int2* rwork = nullptr;
+  std::size_t rwork_n = 0;
+};
+BatchTables& batch_tables()
+{
+  static BatchTables t;
+  return t;
+}
+
+template <typename X>
+void upload_grow(X*& dst, std::size_t& cap, const X* src, std::size_t n)
+{
+  if (n > cap)
+  {
+    dfree(dst);
+    dst = static_cast<X*>(dmalloc(sizeof(X) * n));
+    cap = n;
+  }
+  h2d(dst, src, sizeof(X) * n);
+}
+
+template <typename T>
+void batch_assemble(int n, dfx_csr* const* A, const dfx_form* const* a, const dfx_bc* const* bc)
+{
+  const int kernel = a[0]->d.kernel;
+  const int K = a[0]->d.dofs_per_cell;
+  constexpr int R = 64;
+  std::vector<BatchSys<T>> sys(n);
+  std::vector<int2> cw, rw;
+  std::int32_t max_row = 0;
+  for (int s = 0; s < n; ++s)
+  {
+    dfx_form& f = const_cast<dfx_form&>(*a[s]);
+    dfx_csr& M = *A[s];
+    require(f.d.kernel == kernel && row_path(f), "batched forms must share one affine kernel");
+    require(M.dtype == f.d.dtype && M.bs == 1 && f.d.bs == 1, "matrix and form do not match");
+    require(!bc[s] || bc[s]->n_total == M.nrows, "BC size does not match the matrix");
+    require(f.max_dof < M.nrows, "dofmap exceeds the matrix size");
+    ensure_plan(f, M);
+    require(f.islot_bytes == 1, "row too long for the batched kernel");
+    max_row = std::max(max_row, M.max_row);
+    BatchSys<T>& d = sys[s];
+    d.x = static_cast<const T*>(f.x);
+    d.xdm = f.x_dofmap;
+    d.dm = f.dofmap;
+    d.coeffs = static_cast<const T*>(f.coeffs);
+    d.bc = bc[s] ? bc[s]->marker : nullptr;
+    d.cd = static_cast<T*>(f.cell_data);
+    d.inc_off = f.inc_off;
+    d.inc = f.inc;
+    d.islots = static_cast<const std::uint8_t*>(f.islots);
+    d.rp = M.row_ptr;
+    d.vals = static_cast<T*>(M.values);
+    d.creac = static_cast<T>(f.constants[0]);
+    d.cstride = f.d.cstride;
+    d.nc = f.d.num_cells;
+    d.nrows = M.nrows;
+    d.nplan = f.max_dof + 1;
+    d.accumulate = M.zeroed ? 0 : 1;
+    for (std::int32_t c0 = 0; c0 < d.nc; c0 += kBlock)
+      cw.push_back(make_int2(s, c0));
+    for (std::int32_t r0 = 0; r0 < d.nrows; r0 += R)
+      rw.push_back(make_int2(s, r0));
+  }
+  const bool cf = kernel == DFX_K_Q1HEX_A;
+  const std::size_t lds = ((cf ? 0 : 7 * K * K) + std::size_t(R) * max_row) * sizeof(T);
+  require(lds <= 64 * 1024, "row too long for the LDS slice");
+  BatchTables& t = batch_tables();
+  {
+    std::size_t cap = t.sys_bytes / sizeof(BatchSys<T>);
+    BatchSys<T>* p = static_cast<BatchSys<T>*>(t.sys);
+    upload_grow(p, cap, sys.data(), sys.size());
+    t.sys = p;
+    t.sys_bytes = cap * sizeof(BatchSys<T>);
+  }
+  if (!cw.empty())
+    upload_grow(t.cwork, t.cwork_n, cw.data(), cw.size());
+  if (!rw.empty())
+    upload_grow(t.rwork, t.rwork_n, rw.data(), rw.size());
+  const BatchSys<T>* dsys = static_cast<const BatchSys<T>*>(t.sys);
+  const T* ref = static_cast<const T*>(a[0]->ref_tensors);
+  if (!cw.empty())
+  {
+    if (K == 4)
+      batch_factors_k<T, 4><<<static_cast<int>(cw.size()), kBlock, 0, stream()>>>(dsys, t.cwork);
+    else
+      batch_factors_k<T, 8><<<static_cast<int>(cw.size()), kBlock, 0, stream()>>>(dsys, t.cwork);
+  }
+  if (!rw.empty())
+  {
+    if (K == 4)
+      batch_rows_k<T, 4, false><<<static_cast<int>(rw.size()), R, lds, stream()>>>(dsys, t.rwork, ref);
+    else
+      batch_rows_k<T, 8, true><<<static_cast<int>(rw.size()), R, lds, stream()>>>(dsys, t.rwork, ref);
+  }
+  check_launch("assemble_matrix_batched");
+  for (int s = 0; s < n; ++s)
+  {
+    A[s]->zeroed = false;
+    A[s]->zero_pending = false;
+  }
 }
 
 } // namespace
@@ -2782,6 +3159,28 @@ dfx_status dfx_assemble_matrix(dfx_csr* A, const dfx_form* a, const dfx_bc* bc)
           else
             launch_bilinear<float, std::uint16_t, 0>(f, A, bc, nullptr, 0.0, nullptr);
         }
+      });
+}
+
+dfx_status dfx_assemble_matrix_batched(int32_t n, dfx_csr* const* A, const dfx_form* const* a,
+                                       const dfx_bc* const* bc)
+{
+  return guard(
+      [&]
+      {
+        require(n >= 0 && (n == 0 || (A && a && bc)), "null argument");
+        if (n == 0)
+          return;
+        for (int s = 0; s < n; ++s)
+          require(A[s] && a[s], "null matrix or form");
+        const int dt = a[0]->d.dtype;
+        const int k = a[0]->d.kernel;
+        require(k == DFX_K_P1TET_A || k == DFX_K_Q1HEX_A,
+                "batched assembly supports the P1 and Q1 stiffness forms");
+        if (dt == DFX_F64)
+          batch_assemble<double>(n, A, a, bc);
+        else
+          batch_assemble<float>(n, A, a, bc);
       });
 }
 

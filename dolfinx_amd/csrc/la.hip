@@ -809,11 +809,12 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 /// Re-arms a stopped CG after a residual replacement.
-__global__ void cg_resume_k(CGState* st, double rr0)
+__global__ void cg_resume_k(CGState* st, double rr0, double tol2)
 {
   if (threadIdx.x == 0 && blockIdx.x == 0)
   {
     st->rr0 = rr0;
+    st->tol2 = tol2;
     st->done = 0;
   }
 }
@@ -1607,6 +1608,14 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
         };
         int outer = 0, inner = 0, inner_base = 0;
         double rr = true_residual_split();
+        // an inner solve stops when its residual fell by inner_rtol, or
+        // earlier once it meets half the outer target (the last refinement
+        // is not solved past what the outer test needs)
+        auto inner_tol2 = [&](double rr_outer)
+        {
+          const double fin = 0.25 * tol2 / std::max(rr_outer, 1e-300);
+          return std::max(o->inner_rtol * o->inner_rtol, fin);
+        };
         bool ok = rr <= tol2;
         const bool restart = o->restart != 0;
         if (!ok)
@@ -1622,7 +1631,7 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
           // r = r32 - A32 * 0, p = z: reuse cg_start with y = 0
           cg_start_k<float><<<nbv, kBlock, 0, s>>>(n, w.r, w.q, w.dinv, w.r, w.p, st);
           cg_start_finish_k<<<1, kBlock, 0, s>>>(nbv, st);
-          cg_resume_k<<<1, 1, 0, s>>>(st, rr);
+          cg_resume_k<<<1, 1, 0, s>>>(st, rr, inner_tol2(rr));
           check_launch("mixed start");
         }
         int k = 0;
@@ -1664,7 +1673,7 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
             fill_k<float><<<nb, kBlock, 0, s>>>(w.q, n, 0.0f);
             cg_start_k<float><<<nbv, kBlock, 0, s>>>(n, w.r, w.q, w.dinv, w.r, w.p, st);
             cg_start_finish_k<<<1, kBlock, 0, s>>>(nbv, st);
-            cg_resume_k<<<1, 1, 0, s>>>(st, rr);
+            cg_resume_k<<<1, 1, 0, s>>>(st, rr, inner_tol2(rr));
             k = 0;
           }
           else
@@ -1672,7 +1681,7 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
             // reliable update: keep the search direction; beta is formed
             // from the replaced residual, rz_new / rz_old
             const int kk = h.iter - 1;
-            cg_resume_k<<<1, 1, 0, s>>>(st, rr);
+            cg_resume_k<<<1, 1, 0, s>>>(st, rr, inner_tol2(rr));
             cg_rz_partials_k<float><<<nbv, kBlock, 0, s>>>(n, w.r, w.dinv, st);
             cg_direction_k<float><<<nbv, kBlock, 0, s>>>(n, kk, nbv, w.r, w.dinv, w.p, st);
             k = kk + 1;

@@ -208,7 +208,8 @@ class Form:
         rows per workgroup, 3 one incidence per lane, 4-6 as 0-2 with
         closed-form Q1 entries, 7 element matrices on the matrix cores then
         gathered by rows, 8 one incidence per thread, 9 structured-box Q1
-        rows in registers)."""
+        rows in registers, 10 factors formed from the geometry per
+        incidence)."""
         _lib.call("dfx_form_set_variant", self._h, int(variant))
 
     @property
@@ -254,6 +255,25 @@ def assemble_matrix(a: Form, bcs=(), diag=1.0, A: MatrixCSR | None = None) -> Ma
     for bc in bcs:
         _lib.call("dfx_set_diagonal", A.handle, bc.handle, float(diag))
     return A
+
+
+def assemble_matrix_batched(forms, bcs, matrices, diag=1.0) -> None:
+    """assemble_matrix for many independent systems of one P1 or Q1
+    stiffness kernel in two launches (cell factors of all systems, rows of
+    all systems); bcs[i] may be None. Then ``diag`` on each BC diagonal."""
+    n = len(forms)
+    if not (len(bcs) == len(matrices) == n):
+        raise ValueError("forms, bcs and matrices differ in length")
+    if n == 0:
+        return
+
+    def arr(objs):
+        return (C.c_void_p * n)(*[None if o is None else o.handle.value for o in objs])
+
+    _lib.call("dfx_assemble_matrix_batched", n, arr(matrices), arr(forms), arr(bcs))
+    for A, bc in zip(matrices, bcs):
+        if bc is not None:
+            _lib.call("dfx_set_diagonal", A.handle, bc.handle, float(diag))
 
 
 def assemble_vector(L: Form, b: Vector | None = None) -> Vector:

@@ -9,10 +9,10 @@ the device forms; a request only brings its coefficients and owns its
 matrix values and vectors while it is resident.
 
 Requests are admitted in arrival order while a batch slot is free and
-their HBM footprint fits the budget; admission assembles the request's A
-and b on the device and joins it to the running segmented CG batch
-(include/dfx.h: dfx_batch_*), so new requests start iterating while earlier
-ones are still on the GPU. Converged requests leave the batch and free
+their HBM footprint fits the budget; the requests admitted together are
+assembled by one batched launch pair (dfx_assemble_matrix_batched) and
+joined to the running segmented CG batch (dfx_batch_*), so new requests
+start iterating while earlier ones are still on the GPU. Converged requests leave the batch and free
 their memory. Latency is measured from arrival to the completed solution.
 """
 
@@ -119,22 +119,29 @@ class MeshTemplate:
     def matrix(self) -> la.MatrixCSR:
         return la.MatrixCSR.shared(self.pattern, np.float64)
 
-    def assemble(self, req: Request):
-        """A, b of the request on the device (reference pipeline: BC rows and
-        columns zeroed with unit diagonal; b with lifting and bc.set)."""
-        # coefficients packed on the device from the raw per-cell and
-        # per-vertex arrays (no host gather)
+    def load(self, req: Request) -> None:
+        """The request's coefficients into the template's forms, packed on
+        the device from the raw per-cell and per-vertex arrays."""
         self.a.pack(self.mesh, req.kappa, "cell")
         _lib.call("dfx_form_set_coefficients", self.a.handle, None,
                   _lib.ptr(np.array([req.c], np.float64)))
         self.L.pack(self.mesh, req.f, "vertex")
-        A = self.matrix()
-        fem.assemble_matrix(self.a, [self.bc], 1.0, A)
+
+    def rhs(self) -> la.Vector:
+        """b of the loaded request: assemble_vector, apply_lifting, set_bc."""
         b = la.Vector(self.N)
         fem.assemble_vector(self.L, b)
         fem.apply_lifting(b, [self.a], [[self.bc]])
         fem.set_bc(b, [self.bc])
-        return A, b
+        return b
+
+    def assemble(self, req: Request):
+        """A, b of the request on the device (reference pipeline: BC rows and
+        columns zeroed with unit diagonal; b with lifting and bc.set)."""
+        self.load(req)
+        A = self.matrix()
+        fem.assemble_matrix(self.a, [self.bc], 1.0, A)
+        return A, self.rhs()
 
 
 @dataclass
@@ -203,20 +210,35 @@ class SolveService:
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
                 queue.append(pending.popleft())
-            # admission: FIFO while a slot is free and the footprint fits
-            while queue and len(active) < self.max_slots:
+            # admission: FIFO while a slot is free and the footprint fits;
+            # the admitted group (one request per mesh size, as a template's
+            # forms hold one request's coefficients) is assembled by one
+            # batched launch pair
+            group = []
+            while queue and len(active) + len(group) < self.max_slots:
                 req = queue[0]
                 tpl = self.templates[req.n]
                 fp = tpl.footprint()
-                if used + fp > self.budget and active:
+                if used + fp > self.budget and (active or group):
+                    break
+                if any(g[1] is tpl for g in group):
                     break
                 queue.popleft()
-                A, b = tpl.assemble(req)
-                x = la.Vector(tpl.N)
-                slot = self.batch.add(A, b, x, self.rtol)
-                req.admitted = time.perf_counter() - t0
-                active[slot] = (req, A, b, x, fp)
+                group.append((req, tpl, fp))
                 used += fp
+            if group:
+                mats = []
+                for req, tpl, _ in group:
+                    tpl.load(req)
+                    mats.append(tpl.matrix())
+                fem.assemble_matrix_batched([g[1].a for g in group], [g[1].bc for g in group],
+                                            mats)
+                for (req, tpl, fp), A in zip(group, mats):
+                    b = tpl.rhs()
+                    x = la.Vector(tpl.N)
+                    slot = self.batch.add(A, b, x, self.rtol)
+                    req.admitted = time.perf_counter() - t0
+                    active[slot] = (req, A, b, x, fp)
             stats.max_resident = max(stats.max_resident, len(active))
             stats.max_bytes = max(stats.max_bytes, used)
             if not active:

@@ -223,6 +223,13 @@ dfx_status dfx_bc_destroy(dfx_bc* bc);
  * A += a with rows and columns of the BC dofs zeroed (bc may be NULL). */
 dfx_status dfx_assemble_matrix(dfx_csr* A, const dfx_form* a,
                                const dfx_bc* bc);
+/* Batched fem::assemble_matrix over n independent systems of one affine
+ * P1 or Q1 stiffness kernel (no counterpart in the reference; for many
+ * meshes of uneven size): one launch forms the cell factors of all systems
+ * and one assembles the rows of all systems. A[s] += a[s] with the rows and
+ * columns of bc[s] (may be NULL) zeroed, as dfx_assemble_matrix. */
+dfx_status dfx_assemble_matrix_batched(int32_t n, dfx_csr* const* A, const dfx_form* const* a,
+                                       const dfx_bc* const* bc);
 /* Assembly autotuner (no counterpart in the reference): times the
  * row-kernel variants (rows per workgroup, lanes per row) of form a on
  * matrix A and keeps the fastest for later assemblies of a. Overwrites A's
@@ -236,7 +243,9 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc,
  * flight); 7 element matrices of all cells as one matrix-core product,
  * gathered by rows; 8 one incidence per thread, gathered by rows; 9 Q1 on a
  * structured box of hexahedra: one thread per row accumulating its 27
- * entries in registers (error if the mesh is not such a box). */
+ * entries in registers (error if the mesh is not such a box); 10 as 4 with
+ * the cell factors formed from the geometry by every incidence (no cell
+ * pass; affine P1/Q1). */
 dfx_status dfx_form_set_variant(dfx_form* a, int32_t variant);
 /* fem::set_diagonal (fem/assembler.h:644-686): A[d,d] = diag for BC dofs */
 dfx_status dfx_set_diagonal(dfx_csr* A, const dfx_bc* bc, double diag);
@@ -302,8 +311,9 @@ typedef struct
 /* Mixed-precision iterative refinement (no counterpart in the reference;
  * it wraps the same CG): Jacobi-PCG on A32 in FP32 (FP64 reductions)
  * accumulates a correction d; each time its residual has fallen by
- * inner_rtol, x += d in FP64 and the residual is replaced by the FP64 true
- * residual b - A64 x. With restart = 0 the FP32 CG keeps its search
+ * inner_rtol (or meets half the outer target, so that the last refinement
+ * is not solved further than the outer test needs), x += d in FP64 and the
+ * residual is replaced by the FP64 true residual b - A64 x. With restart = 0 the FP32 CG keeps its search
  * direction across replacements (reliable updates, van der Vorst & Ye
  * 2000); with restart = 1 every refinement starts a new inner CG. */
 dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32,

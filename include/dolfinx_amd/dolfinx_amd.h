@@ -414,6 +414,14 @@ public:
   {
     return std::vector<std::shared_ptr<const FunctionSpace>>(_rank, _V);
   }
+  /// fem::pack_coefficients (fem/pack.h:265-388) on the device: the
+  /// coefficients from values per input cell (DFX_PACK_CELL, DG0) or per
+  /// input vertex (DFX_PACK_VERTEX, P1) of the form's mesh.
+  void pack_coefficients(std::span<const double> values, int layout)
+  {
+    check(dfx_form_pack_coefficients(_h.get(), _V->mesh()->handle(), layout, values.data(),
+                                     static_cast<std::int64_t>(values.size())));
+  }
   /// Pick the assembly kernel variant for the pattern of A by timing
   std::pair<int, float> autotune(la::MatrixCSR<T>& A, const DirichletBC<T>* bc = nullptr)
   {

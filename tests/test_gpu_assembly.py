@@ -61,13 +61,14 @@ def test_assemble_system_matches_oracle(cell, n, dtype):
 
 
 @pytest.mark.parametrize("cell,n", CASES)
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_every_row_kernel_variant_matches_oracle(cell, n, variant, dtype):
     """Variants 0-3 use the reference tensors, 4-6 closed-form entries for
     Q1 hexahedra (eight incidences in flight for tetrahedra), 7 element
     matrices on the matrix cores gathered by rows, 8 incidence-parallel
-    gather, 9 structured-box Q1 rows in registers (hexahedra only)."""
+    gather, 9 structured-box Q1 rows in registers (hexahedra only), 10 rows
+    forming the cell factors from the geometry (no cell pass)."""
     p = build(cell, n, dtype)
     p.a.set_variant(variant)
     if variant == 9 and cell == O.TET:
@@ -200,3 +201,30 @@ def test_device_packing_equals_host_packing(cell, n, dtype):
     assert not np.array_equal(A1.data, A2.data)
     with pytest.raises(DfxError, match="wrong number of values"):
         p.a.pack(p.mesh, kap[:-1], "cell")
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batched_assembly_equals_single(dtype):
+    """One batched call over meshes of uneven size (tetrahedra; and
+    hexahedra in a second batch) gives the matrices of the single calls."""
+    for cell, sizes in [(O.TET, [(3, 4, 2), (6, 5, 4), (2, 2, 2), (8, 7, 6)]),
+                        (O.HEX, [(4, 3, 5), (7, 6, 5), (2, 3, 2)])]:
+        probs = [problems.scalar_problem(n, cell, kappa="exp_normal", c=1.5, f="normal",
+                                         seed=i, dtype=dtype) for i, n in enumerate(sizes)]
+        if cell == O.HEX:
+            for p in probs:   # the batched kernel forms Q1 entries in closed form
+                p.a.set_variant(4)
+        single = [fem.assemble_matrix(p.a, [p.bc]).data for p in probs]
+        mats = [fem.create_matrix(p.a) for p in probs]
+        bcs = [p.bc if i % 2 == 0 else None for i, p in enumerate(probs)]
+        fem.assemble_matrix_batched([p.a for p in probs], bcs, mats)
+        for i, (A, ref, p) in enumerate(zip(mats, single, probs)):
+            if bcs[i] is None:
+                ref = fem.assemble_matrix(p.a, []).data
+            assert np.array_equal(A.data, ref), (cell, i)
+        # a second call accumulates
+        fem.assemble_matrix_batched([p.a for p in probs], [None] * len(probs), mats, 1.0)
+        for i, (A, p) in enumerate(zip(mats, probs)):
+            base = fem.assemble_matrix(p.a, [p.bc] if bcs[i] is not None else []).data
+            add = fem.assemble_matrix(p.a, []).data
+            assert np.allclose(A.data, base + add, rtol=1e-6 if dtype == np.float32 else 1e-14)
