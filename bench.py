@@ -31,6 +31,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes of one FP64 dfx_assemble_matrix call (cell_factors_k + the row
+# kernel, store mode) on the 128^3 hexahedra, per row-kernel variant, from
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this bench with --variant V,
+# scaled by the counters' reading for a 1 GiB device copy (profiles/
+# q1hex128_assembly_spmv_cg.md gives the commands and the per-kernel figures).
+ASSEMBLY_TRAFFIC_BYTES = {1: 1320113935, 4: 1326775916}
+
 
 def ensure_built():
     if not os.path.exists(os.path.join(ROOT, "dolfinx_amd", "libdfx.so")):
@@ -178,6 +185,12 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--only-configs", action="store_true",
+                    help="run only the other workloads (P1 32^3, P2 48^3, P3 24^3) and print "
+                         "their figures (no benchmark line; for profiling them)")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="fix the FP64/FP32 assembly variant of the timed workload "
+                         "(default: autotuned)")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     args = ap.parse_args()
@@ -188,6 +201,10 @@ def main():
     ensure_built()
     from dolfinx_amd import _lib, fem, la, problems, solvers
     from dolfinx_amd.mesh import CellType
+
+    if args.only_configs:
+        print(json.dumps({"other_configs": other_configs(args.steps)}))
+        return
 
     n = args.n
     t_setup = time.perf_counter()
@@ -207,8 +224,13 @@ def main():
     Nv = p64.mesh.geometry.x.shape[0]
     print(f"# setup {setup_s:.1f} s: cells {C} dofs {N} nnz {nnz}", file=sys.stderr, flush=True)
 
-    variant64 = p64.a.autotune(A64, p64.bc)
-    variant32 = p32.a.autotune(A32, p32.bc)
+    if args.variant < 0:
+        variant64 = p64.a.autotune(A64, p64.bc)
+        variant32 = p32.a.autotune(A32, p32.bc)
+    else:
+        p64.a.set_variant(args.variant)
+        p32.a.set_variant(args.variant)
+        variant64 = variant32 = (args.variant, float("nan"))
     print(f"# assembly variants: f64 {variant64} f32 {variant32}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         problems.assemble_system(p64, A64, b64)
@@ -387,7 +409,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(asm_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": ASSEMBLY_TRAFFIC_BYTES.get(variant64[0]) if n == 128 else None,
             "algorithmic_bytes": asm_bytes,
             "kernel_ms": round(asm_ms, 4),
         },

@@ -228,3 +228,21 @@ def test_batched_assembly_equals_single(dtype):
             base = fem.assemble_matrix(p.a, [p.bc] if bcs[i] is not None else []).data
             add = fem.assemble_matrix(p.a, []).data
             assert np.allclose(A.data, base + add, rtol=1e-6 if dtype == np.float32 else 1e-14)
+
+
+@pytest.mark.parametrize("cell", [O.TET, O.HEX])
+def test_single_cell_box_and_zero_rhs(cell):
+    """1x1x1 boxes (every dof on the boundary, fewer rows than a workgroup):
+    the system matches the oracle, A is the identity and CG returns x = 0 at
+    once for b = 0."""
+    from dolfinx_amd import solvers
+
+    p = build(cell, (1, 1, 1), np.float64, f="one")
+    A, b = problems.assemble_system(p)
+    ref = oracle_system(p, (1, 1, 1))
+    assert np.array_equal(A.data, ref["A"]) or rel_err(A.data, ref["A"]) <= 1e-12
+    assert np.allclose(A.to_dense(), np.eye(p.V.num_dofs))
+    assert not np.any(b.array)
+    x = la.Vector(p.V.num_dofs)
+    res = solvers.cg(A, b, x, rtol=1e-10)
+    assert res.converged and res.iterations == 0 and not np.any(x.array)
