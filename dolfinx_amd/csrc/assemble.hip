@@ -1677,26 +1677,30 @@ __global__ void __launch_bounds__(kBlock)
         out[i] += adet / T(8) * fq * phi[i];
     }
   }
-  // cell-major element vectors: a wavefront stores one contiguous range
+  // corner-major element vectors, be[i][c]: consecutive cells of a
+  // wavefront store contiguously, and consecutive rows, whose cells are
+  // consecutive at a given corner, later read the same lines
 #pragma unroll
   for (int i = 0; i < NV; ++i)
-    be[c * NV + i] = out[i];
+    be[std::int64_t(i) * nc + c] = out[i];
 }
 
-/// b[r] += sum of the element vector entries of the row's incidences, in
-/// cell order (inc holds c * k + i, so be is read where the cell pass
-/// stored it).
-template <typename T>
+/// b[BS r + comp] += sum of the element vector entries of the row's
+/// incidences, in cell order. inc holds c * K + i; the cell pass stored
+/// entry (i, comp) of cell c at be[(i BS + comp) nc + c] (corner-major).
+/// Eight incidences per group, loads at clamped positions so that all
+/// sixteen loads of a group are in flight together.
+template <typename T, int K, int BS>
 __global__ void __launch_bounds__(kBlock)
-    row_vector_k(std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
+    row_vector_k(std::int32_t nplan, std::int32_t nc, const std::int32_t* __restrict__ inc_off,
                  const std::int32_t* __restrict__ inc, const T* __restrict__ be,
                  T* __restrict__ b)
 {
-  const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= nplan)
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t >= std::int64_t(nplan) * BS)
     return;
-  // eight incidences per group, loads at clamped positions so that all
-  // sixteen loads of a group are in flight together; summed in cell order
+  const std::int32_t r = static_cast<std::int32_t>(t / BS);
+  const int comp = static_cast<int>(t % BS);
   const std::int32_t p0 = inc_off[r], p1 = inc_off[r + 1];
   T s = 0;
   for (std::int32_t pb = p0; pb < p1; pb += 8)
@@ -1708,13 +1712,13 @@ __global__ void __launch_bounds__(kBlock)
       q[u] = inc[min(pb + u, p1 - 1)];
 #pragma unroll
     for (int u = 0; u < 8; ++u)
-      v[u] = be[q[u]];
+      v[u] = be[std::int64_t((q[u] % K) * BS + comp) * nc + q[u] / K];
 #pragma unroll
     for (int u = 0; u < 8; ++u)
       if (pb + u < p1)
         s += v[u];
   }
-  b[r] += s;
+  b[t] += s;
 }
 
 // ---------------------------------------------------------------------
@@ -2050,7 +2054,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 /// Element vectors of the tetrahedral source forms with constant data,
-/// one thread per cell, stored in incidence order:
+/// one thread per cell, stored corner-major (be[(i BS + comp) nc + c]):
 /// P2 body force b_(i,c) = |det J| w_i f_c; P3 source b_i = |det J| w_i f
 /// (w_i = int phi_i over the reference cell).
 template <typename T, int K, int BS>
@@ -2081,41 +2085,10 @@ __global__ void __launch_bounds__(kBlock)
   {
 #pragma unroll
     for (int b = 0; b < BS; ++b)
-      be[(c * K + i) * BS + b] = adet * w[i] * f[b];
+      be[std::int64_t(i * BS + b) * nc + c] = adet * w[i] * f[b];
   }
 }
 
-/// b[BS r + comp] += sum of the row's incidence entries (incidence order).
-template <typename T, int BS>
-__global__ void __launch_bounds__(kBlock)
-    row_vector_bs_k(std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
-                    const std::int32_t* __restrict__ inc, const T* __restrict__ be,
-                    T* __restrict__ b)
-{
-  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
-  if (t >= std::int64_t(nplan) * BS)
-    return;
-  const std::int32_t r = static_cast<std::int32_t>(t / BS);
-  const int comp = static_cast<int>(t % BS);
-  const std::int32_t p0 = inc_off[r], p1 = inc_off[r + 1];
-  T s = 0;
-  for (std::int32_t pb = p0; pb < p1; pb += 8)
-  {
-    std::int32_t q[8];
-    T v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-      q[u] = inc[min(pb + u, p1 - 1)];
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-      v[u] = be[std::int64_t(q[u]) * BS + comp];
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (pb + u < p1)
-        s += v[u];
-  }
-  b[t] += s;
-}
 
 /// Reference tensors of the affine stiffness + reaction forms, as
 /// [7][k][k]: the symmetrised stiffness integrals S00, S11, S22, S01+S10,
@@ -2617,8 +2590,8 @@ void vector_rows(const dfx_form& f, T* b)
         nc, static_cast<const T*>(f.x), f.x_dofmap, static_cast<const T*>(f.ref_tensors),
         static_cast<T>(f.constants[0]), static_cast<T>(f.constants[1]),
         static_cast<T>(f.constants[2]), be);
-    row_vector_bs_k<T, 3><<<blocks_for(std::int64_t(np) * 3), kBlock, 0, stream()>>>(
-        np, f.inc_off, f.inc, be, b);
+    row_vector_k<T, 10, 3><<<blocks_for(std::int64_t(np) * 3), kBlock, 0, stream()>>>(
+        np, nc, f.inc_off, f.inc, be, b);
     check_launch("assemble_vector");
     return;
   }
@@ -2627,7 +2600,7 @@ void vector_rows(const dfx_form& f, T* b)
     cell_const_source_k<T, 20, 1><<<blocks_for(nc), kBlock, 0, stream()>>>(
         nc, static_cast<const T*>(f.x), f.x_dofmap, static_cast<const T*>(f.ref_tensors),
         static_cast<T>(f.constants[0]), T(0), T(0), be);
-    row_vector_bs_k<T, 1><<<blocks_for(np), kBlock, 0, stream()>>>(np, f.inc_off, f.inc, be, b);
+    row_vector_k<T, 20, 1><<<blocks_for(np), kBlock, 0, stream()>>>(np, nc, f.inc_off, f.inc, be, b);
     check_launch("assemble_vector");
     return;
   }
@@ -2640,7 +2613,12 @@ void vector_rows(const dfx_form& f, T* b)
         nc, static_cast<const T*>(f.x), f.x_dofmap, static_cast<const T*>(f.coeffs),
         f.d.cstride, f.affine ? 1 : 0, be);
   const std::int32_t nplan = f.max_dof + 1;
-  row_vector_k<T><<<blocks_for(nplan), kBlock, 0, stream()>>>(nplan, f.inc_off, f.inc, be, b);
+  if (nv == 4)
+    row_vector_k<T, 4, 1><<<blocks_for(nplan), kBlock, 0, stream()>>>(nplan, nc, f.inc_off, f.inc,
+                                                                       be, b);
+  else
+    row_vector_k<T, 8, 1><<<blocks_for(nplan), kBlock, 0, stream()>>>(nplan, nc, f.inc_off, f.inc,
+                                                                       be, b);
   check_launch("assemble_vector");
 }
 
@@ -3240,6 +3218,31 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t*
           (void)hipEventDestroy(e1);
         }
         f.row_variant = best;
+        // release the plans of the variants not kept (the element matrices
+        // of variant 7 alone are k^2 values per cell)
+        if (best != 7)
+        {
+          dfree(f.cell_elem);
+          f.cell_elem = nullptr;
+        }
+        if (best != 7 && best != 8)
+        {
+          dfree(f.gblk);
+          dfree(f.inc_row);
+          f.gblk = nullptr;
+          f.inc_row = nullptr;
+          f.ngblk = 0;
+          f.gather_pattern = 0;
+        }
+        if (best != 9)
+        {
+          dfree(f.corner_cell);
+          dfree(f.slotmap);
+          f.corner_cell = nullptr;
+          f.slotmap = nullptr;
+          f.struct_pattern = 0;
+          f.struct_ok = -1;
+        }
         // leave A zeroed (stored when next read)
         A->zeroed = true;
         A->zero_pending = true;
