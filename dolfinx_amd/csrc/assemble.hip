@@ -688,7 +688,11 @@ __device__ inline void affine_factors(const T* q0, const T* q1, const T* q2, con
   G[6] = creac * adet;
 }
 
-template <typename T, int NV>
+/// Pass 1: the factor record of every cell (one thread per cell). With SAME
+/// the dofmap is the vertex map (P1/Q1 on the mesh's numbering): the
+/// vertex row, loaded with vector loads, also addresses the BC markers, so
+/// the marker and coordinate gathers are in flight together.
+template <typename T, int NV, bool SAME>
 __global__ void __launch_bounds__(kBlock)
     cell_factors_k(std::int32_t nc, const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
                    const std::int32_t* __restrict__ dm, int k, const T* __restrict__ coeffs,
@@ -699,19 +703,40 @@ __global__ void __launch_bounds__(kBlock)
       = xcd_contiguous(blockIdx.x, gridDim.x) * std::int64_t(kBlock) + threadIdx.x;
   if (c >= nc)
     return;
-  const std::int32_t* v = xdm + NV * c;
+  std::int32_t v[NV];
+  {
+    const int4 a = *reinterpret_cast<const int4*>(xdm + NV * c);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    if constexpr (NV == 8)
+    {
+      const int4 b = *reinterpret_cast<const int4*>(xdm + NV * c + 4);
+      v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    }
+  }
+  std::uint32_t m = 0;
+  if constexpr (SAME)
+  {
+    if (mask)
+    {
+#pragma unroll
+      for (int j = 0; j < NV; ++j)
+        if (bc[v[j]])
+          m |= 1u << j;
+    }
+  }
   T g[7];
   affine_factors<T>(x + 3 * std::int64_t(v[0]), x + 3 * std::int64_t(v[1]),
                     x + 3 * std::int64_t(v[2]), x + 3 * std::int64_t(v[NV == 4 ? 3 : 4]),
                     coeffs[c * cstride], creac, g);
-  std::uint32_t m = 0;
-  if (mask)
+  if constexpr (!SAME)
   {
-    for (int j = 0; j < k; ++j)
-      if (bc[dm[c * k + j]])
-        m |= 1u << j;
-    mask[c] = m;
+    if (mask)
+      for (int j = 0; j < k; ++j)
+        if (bc[dm[c * k + j]])
+          m |= 1u << j;
   }
+  if (mask)
+    mask[c] = m;
   // the eighth slot carries the BC column mask bits (moved, never computed
   // with), so a row kernel reads factors and mask with one load; the record
   // is stored with full-width vector stores
@@ -1354,7 +1379,7 @@ __global__ void __launch_bounds__(kBlock)
                  const O* __restrict__ islots, const T* __restrict__ cd,
                  const T* __restrict__ elem, const std::uint32_t* __restrict__ mask,
                  const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
-                 const T* __restrict__ ref, T* __restrict__ vals, int accumulate, int max_rank,
+                 const T* __restrict__ ref, T* __restrict__ vals, int accumulate, int ngblk,
                  std::int32_t nplan)
 {
   extern __shared__ unsigned char smem_raw[];
@@ -1408,7 +1433,8 @@ __global__ void __launch_bounds__(kBlock)
     }
   }
   T* a = acc + (live ? rp[r] - base : 0);
-  for (int k = 0; k < max_rank; ++k)
+  const int rounds = gblk[ngblk + 1 + b];
+  for (int k = 0; k < rounds; ++k)
   {
     if (live && rank == k)
     {
@@ -2206,16 +2232,21 @@ void launch_cell_factors(const dfx_form& f, const dfx_bc* bc)
   const std::int32_t nc = f.d.num_cells;
   const T creac = static_cast<T>(f.constants[0]);
   std::uint32_t* mask = bc ? f.cell_mask : nullptr;
-  if (nv == 4)
-    cell_factors_k<T, 4><<<blocks_for(nc), kBlock, 0, stream()>>>(
+  const bool same = f.dofmap == f.x_dofmap && f.d.dofs_per_cell == nv;
+  auto go = [&](auto kern)
+  {
+    kern<<<blocks_for(nc), kBlock, 0, stream()>>>(
         nc, static_cast<const T*>(f.x), f.x_dofmap, f.dofmap, f.d.dofs_per_cell,
         static_cast<const T*>(f.coeffs), f.d.cstride, creac, bc ? bc->marker : nullptr,
         static_cast<T*>(f.cell_data), mask);
-  else
-    cell_factors_k<T, 8><<<blocks_for(nc), kBlock, 0, stream()>>>(
-        nc, static_cast<const T*>(f.x), f.x_dofmap, f.dofmap, f.d.dofs_per_cell,
-        static_cast<const T*>(f.coeffs), f.d.cstride, creac, bc ? bc->marker : nullptr,
-        static_cast<T*>(f.cell_data), mask);
+  };
+  if (nc > 0)
+  {
+    if (nv == 4)
+      same ? go(cell_factors_k<T, 4, true>) : go(cell_factors_k<T, 4, false>);
+    else
+      same ? go(cell_factors_k<T, 8, true>) : go(cell_factors_k<T, 8, false>);
+  }
   check_launch("cell_factors");
 }
 
@@ -2337,13 +2368,13 @@ void ensure_gather(dfx_form& f, const dfx_csr& A, std::size_t tsize)
       irow[q] = r;
   const std::int64_t cap_inc = kBlock;
   const std::int64_t cap_slice = (32 * 1024) / static_cast<std::int64_t>(tsize);
-  std::vector<std::int32_t> gb{0};
+  std::vector<std::int32_t> gb{0}, ranks;
   std::int64_t slice = 0;
   std::int32_t r = 0;
   while (r < A.nrows)
   {
     std::int32_t e = r;
-    std::int64_t ninc = 0, nent = 0;
+    std::int64_t ninc = 0, nent = 0, most = 0;
     while (e < A.nrows)
     {
       const std::int64_t ci = e < nplan ? off[e + 1] - off[e] : 0;
@@ -2355,13 +2386,18 @@ void ensure_gather(dfx_form& f, const dfx_csr& A, std::size_t tsize)
       }
       ninc += ci;
       nent += ce;
+      most = std::max(most, ci);
       ++e;
     }
     gb.push_back(e);
+    ranks.push_back(static_cast<std::int32_t>(most));
     slice = std::max(slice, nent);
     r = e;
   }
   f.ngblk = static_cast<std::int32_t>(gb.size()) - 1;
+  // after the ngblk + 1 row boundaries: each block's most incidences of a
+  // row (its number of rounds)
+  gb.insert(gb.end(), ranks.begin(), ranks.end());
   f.gblk_slice = static_cast<std::int32_t>(slice);
   f.gblk = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * gb.size()));
   h2d(f.gblk, gb.data(), sizeof(std::int32_t) * gb.size());
@@ -2399,7 +2435,7 @@ void launch_gather(dfx_form& f, dfx_csr& A, const dfx_bc* bc, bool mat)
         static_cast<const T*>(f.cell_data), static_cast<const T*>(f.cell_elem),
         bc ? f.cell_mask : nullptr, bc ? bc->marker : nullptr, A.row_ptr,
         static_cast<const T*>(f.ref_tensors), static_cast<T*>(A.values), A.zeroed ? 0 : 1,
-        f.max_inc, f.max_dof + 1);
+        f.ngblk, f.max_dof + 1);
   };
   if (mat)
     go(row_gather_k<T, O, K, true>);
