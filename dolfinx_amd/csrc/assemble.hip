@@ -22,6 +22,24 @@ namespace
 {
 
 constexpr int kBlock = 256;
+// LDS slice of one elasticity row group (6 rows at most)
+constexpr std::size_t kTeamSlice = 16 * 1024;
+
+/// Two values of T as one aligned vector access (double2 or float2).
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<double>
+{
+  using type = double2;
+};
+template <>
+struct Vec2<float>
+{
+  using type = float2;
+};
+template <typename T>
+using vec2_t = typename Vec2<T>::type;
 
 struct KernelInfo
 {
@@ -1915,25 +1933,28 @@ std::vector<double> scalar_tensors(const TetBasis& B)
   return t;
 }
 
-/// [3][3][k][k] tensors S_ab(i, j) = int d_a phi_i d_b phi_j (elasticity).
+/// Tensors S_ab(i, j) = int d_a phi_i d_b phi_j (elasticity), stored per
+/// (i, j) as the 3 x 3 matrix over (a, b) padded to 10 values:
+/// t[(i k + j) 10 + 3 a + b], t[(i k + j) 10 + 9] = 0.
 std::vector<double> gradient_tensors(const TetBasis& B)
 {
   const int k = B.n;
-  std::vector<double> t(9 * k * k);
-  for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b)
-      for (int i = 0; i < k; ++i)
-        for (int j = 0; j < k; ++j)
-          t[((a * 3 + b) * k + i) * k + j] = basis_product_integral(B, i, a, j, b);
+  std::vector<double> t(10 * k * k, 0.0);
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j)
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          t[(i * k + j) * 10 + 3 * a + b] = basis_product_integral(B, i, a, j, b);
   return t;
 }
 
 // ---------------------------------------------------------------------
-// P2 vector elasticity, a = sigma(u):eps(v) with DG0 lambda, mu. Per cell:
-// J^{-1}, |det J|, lambda, mu (16 values). Entry block (i, j) (3 x 3, row
-// component c, column component d) with P = J^{-T}-transformed reference
-// gradient products, P_cd = sum_ab Kinv[a][c] S_ab(i,j) Kinv[b][d]:
-//   A_(i,c)(j,d) = |det J| (mu (delta_cd tr P + P_dc) + lambda P_cd).
+// P2 vector elasticity, a = sigma(u):eps(v) with DG0 lambda, mu. Per cell
+// a record of 16 values: J^{-1} (9), 0, |det J| lambda, |det J| mu, 0...
+// Entry block (i, j) (3 x 3, row component c, column component d) with
+// P = J^{-T}-transformed reference gradient products,
+// P_cd = sum_ab Kinv[a][c] S_ab(i,j) Kinv[b][d]:
+//   A_(i,c)(j,d) = |det J| mu (delta_cd tr P + P_dc) + |det J| lambda P_cd.
 // ---------------------------------------------------------------------
 
 template <typename T>
@@ -1943,15 +1964,16 @@ __global__ void __launch_bounds__(kBlock)
                     const std::int8_t* __restrict__ bc, T* __restrict__ cd,
                     std::uint32_t* __restrict__ mask)
 {
+  using T2 = vec2_t<T>;
   const std::int64_t c
       = xcd_contiguous(blockIdx.x, gridDim.x) * std::int64_t(kBlock) + threadIdx.x;
   if (c >= nc)
     return;
-  const std::int32_t* v = xdm + 4 * c;
-  const T* q0 = x + 3 * std::int64_t(v[0]);
-  const T* q1 = x + 3 * std::int64_t(v[1]);
-  const T* q2 = x + 3 * std::int64_t(v[2]);
-  const T* q3 = x + 3 * std::int64_t(v[3]);
+  const int4 v = *reinterpret_cast<const int4*>(xdm + 4 * c);
+  const T* q0 = x + 3 * std::int64_t(v.x);
+  const T* q1 = x + 3 * std::int64_t(v.y);
+  const T* q2 = x + 3 * std::int64_t(v.z);
+  const T* q3 = x + 3 * std::int64_t(v.w);
   T J[9], adj[9];
 #pragma unroll
   for (int r = 0; r < 3; ++r)
@@ -1961,13 +1983,21 @@ __global__ void __launch_bounds__(kBlock)
     J[3 * r + 2] = q3[r] - q0[r];
   }
   const T det = adjugate(J, adj);
-  T* o = cd + 16 * c;
+  const T adet = det < T(0) ? -det : det;
+  T o[16];
 #pragma unroll
   for (int q = 0; q < 9; ++q)
     o[q] = adj[q] / det;
-  o[9] = det < T(0) ? -det : det;
-  o[10] = coeffs[c * cstride];     // lambda
-  o[11] = coeffs[c * cstride + 1]; // mu
+  o[9] = T(0);
+  o[10] = adet * coeffs[c * cstride];     // |det J| lambda
+  o[11] = adet * coeffs[c * cstride + 1]; // |det J| mu
+#pragma unroll
+  for (int q = 12; q < 16; ++q)
+    o[q] = T(0);
+  T2* out = reinterpret_cast<T2*>(cd + 16 * c);
+#pragma unroll
+  for (int h = 0; h < 8; ++h)
+    out[h] = T2{o[2 * h], o[2 * h + 1]};
   if (mask)
   {
     std::uint32_t m = 0;
@@ -1982,101 +2012,245 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-/// L lanes per block row (one incidence per lane); 3 x 3 blocks added into
-/// the row's LDS slice with LDS atomics.
-template <typename T, typename O, int L>
-__global__ void __launch_bounds__(kBlock)
-    elast_rows_k(std::int32_t nrows, std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
-                 const std::int32_t* __restrict__ inc, const O* __restrict__ islots,
-                 const T* __restrict__ cd, const std::uint32_t* __restrict__ mask,
-                 const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
-                 const T* __restrict__ ref, T* __restrict__ vals, int accumulate)
+/// Row teams for the elasticity rows: a wavefront holds 6 teams of 10
+/// lanes; team t takes one block row, lane j the column node j of every
+/// incidence of that row. The team walks the row's incidences in cell
+/// order, lane j adding the 3 x 3 block (i, j) into the row's LDS slice at
+/// the block of column slot j (the 10 slots of one incidence are distinct,
+/// so the additions need no atomics and every entry is summed in cell
+/// order). Rows come in groups of up to 6 with similar incidence counts
+/// (egrp: per group its first position in erow and its LDS slice length,
+/// then a terminating entry; erow: per row (row, slice offset, first
+/// incidence, incidences)); blocks are stored in LDS padded to 10 values.
+template <typename T, typename O>
+__global__ void __launch_bounds__(64)
+    elast_team_k(const int2* __restrict__ egrp, const int4* __restrict__ erow, const std::int32_t* __restrict__ inc,
+                 const O* __restrict__ islots, const T* __restrict__ cd,
+                 const std::uint32_t* __restrict__ mask, const std::int8_t* __restrict__ bc,
+                 const std::int32_t* __restrict__ rp, const T* __restrict__ ref,
+                 T* __restrict__ vals, int accumulate)
 {
+  using T2 = vec2_t<T>;
   constexpr int K = 10;
   extern __shared__ unsigned char smem_raw[];
-  T* S = reinterpret_cast<T*>(smem_raw); // [9][K][K]
-  T* acc = S + 9 * K * K;
-  constexpr int RPB = kBlock / L;
-  const std::int32_t r0 = xcd_contiguous(blockIdx.x, gridDim.x) * RPB;
-  const std::int32_t r1 = min(r0 + RPB, nrows);
-  const std::int32_t base = rp[r0];
-  const std::int32_t len = (rp[r1] - base) * 9;
-  for (int t = threadIdx.x; t < 9 * K * K; t += kBlock)
-    S[t] = ref[t];
-  for (int t = threadIdx.x; t < len; t += kBlock)
-    acc[t] = T(0);
-  __syncthreads();
-  const std::int32_t r = r0 + threadIdx.x / L;
-  const int lane = threadIdx.x % L;
-  if (r < r1 && r < nplan)
+  T* acc = reinterpret_cast<T*>(smem_raw);
+  const int lane = threadIdx.x, team = lane / K, j = lane - K * team;
+  // A group's description as seen by this lane's team: the group's rows
+  // [q0, q1) and slice length, the team's (row, slice offset, first
+  // incidence, incidences), the row's CSR position and length, its
+  // Dirichlet components and its first 30 incidences (3 per lane).
+  struct Meta
   {
-    std::uint32_t rowbc = 0;
-    if (bc)
-      for (int cc = 0; cc < 3; ++cc)
-        if (bc[3 * std::int64_t(r) + cc])
-          rowbc |= 1u << cc;
-    T* a = acc + (rp[r] - base) * 9;
-    const std::int32_t pe = inc_off[r + 1];
-    for (std::int32_t p = inc_off[r] + lane; p < pe && rowbc != 7u; p += L)
+    std::int32_t q0, q1, total;
+    bool has;
+    int4 ro;
+    std::int32_t rbase, rlen;
+    std::uint32_t rowbc;
+    std::int32_t pre0, pre1, pre2;
+  };
+  // first stage: the group's bounds and the team's row entry
+  auto meta_rows = [&](Meta& m, int g)
+  {
+    const int2 gb = egrp[g];
+    m.q0 = gb.x;
+    m.total = gb.y;
+    m.q1 = egrp[g + 1].x;
+    m.has = lane < 6 * K && m.q0 + team < m.q1;
+    m.ro = m.has ? erow[m.q0 + team] : make_int4(0, 0, 0, 0);
+  };
+  // second stage: the loads that need the row entry
+  auto meta_data = [&](Meta& m)
+  {
+    m.rbase = m.rlen = 0;
+    m.rowbc = 0;
+    m.pre0 = m.pre1 = m.pre2 = 0;
+    if (m.has)
     {
-      const std::int32_t cik = inc[p];
-      const std::int64_t c = cik / K;
-      const int i = cik - static_cast<int>(c) * K;
-      const T* o = cd + 16 * c;
-      T Ki[9];
-#pragma unroll
-      for (int q = 0; q < 9; ++q)
-        Ki[q] = o[q];
-      const T adet = o[9], lam = o[10], mu = o[11];
-      const std::uint32_t m = mask ? mask[c] : 0u;
-      O sl[K];
-      load_slots<O, K>(islots, p, sl);
-      for (int j = 0; j < K; ++j)
+      const std::int32_t r = m.ro.x, p0 = m.ro.z, n = m.ro.w;
+      m.rbase = rp[r];
+      m.rlen = rp[r + 1] - m.rbase;
+      if (bc)
+        for (int cc = 0; cc < 3; ++cc)
+          if (bc[3 * std::int64_t(r) + cc])
+            m.rowbc |= 1u << cc;
+      if (n > 0)
       {
-        T Tm[9];
-#pragma unroll
-        for (int q = 0; q < 9; ++q)
-          Tm[q] = S[(q * K + i) * K + j];
-        // W = T Kinv, P = Kinv^T W
-        T W[9], P[9];
-#pragma unroll
-        for (int aa = 0; aa < 3; ++aa)
-#pragma unroll
-          for (int d = 0; d < 3; ++d)
-            W[aa * 3 + d] = Tm[aa * 3 + 0] * Ki[0 * 3 + d] + Tm[aa * 3 + 1] * Ki[1 * 3 + d]
-                            + Tm[aa * 3 + 2] * Ki[2 * 3 + d];
-#pragma unroll
-        for (int cc = 0; cc < 3; ++cc)
-#pragma unroll
-          for (int d = 0; d < 3; ++d)
-            P[cc * 3 + d] = Ki[0 * 3 + cc] * W[0 * 3 + d] + Ki[1 * 3 + cc] * W[1 * 3 + d]
-                            + Ki[2 * 3 + cc] * W[2 * 3 + d];
-        const T tr = P[0] + P[4] + P[8];
-        T* blk = a + 9 * std::int32_t(sl[j]);
-#pragma unroll
-        for (int cc = 0; cc < 3; ++cc)
-        {
-          if ((rowbc >> cc) & 1u)
-            continue;
-#pragma unroll
-          for (int d = 0; d < 3; ++d)
-          {
-            if ((m >> (3 * j + d)) & 1u)
-              continue;
-            const T v = adet * (mu * ((cc == d ? tr : T(0)) + P[d * 3 + cc]) + lam * P[cc * 3 + d]);
-            atomicAdd(blk + cc * 3 + d, v);
-          }
-        }
+        m.pre0 = inc[p0 + min(j, n - 1)];
+        m.pre1 = inc[p0 + min(K + j, n - 1)];
+        m.pre2 = inc[p0 + min(2 * K + j, n - 1)];
       }
     }
-  }
+  };
+  // the team adds its row's incidences into the row's slice
+  auto rows = [&](const Meta m)
+  {
+    const std::uint32_t rowbc = m.rowbc;
+    const std::int32_t p0 = m.ro.z, n = m.ro.w;
+    if (!m.has || rowbc == 7u || n == 0)
+      return;
+    T* a = acc + m.ro.y;
+    // the first 30 incidences are handed out by shuffles within the team
+    // (the team's lanes run the same steps)
+    const std::int32_t pre0 = m.pre0, pre1 = m.pre1, pre2 = m.pre2;
+    auto cik_at = [pre0, pre1, pre2, p0, inc, team](int st) -> std::int32_t
+    {
+      if (st < 3 * K)
+      {
+        const int h = st >= 2 * K ? 2 : (st >= K ? 1 : 0);
+        // (masks rather than a select chain, which the compiler turns into
+        // a table in scratch memory)
+        const std::int32_t m1 = -static_cast<std::int32_t>(h == 1);
+        const std::int32_t m2 = -static_cast<std::int32_t>(h == 2);
+        const std::int32_t v = (pre0 & ~(m1 | m2)) | (pre1 & m1) | (pre2 & m2);
+        return __shfl(v, team * K + (st - K * h), 64);
+      }
+      return inc[p0 + st];
+    };
+    // each step's cell record, mask, slot and reference block are loaded
+    // two steps ahead
+    struct Stage
+    {
+      T2 rec[6];
+      T2 tm[5];
+      std::uint32_t m;
+      std::int32_t sl;
+    };
+    // 32-bit byte offsets from the (uniform) array bases; the launcher
+    // checks that every offset fits
+    const char* cdb = reinterpret_cast<const char*>(cd);
+    const char* refb = reinterpret_cast<const char*>(ref);
+    const char* maskb = reinterpret_cast<const char*>(mask);
+    const char* slb = reinterpret_cast<const char*>(islots);
+    auto fetch = [&](Stage& sg, int st)
+    {
+      const std::int32_t cik = cik_at(st);
+      const std::int32_t c = cik / K;
+      const int i = cik - c * K;
+      const T2* src = reinterpret_cast<const T2*>(
+          cdb + static_cast<std::uint32_t>(c) * std::uint32_t(16 * sizeof(T)));
+#pragma unroll
+      for (int h = 0; h < 6; ++h)
+        sg.rec[h] = src[h];
+      const T2* s2 = reinterpret_cast<const T2*>(
+          refb + static_cast<std::uint32_t>(i * K + j) * std::uint32_t(10 * sizeof(T)));
+#pragma unroll
+      for (int h = 0; h < 5; ++h)
+        sg.tm[h] = s2[h];
+      sg.m = mask ? *reinterpret_cast<const std::uint32_t*>(
+                 maskb + static_cast<std::uint32_t>(c) * std::uint32_t(sizeof(std::uint32_t)))
+                  : 0u;
+      sg.sl = *reinterpret_cast<const O*>(
+          slb + static_cast<std::uint32_t>((p0 + st) * K + j) * std::uint32_t(sizeof(O)));
+    };
+    auto add = [&](const Stage& sg)
+    {
+      T Ki[9], Tm[10];
+#pragma unroll
+      for (int h = 0; h < 4; ++h)
+      {
+        Ki[2 * h] = sg.rec[h].x;
+        Ki[2 * h + 1] = sg.rec[h].y;
+      }
+      Ki[8] = sg.rec[4].x;
+      const T lam = sg.rec[5].x, mu = sg.rec[5].y;
+#pragma unroll
+      for (int h = 0; h < 5; ++h)
+      {
+        Tm[2 * h] = sg.tm[h].x;
+        Tm[2 * h + 1] = sg.tm[h].y;
+      }
+      // W = S Kinv, P = Kinv^T W
+      T W[9], P[9];
+#pragma unroll
+      for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          W[aa * 3 + d] = Tm[aa * 3 + 0] * Ki[0 * 3 + d] + Tm[aa * 3 + 1] * Ki[1 * 3 + d]
+                          + Tm[aa * 3 + 2] * Ki[2 * 3 + d];
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          P[cc * 3 + d] = Ki[0 * 3 + cc] * W[0 * 3 + d] + Ki[1 * 3 + cc] * W[1 * 3 + d]
+                          + Ki[2 * 3 + cc] * W[2 * 3 + d];
+      const T tr = P[0] + P[4] + P[8];
+      T out[9];
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          out[cc * 3 + d] = cc == d ? mu * (tr + P[cc * 4]) + lam * P[cc * 4]
+                                    : mu * P[d * 3 + cc] + lam * P[cc * 3 + d];
+      // Dirichlet components of the row and of column node j add nothing
+      // (rare: branched around)
+      const std::uint32_t colb = (sg.m >> (3 * j)) & 7u;
+      if (rowbc | colb)
+      {
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            if (((rowbc >> cc) | (colb >> d)) & 1u)
+              out[cc * 3 + d] = T(0);
+      }
+      T2* blk = reinterpret_cast<T2*>(a + K * sg.sl);
+#pragma unroll
+      for (int h = 0; h < 4; ++h)
+      {
+        T2 w = blk[h];
+        w.x += out[2 * h];
+        w.y += out[2 * h + 1];
+        blk[h] = w;
+      }
+      a[K * sg.sl + 8] += out[8];
+    };
+    Stage s0, s1, s2;
+    fetch(s0, 0);
+    fetch(s1, min(1, n - 1));
+    for (int st = 0; st < n; st += 3)
+    {
+      fetch(s2, min(st + 2, n - 1));
+      add(s0);
+      if (st + 1 >= n)
+        break;
+      fetch(s0, min(st + 3, n - 1));
+      add(s1);
+      if (st + 2 >= n)
+        break;
+      fetch(s1, min(st + 4, n - 1));
+      add(s2);
+    }
+  };
+  // the group's rows are stored one at a time by the whole wavefront (row
+  // t's position, length and slice offset come from lane K t)
+  auto store = [&](const Meta m)
+  {
+    for (std::int32_t t = 0; t < m.q1 - m.q0; ++t)
+    {
+      const std::int32_t base = __builtin_amdgcn_readlane(m.rbase, K * t);
+      const std::int32_t n9 = 9 * __builtin_amdgcn_readlane(m.rlen, K * t);
+      const T* src = acc + __builtin_amdgcn_readlane(m.ro.y, K * t);
+      T* dst = vals + std::int64_t(base) * 9;
+      for (int e = lane; e < n9; e += 64)
+      {
+        const int blk = e / 9;
+        const T v = src[blk * K + (e - 9 * blk)];
+        if (accumulate)
+          dst[e] += v;
+        else
+          dst[e] = v;
+      }
+    }
+  };
+  Meta cur;
+  meta_rows(cur, xcd_contiguous(blockIdx.x, gridDim.x));
+  for (int t = lane; t < cur.total; t += 64)
+    acc[t] = T(0);
+  meta_data(cur);
   __syncthreads();
-  if (accumulate)
-    for (int t = threadIdx.x; t < len; t += kBlock)
-      vals[std::int64_t(base) * 9 + t] += acc[t];
-  else
-    for (int t = threadIdx.x; t < len; t += kBlock)
-      vals[std::int64_t(base) * 9 + t] = acc[t];
+  rows(cur);
+  __syncthreads();
+  store(cur);
 }
 
 /// Element vectors of the tetrahedral source forms with constant data,
@@ -2279,22 +2453,99 @@ void launch_lanes(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
         A.zeroed ? 0 : 1);
 }
 
+/// Row groups of the elasticity team kernel. Rows are ordered by their
+/// incidence count (longest first) inside chunks of 4096 consecutive rows,
+/// so the 6 rows of a wavefront take similar numbers of steps while a
+/// chunk's cells stay together in the L2; a group is cut at 6 rows or when
+/// its padded LDS slice would exceed kTeamSlice. Groups are stored as
+/// (first position, slice length) pairs ended by (number of rows, 0), rows
+/// as (row, slice offset, first incidence, incidences).
+void ensure_teams(dfx_form& f, const dfx_csr& A, std::size_t tsize)
+{
+  const std::int64_t cap = static_cast<std::int64_t>(kTeamSlice / tsize);
+  if (f.team_grp && f.team_pattern == A.pattern_id && f.team_cap == cap)
+    return;
+  dfree(f.team_grp);
+  f.team_grp = nullptr;
+  f.nteam = 0;
+  if (A.nrows == 0)
+    return;
+  require(10 * std::int64_t(A.max_row) <= cap, "row too long for the LDS slice");
+  const std::int32_t nplan = f.max_dof + 1;
+  std::vector<std::int32_t> off(nplan + 1), rp(A.nrows + 1);
+  d2h(off.data(), f.inc_off, sizeof(std::int32_t) * off.size());
+  d2h(rp.data(), A.row_ptr, sizeof(std::int32_t) * rp.size());
+  auto count = [&](std::int32_t r) { return r < nplan ? off[r + 1] - off[r] : 0; };
+  std::vector<std::int32_t> order(A.nrows);
+  for (std::int32_t r = 0; r < A.nrows; ++r)
+    order[r] = r;
+  std::sort(order.begin(), order.end(),
+            [&](std::int32_t x, std::int32_t y)
+            {
+              if ((x >> 12) != (y >> 12))
+                return x < y;
+              const std::int32_t cx = count(x), cy = count(y);
+              return cx != cy ? cx > cy : x < y;
+            });
+  std::vector<std::int32_t> grp{0, 0}, rows;
+  rows.reserve(4 * std::size_t(A.nrows));
+  std::int64_t slice = 0;
+  int members = 0;
+  for (std::int32_t q = 0; q < A.nrows; ++q)
+  {
+    const std::int32_t r = order[q];
+    const std::int64_t len = 10 * std::int64_t(rp[r + 1] - rp[r]);
+    if (members == 6 || slice + len > cap)
+    {
+      grp.back() = static_cast<std::int32_t>(slice);
+      grp.push_back(q);
+      grp.push_back(0);
+      slice = 0;
+      members = 0;
+    }
+    rows.push_back(r);
+    rows.push_back(static_cast<std::int32_t>(slice));
+    rows.push_back(r < nplan ? off[r] : 0);
+    rows.push_back(count(r));
+    slice += len;
+    ++members;
+  }
+  grp.back() = static_cast<std::int32_t>(slice);
+  grp.push_back(A.nrows);
+  grp.push_back(0);
+  f.nteam = static_cast<std::int32_t>(grp.size() / 2) - 1;
+  // the row quads start at a multiple of 4 (16-byte aligned)
+  const std::size_t at = (grp.size() + 3) & ~std::size_t(3);
+  std::vector<std::int32_t> buf(at + rows.size(), 0);
+  std::copy(grp.begin(), grp.end(), buf.begin());
+  std::copy(rows.begin(), rows.end(), buf.begin() + at);
+  f.team_grp = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * buf.size()));
+  h2d(f.team_grp, buf.data(), sizeof(std::int32_t) * buf.size());
+  f.team_rows_at = static_cast<std::int32_t>(at);
+  f.team_pattern = A.pattern_id;
+  f.team_cap = cap;
+}
+
 template <typename T, typename O>
 void launch_elast(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
 {
   const std::int32_t nc = f.d.num_cells;
-  elast_factors_k<T><<<blocks_for(nc), kBlock, 0, stream()>>>(
-      nc, static_cast<const T*>(f.x), f.x_dofmap, f.dofmap, static_cast<const T*>(f.coeffs),
-      f.d.cstride, bc ? bc->marker : nullptr, static_cast<T*>(f.cell_data),
-      bc ? f.cell_mask : nullptr);
-  constexpr int L = 32;
-  constexpr int RPB = kBlock / L;
-  const std::size_t lds = (9 * 100 + std::size_t(RPB) * A.max_row * 9) * sizeof(T);
-  require(lds <= 64 * 1024, "row too long for the LDS slice");
-  const int nb = (A.nrows + RPB - 1) / RPB;
-  if (nb > 0)
-    elast_rows_k<T, O, L><<<nb, kBlock, lds, stream()>>>(
-        A.nrows, f.max_dof + 1, f.inc_off, f.inc, static_cast<const O*>(f.islots),
+  if (nc > 0)
+    elast_factors_k<T><<<blocks_for(nc), kBlock, 0, stream()>>>(
+        nc, static_cast<const T*>(f.x), f.x_dofmap, f.dofmap, static_cast<const T*>(f.coeffs),
+        f.d.cstride, bc ? bc->marker : nullptr, static_cast<T*>(f.cell_data),
+        bc ? f.cell_mask : nullptr);
+  // the team kernel addresses the cell records and slot offsets with 32-bit
+  // byte offsets
+  require(std::int64_t(nc) * 16 * std::int64_t(sizeof(T)) <= 0xffffffffLL
+              && std::int64_t(nc) * 100 * std::int64_t(sizeof(O)) <= 0xffffffffLL,
+          "mesh too large for the elasticity row kernel (cell records above 4 GiB)");
+  ensure_teams(f, A, sizeof(T));
+  if (f.nteam > 0)
+    elast_team_k<T, O><<<f.nteam, 64, kTeamSlice, stream()>>>(
+        reinterpret_cast<const int2*>(f.team_grp),
+        reinterpret_cast<const int4*>(f.team_grp + f.team_rows_at), f.inc,
+        static_cast<const O*>(f.islots),
         static_cast<const T*>(f.cell_data), bc ? f.cell_mask : nullptr, bc ? bc->marker : nullptr,
         A.row_ptr, static_cast<const T*>(f.ref_tensors), static_cast<T*>(A.values),
         A.zeroed ? 0 : 1);
@@ -2988,6 +3239,7 @@ dfx_status dfx_form_destroy(dfx_form* f)
     dfree(f->cell_vec);
     dfree(f->inc_row);
     dfree(f->gblk);
+    dfree(f->team_grp);
     dfree(f->cell_elem);
     dfree(f->corner_cell);
     dfree(f->slotmap);
@@ -3213,7 +3465,8 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t*
         dfx_form& f = *a;
         int best = f.row_variant;
         float bms = -1;
-        if (row_path(f))
+        // the elasticity form has one row kernel
+        if (row_path(f) && f.d.kernel != DFX_K_P2TET_ELAST_A)
         {
           hipEvent_t e0, e1;
           DFX_HIP(hipEventCreate(&e0));

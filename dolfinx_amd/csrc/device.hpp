@@ -239,6 +239,15 @@ struct dfx_form
   int pack_layout = -1;
   double* pack_stage = nullptr;
   std::int64_t pack_stage_n = 0;
+  // Elasticity row groups: nteam + 1 (first row position, slice length)
+  // pairs, then from team_rows_at the (row, LDS slice offset, first
+  // incidence, incidences) of all rows in group order; built for
+  // team_pattern with team_cap values of slice per group.
+  std::int32_t* team_grp = nullptr;
+  std::int32_t nteam = 0;
+  std::int32_t team_rows_at = 0;
+  std::int64_t team_cap = 0;
+  std::uint64_t team_pattern = 0;
   void* cell_elem = nullptr;
   int row_variant = 1;             // row-kernel variant (dfx_form_autotune)
 };

@@ -245,7 +245,8 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc,
  * structured box of hexahedra: one thread per row accumulating its 27
  * entries in registers (error if the mesh is not such a box); 10 as 4 with
  * the cell factors formed from the geometry by every incidence (no cell
- * pass; affine P1/Q1). */
+ * pass; affine P1/Q1). The P2 elasticity form has a single row kernel and
+ * ignores the variant. */
 dfx_status dfx_form_set_variant(dfx_form* a, int32_t variant);
 /* fem::set_diagonal (fem/assembler.h:644-686): A[d,d] = diag for BC dofs */
 dfx_status dfx_set_diagonal(dfx_csr* A, const dfx_bc* bc, double diag);
