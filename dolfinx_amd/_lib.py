@@ -18,7 +18,7 @@ DFX_OK, DFX_ERR_ARG, DFX_ERR_PATTERN, DFX_ERR_HIP, DFX_ERR_NOCONV, DFX_ERR_NODEV
 F32, F64 = 0, 1
 TET, HEX = 0, 1
 (K_P1TET_A, K_P1TET_L, K_Q1HEX_A, K_Q1HEX_L, K_P2TET_ELAST_A, K_P2TET_ELAST_L,
- K_P3TET_A, K_P3TET_L) = range(8)
+ K_P3TET_A, K_P3TET_L, K_ELEMENT_A, K_ELEMENT_L) = range(10)
 
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
@@ -102,6 +102,8 @@ SIGNATURES = {
     "dfx_bc_create": (C.c_int, [P, I32, I32, P, PP]),
     "dfx_bc_destroy": (C.c_int, [P]),
     "dfx_assemble_matrix": (C.c_int, [P, P, P]),
+    "dfx_assemble_matrix_elements": (C.c_int, [P, P, P, P]),
+    "dfx_assemble_vector_elements": (C.c_int, [P, P, P]),
     "dfx_set_diagonal": (C.c_int, [P, P, D]),
     "dfx_form_autotune": (C.c_int, [P, P, P, P, P]),
     "dfx_form_set_variant": (C.c_int, [P, I32]),

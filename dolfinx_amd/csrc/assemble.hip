@@ -72,6 +72,12 @@ bool kernel_info(int id, KernelInfo& ki)
   case DFX_K_P2TET_ELAST_L:
     ki = {DFX_TET, 4, 10, 3, false};
     return true;
+  case DFX_K_ELEMENT_A:
+    ki = {-1, 0, 0, 1, true}; // any element: k from the descriptor
+    return true;
+  case DFX_K_ELEMENT_L:
+    ki = {-1, 0, 0, 1, false};
+    return true;
   case DFX_K_P3TET_A:
     ki = {DFX_TET, 4, 20, 1, true};
     return true;
@@ -1471,6 +1477,41 @@ __global__ void __launch_bounds__(kBlock)
       vals[base + t] = acc[t];
 }
 
+/// BC column masks of element-tensor forms: bit j of mask[c] is set when
+/// node j of cell c is a Dirichlet dof.
+__global__ void __launch_bounds__(kBlock)
+    element_mask_k(std::int32_t nc, int k, const std::int32_t* __restrict__ dm,
+                   const std::int8_t* __restrict__ bc, std::uint32_t* __restrict__ mask)
+{
+  const std::int64_t c = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (c >= nc)
+    return;
+  std::uint32_t m = 0;
+  for (int j = 0; j < k; ++j)
+    if (bc[dm[c * k + j]])
+      m |= 1u << j;
+  mask[c] = m;
+}
+
+/// b += element vectors supplied by the caller (e: cells x k, cell-major),
+/// one thread per row adding its incidences in cell order onto b[r], the
+/// order of the reference's cell loop.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    element_vector_k(std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
+                     const std::int32_t* __restrict__ inc, const T* __restrict__ e,
+                     T* __restrict__ b)
+{
+  const std::int32_t r = xcd_contiguous(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (r >= nplan)
+    return;
+  const std::int32_t p1 = inc_off[r + 1];
+  T s = b[r];
+  for (std::int32_t p = inc_off[r]; p < p1; ++p)
+    s += e[inc[p]];
+  b[r] = s;
+}
+
 // ---------------------------------------------------------------------
 // Structured Q1 rows. On a box of hexahedra with lexicographic vertex order
 // a row vertex is corner I of at most one cell for each I = 0..7, and
@@ -2694,6 +2735,54 @@ void launch_gather(dfx_form& f, dfx_csr& A, const dfx_bc* bc, bool mat)
     go(row_gather_k<T, O, K, false>);
 }
 
+/// Element matrices supplied by the caller (E: cells x K x K), gathered by
+/// rows with the incidence-parallel kernel.
+template <typename T, typename O, int K>
+void launch_elements(dfx_form& f, dfx_csr& A, const dfx_bc* bc, const T* E)
+{
+  ensure_gather(f, A, sizeof(T));
+  const std::size_t lds = std::size_t(f.gblk_slice) * sizeof(T);
+  require(lds <= 64 * 1024, "row too long for the LDS slice");
+  const std::int32_t nc = f.d.num_cells;
+  if (bc && nc > 0)
+    element_mask_k<<<blocks_for(nc), kBlock, 0, stream()>>>(nc, K, f.dofmap, bc->marker,
+                                                            f.cell_mask);
+  if (f.ngblk > 0)
+    row_gather_k<T, O, K, true><<<f.ngblk, kBlock, lds, stream()>>>(
+        f.gblk, f.inc_off, f.inc, f.inc_row, static_cast<const O*>(f.islots), nullptr, E,
+        bc ? f.cell_mask : nullptr, bc ? bc->marker : nullptr, A.row_ptr, nullptr,
+        static_cast<T*>(A.values), A.zeroed ? 0 : 1, f.ngblk, f.max_dof + 1);
+  check_launch("element matrix assembly");
+}
+
+template <typename T>
+void assemble_elements(dfx_form& f, dfx_csr& A, const dfx_bc* bc, const T* E)
+{
+  ensure_plan(f, A);
+  auto go = [&](auto o)
+  {
+    using O = decltype(o);
+    switch (f.d.dofs_per_cell)
+    {
+    case 4:
+      launch_elements<T, O, 4>(f, A, bc, E);
+      break;
+    case 8:
+      launch_elements<T, O, 8>(f, A, bc, E);
+      break;
+    case 10:
+      launch_elements<T, O, 10>(f, A, bc, E);
+      break;
+    default:
+      launch_elements<T, O, 20>(f, A, bc, E);
+    }
+  };
+  if (f.islot_bytes == 1)
+    go(std::uint8_t{});
+  else
+    go(std::uint16_t{});
+}
+
 template <typename T, typename O, int K>
 void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
 {
@@ -2972,6 +3061,74 @@ void launch_linear(const dfx_form& f, T* b)
 
 /// True if every hexahedron is a parallelepiped (x3 = x1 + x2 - x0 and
 /// likewise for x5, x6, x7), so that its Jacobian is constant.
+bool is_element_kernel(int id) { return id == DFX_K_ELEMENT_A || id == DFX_K_ELEMENT_L; }
+
+/// Element tensors of these sizes k (k x k matrices) have row kernels.
+bool element_matrix_size(int k) { return k == 4 || k == 8 || k == 10 || k == 20; }
+
+/// Incidences (cell, local index) of every dofmap row in cell order: the
+/// row-owner plan of a form (counting sort of the dofmap entries).
+void build_incidences(dfx_form& f, const std::int32_t* dofmap)
+{
+  const std::int64_t nc = f.d.num_cells;
+  const int k = f.d.dofs_per_cell;
+  const std::int32_t nplan = f.max_dof + 1;
+  std::vector<std::int32_t> off(std::size_t(nplan) + 1, 0);
+  for (std::int64_t i = 0; i < nc * k; ++i)
+    ++off[dofmap[i] + 1];
+  std::int32_t mx = 0;
+  for (std::int32_t r = 0; r < nplan; ++r)
+  {
+    mx = std::max(mx, off[r + 1]);
+    off[r + 1] += off[r];
+  }
+  std::vector<std::int32_t> inc(std::max<std::int64_t>(nc * k, 1));
+  std::vector<std::int32_t> pos(off.begin(), off.end() - 1);
+  for (std::int64_t i = 0; i < nc * k; ++i)
+    inc[pos[dofmap[i]]++] = static_cast<std::int32_t>(i);
+  f.max_inc = mx;
+  f.inc_off = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * off.size()));
+  h2d(f.inc_off, off.data(), sizeof(std::int32_t) * off.size());
+  f.inc = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * inc.size()));
+  h2d(f.inc, inc.data(), sizeof(std::int32_t) * inc.size());
+}
+
+/// A form whose element tensors the caller supplies: the dofmap and its
+/// row-owner plan only.
+std::unique_ptr<dfx_form> create_element_form(const dfx_form_desc& d, const KernelInfo& ki)
+{
+  require(d.bs == 1, "element tensors are supported for block size 1");
+  require(d.dofs_per_cell >= 1 && d.dofs_per_cell <= 32, "dofs per cell must be 1..32");
+  require(!ki.bilinear || element_matrix_size(d.dofs_per_cell),
+          "element matrices need k in {4, 8, 10, 20}");
+  require(d.dofmap, "null dofmap");
+  const std::int64_t nc = d.num_cells;
+  std::int32_t max_dof = -1;
+  for (std::int64_t i = 0; i < nc * d.dofs_per_cell; ++i)
+  {
+    require(d.dofmap[i] >= 0, "negative dof");
+    max_dof = std::max(max_dof, d.dofmap[i]);
+  }
+  auto f = std::make_unique<dfx_form>();
+  f->max_dof = max_dof;
+  f->d = d;
+  f->d.x = nullptr;
+  f->d.x_dofmap = nullptr;
+  f->d.dofmap = nullptr;
+  f->d.coeffs = nullptr;
+  f->d.constants = nullptr;
+  f->d.num_nodes = 0;
+  f->d.nodes_per_cell = 0;
+  f->d.cstride = 0;
+  f->d.num_constants = 0;
+  f->dofmap = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * std::max<std::int64_t>(nc * d.dofs_per_cell, 1)));
+  h2d(f->dofmap, d.dofmap, sizeof(std::int32_t) * nc * d.dofs_per_cell);
+  build_incidences(*f, d.dofmap);
+  if (ki.bilinear)
+    f->cell_mask = static_cast<std::uint32_t*>(dmalloc(sizeof(std::uint32_t) * std::max<std::int64_t>(nc, 1)));
+  return f;
+}
+
 bool hex_all_affine(const dfx_form_desc& d)
 {
   for (std::int64_t c = 0; c < d.num_cells; ++c)
@@ -3124,6 +3281,11 @@ dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out)
         require(kernel_info(d.kernel, ki), "unknown kernel id");
         require(d.dtype == DFX_F32 || d.dtype == DFX_F64, "bad dtype");
         require(d.num_cells >= 0 && d.num_nodes >= 0, "negative size");
+        if (is_element_kernel(d.kernel))
+        {
+          *out = create_element_form(d, ki).release();
+          return;
+        }
         require(d.nodes_per_cell == ki.npc, "geometry nodes per cell do not match the kernel");
         require(d.dofs_per_cell == ki.k && d.bs == ki.bs, "dofmap does not match the kernel");
         require(d.x && d.x_dofmap && d.dofmap, "null mesh data");
@@ -3174,30 +3336,12 @@ dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out)
         }
         f->coeffs = upload_as(d.coeffs, nc * d.cstride, d.dtype);
         // Row-owner plan: incidences (cell, local index) of every row, in
-        // cell order (counting sort of the dofmap entries)
+        // cell order
         {
           const int k = d.dofs_per_cell;
-          const std::int32_t nplan = max_dof + 1;
-          std::vector<std::int32_t> off(std::size_t(nplan) + 1, 0);
-          for (std::int64_t i = 0; i < nc * k; ++i)
-            ++off[d.dofmap[i] + 1];
-          std::int32_t mx = 0;
-          for (std::int32_t r = 0; r < nplan; ++r)
-          {
-            mx = std::max(mx, off[r + 1]);
-            off[r + 1] += off[r];
-          }
-          std::vector<std::int32_t> inc(std::max<std::int64_t>(nc * k, 1));
-          std::vector<std::int32_t> pos(off.begin(), off.end() - 1);
-          for (std::int64_t i = 0; i < nc * k; ++i)
-            inc[pos[d.dofmap[i]]++] = static_cast<std::int32_t>(i);
-          f->max_inc = mx;
+          build_incidences(*f, d.dofmap);
           // thread-per-row slices do not fit LDS for wide elements
           f->row_variant = k > 8 ? 3 : 1;
-          f->inc_off = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * off.size()));
-          h2d(f->inc_off, off.data(), sizeof(std::int32_t) * off.size());
-          f->inc = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * inc.size()));
-          h2d(f->inc, inc.data(), sizeof(std::int32_t) * inc.size());
           const std::size_t ts = dtype_size(d.dtype);
           if (ki.bilinear)
           {
@@ -3391,6 +3535,8 @@ dfx_status dfx_assemble_matrix(dfx_csr* A, const dfx_form* a, const dfx_bc* bc)
         KernelInfo ki;
         kernel_info(a->d.kernel, ki);
         require(ki.bilinear, "assemble_matrix needs a bilinear form");
+        require(!is_element_kernel(a->d.kernel),
+                "element-tensor forms are assembled with dfx_assemble_matrix_elements");
         require(A->dtype == a->d.dtype, "matrix and form dtypes differ");
         require(A->bs == a->d.bs, "matrix and form block sizes differ");
         require(!bc || bc->n_total == A->nrows * A->bs, "BC size does not match the matrix");
@@ -3601,6 +3747,8 @@ dfx_status dfx_assemble_vector(dfx_vec* b, const dfx_form* L)
         KernelInfo ki;
         kernel_info(L->d.kernel, ki);
         require(!ki.bilinear, "assemble_vector needs a linear form");
+        require(!is_element_kernel(L->d.kernel),
+                "element-tensor forms are assembled with dfx_assemble_vector_elements");
         require(b->dtype == L->d.dtype, "vector and form dtypes differ");
         require(std::int64_t(L->max_dof + 1) * L->d.bs <= b->n, "dofmap exceeds the vector size");
         if (L->d.kernel == DFX_K_P1TET_L || L->d.kernel == DFX_K_Q1HEX_L
@@ -3619,6 +3767,61 @@ dfx_status dfx_assemble_vector(dfx_vec* b, const dfx_form* L)
       });
 }
 
+dfx_status dfx_assemble_matrix_elements(dfx_csr* A, const dfx_form* a, const dfx_bc* bc,
+                                        const dfx_vec* E)
+{
+  return guard(
+      [&]
+      {
+        require(A && a && E, "null argument");
+        require(a->d.kernel == DFX_K_ELEMENT_A, "the form does not take element matrices");
+        require(A->dtype == a->d.dtype && E->dtype == a->d.dtype,
+                "matrix, form and element matrices dtypes differ");
+        require(A->bs == 1, "element matrices need a block size 1 matrix");
+        const std::int64_t k = a->d.dofs_per_cell;
+        require(std::int64_t(E->n) == std::int64_t(a->d.num_cells) * k * k,
+                "element matrices: num_cells x k x k values expected");
+        require(!bc || bc->n_total == A->nrows, "BC size does not match the matrix");
+        require(a->max_dof < A->nrows && a->max_dof < A->ncols, "dofmap exceeds the matrix size");
+        dfx_form& f = const_cast<dfx_form&>(*a);
+        // store mode (A zeroed) overwrites every entry
+        if (A->dtype == DFX_F64)
+          assemble_elements<double>(f, *A, bc, static_cast<const double*>(E->data));
+        else
+          assemble_elements<float>(f, *A, bc, static_cast<const float*>(E->data));
+        A->zeroed = false;
+        A->zero_pending = false;
+      });
+}
+
+dfx_status dfx_assemble_vector_elements(dfx_vec* b, const dfx_form* L, const dfx_vec* e)
+{
+  return guard(
+      [&]
+      {
+        require(b && L && e, "null argument");
+        require(L->d.kernel == DFX_K_ELEMENT_L, "the form does not take element vectors");
+        require(b->dtype == L->d.dtype && e->dtype == L->d.dtype,
+                "vector, form and element vectors dtypes differ");
+        require(std::int64_t(e->n) == std::int64_t(L->d.num_cells) * L->d.dofs_per_cell,
+                "element vectors: num_cells x k values expected");
+        require(L->max_dof < b->n, "dofmap exceeds the vector size");
+        const std::int32_t nplan = L->max_dof + 1;
+        if (nplan <= 0)
+          return;
+        const int nb = (nplan + kBlock - 1) / kBlock;
+        if (b->dtype == DFX_F64)
+          element_vector_k<double><<<nb, kBlock, 0, stream()>>>(
+              nplan, L->inc_off, L->inc, static_cast<const double*>(e->data),
+              static_cast<double*>(b->data));
+        else
+          element_vector_k<float><<<nb, kBlock, 0, stream()>>>(
+              nplan, L->inc_off, L->inc, static_cast<const float*>(e->data),
+              static_cast<float*>(b->data));
+        check_launch("element vector assembly");
+      });
+}
+
 dfx_status dfx_apply_lifting(dfx_vec* b, const dfx_form* a, const dfx_bc* bc, const dfx_vec* x0,
                              double alpha)
 {
@@ -3629,6 +3832,8 @@ dfx_status dfx_apply_lifting(dfx_vec* b, const dfx_form* a, const dfx_bc* bc, co
         KernelInfo ki;
         kernel_info(a->d.kernel, ki);
         require(ki.bilinear, "apply_lifting needs a bilinear form");
+        require(!is_element_kernel(a->d.kernel),
+                "apply_lifting needs a form with a kernel (element-tensor forms have none)");
         require(b->dtype == a->d.dtype, "vector and form dtypes differ");
         if (!bc || bc->ndofs == 0)
           return;

@@ -36,6 +36,10 @@ Context& ctx();
 void ensure_init();
 hipStream_t stream();
 void check_launch(const char* what);
+/// Workgroups of `kernel` (threads each, lds bytes of dynamic LDS) resident
+/// on the device at once: the grid of a persistent kernel, so that no
+/// workgroup waits for another to finish before it starts its share.
+int resident_blocks(const void* kernel, int threads, std::size_t lds);
 
 inline std::size_t dtype_size(int dtype) { return dtype == DFX_F64 ? 8 : 4; }
 

@@ -9,6 +9,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 namespace dfx
@@ -46,6 +49,23 @@ hipStream_t stream()
 {
   ensure_init();
   return g_ctx.stream;
+}
+
+int resident_blocks(const void* kernel, int threads, std::size_t lds)
+{
+  ensure_init();
+  static std::mutex mu;
+  static std::map<std::tuple<const void*, int, std::size_t>, int> known;
+  std::lock_guard<std::mutex> lock(mu);
+  const auto key = std::make_tuple(kernel, threads, lds);
+  auto it = known.find(key);
+  if (it != known.end())
+    return it->second;
+  int per_cu = 0;
+  DFX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds));
+  const int n = std::max(per_cu, 1) * g_ctx.num_cu;
+  known.emplace(key, n);
+  return n;
 }
 
 void check_launch(const char* what)
@@ -538,6 +558,25 @@ void materialize_zero(const dfx_csr& A)
   A.zero_pending = false;
 }
 
+/// Queries the resident grids of the stream SpMV kernels once, at matrix
+/// creation, so that no solve pays for the first occupancy query.
+void prime_stream_grids()
+{
+  static std::once_flag once;
+  std::call_once(once,
+                 []
+                 {
+                   const void* k[] = {reinterpret_cast<const void*>(spmv_stream_k<double>),
+                                      reinterpret_cast<const void*>(spmv_stream_k<float>),
+                                      reinterpret_cast<const void*>(bsr_stream_k<double, 3>),
+                                      reinterpret_cast<const void*>(bsr_stream_k<float, 3>),
+                                      reinterpret_cast<const void*>(bsr_stream_k<double, 2>),
+                                      reinterpret_cast<const void*>(bsr_stream_k<float, 2>)};
+                   for (const void* f : k)
+                     resident_blocks(f, kBlock, 0);
+                 });
+}
+
 template <typename T>
 void launch_spmv(const dfx_csr& A, const T* x, T* y, bool accumulate)
 {
@@ -545,7 +584,9 @@ void launch_spmv(const dfx_csr& A, const T* x, T* y, bool accumulate)
   hipStream_t s = stream();
   if (A.bs == 1 && lanes_per_row(A) == 1)
   {
-    const int nb = std::max(1, std::min(A.nblk, ctx().num_cu * 8));
+    const int nb = std::max(
+        1, std::min(A.nblk, resident_blocks(reinterpret_cast<const void*>(spmv_stream_k<T>),
+                                            kBlock, 0)));
     spmv_stream_k<T><<<nb, kBlock, 0, s>>>(A.nblk, A.rblk, A.row_ptr, A.cols,
                                            static_cast<const T*>(A.values), x, y,
                                            accumulate, nullptr, nullptr, nullptr);
@@ -578,7 +619,9 @@ void launch_spmv(const dfx_csr& A, const T* x, T* y, bool accumulate)
     const T* v = static_cast<const T*>(A.values);
     if (A.nblk > 0 && A.lanes <= 1 && (A.bs == 2 || A.bs == 3))
     {
-      const int nb = std::max(1, std::min(A.nblk, ctx().num_cu * 8));
+      const void* kern = A.bs == 3 ? reinterpret_cast<const void*>(bsr_stream_k<T, 3>)
+                                   : reinterpret_cast<const void*>(bsr_stream_k<T, 2>);
+      const int nb = std::max(1, std::min(A.nblk, resident_blocks(kern, kBlock, 0)));
       if (A.bs == 3)
         bsr_stream_k<T, 3><<<nb, kBlock, 0, s>>>(A.nblk, A.rblk, A.row_ptr, A.cols, v, x, y,
                                                  accumulate, nullptr, nullptr, nullptr);
@@ -864,7 +907,10 @@ int launch_cg_spmv(const dfx_csr& A, const T* p, T* q, CGState* st)
   const T* v = static_cast<const T*>(A.values);
   if (A.bs == 1 && lanes_per_row(A) == 1)
   {
-    const int nb = std::max(1, std::min({A.nblk, ctx().num_cu * 8, kMaxPartials}));
+    const int nb = std::max(
+        1, std::min({A.nblk,
+                     resident_blocks(reinterpret_cast<const void*>(spmv_stream_k<T>), kBlock, 0),
+                     kMaxPartials}));
     spmv_stream_k<T><<<nb, kBlock, 0, s>>>(A.nblk, A.rblk, A.row_ptr, A.cols, v, p, q, 0, p,
                                            st->partials, &st->done);
     return nb;
@@ -893,7 +939,9 @@ int launch_cg_spmv(const dfx_csr& A, const T* p, T* q, CGState* st)
   }
   if (A.nblk > 0 && A.lanes <= 1 && (A.bs == 2 || A.bs == 3))
   {
-    const int nb = std::max(1, std::min({A.nblk, ctx().num_cu * 8, kMaxPartials}));
+    const void* kern = A.bs == 3 ? reinterpret_cast<const void*>(bsr_stream_k<T, 3>)
+                                 : reinterpret_cast<const void*>(bsr_stream_k<T, 2>);
+    const int nb = std::max(1, std::min({A.nblk, resident_blocks(kern, kBlock, 0), kMaxPartials}));
     if (A.bs == 3)
       bsr_stream_k<T, 3><<<nb, kBlock, 0, s>>>(A.nblk, A.rblk, A.row_ptr, A.cols, v, p, q, 0, p,
                                                st->partials, &st->done);
@@ -1231,6 +1279,7 @@ dfx_status dfx_csr_create(const int64_t* row_ptr, const int32_t* cols, int32_t n
           diag_pos_k<<<(nrows + kBlock - 1) / kBlock, kBlock, 0, stream()>>>(nrows, A->row_ptr, A->cols, A->diag);
         check_launch("csr_create");
         DFX_HIP(hipStreamSynchronize(stream()));
+        prime_stream_grids();
         A->pattern_id = next_pattern_id();
         *out = A.release();
       });

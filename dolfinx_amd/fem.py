@@ -154,6 +154,9 @@ class Form:
     coefficients (fem/Form.h:116-)."""
 
     def __init__(self, name, V: FunctionSpace, coeffs=None, constants=None, dtype=np.float64):
+        if name in ELEMENT_FORMS:
+            self._init_elements(name, V, dtype)
+            return
         if name not in KERNELS:
             raise ValueError(f"unknown form {name}")
         kid, ct, deg, bs, rank = KERNELS[name]
@@ -179,6 +182,20 @@ class Form:
         d.coeffs = _lib.ptr(self.coeffs)
         d.num_constants = len(self.constants)
         d.constants = _lib.ptr(self.constants)
+        self._h = C.c_void_p()
+        _lib.call("dfx_form_create", C.byref(d), C.byref(self._h))
+
+    def _init_elements(self, name, V, dtype):
+        kid, rank = ELEMENT_FORMS[name]
+        if V.dofmap.bs != 1:
+            raise ValueError("element tensors need a scalar space")
+        self.name, self.V, self.rank, self.dtype = name, V, rank, np.dtype(dtype)
+        self.coeffs, self.constants = None, np.zeros(1)
+        dm = np.ascontiguousarray(V.dofmap.list)
+        d = _lib.FormDesc()
+        d.kernel, d.dtype = kid, _lib.code_of(dtype)
+        d.num_cells, d.dofs_per_cell, d.bs, d.dofmap = dm.shape[0], dm.shape[1], 1, _lib.ptr(dm)
+        self._dm = dm
         self._h = C.c_void_p()
         _lib.call("dfx_form_create", C.byref(d), C.byref(self._h))
 
@@ -229,6 +246,32 @@ def form(name, V, coeffs=None, constants=None, dtype=np.float64) -> Form:
     return Form(name, V, coeffs, constants, dtype)
 
 
+# Forms whose element tensors the caller supplies (the reference's custom
+# kernels, cpp/demo/custom_kernel/main.cpp:73-118): name -> (kernel, rank)
+ELEMENT_FORMS = {
+    "element_matrix": (_lib.K_ELEMENT_A, 2),
+    "element_vector": (_lib.K_ELEMENT_L, 1),
+}
+
+
+def element_form(V: FunctionSpace, rank: int, dtype=np.float64) -> Form:
+    """A form on V whose cell tensors come from the caller: rank 2 takes
+    element matrices (cells x k x k, k in {4, 8, 10, 20}), rank 1
+    element vectors (cells x k), in V's cell order and local dof order."""
+    return Form("element_matrix" if rank == 2 else "element_vector", V, dtype=dtype)
+
+
+def _device_tensors(x, n, dtype):
+    if isinstance(x, Vector):
+        if x.size != n:
+            raise ValueError(f"{n} values expected, got {x.size}")
+        return x
+    a = np.ascontiguousarray(x, dtype).ravel()
+    if a.size != n:
+        raise ValueError(f"{n} values expected, got {a.size}")
+    return Vector.from_array(a, dtype)
+
+
 def create_sparsity_pattern(a: Form) -> SparsityPattern:
     """fem::create_sparsity_pattern (fem/utils.h:197-218)."""
     h = C.c_void_p()
@@ -255,6 +298,34 @@ def assemble_matrix(a: Form, bcs=(), diag=1.0, A: MatrixCSR | None = None) -> Ma
     for bc in bcs:
         _lib.call("dfx_set_diagonal", A.handle, bc.handle, float(diag))
     return A
+
+
+def assemble_matrix_elements(a: Form, E, bcs=(), diag=1.0, A: MatrixCSR | None = None) -> MatrixCSR:
+    """fem.assemble_matrix with a custom kernel whose outputs are E (cells x
+    k x k, host array or device Vector): A[dofs_c, dofs_c] += E_c in cell
+    order with BC rows/columns skipped, then ``diag`` on BC rows."""
+    if A is None:
+        A = create_matrix(a)
+    k = a._dm.shape[1]
+    Ed = _device_tensors(E, a._dm.shape[0] * k * k, a.dtype)
+    if len(bcs) > 1:
+        raise ValueError("pass the union of Dirichlet dofs as one DirichletBC")
+    bc = bcs[0] if bcs else None
+    _lib.call("dfx_assemble_matrix_elements", A.handle, a.handle,
+              None if bc is None else bc.handle, Ed.handle)
+    if bc is not None:
+        _lib.call("dfx_set_diagonal", A.handle, bc.handle, float(diag))
+    return A
+
+
+def assemble_vector_elements(L: Form, e, b: Vector | None = None) -> Vector:
+    """fem.assemble_vector with a custom kernel whose outputs are e (cells x
+    k): b[dofs_c] += e_c in cell order."""
+    if b is None:
+        b = Vector(L.V.num_dofs, L.dtype)
+    ed = _device_tensors(e, L._dm.shape[0] * L._dm.shape[1], L.dtype)
+    _lib.call("dfx_assemble_vector_elements", b.handle, L.handle, ed.handle)
+    return b
 
 
 def assemble_matrix_batched(forms, bcs, matrices, diag=1.0) -> None:

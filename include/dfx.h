@@ -51,7 +51,9 @@ enum
   DFX_K_P2TET_ELAST_A = 4, /* sigma(u):eps(v), w = [lambda, mu] (DG0) */
   DFX_K_P2TET_ELAST_L = 5, /* f.v, c = [fx, fy, fz] */
   DFX_K_P3TET_A = 6,       /* kappa grad u.grad v + c u v; w=[kappa] c=[c] */
-  DFX_K_P3TET_L = 7        /* f v; c = [f] */
+  DFX_K_P3TET_L = 7,       /* f v; c = [f] */
+  DFX_K_ELEMENT_A = 8,     /* element matrices supplied by the caller */
+  DFX_K_ELEMENT_L = 9      /* element vectors supplied by the caller */
 };
 
 const char* dfx_last_error(void);
@@ -248,6 +250,21 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc,
  * pass; affine P1/Q1). The P2 elasticity form has a single row kernel and
  * ignores the variant. */
 dfx_status dfx_form_set_variant(dfx_form* a, int32_t variant);
+/* fem::assemble_matrix / fem::assemble_vector with a custom kernel
+ * (cpp/demo/custom_kernel/main.cpp:73-118): the form (kernel
+ * DFX_K_ELEMENT_A / DFX_K_ELEMENT_L) holds only a dofmap (bs = 1; no
+ * geometry or coefficients needed), and the kernel's outputs are given on
+ * the device: E = num_cells row-major k x k element matrices (a vector of
+ * num_cells k^2 values of A's dtype), e = num_cells x k element vectors.
+ * A[dofs_c, dofs_c] += E_c and b[dofs_c] += e_c as la::MatrixCSR::add
+ * (la/matrix_csr_impl.h:67-109) and assemble_cells
+ * (fem/assemble_vector_impl.h:72-116) insert them, each entry summed in
+ * cell order; rows and columns of bc's dofs are skipped
+ * (fem/assemble_matrix_impl.h:156-180); "Entry not in sparsity" if a pair
+ * is missing from A's pattern. Matrices: k in {4, 8, 10, 20}. */
+dfx_status dfx_assemble_matrix_elements(dfx_csr* A, const dfx_form* a, const dfx_bc* bc,
+                                        const dfx_vec* E);
+dfx_status dfx_assemble_vector_elements(dfx_vec* b, const dfx_form* L, const dfx_vec* e);
 /* fem::set_diagonal (fem/assembler.h:644-686): A[d,d] = diag for BC dofs */
 dfx_status dfx_set_diagonal(dfx_csr* A, const dfx_bc* bc, double diag);
 /* fem::assemble_vector(b, L) (fem/assembler.h:230-257): b += L */

@@ -81,6 +81,9 @@ def _declare(L):
                                       C.c_int, P, P, P, P, P, P]
     L.orc_assemble_vector.argtypes = [C.c_int, P, P, C.c_int, I32, P, C.c_int, C.c_int, P,
                                       C.c_int, P, P]
+    L.orc_add_element_matrices.restype = C.c_int
+    L.orc_add_element_matrices.argtypes = [I32, P, C.c_int, C.c_int, P, P, P, P, P, P]
+    L.orc_add_element_vectors.argtypes = [I32, P, C.c_int, C.c_int, P, P]
     L.orc_lift_bc.argtypes = [C.c_int, P, P, C.c_int, I32, P, C.c_int, C.c_int, P, C.c_int,
                               P, P, P, P, D, P]
     L.orc_bc_set.argtypes = [P, P, I32, P, P, D]
@@ -240,6 +243,33 @@ def assemble_vector(kernel, x, x_dofmap, dofmap, bs, coeffs, constants, n):
     b = np.zeros(n, np.float64)
     lib().orc_assemble_vector(kernel, _p(x), _p(xd), xd.shape[1], xd.shape[0], _p(dm),
                               dm.shape[1], bs, _p(co), cs, _p(kc), _p(b))
+    return b
+
+
+def add_element_matrices(dofmap, bs, E, bc0, bc1, row_ptr, cols, values=None):
+    """values += the element matrices E (cells x (bs k) x (bs k)) inserted
+    cell by cell, BC rows (bc0) and columns (bc1) zeroed first."""
+    dm = np.ascontiguousarray(dofmap, np.int32)
+    E = np.ascontiguousarray(E, np.float64)
+    b0 = None if bc0 is None else np.ascontiguousarray(bc0, np.int8)
+    b1 = None if bc1 is None else np.ascontiguousarray(bc1, np.int8)
+    if values is None:
+        values = np.zeros(len(cols) * bs * bs, np.float64)
+    rc = lib().orc_add_element_matrices(dm.shape[0], _p(dm), dm.shape[1], bs, _p(E), _p(b0),
+                                        _p(b1), _p(np.ascontiguousarray(row_ptr, np.int64)),
+                                        _p(np.ascontiguousarray(cols, np.int32)), _p(values))
+    if rc != 0:
+        raise RuntimeError("Entry not in sparsity")
+    return values
+
+
+def add_element_vectors(dofmap, bs, e, n, b=None):
+    """b += the element vectors e (cells x bs k) scattered cell by cell."""
+    dm = np.ascontiguousarray(dofmap, np.int32)
+    e = np.ascontiguousarray(e, np.float64)
+    if b is None:
+        b = np.zeros(n, np.float64)
+    lib().orc_add_element_vectors(dm.shape[0], _p(dm), dm.shape[1], bs, _p(e), _p(b))
     return b
 
 

@@ -707,9 +707,58 @@ void orc_tabulate(int id, double* A, const double* w, const double* c,
 }
 
 /* ====================================================================== */
-/* fem/assemble_matrix_impl.h:92-200 (assemble_cells_matrix, non-lifting)
- * with mat_set = MatrixCSR::add -> la/matrix_csr_impl.h:67-109
- * (insert_csr<BS,BS>, lower_bound per column, "+="). */
+/* One cell of fem/assemble_matrix_impl.h:92-200 after the kernel call: zero
+ * the BC rows (bc0) and columns (bc1) of Ae (:156-180), then mat_set =
+ * MatrixCSR::add -> la/matrix_csr_impl.h:67-109 (insert_csr<BS,BS>,
+ * lower_bound per column, "+="). Ae is modified. Returns -1 if an entry is
+ * not in the sparsity. */
+static int insert_cell(const int32_t* dofs, int k, int bs, double* Ae,
+                       const int8_t* bc0, const int8_t* bc1,
+                       const int64_t* row_ptr, const int32_t* cols,
+                       double* values)
+{
+  const int nd = bs * k;
+  if (bc0)
+    for (int i = 0; i < k; ++i)
+      for (int b = 0; b < bs; ++b)
+        if (bc0[bs * dofs[i] + b])
+          memset(Ae + (bs * i + b) * nd, 0, sizeof(double) * nd);
+  if (bc1)
+    for (int j = 0; j < k; ++j)
+      for (int b = 0; b < bs; ++b)
+        if (bc1[bs * dofs[j] + b])
+          for (int r = 0; r < nd; ++r)
+            Ae[r * nd + bs * j + b] = 0;
+  for (int i = 0; i < k; ++i)
+  {
+    const int32_t row = dofs[i];
+    const int32_t* c0 = cols + row_ptr[row];
+    const int64_t len = row_ptr[row + 1] - row_ptr[row];
+    for (int j = 0; j < k; ++j)
+    {
+      const int32_t col = dofs[j];
+      int64_t lo = 0, hi = len;
+      while (lo < hi)
+      {
+        const int64_t mid = (lo + hi) / 2;
+        if (c0[mid] < col)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      if (lo == len || c0[lo] != col)
+        return -1;
+      double* dst = values + (row_ptr[row] + lo) * bs * bs;
+      for (int a = 0; a < bs; ++a)
+        for (int b = 0; b < bs; ++b)
+          dst[a * bs + b] += Ae[(bs * i + a) * nd + bs * j + b];
+    }
+  }
+  return 0;
+}
+
+/* fem/assemble_matrix_impl.h:92-200 (assemble_cells_matrix, non-lifting):
+ * per cell gather the coordinates, zero Ae, call the kernel, insert. */
 int orc_assemble_matrix(int kernel, const double* x, const int32_t* x_dofmap,
                         int npc, int32_t num_cells, const int32_t* dofmap,
                         int k, int bs, const double* coeffs, int cstride,
@@ -731,48 +780,53 @@ int orc_assemble_matrix(int kernel, const double* x, const int32_t* x_dofmap,
     memset(Ae, 0, sizeof(double) * nd * nd);
     fn(Ae, coeffs ? coeffs + (int64_t)c * cstride : &zero,
        constants ? constants : &zero, cdofs, NULL, NULL, NULL);
-    if (bc0)
-      for (int i = 0; i < k; ++i)
-        for (int b = 0; b < bs; ++b)
-          if (bc0[bs * dofs[i] + b])
-            memset(Ae + (bs * i + b) * nd, 0, sizeof(double) * nd);
-    if (bc1)
-      for (int j = 0; j < k; ++j)
-        for (int b = 0; b < bs; ++b)
-          if (bc1[bs * dofs[j] + b])
-            for (int r = 0; r < nd; ++r)
-              Ae[r * nd + bs * j + b] = 0;
-    for (int i = 0; i < k; ++i)
+    if (insert_cell(dofs, k, bs, Ae, bc0, bc1, row_ptr, cols, values))
     {
-      const int32_t row = dofs[i];
-      const int32_t* c0 = cols + row_ptr[row];
-      const int64_t len = row_ptr[row + 1] - row_ptr[row];
-      for (int j = 0; j < k; ++j)
-      {
-        const int32_t col = dofs[j];
-        int64_t lo = 0, hi = len;
-        while (lo < hi)
-        {
-          const int64_t mid = (lo + hi) / 2;
-          if (c0[mid] < col)
-            lo = mid + 1;
-          else
-            hi = mid;
-        }
-        if (lo == len || c0[lo] != col)
-        {
-          free(Ae);
-          return -1;
-        }
-        double* dst = values + (row_ptr[row] + lo) * bs * bs;
-        for (int a = 0; a < bs; ++a)
-          for (int b = 0; b < bs; ++b)
-            dst[a * bs + b] += Ae[(bs * i + a) * nd + bs * j + b];
-      }
+      free(Ae);
+      return -1;
     }
   }
   free(Ae);
   return 0;
+}
+
+/* The same cell loop with the element tensors supplied by the caller (the
+ * outputs of a user kernel, as in cpp/demo/custom_kernel/main.cpp:73-91):
+ * E holds num_cells row-major (bs k) x (bs k) matrices. */
+int orc_add_element_matrices(int32_t num_cells, const int32_t* dofmap, int k,
+                             int bs, const double* E, const int8_t* bc0,
+                             const int8_t* bc1, const int64_t* row_ptr,
+                             const int32_t* cols, double* values)
+{
+  const int nd = bs * k;
+  double* Ae = (double*)malloc(sizeof(double) * nd * nd);
+  for (int32_t c = 0; c < num_cells; ++c)
+  {
+    memcpy(Ae, E + (int64_t)c * nd * nd, sizeof(double) * nd * nd);
+    if (insert_cell(dofmap + (int64_t)c * k, k, bs, Ae, bc0, bc1, row_ptr, cols,
+                    values))
+    {
+      free(Ae);
+      return -1;
+    }
+  }
+  free(Ae);
+  return 0;
+}
+
+/* fem/assemble_vector_impl.h:72-116 (assemble_cells) with the element
+ * vectors supplied by the caller: b[bs dofs_i + a] += e_c[bs i + a]. */
+void orc_add_element_vectors(int32_t num_cells, const int32_t* dofmap, int k,
+                             int bs, const double* e, double* b)
+{
+  for (int32_t c = 0; c < num_cells; ++c)
+  {
+    const int32_t* dofs = dofmap + (int64_t)c * k;
+    const double* be = e + (int64_t)c * k * bs;
+    for (int i = 0; i < k; ++i)
+      for (int a = 0; a < bs; ++a)
+        b[bs * dofs[i] + a] += be[bs * i + a];
+  }
 }
 
 /* fem/assemble_vector_impl.h:72-116 (assemble_cells) */

@@ -29,6 +29,8 @@
  *                        modes in the kernel and exact energy of linear
  *                        fields; source totals int f (1 and 1/2 for f = 1,
  *                        x_0, python/test/unit/fem/test_assembler.py:67-78).
+ *   orc_add_element_*    equal to orc_assemble_* fed the tabulated tensors;
+ *                        a dense scatter of random tensors on tiny meshes.
  *   orc_assemble_*       P1 mass total = |Omega|, A 1 = 0, "Entry not in
  *                        sparsity", lifting = dense column product,
  *                        manufactured linear solution exact at the nodes;
@@ -137,6 +139,15 @@ int orc_assemble_matrix(int kernel, const double* x, const int32_t* x_dofmap,
                         const double* constants, const int8_t* bc0,
                         const int8_t* bc1, const int64_t* row_ptr,
                         const int32_t* cols, double* values);
+/* Element tensors supplied by the caller (num_cells row-major (bs k)^2
+ * matrices / bs k vectors), inserted exactly as orc_assemble_matrix /
+ * orc_assemble_vector insert the kernel's outputs. */
+int orc_add_element_matrices(int32_t num_cells, const int32_t* dofmap, int k,
+                             int bs, const double* E, const int8_t* bc0,
+                             const int8_t* bc1, const int64_t* row_ptr,
+                             const int32_t* cols, double* values);
+void orc_add_element_vectors(int32_t num_cells, const int32_t* dofmap, int k,
+                             int bs, const double* e, double* b);
 void orc_assemble_vector(int kernel, const double* x, const int32_t* x_dofmap,
                          int nodes_per_cell, int32_t num_cells,
                          const int32_t* dofmap, int k, int bs,

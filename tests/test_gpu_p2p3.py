@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from dolfinx_amd import la, problems, solvers
+from dolfinx_amd import fem, la, problems, solvers
 
 from conftest import rel_err
 
@@ -42,6 +42,27 @@ def test_elasticity_matches_oracle(dtype, tol):
     assert rel_err(A.data, Ar) <= tol
     assert np.max(np.abs(A.data - Ar)) <= tol * np.max(np.abs(Ar))
     assert rel_err(b.array, br) <= tol
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_elasticity_sums_are_deterministic(dtype):
+    """Every entry is summed in cell order by one lane: repeated assemblies
+    are bitwise equal, a second assembly without zeroing doubles the
+    values exactly (x + x), and zeroing then assembling stores the same
+    bits again."""
+    n = (6, 5, 4)
+    p = problems.elasticity_problem(n, seed=11, dtype=dtype)
+    A = fem.assemble_matrix(p.a, [p.bc])
+    ref = A.data.copy()
+    A2 = fem.assemble_matrix(p.a, [p.bc])
+    assert np.array_equal(A2.data, ref)
+    A.set_value(0.0)
+    fem.assemble_matrix(p.a, [p.bc], 1.0, A)
+    assert np.array_equal(A.data, ref)
+    B = fem.assemble_matrix(p.a, [])
+    plain = B.data.copy()
+    fem.assemble_matrix(p.a, [], 1.0, B)          # add mode
+    assert np.array_equal(B.data, plain + plain)
 
 
 def test_elasticity_solve_matches_oracle():
@@ -115,8 +136,6 @@ def test_p3_lifting_nonzero_g():
     """Lifting with nonzero boundary data on the P3 space vs the oracle."""
     n = (2, 2, 2)
     p = problems.p3_problem(n, seed=10)
-    from dolfinx_amd import fem
-
     dofs = fem.locate_boundary_dofs(p.V)
     g = np.zeros(p.V.num_dofs)
     g[dofs] = np.cos(np.arange(len(dofs)))

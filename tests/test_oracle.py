@@ -484,6 +484,65 @@ def test_lifting_equals_dense_column_product():
     assert np.allclose(b, -(D @ g), atol=1e-14)
 
 
+def tabulate_cells(kernel, m, coeffs, constants, ndim):
+    """Element tensors of every cell of m from the oracle's kernel."""
+    return np.stack([O.tabulate(kernel, m.x[m.cells[c]], w=coeffs[c], c=constants, ndim=ndim)
+                     for c in range(m.num_cells)])
+
+
+@pytest.mark.parametrize("ct,kA,kL", [(O.TET, O.K_P1TET_A, O.K_P1TET_L),
+                                      (O.HEX, O.K_Q1HEX_A, O.K_Q1HEX_L)])
+def test_element_insertion_equals_kernel_assembly(ct, kA, kL):
+    """Inserting the tabulated element tensors reproduces the assembly that
+    calls the kernel inside the cell loop, bit for bit (with BCs)."""
+    m = O.Mesh((3, 2, 2), ct)
+    dm, nd = m.dofmap(1)
+    rp, cols = O.sparsity(dm, dm, nd)
+    rng = np.random.default_rng(3)
+    kap = rng.uniform(0.5, 2.0, (m.num_cells, 1))
+    c = np.array([0.7])
+    bn = m.boundary_nodes(1, dm)
+    mark = np.zeros(nd, np.int8)
+    mark[bn] = 1
+    k = dm.shape[1]
+    E = tabulate_cells(kA, m, kap, c, k * k)
+    A = O.assemble_matrix(kA, m.x, m.cells, dm, 1, kap, c, mark, mark, rp, cols)
+    assert np.array_equal(O.add_element_matrices(dm, 1, E, mark, mark, rp, cols), A)
+    f = rng.standard_normal((m.num_cells, k))
+    e = tabulate_cells(kL, m, f, None, k)
+    assert np.array_equal(O.add_element_vectors(dm, 1, e, nd),
+                          O.assemble_vector(kL, m.x, m.cells, dm, 1, f, None, nd))
+
+
+def test_element_insertion_dense_scatter():
+    """Random element tensors against a dense scatter D[dofs, dofs] += E_c
+    with BC rows and columns removed (brute force), and a missing pattern
+    entry raises."""
+    m = O.Mesh((2, 1, 2), O.HEX)
+    dm, nd = m.dofmap(1)
+    rp, cols = O.sparsity(dm, dm, nd)
+    rng = np.random.default_rng(4)
+    E = rng.standard_normal((m.num_cells, 8, 8))
+    mark = (rng.uniform(size=nd) < 0.3).astype(np.int8)
+    D = np.zeros((nd, nd))
+    for cc in range(m.num_cells):
+        d = dm[cc]
+        Ec = E[cc].copy()
+        Ec[mark[d] == 1, :] = 0
+        Ec[:, mark[d] == 1] = 0
+        D[np.ix_(d, d)] += Ec
+    vals = O.add_element_matrices(dm, 1, E, mark, mark, rp, cols)
+    assert np.allclose(sp.csr_matrix((vals, cols, rp), shape=(nd, nd)).toarray(), D,
+                       rtol=1e-15, atol=1e-15)
+    e = rng.standard_normal((m.num_cells, 8))
+    b = np.zeros(nd)
+    np.add.at(b, dm.ravel(), e.ravel())
+    assert np.allclose(O.add_element_vectors(dm, 1, e, nd), b, rtol=1e-15, atol=1e-15)
+    with pytest.raises(RuntimeError, match="Entry not in sparsity"):
+        O.add_element_matrices(dm, 1, E, None, None, np.arange(nd + 1, dtype=np.int64),
+                               np.arange(nd, dtype=np.int32))
+
+
 @pytest.mark.parametrize("jacobi", [True, False])
 def test_cg_against_scipy(jacobi):
     m, dm, nd, rp, cols, A, bn, mark = assemble_p1((8, 8, 8), kappa=1.0)
