@@ -104,6 +104,7 @@ SIGNATURES = {
     "dfx_assemble_matrix": (C.c_int, [P, P, P]),
     "dfx_assemble_matrix_elements": (C.c_int, [P, P, P, P]),
     "dfx_assemble_vector_elements": (C.c_int, [P, P, P]),
+    "dfx_apply_lifting_elements": (C.c_int, [P, P, P, P, P, C.c_double]),
     "dfx_set_diagonal": (C.c_int, [P, P, D]),
     "dfx_form_autotune": (C.c_int, [P, P, P, P, P]),
     "dfx_form_set_variant": (C.c_int, [P, I32]),

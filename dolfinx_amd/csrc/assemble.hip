@@ -1512,6 +1512,43 @@ __global__ void __launch_bounds__(kBlock)
   b[r] = s;
 }
 
+/// Lifting with caller-supplied element matrices (E: cells x k x k):
+/// b[r] -= alpha sum_{(c,i) at r} sum_{j in BC} E_c[i][j] (g_j - x0_j), one
+/// thread per row over the cells whose mask marks a BC column.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    element_lift_k(std::int32_t nplan, int k, const std::int32_t* __restrict__ inc_off,
+                   const std::int32_t* __restrict__ inc, const std::int32_t* __restrict__ dm,
+                   const T* __restrict__ E, const std::uint32_t* __restrict__ mask,
+                   const double* __restrict__ g, const T* __restrict__ x0, double alpha,
+                   T* __restrict__ b)
+{
+  const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nplan)
+    return;
+  double s = 0.0;
+  for (std::int32_t p = inc_off[r]; p < inc_off[r + 1]; ++p)
+  {
+    const std::int32_t cik = inc[p];
+    const std::int32_t c = cik / k;
+    const std::uint32_t m = mask[c];
+    if (!m)
+      continue;
+    const T* er = E + std::int64_t(cik) * k;
+    for (int j = 0; j < k; ++j)
+    {
+      if (!((m >> j) & 1u))
+        continue;
+      const std::int32_t col = dm[std::int64_t(c) * k + j];
+      const double gv = g ? g[col] : 0.0;
+      const double xv = x0 ? double(x0[col]) : 0.0;
+      s += double(er[j]) * (gv - xv);
+    }
+  }
+  if (s != 0.0)
+    b[r] = static_cast<T>(double(b[r]) - alpha * s);
+}
+
 // ---------------------------------------------------------------------
 // Structured Q1 rows. On a box of hexahedra with lexicographic vertex order
 // a row vertex is corner I of at most one cell for each I = 0..7, and
@@ -3819,6 +3856,52 @@ dfx_status dfx_assemble_vector_elements(dfx_vec* b, const dfx_form* L, const dfx
               nplan, L->inc_off, L->inc, static_cast<const float*>(e->data),
               static_cast<float*>(b->data));
         check_launch("element vector assembly");
+      });
+}
+
+dfx_status dfx_apply_lifting_elements(dfx_vec* b, const dfx_form* a, const dfx_bc* bc,
+                                      const dfx_vec* E, const dfx_vec* x0, double alpha)
+{
+  return guard(
+      [&]
+      {
+        require(b && a && E, "null argument");
+        require(a->d.kernel == DFX_K_ELEMENT_A, "the form does not take element matrices");
+        require(b->dtype == a->d.dtype && E->dtype == a->d.dtype,
+                "vector, form and element matrices dtypes differ");
+        const std::int64_t k = a->d.dofs_per_cell;
+        require(std::int64_t(E->n) == std::int64_t(a->d.num_cells) * k * k,
+                "element matrices: num_cells x k x k values expected");
+        if (!bc || bc->ndofs == 0)
+          return;
+        require(bc->n_total == b->n, "BC size does not match the vector");
+        require(!x0 || (x0->n == b->n && x0->dtype == b->dtype), "x0 does not match b");
+        require(a->max_dof < b->n, "dofmap exceeds the vector size");
+        // g = 0 and x0 = 0: every term alpha E_ij (g_j - x0_j) is zero
+        if (!bc->g && !x0)
+          return;
+        const std::int32_t nc = a->d.num_cells;
+        const std::int32_t nplan = a->max_dof + 1;
+        if (nc > 0)
+          element_mask_k<<<blocks_for(nc), kBlock, 0, stream()>>>(nc, a->d.dofs_per_cell,
+                                                                  a->dofmap, bc->marker,
+                                                                  a->cell_mask);
+        if (nplan > 0)
+        {
+          if (b->dtype == DFX_F64)
+            element_lift_k<double><<<blocks_for(nplan), kBlock, 0, stream()>>>(
+                nplan, a->d.dofs_per_cell, a->inc_off, a->inc, a->dofmap,
+                static_cast<const double*>(E->data), a->cell_mask, bc->g,
+                x0 ? static_cast<const double*>(x0->data) : nullptr, alpha,
+                static_cast<double*>(b->data));
+          else
+            element_lift_k<float><<<blocks_for(nplan), kBlock, 0, stream()>>>(
+                nplan, a->d.dofs_per_cell, a->inc_off, a->inc, a->dofmap,
+                static_cast<const float*>(E->data), a->cell_mask, bc->g,
+                x0 ? static_cast<const float*>(x0->data) : nullptr, alpha,
+                static_cast<float*>(b->data));
+        }
+        check_launch("element lifting");
       });
 }
 

@@ -328,6 +328,17 @@ def assemble_vector_elements(L: Form, e, b: Vector | None = None) -> Vector:
     return b
 
 
+def apply_lifting_elements(b: Vector, a: Form, E, bcs, x0: Vector | None = None,
+                           alpha=1.0) -> None:
+    """fem.apply_lifting for a form with caller-supplied element matrices
+    E (cells x k x k): b -= alpha E[:, bc] (g - x0) over the BC columns."""
+    k = a._dm.shape[1]
+    Ed = _device_tensors(E, a._dm.shape[0] * k * k, a.dtype)
+    for bc in bcs:
+        _lib.call("dfx_apply_lifting_elements", b.handle, a.handle, bc.handle, Ed.handle,
+                  None if x0 is None else x0.handle, float(alpha))
+
+
 def assemble_matrix_batched(forms, bcs, matrices, diag=1.0) -> None:
     """assemble_matrix for many independent systems of one P1 or Q1
     stiffness kernel in two launches (cell factors of all systems, rows of

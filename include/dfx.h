@@ -265,6 +265,12 @@ dfx_status dfx_form_set_variant(dfx_form* a, int32_t variant);
 dfx_status dfx_assemble_matrix_elements(dfx_csr* A, const dfx_form* a, const dfx_bc* bc,
                                         const dfx_vec* E);
 dfx_status dfx_assemble_vector_elements(dfx_vec* b, const dfx_form* L, const dfx_vec* e);
+/* fem::apply_lifting (fem/assembler.h:336-493 -> impl::lift_bc,
+ * fem/assemble_vector_impl.h:361-414) for a DFX_K_ELEMENT_A form, with the
+ * element matrices E the kernel would recompute for the cells with a BC
+ * column dof: b -= alpha E[:, bc] (g - x0). */
+dfx_status dfx_apply_lifting_elements(dfx_vec* b, const dfx_form* a, const dfx_bc* bc,
+                                      const dfx_vec* E, const dfx_vec* x0, double alpha);
 /* fem::set_diagonal (fem/assembler.h:644-686): A[d,d] = diag for BC dofs */
 dfx_status dfx_set_diagonal(dfx_csr* A, const dfx_bc* bc, double diag);
 /* fem::assemble_vector(b, L) (fem/assembler.h:230-257): b += L */

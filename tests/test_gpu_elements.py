@@ -6,6 +6,7 @@ for bit (every entry is summed in cell order on both sides), FP32 within
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 import oracle as O
 from dolfinx_amd import fem, la, mesh
@@ -94,6 +95,49 @@ def test_element_vectors(name, dtype):
     fem.assemble_vector_elements(L, e.astype(dtype), b)
     if dtype == np.float64:
         assert np.array_equal(b.array, O.add_element_vectors(dm, 1, e, nd, ref.copy()))
+
+
+def test_element_lifting_p1_matches_oracle():
+    """apply_lifting with the supplied element matrices equals the oracle's
+    lift_bc, which recomputes them with the kernel (nonzero g, x0, alpha)."""
+    n = (3, 3, 2)
+    om = O.Mesh(n, O.TET)
+    dm, nd = om.dofmap(1)
+    msh = mesh.create_box([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], n, mesh.CellType.tetrahedron)
+    V = fem.functionspace(msh, ("Lagrange", 1))
+    kap = np.random.default_rng(7).uniform(0.5, 2.0, om.num_cells)
+    E = np.stack([O.tabulate(O.K_P1TET_A, om.x[om.cells[c]], w=[kap[c]], c=[0.4], ndim=16)
+                  for c in range(om.num_cells)])
+    bn = om.boundary_nodes(1, dm)
+    mark = np.zeros(nd, np.int8)
+    mark[bn] = 1
+    g = np.zeros(nd)
+    g[bn] = 1.0 + np.cos(np.arange(len(bn)))
+    x0 = np.random.default_rng(8).standard_normal(nd)
+    b0 = np.random.default_rng(9).standard_normal(nd)
+    ref = O.lift_bc(b0.copy(), O.K_P1TET_A, om.x, om.cells, dm, 1, kap.reshape(-1, 1),
+                    np.array([0.4]), g, mark, x0=x0, alpha=0.7)
+    a = fem.element_form(V, 2)
+    b = la.Vector.from_array(b0)
+    fem.apply_lifting_elements(b, a, E, [fem.dirichletbc(V, bn, g)],
+                               la.Vector.from_array(x0), alpha=0.7)
+    assert rel_err(b.array, ref) <= 1e-12
+
+
+def test_element_lifting_dense_column_product():
+    """Random P2 element matrices: lifting = -alpha A_full[:, bc] g with
+    A_full the assembled matrix without BCs (brute force)."""
+    om, dm, nd, V, E = setup("p2_tet", (2, 2, 2), 10)
+    rp, cols = O.sparsity(dm, dm, nd)
+    Afull = O.add_element_matrices(dm, 1, E, None, None, rp, cols)
+    bn = om.boundary_nodes(2, dm)
+    g = np.zeros(nd)
+    g[bn] = np.sin(np.arange(len(bn)) + 1.0)
+    D = sp.csr_matrix((Afull, cols, rp), shape=(nd, nd))
+    b = la.Vector(nd)
+    fem.apply_lifting_elements(b, fem.element_form(V, 2), E, [fem.dirichletbc(V, bn, g)],
+                               alpha=1.5)
+    assert rel_err(b.array, -1.5 * (D @ g)) <= 1e-12
 
 
 def test_element_matrix_outside_pattern_raises():
