@@ -165,7 +165,8 @@ struct dfx_csr
   mutable bool zero_pending = false;
   int lanes = 0;                   // SpMV variant: 0 default, 1 stream, G>1 vector
   // Row blocks of the stream SpMV: consecutive rows whose entries fit one
-  // LDS chunk (or a single longer row); device, nblk + 1 row boundaries.
+  // LDS chunk (or a single longer row); device, nblk + 1 boundaries as
+  // (row, entry) pairs.
   std::int32_t* rblk = nullptr;
   std::int32_t nblk = 0;
   // pattern arrays borrowed from another matrix (not freed by this one)

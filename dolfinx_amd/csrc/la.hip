@@ -210,14 +210,17 @@ constexpr int stream_chunk()
 /// entries fit one chunk) is processed by one workgroup: the products
 /// v_j x[c_j] of the block's contiguous entry range are formed with fully
 /// coalesced loads into LDS, then one thread per row sums its products left
-/// to right, the order of the reference's row-serial loop. The entries of
-/// the workgroup's next row block are loaded into registers while the rows
-/// of the current one are summed. A block holding a single longer row is
-/// reduced with a strided sum. With dotv, partial sums of y.dotv go to
-/// partials.
+/// to right, the order of the reference's row-serial loop. Row blocks are
+/// given as (first row, first entry) pairs (rbj, nblk + 1 of them), so a
+/// block's bounds are one load. While the rows of the current block are
+/// summed, the next block's entries and its rows' positions (and y, dotv)
+/// are in flight, and the bounds of the block after it: each block waits
+/// for one memory round trip (its x gather) instead of a chain of them. A
+/// block holding a single longer row is reduced with a strided sum. With
+/// dotv, partial sums of y.dotv go to partials.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-    spmv_stream_k(std::int32_t nblk, const std::int32_t* __restrict__ rblk,
+    spmv_stream_k(std::int32_t nblk, const int2* __restrict__ rbj,
                   const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
                   const T* __restrict__ v, const T* __restrict__ x, T* __restrict__ y,
                   int accumulate, const T* __restrict__ dotv, double* partials,
@@ -230,115 +233,144 @@ __global__ void __launch_bounds__(kBlock)
   __shared__ T prod[NC];
   __shared__ double red[4];
   double d[1] = {0.0};
+  struct Bounds
+  {
+    std::int32_t r0, r1, j0, j1;
+  };
+  // this thread's row of a block: its products [kb, ke) in the chunk, and
+  // y and dotv at the row
+  struct Row
+  {
+    std::int32_t kb, ke;
+    T yold, dv;
+  };
+  auto bounds = [&](int b) -> Bounds
+  {
+    const int2 lo = rbj[b], hi = rbj[b + 1];
+    return {lo.x, hi.x, lo.y, hi.y};
+  };
+  auto row_of = [&](const Bounds& B) -> Row
+  {
+    Row w{0, 0, T(0), T(0)};
+    const std::int32_t r = B.r0 + static_cast<std::int32_t>(threadIdx.x);
+    if (r < B.r1)
+    {
+      w.kb = rp[r] - B.j0;
+      w.ke = rp[r + 1] - B.j0;
+      if (accumulate)
+        w.yold = y[r];
+      if (dotv)
+        w.dv = dotv[r];
+    }
+    return w;
+  };
   T vv[U];
   std::int32_t cc[U];
-  std::int32_t r0 = 0, r1 = 0, j0 = 0, j1 = 0;
-  auto bounds = [&](int b)
-  {
-    r0 = rblk[b];
-    r1 = rblk[b + 1];
-    j0 = rp[r0];
-    j1 = rp[r1];
-  };
-  // unconditional loads at clamped positions (no branch around a load, so
-  // all of them are in flight together); entries past j1 are not used
-  auto stage = [&]()
-  {
-    const std::int32_t last = j1 - 1;
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-    {
-      const std::int32_t j = min(j0 + static_cast<std::int32_t>(threadIdx.x) + u * kBlock, last);
-      vv[u] = v[j];
-      cc[u] = ci[j];
-    }
-  };
 #pragma unroll
   for (int u = 0; u < U; ++u)
   {
     vv[u] = T(0);
     cc[u] = 0;
   }
+  // unconditional loads at clamped positions (no branch around a load, so
+  // all of them are in flight together); entries past j1 are not used
+  auto stage = [&](const Bounds& B)
+  {
+    const std::int32_t last = max(B.j1 - 1, B.j0);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      const std::int32_t j = min(B.j0 + static_cast<std::int32_t>(threadIdx.x) + u * kBlock, last);
+      vv[u] = v[j];
+      cc[u] = ci[j];
+    }
+  };
   const XcdRange xr = xcd_range(nblk);
   int bi = xr.first;
   if (bi < xr.end)
   {
-    bounds(bi);
-    if (j1 > j0 && j1 - j0 <= NC)
-      stage();
-  }
-  while (bi < xr.end)
-  {
-    const int nbi = bi + xr.step;
-    if (j1 - j0 <= NC)
+    Bounds cur = bounds(bi);
+    const bool cur_short = cur.j1 - cur.j0 <= NC;
+    if (cur_short && cur.j1 > cur.j0)
+      stage(cur);
+    Row row = row_of(cur);
+    int nbi = bi + xr.step;
+    Bounds nxt = nbi < xr.end ? bounds(nbi) : Bounds{0, 0, 0, 0};
+    for (;;)
     {
-      T xv[U];
+      const bool more = nbi < xr.end;
+      const int nnbi = nbi + xr.step;
+      Row nrow{0, 0, T(0), T(0)};
+      Bounds nnxt{0, 0, 0, 0};
+      if (cur.j1 - cur.j0 <= NC)
+      {
+        if (cur.j1 > cur.j0)
+        {
+          T xv[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u)
-        xv[u] = x[cc[u]];
+          for (int u = 0; u < U; ++u)
+            xv[u] = x[cc[u]];
 #pragma unroll
-      for (int u = 0; u < U; ++u)
-      {
-        const std::int32_t j = j0 + threadIdx.x + u * kBlock;
-        if (j < j1)
-          prod[j - j0] = vv[u] * xv[u];
+          for (int u = 0; u < U; ++u)
+          {
+            const std::int32_t j = cur.j0 + threadIdx.x + u * kBlock;
+            if (j < cur.j1)
+              prod[j - cur.j0] = vv[u] * xv[u];
+          }
+        }
+        lds_barrier();
+        // the next block's loads go out before the sums (a row block has
+        // at most kBlock rows: one row per thread)
+        if (more)
+        {
+          if (nxt.j1 - nxt.j0 <= NC && nxt.j1 > nxt.j0)
+            stage(nxt);
+          nrow = row_of(nxt);
+          if (nnbi < xr.end)
+            nnxt = bounds(nnbi);
+        }
+        const std::int32_t r = cur.r0 + threadIdx.x;
+        if (r < cur.r1)
+        {
+          T sum = 0;
+          for (std::int32_t k = row.kb; k < row.ke; ++k)
+            sum += prod[k];
+          const T yr = accumulate ? row.yold + sum : sum;
+          y[r] = yr;
+          if (dotv)
+            d[0] += double(yr) * double(row.dv);
+        }
+        lds_barrier();
       }
-      lds_barrier();
-      // a row block has at most kBlock rows: one row per thread. Its global
-      // reads come first, so that the next block's loads issued below are
-      // not waited for before the LDS sums.
-      const std::int32_t r = r0 + threadIdx.x;
-      const bool has = r < r1;
-      std::int32_t kb = 0, ke = 0;
-      T yold = T(0), dv = T(0);
-      if (has)
+      else
       {
-        kb = rp[r] - j0;
-        ke = rp[r + 1] - j0;
-        if (accumulate)
-          yold = y[r];
-        if (dotv)
-          dv = dotv[r];
+        double sum = 0;
+        for (std::int32_t j = cur.j0 + threadIdx.x; j < cur.j1; j += kBlock)
+          sum += double(v[j] * x[ci[j]]);
+        const double t = block_sum_256(sum, red);
+        if (threadIdx.x == 0)
+        {
+          const T yr = accumulate ? row.yold + static_cast<T>(t) : static_cast<T>(t);
+          y[cur.r0] = yr;
+          if (dotv)
+            d[0] += double(yr) * double(row.dv);
+        }
+        if (more)
+        {
+          if (nxt.j1 - nxt.j0 <= NC && nxt.j1 > nxt.j0)
+            stage(nxt);
+          nrow = row_of(nxt);
+          if (nnbi < xr.end)
+            nnxt = bounds(nnbi);
+        }
       }
-      if (nbi < xr.end)
-      {
-        bounds(nbi);
-        if (j1 > j0 && j1 - j0 <= NC)
-          stage();
-      }
-      if (has)
-      {
-        T s = 0;
-        for (std::int32_t k = kb; k < ke; ++k)
-          s += prod[k];
-        const T yr = accumulate ? yold + s : s;
-        y[r] = yr;
-        if (dotv)
-          d[0] += double(yr) * double(dv);
-      }
-      lds_barrier();
+      if (!more)
+        break;
+      cur = nxt;
+      row = nrow;
+      nxt = nnxt;
+      nbi = nnbi;
     }
-    else
-    {
-      double s = 0;
-      for (std::int32_t j = j0 + threadIdx.x; j < j1; j += kBlock)
-        s += double(v[j] * x[ci[j]]);
-      const double t = block_sum_256(s, red);
-      if (threadIdx.x == 0)
-      {
-        const T yr = accumulate ? y[r0] + static_cast<T>(t) : static_cast<T>(t);
-        y[r0] = yr;
-        if (dotv)
-          d[0] += double(yr) * double(dotv[r0]);
-      }
-      if (nbi < xr.end)
-      {
-        bounds(nbi);
-        if (j1 > j0 && j1 - j0 <= NC)
-          stage();
-      }
-    }
-    bi = nbi;
   }
   if (partials)
     block_partials<1>(d, partials);
@@ -354,7 +386,7 @@ __global__ void __launch_bounds__(kBlock)
 /// are loaded while the current rows are summed.
 template <typename T, int BS>
 __global__ void __launch_bounds__(kBlock)
-    bsr_stream_k(std::int32_t nblk, const std::int32_t* __restrict__ rblk,
+    bsr_stream_k(std::int32_t nblk, const int2* __restrict__ rbj,
                  const std::int32_t* __restrict__ rp, const std::int32_t* __restrict__ ci,
                  const T* __restrict__ v, const T* __restrict__ x, T* __restrict__ y,
                  int accumulate, const T* __restrict__ dotv, double* partials,
@@ -379,10 +411,11 @@ __global__ void __launch_bounds__(kBlock)
   std::int32_t r0 = 0, r1 = 0, j0 = 0, j1 = 0;
   auto bounds = [&](int b)
   {
-    r0 = rblk[b];
-    r1 = rblk[b + 1];
-    j0 = rp[r0];
-    j1 = rp[r1];
+    const int2 lo = rbj[b], hi = rbj[b + 1];
+    r0 = lo.x;
+    r1 = hi.x;
+    j0 = lo.y;
+    j1 = hi.y;
   };
   // task t of the row block: block j0 + t / BS, component t % BS; loads at
   // clamped positions so that no load sits behind a branch
@@ -558,6 +591,9 @@ void materialize_zero(const dfx_csr& A)
   A.zero_pending = false;
 }
 
+/// The stream SpMV row blocks of A as (first row, first entry) pairs.
+inline const int2* row_blocks(const dfx_csr& A) { return reinterpret_cast<const int2*>(A.rblk); }
+
 /// Queries the resident grids of the stream SpMV kernels once, at matrix
 /// creation, so that no solve pays for the first occupancy query.
 void prime_stream_grids()
@@ -587,7 +623,7 @@ void launch_spmv(const dfx_csr& A, const T* x, T* y, bool accumulate)
     const int nb = std::max(
         1, std::min(A.nblk, resident_blocks(reinterpret_cast<const void*>(spmv_stream_k<T>),
                                             kBlock, 0)));
-    spmv_stream_k<T><<<nb, kBlock, 0, s>>>(A.nblk, A.rblk, A.row_ptr, A.cols,
+    spmv_stream_k<T><<<nb, kBlock, 0, s>>>(A.nblk, row_blocks(A), A.row_ptr, A.cols,
                                            static_cast<const T*>(A.values), x, y,
                                            accumulate, nullptr, nullptr, nullptr);
   }
@@ -623,10 +659,10 @@ void launch_spmv(const dfx_csr& A, const T* x, T* y, bool accumulate)
                                    : reinterpret_cast<const void*>(bsr_stream_k<T, 2>);
       const int nb = std::max(1, std::min(A.nblk, resident_blocks(kern, kBlock, 0)));
       if (A.bs == 3)
-        bsr_stream_k<T, 3><<<nb, kBlock, 0, s>>>(A.nblk, A.rblk, A.row_ptr, A.cols, v, x, y,
+        bsr_stream_k<T, 3><<<nb, kBlock, 0, s>>>(A.nblk, row_blocks(A), A.row_ptr, A.cols, v, x, y,
                                                  accumulate, nullptr, nullptr, nullptr);
       else
-        bsr_stream_k<T, 2><<<nb, kBlock, 0, s>>>(A.nblk, A.rblk, A.row_ptr, A.cols, v, x, y,
+        bsr_stream_k<T, 2><<<nb, kBlock, 0, s>>>(A.nblk, row_blocks(A), A.row_ptr, A.cols, v, x, y,
                                                  accumulate, nullptr, nullptr, nullptr);
       check_launch("spmv");
       return;
@@ -911,7 +947,7 @@ int launch_cg_spmv(const dfx_csr& A, const T* p, T* q, CGState* st)
         1, std::min({A.nblk,
                      resident_blocks(reinterpret_cast<const void*>(spmv_stream_k<T>), kBlock, 0),
                      kMaxPartials}));
-    spmv_stream_k<T><<<nb, kBlock, 0, s>>>(A.nblk, A.rblk, A.row_ptr, A.cols, v, p, q, 0, p,
+    spmv_stream_k<T><<<nb, kBlock, 0, s>>>(A.nblk, row_blocks(A), A.row_ptr, A.cols, v, p, q, 0, p,
                                            st->partials, &st->done);
     return nb;
   }
@@ -943,10 +979,10 @@ int launch_cg_spmv(const dfx_csr& A, const T* p, T* q, CGState* st)
                                  : reinterpret_cast<const void*>(bsr_stream_k<T, 2>);
     const int nb = std::max(1, std::min({A.nblk, resident_blocks(kern, kBlock, 0), kMaxPartials}));
     if (A.bs == 3)
-      bsr_stream_k<T, 3><<<nb, kBlock, 0, s>>>(A.nblk, A.rblk, A.row_ptr, A.cols, v, p, q, 0, p,
+      bsr_stream_k<T, 3><<<nb, kBlock, 0, s>>>(A.nblk, row_blocks(A), A.row_ptr, A.cols, v, p, q, 0, p,
                                                st->partials, &st->done);
     else
-      bsr_stream_k<T, 2><<<nb, kBlock, 0, s>>>(A.nblk, A.rblk, A.row_ptr, A.cols, v, p, q, 0, p,
+      bsr_stream_k<T, 2><<<nb, kBlock, 0, s>>>(A.nblk, row_blocks(A), A.row_ptr, A.cols, v, p, q, 0, p,
                                                st->partials, &st->done);
     return nb;
   }
@@ -1259,7 +1295,8 @@ dfx_status dfx_csr_create(const int64_t* row_ptr, const int32_t* cols, int32_t n
           // and entries fitting the LDS chunk of the matrix precision (a
           // longer scalar row gets a block of its own)
           const std::int32_t max_rows = kBlock / bs;
-          std::vector<std::int32_t> rb{0};
+          // (first row, first entry) of each block, then (nrows, nnz)
+          std::vector<std::int32_t> rb{0, 0};
           std::int32_t r = 0;
           while (r < nrows)
           {
@@ -1267,9 +1304,10 @@ dfx_status dfx_csr_create(const int64_t* row_ptr, const int32_t* cols, int32_t n
             while (e < nrows && e - r < max_rows && rp32[e + 1] - rp32[r] <= chunk)
               ++e;
             rb.push_back(e);
+            rb.push_back(rp32[e]);
             r = e;
           }
-          A->nblk = static_cast<std::int32_t>(rb.size()) - 1;
+          A->nblk = static_cast<std::int32_t>(rb.size() / 2) - 1;
           A->rblk = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * rb.size()));
           h2d(A->rblk, rb.data(), sizeof(std::int32_t) * rb.size());
         }
