@@ -171,6 +171,22 @@ struct dfx_csr
   std::int32_t nblk = 0;
   // pattern arrays borrowed from another matrix (not freed by this one)
   const dfx_csr* pattern_owner = nullptr;
+  // Sliced-ELL copy used inside the solvers (bs = 1), refreshed from the CSR
+  // values at the start of every solve: slices of 64 consecutive rows,
+  // a slice's entries stored column-major (entry k of its rows together),
+  // padded to the slice's longest row. sell_off: nslices + 1 entry offsets,
+  // then nslices + 1 byte offsets into sell_cols; sell_base: per slice the
+  // base column of its 16-bit column offsets, or -1 - (first column) for a
+  // slice with 32-bit columns. Layout and columns are kept while the
+  // pattern (sell_pattern) is unchanged; sell_use says whether the solvers
+  // take this copy. Owned by this matrix.
+  mutable void* sell_vals = nullptr;
+  mutable void* sell_cols = nullptr;
+  mutable std::int64_t* sell_off = nullptr;
+  mutable std::int32_t* sell_base = nullptr;
+  mutable std::int32_t nslices = 0;
+  mutable std::uint64_t sell_pattern = 0;
+  mutable bool sell_use = false;
 };
 
 struct dfx_vec

@@ -376,6 +376,178 @@ __global__ void __launch_bounds__(kBlock)
     block_partials<1>(d, partials);
 }
 
+/// Rows per slice of the sliced-ELL solver copy (one wavefront).
+constexpr int kSlice = 64;
+
+/// Slice shapes of the sliced-ELL copy: for slice s, out[3 s] = its
+/// longest row and out[3 s + 1], out[3 s + 2] = its smallest and largest
+/// column (one wavefront per slice, lane = row; columns are sorted in a row).
+__global__ void __launch_bounds__(kBlock)
+    sell_shape_k(std::int32_t nrows, std::int32_t nslices, const std::int32_t* __restrict__ rp,
+                 const std::int32_t* __restrict__ ci, std::int32_t* __restrict__ out)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  const std::int32_t sl = static_cast<std::int32_t>(t / kSlice);
+  if (sl >= nslices)
+    return;
+  const std::int32_t r = static_cast<std::int32_t>(t);
+  std::int32_t len = 0, lo = 0x7fffffff, hi = -1;
+  if (r < nrows)
+  {
+    const std::int32_t j0 = rp[r], j1 = rp[r + 1];
+    len = j1 - j0;
+    if (len > 0)
+    {
+      lo = ci[j0];
+      hi = ci[j1 - 1];
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+  {
+    len = max(len, __shfl_xor(len, o, 64));
+    lo = min(lo, __shfl_xor(lo, o, 64));
+    hi = max(hi, __shfl_xor(hi, o, 64));
+  }
+  if (t % kSlice == 0)
+  {
+    out[3 * std::int64_t(sl)] = len;
+    out[3 * std::int64_t(sl) + 1] = lo;
+    out[3 * std::int64_t(sl) + 2] = hi;
+  }
+}
+
+/// Sliced-ELL copy of a CSR matrix: thread (slice, lane) writes entry k of
+/// row slice*64 + lane at voff[slice] + 64 k + lane, padded with value 0.
+/// Columns (written when cols is given) are 16-bit offsets from the slice's
+/// base column where the slice's columns span less than 2^16 (base >= 0),
+/// else 32-bit indices (base = -1); padding columns are the slice's first
+/// column. off holds voff (nslices + 1 entries), then the column byte
+/// offsets (nslices + 1).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    sell_build_k(std::int32_t nrows, std::int32_t nslices, const std::int64_t* __restrict__ off,
+                 const std::int32_t* __restrict__ cbase, const std::int32_t* __restrict__ rp,
+                 const std::int32_t* __restrict__ ci, const T* __restrict__ v,
+                 unsigned char* __restrict__ cols, T* __restrict__ sval)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  const std::int32_t sl = static_cast<std::int32_t>(t / kSlice);
+  if (sl >= nslices)
+    return;
+  const int lane = static_cast<int>(t % kSlice);
+  const std::int32_t r = sl * kSlice + lane;
+  const std::int64_t base = off[sl];
+  const std::int32_t w = static_cast<std::int32_t>((off[sl + 1] - base) / kSlice);
+  const std::int32_t j0 = r < nrows ? rp[r] : 0;
+  const std::int32_t len = r < nrows ? rp[r + 1] - j0 : 0;
+  for (std::int32_t k = 0; k < w; ++k)
+    sval[base + std::int64_t(k) * kSlice + lane] = k < len ? v[j0 + k] : T(0);
+  if (!cols)
+    return;
+  const std::int32_t b = cbase[sl];
+  unsigned char* c = cols + off[nslices + 1 + sl];
+  if (b >= 0)
+  {
+    std::uint16_t* c16 = reinterpret_cast<std::uint16_t*>(c);
+    for (std::int32_t k = 0; k < w; ++k)
+      c16[std::int64_t(k) * kSlice + lane] = static_cast<std::uint16_t>(k < len ? ci[j0 + k] - b : 0);
+  }
+  else
+  {
+    std::int32_t* c32 = reinterpret_cast<std::int32_t*>(c);
+    const std::int32_t pad = -b - 1; // the slice's first column, encoded
+    for (std::int32_t k = 0; k < w; ++k)
+      c32[std::int64_t(k) * kSlice + lane] = k < len ? ci[j0 + k] : pad;
+  }
+}
+
+/// Sliced-ELL SpMV for the solvers, y (+)= A x: one wavefront per slice of
+/// 64 rows, lane = row, each load instruction reading entry k of 64
+/// consecutive rows (coalesced, no LDS), 16-bit column offsets where the
+/// slice allows. Entries are taken eight at a time (values and columns,
+/// then the x gathers) and each row is summed left to right with separate
+/// multiply and add, the order and rounding of the stream kernel and of the
+/// reference's row loop; padding is skipped. Workgroups take groups of 4
+/// slices, each XCD a contiguous range. With dotv, partial sums of y.dotv
+/// go to partials.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    sell_spmv_k(std::int32_t nrows, std::int32_t nslices, const std::int64_t* __restrict__ off,
+                const std::int32_t* __restrict__ cbase, const std::int32_t* __restrict__ rp,
+                const unsigned char* __restrict__ cols, const T* __restrict__ sval,
+                const T* __restrict__ x, T* __restrict__ y, int accumulate,
+                const T* __restrict__ dotv, double* partials, const int* stop)
+{
+  if (stop && block_flag(stop))
+    return;
+  constexpr int G = 8;
+  double d[1] = {0.0};
+  const int lane = threadIdx.x % kSlice, wave = threadIdx.x / kSlice;
+  const std::int32_t ngroups = (nslices + 3) / 4;
+  const XcdRange xr = xcd_range(ngroups);
+  for (std::int32_t g = xr.first; g < xr.end; g += xr.step)
+  {
+    const std::int32_t sl = g * 4 + wave;
+    if (sl >= nslices)
+      continue;
+    const std::int64_t vb = off[sl] + lane;
+    const std::int32_t w = static_cast<std::int32_t>((off[sl + 1] - off[sl]) / kSlice);
+    const std::int32_t b = cbase[sl];
+    const unsigned char* c = cols + off[nslices + 1 + sl];
+    const std::int32_t r = sl * kSlice + lane;
+    const std::int32_t len = r < nrows ? rp[r + 1] - rp[r] : 0;
+    T sum = T(0);
+    for (std::int32_t k = 0; k < w; k += G)
+    {
+      T vv[G], xv[G];
+      std::int32_t cc[G];
+      if (b >= 0)
+      {
+        const std::uint16_t* c16 = reinterpret_cast<const std::uint16_t*>(c) + lane;
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+        {
+          const std::int64_t e = std::int64_t(min(k + u, w - 1)) * kSlice;
+          vv[u] = sval[vb + e];
+          cc[u] = b + c16[e];
+        }
+      }
+      else
+      {
+        const std::int32_t* c32 = reinterpret_cast<const std::int32_t*>(c) + lane;
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+        {
+          const std::int64_t e = std::int64_t(min(k + u, w - 1)) * kSlice;
+          vv[u] = sval[vb + e];
+          cc[u] = c32[e];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < G; ++u)
+        xv[u] = x[cc[u]];
+      {
+        // product then sum, not a fused multiply-add
+#pragma clang fp contract(off)
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+          if (k + u < len)
+            sum = sum + vv[u] * xv[u];
+      }
+    }
+    if (r < nrows)
+    {
+      const T yr = accumulate ? y[r] + sum : sum;
+      y[r] = yr;
+      if (dotv)
+        d[0] += double(yr) * double(dotv[r]);
+    }
+  }
+  if (partials)
+    block_partials<1>(d, partials);
+}
+
 /// Stream SpMV of a block-CSR matrix with BS x BS blocks (row-major inside
 /// a block), y (+)= A x. Row blocks hold at most kBlock / BS block rows and
 /// stream_chunk / BS blocks. Phase 1: task t = (block b, block-row component
@@ -936,11 +1108,96 @@ struct StateHolder
   ~StateHolder() { dfree(st); }
 };
 
+/// Refresh the sliced-ELL copy of A from its CSR values; the slice layout
+/// and columns are rebuilt only when the pattern changed. Returns whether
+/// the copy is used: scalar CSR with the stream kernel as its SpMV, and
+/// padding to the slices' longest rows adding at most 15 % entries (longer
+/// and shorter rows mixed in one slice, as in P3 patterns, stay on the
+/// stream kernel).
 template <typename T>
-int launch_cg_spmv(const dfx_csr& A, const T* p, T* q, CGState* st)
+bool refresh_sell(const dfx_csr& A)
+{
+  if (A.bs != 1 || A.nrows == 0 || lanes_per_row(A) != 1)
+    return false;
+  const std::int32_t ns = (A.nrows + kSlice - 1) / kSlice;
+  if (A.sell_pattern != A.pattern_id)
+  {
+    dfree(A.sell_vals);
+    dfree(A.sell_cols);
+    dfree(A.sell_off);
+    dfree(A.sell_base);
+    A.sell_vals = A.sell_cols = nullptr;
+    A.sell_off = nullptr;
+    A.sell_base = nullptr;
+    A.sell_use = false;
+    std::int32_t* shape = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * 3 * ns));
+    const std::int64_t nt = std::int64_t(ns) * kSlice;
+    sell_shape_k<<<static_cast<int>((nt + kBlock - 1) / kBlock), kBlock, 0, stream()>>>(
+        A.nrows, ns, A.row_ptr, A.cols, shape);
+    check_launch("sell_shape");
+    std::vector<std::int32_t> h(3 * std::size_t(ns));
+    d2h(h.data(), shape, sizeof(std::int32_t) * h.size());
+    dfree(shape);
+    // entry offsets, column byte offsets (16-byte aligned), slice bases
+    std::vector<std::int64_t> off(2 * (std::size_t(ns) + 1), 0);
+    std::vector<std::int32_t> base(ns);
+    std::int64_t cb = 0;
+    for (std::int32_t sl = 0; sl < ns; ++sl)
+    {
+      const std::int64_t w = h[3 * sl];
+      const std::int32_t lo = w > 0 ? h[3 * sl + 1] : 0, hi = w > 0 ? h[3 * sl + 2] : 0;
+      const bool narrow = std::int64_t(hi) - lo < 65536;
+      base[sl] = narrow ? lo : -1 - lo; // -1 - first column: 32-bit slice
+      off[sl + 1] = off[sl] + w * kSlice;
+      off[ns + 1 + sl] = cb;
+      cb += (w * kSlice * (narrow ? 2 : 4) + 15) / 16 * 16;
+    }
+    off[2 * std::size_t(ns) + 1] = cb;
+    A.sell_pattern = A.pattern_id;
+    if (off[ns] > A.nnz + A.nnz / 20 * 3)
+      return false; // padding above 15 %: the stream kernel is used
+    A.sell_off = static_cast<std::int64_t*>(dmalloc(sizeof(std::int64_t) * off.size()));
+    h2d(A.sell_off, off.data(), sizeof(std::int64_t) * off.size());
+    A.sell_base = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * std::max(ns, 1)));
+    h2d(A.sell_base, base.data(), sizeof(std::int32_t) * ns);
+    A.sell_cols = dmalloc(std::max<std::int64_t>(cb, 16));
+    A.sell_vals = dmalloc(sizeof(T) * std::max<std::int64_t>(off[ns], 1));
+    A.nslices = ns;
+    A.sell_use = true;
+    const int nb = static_cast<int>((nt + kBlock - 1) / kBlock);
+    sell_build_k<T><<<nb, kBlock, 0, stream()>>>(
+        A.nrows, ns, A.sell_off, A.sell_base, A.row_ptr, A.cols, static_cast<const T*>(A.values),
+        static_cast<unsigned char*>(A.sell_cols), static_cast<T*>(A.sell_vals));
+    check_launch("sell_build");
+    return true;
+  }
+  if (!A.sell_use)
+    return false;
+  const std::int64_t nt = std::int64_t(ns) * kSlice;
+  sell_build_k<T><<<static_cast<int>((nt + kBlock - 1) / kBlock), kBlock, 0, stream()>>>(
+      A.nrows, ns, A.sell_off, A.sell_base, A.row_ptr, A.cols, static_cast<const T*>(A.values),
+      nullptr, static_cast<T*>(A.sell_vals));
+  check_launch("sell_build");
+  return true;
+}
+
+template <typename T>
+int launch_cg_spmv(const dfx_csr& A, const T* p, T* q, CGState* st, bool sell)
 {
   hipStream_t s = stream();
   const T* v = static_cast<const T*>(A.values);
+  if (sell)
+  {
+    // 1024 workgroups (16 wavefronts per CU) keep the stream fed and halve
+    // the partials every update kernel sums
+    const int ngroups = (A.nslices + 3) / 4;
+    const int nb = std::max(1, std::min(ngroups, kMaxPartials / 2));
+    sell_spmv_k<T><<<nb, kBlock, 0, s>>>(
+        A.nrows, A.nslices, A.sell_off, A.sell_base, A.row_ptr,
+        static_cast<const unsigned char*>(A.sell_cols), static_cast<const T*>(A.sell_vals), p, q,
+        0, p, st->partials, &st->done);
+    return nb;
+  }
   if (A.bs == 1 && lanes_per_row(A) == 1)
   {
     const int nb = std::max(
@@ -1006,6 +1263,7 @@ int cg_run(const dfx_csr& A, const T* b, T* x, int max_it, double rtol, bool jac
   hipStream_t s = stream();
   const std::int64_t n = std::int64_t(A.nrows) * A.bs;
   const int nbv = partial_blocks(n);
+  const bool sell = refresh_sell<T>(A);
   if (jacobi)
   {
     const int nb = static_cast<int>((n + kBlock - 1) / kBlock);
@@ -1034,7 +1292,7 @@ int cg_run(const dfx_csr& A, const T* b, T* x, int max_it, double rtol, bool jac
   {
     for (int j = 0; j < batch && k < max_it; ++j, ++k)
     {
-      const int nbs = launch_cg_spmv<T>(A, w.p, w.q, st);
+      const int nbs = launch_cg_spmv<T>(A, w.p, w.q, st, sell);
       cg_update_k<T><<<nbv, kBlock, 0, s>>>(n, k, nbs, x, w.r, w.p, w.q,
                                             jacobi ? w.dinv : nullptr, st);
       cg_direction_k<T><<<nbv, kBlock, 0, s>>>(n, k, nbv, w.r, jacobi ? w.dinv : nullptr,
@@ -1336,6 +1594,13 @@ dfx_status dfx_csr_create_shared(const dfx_csr* P, int dtype, dfx_csr** out)
         A->dtype = dtype;
         A->pattern_owner = P;
         A->lanes = 0;
+        A->sell_vals = nullptr;
+        A->sell_cols = nullptr;
+        A->sell_off = nullptr;
+        A->sell_base = nullptr;
+        A->nslices = 0;
+        A->sell_pattern = 0;
+        A->sell_use = false;
         if (dtype != P->dtype)
         {
           // the stream SpMV row blocks depend on the value size
@@ -1363,6 +1628,10 @@ dfx_status dfx_csr_destroy(dfx_csr* A)
       dfree(A->diag);
     }
     dfree(A->values);
+    dfree(A->sell_vals);
+    dfree(A->sell_cols);
+    dfree(A->sell_off);
+    dfree(A->sell_base);
     delete A;
   }
   return DFX_OK;
@@ -1662,6 +1931,7 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
         require(b->n == A64->nrows * A64->bs && x->n == b->n, "size mismatch");
         materialize_zero(*A64);
         materialize_zero(*A32);
+        const bool sell = refresh_sell<float>(*A32);
         *res = dfx_mixed_result{};
         hipStream_t s = stream();
         const std::int64_t n = b->n;
@@ -1730,7 +2000,7 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
         {
           for (int j = 0; j < std::max(1, o->check_every) && k < o->max_inner; ++j, ++k)
           {
-            const int nbs = launch_cg_spmv<float>(*A32, w.p, w.q, st);
+            const int nbs = launch_cg_spmv<float>(*A32, w.p, w.q, st, sell);
             cg_update_k<float><<<nbv, kBlock, 0, s>>>(n, k, nbs, d32, w.r, w.p, w.q, w.dinv, st);
             cg_direction_k<float><<<nbv, kBlock, 0, s>>>(n, k, nbv, w.r, w.dinv, w.p, st);
           }
