@@ -465,7 +465,7 @@ __global__ void __launch_bounds__(kBlock)
 /// Sliced-ELL SpMV for the solvers, y (+)= A x: one wavefront per slice of
 /// 64 rows, lane = row, each load instruction reading entry k of 64
 /// consecutive rows (coalesced, no LDS), 16-bit column offsets where the
-/// slice allows. Entries are taken eight at a time (values and columns,
+/// slice allows. Entries are taken sixteen at a time (values and columns,
 /// then the x gathers) and each row is summed left to right with separate
 /// multiply and add, the order and rounding of the stream kernel and of the
 /// reference's row loop; padding is skipped. Workgroups take groups of 4
@@ -481,7 +481,7 @@ __global__ void __launch_bounds__(kBlock)
 {
   if (stop && block_flag(stop))
     return;
-  constexpr int G = 8;
+  constexpr int G = 16;
   double d[1] = {0.0};
   const int lane = threadIdx.x % kSlice, wave = threadIdx.x / kSlice;
   const std::int32_t ngroups = (nslices + 3) / 4;
