@@ -61,6 +61,7 @@ SIGNATURES = {
     "dfx_synchronize": (C.c_int, []),
     "dfx_stream": (P, []),
     "dfx_mesh_create_box": (C.c_int, [P, P, P, C.c_int, PP]),
+    "dfx_mesh_create": (C.c_int, [C.c_int, I32, P, I32, P, PP]),
     "dfx_mesh_destroy": (C.c_int, [P]),
     "dfx_mesh_sizes": (C.c_int, [P, P, P, P]),
     "dfx_mesh_copy": (C.c_int, [P, P, P, P, P]),

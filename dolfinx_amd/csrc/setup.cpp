@@ -365,6 +365,47 @@ int facet_closure(int cell, int degree, int f, int* out)
   return n;
 }
 
+/// mesh::create_mesh on one rank (mesh/utils.h:1073-1259) from input cells
+/// (input vertex indices) and input geometry: cells reordered by RCM on the
+/// dual graph, vertices numbered by first appearance in the reordered cells.
+void build_mesh(dfx_mesh& mesh, const std::vector<std::int32_t>& cin,
+                const std::vector<double>& xin)
+{
+  dfx_mesh* m = &mesh;
+  const int cell = m->cell;
+  const int nvpc = m->nvpc;
+  const std::int64_t nc = m->num_cells;
+  const std::int64_t nv = m->num_vertices;
+  // Reorder cells by RCM on the dual graph (mesh/utils.h:232-309)
+  const Graph g = dual_graph(cell, cin, m->num_cells, m->num_vertices);
+  const std::vector<std::int32_t> remap = reorder_rcm(g);
+  std::vector<std::int32_t> cr(nc * nvpc);
+  m->original_cell.resize(nc);
+  for (std::int64_t c = 0; c < nc; ++c)
+  {
+    std::copy_n(cin.begin() + c * nvpc, nvpc, cr.begin() + std::int64_t(remap[c]) * nvpc);
+    m->original_cell[remap[c]] = c;
+  }
+  // Vertices numbered by first appearance (Topology.cpp:1222-1258)
+  std::vector<std::int32_t> local(nv, -1);
+  std::int32_t next = 0;
+  m->cells.resize(nc * nvpc);
+  for (std::int64_t i = 0; i < nc * nvpc; ++i)
+  {
+    std::int32_t& l = local[cr[i]];
+    if (l < 0)
+      l = next++;
+    m->cells[i] = l;
+  }
+  m->x.resize(3 * nv);
+  m->input_vertex.resize(nv);
+  for (std::int64_t v = 0; v < nv; ++v)
+  {
+    std::copy_n(xin.begin() + 3 * v, 3, m->x.begin() + 3 * std::int64_t(local[v]));
+    m->input_vertex[local[v]] = v;
+  }
+}
+
 } // namespace
 
 extern "C" {
@@ -430,34 +471,45 @@ dfx_status dfx_mesh_create_box(const double p0[3], const double p1[3],
                   cin[q++] = v[k];
             }
 
-        // Reorder cells by RCM on the dual graph (mesh/utils.h:232-309)
-        const Graph g = dual_graph(cell, cin, m->num_cells, m->num_vertices);
-        const std::vector<std::int32_t> remap = reorder_rcm(g);
-        std::vector<std::int32_t> cr(nc * nvpc);
-        m->original_cell.resize(nc);
-        for (std::int64_t c = 0; c < nc; ++c)
+        build_mesh(*m, cin, xin);
+        *out = m.release();
+      });
+}
+
+dfx_status dfx_mesh_create(int cell, int32_t num_cells, const int64_t* cells,
+                           int32_t num_vertices, const double* x, dfx_mesh** out)
+{
+  return dfx::guard(
+      [&]
+      {
+        dfx::require(out && (num_cells == 0 || cells) && (num_vertices == 0 || x),
+                     "null argument");
+        dfx::require(cell == DFX_TET || cell == DFX_HEX, "unsupported cell type");
+        dfx::require(num_cells >= 0 && num_vertices >= 0, "negative size");
+        auto m = std::make_unique<dfx_mesh>();
+        m->cell = cell;
+        m->nvpc = cell == DFX_TET ? 4 : 8;
+        dfx::require(std::int64_t(num_cells) * m->nvpc < INT32_MAX, "mesh too large");
+        m->num_vertices = num_vertices;
+        m->num_cells = num_cells;
+        const std::int64_t n = std::int64_t(num_cells) * m->nvpc;
+        std::vector<std::int32_t> cin(n);
+        std::vector<char> used(num_vertices, 0);
+        for (std::int64_t i = 0; i < n; ++i)
         {
-          std::copy_n(cin.begin() + c * nvpc, nvpc, cr.begin() + std::int64_t(remap[c]) * nvpc);
-          m->original_cell[remap[c]] = c;
+          dfx::require(cells[i] >= 0 && cells[i] < num_vertices, "cell vertex out of range");
+          cin[i] = static_cast<std::int32_t>(cells[i]);
+          used[cin[i]] = 1;
         }
-        // Vertices numbered by first appearance (Topology.cpp:1222-1258)
-        std::vector<std::int32_t> local(nv, -1);
-        std::int32_t next = 0;
-        m->cells.resize(nc * nvpc);
-        for (std::int64_t i = 0; i < nc * nvpc; ++i)
-        {
-          std::int32_t& l = local[cr[i]];
-          if (l < 0)
-            l = next++;
-          m->cells[i] = l;
-        }
-        m->x.resize(3 * nv);
-        m->input_vertex.resize(nv);
-        for (std::int64_t v = 0; v < nv; ++v)
-        {
-          std::copy_n(xin.begin() + 3 * v, 3, m->x.begin() + 3 * std::int64_t(local[v]));
-          m->input_vertex[local[v]] = v;
-        }
+        for (std::int32_t v = 0; v < num_vertices; ++v)
+          dfx::require(used[v], "every vertex must belong to a cell");
+        for (std::int32_t c = 0; c < num_cells; ++c)
+          for (int a = 0; a < m->nvpc; ++a)
+            for (int b = a + 1; b < m->nvpc; ++b)
+              dfx::require(cin[std::int64_t(c) * m->nvpc + a] != cin[std::int64_t(c) * m->nvpc + b],
+                           "a cell repeats a vertex");
+        std::vector<double> xin(x, x + 3 * std::int64_t(num_vertices));
+        build_mesh(*m, cin, xin);
         *out = m.release();
       });
 }

@@ -80,5 +80,18 @@ def create_box(points, n, cell_type=CellType.tetrahedron) -> Mesh:
     return Mesh(h, CellType(int(cell_type)))
 
 
+def create_mesh(cell_type, cells, x) -> Mesh:
+    """mesh::create_mesh (cpp/dolfinx/mesh/utils.h:1336-1356) on one rank:
+    cells (num_cells x vertices, input vertex indices, DOLFINx vertex order)
+    and x (num_vertices x 3); cells reordered by RCM, vertices numbered by
+    first appearance."""
+    cells = np.ascontiguousarray(cells, np.int64)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    h = C.c_void_p()
+    _lib.call("dfx_mesh_create", int(cell_type), cells.shape[0], _lib.ptr(cells), x.shape[0],
+              _lib.ptr(x), C.byref(h))
+    return Mesh(h, CellType(int(cell_type)))
+
+
 def create_unit_cube(nx, ny, nz, cell_type=CellType.tetrahedron) -> Mesh:
     return create_box([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], [nx, ny, nz], cell_type)

@@ -77,6 +77,14 @@ typedef struct dfx_sparsity dfx_sparsity;
  * default graph::reorder_rcm cell reordering (graph/ordering.cpp:148). */
 dfx_status dfx_mesh_create_box(const double p0[3], const double p1[3],
                                const int64_t n[3], int cell, dfx_mesh** out);
+/* mesh::create_mesh (mesh/utils.h:1336-1356 -> :1074-1259) on one rank:
+ * cells (num_cells x 4 tetrahedra / 8 hexahedra, input vertex indices in
+ * the DOLFINx vertex order) and geometry x (num_vertices x 3, input
+ * order); every vertex must belong to a cell. Cells are reordered by RCM
+ * on the dual graph and vertices numbered by first appearance, as for
+ * create_box. */
+dfx_status dfx_mesh_create(int cell, int32_t num_cells, const int64_t* cells,
+                           int32_t num_vertices, const double* x, dfx_mesh** out);
 dfx_status dfx_mesh_destroy(dfx_mesh* m);
 dfx_status dfx_mesh_sizes(const dfx_mesh* m, int32_t* num_cells,
                           int32_t* num_vertices, int32_t* vertices_per_cell);

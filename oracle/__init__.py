@@ -61,6 +61,8 @@ def _declare(L):
     L.orc_mesh_create_box.restype = C.POINTER(_Mesh)
     L.orc_mesh_create_box.argtypes = [P, P, P, C.c_int]
     L.orc_mesh_free.argtypes = [C.POINTER(_Mesh)]
+    L.orc_mesh_create.restype = C.POINTER(_Mesh)
+    L.orc_mesh_create.argtypes = [C.c_int, P, I32, P, I32]
     L.orc_dual_graph.argtypes = [C.c_int, P, I32, C.POINTER(P), C.POINTER(P)]
     L.orc_rcm.argtypes = [P, P, I32, P]
     L.orc_entities.restype = I32
@@ -117,12 +119,15 @@ def _take(ptr, n, dtype):
 class Mesh:
     """Oracle mesh: create_box -> dual graph -> RCM -> topology."""
 
-    def __init__(self, n, celltype, p0=(0.0, 0.0, 0.0), p1=(1.0, 1.0, 1.0)):
+    def __init__(self, n, celltype, p0=(0.0, 0.0, 0.0), p1=(1.0, 1.0, 1.0), _ptr=None):
         L = lib()
-        p0 = np.asarray(p0, np.float64)
-        p1 = np.asarray(p1, np.float64)
-        nn = np.asarray(n, np.int64)
-        mp = L.orc_mesh_create_box(_p(p0), _p(p1), _p(nn), celltype)
+        if _ptr is None:
+            p0 = np.asarray(p0, np.float64)
+            p1 = np.asarray(p1, np.float64)
+            nn = np.asarray(n, np.int64)
+            mp = L.orc_mesh_create_box(_p(p0), _p(p1), _p(nn), celltype)
+        else:
+            mp = _ptr
         m = mp.contents
         self.celltype = celltype
         self.nvpc = m.nvpc
@@ -135,6 +140,15 @@ class Mesh:
         self.input_vertex_index = np.ctypeslib.as_array(
             m.input_vertex_index, shape=(m.num_vertices,)).copy()
         self._ptr = mp
+
+    @classmethod
+    def from_cells(cls, celltype, cells, x):
+        """mesh::create_mesh on one rank from input cells (input vertex
+        indices, DOLFINx vertex order) and input geometry."""
+        cells = np.ascontiguousarray(cells, np.int64)
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        mp = lib().orc_mesh_create(celltype, _p(cells), cells.shape[0], _p(x), x.shape[0])
+        return cls(None, celltype, _ptr=mp)
 
     def __del__(self):
         try:

@@ -421,22 +421,18 @@ void orc_rcm(const int32_t* offsets, const int32_t* data, int32_t n,
  * partitioner) -> utils.h:232-309 (dual graph + reorder_fn + reorder_list)
  * -> Topology.cpp:1222-1286 (vertex numbering by first appearance) ->
  * Geometry (P1 geometry dofmap equals the cell-vertex map). */
-orc_mesh* orc_mesh_create_box(const double p0[3], const double p1[3],
-                              const int64_t n[3], int celltype)
+/* mesh::create_mesh on one rank (mesh/utils.h:1073-1259): dual graph,
+ * RCM cell reordering, vertices numbered by first appearance. cells_in
+ * (nc x nvpc, input vertex indices) and x_in (nv x 3) are input order. */
+orc_mesh* orc_mesh_create(int celltype, const int64_t* cells_in, int32_t nc,
+                          const double* x_in, int32_t nv)
 {
   orc_mesh* m = (orc_mesh*)calloc(1, sizeof(orc_mesh));
   m->celltype = celltype;
   m->nvpc = celltype == ORC_TET ? 4 : 8;
-  const int64_t nv = orc_box_num_vertices(n);
-  const int64_t nc = orc_box_num_cells(n, celltype);
-  m->num_cells = (int32_t)nc;
-  m->num_vertices = (int32_t)nv;
+  m->num_cells = nc;
+  m->num_vertices = nv;
   const int nvpc = m->nvpc;
-
-  double* x_in = (double*)malloc(sizeof(double) * 3 * nv);
-  int64_t* cells_in = (int64_t*)malloc(sizeof(int64_t) * nvpc * nc);
-  orc_box_geometry(p0, p1, n, x_in);
-  orc_box_cells(n, celltype, cells_in);
 
   int32_t *goff, *gdata;
   orc_dual_graph(celltype, cells_in, (int32_t)nc, &goff, &gdata);
@@ -455,7 +451,6 @@ orc_mesh* orc_mesh_create_box(const double p0[3], const double p1[3],
     m->original_cell_index[remap[c]] = c;
   }
   free(remap);
-  free(cells_in);
 
   /* Vertex numbering by first appearance (single rank: all owned,
    * owned_vertices == 0..nv-1, Topology.cpp:1224-1242) */
@@ -480,6 +475,23 @@ orc_mesh* orc_mesh_create_box(const double p0[3], const double p1[3],
   }
   free(local);
   free(cells_r);
+  return m;
+}
+
+/* mesh::create_box (mesh/generation.h:104-134): the box's cells and
+ * geometry through create_mesh. */
+orc_mesh* orc_mesh_create_box(const double p0[3], const double p1[3],
+                              const int64_t n[3], int celltype)
+{
+  const int nvpc = celltype == ORC_TET ? 4 : 8;
+  const int64_t nv = orc_box_num_vertices(n);
+  const int64_t nc = orc_box_num_cells(n, celltype);
+  double* x_in = (double*)malloc(sizeof(double) * 3 * nv);
+  int64_t* cells_in = (int64_t*)malloc(sizeof(int64_t) * nvpc * nc);
+  orc_box_geometry(p0, p1, n, x_in);
+  orc_box_cells(n, celltype, cells_in);
+  orc_mesh* m = orc_mesh_create(celltype, cells_in, (int32_t)nc, x_in, (int32_t)nv);
+  free(cells_in);
   free(x_in);
   return m;
 }

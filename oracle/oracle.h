@@ -17,6 +17,9 @@
  *   orc_rcm              independent Python restatement of ordering.cpp on
  *                        box dual graphs and random (disconnected) graphs.
  *   orc_mesh_create_box  first-appearance vertex numbering invariant.
+ *   orc_mesh_create      the box through it equals orc_mesh_create_box; on
+ *                        other inputs it composes the pinned dual graph,
+ *                        RCM and numbering (invariants on a bicone fan).
  *   orc_entities         edges = lexicographic ranks of sorted vertex pairs;
  *                        Euler characteristic V - E + F - C = 1.
  *   orc_dofmap           independent Python restatement (P1, P2, P3); P3
@@ -77,6 +80,10 @@ void orc_rcm(const int32_t* offsets, const int32_t* data, int32_t n,
              int32_t* remap);
 orc_mesh* orc_mesh_create_box(const double p0[3], const double p1[3],
                               const int64_t n[3], int celltype);
+/* mesh::create_mesh on one rank from input cells (nc x nvpc input vertex
+ * indices) and geometry (nv x 3). */
+orc_mesh* orc_mesh_create(int celltype, const int64_t* cells_in, int32_t nc,
+                          const double* x_in, int32_t nv);
 void orc_mesh_free(orc_mesh* m);
 int orc_cell_num_entities(int celltype, int dim);
 /* Entities of dimension dim (1 or 2): c2e (num_cells x ne), e2v (n x nvpe).
