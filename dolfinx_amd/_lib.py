@@ -103,6 +103,7 @@ SIGNATURES = {
     "dfx_bc_create": (C.c_int, [P, I32, I32, P, PP]),
     "dfx_bc_destroy": (C.c_int, [P]),
     "dfx_assemble_matrix": (C.c_int, [P, P, P]),
+    "dfx_assemble_system": (C.c_int, [P, P, P, C.c_double, P, P, I32]),
     "dfx_assemble_matrix_elements": (C.c_int, [P, P, P, P]),
     "dfx_assemble_vector_elements": (C.c_int, [P, P, P]),
     "dfx_apply_lifting_elements": (C.c_int, [P, P, P, P, P, C.c_double]),

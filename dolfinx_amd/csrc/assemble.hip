@@ -3973,4 +3973,41 @@ dfx_status dfx_bc_set(dfx_vec* b, const dfx_bc* bc, const dfx_vec* x0, double al
       });
 }
 
+dfx_status dfx_assemble_system(dfx_csr* A, const dfx_form* a, const dfx_bc* bc, double diag,
+                               dfx_vec* b, const dfx_form* L, int32_t zero_b)
+{
+  return guard(
+      [&]
+      {
+        require(A && a && b && L, "null argument");
+        auto ok = [](dfx_status st)
+        {
+          if (st != DFX_OK)
+            throw Error(st, dfx_last_error());
+        };
+        Context& c = ctx();
+        ensure_init();
+        // the vector assembly reads only L's data and writes only b: it
+        // runs on the second stream while the matrix is assembled
+        DFX_HIP(hipEventRecord(c.fork, c.stream));
+        DFX_HIP(hipStreamWaitEvent(c.aux, c.fork, 0));
+        {
+          StreamScope on_aux(c.aux);
+          if (zero_b)
+            ok(dfx_vec_set(b, 0.0));
+          ok(dfx_assemble_vector(b, L));
+        }
+        DFX_HIP(hipEventRecord(c.join, c.aux));
+        ok(dfx_assemble_matrix(A, a, bc));
+        if (bc)
+          ok(dfx_set_diagonal(A, bc, diag));
+        DFX_HIP(hipStreamWaitEvent(c.stream, c.join, 0));
+        if (bc)
+        {
+          ok(dfx_apply_lifting(b, a, bc, nullptr, 1.0));
+          ok(dfx_bc_set(b, bc, nullptr, 1.0));
+        }
+      });
+}
+
 } // extern "C"

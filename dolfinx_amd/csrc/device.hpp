@@ -22,19 +22,33 @@ namespace dfx
                                           + hipGetErrorString(e_));            \
   } while (0)
 
-/// Library-wide device state: one stream at default priority.
+/// Library-wide device state: the library stream, a second stream for
+/// work that overlaps it inside one call (both at default priority), and
+/// the events that fork and join them.
 struct Context
 {
   bool ready = false;
   int device = 0;
   int num_cu = 256;
   hipStream_t stream = nullptr;
+  hipStream_t aux = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   hipEvent_t events[64] = {};
 };
 
 Context& ctx();
 void ensure_init();
 hipStream_t stream();
+
+/// Routes the library's launches and copies to stream s while it lives.
+struct StreamScope
+{
+  hipStream_t saved;
+  explicit StreamScope(hipStream_t s) : saved(ctx().stream) { ctx().stream = s; }
+  ~StreamScope() { ctx().stream = saved; }
+  StreamScope(const StreamScope&) = delete;
+  StreamScope& operator=(const StreamScope&) = delete;
+};
 void check_launch(const char* what);
 /// Workgroups of `kernel` (threads each, lds bytes of dynamic LDS) resident
 /// on the device at once: the grid of a persistent kernel, so that no

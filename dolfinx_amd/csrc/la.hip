@@ -40,6 +40,9 @@ void ensure_init()
   DFX_HIP(hipGetDeviceProperties(&prop, 0));
   g_ctx.num_cu = prop.multiProcessorCount;
   DFX_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+  DFX_HIP(hipStreamCreateWithFlags(&g_ctx.aux, hipStreamNonBlocking));
+  DFX_HIP(hipEventCreateWithFlags(&g_ctx.fork, hipEventDisableTiming));
+  DFX_HIP(hipEventCreateWithFlags(&g_ctx.join, hipEventDisableTiming));
   for (auto& e : g_ctx.events)
     DFX_HIP(hipEventCreate(&e));
   g_ctx.ready = true;

@@ -339,6 +339,27 @@ def apply_lifting_elements(b: Vector, a: Form, E, bcs, x0: Vector | None = None,
                   None if x0 is None else x0.handle, float(alpha))
 
 
+def assemble_system(a: Form, L: Form, bcs=(), diag=1.0, A: MatrixCSR | None = None,
+                    b: Vector | None = None):
+    """The demos' system sequence (cpp/demo/poisson/main.cpp:207-226):
+    A = assemble_matrix(a, bcs) with ``diag`` on BC rows (A zeroed first),
+    b = assemble_vector(L) from zero, apply_lifting(b, [a], [bcs]),
+    set_bc(b, bcs); the vector assembly overlaps the matrix assembly on a
+    second stream. Returns (A, b)."""
+    if A is None:
+        A = create_matrix(a)
+    else:
+        A.set_value(0.0)
+    if b is None:
+        b = Vector(L.V.num_dofs, L.dtype)
+    if len(bcs) > 1:
+        raise ValueError("pass the union of Dirichlet dofs as one DirichletBC")
+    bc = bcs[0] if bcs else None
+    _lib.call("dfx_assemble_system", A.handle, a.handle, None if bc is None else bc.handle,
+              float(diag), b.handle, L.handle, 1)
+    return A, b
+
+
 def assemble_matrix_batched(forms, bcs, matrices, diag=1.0) -> None:
     """assemble_matrix for many independent systems of one P1 or Q1
     stiffness kernel in two launches (cell factors of all systems, rows of

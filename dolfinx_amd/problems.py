@@ -74,19 +74,7 @@ def assemble_system(p: ScalarProblem, A: la.MatrixCSR | None = None, b: la.Vecto
     """The reference's Poisson pipeline: A = assemble_matrix(a, [bc]) (BC
     rows/cols zeroed, diagonal 1); b = assemble_vector(L); apply_lifting;
     set_bc."""
-    if A is None:
-        A = fem.create_matrix(p.a)
-    else:
-        A.set_value(0.0)
-    fem.assemble_matrix(p.a, [p.bc], 1.0, A)
-    if b is None:
-        b = la.Vector(p.V.num_dofs, p.dtype)
-    else:
-        b.set(0.0)
-    fem.assemble_vector(p.L, b)
-    fem.apply_lifting(b, [p.a], [[p.bc]])
-    fem.set_bc(b, [p.bc])
-    return A, b
+    return fem.assemble_system(p.a, p.L, [p.bc], 1.0, A, b)
 
 
 @dataclass

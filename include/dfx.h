@@ -279,6 +279,15 @@ dfx_status dfx_assemble_vector_elements(dfx_vec* b, const dfx_form* L, const dfx
  * column dof: b -= alpha E[:, bc] (g - x0). */
 dfx_status dfx_apply_lifting_elements(dfx_vec* b, const dfx_form* a, const dfx_bc* bc,
                                       const dfx_vec* E, const dfx_vec* x0, double alpha);
+/* The system sequence of the reference's demos (cpp/demo/poisson/
+ * main.cpp:207-226): assemble_matrix(A, a, {bc}) and set_diagonal(A, bc,
+ * diag); b = 0 (if zero_b) and assemble_vector(b, L); apply_lifting(b,
+ * {a}, {{bc}}); bc.set(b). The vector assembly runs on a second stream
+ * while the matrix is assembled (it reads L's data and writes b only);
+ * lifting and set_bc follow both. Results equal the separate calls. bc
+ * may be NULL. */
+dfx_status dfx_assemble_system(dfx_csr* A, const dfx_form* a, const dfx_bc* bc, double diag,
+                               dfx_vec* b, const dfx_form* L, int32_t zero_b);
 /* fem::set_diagonal (fem/assembler.h:644-686): A[d,d] = diag for BC dofs */
 dfx_status dfx_set_diagonal(dfx_csr* A, const dfx_bc* bc, double diag);
 /* fem::assemble_vector(b, L) (fem/assembler.h:230-257): b += L */
