@@ -112,6 +112,21 @@ inline std::shared_ptr<Mesh> create_box(std::array<std::array<double, 3>, 2> p,
   check(dfx_mesh_create_box(p[0].data(), p[1].data(), n.data(), static_cast<int>(celltype), &h));
   return std::make_shared<Mesh>(h, celltype);
 }
+
+/// mesh::create_mesh(comm, cells, element, x, ...) (mesh/utils.h:1336-1356)
+/// on one rank: cells as input vertex indices in DOLFINx vertex order
+/// (num_cells x 4 or 8), x as num_vertices x 3.
+inline std::shared_ptr<Mesh> create_mesh(CellType celltype, std::span<const std::int64_t> cells,
+                                         std::span<const double> x)
+{
+  const int npc = celltype == CellType::tetrahedron ? 4 : 8;
+  if (cells.size() % npc != 0 || x.size() % 3 != 0)
+    throw std::runtime_error("cells and x sizes do not match the cell type");
+  dfx_mesh* h = nullptr;
+  check(dfx_mesh_create(static_cast<int>(celltype), static_cast<std::int32_t>(cells.size() / npc),
+                        cells.data(), static_cast<std::int32_t>(x.size() / 3), x.data(), &h));
+  return std::make_shared<Mesh>(h, celltype);
+}
 } // namespace mesh
 
 namespace la
@@ -489,6 +504,49 @@ void apply_lifting(la::Vector<T>& b, const std::vector<std::reference_wrapper<co
     for (auto& bc : bcs1[i])
       check(dfx_apply_lifting(b.handle(), a[i].get().handle(), bc.get().handle(),
                               x0.empty() ? nullptr : x0[i].get().handle(), alpha));
+}
+/// fem::assemble_matrix / assemble_vector with a custom kernel
+/// (cpp/demo/custom_kernel/main.cpp:73-118): a Form of kernel
+/// DFX_K_ELEMENT_A / DFX_K_ELEMENT_L, the kernel's outputs E (cells x k x k)
+/// and e (cells x k) in device vectors of the form's scalar type.
+template <typename T>
+void assemble_matrix_elements(la::DeviceMatAdd<T> mat_add, const Form<T>& a,
+                              const la::Vector<T>& E, const BCs<T>& bcs = {})
+{
+  if (bcs.size() > 1)
+    throw std::runtime_error("pass the union of Dirichlet dofs as one DirichletBC");
+  check(dfx_assemble_matrix_elements(mat_add.A, a.handle(),
+                                     bcs.empty() ? nullptr : bcs[0].get().handle(), E.handle()));
+}
+
+template <typename T>
+void assemble_vector_elements(la::Vector<T>& b, const Form<T>& L, const la::Vector<T>& e)
+{
+  check(dfx_assemble_vector_elements(b.handle(), L.handle(), e.handle()));
+}
+
+/// apply_lifting for a Form with caller-supplied element matrices E
+template <typename T>
+void apply_lifting_elements(la::Vector<T>& b, const Form<T>& a, const la::Vector<T>& E,
+                            const BCs<T>& bcs, const la::Vector<T>* x0 = nullptr, T alpha = 1)
+{
+  for (auto& bc : bcs)
+    check(dfx_apply_lifting_elements(b.handle(), a.handle(), bc.get().handle(), E.handle(),
+                                     x0 ? x0->handle() : nullptr, alpha));
+}
+
+/// The demos' system sequence (cpp/demo/poisson/main.cpp:207-226) in one
+/// call: assemble_matrix + set_diagonal, b = 0 + assemble_vector,
+/// apply_lifting, DirichletBC::set; the vector assembly overlaps the
+/// matrix assembly on a second stream.
+template <typename T>
+void assemble_system(la::MatrixCSR<T>& A, const Form<T>& a, la::Vector<T>& b, const Form<T>& L,
+                     const BCs<T>& bcs = {}, T diagonal = 1)
+{
+  if (bcs.size() > 1)
+    throw std::runtime_error("pass the union of Dirichlet dofs as one DirichletBC");
+  check(dfx_assemble_system(A.handle(), a.handle(), bcs.empty() ? nullptr : bcs[0].get().handle(),
+                            diagonal, b.handle(), L.handle(), 1));
 }
 } // namespace fem
 
