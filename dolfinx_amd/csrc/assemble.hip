@@ -957,26 +957,27 @@ __device__ inline void row_block(std::int32_t r0, std::int32_t nrows, std::int32
     const std::int32_t pe = inc_off[r + 1];
     for (std::int32_t pb = inc_off[r]; pb < pe; pb += U)
     {
+      // loads at clamped positions, none behind a branch, so that all of a
+      // group's loads are in flight together; incidences past pe are
+      // skipped below
       std::int32_t cik[U];
 #pragma unroll
       for (int u = 0; u < U; ++u)
-        cik[u] = pb + u < pe ? inc[pb + u] : -1;
+        cik[u] = inc[min(pb + u, pe - 1)];
       T g[U][7];
       std::uint32_t m[U];
       O sl[U][K];
 #pragma unroll
       for (int u = 0; u < U; ++u)
       {
-        if (cik[u] < 0)
-          continue;
         const std::int64_t c = cik[u] / K;
         load_factors<T>(cd, c, g[u], m[u]);
-        load_slots<O, K>(islots, pb + u, sl[u]);
+        load_slots<O, K>(islots, min(pb + u, pe - 1), sl[u]);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
       {
-        if (cik[u] < 0)
+        if (pb + u >= pe)
           continue;
         const int i = cik[u] % K;
         if constexpr (CF)
@@ -2512,9 +2513,11 @@ int incidence_lanes(const dfx_form& f)
   return L;
 }
 
-/// Row-kernel variants, picked per form by dfx_form_autotune:
-/// 0..2: "rows" with 64/128/256 rows per workgroup; 3: "lanes".
-constexpr int kRowVariants = 11;
+/// Row-kernel variants, picked per form by dfx_form_autotune: 0..2 "rows"
+/// with 64/128/256 rows per workgroup; 3 "lanes"; 4..6 closed-form Q1 rows;
+/// 7, 8 incidence-parallel gathers; 9 structured Q1; 10 fused geometry;
+/// 11 as 1 with eight incidences in flight.
+constexpr int kRowVariants = 12;
 
 template <typename T, typename O, int K, int L>
 void launch_lanes(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
@@ -2885,10 +2888,10 @@ void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
   {
     // 0-2: four incidences per group, reference tensors in LDS;
     // 4-6: closed-form entries for Q1 hexahedra (four per group), else
-    // eight incidences per group
-    const bool wide = variant >= 4;
+    // eight incidences per group; 11: as 1 with eight per group
+    const bool wide = variant >= 4 && variant != 11;
     const bool cf = wide && K == 8 && f.d.kernel == DFX_K_Q1HEX_A;
-    const int R = 64 << (wide ? variant - 4 : variant);
+    const int R = variant == 11 ? 128 : 64 << (wide ? variant - 4 : variant);
     const std::size_t lds
         = ((cf ? 0 : 7 * K * K) + std::size_t(R) * A.max_row) * sizeof(T);
     require(lds <= 64 * 1024, "row too long for the LDS slice");
@@ -2903,7 +2906,9 @@ void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
     };
     if (nb > 0)
     {
-      if (!wide)
+      if (variant == 11)
+        go(row_assemble_k<T, O, K, 8, false>);
+      else if (!wide)
         go(row_assemble_k<T, O, K, 4, false>);
       else if constexpr (K == 8)
       {

@@ -226,7 +226,7 @@ class Form:
         closed-form Q1 entries, 7 element matrices on the matrix cores then
         gathered by rows, 8 one incidence per thread, 9 structured-box Q1
         rows in registers, 10 factors formed from the geometry per
-        incidence)."""
+        incidence, 11 as 1 with eight incidences in flight)."""
         _lib.call("dfx_form_set_variant", self._h, int(variant))
 
     @property

@@ -255,8 +255,8 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc,
  * structured box of hexahedra: one thread per row accumulating its 27
  * entries in registers (error if the mesh is not such a box); 10 as 4 with
  * the cell factors formed from the geometry by every incidence (no cell
- * pass; affine P1/Q1). The P2 elasticity form has a single row kernel and
- * ignores the variant. */
+ * pass; affine P1/Q1); 11 as 1 with eight incidences in flight. The P2
+ * elasticity form has a single row kernel and ignores the variant. */
 dfx_status dfx_form_set_variant(dfx_form* a, int32_t variant);
 /* fem::assemble_matrix / fem::assemble_vector with a custom kernel
  * (cpp/demo/custom_kernel/main.cpp:73-118): the form (kernel
