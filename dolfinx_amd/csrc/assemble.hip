@@ -1420,25 +1420,27 @@ __global__ void __launch_bounds__(kBlock)
   for (int t = threadIdx.x; t < len; t += kBlock)
     acc[t] = T(0);
   // rows from nplan on have no incidences (inc_off has nplan + 1 entries)
-  const std::int32_t p = inc_off[min(R0, nplan)] + threadIdx.x;
-  bool live = p < inc_off[min(R1, nplan)];
+  const std::int32_t pb = inc_off[min(R0, nplan)], pe = inc_off[min(R1, nplan)];
+  const std::int32_t p = pb + threadIdx.x;
+  bool live = p < pe;
   std::int32_t r = 0, rank = -1;
-  if (live)
-  {
-    r = inc_row[p];
-    rank = p - inc_off[r];
-    live = !(bc && bc[r]);
-  }
   T e[K];
   O sl[K];
   std::uint32_t m = 0;
-  __syncthreads();
-  if (live)
+  T g[7];
+  int i = 0;
+  if (pe > pb)
   {
-    const std::int32_t cik = inc[p];
+    // every load at a clamped position and none behind a branch: the
+    // incidence's row, cell, slots and element row (or cell factors) are
+    // in flight together
+    const std::int32_t pc = min(p, pe - 1);
+    r = inc_row[pc];
+    const std::int32_t cik = inc[pc];
+    load_slots<O, K>(islots, pc, sl);
     const std::int64_t c = cik / K;
+    i = cik - static_cast<int>(c) * K;
     m = mask ? mask[c] : 0u;
-    load_slots<O, K>(islots, p, sl);
     if constexpr (MAT)
     {
       const T* er = elem + std::int64_t(cik) * K;
@@ -1448,15 +1450,21 @@ __global__ void __launch_bounds__(kBlock)
     }
     else
     {
-      T g[7];
       std::uint32_t mf;
       load_factors<T>(cd, c, g, mf);
-      const int i = cik - static_cast<int>(c) * K;
+    }
+    rank = p - inc_off[r];
+    live = live && !(bc && bc[r]);
+  }
+  __syncthreads();
+  if constexpr (!MAT)
+    if (pe > pb)
+    {
+      // entries from the factors and the tensors staged in LDS above
 #pragma unroll
       for (int j = 0; j < K; ++j)
         e[j] = element_entry<T, K>(g, S, i, j);
     }
-  }
   T* a = acc + (live ? rp[r] - base : 0);
   const int rounds = gblk[ngblk + 1 + b];
   for (int k = 0; k < rounds; ++k)
@@ -3992,21 +4000,29 @@ dfx_status dfx_assemble_system(dfx_csr* A, const dfx_form* a, const dfx_bc* bc, 
         };
         Context& c = ctx();
         ensure_init();
-        // the vector assembly reads only L's data and writes only b: it
-        // runs on the second stream while the matrix is assembled
-        DFX_HIP(hipEventRecord(c.fork, c.stream));
-        DFX_HIP(hipStreamWaitEvent(c.aux, c.fork, 0));
+        // the vector assembly reads only L's data and writes only b: on
+        // large meshes it runs on the second stream while the matrix is
+        // assembled (below ~1M cells the fork and join cost more than the
+        // overlap gains)
+        const bool fork = L->d.num_cells >= (1 << 20);
+        if (fork)
         {
-          StreamScope on_aux(c.aux);
+          DFX_HIP(hipEventRecord(c.fork, c.stream));
+          DFX_HIP(hipStreamWaitEvent(c.aux, c.fork, 0));
+        }
+        {
+          StreamScope on(fork ? c.aux : c.stream);
           if (zero_b)
             ok(dfx_vec_set(b, 0.0));
           ok(dfx_assemble_vector(b, L));
         }
-        DFX_HIP(hipEventRecord(c.join, c.aux));
+        if (fork)
+          DFX_HIP(hipEventRecord(c.join, c.aux));
         ok(dfx_assemble_matrix(A, a, bc));
         if (bc)
           ok(dfx_set_diagonal(A, bc, diag));
-        DFX_HIP(hipStreamWaitEvent(c.stream, c.join, 0));
+        if (fork)
+          DFX_HIP(hipStreamWaitEvent(c.stream, c.join, 0));
         if (bc)
         {
           ok(dfx_apply_lifting(b, a, bc, nullptr, 1.0));

@@ -282,10 +282,10 @@ dfx_status dfx_apply_lifting_elements(dfx_vec* b, const dfx_form* a, const dfx_b
 /* The system sequence of the reference's demos (cpp/demo/poisson/
  * main.cpp:207-226): assemble_matrix(A, a, {bc}) and set_diagonal(A, bc,
  * diag); b = 0 (if zero_b) and assemble_vector(b, L); apply_lifting(b,
- * {a}, {{bc}}); bc.set(b). The vector assembly runs on a second stream
- * while the matrix is assembled (it reads L's data and writes b only);
- * lifting and set_bc follow both. Results equal the separate calls. bc
- * may be NULL. */
+ * {a}, {{bc}}); bc.set(b). On meshes of 2^20 cells or more the vector
+ * assembly runs on a second stream while the matrix is assembled (it reads
+ * L's data and writes b only); lifting and set_bc follow both. Results
+ * equal the separate calls. bc may be NULL. */
 dfx_status dfx_assemble_system(dfx_csr* A, const dfx_form* a, const dfx_bc* bc, double diag,
                                dfx_vec* b, const dfx_form* L, int32_t zero_b);
 /* fem::set_diagonal (fem/assembler.h:644-686): A[d,d] = diag for BC dofs */
