@@ -18,7 +18,7 @@ DFX_OK, DFX_ERR_ARG, DFX_ERR_PATTERN, DFX_ERR_HIP, DFX_ERR_NOCONV, DFX_ERR_NODEV
 F32, F64 = 0, 1
 TET, HEX = 0, 1
 (K_P1TET_A, K_P1TET_L, K_Q1HEX_A, K_Q1HEX_L, K_P2TET_ELAST_A, K_P2TET_ELAST_L,
- K_P3TET_A, K_P3TET_L, K_ELEMENT_A, K_ELEMENT_L) = range(10)
+ K_P3TET_A, K_P3TET_L, K_ELEMENT_A, K_ELEMENT_L, K_P1TET_NEUMANN, K_Q1HEX_NEUMANN) = range(12)
 
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
@@ -62,6 +62,8 @@ SIGNATURES = {
     "dfx_stream": (P, []),
     "dfx_mesh_create_box": (C.c_int, [P, P, P, C.c_int, PP]),
     "dfx_mesh_create": (C.c_int, [C.c_int, I32, P, I32, P, PP]),
+    "dfx_facet_domain": (C.c_int, [P, C.c_int, C.c_int, C.c_double, P, P]),
+    "dfx_form_create_facets": (C.c_int, [P, I32, P, PP]),
     "dfx_mesh_destroy": (C.c_int, [P]),
     "dfx_mesh_sizes": (C.c_int, [P, P, P, P]),
     "dfx_mesh_copy": (C.c_int, [P, P, P, P, P]),

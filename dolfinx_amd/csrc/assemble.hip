@@ -72,6 +72,12 @@ bool kernel_info(int id, KernelInfo& ki)
   case DFX_K_P2TET_ELAST_L:
     ki = {DFX_TET, 4, 10, 3, false};
     return true;
+  case DFX_K_P1TET_NEUMANN:
+    ki = {DFX_TET, 4, 4, 1, false};
+    return true;
+  case DFX_K_Q1HEX_NEUMANN:
+    ki = {DFX_HEX, 8, 8, 1, false};
+    return true;
   case DFX_K_ELEMENT_A:
     ki = {-1, 0, 0, 1, true}; // any element: k from the descriptor
     return true;
@@ -1559,6 +1565,103 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---------------------------------------------------------------------
+// Exterior facet loads g v ds with g given at the cell's vertices (P1 on a
+// tetrahedron's facet triangle, Q1 on a hexahedron's facet
+// parallelogram). Facet vertices in DOLFINx sub-entity order.
+// ---------------------------------------------------------------------
+
+__constant__ int kTetFacetV[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+__constant__ int kHexFacetV[6][4] = {{0, 1, 2, 3}, {0, 1, 4, 5}, {0, 2, 4, 6},
+                                     {1, 3, 5, 7}, {2, 3, 6, 7}, {4, 5, 6, 7}};
+
+/// One thread per (cell, local facet) pair e: the facet's contribution to
+/// each of its vertices, out[4 e + a] = int_F g phi_a ds: the facet mass
+/// matrix times g, |F| (1 + d_ab) / 12 on triangles and |F| M_ab / 36 with
+/// M the tensor product of [2 1; 1 2] on parallelograms.
+template <typename T, bool HEX>
+__global__ void __launch_bounds__(kBlock)
+    facet_load_k(std::int32_t npairs, const std::int32_t* __restrict__ pairs,
+                 const T* __restrict__ x, const std::int32_t* __restrict__ xdm, int npc,
+                 const T* __restrict__ coeffs, int cstride, T* __restrict__ out)
+{
+  const std::int32_t e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= npairs)
+    return;
+  constexpr int NF = HEX ? 4 : 3;
+  const std::int32_t c = pairs[2 * e];
+  const int lf = pairs[2 * e + 1];
+  T X[NF][3], g[NF];
+#pragma unroll
+  for (int a = 0; a < NF; ++a)
+  {
+    const int lv = HEX ? kHexFacetV[lf][a] : kTetFacetV[lf][a];
+    const std::int64_t v = xdm[std::int64_t(c) * npc + lv];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      X[a][d] = x[3 * v + d];
+    g[a] = coeffs[std::int64_t(c) * cstride + lv];
+  }
+  T t0[3], t1[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    t0[d] = X[1][d] - X[0][d];
+    t1[d] = X[2][d] - X[0][d];
+  }
+  const T n0 = t0[1] * t1[2] - t0[2] * t1[1];
+  const T n1 = t0[2] * t1[0] - t0[0] * t1[2];
+  const T n2 = t0[0] * t1[1] - t0[1] * t1[0];
+  const T nrm = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+  if constexpr (HEX)
+  {
+    constexpr T M[4][4] = {{4, 2, 2, 1}, {2, 4, 1, 2}, {2, 1, 4, 2}, {1, 2, 2, 4}};
+    const T w = nrm / T(36);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+    {
+      T s = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        s += w * M[a][b] * g[b];
+      out[4 * std::int64_t(e) + a] = s;
+    }
+  }
+  else
+  {
+    const T area = T(0.5) * nrm;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+      T s = 0;
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        s += area * (a == b ? T(2) : T(1)) / T(12) * g[b];
+      out[4 * std::int64_t(e) + a] = s;
+    }
+    out[4 * std::int64_t(e) + 3] = T(0);
+  }
+}
+
+/// b[r] += the facet contributions at row r in facet order (one thread per
+/// row over its facet incidences finc = 4 e + facet vertex).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    facet_rows_k(std::int32_t nplan, const std::int32_t* __restrict__ off,
+                 const std::int32_t* __restrict__ finc, const T* __restrict__ fv,
+                 T* __restrict__ b)
+{
+  const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nplan)
+    return;
+  const std::int32_t p1 = off[r + 1];
+  T s = 0;
+  for (std::int32_t p = off[r]; p < p1; ++p)
+    s += fv[finc[p]];
+  if (p1 > off[r])
+    b[r] += s;
+}
+
+// ---------------------------------------------------------------------
 // Structured Q1 rows. On a box of hexahedra with lexicographic vertex order
 // a row vertex is corner I of at most one cell for each I = 0..7, and
 // column j of that cell is the neighbour at offset bit(j) - bit(I) in each
@@ -3003,6 +3106,27 @@ void lift_rows(dfx_form& f, const dfx_bc& bc, const T* x0, double alpha, T* b)
   check_launch("row_lift");
 }
 
+/// Exterior facet loads: the facets' vertex contributions, then one
+/// thread per row adding its facet incidences.
+template <typename T>
+void facet_rows(const dfx_form& f, T* b)
+{
+  if (f.nfacet == 0)
+    return;
+  T* fv = static_cast<T*>(f.fvals);
+  const auto* x = static_cast<const T*>(f.x);
+  const auto* co = static_cast<const T*>(f.coeffs);
+  if (f.d.kernel == DFX_K_Q1HEX_NEUMANN)
+    facet_load_k<T, true><<<blocks_for(f.nfacet), kBlock, 0, stream()>>>(
+        f.nfacet, f.facets, x, f.x_dofmap, f.d.nodes_per_cell, co, f.d.cstride, fv);
+  else
+    facet_load_k<T, false><<<blocks_for(f.nfacet), kBlock, 0, stream()>>>(
+        f.nfacet, f.facets, x, f.x_dofmap, f.d.nodes_per_cell, co, f.d.cstride, fv);
+  const std::int32_t np = f.max_dof + 1;
+  facet_rows_k<T><<<blocks_for(np), kBlock, 0, stream()>>>(np, f.finc_off, f.finc, fv, b);
+  check_launch("facet loads");
+}
+
 template <typename T>
 void vector_rows(const dfx_form& f, T* b)
 {
@@ -3351,6 +3475,10 @@ dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out)
           need = 8;
         else if (d.kernel == DFX_K_P2TET_ELAST_A)
           need = 2;
+        else if (d.kernel == DFX_K_P1TET_NEUMANN)
+          need = 4;
+        else if (d.kernel == DFX_K_Q1HEX_NEUMANN)
+          need = 8;
         require(d.cstride >= need, "too few packed coefficients for this kernel");
         const std::int64_t nc = d.num_cells;
         for (std::int64_t i = 0; i < nc * d.nodes_per_cell; ++i)
@@ -3414,6 +3542,62 @@ dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out)
       });
 }
 
+dfx_status dfx_form_create_facets(const dfx_form_desc* desc, int32_t npairs,
+                                  const int32_t* pairs, dfx_form** out)
+{
+  return guard(
+      [&]
+      {
+        require(desc && out && (npairs == 0 || pairs), "null argument");
+        require(desc->kernel == DFX_K_P1TET_NEUMANN || desc->kernel == DFX_K_Q1HEX_NEUMANN,
+                "the kernel has no exterior facet integral");
+        require(npairs >= 0, "negative size");
+        const bool hex = desc->kernel == DFX_K_Q1HEX_NEUMANN;
+        const int nfc = hex ? 6 : 4, nfv = hex ? 4 : 3, k = desc->dofs_per_cell;
+        static constexpr int tetv[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+        static constexpr int hexv[6][4] = {{0, 1, 2, 3}, {0, 1, 4, 5}, {0, 2, 4, 6},
+                                           {1, 3, 5, 7}, {2, 3, 6, 7}, {4, 5, 6, 7}};
+        for (std::int32_t e = 0; e < npairs; ++e)
+          require(pairs[2 * e] >= 0 && pairs[2 * e] < desc->num_cells && pairs[2 * e + 1] >= 0
+                      && pairs[2 * e + 1] < nfc,
+                  "facet pair out of range");
+        dfx_form* raw = nullptr;
+        const dfx_status st = dfx_form_create(desc, &raw);
+        if (st != DFX_OK)
+          throw Error(st, dfx_last_error());
+        std::unique_ptr<dfx_form, dfx_status (*)(dfx_form*)> f(raw, dfx_form_destroy);
+        if (hex)
+          require(f->affine, "facet loads need affine hexahedra");
+        // facet incidences of every row in facet order (counting sort)
+        const std::int32_t nplan = f->max_dof + 1;
+        std::vector<std::int32_t> off(std::size_t(nplan) + 1, 0);
+        auto dof_of = [&](std::int32_t e, int a)
+        {
+          const int lv = hex ? hexv[pairs[2 * e + 1]][a] : tetv[pairs[2 * e + 1]][a];
+          return desc->dofmap[std::int64_t(pairs[2 * e]) * k + lv];
+        };
+        for (std::int32_t e = 0; e < npairs; ++e)
+          for (int a = 0; a < nfv; ++a)
+            ++off[dof_of(e, a) + 1];
+        for (std::int32_t r = 0; r < nplan; ++r)
+          off[r + 1] += off[r];
+        std::vector<std::int32_t> finc(std::max<std::int64_t>(std::int64_t(npairs) * nfv, 1));
+        std::vector<std::int32_t> pos(off.begin(), off.end() - 1);
+        for (std::int32_t e = 0; e < npairs; ++e)
+          for (int a = 0; a < nfv; ++a)
+            finc[pos[dof_of(e, a)]++] = 4 * e + a;
+        f->nfacet = npairs;
+        f->facets = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * std::max(2 * npairs, 1)));
+        h2d(f->facets, pairs, sizeof(std::int32_t) * 2 * npairs);
+        f->finc_off = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * off.size()));
+        h2d(f->finc_off, off.data(), sizeof(std::int32_t) * off.size());
+        f->finc = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * finc.size()));
+        h2d(f->finc, finc.data(), sizeof(std::int32_t) * finc.size());
+        f->fvals = dmalloc(dtype_size(desc->dtype) * 4 * std::max(npairs, 1));
+        *out = f.release();
+      });
+}
+
 dfx_status dfx_form_destroy(dfx_form* f)
 {
   if (f)
@@ -3434,6 +3618,10 @@ dfx_status dfx_form_destroy(dfx_form* f)
     dfree(f->inc_row);
     dfree(f->gblk);
     dfree(f->team_grp);
+    dfree(f->facets);
+    dfree(f->finc_off);
+    dfree(f->finc);
+    dfree(f->fvals);
     dfree(f->cell_elem);
     dfree(f->corner_cell);
     dfree(f->slotmap);
@@ -3801,6 +3989,14 @@ dfx_status dfx_assemble_vector(dfx_vec* b, const dfx_form* L)
                 "element-tensor forms are assembled with dfx_assemble_vector_elements");
         require(b->dtype == L->d.dtype, "vector and form dtypes differ");
         require(std::int64_t(L->max_dof + 1) * L->d.bs <= b->n, "dofmap exceeds the vector size");
+        if (L->d.kernel == DFX_K_P1TET_NEUMANN || L->d.kernel == DFX_K_Q1HEX_NEUMANN)
+        {
+          if (b->dtype == DFX_F64)
+            facet_rows<double>(*L, static_cast<double*>(b->data));
+          else
+            facet_rows<float>(*L, static_cast<float*>(b->data));
+          return;
+        }
         if (L->d.kernel == DFX_K_P1TET_L || L->d.kernel == DFX_K_Q1HEX_L
             || L->d.kernel == DFX_K_P2TET_ELAST_L || L->d.kernel == DFX_K_P3TET_L)
         {

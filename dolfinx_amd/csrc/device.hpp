@@ -283,6 +283,14 @@ struct dfx_form
   std::int32_t team_rows_at = 0;
   std::int64_t team_cap = 0;
   std::uint64_t team_pattern = 0;
+  // Exterior facet integrals: (cell, local facet) pairs, per row the
+  // facet incidences (4 e + facet vertex, entity order) and each facet's
+  // vertex contributions (nfacet x 4 of dtype).
+  std::int32_t nfacet = 0;
+  std::int32_t* facets = nullptr;
+  std::int32_t* finc_off = nullptr;
+  std::int32_t* finc = nullptr;
+  void* fvals = nullptr;
   void* cell_elem = nullptr;
   int row_variant = 1;             // row-kernel variant (dfx_form_autotune)
 };

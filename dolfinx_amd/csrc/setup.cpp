@@ -714,6 +714,53 @@ dfx_status dfx_locate_boundary_dofs(const dfx_space* V, int marker, int axis,
       });
 }
 
+dfx_status dfx_facet_domain(const dfx_mesh* mesh, int marker, int axis, double value,
+                            int32_t* count, int32_t* pairs)
+{
+  return dfx::guard(
+      [&]
+      {
+        dfx::require(mesh && count, "null argument");
+        dfx::require(marker == 0 || (marker == 1 && axis >= 0 && axis < 3), "bad marker");
+        dfx_mesh& m = const_cast<dfx_mesh&>(*mesh);
+        compute_entities(m, 2);
+        const int nf = num_entities_per_cell(m.cell, 2);
+        const int fv = entity_vertex_count(m.cell, 2);
+        // cells per facet and the first (cell, local facet) of each
+        std::vector<std::int32_t> ncell(m.num_entities[2], 0), first(m.num_entities[2], 0);
+        for (std::int64_t i = 0; i < std::int64_t(m.num_cells) * nf; ++i)
+          if (ncell[m.c2e[2][i]]++ == 0)
+            first[m.c2e[2][i]] = static_cast<std::int32_t>(i);
+        std::int32_t n = 0;
+        for (std::int32_t f = 0; f < m.num_entities[2]; ++f)
+        {
+          if (ncell[f] != 1)
+            continue; // interior facet
+          const std::int32_t c = first[f] / nf, lf = first[f] % nf;
+          if (marker == 1)
+          {
+            bool all = true;
+            const int* lv = entity_vertices(m.cell, 2, lf);
+            for (int j = 0; j < fv; ++j)
+            {
+              const std::int32_t v = m.cells[std::int64_t(c) * m.nvpc + lv[j]];
+              all = all && std::abs(m.x[3 * std::int64_t(v) + axis] - value) <= 1e-12;
+            }
+            if (!all)
+              continue;
+          }
+          if (pairs)
+          {
+            dfx::require(n < *count, "pairs buffer too small");
+            pairs[2 * n] = c;
+            pairs[2 * n + 1] = lf;
+          }
+          ++n;
+        }
+        *count = n;
+      });
+}
+
 dfx_status dfx_sparsity_create(const dfx_space* V0, const dfx_space* V1,
                                dfx_sparsity** out)
 {

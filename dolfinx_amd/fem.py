@@ -129,6 +129,8 @@ KERNELS = {
     "p2_elasticity": (_lib.K_P2TET_ELAST_A, CellType.tetrahedron, 2, 3, 2),
     "p2_body_force": (_lib.K_P2TET_ELAST_L, CellType.tetrahedron, 2, 3, 1),
     "p3_stiffness_reaction": (_lib.K_P3TET_A, CellType.tetrahedron, 3, 1, 2),
+    "p1_neumann": (_lib.K_P1TET_NEUMANN, CellType.tetrahedron, 1, 1, 1),
+    "q1_neumann": (_lib.K_Q1HEX_NEUMANN, CellType.hexahedron, 1, 1, 1),
     "p3_source": (_lib.K_P3TET_L, CellType.tetrahedron, 3, 1, 1),
 }
 
@@ -153,7 +155,8 @@ class Form:
     """fem::Form holding device copies of geometry, dofmap and packed
     coefficients (fem/Form.h:116-)."""
 
-    def __init__(self, name, V: FunctionSpace, coeffs=None, constants=None, dtype=np.float64):
+    def __init__(self, name, V: FunctionSpace, coeffs=None, constants=None, dtype=np.float64,
+                 facets=None):
         if name in ELEMENT_FORMS:
             self._init_elements(name, V, dtype)
             return
@@ -183,7 +186,13 @@ class Form:
         d.num_constants = len(self.constants)
         d.constants = _lib.ptr(self.constants)
         self._h = C.c_void_p()
-        _lib.call("dfx_form_create", C.byref(d), C.byref(self._h))
+        if facets is not None:
+            pr = np.ascontiguousarray(facets, np.int32).reshape(-1, 2)
+            self.facets = pr
+            _lib.call("dfx_form_create_facets", C.byref(d), pr.shape[0], _lib.ptr(pr),
+                      C.byref(self._h))
+        else:
+            _lib.call("dfx_form_create", C.byref(d), C.byref(self._h))
 
     def _init_elements(self, name, V, dtype):
         kid, rank = ELEMENT_FORMS[name]
@@ -242,8 +251,24 @@ class Form:
                 pass
 
 
-def form(name, V, coeffs=None, constants=None, dtype=np.float64) -> Form:
-    return Form(name, V, coeffs, constants, dtype)
+def form(name, V, coeffs=None, constants=None, dtype=np.float64, facets=None) -> Form:
+    """A form of a named kernel; exterior facet kernels ("p1_neumann",
+    "q1_neumann") take their integration entities as (cell, local facet)
+    pairs (facet_domain)."""
+    return Form(name, V, coeffs, constants, dtype, facets)
+
+
+def facet_domain(mesh: Mesh, axis=None, value=0.0) -> np.ndarray:
+    """fem::compute_integration_domains(exterior_facet) on the exterior
+    facets, or on those with every vertex at x[axis] == value: (cell, local
+    facet) pairs in facet-index order (fem/utils.cpp:154-273)."""
+    marker, ax = (0, 0) if axis is None else (1, int(axis))
+    n = C.c_int32()
+    _lib.call("dfx_facet_domain", mesh.handle, marker, ax, float(value), C.byref(n), None)
+    out = np.zeros((n.value, 2), np.int32)
+    _lib.call("dfx_facet_domain", mesh.handle, marker, ax, float(value), C.byref(n),
+              _lib.ptr(out))
+    return out
 
 
 # Forms whose element tensors the caller supplies (the reference's custom

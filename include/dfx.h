@@ -53,7 +53,9 @@ enum
   DFX_K_P3TET_A = 6,       /* kappa grad u.grad v + c u v; w=[kappa] c=[c] */
   DFX_K_P3TET_L = 7,       /* f v; c = [f] */
   DFX_K_ELEMENT_A = 8,     /* element matrices supplied by the caller */
-  DFX_K_ELEMENT_L = 9      /* element vectors supplied by the caller */
+  DFX_K_ELEMENT_L = 9,     /* element vectors supplied by the caller */
+  DFX_K_P1TET_NEUMANN = 10,/* g v ds on exterior facets; w = [g at the 4 vertices] */
+  DFX_K_Q1HEX_NEUMANN = 11 /* g v ds on exterior facets; w = [g at the 8 vertices] */
 };
 
 const char* dfx_last_error(void);
@@ -94,6 +96,15 @@ dfx_status dfx_mesh_sizes(const dfx_mesh* m, int32_t* num_cells,
 dfx_status dfx_mesh_copy(const dfx_mesh* m, double* x, int32_t* cells,
                          int64_t* original_cell_index,
                          int64_t* input_vertex_index);
+
+/* fem::compute_integration_domains(IntegralType::exterior_facet, topology,
+ * facets) (fem/utils.cpp:154-273) on mesh::exterior_facet_indices
+ * (mesh/utils.h:180-194): the (cell, local facet) pair of every exterior
+ * facet in facet-index order, or (marker 1) of those whose vertices all
+ * lie at x[axis] == value. With pairs NULL only *count is set; otherwise
+ * *count is the capacity in pairs on entry. */
+dfx_status dfx_facet_domain(const dfx_mesh* m, int marker, int axis, double value,
+                            int32_t* count, int32_t* pairs);
 
 /* fem::create_functionspace -> fem::create_dofmap (fem/utils.cpp:32-76)
  * for Lagrange degree 1-3 (tetrahedra) or 1 (hexahedra), block size bs. */
@@ -209,6 +220,13 @@ typedef struct
 
 /* fem::Form (fem/Form.h:116-) holding device copies of the form data. */
 dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out);
+/* A form with an exterior facet integral (kernel DFX_K_*_NEUMANN): its
+ * integration entities are the npairs (cell, local facet) pairs
+ * (fem/Form.h integral_data entities, e.g. from dfx_facet_domain);
+ * assemble_vector adds the facets in this order
+ * (fem/assemble_vector_impl.h:118-174). */
+dfx_status dfx_form_create_facets(const dfx_form_desc* desc, int32_t npairs,
+                                  const int32_t* pairs, dfx_form** out);
 dfx_status dfx_form_destroy(dfx_form* f);
 /* Replace packed coefficients / constants (same sizes as at creation). */
 dfx_status dfx_form_set_coefficients(dfx_form* f, const double* coeffs,

@@ -18,6 +18,7 @@
 
 #include "../dfx.h"
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -111,6 +112,20 @@ inline std::shared_ptr<Mesh> create_box(std::array<std::array<double, 3>, 2> p,
   dfx_mesh* h = nullptr;
   check(dfx_mesh_create_box(p[0].data(), p[1].data(), n.data(), static_cast<int>(celltype), &h));
   return std::make_shared<Mesh>(h, celltype);
+}
+
+/// fem::compute_integration_domains(IntegralType::exterior_facet, ...)
+/// (fem/utils.cpp:154-273) on the exterior facets, or (axis >= 0) on those
+/// with every vertex at x[axis] == value: flattened (cell, local facet)
+/// pairs in facet-index order.
+inline std::vector<std::int32_t> facet_domain(const Mesh& m, int axis = -1, double value = 0.0)
+{
+  std::int32_t n = 0;
+  const int marker = axis >= 0 ? 1 : 0;
+  check(dfx_facet_domain(m.handle(), marker, std::max(axis, 0), value, &n, nullptr));
+  std::vector<std::int32_t> pairs(2 * std::size_t(n));
+  check(dfx_facet_domain(m.handle(), marker, std::max(axis, 0), value, &n, pairs.data()));
+  return pairs;
 }
 
 /// mesh::create_mesh(comm, cells, element, x, ...) (mesh/utils.h:1336-1356)
@@ -395,8 +410,11 @@ template <typename T>
 class Form
 {
 public:
+  /// With facets (flattened (cell, local facet) pairs) the form's integral
+  /// is an exterior facet integral over them (DFX_K_*_NEUMANN kernels).
   Form(int kernel, std::shared_ptr<const FunctionSpace> V, std::vector<double> coeffs,
-       int cstride, std::vector<double> constants)
+       int cstride, std::vector<double> constants,
+       std::span<const std::int32_t> facets = {}, bool facet_integral = false)
       : _V(std::move(V))
   {
     auto m = _V->mesh();
@@ -419,7 +437,11 @@ public:
     d.num_constants = static_cast<std::int32_t>(constants.size());
     d.constants = constants.empty() ? nullptr : constants.data();
     dfx_form* h = nullptr;
-    check(dfx_form_create(&d, &h));
+    if (facet_integral || !facets.empty())
+      check(dfx_form_create_facets(&d, static_cast<std::int32_t>(facets.size() / 2),
+                                   facets.data(), &h));
+    else
+      check(dfx_form_create(&d, &h));
     _h.reset(h, dfx_form_destroy);
     _rank = (kernel % 2 == 0) ? 2 : 1;
   }

@@ -22,6 +22,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 TET, HEX = 0, 1
 K_P1TET_A, K_P1TET_L, K_Q1HEX_A, K_Q1HEX_L = 0, 1, 2, 3
 K_P2TET_ELAST_A, K_P2TET_ELAST_L, K_P3TET_A, K_P3TET_L = 4, 5, 6, 7
+K_P1TET_NEUMANN, K_Q1HEX_NEUMANN = 10, 11
 
 _lib = None
 
@@ -83,6 +84,10 @@ def _declare(L):
                                       C.c_int, P, P, P, P, P, P]
     L.orc_assemble_vector.argtypes = [C.c_int, P, P, C.c_int, I32, P, C.c_int, C.c_int, P,
                                       C.c_int, P, P]
+    L.orc_facet_domain.restype = I32
+    L.orc_facet_domain.argtypes = [C.POINTER(_Mesh), C.c_int, C.c_int, D, C.POINTER(P)]
+    L.orc_assemble_vector_facets.argtypes = [C.c_int, P, P, C.c_int, P, C.c_int, C.c_int, P,
+                                             C.c_int, P, P, I32, P]
     L.orc_add_element_matrices.restype = C.c_int
     L.orc_add_element_matrices.argtypes = [I32, P, C.c_int, C.c_int, P, P, P, P, P, P]
     L.orc_add_element_vectors.argtypes = [I32, P, C.c_int, C.c_int, P, P]
@@ -179,6 +184,14 @@ class Mesh:
         n = lib().orc_exterior_facets(self._ptr, C.byref(out))
         return _take(out.value, 2 * n, np.int32).reshape(n, 2)
 
+    def facet_domain(self, axis=None, value=0.0):
+        """(cell, local facet) of the exterior facets in facet-index order,
+        or of those with every vertex at x[axis] == value."""
+        out = P()
+        n = lib().orc_facet_domain(self._ptr, 0 if axis is None else 1,
+                                   0 if axis is None else int(axis), float(value), C.byref(out))
+        return _take(out.value, 2 * n, np.int32).reshape(n, 2)
+
     def dofmap(self, degree):
         k = lib().orc_element_ndofs(self.celltype, degree)
         dm = np.zeros((self.num_cells, k), np.int32)
@@ -257,6 +270,20 @@ def assemble_vector(kernel, x, x_dofmap, dofmap, bs, coeffs, constants, n):
     b = np.zeros(n, np.float64)
     lib().orc_assemble_vector(kernel, _p(x), _p(xd), xd.shape[1], xd.shape[0], _p(dm),
                               dm.shape[1], bs, _p(co), cs, _p(kc), _p(b))
+    return b
+
+
+def assemble_vector_facets(kernel, x, x_dofmap, dofmap, coeffs, pairs, n, constants=None):
+    """b from an exterior-facet kernel over (cell, local facet) pairs."""
+    x = np.ascontiguousarray(x, np.float64)
+    xd = np.ascontiguousarray(x_dofmap, np.int32)
+    dm = np.ascontiguousarray(dofmap, np.int32)
+    co = np.ascontiguousarray(coeffs, np.float64)
+    kc = None if constants is None else np.ascontiguousarray(constants, np.float64)
+    pr = np.ascontiguousarray(pairs, np.int32)
+    b = np.zeros(n, np.float64)
+    lib().orc_assemble_vector_facets(kernel, _p(x), _p(xd), xd.shape[1], _p(dm), dm.shape[1], 1,
+                                     _p(co), co.shape[1], _p(kc), _p(pr), pr.shape[0], _p(b))
     return b
 
 

@@ -378,6 +378,79 @@ static void k_q1hex_l(double* A, const double* w, const double* c,
   }
 }
 
+/* ---- exterior facet loads, g v ds with g at the cell's vertices ---- */
+/* Facet vertices of the reference cells in DOLFINx sub-entity order
+ * (cell_types.cpp facet tables; quadrilateral facets in tensor order). */
+static const int TET_FV[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+static const int HEX_FV[6][4] = {{0, 1, 2, 3}, {0, 1, 4, 5}, {0, 2, 4, 6},
+                                 {1, 3, 5, 7}, {2, 3, 6, 7}, {4, 5, 6, 7}};
+
+static void cross3(const double a[3], const double b[3], double n[3])
+{
+  n[0] = a[1] * b[2] - a[2] * b[1];
+  n[1] = a[2] * b[0] - a[0] * b[2];
+  n[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* P1 tet, facet e[0]: b_i = sum_j |F| (1 + d_ij) / 12 g_j over the facet's
+ * vertices (the P1 mass matrix of the facet triangle). */
+static void k_p1tet_neumann(double* A, const double* w, const double* c,
+                            const double* cd, const int* e, const uint8_t* q,
+                            void* u)
+{
+  (void)c;
+  (void)q;
+  (void)u;
+  const int* fv = TET_FV[e[0]];
+  double t0[3], t1[3], n[3];
+  for (int d = 0; d < 3; ++d)
+  {
+    t0[d] = cd[3 * fv[1] + d] - cd[3 * fv[0] + d];
+    t1[d] = cd[3 * fv[2] + d] - cd[3 * fv[0] + d];
+  }
+  cross3(t0, t1, n);
+  const double area = 0.5 * sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      A[fv[a]] += area * (a == b ? 2.0 : 1.0) / 12.0 * w[fv[b]];
+}
+
+/* Q1 hex, facet e[0]: 2 x 2 Gauss-Legendre on the bilinear facet map
+ * through its four vertices (exact for g v on parallelogram facets). */
+static void k_q1hex_neumann(double* A, const double* w, const double* c,
+                            const double* cd, const int* e, const uint8_t* qq,
+                            void* u)
+{
+  (void)c;
+  (void)qq;
+  (void)u;
+  const int* fv = HEX_FV[e[0]];
+  const double g0 = 0.5 - 0.5 / sqrt(3.0), g1 = 0.5 + 0.5 / sqrt(3.0);
+  const double pts[2] = {g0, g1};
+  for (int qs = 0; qs < 2; ++qs)
+    for (int qt = 0; qt < 2; ++qt)
+    {
+      const double s = pts[qs], t = pts[qt];
+      const double phi[4] = {(1 - s) * (1 - t), s * (1 - t), (1 - s) * t, s * t};
+      const double ds[4] = {-(1 - t), 1 - t, -t, t};
+      const double dt[4] = {-(1 - s), -s, 1 - s, s};
+      double xs[3] = {0, 0, 0}, xt[3] = {0, 0, 0}, n[3];
+      for (int a = 0; a < 4; ++a)
+        for (int d = 0; d < 3; ++d)
+        {
+          xs[d] += ds[a] * cd[3 * fv[a] + d];
+          xt[d] += dt[a] * cd[3 * fv[a] + d];
+        }
+      cross3(xs, xt, n);
+      const double dA = 0.25 * sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      double g = 0;
+      for (int a = 0; a < 4; ++a)
+        g += phi[a] * w[fv[a]];
+      for (int a = 0; a < 4; ++a)
+        A[fv[a]] += dA * g * phi[a];
+    }
+}
+
 /* ---- P2/P3 tetrahedra: nodal basis from the monomial Vandermonde ---- */
 typedef struct
 {
@@ -695,6 +768,10 @@ orc_kernel_fn orc_get_kernel(int id)
     return k_p3_a;
   case ORC_K_P3TET_L:
     return k_p3_l;
+  case ORC_K_P1TET_NEUMANN:
+    return k_p1tet_neumann;
+  case ORC_K_Q1HEX_NEUMANN:
+    return k_q1hex_neumann;
   default:
     return NULL;
   }
@@ -847,6 +924,37 @@ void orc_assemble_vector(int kernel, const double* x, const int32_t* x_dofmap,
     memset(be, 0, sizeof(be));
     fn(be, coeffs ? coeffs + (int64_t)c * cstride : &zero,
        constants ? constants : &zero, cdofs, NULL, NULL, NULL);
+    const int32_t* dofs = dofmap + (int64_t)c * k;
+    for (int i = 0; i < k; ++i)
+      for (int a = 0; a < bs; ++a)
+        b[bs * dofs[i] + a] += be[bs * i + a];
+  }
+}
+
+/* fem/assemble_vector_impl.h:118-174 (assemble_exterior_facets): for each
+ * (cell, local facet) pair in order, the kernel with the local facet as
+ * entity_local_index, then b[bs dofs_i + a] += be[bs i + a]. */
+void orc_assemble_vector_facets(int kernel, const double* x,
+                                const int32_t* x_dofmap, int npc,
+                                const int32_t* dofmap, int k, int bs,
+                                const double* coeffs, int cstride,
+                                const double* constants, const int32_t* pairs,
+                                int32_t npairs, double* b)
+{
+  orc_kernel_fn fn = orc_get_kernel(kernel);
+  double be[60];
+  double cdofs[3 * 20];
+  const double zero = 0.0;
+  for (int32_t p = 0; p < npairs; ++p)
+  {
+    const int32_t c = pairs[2 * p];
+    const int lf = pairs[2 * p + 1];
+    for (int i = 0; i < npc; ++i)
+      memcpy(cdofs + 3 * i, x + 3 * (int64_t)x_dofmap[(int64_t)c * npc + i],
+             3 * sizeof(double));
+    memset(be, 0, sizeof(be));
+    fn(be, coeffs ? coeffs + (int64_t)c * cstride : &zero,
+       constants ? constants : &zero, cdofs, &lf, NULL, NULL);
     const int32_t* dofs = dofmap + (int64_t)c * k;
     for (int i = 0; i < k; ++i)
       for (int a = 0; a < bs; ++a)

@@ -598,6 +598,52 @@ void orc_edge_reflections(const orc_mesh* m, const int32_t* c2e,
 }
 
 /* mesh/utils.cpp:61-89 (exterior_facet_indices): facets with one cell. */
+/* fem::compute_integration_domains(IntegralType::exterior_facet, ...)
+ * (fem/utils.cpp:154-273) over the exterior facets (mesh/utils.h:180-194)
+ * in facet-index order: for each, its cell and local index in that cell. */
+int32_t orc_facet_domain(const orc_mesh* m, int marker, int axis, double value,
+                         int32_t** pairs)
+{
+  const int tdim = 3;
+  int32_t *c2f, *f2v;
+  const int32_t nfac = orc_entities(m, tdim - 1, &c2f, &f2v);
+  const int nf = orc_cell_num_entities(m->celltype, tdim - 1);
+  const int nvf = m->celltype == ORC_TET ? 3 : 4;
+  int32_t* count = (int32_t*)calloc(nfac, sizeof(int32_t));
+  int32_t* first = (int32_t*)malloc(sizeof(int32_t) * (nfac > 0 ? nfac : 1));
+  for (int64_t i = 0; i < (int64_t)m->num_cells * nf; ++i)
+  {
+    if (count[c2f[i]] == 0)
+      first[c2f[i]] = (int32_t)i;
+    count[c2f[i]]++;
+  }
+  int32_t* out = (int32_t*)malloc(sizeof(int32_t) * 2 * (nfac > 0 ? nfac : 1));
+  int32_t n = 0;
+  for (int32_t f = 0; f < nfac; ++f)
+  {
+    if (count[f] != 1)
+      continue;
+    if (marker)
+    {
+      int on = 1;
+      for (int a = 0; a < nvf; ++a)
+        if (m->x[3 * (int64_t)f2v[(int64_t)f * nvf + a] + axis] != value)
+          on = 0;
+      if (!on)
+        continue;
+    }
+    out[2 * n] = first[f] / nf;
+    out[2 * n + 1] = first[f] % nf;
+    ++n;
+  }
+  free(count);
+  free(first);
+  free(c2f);
+  free(f2v);
+  *pairs = out;
+  return n;
+}
+
 int32_t orc_exterior_facets(const orc_mesh* m, int32_t** cell_facet)
 {
   const int tdim = 3;

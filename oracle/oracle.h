@@ -32,6 +32,9 @@
  *                        modes in the kernel and exact energy of linear
  *                        fields; source totals int f (1 and 1/2 for f = 1,
  *                        x_0, python/test/unit/fem/test_assembler.py:67-78).
+ *   orc_facet_domain     brute-force facet vertex sets, facet-index order.
+ *   orc_assemble_vector_facets  int_Gamma g ds closed forms: surface area,
+ *                        one face, linear g (exact in P1/Q1).
  *   orc_add_element_*    equal to orc_assemble_* fed the tabulated tensors;
  *                        a dense scatter of random tensors on tiny meshes.
  *   orc_assemble_*       P1 mass total = |Omega|, A 1 = 0, "Entry not in
@@ -95,6 +98,12 @@ void orc_edge_reflections(const orc_mesh* m, const int32_t* c2e,
                           const int32_t* e2v, uint32_t* bits);
 /* Exterior facets as (cell, local facet) pairs; returns count. Caller frees */
 int32_t orc_exterior_facets(const orc_mesh* m, int32_t** cell_facet);
+/* fem::compute_integration_domains(exterior_facet) (fem/utils.cpp:154-273)
+ * on mesh::exterior_facet_indices, optionally only the facets with all
+ * vertices at x[axis] == value (marker 1): (cell, local facet) pairs in
+ * facet-index order. Returns count, caller frees. */
+int32_t orc_facet_domain(const orc_mesh* m, int marker, int axis, double value,
+                         int32_t** pairs);
 
 /* ---- dofmaps, BCs, sparsity (fem/dofmapbuilder.cpp, fem/utils.cpp,
  *      fem/DirichletBC.cpp, la/SparsityPattern.cpp) ---- */
@@ -130,7 +139,9 @@ enum
   ORC_K_P2TET_ELAST_L = 5, /* f.v; c = [fx, fy, fz] */
   ORC_K_P3TET_A = 6,   /* kappa grad u.grad v + c u v; w=[kappa], c=[c] */
   ORC_K_P3TET_L = 7,   /* f v; c = [f] */
-  ORC_K_COUNT = 8
+  ORC_K_P1TET_NEUMANN = 10, /* g v ds on an exterior facet; w = [g at 4 vertices] */
+  ORC_K_Q1HEX_NEUMANN = 11, /* g v ds on an exterior facet; w = [g at 8 vertices] */
+  ORC_K_COUNT = 12
 };
 orc_kernel_fn orc_get_kernel(int id);
 void orc_tabulate(int id, double* A, const double* w, const double* c,
@@ -155,6 +166,13 @@ int orc_add_element_matrices(int32_t num_cells, const int32_t* dofmap, int k,
                              const int32_t* cols, double* values);
 void orc_add_element_vectors(int32_t num_cells, const int32_t* dofmap, int k,
                              int bs, const double* e, double* b);
+/* Exterior facet loads over (cell, local facet) pairs in the given order. */
+void orc_assemble_vector_facets(int kernel, const double* x,
+                                const int32_t* x_dofmap, int npc,
+                                const int32_t* dofmap, int k, int bs,
+                                const double* coeffs, int cstride,
+                                const double* constants, const int32_t* pairs,
+                                int32_t npairs, double* b);
 void orc_assemble_vector(int kernel, const double* x, const int32_t* x_dofmap,
                          int nodes_per_cell, int32_t num_cells,
                          const int32_t* dofmap, int k, int bs,

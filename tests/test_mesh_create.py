@@ -107,3 +107,20 @@ def test_create_mesh_rejects_bad_input():
         mesh.create_mesh(mesh.CellType.tetrahedron, bad, x)
     with pytest.raises(DfxError, match="belong to a cell"):
         mesh.create_mesh(mesh.CellType.tetrahedron, cells, np.vstack([x, [[5.0, 5.0, 5.0]]]))
+
+
+@pytest.mark.parametrize("ct,n", [(O.TET, (3, 2, 4)), (O.HEX, (2, 3, 3))])
+@pytest.mark.parametrize("axis,value", [(None, 0.0), (0, 1.0), (2, 0.0)])
+def test_facet_domain_matches_oracle(ct, n, axis, value):
+    """Exterior facet integration entities (fem::compute_integration_domains)
+    equal the oracle's pairs, in facet-index order, on a box and on the same
+    box given in shuffled input order."""
+    for cells, x in [(None, None), box_input(n, ct, 5)]:
+        if cells is None:
+            om = O.Mesh(n, ct)
+            pm = mesh.create_box([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], n, mesh.CellType(ct))
+        else:
+            om = O.Mesh.from_cells(ct, cells, x)
+            pm = mesh.create_mesh(mesh.CellType(ct), cells, x)
+        ref = om.facet_domain() if axis is None else om.facet_domain(axis, value)
+        assert np.array_equal(fem.facet_domain(pm, axis, value), ref)

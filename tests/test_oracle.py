@@ -543,6 +543,48 @@ def test_element_insertion_dense_scatter():
                                np.arange(nd, dtype=np.int32))
 
 
+@pytest.mark.parametrize("ct,n", [(O.TET, (3, 2, 4)), (O.HEX, (2, 3, 3))])
+def test_facet_domain_brute_force(ct, n):
+    """(cell, local facet) of every exterior facet, in facet-index order:
+    facets found by matching sorted vertex tuples; marked facets have all
+    vertices on the plane."""
+    m = O.Mesh(n, ct)
+    table = TET_FACETS if ct == O.TET else HEX_FACETS
+    seen = {}
+    for c in range(m.num_cells):
+        for lf, fv in enumerate(table):
+            key = tuple(sorted(m.cells[c][list(fv)]))
+            seen.setdefault(key, []).append((c, lf))
+    ext = {v[0] for v in seen.values() if len(v) == 1}
+    pairs = m.facet_domain()
+    assert {tuple(p) for p in pairs} == ext and len(pairs) == len(ext)
+    c2f, _ = m.entities(2)
+    idx = [c2f[c, lf] for c, lf in pairs]
+    assert all(a < b for a, b in zip(idx, idx[1:]))
+    on = m.facet_domain(axis=2, value=1.0)
+    assert {tuple(p) for p in on} == {
+        (c, lf) for (c, lf) in ext if np.all(m.x[m.cells[c][list(table[lf])], 2] == 1.0)}
+
+
+@pytest.mark.parametrize("ct,kN,nv", [(O.TET, O.K_P1TET_NEUMANN, 4), (O.HEX, O.K_Q1HEX_NEUMANN, 8)])
+def test_facet_loads_closed_forms(ct, kN, nv):
+    """int_Gamma g v ds summed over v: the unit cube's surface area 6 for
+    g = 1; int over x = 1 of 1 + 2y + 3z = 3.5 and over y = 0 of 1 = 1
+    (g linear is exact in P1 and Q1)."""
+    m = O.Mesh((3, 2, 4), ct)
+    dm, nd = m.dofmap(1)
+    one = np.ones((m.num_cells, nv))
+    b = O.assemble_vector_facets(kN, m.x, m.cells, dm, one, m.facet_domain(), nd)
+    assert np.isclose(b.sum(), 6.0, rtol=1e-14)
+    g = (m.x[:, 0] + 2 * m.x[:, 1] + 3 * m.x[:, 2])[m.cells]
+    b1 = O.assemble_vector_facets(kN, m.x, m.cells, dm, g, m.facet_domain(0, 1.0), nd)
+    assert np.isclose(b1.sum(), 3.5, rtol=1e-14)
+    b2 = O.assemble_vector_facets(kN, m.x, m.cells, dm, one, m.facet_domain(1, 0.0), nd)
+    assert np.isclose(b2.sum(), 1.0, rtol=1e-14)
+    # only nodes on the face receive load
+    assert np.all(b2[m.x[:, 1] != 0.0] == 0.0)
+
+
 @pytest.mark.parametrize("jacobi", [True, False])
 def test_cg_against_scipy(jacobi):
     m, dm, nd, rp, cols, A, bn, mark = assemble_p1((8, 8, 8), kappa=1.0)
