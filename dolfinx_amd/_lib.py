@@ -18,7 +18,8 @@ DFX_OK, DFX_ERR_ARG, DFX_ERR_PATTERN, DFX_ERR_HIP, DFX_ERR_NOCONV, DFX_ERR_NODEV
 F32, F64 = 0, 1
 TET, HEX = 0, 1
 (K_P1TET_A, K_P1TET_L, K_Q1HEX_A, K_Q1HEX_L, K_P2TET_ELAST_A, K_P2TET_ELAST_L,
- K_P3TET_A, K_P3TET_L, K_ELEMENT_A, K_ELEMENT_L, K_P1TET_NEUMANN, K_Q1HEX_NEUMANN) = range(12)
+ K_P3TET_A, K_P3TET_L, K_ELEMENT_A, K_ELEMENT_L, K_P1TET_NEUMANN, K_Q1HEX_NEUMANN,
+ K_P1TET_ROBIN_A, K_Q1HEX_ROBIN_A) = range(14)
 
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)

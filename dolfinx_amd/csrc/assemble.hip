@@ -78,6 +78,12 @@ bool kernel_info(int id, KernelInfo& ki)
   case DFX_K_Q1HEX_NEUMANN:
     ki = {DFX_HEX, 8, 8, 1, false};
     return true;
+  case DFX_K_P1TET_ROBIN_A:
+    ki = {DFX_TET, 4, 4, 1, true};
+    return true;
+  case DFX_K_Q1HEX_ROBIN_A:
+    ki = {DFX_HEX, 8, 8, 1, true};
+    return true;
   case DFX_K_ELEMENT_A:
     ki = {-1, 0, 0, 1, true}; // any element: k from the descriptor
     return true;
@@ -93,6 +99,12 @@ bool kernel_info(int id, KernelInfo& ki)
   default:
     return false;
   }
+}
+
+/// Kernels of exterior facet matrices (beta u v ds).
+bool is_facet_matrix_kernel(int id)
+{
+  return id == DFX_K_P1TET_ROBIN_A || id == DFX_K_Q1HEX_ROBIN_A;
 }
 
 // ---------------------------------------------------------------------
@@ -1642,6 +1654,159 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+/// One thread per (cell, local facet) pair e: the facet matrix of
+/// beta u v ds, out[16 e + 4 a + b] = beta |F| (1 + d_ab) / 12 on triangles
+/// (a, b < 3, the rest zero) and beta |F| M_ab / 36 on parallelograms, with
+/// beta the cell's first packed coefficient.
+template <typename T, bool HEX>
+__global__ void __launch_bounds__(kBlock)
+    facet_matrix_k(std::int32_t npairs, const std::int32_t* __restrict__ pairs,
+                   const T* __restrict__ x, const std::int32_t* __restrict__ xdm, int npc,
+                   const T* __restrict__ coeffs, int cstride, T* __restrict__ out)
+{
+  const std::int32_t e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= npairs)
+    return;
+  const std::int32_t c = pairs[2 * e];
+  const int lf = pairs[2 * e + 1];
+  T X[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+  {
+    const int lv = HEX ? kHexFacetV[lf][a] : kTetFacetV[lf][a];
+    const std::int64_t v = xdm[std::int64_t(c) * npc + lv];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      X[a][d] = x[3 * v + d];
+  }
+  const T beta = coeffs[std::int64_t(c) * cstride];
+  T t0[3], t1[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    t0[d] = X[1][d] - X[0][d];
+    t1[d] = X[2][d] - X[0][d];
+  }
+  const T n0 = t0[1] * t1[2] - t0[2] * t1[1];
+  const T n1 = t0[2] * t1[0] - t0[0] * t1[2];
+  const T n2 = t0[0] * t1[1] - t0[1] * t1[0];
+  const T nrm = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+  T* o = out + 16 * std::int64_t(e);
+  if constexpr (HEX)
+  {
+    constexpr T M[4][4] = {{4, 2, 2, 1}, {2, 4, 1, 2}, {2, 1, 4, 2}, {1, 2, 2, 4}};
+    const T w = beta * (nrm / T(36));
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        o[4 * a + b] = w * M[a][b];
+  }
+  else
+  {
+    const T area = T(0.5) * nrm;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        o[4 * a + b] = (a < 3 && b < 3) ? beta * area * (a == b ? T(2) : T(1)) / T(12) : T(0);
+  }
+}
+
+/// A[r, :] += the facet matrices' rows at row r, facet by facet in facet
+/// order (one thread per row over its facet incidences finc = 4 e + facet
+/// vertex); entries in a BC row or column add zero. The column's slot is
+/// found by binary search in the row; an absent one sets *err.
+template <typename T, bool HEX>
+__global__ void __launch_bounds__(kBlock)
+    facet_matrix_rows_k(std::int32_t nplan, const std::int32_t* __restrict__ off,
+                        const std::int32_t* __restrict__ finc,
+                        const std::int32_t* __restrict__ pairs,
+                        const std::int32_t* __restrict__ dofmap, int k,
+                        const T* __restrict__ fm, const std::int8_t* __restrict__ mark,
+                        const std::int32_t* __restrict__ row_ptr,
+                        const std::int32_t* __restrict__ cols, T* __restrict__ values,
+                        int* __restrict__ err)
+{
+  const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nplan)
+    return;
+  constexpr int NF = HEX ? 4 : 3;
+  const std::int32_t p1 = off[r + 1];
+  if (off[r] == p1)
+    return;
+  const std::int32_t rb = row_ptr[r], re = row_ptr[r + 1];
+  const bool rmask = mark && mark[r];
+  for (std::int32_t p = off[r]; p < p1; ++p)
+  {
+    const std::int32_t e = finc[p] >> 2;
+    const int a = finc[p] & 3;
+    const std::int32_t c = pairs[2 * e];
+    const int lf = pairs[2 * e + 1];
+#pragma unroll
+    for (int b = 0; b < NF; ++b)
+    {
+      const int lv = HEX ? kHexFacetV[lf][b] : kTetFacetV[lf][b];
+      const std::int32_t col = dofmap[std::int64_t(c) * k + lv];
+      std::int32_t lo = rb, hi = re;
+      while (lo < hi)
+      {
+        const std::int32_t mid = (lo + hi) >> 1;
+        if (cols[mid] < col)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      if (lo == re || cols[lo] != col)
+      {
+        *err = 1;
+        continue;
+      }
+      const bool zero = rmask || (mark && mark[col]);
+      values[lo] += zero ? T(0) : fm[16 * std::int64_t(e) + 4 * a + b];
+    }
+  }
+}
+
+/// Lifting by the facet matrices: b[r] -= alpha sum over the row's facet
+/// incidences and the facet's BC columns j of F_rj (g_j - x0_j).
+template <typename T, bool HEX>
+__global__ void __launch_bounds__(kBlock)
+    facet_lift_rows_k(std::int32_t nplan, const std::int32_t* __restrict__ off,
+                      const std::int32_t* __restrict__ finc,
+                      const std::int32_t* __restrict__ pairs,
+                      const std::int32_t* __restrict__ dofmap, int k,
+                      const T* __restrict__ fm, const std::int8_t* __restrict__ mark,
+                      const double* __restrict__ g, const T* __restrict__ x0, double alpha,
+                      T* __restrict__ bvec)
+{
+  const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nplan)
+    return;
+  constexpr int NF = HEX ? 4 : 3;
+  double s = 0;
+  for (std::int32_t p = off[r]; p < off[r + 1]; ++p)
+  {
+    const std::int32_t e = finc[p] >> 2;
+    const int a = finc[p] & 3;
+    const std::int32_t c = pairs[2 * e];
+    const int lf = pairs[2 * e + 1];
+#pragma unroll
+    for (int b = 0; b < NF; ++b)
+    {
+      const int lv = HEX ? kHexFacetV[lf][b] : kTetFacetV[lf][b];
+      const std::int32_t col = dofmap[std::int64_t(c) * k + lv];
+      if (!mark[col])
+        continue;
+      const double gv = g ? g[col] : 0.0;
+      const double xv = x0 ? double(x0[col]) : 0.0;
+      s += double(fm[16 * std::int64_t(e) + 4 * a + b]) * (gv - xv);
+    }
+  }
+  if (s != 0.0)
+    bvec[r] = static_cast<T>(double(bvec[r]) - alpha * s);
+}
+
 /// b[r] += the facet contributions at row r in facet order (one thread per
 /// row over its facet incidences finc = 4 e + facet vertex).
 template <typename T>
@@ -3108,6 +3273,71 @@ void lift_rows(dfx_form& f, const dfx_bc& bc, const T* x0, double alpha, T* b)
 
 /// Exterior facet loads: the facets' vertex contributions, then one
 /// thread per row adding its facet incidences.
+/// The facet matrices of a DFX_K_*_ROBIN_A form into f.fvals.
+template <typename T>
+void facet_matrices(const dfx_form& f)
+{
+  const auto* x = static_cast<const T*>(f.x);
+  const auto* co = static_cast<const T*>(f.coeffs);
+  T* fm = static_cast<T*>(f.fvals);
+  if (f.d.kernel == DFX_K_Q1HEX_ROBIN_A)
+    facet_matrix_k<T, true><<<blocks_for(f.nfacet), kBlock, 0, stream()>>>(
+        f.nfacet, f.facets, x, f.x_dofmap, f.d.nodes_per_cell, co, f.d.cstride, fm);
+  else
+    facet_matrix_k<T, false><<<blocks_for(f.nfacet), kBlock, 0, stream()>>>(
+        f.nfacet, f.facets, x, f.x_dofmap, f.d.nodes_per_cell, co, f.d.cstride, fm);
+  check_launch("facet matrices");
+}
+
+/// Exterior facet matrices added into A row by row (facet order per row).
+template <typename T>
+void facet_matrix_rows(const dfx_form& f, dfx_csr& A, const dfx_bc* bc)
+{
+  if (f.nfacet == 0)
+    return;
+  facet_matrices<T>(f);
+  int* err = static_cast<int*>(dmalloc(sizeof(int)));
+  DFX_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
+  const std::int32_t np = f.max_dof + 1;
+  const std::int8_t* mark = bc ? bc->marker : nullptr;
+  auto* vals = static_cast<T*>(A.values);
+  const auto* fm = static_cast<const T*>(f.fvals);
+  if (f.d.kernel == DFX_K_Q1HEX_ROBIN_A)
+    facet_matrix_rows_k<T, true><<<blocks_for(np), kBlock, 0, stream()>>>(
+        np, f.finc_off, f.finc, f.facets, f.dofmap, f.d.dofs_per_cell, fm, mark, A.row_ptr,
+        A.cols, vals, err);
+  else
+    facet_matrix_rows_k<T, false><<<blocks_for(np), kBlock, 0, stream()>>>(
+        np, f.finc_off, f.finc, f.facets, f.dofmap, f.d.dofs_per_cell, fm, mark, A.row_ptr,
+        A.cols, vals, err);
+  check_launch("facet matrix rows");
+  int h = 0;
+  d2h(&h, err, sizeof(int));
+  dfree(err);
+  if (h)
+    throw Error(DFX_ERR_PATTERN, "Entry not in sparsity");
+}
+
+/// Lifting by the exterior facet matrices.
+template <typename T>
+void facet_lift_rows(const dfx_form& f, const dfx_bc& bc, const T* x0, double alpha, T* b)
+{
+  if (f.nfacet == 0)
+    return;
+  facet_matrices<T>(f);
+  const std::int32_t np = f.max_dof + 1;
+  const auto* fm = static_cast<const T*>(f.fvals);
+  if (f.d.kernel == DFX_K_Q1HEX_ROBIN_A)
+    facet_lift_rows_k<T, true><<<blocks_for(np), kBlock, 0, stream()>>>(
+        np, f.finc_off, f.finc, f.facets, f.dofmap, f.d.dofs_per_cell, fm, bc.marker, bc.g, x0,
+        alpha, b);
+  else
+    facet_lift_rows_k<T, false><<<blocks_for(np), kBlock, 0, stream()>>>(
+        np, f.finc_off, f.finc, f.facets, f.dofmap, f.d.dofs_per_cell, fm, bc.marker, bc.g, x0,
+        alpha, b);
+  check_launch("facet lifting");
+}
+
 template <typename T>
 void facet_rows(const dfx_form& f, T* b)
 {
@@ -3236,6 +3466,13 @@ void launch_linear(const dfx_form& f, T* b)
 /// True if every hexahedron is a parallelepiped (x3 = x1 + x2 - x0 and
 /// likewise for x5, x6, x7), so that its Jacobian is constant.
 bool is_element_kernel(int id) { return id == DFX_K_ELEMENT_A || id == DFX_K_ELEMENT_L; }
+
+/// Kernels of exterior facet integrals.
+bool is_facet_kernel(int id)
+{
+  return id == DFX_K_P1TET_NEUMANN || id == DFX_K_Q1HEX_NEUMANN || id == DFX_K_P1TET_ROBIN_A
+         || id == DFX_K_Q1HEX_ROBIN_A;
+}
 
 /// Element tensors of these sizes k (k x k matrices) have row kernels.
 bool element_matrix_size(int k) { return k == 4 || k == 8 || k == 10 || k == 20; }
@@ -3479,6 +3716,8 @@ dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out)
           need = 4;
         else if (d.kernel == DFX_K_Q1HEX_NEUMANN)
           need = 8;
+        else if (is_facet_matrix_kernel(d.kernel))
+          need = 1;
         require(d.cstride >= need, "too few packed coefficients for this kernel");
         const std::int64_t nc = d.num_cells;
         for (std::int64_t i = 0; i < nc * d.nodes_per_cell; ++i)
@@ -3521,7 +3760,11 @@ dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out)
           // thread-per-row slices do not fit LDS for wide elements
           f->row_variant = k > 8 ? 3 : 1;
           const std::size_t ts = dtype_size(d.dtype);
-          if (ki.bilinear)
+          if (is_facet_kernel(d.kernel))
+          {
+            // facet forms keep per-facet data only (dfx_form_create_facets)
+          }
+          else if (ki.bilinear)
           {
             const int per_cell = d.kernel == DFX_K_P2TET_ELAST_A ? 16 : 8;
             f->cell_data = dmalloc(ts * per_cell * std::max<std::int64_t>(nc, 1));
@@ -3549,10 +3792,10 @@ dfx_status dfx_form_create_facets(const dfx_form_desc* desc, int32_t npairs,
       [&]
       {
         require(desc && out && (npairs == 0 || pairs), "null argument");
-        require(desc->kernel == DFX_K_P1TET_NEUMANN || desc->kernel == DFX_K_Q1HEX_NEUMANN,
-                "the kernel has no exterior facet integral");
+        require(is_facet_kernel(desc->kernel), "the kernel has no exterior facet integral");
         require(npairs >= 0, "negative size");
-        const bool hex = desc->kernel == DFX_K_Q1HEX_NEUMANN;
+        const bool hex = desc->kernel == DFX_K_Q1HEX_NEUMANN || desc->kernel == DFX_K_Q1HEX_ROBIN_A;
+        const bool matrix = is_facet_matrix_kernel(desc->kernel);
         const int nfc = hex ? 6 : 4, nfv = hex ? 4 : 3, k = desc->dofs_per_cell;
         static constexpr int tetv[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
         static constexpr int hexv[6][4] = {{0, 1, 2, 3}, {0, 1, 4, 5}, {0, 2, 4, 6},
@@ -3567,7 +3810,7 @@ dfx_status dfx_form_create_facets(const dfx_form_desc* desc, int32_t npairs,
           throw Error(st, dfx_last_error());
         std::unique_ptr<dfx_form, dfx_status (*)(dfx_form*)> f(raw, dfx_form_destroy);
         if (hex)
-          require(f->affine, "facet loads need affine hexahedra");
+          require(f->affine, "facet integrals need affine hexahedra");
         // facet incidences of every row in facet order (counting sort)
         const std::int32_t nplan = f->max_dof + 1;
         std::vector<std::int32_t> off(std::size_t(nplan) + 1, 0);
@@ -3593,7 +3836,7 @@ dfx_status dfx_form_create_facets(const dfx_form_desc* desc, int32_t npairs,
         h2d(f->finc_off, off.data(), sizeof(std::int32_t) * off.size());
         f->finc = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * finc.size()));
         h2d(f->finc, finc.data(), sizeof(std::int32_t) * finc.size());
-        f->fvals = dmalloc(dtype_size(desc->dtype) * 4 * std::max(npairs, 1));
+        f->fvals = dmalloc(dtype_size(desc->dtype) * (matrix ? 16 : 4) * std::max(npairs, 1));
         *out = f.release();
       });
 }
@@ -3780,6 +4023,16 @@ dfx_status dfx_assemble_matrix(dfx_csr* A, const dfx_form* a, const dfx_bc* bc)
         require(!bc || bc->n_total == A->nrows * A->bs, "BC size does not match the matrix");
         require(a->max_dof < A->nrows && a->max_dof < A->ncols, "dofmap exceeds the matrix size");
         dfx_form& f = const_cast<dfx_form&>(*a);
+        if (is_facet_matrix_kernel(f.d.kernel))
+        {
+          materialize_zero(*A);
+          A->zeroed = false;
+          if (A->dtype == DFX_F64)
+            facet_matrix_rows<double>(f, *A, bc);
+          else
+            facet_matrix_rows<float>(f, *A, bc);
+          return;
+        }
         if (row_path(f))
         {
           // store mode (A zeroed) overwrites every entry; add mode needs
@@ -3846,6 +4099,7 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t*
         require(ki.bilinear && A->dtype == a->d.dtype && A->bs == a->d.bs,
                 "form and matrix do not match");
         require(a->max_dof < A->nrows, "dofmap exceeds the matrix size");
+        require(!is_facet_matrix_kernel(a->d.kernel), "facet forms have no kernel variants");
         dfx_form& f = *a;
         int best = f.row_variant;
         float bms = -1;
@@ -3989,7 +4243,7 @@ dfx_status dfx_assemble_vector(dfx_vec* b, const dfx_form* L)
                 "element-tensor forms are assembled with dfx_assemble_vector_elements");
         require(b->dtype == L->d.dtype, "vector and form dtypes differ");
         require(std::int64_t(L->max_dof + 1) * L->d.bs <= b->n, "dofmap exceeds the vector size");
-        if (L->d.kernel == DFX_K_P1TET_NEUMANN || L->d.kernel == DFX_K_Q1HEX_NEUMANN)
+        if (is_facet_kernel(L->d.kernel))
         {
           if (b->dtype == DFX_F64)
             facet_rows<double>(*L, static_cast<double*>(b->data));
@@ -4139,6 +4393,16 @@ dfx_status dfx_apply_lifting(dfx_vec* b, const dfx_form* a, const dfx_bc* bc, co
         require(f.d.kernel != DFX_K_P2TET_ELAST_A,
                 "apply_lifting with nonzero Dirichlet values is not implemented for the "
                 "P2 elasticity form (g = 0 and x0 = 0 are)");
+        if (is_facet_matrix_kernel(f.d.kernel))
+        {
+          if (b->dtype == DFX_F64)
+            facet_lift_rows<double>(f, *bc, x0 ? static_cast<const double*>(x0->data) : nullptr,
+                                    alpha, static_cast<double*>(b->data));
+          else
+            facet_lift_rows<float>(f, *bc, x0 ? static_cast<const float*>(x0->data) : nullptr,
+                                   alpha, static_cast<float*>(b->data));
+          return;
+        }
         if (row_path(f))
         {
           if (b->dtype == DFX_F64)

@@ -59,6 +59,30 @@ class FunctionSpace:
                 pass
 
 
+class Function:
+    """fem::Function (fem/Function.h): a space and its dof values, held on
+    the GPU in a la.Vector (x.array copies them to the host)."""
+
+    def __init__(self, V: FunctionSpace, x: Vector | None = None, name="f",
+                 dtype=np.float64):
+        self.function_space = V
+        self.x = Vector(V.num_dofs, dtype) if x is None else x
+        if self.x.size != V.num_dofs:
+            raise ValueError(f"{V.num_dofs} values expected, got {self.x.size}")
+        self.name = name
+
+    def interpolate(self, f) -> None:
+        """Nodal interpolation of f(x) (x: (3, n) points; f returns (n,) or
+        (bs, n)) on a degree-1 space, whose dofs sit at the mesh nodes."""
+        V = self.function_space
+        if V.degree != 1:
+            raise NotImplementedError("interpolation is implemented for degree-1 spaces")
+        X = np.zeros((V.dofmap.index_map_size, 3))
+        X[V.dofmap.list.ravel()] = V.mesh.geometry.x[V.mesh.geometry.dofmap.ravel()]
+        v = np.asarray(f(X.T), np.float64).reshape(V.dofmap.bs, -1)
+        self.x.array = np.ascontiguousarray(v.T).ravel()
+
+
 def functionspace(mesh: Mesh, element) -> FunctionSpace:
     """functionspace(mesh, ("Lagrange", degree[, (3,)]))."""
     family, degree = element[0], element[1]
@@ -120,6 +144,10 @@ def dirichletbc(V: FunctionSpace, dofs, g=None) -> DirichletBC:
     return DirichletBC(V, dofs, g)
 
 
+# Kernels of exterior facet integrals (integration entities: (cell, local
+# facet) pairs)
+FACET_KERNELS = {"p1_neumann", "q1_neumann", "p1_robin", "q1_robin"}
+
 # name -> (kernel id, cell type, degree, block size, rank)
 KERNELS = {
     "p1_stiffness_reaction": (_lib.K_P1TET_A, CellType.tetrahedron, 1, 1, 2),
@@ -131,6 +159,8 @@ KERNELS = {
     "p3_stiffness_reaction": (_lib.K_P3TET_A, CellType.tetrahedron, 3, 1, 2),
     "p1_neumann": (_lib.K_P1TET_NEUMANN, CellType.tetrahedron, 1, 1, 1),
     "q1_neumann": (_lib.K_Q1HEX_NEUMANN, CellType.hexahedron, 1, 1, 1),
+    "p1_robin": (_lib.K_P1TET_ROBIN_A, CellType.tetrahedron, 1, 1, 2),
+    "q1_robin": (_lib.K_Q1HEX_ROBIN_A, CellType.hexahedron, 1, 1, 2),
     "p3_source": (_lib.K_P3TET_L, CellType.tetrahedron, 3, 1, 1),
 }
 
@@ -167,6 +197,7 @@ class Form:
         if mesh.topology.cell_type != ct or V.degree != deg or V.dofmap.bs != bs:
             raise ValueError(f"form {name} does not match the space")
         self.name, self.V, self.rank, self.dtype = name, V, rank, np.dtype(dtype)
+        self.integral = "exterior_facet" if name in FACET_KERNELS else "cell"
         self.coeffs = None if coeffs is None else np.ascontiguousarray(coeffs, np.float64)
         nc = mesh.topology.cells.shape[0]
         if self.coeffs is not None:
@@ -196,6 +227,7 @@ class Form:
 
     def _init_elements(self, name, V, dtype):
         kid, rank = ELEMENT_FORMS[name]
+        self.integral = "cell"
         if V.dofmap.bs != 1:
             raise ValueError("element tensors need a scalar space")
         self.name, self.V, self.rank, self.dtype = name, V, rank, np.dtype(dtype)
@@ -253,9 +285,23 @@ class Form:
 
 def form(name, V, coeffs=None, constants=None, dtype=np.float64, facets=None) -> Form:
     """A form of a named kernel; exterior facet kernels ("p1_neumann",
-    "q1_neumann") take their integration entities as (cell, local facet)
-    pairs (facet_domain)."""
+    "q1_neumann": g v ds with g packed per cell vertex; "p1_robin",
+    "q1_robin": beta u v ds with DG0 beta) take their integration entities as
+    (cell, local facet) pairs (facet_domain). A form with several integrals
+    is passed to the assemblers as a tuple of such forms."""
     return Form(name, V, coeffs, constants, dtype, facets)
+
+
+def _integrals(a):
+    """The forms of a (one form or a tuple of forms of one form's
+    integrals) in the reference's order: cell integrals, then exterior
+    facets (fem/assemble_matrix_impl.h:776-880,
+    fem/assemble_vector_impl.h:540-615)."""
+    forms = list(a) if isinstance(a, (list, tuple)) else [a]
+    if not forms:
+        raise ValueError("no forms")
+    return [f for f in forms if f.integral == "cell"] + [
+        f for f in forms if f.integral != "cell"]
 
 
 def facet_domain(mesh: Mesh, axis=None, value=0.0) -> np.ndarray:
@@ -310,16 +356,18 @@ def create_matrix(a: Form, dtype=None) -> MatrixCSR:
     return MatrixCSR.from_sparsity(sp, a.dtype if dtype is None else dtype)
 
 
-def assemble_matrix(a: Form, bcs=(), diag=1.0, A: MatrixCSR | None = None) -> MatrixCSR:
+def assemble_matrix(a, bcs=(), diag=1.0, A: MatrixCSR | None = None) -> MatrixCSR:
     """fem.assemble_matrix (python/dolfinx/fem/assemble.py:283-376): assemble
-    with BC rows/columns zeroed, then insert ``diag`` on BC rows."""
+    with BC rows/columns zeroed, then insert ``diag`` on BC rows. ``a`` is a
+    Form or a tuple of the forms of one form's integrals."""
+    forms = _integrals(a)
     if A is None:
-        A = create_matrix(a)
-    for bc in bcs or [None]:
-        _lib.call("dfx_assemble_matrix", A.handle, a.handle, None if bc is None else bc.handle)
-        break
+        A = create_matrix(forms[0])
     if len(bcs) > 1:
         raise ValueError("pass the union of Dirichlet dofs as one DirichletBC")
+    bc = bcs[0] if bcs else None
+    for f in forms:
+        _lib.call("dfx_assemble_matrix", A.handle, f.handle, None if bc is None else bc.handle)
     for bc in bcs:
         _lib.call("dfx_set_diagonal", A.handle, bc.handle, float(diag))
     return A
@@ -370,7 +418,20 @@ def assemble_system(a: Form, L: Form, bcs=(), diag=1.0, A: MatrixCSR | None = No
     A = assemble_matrix(a, bcs) with ``diag`` on BC rows (A zeroed first),
     b = assemble_vector(L) from zero, apply_lifting(b, [a], [bcs]),
     set_bc(b, bcs); the vector assembly overlaps the matrix assembly on a
-    second stream. Returns (A, b)."""
+    second stream. Returns (A, b). Forms with several integrals (tuples)
+    are assembled by the separate calls in the same sequence."""
+    if isinstance(a, (list, tuple)) or isinstance(L, (list, tuple)):
+        if A is not None:
+            A.set_value(0.0)
+        A = assemble_matrix(a, bcs, diag, A)
+        if b is None:
+            b = Vector(_integrals(L)[0].V.num_dofs, _integrals(L)[0].dtype)
+        else:
+            b.set(0.0)
+        assemble_vector(L, b)
+        apply_lifting(b, [a], [bcs])
+        set_bc(b, bcs)
+        return A, b
     if A is None:
         A = create_matrix(a)
     else:
@@ -404,21 +465,26 @@ def assemble_matrix_batched(forms, bcs, matrices, diag=1.0) -> None:
             _lib.call("dfx_set_diagonal", A.handle, bc.handle, float(diag))
 
 
-def assemble_vector(L: Form, b: Vector | None = None) -> Vector:
-    """fem.assemble_vector (fem/assembler.h:230-257): b += L."""
+def assemble_vector(L, b: Vector | None = None) -> Vector:
+    """fem.assemble_vector (fem/assembler.h:230-257): b += L (a Form or a
+    tuple of the forms of one form's integrals)."""
+    forms = _integrals(L)
     if b is None:
-        b = Vector(L.V.num_dofs, L.dtype)
-    _lib.call("dfx_assemble_vector", b.handle, L.handle)
+        b = Vector(forms[0].V.num_dofs, forms[0].dtype)
+    for f in forms:
+        _lib.call("dfx_assemble_vector", b.handle, f.handle)
     return b
 
 
 def apply_lifting(b: Vector, a_list, bcs_list, x0=None, alpha=1.0) -> None:
-    """fem.apply_lifting (python/dolfinx/fem/assemble.py:378-493)."""
+    """fem.apply_lifting (python/dolfinx/fem/assemble.py:378-493); each
+    entry of a_list is a Form or a tuple of the forms of its integrals."""
     for i, (a, bcs) in enumerate(zip(a_list, bcs_list)):
         xx = None if not x0 else x0[i]
-        for bc in bcs:
-            _lib.call("dfx_apply_lifting", b.handle, a.handle, bc.handle,
-                      None if xx is None else xx.handle, float(alpha))
+        for f in _integrals(a):
+            for bc in bcs:
+                _lib.call("dfx_apply_lifting", b.handle, f.handle, bc.handle,
+                          None if xx is None else xx.handle, float(alpha))
 
 
 def set_bc(b: Vector, bcs, x0: Vector | None = None, alpha=1.0) -> None:

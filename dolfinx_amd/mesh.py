@@ -52,6 +52,7 @@ class Mesh:
         self.topology = Topology(cell_type, cells, orig, nv.value)
         # P1 geometry: the geometry dofmap is the cell-vertex map
         self.geometry = Geometry(x, cells, inp)
+        self.name = "mesh"
 
     @property
     def handle(self):

@@ -55,7 +55,9 @@ enum
   DFX_K_ELEMENT_A = 8,     /* element matrices supplied by the caller */
   DFX_K_ELEMENT_L = 9,     /* element vectors supplied by the caller */
   DFX_K_P1TET_NEUMANN = 10,/* g v ds on exterior facets; w = [g at the 4 vertices] */
-  DFX_K_Q1HEX_NEUMANN = 11 /* g v ds on exterior facets; w = [g at the 8 vertices] */
+  DFX_K_Q1HEX_NEUMANN = 11,/* g v ds on exterior facets; w = [g at the 8 vertices] */
+  DFX_K_P1TET_ROBIN_A = 12,/* beta u v ds on exterior facets; w = [beta] (DG0) */
+  DFX_K_Q1HEX_ROBIN_A = 13 /* beta u v ds on exterior facets; w = [beta] (DG0) */
 };
 
 const char* dfx_last_error(void);
@@ -220,11 +222,15 @@ typedef struct
 
 /* fem::Form (fem/Form.h:116-) holding device copies of the form data. */
 dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out);
-/* A form with an exterior facet integral (kernel DFX_K_*_NEUMANN): its
- * integration entities are the npairs (cell, local facet) pairs
- * (fem/Form.h integral_data entities, e.g. from dfx_facet_domain);
- * assemble_vector adds the facets in this order
- * (fem/assemble_vector_impl.h:118-174). */
+/* A form with an exterior facet integral (kernels DFX_K_*_NEUMANN and
+ * DFX_K_*_ROBIN_A): its integration entities are the npairs (cell, local
+ * facet) pairs (fem/Form.h integral_data entities, e.g. from
+ * dfx_facet_domain). assemble_vector adds the facets in this order
+ * (fem/assemble_vector_impl.h:118-174); assemble_matrix adds the facet
+ * matrices, BC rows and columns zeroed, in this order on top of A's values
+ * (assemble_entities, fem/assemble_matrix_impl.h:264-383, run after the
+ * cell integrals at :844-880: assemble the cell form first); apply_lifting
+ * subtracts them like lift_bc (fem/assemble_vector_impl.h:361-414). */
 dfx_status dfx_form_create_facets(const dfx_form_desc* desc, int32_t npairs,
                                   const int32_t* pairs, dfx_form** out);
 dfx_status dfx_form_destroy(dfx_form* f);

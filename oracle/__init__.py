@@ -23,6 +23,7 @@ TET, HEX = 0, 1
 K_P1TET_A, K_P1TET_L, K_Q1HEX_A, K_Q1HEX_L = 0, 1, 2, 3
 K_P2TET_ELAST_A, K_P2TET_ELAST_L, K_P3TET_A, K_P3TET_L = 4, 5, 6, 7
 K_P1TET_NEUMANN, K_Q1HEX_NEUMANN = 10, 11
+K_P1TET_ROBIN_A, K_Q1HEX_ROBIN_A = 12, 13
 
 _lib = None
 
@@ -88,6 +89,11 @@ def _declare(L):
     L.orc_facet_domain.argtypes = [C.POINTER(_Mesh), C.c_int, C.c_int, D, C.POINTER(P)]
     L.orc_assemble_vector_facets.argtypes = [C.c_int, P, P, C.c_int, P, C.c_int, C.c_int, P,
                                              C.c_int, P, P, I32, P]
+    L.orc_assemble_matrix_facets.restype = C.c_int
+    L.orc_assemble_matrix_facets.argtypes = [C.c_int, P, P, C.c_int, P, C.c_int, C.c_int, P,
+                                             C.c_int, P, P, I32, P, P, P, P, P]
+    L.orc_lift_bc_facets.argtypes = [C.c_int, P, P, C.c_int, P, C.c_int, C.c_int, P, C.c_int,
+                                     P, P, I32, P, P, P, D, P]
     L.orc_add_element_matrices.restype = C.c_int
     L.orc_add_element_matrices.argtypes = [I32, P, C.c_int, C.c_int, P, P, P, P, P, P]
     L.orc_add_element_vectors.argtypes = [I32, P, C.c_int, C.c_int, P, P]
@@ -284,6 +290,46 @@ def assemble_vector_facets(kernel, x, x_dofmap, dofmap, coeffs, pairs, n, consta
     b = np.zeros(n, np.float64)
     lib().orc_assemble_vector_facets(kernel, _p(x), _p(xd), xd.shape[1], _p(dm), dm.shape[1], 1,
                                      _p(co), co.shape[1], _p(kc), _p(pr), pr.shape[0], _p(b))
+    return b
+
+
+def assemble_matrix_facets(kernel, x, x_dofmap, dofmap, coeffs, pairs, bc0, bc1, row_ptr, cols,
+                           values=None, constants=None):
+    """values += the exterior facet matrices over (cell, local facet) pairs,
+    BC rows (bc0) and columns (bc1) zeroed (scalar spaces)."""
+    x = np.ascontiguousarray(x, np.float64)
+    xd = np.ascontiguousarray(x_dofmap, np.int32)
+    dm = np.ascontiguousarray(dofmap, np.int32)
+    co = np.ascontiguousarray(coeffs, np.float64).reshape(xd.shape[0], -1)
+    kc = None if constants is None else np.ascontiguousarray(constants, np.float64)
+    pr = np.ascontiguousarray(pairs, np.int32)
+    b0 = None if bc0 is None else np.ascontiguousarray(bc0, np.int8)
+    b1 = None if bc1 is None else np.ascontiguousarray(bc1, np.int8)
+    if values is None:
+        values = np.zeros(len(cols), np.float64)
+    rc = lib().orc_assemble_matrix_facets(
+        kernel, _p(x), _p(xd), xd.shape[1], _p(dm), dm.shape[1], 1, _p(co), co.shape[1], _p(kc),
+        _p(pr), pr.shape[0], _p(b0), _p(b1), _p(np.ascontiguousarray(row_ptr, np.int64)),
+        _p(np.ascontiguousarray(cols, np.int32)), _p(values))
+    if rc != 0:
+        raise RuntimeError("Entry not in sparsity")
+    return values
+
+
+def lift_bc_facets(b, kernel, x, x_dofmap, dofmap, coeffs, pairs, bc_values, bc_markers,
+                   x0=None, alpha=1.0):
+    """b -= alpha F (g - x0) over the BC columns of each facet matrix F."""
+    x = np.ascontiguousarray(x, np.float64)
+    xd = np.ascontiguousarray(x_dofmap, np.int32)
+    dm = np.ascontiguousarray(dofmap, np.int32)
+    co = np.ascontiguousarray(coeffs, np.float64).reshape(xd.shape[0], -1)
+    pr = np.ascontiguousarray(pairs, np.int32)
+    lib().orc_lift_bc_facets(kernel, _p(x), _p(xd), xd.shape[1], _p(dm), dm.shape[1], 1, _p(co),
+                             co.shape[1], None, _p(pr), pr.shape[0],
+                             _p(np.ascontiguousarray(bc_values, np.float64)),
+                             _p(np.ascontiguousarray(bc_markers, np.int8)),
+                             None if x0 is None else _p(np.ascontiguousarray(x0, np.float64)),
+                             alpha, _p(b))
     return b
 
 

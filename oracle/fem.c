@@ -451,6 +451,64 @@ static void k_q1hex_neumann(double* A, const double* w, const double* c,
     }
 }
 
+/* P1 tet, facet e[0]: A_ab = beta |F| (1 + d_ab) / 12 over the facet's
+ * vertices (beta u v ds, beta = w[0] constant on the cell), written into the
+ * 4 x 4 cell matrix. */
+static void k_p1tet_robin(double* A, const double* w, const double* c,
+                          const double* cd, const int* e, const uint8_t* q,
+                          void* u)
+{
+  (void)c;
+  (void)q;
+  (void)u;
+  const int* fv = TET_FV[e[0]];
+  double t0[3], t1[3], n[3];
+  for (int d = 0; d < 3; ++d)
+  {
+    t0[d] = cd[3 * fv[1] + d] - cd[3 * fv[0] + d];
+    t1[d] = cd[3 * fv[2] + d] - cd[3 * fv[0] + d];
+  }
+  cross3(t0, t1, n);
+  const double area = 0.5 * sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      A[4 * fv[a] + fv[b]] += w[0] * area * (a == b ? 2.0 : 1.0) / 12.0;
+}
+
+/* Q1 hex, facet e[0]: beta u v ds by 2 x 2 Gauss-Legendre on the bilinear
+ * facet map (exact on parallelogram facets), into the 8 x 8 cell matrix. */
+static void k_q1hex_robin(double* A, const double* w, const double* c,
+                          const double* cd, const int* e, const uint8_t* qq,
+                          void* u)
+{
+  (void)c;
+  (void)qq;
+  (void)u;
+  const int* fv = HEX_FV[e[0]];
+  const double g0 = 0.5 - 0.5 / sqrt(3.0), g1 = 0.5 + 0.5 / sqrt(3.0);
+  const double pts[2] = {g0, g1};
+  for (int qs = 0; qs < 2; ++qs)
+    for (int qt = 0; qt < 2; ++qt)
+    {
+      const double s = pts[qs], t = pts[qt];
+      const double phi[4] = {(1 - s) * (1 - t), s * (1 - t), (1 - s) * t, s * t};
+      const double ds[4] = {-(1 - t), 1 - t, -t, t};
+      const double dt[4] = {-(1 - s), -s, 1 - s, s};
+      double xs[3] = {0, 0, 0}, xt[3] = {0, 0, 0}, n[3];
+      for (int a = 0; a < 4; ++a)
+        for (int d = 0; d < 3; ++d)
+        {
+          xs[d] += ds[a] * cd[3 * fv[a] + d];
+          xt[d] += dt[a] * cd[3 * fv[a] + d];
+        }
+      cross3(xs, xt, n);
+      const double dA = 0.25 * sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b)
+          A[8 * fv[a] + fv[b]] += dA * w[0] * phi[a] * phi[b];
+    }
+}
+
 /* ---- P2/P3 tetrahedra: nodal basis from the monomial Vandermonde ---- */
 typedef struct
 {
@@ -772,6 +830,10 @@ orc_kernel_fn orc_get_kernel(int id)
     return k_p1tet_neumann;
   case ORC_K_Q1HEX_NEUMANN:
     return k_q1hex_neumann;
+  case ORC_K_P1TET_ROBIN_A:
+    return k_p1tet_robin;
+  case ORC_K_Q1HEX_ROBIN_A:
+    return k_q1hex_robin;
   default:
     return NULL;
   }
@@ -960,6 +1022,98 @@ void orc_assemble_vector_facets(int kernel, const double* x,
       for (int a = 0; a < bs; ++a)
         b[bs * dofs[i] + a] += be[bs * i + a];
   }
+}
+
+/* fem/assemble_matrix_impl.h:264-383 (assemble_entities, non-lifting) on
+ * exterior facets: for each (cell, local facet) pair in order, zero Ae, the
+ * kernel with the local facet as entity_local_index, zero BC rows and
+ * columns, insert (+=) into the CSR values. Returns -1 if an entry is not
+ * in the sparsity. */
+int orc_assemble_matrix_facets(int kernel, const double* x,
+                               const int32_t* x_dofmap, int npc,
+                               const int32_t* dofmap, int k, int bs,
+                               const double* coeffs, int cstride,
+                               const double* constants, const int32_t* pairs,
+                               int32_t npairs, const int8_t* bc0,
+                               const int8_t* bc1, const int64_t* row_ptr,
+                               const int32_t* cols, double* values)
+{
+  orc_kernel_fn fn = orc_get_kernel(kernel);
+  const int nd = bs * k;
+  double* Ae = (double*)malloc(sizeof(double) * nd * nd);
+  double cdofs[3 * 20];
+  const double zero = 0.0;
+  for (int32_t p = 0; p < npairs; ++p)
+  {
+    const int32_t c = pairs[2 * p];
+    const int lf = pairs[2 * p + 1];
+    for (int i = 0; i < npc; ++i)
+      memcpy(cdofs + 3 * i, x + 3 * (int64_t)x_dofmap[(int64_t)c * npc + i],
+             3 * sizeof(double));
+    memset(Ae, 0, sizeof(double) * nd * nd);
+    fn(Ae, coeffs ? coeffs + (int64_t)c * cstride : &zero,
+       constants ? constants : &zero, cdofs, &lf, NULL, NULL);
+    if (insert_cell(dofmap + (int64_t)c * k, k, bs, Ae, bc0, bc1, row_ptr, cols,
+                    values))
+    {
+      free(Ae);
+      return -1;
+    }
+  }
+  free(Ae);
+  return 0;
+}
+
+/* fem/assemble_vector_impl.h:361-414 (lift_bc) on exterior facets
+ * (assemble_entities<true>, fem/assemble_matrix_impl.h:316-322: facets whose
+ * cell has a BC column dof): b[row] -= Ae[row, col] alpha (g - x0) over the
+ * BC columns, facet by facet in order. */
+void orc_lift_bc_facets(int kernel, const double* x, const int32_t* x_dofmap,
+                        int npc, const int32_t* dofmap, int k, int bs,
+                        const double* coeffs, int cstride,
+                        const double* constants, const int32_t* pairs,
+                        int32_t npairs, const double* bc_values,
+                        const int8_t* bc_markers, const double* x0,
+                        double alpha, double* b)
+{
+  orc_kernel_fn fn = orc_get_kernel(kernel);
+  const int nd = bs * k;
+  double* Ae = (double*)malloc(sizeof(double) * nd * nd);
+  double cdofs[3 * 20];
+  const double zero = 0.0;
+  for (int32_t p = 0; p < npairs; ++p)
+  {
+    const int32_t c = pairs[2 * p];
+    const int lf = pairs[2 * p + 1];
+    const int32_t* dofs = dofmap + (int64_t)c * k;
+    int has = 0;
+    for (int j = 0; j < k && !has; ++j)
+      for (int a = 0; a < bs; ++a)
+        if (bc_markers[bs * dofs[j] + a])
+          has = 1;
+    if (!has)
+      continue;
+    for (int i = 0; i < npc; ++i)
+      memcpy(cdofs + 3 * i, x + 3 * (int64_t)x_dofmap[(int64_t)c * npc + i],
+             3 * sizeof(double));
+    memset(Ae, 0, sizeof(double) * nd * nd);
+    fn(Ae, coeffs ? coeffs + (int64_t)c * cstride : &zero,
+       constants ? constants : &zero, cdofs, &lf, NULL, NULL);
+    for (int j = 0; j < k; ++j)
+      for (int a = 0; a < bs; ++a)
+      {
+        const int32_t jj = bs * dofs[j] + a;
+        if (!bc_markers[jj])
+          continue;
+        const double xb = bc_values[jj];
+        const double x0v = x0 ? x0[jj] : 0.0;
+        for (int i = 0; i < k; ++i)
+          for (int m = 0; m < bs; ++m)
+            b[bs * dofs[i] + m]
+                -= Ae[(bs * i + m) * nd + bs * j + a] * alpha * (xb - x0v);
+      }
+  }
+  free(Ae);
 }
 
 /* fem/assemble_vector_impl.h:361-414 (lift_bc) through

@@ -35,6 +35,11 @@
  *   orc_facet_domain     brute-force facet vertex sets, facet-index order.
  *   orc_assemble_vector_facets  int_Gamma g ds closed forms: surface area,
  *                        one face, linear g (exact in P1/Q1).
+ *   orc_assemble_matrix_facets  sum of entries = beta |Gamma|, 1' A x =
+ *                        int_Gamma x ds = 3 on the unit cube, a unit facet's
+ *                        block = the facet mass matrix; BC rows/columns by
+ *                        brute force; orc_lift_bc_facets = dense column
+ *                        product of the facet matrix.
  *   orc_add_element_*    equal to orc_assemble_* fed the tabulated tensors;
  *                        a dense scatter of random tensors on tiny meshes.
  *   orc_assemble_*       P1 mass total = |Omega|, A 1 = 0, "Entry not in
@@ -141,7 +146,9 @@ enum
   ORC_K_P3TET_L = 7,   /* f v; c = [f] */
   ORC_K_P1TET_NEUMANN = 10, /* g v ds on an exterior facet; w = [g at 4 vertices] */
   ORC_K_Q1HEX_NEUMANN = 11, /* g v ds on an exterior facet; w = [g at 8 vertices] */
-  ORC_K_COUNT = 12
+  ORC_K_P1TET_ROBIN_A = 12, /* beta u v ds on an exterior facet; w = [beta] */
+  ORC_K_Q1HEX_ROBIN_A = 13, /* beta u v ds on an exterior facet; w = [beta] */
+  ORC_K_COUNT = 14
 };
 orc_kernel_fn orc_get_kernel(int id);
 void orc_tabulate(int id, double* A, const double* w, const double* c,
@@ -166,6 +173,25 @@ int orc_add_element_matrices(int32_t num_cells, const int32_t* dofmap, int k,
                              const int32_t* cols, double* values);
 void orc_add_element_vectors(int32_t num_cells, const int32_t* dofmap, int k,
                              int bs, const double* e, double* b);
+/* Exterior facet matrices over (cell, local facet) pairs in the given
+ * order, added into the CSR values with BC rows/columns zeroed; -1 if an
+ * entry is not in the sparsity. */
+int orc_assemble_matrix_facets(int kernel, const double* x,
+                               const int32_t* x_dofmap, int npc,
+                               const int32_t* dofmap, int k, int bs,
+                               const double* coeffs, int cstride,
+                               const double* constants, const int32_t* pairs,
+                               int32_t npairs, const int8_t* bc0,
+                               const int8_t* bc1, const int64_t* row_ptr,
+                               const int32_t* cols, double* values);
+/* Lifting by exterior facet matrices (lift_bc over the facet integral). */
+void orc_lift_bc_facets(int kernel, const double* x, const int32_t* x_dofmap,
+                        int npc, const int32_t* dofmap, int k, int bs,
+                        const double* coeffs, int cstride,
+                        const double* constants, const int32_t* pairs,
+                        int32_t npairs, const double* bc_values,
+                        const int8_t* bc_markers, const double* x0,
+                        double alpha, double* b);
 /* Exterior facet loads over (cell, local facet) pairs in the given order. */
 void orc_assemble_vector_facets(int kernel, const double* x,
                                 const int32_t* x_dofmap, int npc,

@@ -1,13 +1,16 @@
-"""Exterior facet loads int_Gamma g v ds (fem::assemble_vector over an
-exterior_facet integral, fem/assemble_vector_impl.h:118-174) on P1
-tetrahedra and Q1 hexahedra against the oracle, on all exterior facets and
-on one face; g is a P1/Q1 field given at the vertices."""
+"""Exterior facet integrals on P1 tetrahedra and Q1 hexahedra against the
+oracle: loads int_Gamma g v ds (fem::assemble_vector over an exterior_facet
+integral, fem/assemble_vector_impl.h:118-174) with g a P1/Q1 field given at
+the vertices, on all exterior facets and on one face; facet matrices beta u
+v ds (assemble_entities, fem/assemble_matrix_impl.h:264-383) alone and after
+a cell integral, with BC rows/columns, and their lifting."""
 
 import numpy as np
 import pytest
 
 import oracle as O
-from dolfinx_amd import fem, la, mesh
+from dolfinx_amd import fem, la, mesh, solvers
+from dolfinx_amd._lib import DfxError
 
 from conftest import rel_err
 
@@ -47,3 +50,112 @@ def test_facet_pairs_out_of_range_raise():
     nc = pm.topology.cells.shape[0]
     with pytest.raises(Exception, match="out of range"):
         fem.form("p1_neumann", V, np.ones((nc, 4)), facets=np.array([[0, 4]], np.int32))
+
+
+ROBIN = [(O.TET, "p1_robin", O.K_P1TET_ROBIN_A, "p1_stiffness_reaction", O.K_P1TET_A),
+         (O.HEX, "q1_robin", O.K_Q1HEX_ROBIN_A, "q1_stiffness_reaction", O.K_Q1HEX_A)]
+
+
+def robin_setup(ct, n=(4, 3, 3)):
+    om = O.Mesh(n, ct)
+    pm = mesh.create_box([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], n, mesh.CellType(ct))
+    V = fem.functionspace(pm, ("Lagrange", 1))
+    dm, nd = om.dofmap(1)
+    assert np.array_equal(V.dofmap.list, dm)
+    rp, cols = O.sparsity(dm, dm, nd)
+    rng = np.random.default_rng(21)
+    beta_in = rng.uniform(0.5, 2.0, om.num_cells)
+    kap_in = rng.uniform(1.0, 3.0, om.num_cells)
+    return om, pm, V, dm, nd, rp, cols, beta_in, kap_in
+
+
+@pytest.mark.parametrize("ct,rname,kR,aname,kA", ROBIN)
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 2e-6)])
+def test_robin_matrix_matches_oracle(ct, rname, kR, aname, kA, dtype, tol):
+    """beta u v ds on every exterior facet, BC rows/columns of the x = 0
+    face zeroed, unit diagonal; then the same facets added after the cell
+    integral (a form with two integrals, assembled cells first)."""
+    om, pm, V, dm, nd, rp, cols, beta_in, kap_in = robin_setup(ct)
+    pairs = fem.facet_domain(pm)
+    beta = fem.pack_cellwise(pm, beta_in)
+    aR = fem.form(rname, V, beta, None, dtype, facets=pairs)
+    bn = om.boundary_nodes(1, dm, marker=1, axis=0, value=0.0)
+    bc = fem.dirichletbc(V, bn)
+    mark = np.zeros(nd, np.int8)
+    mark[bn] = 1
+    A = fem.assemble_matrix(aR, [bc])
+    co = beta_in[om.original_cell_index].reshape(-1, 1)
+    ref = O.assemble_matrix_facets(kR, om.x, om.cells, dm, co, pairs, mark, mark, rp, cols)
+    O.set_diagonal(ref, bn, 1, 1.0, rp, cols)
+    assert np.array_equal(A.indptr, rp) and np.array_equal(A.indices, cols)
+    assert rel_err(A.data, ref) <= tol
+    # cell integral + facet integral on the face x = 1
+    right = fem.facet_domain(pm, 0, 1.0)
+    aC = fem.form(aname, V, fem.pack_cellwise(pm, kap_in), [0.25], dtype)
+    aR1 = fem.form(rname, V, beta, None, dtype, facets=right)
+    A2 = fem.assemble_matrix((aR1, aC), [bc])
+    kap = kap_in[om.original_cell_index].reshape(-1, 1)
+    ref2 = O.assemble_matrix(kA, om.x, om.cells, dm, 1, kap, np.array([0.25]), mark, mark, rp,
+                             cols)
+    O.assemble_matrix_facets(kR, om.x, om.cells, dm, co, right, mark, mark, rp, cols, ref2)
+    O.set_diagonal(ref2, bn, 1, 1.0, rp, cols)
+    assert rel_err(A2.data, ref2) <= tol
+
+
+@pytest.mark.parametrize("ct,rname,kR,aname,kA", ROBIN)
+def test_robin_lifting_matches_oracle(ct, rname, kR, aname, kA):
+    """apply_lifting over a form with a cell and a facet integral (nonzero g,
+    x0, alpha) equals the oracle's lift_bc over both integrals."""
+    om, pm, V, dm, nd, rp, cols, beta_in, kap_in = robin_setup(ct, (3, 4, 3))
+    pairs = fem.facet_domain(pm)
+    aC = fem.form(aname, V, fem.pack_cellwise(pm, kap_in), [0.25])
+    aR = fem.form(rname, V, fem.pack_cellwise(pm, beta_in), facets=pairs)
+    bn = om.boundary_nodes(1, dm, marker=1, axis=2, value=0.0)
+    g = np.zeros(nd)
+    g[bn] = 1.0 + np.sin(np.arange(len(bn)))
+    mark = np.zeros(nd, np.int8)
+    mark[bn] = 1
+    x0 = np.random.default_rng(4).standard_normal(nd)
+    b0 = np.random.default_rng(5).standard_normal(nd)
+    b = la.Vector.from_array(b0)
+    fem.apply_lifting(b, [(aC, aR)], [[fem.dirichletbc(V, bn, g)]], [la.Vector.from_array(x0)],
+                      alpha=0.7)
+    kap = kap_in[om.original_cell_index].reshape(-1, 1)
+    co = beta_in[om.original_cell_index].reshape(-1, 1)
+    ref = O.lift_bc(b0.copy(), kA, om.x, om.cells, dm, 1, kap, np.array([0.25]), g, mark, x0=x0,
+                    alpha=0.7)
+    O.lift_bc_facets(ref, kR, om.x, om.cells, dm, co, pairs, g, mark, x0=x0, alpha=0.7)
+    assert rel_err(b.array, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("ct,rname,kR,aname,kA", ROBIN)
+def test_robin_problem_has_constant_solution(ct, rname, kR, aname, kA):
+    """-div(kappa grad u) = 0 with kappa du/dn + beta u = beta on the whole
+    boundary has the solution u = 1, which P1 and Q1 hold exactly: CG on
+    the assembled system returns 1 at every node."""
+    om, pm, V, dm, nd, rp, cols, beta_in, kap_in = robin_setup(ct, (5, 4, 6))
+    pairs = fem.facet_domain(pm)
+    beta = fem.pack_cellwise(pm, beta_in)
+    a = (fem.form(aname, V, fem.pack_cellwise(pm, kap_in), [0.0]),
+         fem.form(rname, V, beta, facets=pairs))
+    nv = 4 if ct == O.TET else 8
+    L = fem.form("p1_neumann" if ct == O.TET else "q1_neumann", V, np.repeat(beta, nv, axis=1),
+                 facets=pairs)
+    A, b = fem.assemble_system(a, L, [])
+    x = la.Vector(nd)
+    res = solvers.cg(A, b, x, rtol=1e-12, max_it=5000)
+    assert res.converged
+    assert np.abs(x.array - 1.0).max() <= 1e-8
+
+
+def test_robin_forms_reject_misuse():
+    pm = mesh.create_box([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], (2, 2, 2), mesh.CellType.hexahedron)
+    V = fem.functionspace(pm, ("Lagrange", 1))
+    nc = pm.topology.cells.shape[0]
+    aR = fem.form("q1_robin", V, np.ones((nc, 1)), facets=fem.facet_domain(pm))
+    with pytest.raises(DfxError, match="linear form"):
+        fem.assemble_vector(aR)
+    diag = la.MatrixCSR(np.arange(V.num_dofs + 1, dtype=np.int64),
+                        np.arange(V.num_dofs, dtype=np.int32))
+    with pytest.raises(DfxError, match="Entry not in sparsity"):
+        fem.assemble_matrix(aR, [], 1.0, diag)

@@ -597,3 +597,85 @@ def test_cg_against_scipy(jacobi):
     # the reference CG raises after maxit without convergence
     _, it = O.cg(rp, cols, A, 1, b, kmax=2, rtol=1e-14, jacobi=jacobi)
     assert it == -1
+
+
+@pytest.mark.parametrize("ct,kR", [(O.TET, O.K_P1TET_ROBIN_A), (O.HEX, O.K_Q1HEX_ROBIN_A)])
+def test_facet_matrices_closed_forms(ct, kR):
+    """beta u v ds over the unit cube's surface: 1' A 1 = beta |Gamma| = 6
+    beta; 1' A x = beta int_Gamma x ds = 3 beta (x is in P1 and Q1);
+    1' A y = 3 beta on one face y = 1 only for its area 1; the entries of a
+    single unit facet form the facet mass matrix (|F| (1 + d_ab) / 12 on a
+    triangle, the tensor product of [2 1; 1 2] / 6 on a square)."""
+    n = (3, 2, 4)
+    m = O.Mesh(n, ct)
+    dm, nd = m.dofmap(1)
+    rp, cols = O.sparsity(dm, dm, nd)
+    beta = 2.5
+    co = np.full((m.num_cells, 1), beta)
+    A = O.assemble_matrix_facets(kR, m.x, m.cells, dm, co, m.facet_domain(), None, None, rp, cols)
+    D = sp.csr_matrix((A, cols, rp), shape=(nd, nd))
+    one = np.ones(nd)
+    assert np.isclose(one @ (D @ one), 6.0 * beta, rtol=1e-14)
+    assert np.isclose(one @ (D @ m.x[:, 0]), 3.0 * beta, rtol=1e-14)
+    assert abs(D - D.T).max() <= 1e-15
+    # the face y = 1: 1' A 1 = beta (its area), 1' A z = beta / 2
+    A1 = O.assemble_matrix_facets(kR, m.x, m.cells, dm, co, m.facet_domain(1, 1.0), None, None,
+                                  rp, cols)
+    D1 = sp.csr_matrix((A1, cols, rp), shape=(nd, nd))
+    assert np.isclose(one @ (D1 @ one), beta, rtol=1e-14)
+    assert np.isclose(one @ (D1 @ m.x[:, 2]), beta / 2, rtol=1e-14)
+    off = m.x[:, 1] != 1.0
+    assert np.all(D1[off].data == 0.0)
+    # one facet of a single cell with unit edges
+    s = O.Mesh((1, 1, 1), ct)
+    sdm, snd = s.dofmap(1)
+    srp, scols = O.sparsity(sdm, sdm, snd)
+    pairs = s.facet_domain(2, 0.0)[:1]
+    As = O.assemble_matrix_facets(kR, s.x, s.cells, sdm, np.ones((s.num_cells, 1)), pairs, None,
+                                  None, srp, scols)
+    Ds = sp.csr_matrix((As, scols, srp), shape=(snd, snd)).toarray()
+    table = TET_FACETS if ct == O.TET else HEX_FACETS
+    fv = sdm[pairs[0][0]][list(table[pairs[0][1]])]
+    F = Ds[np.ix_(fv, fv)]
+    if ct == O.TET:
+        area = 0.5 * np.linalg.norm(np.cross(s.x[fv[1]] - s.x[fv[0]], s.x[fv[2]] - s.x[fv[0]]))
+        assert np.allclose(F, area * (np.ones((3, 3)) + np.eye(3)) / 12, rtol=1e-14, atol=0)
+    else:
+        M1 = np.array([[2.0, 1.0], [1.0, 2.0]]) / 6
+        # facet vertices in tensor order: (0,0), (1,0), (0,1), (1,1)
+        assert np.allclose(F, np.kron(M1, M1), rtol=1e-14, atol=0)
+    mask = np.ones(snd, bool)
+    mask[fv] = False
+    assert np.all(Ds[mask] == 0.0) and np.all(Ds[:, mask] == 0.0)
+
+
+@pytest.mark.parametrize("ct,kR", [(O.TET, O.K_P1TET_ROBIN_A), (O.HEX, O.K_Q1HEX_ROBIN_A)])
+def test_facet_matrices_bcs_and_lifting(ct, kR):
+    """With BC markers the facet matrices lose exactly the marked rows and
+    columns (entry by entry equal to the unmarked assembly elsewhere);
+    lifting = -alpha A[:, bc] (g - x0) with A assembled without BCs; an
+    entry outside the pattern raises."""
+    m = O.Mesh((3, 3, 2), ct)
+    dm, nd = m.dofmap(1)
+    rp, cols = O.sparsity(dm, dm, nd)
+    rng = np.random.default_rng(11)
+    co = rng.uniform(0.5, 2.0, (m.num_cells, 1))
+    pairs = m.facet_domain()
+    A0 = O.assemble_matrix_facets(kR, m.x, m.cells, dm, co, pairs, None, None, rp, cols)
+    bn = m.boundary_nodes(1, dm, marker=1, axis=0, value=0.0)
+    mark = np.zeros(nd, np.int8)
+    mark[bn] = 1
+    Ab = O.assemble_matrix_facets(kR, m.x, m.cells, dm, co, pairs, mark, mark, rp, cols)
+    rows = np.repeat(np.arange(nd), np.diff(rp))
+    keep = (mark[rows] == 0) & (mark[cols] == 0)
+    assert np.array_equal(Ab[keep], A0[keep]) and np.all(Ab[~keep] == 0.0)
+    g = np.zeros(nd)
+    g[bn] = np.cos(np.arange(len(bn)))
+    x0 = rng.standard_normal(nd)
+    b = O.lift_bc_facets(np.zeros(nd), kR, m.x, m.cells, dm, co, pairs, g, mark, x0=x0, alpha=0.6)
+    D = sp.csr_matrix((A0, cols, rp), shape=(nd, nd))
+    w = np.where(mark == 1, g - x0, 0.0)
+    assert np.allclose(b, -0.6 * (D @ w), rtol=0, atol=1e-14 * np.abs(D @ w).max())
+    with pytest.raises(RuntimeError, match="Entry not in sparsity"):
+        O.assemble_matrix_facets(kR, m.x, m.cells, dm, co, pairs, None, None,
+                                 np.arange(nd + 1, dtype=np.int64), np.arange(nd, dtype=np.int32))
