@@ -1,0 +1,214 @@
+"""XDMF files with ASCII ("XML") data items: meshes and nodal functions.
+
+Follows io::XDMFFile (cpp/dolfinx/io/XDMFFile.cpp:31-300) with
+Encoding::ASCII:
+- write_mesh: a Uniform Grid named after the mesh holding a Topology (the
+  geometry dofmap rows in VTK node order, io/xdmf_mesh.cpp:60-166) and a
+  Geometry "XYZ" (io/xdmf_mesh.cpp:170-222); floating-point data items are
+  written with 16 significant digits, a row per line
+  (io/xdmf_utils.h:100-116);
+- write_function: a Temporal Collection Grid named after the function, a
+  Uniform Grid per call that includes the mesh's Topology and Geometry by
+  xpointer, its Time and an Attribute at the nodes (degree-1 spaces)
+  (io/XDMFFile.cpp:244-293, io/xdmf_function.cpp:131-215);
+- read_mesh: the Topology and Geometry of the named Grid, VTK node order
+  back to DOLFINx order (io/cells.cpp:402-420, 534-569), then
+  mesh::create_mesh.
+HDF5 data items need the HDF5 library, which this package does not use:
+Encoding.HDF5 raises.
+"""
+
+from __future__ import annotations
+
+import enum
+import xml.etree.ElementTree as ET
+
+import numpy as np
+
+from .mesh import CellType, Mesh, create_mesh
+
+_XI = "https://www.w3.org/2001/XInclude"
+
+# DOLFINx local vertex i is VTK node _VTK[ct][i] (an involution for these
+# cells)
+_VTK = {CellType.tetrahedron: np.array([0, 1, 2, 3]),
+        CellType.hexahedron: np.array([0, 1, 3, 2, 4, 5, 7, 6])}
+_TOPOLOGY = {CellType.tetrahedron: "Tetrahedron", CellType.hexahedron: "Hexahedron"}
+
+
+class Encoding(enum.Enum):
+    HDF5 = 0
+    ASCII = 1
+
+
+def _fmt_float(v) -> str:
+    # std::format("{:.16}") = general format with 16 significant digits
+    return "%.16g" % float(v)
+
+
+def _fmt_time(t) -> str:
+    # boost::lexical_cast<std::string>(double): 17 significant digits
+    return "%.17g" % float(t)
+
+
+def _data_item(parent, a, number_type=None):
+    a = np.asarray(a)
+    a2 = a.reshape(a.shape[0], -1)
+    item = ET.SubElement(parent, "DataItem")
+    item.set("Dimensions", f"{a2.shape[0]} {a2.shape[1]}")
+    if number_type:
+        item.set("NumberType", number_type)
+    item.set("Format", "XML")
+    if np.issubdtype(a2.dtype, np.floating):
+        rows = (" ".join(_fmt_float(v) for v in r) for r in a2)
+    else:
+        rows = (" ".join(str(int(v)) for v in r) for r in a2)
+    item.text = "".join(r + "\n" for r in rows)
+    return item
+
+
+def _read_data_item(item, dtype):
+    fmt = item.get("Format", "XML")
+    if fmt != "XML":
+        raise NotImplementedError(f"DataItem Format {fmt}: only XML (ASCII) data is read")
+    dims = [int(d) for d in item.get("Dimensions").split()]
+    vals = np.array(item.text.split(), dtype=dtype)
+    return vals.reshape(dims if len(dims) > 1 else (dims[0], 1))
+
+
+class XDMFFile:
+    """io::XDMFFile over one file, ASCII encoding; modes "r", "w", "a"."""
+
+    Encoding = Encoding
+
+    def __init__(self, filename, mode="r", encoding=Encoding.ASCII):
+        if encoding != Encoding.ASCII:
+            raise NotImplementedError(
+                "HDF5 data items need the HDF5 library; use XDMFFile.Encoding.ASCII")
+        if mode not in ("r", "w", "a"):
+            raise ValueError(f"file mode {mode!r}")
+        self.filename, self.mode = str(filename), mode
+        if mode == "w":
+            self._root = ET.Element("Xdmf", {"Version": "3.0", "xmlns:xi": _XI})
+            ET.SubElement(self._root, "Domain")
+        else:
+            self._root = ET.parse(self.filename).getroot()
+            # keep the XInclude prefix literal, as written
+            for e in self._root.iter(f"{{{_XI}}}include"):
+                e.tag = "xi:include"
+            self._root.set("xmlns:xi", _XI)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def close(self):
+        self._root = None
+
+    # -- writing --------------------------------------------------------
+    def _domain(self):
+        if self.mode == "r":
+            raise RuntimeError("file opened for reading")
+        d = self._root.find("Domain")
+        if d is None:
+            raise RuntimeError("XML node '/Xdmf/Domain' not found.")
+        return d
+
+    def _save(self):
+        tree = ET.ElementTree(self._root)
+        ET.indent(tree, space="  ")
+        with open(self.filename, "w") as f:
+            f.write('<?xml version="1.0"?>\n<!DOCTYPE Xdmf SYSTEM "Xdmf.dtd" []>\n')
+            f.write(ET.tostring(self._root, encoding="unicode"))
+            f.write("\n")
+
+    def write_mesh(self, mesh: Mesh, name: str | None = None) -> None:
+        """A Uniform Grid with the mesh's Topology and Geometry."""
+        name = name or getattr(mesh, "name", "mesh")
+        ct = CellType(mesh.topology.cell_type)
+        grid = ET.SubElement(self._domain(), "Grid", {"Name": name, "GridType": "Uniform"})
+        dm = np.asarray(mesh.geometry.dofmap)
+        topo = ET.SubElement(grid, "Topology", {"TopologyType": _TOPOLOGY[ct]})
+        topo.set("NumberOfElements", str(dm.shape[0]))
+        topo.set("NodesPerElement", str(dm.shape[1]))
+        _data_item(topo, dm[:, _VTK[ct]].astype(np.int64), "Int")
+        geo = ET.SubElement(grid, "Geometry", {"GeometryType": "XYZ"})
+        _data_item(geo, np.asarray(mesh.geometry.x, np.float64))
+        self._save()
+
+    def write_function(self, u, t: float = 0.0, mesh_name: str | None = None) -> None:
+        """One time step of u (a fem.Function on a degree-1 Lagrange space,
+        scalar or blocked) in the Temporal Collection named u.name,
+        referring to the mesh Grid written before."""
+        V = u.function_space
+        mesh = V.mesh
+        mesh_name = mesh_name or getattr(mesh, "name", "mesh")
+        dom = self._domain()
+        if dom.find(f"Grid[@GridType='Uniform'][@Name='{mesh_name}']") is None:
+            raise RuntimeError(f"No mesh '{mesh_name}' found. Write the mesh before functions.")
+        tg = dom.find(f"Grid[@GridType='Collection'][@Name='{u.name}']")
+        if tg is None:
+            tg = ET.SubElement(dom, "Grid", {"Name": u.name, "GridType": "Collection",
+                                             "CollectionType": "Temporal"})
+        grid = ET.SubElement(tg, "Grid", {"Name": u.name, "GridType": "Uniform"})
+        ptr = (f"xpointer(/Xdmf/Domain/Grid[@GridType='Uniform'][@Name='{mesh_name}']"
+               "/*[self::Topology or self::Geometry])")
+        ET.SubElement(grid, "xi:include", {"xpointer": ptr})
+        ET.SubElement(grid, "Time", {"Value": _fmt_time(t)})
+        vals, centre, ncomp = _point_or_cell_values(u)
+        attr = ET.SubElement(grid, "Attribute", {
+            "Name": u.name, "AttributeType": {1: "Scalar", 3: "Vector"}.get(ncomp, "Matrix"),
+            "Center": centre})
+        _data_item(attr, vals.reshape(-1, ncomp))
+        self._save()
+
+    # -- reading --------------------------------------------------------
+    def _grid(self, name):
+        dom = self._root.find("Domain")
+        g = None if dom is None else dom.find(f"Grid[@Name='{name}']")
+        if g is None:
+            raise RuntimeError(f"XML node '/Xdmf/Domain/Grid[@Name='{name}']' not found.")
+        return g
+
+    def read_topology_data(self, name="mesh"):
+        """(cell type, cells in DOLFINx node order (int64))."""
+        topo = self._grid(name).find("Topology")
+        tt = topo.get("TopologyType") or topo.get("Type")
+        types = {v.lower(): k for k, v in _TOPOLOGY.items()}
+        if tt is None or tt.lower() not in types:
+            raise NotImplementedError(f"TopologyType {tt}: tetrahedra and hexahedra are read")
+        ct = types[tt.lower()]
+        cells = _read_data_item(topo.find("DataItem"), np.int64)
+        return ct, np.ascontiguousarray(cells[:, _VTK[ct]])
+
+    def read_geometry_data(self, name="mesh"):
+        geo = self._grid(name).find("Geometry")
+        gt = geo.get("GeometryType") or geo.get("Type")
+        if gt != "XYZ":
+            raise NotImplementedError(f"GeometryType {gt}: XYZ is read")
+        return _read_data_item(geo.find("DataItem"), np.float64).reshape(-1, 3)
+
+    def read_mesh(self, name="mesh") -> Mesh:
+        """mesh::create_mesh on the named Grid's cells and points."""
+        ct, cells = self.read_topology_data(name)
+        x = self.read_geometry_data(name)
+        m = create_mesh(ct, cells, x)
+        m.name = name
+        return m
+
+
+def _point_or_cell_values(u):
+    V = u.function_space
+    a = np.asarray(u.x.array)
+    bs = V.dofmap.bs
+    if V.degree != 1:
+        raise RuntimeError("Degree of output Function must be same as mesh degree. "
+                           "Maybe the Function needs to be interpolated?")
+    # degree 1: the space's dofs are the geometry nodes
+    dm = np.asarray(V.dofmap.list)
+    xd = np.asarray(V.mesh.geometry.dofmap)
+    out = np.zeros((V.mesh.geometry.x.shape[0], bs), a.dtype)
+    out[xd.ravel()] = a.reshape(-1, bs)[dm.ravel()]
+    return out, "Node", bs
