@@ -19,7 +19,7 @@ F32, F64 = 0, 1
 TET, HEX = 0, 1
 (K_P1TET_A, K_P1TET_L, K_Q1HEX_A, K_Q1HEX_L, K_P2TET_ELAST_A, K_P2TET_ELAST_L,
  K_P3TET_A, K_P3TET_L, K_ELEMENT_A, K_ELEMENT_L, K_P1TET_NEUMANN, K_Q1HEX_NEUMANN,
- K_P1TET_ROBIN_A, K_Q1HEX_ROBIN_A) = range(14)
+ K_P1TET_ROBIN_A, K_Q1HEX_ROBIN_A, K_P1TET_GRADJUMP_A) = range(15)
 
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
@@ -65,6 +65,9 @@ SIGNATURES = {
     "dfx_mesh_create": (C.c_int, [C.c_int, I32, P, I32, P, PP]),
     "dfx_facet_domain": (C.c_int, [P, C.c_int, C.c_int, C.c_double, P, P]),
     "dfx_form_create_facets": (C.c_int, [P, I32, P, PP]),
+    "dfx_form_create_interior_facets": (C.c_int, [P, I32, P, PP]),
+    "dfx_interior_facet_domain": (C.c_int, [P, P, P]),
+    "dfx_sparsity_create_facets": (C.c_int, [P, P, I32, P, PP]),
     "dfx_mesh_destroy": (C.c_int, [P]),
     "dfx_mesh_sizes": (C.c_int, [P, P, P, P]),
     "dfx_mesh_copy": (C.c_int, [P, P, P, P, P]),

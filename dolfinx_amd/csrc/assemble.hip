@@ -84,6 +84,9 @@ bool kernel_info(int id, KernelInfo& ki)
   case DFX_K_Q1HEX_ROBIN_A:
     ki = {DFX_HEX, 8, 8, 1, true};
     return true;
+  case DFX_K_P1TET_GRADJUMP_A:
+    ki = {DFX_TET, 4, 4, 1, true};
+    return true;
   case DFX_K_ELEMENT_A:
     ki = {-1, 0, 0, 1, true}; // any element: k from the descriptor
     return true;
@@ -106,6 +109,9 @@ bool is_facet_matrix_kernel(int id)
 {
   return id == DFX_K_P1TET_ROBIN_A || id == DFX_K_Q1HEX_ROBIN_A;
 }
+
+/// Kernels of interior facet integrals.
+bool is_interior_kernel(int id) { return id == DFX_K_P1TET_GRADJUMP_A; }
 
 // ---------------------------------------------------------------------
 // Geometry helpers
@@ -1807,6 +1813,194 @@ __global__ void __launch_bounds__(kBlock)
     bvec[r] = static_cast<T>(double(bvec[r]) - alpha * s);
 }
 
+// ---------------------------------------------------------------------
+// Interior facet integrals on P1 tetrahedra: the macro element of facet e
+// has the 4 dofs of its first cell then the 4 of its second; its 8 x 8
+// matrix is added entry by entry like mat_set(dmapjoint0, dmapjoint1, Ae)
+// (shared dofs appear twice). Incidences finc = 8 e + macro index.
+// ---------------------------------------------------------------------
+
+/// P1 gradients of a tetrahedron (rows of the inverse Jacobian, lambda_0 =
+/// 1 - lambda_1 - lambda_2 - lambda_3).
+template <typename T>
+__device__ inline void p1_gradients(const T X[4][3], T g[4][3])
+{
+  T J[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      J[r][a] = X[a + 1][r] - X[0][r];
+  const T c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const T c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const T c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const T det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  const T id = T(1) / det;
+  // K = J^-1: row a is grad lambda_{a+1}
+  const T K[3][3] = {{c00 * id, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id,
+                      (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id},
+                     {c01 * id, (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id,
+                      (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id},
+                     {c02 * id, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id,
+                      (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id}};
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    g[1][d] = K[0][d];
+    g[2][d] = K[1][d];
+    g[3][d] = K[2][d];
+    g[0][d] = -(K[0][d] + K[1][d] + K[2][d]);
+  }
+}
+
+/// One thread per interior facet e: out[64 e + 8 a + b] = gamma |F| j_a j_b
+/// with j the jump of the normal derivative of the macro basis functions,
+/// grad phi.n on the first cell and -grad phi.n on the second (n the unit
+/// normal of the facet).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    interior_matrix_k(std::int32_t nfacets, const std::int32_t* __restrict__ quads,
+                      const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
+                      T gamma, T* __restrict__ out)
+{
+  const std::int32_t e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= nfacets)
+    return;
+  T X[2][4][3];
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+  {
+    const std::int32_t c = quads[4 * e + 2 * s];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+    {
+      const std::int64_t v = xdm[std::int64_t(c) * 4 + a];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        X[s][a][d] = x[3 * v + d];
+    }
+  }
+  const int lf = quads[4 * e + 1];
+  const int f0 = kTetFacetV[lf][0], f1 = kTetFacetV[lf][1], f2 = kTetFacetV[lf][2];
+  T t0[3], t1[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+  {
+    t0[d] = X[0][f1][d] - X[0][f0][d];
+    t1[d] = X[0][f2][d] - X[0][f0][d];
+  }
+  T n[3] = {t0[1] * t1[2] - t0[2] * t1[1], t0[2] * t1[0] - t0[0] * t1[2],
+            t0[0] * t1[1] - t0[1] * t1[0]};
+  const T nrm = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    n[d] /= nrm;
+  const T w = gamma * (T(0.5) * nrm);
+  T jv[8];
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+  {
+    T g[4][3];
+    p1_gradients<T>(X[s], g);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+    {
+      const T dn = g[a][0] * n[0] + g[a][1] * n[1] + g[a][2] * n[2];
+      jv[4 * s + a] = s == 0 ? dn : -dn;
+    }
+  }
+  T* o = out + 64 * std::int64_t(e);
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      o[8 * a + b] = w * jv[a] * jv[b];
+}
+
+/// A[r, :] += the macro-element rows at row r, facet by facet and macro
+/// index by macro index (one thread per row); BC rows and columns add zero;
+/// slots by binary search, an absent one sets *err.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    interior_rows_k(std::int32_t nplan, const std::int32_t* __restrict__ off,
+                    const std::int32_t* __restrict__ finc, const std::int32_t* __restrict__ quads,
+                    const std::int32_t* __restrict__ dofmap, const T* __restrict__ fm,
+                    const std::int8_t* __restrict__ mark, const std::int32_t* __restrict__ row_ptr,
+                    const std::int32_t* __restrict__ cols, T* __restrict__ values,
+                    int* __restrict__ err)
+{
+  const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nplan)
+    return;
+  const std::int32_t p1 = off[r + 1];
+  if (off[r] == p1)
+    return;
+  const std::int32_t rb = row_ptr[r], re = row_ptr[r + 1];
+  const bool rmask = mark && mark[r];
+  for (std::int32_t p = off[r]; p < p1; ++p)
+  {
+    const std::int32_t e = finc[p] >> 3;
+    const int i = finc[p] & 7;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+    {
+      const std::int32_t c = quads[4 * e + 2 * (j >> 2)];
+      const std::int32_t col = dofmap[std::int64_t(c) * 4 + (j & 3)];
+      std::int32_t lo = rb, hi = re;
+      while (lo < hi)
+      {
+        const std::int32_t mid = (lo + hi) >> 1;
+        if (cols[mid] < col)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      if (lo == re || cols[lo] != col)
+      {
+        *err = 1;
+        continue;
+      }
+      const bool zero = rmask || (mark && mark[col]);
+      values[lo] += zero ? T(0) : fm[64 * std::int64_t(e) + 8 * i + j];
+    }
+  }
+}
+
+/// Lifting by the macro-element matrices: b[r] -= alpha sum over the row's
+/// incidences and the macro element's BC columns j of F_ij (g_j - x0_j).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    interior_lift_rows_k(std::int32_t nplan, const std::int32_t* __restrict__ off,
+                         const std::int32_t* __restrict__ finc,
+                         const std::int32_t* __restrict__ quads,
+                         const std::int32_t* __restrict__ dofmap, const T* __restrict__ fm,
+                         const std::int8_t* __restrict__ mark, const double* __restrict__ g,
+                         const T* __restrict__ x0, double alpha, T* __restrict__ bvec)
+{
+  const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nplan)
+    return;
+  double s = 0;
+  for (std::int32_t p = off[r]; p < off[r + 1]; ++p)
+  {
+    const std::int32_t e = finc[p] >> 3;
+    const int i = finc[p] & 7;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+    {
+      const std::int32_t c = quads[4 * e + 2 * (j >> 2)];
+      const std::int32_t col = dofmap[std::int64_t(c) * 4 + (j & 3)];
+      if (!mark[col])
+        continue;
+      const double gv = g ? g[col] : 0.0;
+      const double xv = x0 ? double(x0[col]) : 0.0;
+      s += double(fm[64 * std::int64_t(e) + 8 * i + j]) * (gv - xv);
+    }
+  }
+  if (s != 0.0)
+    bvec[r] = static_cast<T>(double(bvec[r]) - alpha * s);
+}
+
 /// b[r] += the facet contributions at row r in facet order (one thread per
 /// row over its facet incidences finc = 4 e + facet vertex).
 template <typename T>
@@ -3318,6 +3512,51 @@ void facet_matrix_rows(const dfx_form& f, dfx_csr& A, const dfx_bc* bc)
     throw Error(DFX_ERR_PATTERN, "Entry not in sparsity");
 }
 
+/// Interior facet macro-element matrices into f.fvals.
+template <typename T>
+void interior_matrices(const dfx_form& f)
+{
+  interior_matrix_k<T><<<blocks_for(f.nfacet), kBlock, 0, stream()>>>(
+      f.nfacet, f.facets, static_cast<const T*>(f.x), f.x_dofmap, static_cast<T>(f.constants[0]),
+      static_cast<T*>(f.fvals));
+  check_launch("interior facet matrices");
+}
+
+/// Interior facet integral added into A row by row.
+template <typename T>
+void interior_rows(const dfx_form& f, dfx_csr& A, const dfx_bc* bc)
+{
+  if (f.nfacet == 0)
+    return;
+  interior_matrices<T>(f);
+  int* err = static_cast<int*>(dmalloc(sizeof(int)));
+  DFX_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
+  const std::int32_t np = f.max_dof + 1;
+  interior_rows_k<T><<<blocks_for(np), kBlock, 0, stream()>>>(
+      np, f.finc_off, f.finc, f.facets, f.dofmap, static_cast<const T*>(f.fvals),
+      bc ? bc->marker : nullptr, A.row_ptr, A.cols, static_cast<T*>(A.values), err);
+  check_launch("interior facet rows");
+  int h = 0;
+  d2h(&h, err, sizeof(int));
+  dfree(err);
+  if (h)
+    throw Error(DFX_ERR_PATTERN, "Entry not in sparsity");
+}
+
+/// Lifting by the interior facet matrices.
+template <typename T>
+void interior_lift_rows(const dfx_form& f, const dfx_bc& bc, const T* x0, double alpha, T* b)
+{
+  if (f.nfacet == 0)
+    return;
+  interior_matrices<T>(f);
+  const std::int32_t np = f.max_dof + 1;
+  interior_lift_rows_k<T><<<blocks_for(np), kBlock, 0, stream()>>>(
+      np, f.finc_off, f.finc, f.facets, f.dofmap, static_cast<const T*>(f.fvals), bc.marker, bc.g,
+      x0, alpha, b);
+  check_launch("interior facet lifting");
+}
+
 /// Lifting by the exterior facet matrices.
 template <typename T>
 void facet_lift_rows(const dfx_form& f, const dfx_bc& bc, const T* x0, double alpha, T* b)
@@ -3760,7 +3999,7 @@ dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out)
           // thread-per-row slices do not fit LDS for wide elements
           f->row_variant = k > 8 ? 3 : 1;
           const std::size_t ts = dtype_size(d.dtype);
-          if (is_facet_kernel(d.kernel))
+          if (is_facet_kernel(d.kernel) || is_interior_kernel(d.kernel))
           {
             // facet forms keep per-facet data only (dfx_form_create_facets)
           }
@@ -3837,6 +4076,52 @@ dfx_status dfx_form_create_facets(const dfx_form_desc* desc, int32_t npairs,
         f->finc = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * finc.size()));
         h2d(f->finc, finc.data(), sizeof(std::int32_t) * finc.size());
         f->fvals = dmalloc(dtype_size(desc->dtype) * (matrix ? 16 : 4) * std::max(npairs, 1));
+        *out = f.release();
+      });
+}
+
+dfx_status dfx_form_create_interior_facets(const dfx_form_desc* desc, int32_t nfacets,
+                                           const int32_t* quads, dfx_form** out)
+{
+  return guard(
+      [&]
+      {
+        require(desc && out && (nfacets == 0 || quads), "null argument");
+        require(is_interior_kernel(desc->kernel), "the kernel has no interior facet integral");
+        require(nfacets >= 0 && nfacets < (1 << 28), "bad interior facet count");
+        for (std::int64_t e = 0; e < 2 * std::int64_t(nfacets); ++e)
+          require(quads[2 * e] >= 0 && quads[2 * e] < desc->num_cells && quads[2 * e + 1] >= 0
+                      && quads[2 * e + 1] < 4,
+                  "interior facet out of range");
+        dfx_form* raw = nullptr;
+        const dfx_status st = dfx_form_create(desc, &raw);
+        if (st != DFX_OK)
+          throw Error(st, dfx_last_error());
+        std::unique_ptr<dfx_form, dfx_status (*)(dfx_form*)> f(raw, dfx_form_destroy);
+        // macro-element incidences of every row, facet by facet (counting
+        // sort of 8 e + macro index)
+        const std::int32_t nplan = f->max_dof + 1;
+        std::vector<std::int32_t> off(std::size_t(nplan) + 1, 0);
+        auto dof_of = [&](std::int64_t e, int i)
+        { return desc->dofmap[std::int64_t(quads[4 * e + 2 * (i >> 2)]) * 4 + (i & 3)]; };
+        for (std::int64_t e = 0; e < nfacets; ++e)
+          for (int i = 0; i < 8; ++i)
+            ++off[dof_of(e, i) + 1];
+        for (std::int32_t r = 0; r < nplan; ++r)
+          off[r + 1] += off[r];
+        std::vector<std::int32_t> finc(std::max<std::int64_t>(8 * std::int64_t(nfacets), 1));
+        std::vector<std::int32_t> pos(off.begin(), off.end() - 1);
+        for (std::int64_t e = 0; e < nfacets; ++e)
+          for (int i = 0; i < 8; ++i)
+            finc[pos[dof_of(e, i)]++] = static_cast<std::int32_t>(8 * e + i);
+        f->nfacet = nfacets;
+        f->facets = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * std::max(4 * nfacets, 1)));
+        h2d(f->facets, quads, sizeof(std::int32_t) * 4 * nfacets);
+        f->finc_off = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * off.size()));
+        h2d(f->finc_off, off.data(), sizeof(std::int32_t) * off.size());
+        f->finc = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * finc.size()));
+        h2d(f->finc, finc.data(), sizeof(std::int32_t) * finc.size());
+        f->fvals = dmalloc(dtype_size(desc->dtype) * 64 * std::max(nfacets, 1));
         *out = f.release();
       });
 }
@@ -4023,14 +4308,15 @@ dfx_status dfx_assemble_matrix(dfx_csr* A, const dfx_form* a, const dfx_bc* bc)
         require(!bc || bc->n_total == A->nrows * A->bs, "BC size does not match the matrix");
         require(a->max_dof < A->nrows && a->max_dof < A->ncols, "dofmap exceeds the matrix size");
         dfx_form& f = const_cast<dfx_form&>(*a);
-        if (is_facet_matrix_kernel(f.d.kernel))
+        if (is_facet_matrix_kernel(f.d.kernel) || is_interior_kernel(f.d.kernel))
         {
           materialize_zero(*A);
           A->zeroed = false;
+          const bool in = is_interior_kernel(f.d.kernel);
           if (A->dtype == DFX_F64)
-            facet_matrix_rows<double>(f, *A, bc);
+            in ? interior_rows<double>(f, *A, bc) : facet_matrix_rows<double>(f, *A, bc);
           else
-            facet_matrix_rows<float>(f, *A, bc);
+            in ? interior_rows<float>(f, *A, bc) : facet_matrix_rows<float>(f, *A, bc);
           return;
         }
         if (row_path(f))
@@ -4099,7 +4385,8 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t*
         require(ki.bilinear && A->dtype == a->d.dtype && A->bs == a->d.bs,
                 "form and matrix do not match");
         require(a->max_dof < A->nrows, "dofmap exceeds the matrix size");
-        require(!is_facet_matrix_kernel(a->d.kernel), "facet forms have no kernel variants");
+        require(!is_facet_matrix_kernel(a->d.kernel) && !is_interior_kernel(a->d.kernel),
+                "facet forms have no kernel variants");
         dfx_form& f = *a;
         int best = f.row_variant;
         float bms = -1;
@@ -4393,6 +4680,16 @@ dfx_status dfx_apply_lifting(dfx_vec* b, const dfx_form* a, const dfx_bc* bc, co
         require(f.d.kernel != DFX_K_P2TET_ELAST_A,
                 "apply_lifting with nonzero Dirichlet values is not implemented for the "
                 "P2 elasticity form (g = 0 and x0 = 0 are)");
+        if (is_interior_kernel(f.d.kernel))
+        {
+          if (b->dtype == DFX_F64)
+            interior_lift_rows<double>(f, *bc, x0 ? static_cast<const double*>(x0->data) : nullptr,
+                                       alpha, static_cast<double*>(b->data));
+          else
+            interior_lift_rows<float>(f, *bc, x0 ? static_cast<const float*>(x0->data) : nullptr,
+                                      alpha, static_cast<float*>(b->data));
+          return;
+        }
         if (is_facet_matrix_kernel(f.d.kernel))
         {
           if (b->dtype == DFX_F64)

@@ -761,8 +761,53 @@ dfx_status dfx_facet_domain(const dfx_mesh* mesh, int marker, int axis, double v
       });
 }
 
+dfx_status dfx_interior_facet_domain(const dfx_mesh* mesh, int32_t* count, int32_t* quads)
+{
+  return dfx::guard(
+      [&]
+      {
+        dfx::require(mesh && count, "null argument");
+        dfx_mesh& m = const_cast<dfx_mesh&>(*mesh);
+        compute_entities(m, 2);
+        const int nf = num_entities_per_cell(m.cell, 2);
+        // the (cell, local facet) positions of each facet in cell order:
+        // the facet-to-cell links of the transposed connectivity
+        std::vector<std::int32_t> ncell(m.num_entities[2], 0), at(2 * std::size_t(m.num_entities[2]), 0);
+        for (std::int64_t i = 0; i < std::int64_t(m.num_cells) * nf; ++i)
+        {
+          const std::int32_t f = m.c2e[2][i];
+          if (ncell[f] < 2)
+            at[2 * std::size_t(f) + ncell[f]] = static_cast<std::int32_t>(i);
+          ++ncell[f];
+        }
+        std::int32_t n = 0;
+        for (std::int32_t f = 0; f < m.num_entities[2]; ++f)
+        {
+          if (ncell[f] != 2)
+            continue;
+          if (quads)
+          {
+            dfx::require(n < *count, "quads buffer too small");
+            for (int s = 0; s < 2; ++s)
+            {
+              quads[4 * n + 2 * s] = at[2 * std::size_t(f) + s] / nf;
+              quads[4 * n + 2 * s + 1] = at[2 * std::size_t(f) + s] % nf;
+            }
+          }
+          ++n;
+        }
+        *count = n;
+      });
+}
+
 dfx_status dfx_sparsity_create(const dfx_space* V0, const dfx_space* V1,
                                dfx_sparsity** out)
+{
+  return dfx_sparsity_create_facets(V0, V1, 0, nullptr, out);
+}
+
+dfx_status dfx_sparsity_create_facets(const dfx_space* V0, const dfx_space* V1, int32_t nfacets,
+                                      const int32_t* quads, dfx_sparsity** out)
 {
   return dfx::guard(
       [&]
@@ -770,9 +815,35 @@ dfx_status dfx_sparsity_create(const dfx_space* V0, const dfx_space* V1,
         dfx::require(V0 && V1 && out, "null argument");
         dfx::require(V0->mesh == V1->mesh, "spaces on different meshes");
         dfx::require(V0->bs == V1->bs, "rectangular blocks are not supported");
+        dfx::require(nfacets >= 0 && (nfacets == 0 || quads), "bad interior facets");
         const std::int64_t nc = V0->mesh->num_cells;
         const int k0 = V0->dofs_per_cell, k1 = V1->dofs_per_cell;
         const std::int32_t nrows = V0->num_nodes;
+        const int nfc = num_entities_per_cell(V0->mesh->cell, 2);
+        for (std::int64_t e = 0; e < 2 * std::int64_t(nfacets); ++e)
+          dfx::require(quads[2 * e] >= 0 && quads[2 * e] < nc && quads[2 * e + 1] >= 0
+                           && quads[2 * e + 1] < nfc,
+                       "interior facet out of range");
+        // Interior facets couple the dofs of both their cells
+        // (sparsitybuild::interior_facets, fem/sparsitybuild.cpp:16-): row
+        // -> incident facets
+        std::vector<std::int64_t> finc_off(std::size_t(nrows) + 1, 0);
+        std::vector<std::int32_t> finc;
+        if (nfacets > 0)
+        {
+          for (std::int64_t e = 0; e < nfacets; ++e)
+            for (int s = 0; s < 2; ++s)
+              for (int i = 0; i < k0; ++i)
+                ++finc_off[V0->dofmap[std::int64_t(quads[4 * e + 2 * s]) * k0 + i] + 1];
+          std::partial_sum(finc_off.begin(), finc_off.end(), finc_off.begin());
+          finc.resize(finc_off.back());
+          std::vector<std::int64_t> pos(finc_off.begin(), finc_off.end() - 1);
+          for (std::int64_t e = 0; e < nfacets; ++e)
+            for (int s = 0; s < 2; ++s)
+              for (int i = 0; i < k0; ++i)
+                finc[pos[V0->dofmap[std::int64_t(quads[4 * e + 2 * s]) * k0 + i]]++]
+                    = static_cast<std::int32_t>(e);
+        }
         // Transpose of the row dofmap: row -> incident (cell) list
         std::vector<std::int64_t> inc_off(std::size_t(nrows) + 1, 0);
         for (std::int32_t d : V0->dofmap)
@@ -801,6 +872,14 @@ dfx_status dfx_sparsity_create(const dfx_space* V0, const dfx_space* V1,
               const std::int32_t* d = V1->dofmap.data() + std::int64_t(inc[p]) * k1;
               buf.insert(buf.end(), d, d + k1);
             }
+            if (nfacets > 0)
+              for (std::int64_t p = finc_off[r]; p < finc_off[r + 1]; ++p)
+                for (int s = 0; s < 2; ++s)
+                {
+                  const std::int32_t* d
+                      = V1->dofmap.data() + std::int64_t(quads[4 * std::int64_t(finc[p]) + 2 * s]) * k1;
+                  buf.insert(buf.end(), d, d + k1);
+                }
             std::sort(buf.begin(), buf.end());
             buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
             rowlen[r] = static_cast<std::int32_t>(buf.size());

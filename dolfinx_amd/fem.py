@@ -147,6 +147,9 @@ def dirichletbc(V: FunctionSpace, dofs, g=None) -> DirichletBC:
 # Kernels of exterior facet integrals (integration entities: (cell, local
 # facet) pairs)
 FACET_KERNELS = {"p1_neumann", "q1_neumann", "p1_robin", "q1_robin"}
+# Kernels of interior facet integrals (integration entities: (cell0, local
+# facet0, cell1, local facet1) quads)
+INTERIOR_KERNELS = {"p1_gradient_jump"}
 
 # name -> (kernel id, cell type, degree, block size, rank)
 KERNELS = {
@@ -161,6 +164,7 @@ KERNELS = {
     "q1_neumann": (_lib.K_Q1HEX_NEUMANN, CellType.hexahedron, 1, 1, 1),
     "p1_robin": (_lib.K_P1TET_ROBIN_A, CellType.tetrahedron, 1, 1, 2),
     "q1_robin": (_lib.K_Q1HEX_ROBIN_A, CellType.hexahedron, 1, 1, 2),
+    "p1_gradient_jump": (_lib.K_P1TET_GRADJUMP_A, CellType.tetrahedron, 1, 1, 2),
     "p3_source": (_lib.K_P3TET_L, CellType.tetrahedron, 3, 1, 1),
 }
 
@@ -197,7 +201,9 @@ class Form:
         if mesh.topology.cell_type != ct or V.degree != deg or V.dofmap.bs != bs:
             raise ValueError(f"form {name} does not match the space")
         self.name, self.V, self.rank, self.dtype = name, V, rank, np.dtype(dtype)
-        self.integral = "exterior_facet" if name in FACET_KERNELS else "cell"
+        self.integral = ("exterior_facet" if name in FACET_KERNELS else
+                         "interior_facet" if name in INTERIOR_KERNELS else "cell")
+        self.facets = None
         self.coeffs = None if coeffs is None else np.ascontiguousarray(coeffs, np.float64)
         nc = mesh.topology.cells.shape[0]
         if self.coeffs is not None:
@@ -217,7 +223,13 @@ class Form:
         d.num_constants = len(self.constants)
         d.constants = _lib.ptr(self.constants)
         self._h = C.c_void_p()
-        if facets is not None:
+        if self.integral == "interior_facet":
+            q = np.ascontiguousarray(np.zeros((0, 4)) if facets is None else facets,
+                                     np.int32).reshape(-1, 4)
+            self.facets = q
+            _lib.call("dfx_form_create_interior_facets", C.byref(d), q.shape[0], _lib.ptr(q),
+                      C.byref(self._h))
+        elif facets is not None:
             pr = np.ascontiguousarray(facets, np.int32).reshape(-1, 2)
             self.facets = pr
             _lib.call("dfx_form_create_facets", C.byref(d), pr.shape[0], _lib.ptr(pr),
@@ -287,21 +299,37 @@ def form(name, V, coeffs=None, constants=None, dtype=np.float64, facets=None) ->
     """A form of a named kernel; exterior facet kernels ("p1_neumann",
     "q1_neumann": g v ds with g packed per cell vertex; "p1_robin",
     "q1_robin": beta u v ds with DG0 beta) take their integration entities as
-    (cell, local facet) pairs (facet_domain). A form with several integrals
-    is passed to the assemblers as a tuple of such forms."""
+    (cell, local facet) pairs (facet_domain); the interior facet kernel
+    "p1_gradient_jump" (gamma [grad u.n][grad v.n] dS, constants [gamma])
+    takes (cell0, local facet0, cell1, local facet1) quads
+    (interior_facet_domain). A form with several integrals is passed to the
+    assemblers as a tuple of such forms."""
     return Form(name, V, coeffs, constants, dtype, facets)
+
+
+def interior_facet_domain(mesh: Mesh) -> np.ndarray:
+    """fem::compute_integration_domains(interior_facet) on every interior
+    facet: (cell0, local facet0, cell1, local facet1), cell0 < cell1, in
+    facet-index order (fem/utils.cpp:223-253)."""
+    n = C.c_int32()
+    _lib.call("dfx_interior_facet_domain", mesh.handle, C.byref(n), None)
+    out = np.zeros((n.value, 4), np.int32)
+    _lib.call("dfx_interior_facet_domain", mesh.handle, C.byref(n), _lib.ptr(out))
+    return out
+
+
+_ORDER = {"cell": 0, "interior_facet": 1, "exterior_facet": 2}
 
 
 def _integrals(a):
     """The forms of a (one form or a tuple of forms of one form's
-    integrals) in the reference's order: cell integrals, then exterior
-    facets (fem/assemble_matrix_impl.h:776-880,
+    integrals) in the reference's order: cell integrals, interior facets,
+    then exterior facets (fem/assemble_matrix_impl.h:776-880,
     fem/assemble_vector_impl.h:540-615)."""
     forms = list(a) if isinstance(a, (list, tuple)) else [a]
     if not forms:
         raise ValueError("no forms")
-    return [f for f in forms if f.integral == "cell"] + [
-        f for f in forms if f.integral != "cell"]
+    return sorted(forms, key=lambda f: _ORDER[f.integral])
 
 
 def facet_domain(mesh: Mesh, axis=None, value=0.0) -> np.ndarray:
@@ -343,17 +371,24 @@ def _device_tensors(x, n, dtype):
     return Vector.from_array(a, dtype)
 
 
-def create_sparsity_pattern(a: Form) -> SparsityPattern:
-    """fem::create_sparsity_pattern (fem/utils.h:197-218)."""
+def create_sparsity_pattern(a) -> SparsityPattern:
+    """fem::create_sparsity_pattern (fem/utils.h:197-314): the cell
+    couplings, plus those of the interior facets of any interior facet
+    integral of a (a Form or a tuple of one form's integrals)."""
+    forms = _integrals(a)
+    q = [f.facets for f in forms if f.integral == "interior_facet"]
+    q = np.ascontiguousarray(np.concatenate(q) if q else np.zeros((0, 4)), np.int32)
+    V = forms[0].V
     h = C.c_void_p()
-    _lib.call("dfx_sparsity_create", a.V.handle, a.V.handle, C.byref(h))
+    _lib.call("dfx_sparsity_create_facets", V.handle, V.handle, q.shape[0], _lib.ptr(q),
+              C.byref(h))
     return SparsityPattern(h)
 
 
-def create_matrix(a: Form, dtype=None) -> MatrixCSR:
+def create_matrix(a, dtype=None) -> MatrixCSR:
     """python/dolfinx/fem/assemble.py:127-143 (block_mode compact)."""
     sp = create_sparsity_pattern(a)
-    return MatrixCSR.from_sparsity(sp, a.dtype if dtype is None else dtype)
+    return MatrixCSR.from_sparsity(sp, _integrals(a)[0].dtype if dtype is None else dtype)
 
 
 def assemble_matrix(a, bcs=(), diag=1.0, A: MatrixCSR | None = None) -> MatrixCSR:
@@ -362,7 +397,7 @@ def assemble_matrix(a, bcs=(), diag=1.0, A: MatrixCSR | None = None) -> MatrixCS
     Form or a tuple of the forms of one form's integrals."""
     forms = _integrals(a)
     if A is None:
-        A = create_matrix(forms[0])
+        A = create_matrix(forms)
     if len(bcs) > 1:
         raise ValueError("pass the union of Dirichlet dofs as one DirichletBC")
     bc = bcs[0] if bcs else None

@@ -57,7 +57,9 @@ enum
   DFX_K_P1TET_NEUMANN = 10,/* g v ds on exterior facets; w = [g at the 4 vertices] */
   DFX_K_Q1HEX_NEUMANN = 11,/* g v ds on exterior facets; w = [g at the 8 vertices] */
   DFX_K_P1TET_ROBIN_A = 12,/* beta u v ds on exterior facets; w = [beta] (DG0) */
-  DFX_K_Q1HEX_ROBIN_A = 13 /* beta u v ds on exterior facets; w = [beta] (DG0) */
+  DFX_K_Q1HEX_ROBIN_A = 13,/* beta u v ds on exterior facets; w = [beta] (DG0) */
+  DFX_K_P1TET_GRADJUMP_A = 14 /* gamma [grad u.n][grad v.n] dS on interior
+                                 facets; c = [gamma] */
 };
 
 const char* dfx_last_error(void);
@@ -108,6 +110,13 @@ dfx_status dfx_mesh_copy(const dfx_mesh* m, double* x, int32_t* cells,
 dfx_status dfx_facet_domain(const dfx_mesh* m, int marker, int axis, double value,
                             int32_t* count, int32_t* pairs);
 
+/* fem::compute_integration_domains(IntegralType::interior_facet, ...)
+ * (fem/utils.cpp:223-253) on every interior facet, in facet-index order:
+ * (cell0, local facet0, cell1, local facet1) with cell0 < cell1 (the
+ * facet-to-cell links). With quads NULL only *count is set; otherwise
+ * *count is the capacity in facets on entry. */
+dfx_status dfx_interior_facet_domain(const dfx_mesh* m, int32_t* count, int32_t* quads);
+
 /* fem::create_functionspace -> fem::create_dofmap (fem/utils.cpp:32-76)
  * for Lagrange degree 1-3 (tetrahedra) or 1 (hexahedra), block size bs. */
 dfx_status dfx_space_create(dfx_mesh* m, int degree, int bs, dfx_space** out);
@@ -128,6 +137,13 @@ dfx_status dfx_locate_boundary_dofs(const dfx_space* V, int marker, int axis,
  * 197-314, la/SparsityPattern.cpp:264-491). Block pattern over nodes. */
 dfx_status dfx_sparsity_create(const dfx_space* V0, const dfx_space* V1,
                                dfx_sparsity** out);
+/* The same pattern with the couplings of interior facet integrals over the
+ * nfacets (cell0, lf0, cell1, lf1) quads added: every dof of both cells of
+ * a facet couples to every other (sparsitybuild::interior_facets,
+ * fem/sparsitybuild.cpp:16-, called from fem/utils.h:288). */
+dfx_status dfx_sparsity_create_facets(const dfx_space* V0, const dfx_space* V1,
+                                      int32_t nfacets, const int32_t* quads,
+                                      dfx_sparsity** out);
 dfx_status dfx_sparsity_destroy(dfx_sparsity* sp);
 dfx_status dfx_sparsity_sizes(const dfx_sparsity* sp, int32_t* nrows,
                               int64_t* nnz, int32_t* bs);
@@ -233,6 +249,16 @@ dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out);
  * subtracts them like lift_bc (fem/assemble_vector_impl.h:361-414). */
 dfx_status dfx_form_create_facets(const dfx_form_desc* desc, int32_t npairs,
                                   const int32_t* pairs, dfx_form** out);
+/* A form with an interior facet integral (kernel DFX_K_P1TET_GRADJUMP_A)
+ * over the nfacets (cell0, lf0, cell1, lf1) quads of
+ * dfx_interior_facet_domain: assemble_matrix adds each facet's 8 x 8
+ * macro-element matrix (dofs of cell0 then cell1), BC rows and columns
+ * zeroed, in facet order (assemble_interior_facets,
+ * fem/assemble_matrix_impl.h:442-667, run after the cell integrals at
+ * :799-842); apply_lifting subtracts them. The matrix needs the pattern of
+ * dfx_sparsity_create_facets. */
+dfx_status dfx_form_create_interior_facets(const dfx_form_desc* desc, int32_t nfacets,
+                                           const int32_t* quads, dfx_form** out);
 dfx_status dfx_form_destroy(dfx_form* f);
 /* Replace packed coefficients / constants (same sizes as at creation). */
 dfx_status dfx_form_set_coefficients(dfx_form* f, const double* coeffs,

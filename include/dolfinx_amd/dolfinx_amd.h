@@ -128,6 +128,18 @@ inline std::vector<std::int32_t> facet_domain(const Mesh& m, int axis = -1, doub
   return pairs;
 }
 
+/// fem::compute_integration_domains(IntegralType::interior_facet, ...)
+/// (fem/utils.cpp:223-253) on every interior facet: flattened (cell0,
+/// local facet0, cell1, local facet1) quads in facet-index order.
+inline std::vector<std::int32_t> interior_facet_domain(const Mesh& m)
+{
+  std::int32_t n = 0;
+  check(dfx_interior_facet_domain(m.handle(), &n, nullptr));
+  std::vector<std::int32_t> quads(4 * std::size_t(n));
+  check(dfx_interior_facet_domain(m.handle(), &n, quads.data()));
+  return quads;
+}
+
 /// mesh::create_mesh(comm, cells, element, x, ...) (mesh/utils.h:1336-1356)
 /// on one rank: cells as input vertex indices in DOLFINx vertex order
 /// (num_cells x 4 or 8), x as num_vertices x 3.
@@ -410,8 +422,10 @@ template <typename T>
 class Form
 {
 public:
-  /// With facets (flattened (cell, local facet) pairs) the form's integral
-  /// is an exterior facet integral over them (DFX_K_*_NEUMANN kernels).
+  /// With facets the form's integral is a facet integral over them:
+  /// flattened (cell, local facet) pairs for the exterior facet kernels
+  /// (DFX_K_*_NEUMANN, DFX_K_*_ROBIN_A), (cell0, lf0, cell1, lf1) quads
+  /// for the interior facet kernel (DFX_K_P1TET_GRADJUMP_A).
   Form(int kernel, std::shared_ptr<const FunctionSpace> V, std::vector<double> coeffs,
        int cstride, std::vector<double> constants,
        std::span<const std::int32_t> facets = {}, bool facet_integral = false)
@@ -437,14 +451,35 @@ public:
     d.num_constants = static_cast<std::int32_t>(constants.size());
     d.constants = constants.empty() ? nullptr : constants.data();
     dfx_form* h = nullptr;
-    if (facet_integral || !facets.empty())
+    if (kernel == DFX_K_P1TET_GRADJUMP_A)
+    {
+      _interior.assign(facets.begin(), facets.end());
+      check(dfx_form_create_interior_facets(&d, static_cast<std::int32_t>(facets.size() / 4),
+                                            facets.data(), &h));
+    }
+    else if (facet_integral || !facets.empty())
       check(dfx_form_create_facets(&d, static_cast<std::int32_t>(facets.size() / 2),
                                    facets.data(), &h));
     else
       check(dfx_form_create(&d, &h));
     _h.reset(h, dfx_form_destroy);
-    _rank = (kernel % 2 == 0) ? 2 : 1;
+    switch (kernel)
+    {
+    case DFX_K_P1TET_L:
+    case DFX_K_Q1HEX_L:
+    case DFX_K_P2TET_ELAST_L:
+    case DFX_K_P3TET_L:
+    case DFX_K_ELEMENT_L:
+    case DFX_K_P1TET_NEUMANN:
+    case DFX_K_Q1HEX_NEUMANN:
+      _rank = 1;
+      break;
+    default:
+      _rank = 2;
+    }
   }
+  /// The (cell0, lf0, cell1, lf1) quads of an interior facet integral.
+  const std::vector<std::int32_t>& interior_facets() const { return _interior; }
   dfx_form* handle() const { return _h.get(); }
   int rank() const { return _rank; }
   std::vector<std::shared_ptr<const FunctionSpace>> function_spaces() const
@@ -471,16 +506,20 @@ public:
 private:
   std::shared_ptr<const FunctionSpace> _V;
   std::shared_ptr<dfx_form> _h;
+  std::vector<std::int32_t> _interior;
   int _rank = 2;
 };
 
-/// fem::create_sparsity_pattern (fem/utils.h:197-218)
+/// fem::create_sparsity_pattern (fem/utils.h:197-314): the cell couplings
+/// and those of the form's interior facets
 template <typename T>
 la::SparsityPattern create_sparsity_pattern(const Form<T>& a)
 {
   auto V = a.function_spaces().at(0);
+  const std::vector<std::int32_t>& q = a.interior_facets();
   dfx_sparsity* h = nullptr;
-  check(dfx_sparsity_create(V->handle(), V->handle(), &h));
+  check(dfx_sparsity_create_facets(V->handle(), V->handle(),
+                                   static_cast<std::int32_t>(q.size() / 4), q.data(), &h));
   return la::SparsityPattern(h);
 }
 

@@ -24,6 +24,7 @@ K_P1TET_A, K_P1TET_L, K_Q1HEX_A, K_Q1HEX_L = 0, 1, 2, 3
 K_P2TET_ELAST_A, K_P2TET_ELAST_L, K_P3TET_A, K_P3TET_L = 4, 5, 6, 7
 K_P1TET_NEUMANN, K_Q1HEX_NEUMANN = 10, 11
 K_P1TET_ROBIN_A, K_Q1HEX_ROBIN_A = 12, 13
+K_P1TET_GRADJUMP_A = 14
 
 _lib = None
 
@@ -89,6 +90,13 @@ def _declare(L):
     L.orc_facet_domain.argtypes = [C.POINTER(_Mesh), C.c_int, C.c_int, D, C.POINTER(P)]
     L.orc_assemble_vector_facets.argtypes = [C.c_int, P, P, C.c_int, P, C.c_int, C.c_int, P,
                                              C.c_int, P, P, I32, P]
+    L.orc_interior_facet_domain.restype = I32
+    L.orc_interior_facet_domain.argtypes = [C.POINTER(_Mesh), C.POINTER(P)]
+    L.orc_assemble_matrix_interior.restype = C.c_int
+    L.orc_assemble_matrix_interior.argtypes = [C.c_int, P, P, C.c_int, P, C.c_int, P, P, I32, P,
+                                               P, P, P, P]
+    L.orc_lift_bc_interior.argtypes = [C.c_int, P, P, C.c_int, P, C.c_int, P, P, I32, P, P, P, D,
+                                       P]
     L.orc_assemble_matrix_facets.restype = C.c_int
     L.orc_assemble_matrix_facets.argtypes = [C.c_int, P, P, C.c_int, P, C.c_int, C.c_int, P,
                                              C.c_int, P, P, I32, P, P, P, P, P]
@@ -189,6 +197,13 @@ class Mesh:
         out = P()
         n = lib().orc_exterior_facets(self._ptr, C.byref(out))
         return _take(out.value, 2 * n, np.int32).reshape(n, 2)
+
+    def interior_facet_domain(self):
+        """(cell0, lf0, cell1, lf1) of every interior facet, facet-index
+        order."""
+        out = P()
+        n = lib().orc_interior_facet_domain(self._ptr, C.byref(out))
+        return _take(out.value, 4 * n, np.int32).reshape(n, 4)
 
     def facet_domain(self, axis=None, value=0.0):
         """(cell, local facet) of the exterior facets in facet-index order,
@@ -314,6 +329,44 @@ def assemble_matrix_facets(kernel, x, x_dofmap, dofmap, coeffs, pairs, bc0, bc1,
     if rc != 0:
         raise RuntimeError("Entry not in sparsity")
     return values
+
+
+def assemble_matrix_interior(kernel, x, x_dofmap, dofmap, quads, bc0, bc1, row_ptr, cols,
+                             constants, values=None):
+    """values += an interior facet integral over (cell0, lf0, cell1, lf1)
+    quads, BC rows (bc0) and columns (bc1) zeroed (scalar spaces)."""
+    x = np.ascontiguousarray(x, np.float64)
+    xd = np.ascontiguousarray(x_dofmap, np.int32)
+    dm = np.ascontiguousarray(dofmap, np.int32)
+    q = np.ascontiguousarray(quads, np.int32).reshape(-1, 4)
+    kc = np.ascontiguousarray(np.atleast_1d(constants), np.float64)
+    b0 = None if bc0 is None else np.ascontiguousarray(bc0, np.int8)
+    b1 = None if bc1 is None else np.ascontiguousarray(bc1, np.int8)
+    if values is None:
+        values = np.zeros(len(cols), np.float64)
+    rc = lib().orc_assemble_matrix_interior(
+        kernel, _p(x), _p(xd), xd.shape[1], _p(dm), dm.shape[1], _p(kc), _p(q), q.shape[0],
+        _p(b0), _p(b1), _p(np.ascontiguousarray(row_ptr, np.int64)),
+        _p(np.ascontiguousarray(cols, np.int32)), _p(values))
+    if rc != 0:
+        raise RuntimeError("Entry not in sparsity")
+    return values
+
+
+def lift_bc_interior(b, kernel, x, x_dofmap, dofmap, quads, constants, bc_values, bc_markers,
+                     x0=None, alpha=1.0):
+    x = np.ascontiguousarray(x, np.float64)
+    xd = np.ascontiguousarray(x_dofmap, np.int32)
+    dm = np.ascontiguousarray(dofmap, np.int32)
+    q = np.ascontiguousarray(quads, np.int32).reshape(-1, 4)
+    kc = np.ascontiguousarray(np.atleast_1d(constants), np.float64)
+    lib().orc_lift_bc_interior(kernel, _p(x), _p(xd), xd.shape[1], _p(dm), dm.shape[1], _p(kc),
+                               _p(q), q.shape[0],
+                               _p(np.ascontiguousarray(bc_values, np.float64)),
+                               _p(np.ascontiguousarray(bc_markers, np.int8)),
+                               None if x0 is None else _p(np.ascontiguousarray(x0, np.float64)),
+                               alpha, _p(b))
+    return b
 
 
 def lift_bc_facets(b, kernel, x, x_dofmap, dofmap, coeffs, pairs, bc_values, bc_markers,

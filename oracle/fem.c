@@ -509,6 +509,54 @@ static void k_q1hex_robin(double* A, const double* w, const double* c,
     }
 }
 
+/* P1 tet pair across an interior facet: gamma [grad u.n][grad v.n] dS on
+ * the 8 x 8 macro element (cell 0's dofs, then cell 1's; cd holds both
+ * cells' coordinates), n the unit normal of cell 0's facet e[0],
+ * j_a = grad phi_a.n on cell 0 and -grad phi_a.n on cell 1, A_ab =
+ * gamma |F| j_a j_b (the gradients are constant: exact). */
+static void k_p1tet_gradjump(double* A, const double* w, const double* c,
+                             const double* cd, const int* e, const uint8_t* q,
+                             void* u)
+{
+  (void)w;
+  (void)q;
+  (void)u;
+  const int* fv = TET_FV[e[0]];
+  double t0[3], t1[3], n[3];
+  for (int d = 0; d < 3; ++d)
+  {
+    t0[d] = cd[3 * fv[1] + d] - cd[3 * fv[0] + d];
+    t1[d] = cd[3 * fv[2] + d] - cd[3 * fv[0] + d];
+  }
+  cross3(t0, t1, n);
+  const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  for (int d = 0; d < 3; ++d)
+    n[d] /= nn;
+  const double area = 0.5 * nn;
+  double jmp[8];
+  for (int s = 0; s < 2; ++s)
+  {
+    double J[9], K[9];
+    const double* X = cd + 12 * s;
+    for (int r = 0; r < 3; ++r)
+      for (int a = 0; a < 3; ++a)
+        J[3 * r + a] = X[3 * (a + 1) + r] - X[r];
+    inv3(J, K);
+    for (int i = 0; i < 4; ++i)
+    {
+      /* grad lambda_i: row i-1 of K, lambda_0 = 1 - sum */
+      double g[3];
+      for (int d = 0; d < 3; ++d)
+        g[d] = i == 0 ? -(K[d] + K[3 + d] + K[6 + d]) : K[3 * (i - 1) + d];
+      const double dn = g[0] * n[0] + g[1] * n[1] + g[2] * n[2];
+      jmp[4 * s + i] = s == 0 ? dn : -dn;
+    }
+  }
+  for (int a = 0; a < 8; ++a)
+    for (int b = 0; b < 8; ++b)
+      A[8 * a + b] += c[0] * area * jmp[a] * jmp[b];
+}
+
 /* ---- P2/P3 tetrahedra: nodal basis from the monomial Vandermonde ---- */
 typedef struct
 {
@@ -834,6 +882,8 @@ orc_kernel_fn orc_get_kernel(int id)
     return k_p1tet_robin;
   case ORC_K_Q1HEX_ROBIN_A:
     return k_q1hex_robin;
+  case ORC_K_P1TET_GRADJUMP_A:
+    return k_p1tet_gradjump;
   default:
     return NULL;
   }
@@ -1062,6 +1112,119 @@ int orc_assemble_matrix_facets(int kernel, const double* x,
   }
   free(Ae);
   return 0;
+}
+
+/* fem/assemble_matrix_impl.h:442-667 (assemble_interior_facets,
+ * non-lifting): per facet (cell0, lf0, cell1, lf1) in order, both cells'
+ * coordinates, the kernel with local facets {lf0, lf1}, zero the rows and
+ * columns of BC dofs of the joint dofmap (dofs of cell0 then cell1), then
+ * mat_set(joint, joint, Ae): entry by entry, shared dofs twice. Scalar
+ * spaces. Returns -1 if an entry is not in the sparsity. */
+int orc_assemble_matrix_interior(int kernel, const double* x,
+                                 const int32_t* x_dofmap, int npc,
+                                 const int32_t* dofmap, int k,
+                                 const double* constants, const int32_t* quads,
+                                 int32_t nfacets, const int8_t* bc0,
+                                 const int8_t* bc1, const int64_t* row_ptr,
+                                 const int32_t* cols, double* values)
+{
+  orc_kernel_fn fn = orc_get_kernel(kernel);
+  const int nd = 2 * k;
+  double* Ae = (double*)malloc(sizeof(double) * nd * nd);
+  int32_t joint[40];
+  double cdofs[2 * 3 * 20];
+  const double zero = 0.0;
+  for (int32_t f = 0; f < nfacets; ++f)
+  {
+    const int lf[2] = {quads[4 * f + 1], quads[4 * f + 3]};
+    for (int s = 0; s < 2; ++s)
+    {
+      const int32_t c = quads[4 * f + 2 * s];
+      for (int i = 0; i < npc; ++i)
+        memcpy(cdofs + 3 * (s * npc + i),
+               x + 3 * (int64_t)x_dofmap[(int64_t)c * npc + i], 3 * sizeof(double));
+      for (int i = 0; i < k; ++i)
+        joint[s * k + i] = dofmap[(int64_t)c * k + i];
+    }
+    memset(Ae, 0, sizeof(double) * nd * nd);
+    fn(Ae, &zero, constants ? constants : &zero, cdofs, lf, NULL, NULL);
+    if (bc0)
+      for (int i = 0; i < nd; ++i)
+        if (bc0[joint[i]])
+          memset(Ae + i * nd, 0, sizeof(double) * nd);
+    if (bc1)
+      for (int j = 0; j < nd; ++j)
+        if (bc1[joint[j]])
+          for (int i = 0; i < nd; ++i)
+            Ae[i * nd + j] = 0;
+    for (int i = 0; i < nd; ++i)
+    {
+      const int32_t row = joint[i];
+      for (int j = 0; j < nd; ++j)
+      {
+        int64_t p = -1;
+        for (int64_t q = row_ptr[row]; q < row_ptr[row + 1]; ++q)
+          if (cols[q] == joint[j])
+            p = q;
+        if (p < 0)
+        {
+          free(Ae);
+          return -1;
+        }
+        values[p] += Ae[i * nd + j];
+      }
+    }
+  }
+  free(Ae);
+  return 0;
+}
+
+/* lift_bc (fem/assemble_vector_impl.h:361-414) over an interior facet
+ * integral: facets whose joint dofmap has a BC dof; b[joint_i] -=
+ * Ae[i, j] alpha (g_j - x0_j) over the BC columns j, column by column. */
+void orc_lift_bc_interior(int kernel, const double* x, const int32_t* x_dofmap,
+                          int npc, const int32_t* dofmap, int k,
+                          const double* constants, const int32_t* quads,
+                          int32_t nfacets, const double* bc_values,
+                          const int8_t* bc_markers, const double* x0,
+                          double alpha, double* b)
+{
+  orc_kernel_fn fn = orc_get_kernel(kernel);
+  const int nd = 2 * k;
+  double* Ae = (double*)malloc(sizeof(double) * nd * nd);
+  int32_t joint[40];
+  double cdofs[2 * 3 * 20];
+  const double zero = 0.0;
+  for (int32_t f = 0; f < nfacets; ++f)
+  {
+    const int lf[2] = {quads[4 * f + 1], quads[4 * f + 3]};
+    int has = 0;
+    for (int s = 0; s < 2; ++s)
+    {
+      const int32_t c = quads[4 * f + 2 * s];
+      for (int i = 0; i < npc; ++i)
+        memcpy(cdofs + 3 * (s * npc + i),
+               x + 3 * (int64_t)x_dofmap[(int64_t)c * npc + i], 3 * sizeof(double));
+      for (int i = 0; i < k; ++i)
+      {
+        joint[s * k + i] = dofmap[(int64_t)c * k + i];
+        has |= bc_markers[joint[s * k + i]];
+      }
+    }
+    if (!has)
+      continue;
+    memset(Ae, 0, sizeof(double) * nd * nd);
+    fn(Ae, &zero, constants ? constants : &zero, cdofs, lf, NULL, NULL);
+    for (int j = 0; j < nd; ++j)
+    {
+      if (!bc_markers[joint[j]])
+        continue;
+      const double v = alpha * (bc_values[joint[j]] - (x0 ? x0[joint[j]] : 0.0));
+      for (int i = 0; i < nd; ++i)
+        b[joint[i]] -= Ae[i * nd + j] * v;
+    }
+  }
+  free(Ae);
 }
 
 /* fem/assemble_vector_impl.h:361-414 (lift_bc) on exterior facets

@@ -644,6 +644,46 @@ int32_t orc_facet_domain(const orc_mesh* m, int marker, int axis, double value,
   return n;
 }
 
+/* fem::compute_integration_domains(interior_facet) (fem/utils.cpp:223-253,
+ * impl::get_cell_facet_pairs<2>): every facet with two cells, in facet-index
+ * order, as (cell0, lf0, cell1, lf1) in the order of the facet-to-cell
+ * links (the transpose of cell-to-facet: ascending cells). */
+int32_t orc_interior_facet_domain(const orc_mesh* m, int32_t** quads)
+{
+  const int tdim = 3;
+  int32_t *c2f, *f2v;
+  const int32_t nfac = orc_entities(m, tdim - 1, &c2f, &f2v);
+  const int nf = orc_cell_num_entities(m->celltype, tdim - 1);
+  int32_t* count = (int32_t*)calloc(nfac > 0 ? nfac : 1, sizeof(int32_t));
+  int32_t* at = (int32_t*)malloc(sizeof(int32_t) * 2 * (nfac > 0 ? nfac : 1));
+  for (int64_t i = 0; i < (int64_t)m->num_cells * nf; ++i)
+  {
+    const int32_t f = c2f[i];
+    if (count[f] < 2)
+      at[2 * f + count[f]] = (int32_t)i;
+    count[f]++;
+  }
+  int32_t* out = (int32_t*)malloc(sizeof(int32_t) * 4 * (nfac > 0 ? nfac : 1));
+  int32_t n = 0;
+  for (int32_t f = 0; f < nfac; ++f)
+  {
+    if (count[f] != 2)
+      continue;
+    for (int s = 0; s < 2; ++s)
+    {
+      out[4 * n + 2 * s] = at[2 * f + s] / nf;
+      out[4 * n + 2 * s + 1] = at[2 * f + s] % nf;
+    }
+    ++n;
+  }
+  free(count);
+  free(at);
+  free(c2f);
+  free(f2v);
+  *quads = out;
+  return n;
+}
+
 int32_t orc_exterior_facets(const orc_mesh* m, int32_t** cell_facet)
 {
   const int tdim = 3;

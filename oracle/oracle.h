@@ -40,6 +40,11 @@
  *                        block = the facet mass matrix; BC rows/columns by
  *                        brute force; orc_lift_bc_facets = dense column
  *                        product of the facet matrix.
+ *   orc_interior_facet_domain  brute-force shared facet vertex sets.
+ *   orc_assemble_matrix_interior  A u = 0 for u linear (no gradient jump),
+ *                        u' A u = gamma |plane| for a kink across a mesh
+ *                        plane, symmetric positive semidefinite; BC
+ *                        rows/columns and lifting by brute force.
  *   orc_add_element_*    equal to orc_assemble_* fed the tabulated tensors;
  *                        a dense scatter of random tensors on tiny meshes.
  *   orc_assemble_*       P1 mass total = |Omega|, A 1 = 0, "Entry not in
@@ -101,6 +106,9 @@ int32_t orc_entities(const orc_mesh* m, int dim, int32_t** c2e,
 /* Per-cell edge reflection bits (bit e set iff edge e is reflected). */
 void orc_edge_reflections(const orc_mesh* m, const int32_t* c2e,
                           const int32_t* e2v, uint32_t* bits);
+/* Interior facets as (cell0, lf0, cell1, lf1), facet-index order, cell0 <
+ * cell1; returns count, caller frees. */
+int32_t orc_interior_facet_domain(const orc_mesh* m, int32_t** quads);
 /* Exterior facets as (cell, local facet) pairs; returns count. Caller frees */
 int32_t orc_exterior_facets(const orc_mesh* m, int32_t** cell_facet);
 /* fem::compute_integration_domains(exterior_facet) (fem/utils.cpp:154-273)
@@ -148,7 +156,8 @@ enum
   ORC_K_Q1HEX_NEUMANN = 11, /* g v ds on an exterior facet; w = [g at 8 vertices] */
   ORC_K_P1TET_ROBIN_A = 12, /* beta u v ds on an exterior facet; w = [beta] */
   ORC_K_Q1HEX_ROBIN_A = 13, /* beta u v ds on an exterior facet; w = [beta] */
-  ORC_K_COUNT = 14
+  ORC_K_P1TET_GRADJUMP_A = 14, /* gamma [grad u.n][grad v.n] dS, c = [gamma] */
+  ORC_K_COUNT = 15
 };
 orc_kernel_fn orc_get_kernel(int id);
 void orc_tabulate(int id, double* A, const double* w, const double* c,
@@ -184,6 +193,22 @@ int orc_assemble_matrix_facets(int kernel, const double* x,
                                int32_t npairs, const int8_t* bc0,
                                const int8_t* bc1, const int64_t* row_ptr,
                                const int32_t* cols, double* values);
+/* Interior facet integral over (cell0, lf0, cell1, lf1) quads in order,
+ * added into the CSR values with BC rows/columns zeroed; -1 if an entry is
+ * not in the sparsity. Lifting over the same integral. */
+int orc_assemble_matrix_interior(int kernel, const double* x,
+                                 const int32_t* x_dofmap, int npc,
+                                 const int32_t* dofmap, int k,
+                                 const double* constants, const int32_t* quads,
+                                 int32_t nfacets, const int8_t* bc0,
+                                 const int8_t* bc1, const int64_t* row_ptr,
+                                 const int32_t* cols, double* values);
+void orc_lift_bc_interior(int kernel, const double* x, const int32_t* x_dofmap,
+                          int npc, const int32_t* dofmap, int k,
+                          const double* constants, const int32_t* quads,
+                          int32_t nfacets, const double* bc_values,
+                          const int8_t* bc_markers, const double* x0,
+                          double alpha, double* b);
 /* Lifting by exterior facet matrices (lift_bc over the facet integral). */
 void orc_lift_bc_facets(int kernel, const double* x, const int32_t* x_dofmap,
                         int npc, const int32_t* dofmap, int k, int bs,

@@ -124,3 +124,34 @@ def test_facet_domain_matches_oracle(ct, n, axis, value):
             pm = mesh.create_mesh(mesh.CellType(ct), cells, x)
         ref = om.facet_domain() if axis is None else om.facet_domain(axis, value)
         assert np.array_equal(fem.facet_domain(pm, axis, value), ref)
+
+
+@pytest.mark.parametrize("shuffled", [False, True])
+def test_interior_facet_domain_and_pattern_match_oracle(shuffled):
+    """Interior facet quads (fem::compute_integration_domains(interior_facet))
+    equal the oracle's, and the pattern with their couplings
+    (sparsitybuild::interior_facets) equals the brute-force rows."""
+    from test_oracle import pattern_with_interior
+    n = (3, 2, 3)
+    if shuffled:
+        cells, x = box_input(n, O.TET, 9)
+        om = O.Mesh.from_cells(O.TET, cells, x)
+        pm = mesh.create_mesh(mesh.CellType.tetrahedron, cells, x)
+    else:
+        om = O.Mesh(n, O.TET)
+        pm = mesh.create_box([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], n, mesh.CellType.tetrahedron)
+    q = fem.interior_facet_domain(pm)
+    assert np.array_equal(q, om.interior_facet_domain())
+    V = fem.functionspace(pm, ("Lagrange", 1))
+    dm, nd = om.dofmap(1)
+    h = C.c_void_p()
+    _lib.call("dfx_sparsity_create_facets", V.handle, V.handle, q.shape[0], _lib.ptr(q),
+              C.byref(h))
+    sp = SparsityPattern(h)
+    rp, cols = pattern_with_interior(dm, nd, q)
+    assert np.array_equal(sp.row_ptr, rp) and np.array_equal(sp.cols, cols)
+    bad = q.copy()
+    bad[0, 3] = 4
+    with pytest.raises(DfxError, match="out of range"):
+        _lib.call("dfx_sparsity_create_facets", V.handle, V.handle, 1, _lib.ptr(bad),
+                  C.byref(h))

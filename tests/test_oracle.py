@@ -679,3 +679,80 @@ def test_facet_matrices_bcs_and_lifting(ct, kR):
     with pytest.raises(RuntimeError, match="Entry not in sparsity"):
         O.assemble_matrix_facets(kR, m.x, m.cells, dm, co, pairs, None, None,
                                  np.arange(nd + 1, dtype=np.int64), np.arange(nd, dtype=np.int32))
+
+
+def pattern_with_interior(dm, nd, quads):
+    """Brute-force rows: cell couplings plus all dofs of both cells of each
+    interior facet."""
+    rows = [set() for _ in range(nd)]
+    for d in dm:
+        for r in d:
+            rows[r].update(d.tolist())
+    for c0, _, c1, _ in quads:
+        joint = np.concatenate([dm[c0], dm[c1]])
+        for r in joint:
+            rows[r].update(joint.tolist())
+    rp = np.zeros(nd + 1, np.int64)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    cols = np.array([c for r in rows for c in sorted(r)], np.int32)
+    return rp, cols
+
+
+def test_interior_facet_domain_brute_force():
+    """Every facet shared by two cells once, as (cell0, lf0, cell1, lf1)
+    with cell0 < cell1 and both local facets naming the same vertex set, in
+    facet-index order."""
+    m = O.Mesh((3, 2, 2), O.TET)
+    seen = {}
+    for c in range(m.num_cells):
+        for lf, fv in enumerate(TET_FACETS):
+            seen.setdefault(tuple(sorted(m.cells[c][list(fv)])), []).append((c, lf))
+    shared = {tuple(v[0] + v[1]) for v in seen.values() if len(v) == 2}
+    q = m.interior_facet_domain()
+    assert {tuple(r) for r in q} == shared and len(q) == len(shared)
+    assert np.all(q[:, 0] < q[:, 2])
+    c2f, _ = m.entities(2)
+    idx = [c2f[c, lf] for c, lf, _, _ in q]
+    assert all(a < b for a, b in zip(idx, idx[1:]))
+    assert all(c2f[c0, l0] == c2f[c1, l1] for c0, l0, c1, l1 in q)
+
+
+def test_interior_gradient_jump_closed_forms():
+    """gamma [grad u.n][grad v.n] dS on P1: A u = 0 for every linear u;
+    u = max(x - 1/2, 0) (piecewise linear on a box with a mesh plane at
+    x = 1/2) has u' A u = gamma * 1 (a unit jump over the unit square);
+    A symmetric positive semidefinite; the pattern needs the facet
+    couplings (the cell pattern raises)."""
+    m = O.Mesh((4, 3, 2), O.TET)
+    dm, nd = m.dofmap(1)
+    q = m.interior_facet_domain()
+    rp, cols = pattern_with_interior(dm, nd, q)
+    gamma = 1.75
+    A = O.assemble_matrix_interior(O.K_P1TET_GRADJUMP_A, m.x, m.cells, dm, q, None, None, rp,
+                                   cols, [gamma])
+    D = sp.csr_matrix((A, cols, rp), shape=(nd, nd))
+    lin = 0.3 + m.x[:, 0] - 2.0 * m.x[:, 1] + 0.5 * m.x[:, 2]
+    assert np.abs(D @ lin).max() <= 1e-12 * np.abs(D).max()
+    kink = np.maximum(m.x[:, 0] - 0.5, 0.0)
+    assert np.isclose(kink @ (D @ kink), gamma, rtol=1e-13)
+    assert abs(D - D.T).max() <= 1e-14 * np.abs(D).max()
+    assert np.linalg.eigvalsh(D.toarray()).min() >= -1e-12 * np.abs(D).max()
+    crp, ccols = O.sparsity(dm, dm, nd)
+    with pytest.raises(RuntimeError, match="Entry not in sparsity"):
+        O.assemble_matrix_interior(O.K_P1TET_GRADJUMP_A, m.x, m.cells, dm, q, None, None, crp,
+                                   ccols, [gamma])
+    # BC rows and columns vanish, the rest is unchanged; lifting is the
+    # dense column product
+    bn = m.boundary_nodes(1, dm)
+    mark = np.zeros(nd, np.int8)
+    mark[bn] = 1
+    Ab = O.assemble_matrix_interior(O.K_P1TET_GRADJUMP_A, m.x, m.cells, dm, q, mark, mark, rp,
+                                    cols, [gamma])
+    rows = np.repeat(np.arange(nd), np.diff(rp))
+    keep = (mark[rows] == 0) & (mark[cols] == 0)
+    assert np.array_equal(Ab[keep], A[keep]) and np.all(Ab[~keep] == 0.0)
+    g = np.zeros(nd)
+    g[bn] = np.sin(np.arange(len(bn)) + 0.5)
+    b = O.lift_bc_interior(np.zeros(nd), O.K_P1TET_GRADJUMP_A, m.x, m.cells, dm, q, [gamma], g,
+                           mark, alpha=1.3)
+    assert np.allclose(b, -1.3 * (D @ g), rtol=0, atol=1e-13 * np.abs(D @ g).max())
