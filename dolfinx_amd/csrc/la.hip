@@ -474,7 +474,7 @@ __global__ void __launch_bounds__(kBlock)
 /// reference's row loop; padding is skipped. Workgroups take groups of 4
 /// slices, each XCD a contiguous range. With dotv, partial sums of y.dotv
 /// go to partials.
-template <typename T>
+template <typename T, int G>
 __global__ void __launch_bounds__(kBlock)
     sell_spmv_k(std::int32_t nrows, std::int32_t nslices, const std::int64_t* __restrict__ off,
                 const std::int32_t* __restrict__ cbase, const std::int32_t* __restrict__ rp,
@@ -484,7 +484,6 @@ __global__ void __launch_bounds__(kBlock)
 {
   if (stop && block_flag(stop))
     return;
-  constexpr int G = 16;
   double d[1] = {0.0};
   const int lane = threadIdx.x % kSlice, wave = threadIdx.x / kSlice;
   const std::int32_t ngroups = (nslices + 3) / 4;
@@ -1195,10 +1194,27 @@ int launch_cg_spmv(const dfx_csr& A, const T* p, T* q, CGState* st, bool sell)
     // the partials every update kernel sums
     const int ngroups = (A.nslices + 3) / 4;
     const int nb = std::max(1, std::min(ngroups, kMaxPartials / 2));
-    sell_spmv_k<T><<<nb, kBlock, 0, s>>>(
-        A.nrows, A.nslices, A.sell_off, A.sell_base, A.row_ptr,
-        static_cast<const unsigned char*>(A.sell_cols), static_cast<const T*>(A.sell_vals), p, q,
-        0, p, st->partials, &st->done);
+    // entries per step: the width among 16, 12, 9, 8 that pads the longest
+    // row least (27-entry hexahedral rows: 3 steps of 9, no clamped loads)
+    int G = 16;
+    for (int g : {12, 9, 8})
+      if ((A.max_row + g - 1) / g * g < (A.max_row + G - 1) / G * G)
+        G = g;
+    auto go = [&](auto kern)
+    {
+      kern<<<nb, kBlock, 0, s>>>(A.nrows, A.nslices, A.sell_off, A.sell_base, A.row_ptr,
+                                 static_cast<const unsigned char*>(A.sell_cols),
+                                 static_cast<const T*>(A.sell_vals), p, q, 0, p, st->partials,
+                                 &st->done);
+    };
+    if (G == 9)
+      go(sell_spmv_k<T, 9>);
+    else if (G == 12)
+      go(sell_spmv_k<T, 12>);
+    else if (G == 8)
+      go(sell_spmv_k<T, 8>);
+    else
+      go(sell_spmv_k<T, 16>);
     return nb;
   }
   if (A.bs == 1 && lanes_per_row(A) == 1)
