@@ -1722,7 +1722,8 @@ __global__ void __launch_bounds__(kBlock)
 /// A[r, :] += the facet matrices' rows at row r, facet by facet in facet
 /// order (one thread per row over its facet incidences finc = 4 e + facet
 /// vertex); entries in a BC row or column add zero. The column's slot is
-/// found by binary search in the row; an absent one sets *err.
+/// found by binary search in the row; an absent one sets *err (err may be
+/// null once the pattern is known to hold every entry).
 template <typename T, bool HEX>
 __global__ void __launch_bounds__(kBlock)
     facet_matrix_rows_k(std::int32_t nplan, const std::int32_t* __restrict__ off,
@@ -1765,7 +1766,8 @@ __global__ void __launch_bounds__(kBlock)
       }
       if (lo == re || cols[lo] != col)
       {
-        *err = 1;
+        if (err)
+          *err = 1;
         continue;
       }
       const bool zero = rmask || (mark && mark[col]);
@@ -1919,7 +1921,7 @@ __global__ void __launch_bounds__(kBlock)
 
 /// A[r, :] += the macro-element rows at row r, facet by facet and macro
 /// index by macro index (one thread per row); BC rows and columns add zero;
-/// slots by binary search, an absent one sets *err.
+/// slots by binary search, an absent one sets *err (if err is not null).
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     interior_rows_k(std::int32_t nplan, const std::int32_t* __restrict__ off,
@@ -1957,7 +1959,8 @@ __global__ void __launch_bounds__(kBlock)
       }
       if (lo == re || cols[lo] != col)
       {
-        *err = 1;
+        if (err)
+          *err = 1;
         continue;
       }
       const bool zero = rmask || (mark && mark[col]);
@@ -3490,8 +3493,10 @@ void facet_matrix_rows(const dfx_form& f, dfx_csr& A, const dfx_bc* bc)
   if (f.nfacet == 0)
     return;
   facet_matrices<T>(f);
-  int* err = static_cast<int*>(dmalloc(sizeof(int)));
-  DFX_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
+  const bool check = f.facet_pattern != A.pattern_id;
+  int* err = check ? static_cast<int*>(dmalloc(sizeof(int))) : nullptr;
+  if (check)
+    DFX_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
   const std::int32_t np = f.max_dof + 1;
   const std::int8_t* mark = bc ? bc->marker : nullptr;
   auto* vals = static_cast<T*>(A.values);
@@ -3505,11 +3510,14 @@ void facet_matrix_rows(const dfx_form& f, dfx_csr& A, const dfx_bc* bc)
         np, f.finc_off, f.finc, f.facets, f.dofmap, f.d.dofs_per_cell, fm, mark, A.row_ptr,
         A.cols, vals, err);
   check_launch("facet matrix rows");
+  if (!check)
+    return;
   int h = 0;
   d2h(&h, err, sizeof(int));
   dfree(err);
   if (h)
     throw Error(DFX_ERR_PATTERN, "Entry not in sparsity");
+  const_cast<dfx_form&>(f).facet_pattern = A.pattern_id;
 }
 
 /// Interior facet macro-element matrices into f.fvals.
@@ -3529,18 +3537,23 @@ void interior_rows(const dfx_form& f, dfx_csr& A, const dfx_bc* bc)
   if (f.nfacet == 0)
     return;
   interior_matrices<T>(f);
-  int* err = static_cast<int*>(dmalloc(sizeof(int)));
-  DFX_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
+  const bool check = f.facet_pattern != A.pattern_id;
+  int* err = check ? static_cast<int*>(dmalloc(sizeof(int))) : nullptr;
+  if (check)
+    DFX_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
   const std::int32_t np = f.max_dof + 1;
   interior_rows_k<T><<<blocks_for(np), kBlock, 0, stream()>>>(
       np, f.finc_off, f.finc, f.facets, f.dofmap, static_cast<const T*>(f.fvals),
       bc ? bc->marker : nullptr, A.row_ptr, A.cols, static_cast<T*>(A.values), err);
   check_launch("interior facet rows");
+  if (!check)
+    return;
   int h = 0;
   d2h(&h, err, sizeof(int));
   dfree(err);
   if (h)
     throw Error(DFX_ERR_PATTERN, "Entry not in sparsity");
+  const_cast<dfx_form&>(f).facet_pattern = A.pattern_id;
 }
 
 /// Lifting by the interior facet matrices.

@@ -283,14 +283,19 @@ struct dfx_form
   std::int32_t team_rows_at = 0;
   std::int64_t team_cap = 0;
   std::uint64_t team_pattern = 0;
-  // Exterior facet integrals: (cell, local facet) pairs, per row the
+  // Facet integrals. Exterior: (cell, local facet) pairs, per row the
   // facet incidences (4 e + facet vertex, entity order) and each facet's
-  // vertex contributions (nfacet x 4 of dtype).
+  // vertex contributions (nfacet x 4) or matrix (nfacet x 16) of dtype.
+  // Interior: (cell0, lf0, cell1, lf1) quads, incidences 8 e + macro
+  // index, macro-element matrices (nfacet x 64).
   std::int32_t nfacet = 0;
   std::int32_t* facets = nullptr;
   std::int32_t* finc_off = nullptr;
   std::int32_t* finc = nullptr;
   void* fvals = nullptr;
+  // pattern whose slots were found for every facet entry (later
+  // assemblies into it skip the check and its host synchronisation)
+  std::uint64_t facet_pattern = 0;
   void* cell_elem = nullptr;
   int row_variant = 1;             // row-kernel variant (dfx_form_autotune)
 };

@@ -89,6 +89,11 @@ def test_robin_matrix_matches_oracle(ct, rname, kR, aname, kA, dtype, tol):
     O.set_diagonal(ref, bn, 1, 1.0, rp, cols)
     assert np.array_equal(A.indptr, rp) and np.array_equal(A.indices, cols)
     assert rel_err(A.data, ref) <= tol
+    # a second assembly into the same pattern (entries already checked) adds
+    fem.assemble_matrix(aR, [bc], 1.0, A)
+    O.assemble_matrix_facets(kR, om.x, om.cells, dm, co, pairs, mark, mark, rp, cols, ref)
+    O.set_diagonal(ref, bn, 1, 1.0, rp, cols)
+    assert rel_err(A.data, ref) <= tol
     # cell integral + facet integral on the face x = 1
     right = fem.facet_domain(pm, 0, 1.0)
     aC = fem.form(aname, V, fem.pack_cellwise(pm, kap_in), [0.25], dtype)

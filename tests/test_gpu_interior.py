@@ -53,6 +53,12 @@ def test_gradient_jump_matches_oracle(shuffled, dtype, tol):
                                      cols, [gamma])
     O.set_diagonal(ref, bn, 1, 1.0, rp, cols)
     assert rel_err(A.data, ref) <= tol
+    # again into the same pattern (entries already checked): adds
+    fem.assemble_matrix(a, [bc], 1.0, A)
+    O.assemble_matrix_interior(O.K_P1TET_GRADJUMP_A, om.x, om.cells, dm, q, mark, mark, rp, cols,
+                               [gamma], ref)
+    O.set_diagonal(ref, bn, 1, 1.0, rp, cols)
+    assert rel_err(A.data, ref) <= tol
     # stiffness + gradient jump (cells first), kappa per cell
     kap_in = np.random.default_rng(2).uniform(1.0, 2.0, om.num_cells)
     aC = fem.form("p1_stiffness_reaction", V, fem.pack_cellwise(pm, kap_in), [0.1], dtype)
