@@ -195,7 +195,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "one GPU only: the assembly path is measured on a single MI355X; --gpus must be 1"}))
         sys.exit(2)
 
     ensure_built()
