@@ -174,6 +174,82 @@ def other_configs(steps):
     return out
 
 
+def facet_integrals(steps, V_hex, A_hex, do_cpu):
+    """Facet integrals on the GPU, each a dfx_assemble_matrix of a facet
+    form timed with events around the call, with the oracle's serial loop
+    on the same facets beside it: beta u v ds over the 98,304 boundary
+    facets of the 128^3 hexahedra (into the headline matrix, add mode), and
+    gamma [grad u.n][grad v.n] dS over the 387,072 interior facets of the
+    32^3 tetrahedra. Algorithmic bytes per facet: the entity (8 / 16 B), its
+    coefficient (8 B), the vertex coordinates the facet matrix uses (4 x 24
+    / 8 x 24 B) and a read and a write of every entry of its matrix (16 x 16
+    / 64 x 16 B)."""
+    from dolfinx_amd import _lib, fem, mesh as dmesh
+    from dolfinx_amd.mesh import CellType
+
+    out = {}
+
+    def kernel_ms(call):
+        call()
+        sync()
+        ts = []
+        for _ in range(max(steps, 5)):
+            record(0)
+            call()
+            record(1)
+            ts.append(event_ms(0, 1))
+        return float(np.median(ts))
+
+    pairs = fem.facet_domain(V_hex.mesh)
+    nc = V_hex.mesh.topology.cells.shape[0]
+    beta = np.full((nc, 1), 2.0)
+    aR = fem.form("q1_robin", V_hex, beta, facets=pairs)
+    ms = kernel_ms(lambda: _lib.call("dfx_assemble_matrix", A_hex.handle, aR.handle, None))
+    nbytes = len(pairs) * (8 + 8 + 4 * 24 + 16 * 16)
+    rec = {"workload": "beta u v ds on the exterior facets of 128^3 hexahedra (Q1)",
+           "facets": int(len(pairs)), "kernel_ms": round(ms, 4),
+           "facets_per_s": round(len(pairs) / ms * 1e3, 1),
+           "roofline": {"bound": "hbm", "achieved": round(nbytes / ms / 1e6, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4)}}
+    if do_cpu:
+        import oracle as O
+
+        xd = np.ascontiguousarray(V_hex.mesh.geometry.dofmap)
+        vals = np.zeros(A_hex.nnz)
+        t0 = time.perf_counter()
+        O.assemble_matrix_facets(O.K_Q1HEX_ROBIN_A, V_hex.mesh.geometry.x, xd, V_hex.dofmap.list,
+                                 beta, pairs, None, None, A_hex.indptr, A_hex.indices, vals)
+        rec["cpu_baseline_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    out["exterior_robin_hex_128"] = rec
+    del aR
+
+    m = dmesh.create_box([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], (32, 32, 32), CellType.tetrahedron)
+    V = fem.functionspace(m, ("Lagrange", 1))
+    q = fem.interior_facet_domain(m)
+    aJ = fem.form("p1_gradient_jump", V, None, [0.5], facets=q)
+    A = fem.create_matrix(aJ)
+    ms = kernel_ms(lambda: _lib.call("dfx_assemble_matrix", A.handle, aJ.handle, None))
+    nbytes = len(q) * (16 + 8 * 24 + 64 * 16)
+    rec = {"workload": "gamma [grad u.n][grad v.n] dS on the interior facets of 32^3 "
+                       "tetrahedra (P1)",
+           "facets": int(len(q)), "kernel_ms": round(ms, 4),
+           "facets_per_s": round(len(q) / ms * 1e3, 1),
+           "roofline": {"bound": "hbm", "achieved": round(nbytes / ms / 1e6, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4)}}
+    if do_cpu:
+        import oracle as O
+
+        vals = np.zeros(A.nnz)
+        t0 = time.perf_counter()
+        O.assemble_matrix_interior(O.K_P1TET_GRADJUMP_A, m.geometry.x, m.geometry.dofmap,
+                                   V.dofmap.list, q, None, None, A.indptr, A.indices, [0.5], vals)
+        rec["cpu_baseline_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    out["interior_gradjump_tet_32"] = rec
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -373,6 +449,11 @@ def main():
         })
         del svc
 
+    facets_out = None
+    if not args.no_configs:
+        facets_out = facet_integrals(args.steps, p64.V, A64, not args.no_cpu)
+        print(f"# facet integrals: {facets_out}", file=sys.stderr, flush=True)
+
     configs_out = None
     if not args.no_configs:
         del A32, b32
@@ -426,6 +507,7 @@ def main():
         **out_solve,
         "serve": serve_out,
         "other_configs": configs_out,
+        "facet_integrals": facets_out,
         "cpu_baseline": cpu,
         "setup_s": round(setup_s, 2),
     }

@@ -1733,7 +1733,7 @@ __global__ void __launch_bounds__(kBlock)
                         const T* __restrict__ fm, const std::int8_t* __restrict__ mark,
                         const std::int32_t* __restrict__ row_ptr,
                         const std::int32_t* __restrict__ cols, T* __restrict__ values,
-                        int* __restrict__ err)
+                        int* __restrict__ err, std::uint8_t* __restrict__ fslot)
 {
   const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
   if (r >= nplan)
@@ -1770,6 +1770,8 @@ __global__ void __launch_bounds__(kBlock)
           *err = 1;
         continue;
       }
+      if (fslot)
+        fslot[4 * std::int64_t(p) + b] = static_cast<std::uint8_t>(lo - rb);
       const bool zero = rmask || (mark && mark[col]);
       values[lo] += zero ? T(0) : fm[16 * std::int64_t(e) + 4 * a + b];
     }
@@ -1929,7 +1931,7 @@ __global__ void __launch_bounds__(kBlock)
                     const std::int32_t* __restrict__ dofmap, const T* __restrict__ fm,
                     const std::int8_t* __restrict__ mark, const std::int32_t* __restrict__ row_ptr,
                     const std::int32_t* __restrict__ cols, T* __restrict__ values,
-                    int* __restrict__ err)
+                    int* __restrict__ err, std::uint8_t* __restrict__ fslot)
 {
   const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
   if (r >= nplan)
@@ -1963,10 +1965,55 @@ __global__ void __launch_bounds__(kBlock)
           *err = 1;
         continue;
       }
+      if (fslot)
+        fslot[8 * std::int64_t(p) + j] = static_cast<std::uint8_t>(lo - rb);
       const bool zero = rmask || (mark && mark[col]);
       values[lo] += zero ? T(0) : fm[64 * std::int64_t(e) + 8 * i + j];
     }
   }
+}
+
+/// Facet matrix rows once their slots are known (fslot): one thread per
+/// row with facet incidences, the row's values staged in LDS, the
+/// contributions added there in the same order as the search kernels add
+/// them (facet, then entry), the row written back. W entries per
+/// incidence (NF used), EW values per facet matrix row, rows of at most
+/// kFacetRowMax entries.
+constexpr int kFacetRowMax = 64;
+template <typename T, int NF, int W, int SH>
+__global__ void __launch_bounds__(64)
+    facet_rows_fast_k(std::int32_t nfrows, const std::int32_t* __restrict__ frows,
+                      const std::int32_t* __restrict__ off, const std::int32_t* __restrict__ finc,
+                      const std::uint8_t* __restrict__ fslot, const T* __restrict__ fm,
+                      const std::int8_t* __restrict__ mark, const std::int32_t* __restrict__ row_ptr,
+                      const std::int32_t* __restrict__ cols, T* __restrict__ values)
+{
+  __shared__ T acc[kFacetRowMax][64];
+  const int t = threadIdx.x;
+  const std::int32_t i = blockIdx.x * 64 + t;
+  if (i >= nfrows)
+    return;
+  const std::int32_t r = frows[i];
+  const std::int32_t rb = row_ptr[r], len = row_ptr[r + 1] - rb;
+  for (std::int32_t s = 0; s < len; ++s)
+    acc[s][t] = values[rb + s];
+  const bool rmask = mark && mark[r];
+  for (std::int32_t p = off[r]; p < off[r + 1]; ++p)
+  {
+    const std::int32_t e = finc[p] >> SH;
+    const int a = finc[p] & ((1 << SH) - 1);
+    const T* fr = fm + std::int64_t(e) * (W * W) + W * a;
+    const std::uint8_t* sl = fslot + std::int64_t(p) * W;
+#pragma unroll
+    for (int b = 0; b < NF; ++b)
+    {
+      const int s = sl[b];
+      const bool zero = rmask || (mark && mark[cols[rb + s]]);
+      acc[s][t] += zero ? T(0) : fr[b];
+    }
+  }
+  for (std::int32_t s = 0; s < len; ++s)
+    values[rb + s] = acc[s][t];
 }
 
 /// Lifting by the macro-element matrices: b[r] -= alpha sum over the row's
@@ -3493,22 +3540,36 @@ void facet_matrix_rows(const dfx_form& f, dfx_csr& A, const dfx_bc* bc)
   if (f.nfacet == 0)
     return;
   facet_matrices<T>(f);
+  const std::int8_t* mark = bc ? bc->marker : nullptr;
+  auto* vals = static_cast<T*>(A.values);
+  const auto* fm = static_cast<const T*>(f.fvals);
+  const bool hex = f.d.kernel == DFX_K_Q1HEX_ROBIN_A;
+  if (f.facet_pattern == A.pattern_id && A.max_row <= kFacetRowMax)
+  {
+    const int nb = static_cast<int>((f.nfrows + 63) / 64);
+    if (hex)
+      facet_rows_fast_k<T, 4, 4, 2><<<nb, 64, 0, stream()>>>(
+          f.nfrows, f.frows, f.finc_off, f.finc, f.fslot, fm, mark, A.row_ptr, A.cols, vals);
+    else
+      facet_rows_fast_k<T, 3, 4, 2><<<nb, 64, 0, stream()>>>(
+          f.nfrows, f.frows, f.finc_off, f.finc, f.fslot, fm, mark, A.row_ptr, A.cols, vals);
+    check_launch("facet matrix rows");
+    return;
+  }
   const bool check = f.facet_pattern != A.pattern_id;
   int* err = check ? static_cast<int*>(dmalloc(sizeof(int))) : nullptr;
   if (check)
     DFX_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
   const std::int32_t np = f.max_dof + 1;
-  const std::int8_t* mark = bc ? bc->marker : nullptr;
-  auto* vals = static_cast<T*>(A.values);
-  const auto* fm = static_cast<const T*>(f.fvals);
-  if (f.d.kernel == DFX_K_Q1HEX_ROBIN_A)
+  std::uint8_t* fslot = check && A.max_row <= 256 ? f.fslot : nullptr;
+  if (hex)
     facet_matrix_rows_k<T, true><<<blocks_for(np), kBlock, 0, stream()>>>(
         np, f.finc_off, f.finc, f.facets, f.dofmap, f.d.dofs_per_cell, fm, mark, A.row_ptr,
-        A.cols, vals, err);
+        A.cols, vals, err, fslot);
   else
     facet_matrix_rows_k<T, false><<<blocks_for(np), kBlock, 0, stream()>>>(
         np, f.finc_off, f.finc, f.facets, f.dofmap, f.d.dofs_per_cell, fm, mark, A.row_ptr,
-        A.cols, vals, err);
+        A.cols, vals, err, fslot);
   check_launch("facet matrix rows");
   if (!check)
     return;
@@ -3517,7 +3578,7 @@ void facet_matrix_rows(const dfx_form& f, dfx_csr& A, const dfx_bc* bc)
   dfree(err);
   if (h)
     throw Error(DFX_ERR_PATTERN, "Entry not in sparsity");
-  const_cast<dfx_form&>(f).facet_pattern = A.pattern_id;
+  const_cast<dfx_form&>(f).facet_pattern = A.max_row <= 256 ? A.pattern_id : 0;
 }
 
 /// Interior facet macro-element matrices into f.fvals.
@@ -3537,6 +3598,14 @@ void interior_rows(const dfx_form& f, dfx_csr& A, const dfx_bc* bc)
   if (f.nfacet == 0)
     return;
   interior_matrices<T>(f);
+  if (f.facet_pattern == A.pattern_id && A.max_row <= kFacetRowMax)
+  {
+    facet_rows_fast_k<T, 8, 8, 3><<<static_cast<int>((f.nfrows + 63) / 64), 64, 0, stream()>>>(
+        f.nfrows, f.frows, f.finc_off, f.finc, f.fslot, static_cast<const T*>(f.fvals),
+        bc ? bc->marker : nullptr, A.row_ptr, A.cols, static_cast<T*>(A.values));
+    check_launch("interior facet rows");
+    return;
+  }
   const bool check = f.facet_pattern != A.pattern_id;
   int* err = check ? static_cast<int*>(dmalloc(sizeof(int))) : nullptr;
   if (check)
@@ -3544,7 +3613,8 @@ void interior_rows(const dfx_form& f, dfx_csr& A, const dfx_bc* bc)
   const std::int32_t np = f.max_dof + 1;
   interior_rows_k<T><<<blocks_for(np), kBlock, 0, stream()>>>(
       np, f.finc_off, f.finc, f.facets, f.dofmap, static_cast<const T*>(f.fvals),
-      bc ? bc->marker : nullptr, A.row_ptr, A.cols, static_cast<T*>(A.values), err);
+      bc ? bc->marker : nullptr, A.row_ptr, A.cols, static_cast<T*>(A.values), err,
+      check && A.max_row <= 256 ? f.fslot : nullptr);
   check_launch("interior facet rows");
   if (!check)
     return;
@@ -3553,7 +3623,7 @@ void interior_rows(const dfx_form& f, dfx_csr& A, const dfx_bc* bc)
   dfree(err);
   if (h)
     throw Error(DFX_ERR_PATTERN, "Entry not in sparsity");
-  const_cast<dfx_form&>(f).facet_pattern = A.pattern_id;
+  const_cast<dfx_form&>(f).facet_pattern = A.max_row <= 256 ? A.pattern_id : 0;
 }
 
 /// Lifting by the interior facet matrices.
@@ -4037,6 +4107,19 @@ dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out)
       });
 }
 
+/// The rows with facet incidences and room for their slots (nslot bytes).
+void set_facet_rows(dfx_form& f, const std::vector<std::int32_t>& off, std::int64_t nslot)
+{
+  std::vector<std::int32_t> rows;
+  for (std::size_t r = 0; r + 1 < off.size(); ++r)
+    if (off[r + 1] > off[r])
+      rows.push_back(static_cast<std::int32_t>(r));
+  f.nfrows = static_cast<std::int32_t>(rows.size());
+  f.frows = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * std::max<std::size_t>(rows.size(), 1)));
+  h2d(f.frows, rows.data(), sizeof(std::int32_t) * rows.size());
+  f.fslot = static_cast<std::uint8_t*>(dmalloc(std::max<std::int64_t>(nslot, 1)));
+}
+
 dfx_status dfx_form_create_facets(const dfx_form_desc* desc, int32_t npairs,
                                   const int32_t* pairs, dfx_form** out)
 {
@@ -4089,6 +4172,8 @@ dfx_status dfx_form_create_facets(const dfx_form_desc* desc, int32_t npairs,
         f->finc = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * finc.size()));
         h2d(f->finc, finc.data(), sizeof(std::int32_t) * finc.size());
         f->fvals = dmalloc(dtype_size(desc->dtype) * (matrix ? 16 : 4) * std::max(npairs, 1));
+        if (matrix)
+          set_facet_rows(*f, off, 4 * std::int64_t(npairs) * nfv);
         *out = f.release();
       });
 }
@@ -4135,6 +4220,7 @@ dfx_status dfx_form_create_interior_facets(const dfx_form_desc* desc, int32_t nf
         f->finc = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * finc.size()));
         h2d(f->finc, finc.data(), sizeof(std::int32_t) * finc.size());
         f->fvals = dmalloc(dtype_size(desc->dtype) * 64 * std::max(nfacets, 1));
+        set_facet_rows(*f, off, 8 * 8 * std::int64_t(nfacets));
         *out = f.release();
       });
 }
@@ -4163,6 +4249,8 @@ dfx_status dfx_form_destroy(dfx_form* f)
     dfree(f->finc_off);
     dfree(f->finc);
     dfree(f->fvals);
+    dfree(f->fslot);
+    dfree(f->frows);
     dfree(f->cell_elem);
     dfree(f->corner_cell);
     dfree(f->slotmap);

@@ -294,8 +294,14 @@ struct dfx_form
   std::int32_t* finc = nullptr;
   void* fvals = nullptr;
   // pattern whose slots were found for every facet entry (later
-  // assemblies into it skip the check and its host synchronisation)
+  // assemblies into it skip the check and its host synchronisation), the
+  // slots found (offset in the row of entry b of incidence p at
+  // fslot[p W + b], W = 4 exterior / 8 interior), and the rows that have
+  // facet incidences
   std::uint64_t facet_pattern = 0;
+  std::uint8_t* fslot = nullptr;
+  std::int32_t* frows = nullptr;
+  std::int32_t nfrows = 0;
   void* cell_elem = nullptr;
   int row_variant = 1;             // row-kernel variant (dfx_form_autotune)
 };
