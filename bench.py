@@ -175,10 +175,12 @@ def other_configs(steps):
 
 
 def facet_integrals(steps, V_hex, A_hex, do_cpu):
-    """Facet integrals on the GPU, each a dfx_assemble_matrix of a facet
-    form timed with events around the call, with the oracle's serial loop
-    on the same facets beside it: beta u v ds over the 98,304 boundary
-    facets of the 128^3 hexahedra (into the headline matrix, add mode), and
+    """Facet integrals on the GPU, each one assembly call of a facet form
+    timed with events around the call, with the oracle's serial loop on the
+    same facets beside it for the matrices: beta u v ds over the 98,304
+    boundary facets of the 128^3 hexahedra (into the headline matrix, add
+    mode), g v ds on the same facets (vector; per facet its pair, 4 values
+    of g, 4 vertices and 4 entries read and written: 200 B), and
     gamma [grad u.n][grad v.n] dS over the 387,072 interior facets of the
     32^3 tetrahedra. Algorithmic bytes per facet: the entity (8 / 16 B), its
     coefficient (8 B), the vertex coordinates the facet matrix uses (4 x 24
@@ -223,6 +225,22 @@ def facet_integrals(steps, V_hex, A_hex, do_cpu):
         rec["cpu_baseline_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
     out["exterior_robin_hex_128"] = rec
     del aR
+
+    # g v ds on the same facets (g = 1 at every vertex), into a vector
+    from dolfinx_amd import la as dla
+
+    aN = fem.form("q1_neumann", V_hex, np.ones((nc, 8)), facets=pairs)
+    bN = dla.Vector(V_hex.num_dofs)
+    ms = kernel_ms(lambda: _lib.call("dfx_assemble_vector", bN.handle, aN.handle))
+    nbytes = len(pairs) * (8 + 4 * 8 + 4 * 24 + 4 * 16)
+    out["exterior_neumann_hex_128"] = {
+        "workload": "g v ds on the exterior facets of 128^3 hexahedra (Q1, g at the vertices)",
+        "facets": int(len(pairs)), "kernel_ms": round(ms, 4),
+        "facets_per_s": round(len(pairs) / ms * 1e3, 1),
+        "roofline": {"bound": "hbm", "achieved": round(nbytes / ms / 1e6, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4)}}
+    del aN, bN
 
     m = dmesh.create_box([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], (32, 32, 32), CellType.tetrahedron)
     V = fem.functionspace(m, ("Lagrange", 1))

@@ -2052,16 +2052,18 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 /// b[r] += the facet contributions at row r in facet order (one thread per
-/// row over its facet incidences finc = 4 e + facet vertex).
+/// row with facet incidences, rows[i], over its incidences finc = 4 e +
+/// facet vertex).
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-    facet_rows_k(std::int32_t nplan, const std::int32_t* __restrict__ off,
-                 const std::int32_t* __restrict__ finc, const T* __restrict__ fv,
-                 T* __restrict__ b)
+    facet_rows_k(std::int32_t nrows, const std::int32_t* __restrict__ rows,
+                 const std::int32_t* __restrict__ off, const std::int32_t* __restrict__ finc,
+                 const T* __restrict__ fv, T* __restrict__ b)
 {
-  const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= nplan)
+  const std::int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nrows)
     return;
+  const std::int32_t r = rows[i];
   const std::int32_t p1 = off[r + 1];
   T s = 0;
   for (std::int32_t p = off[r]; p < p1; ++p)
@@ -3674,8 +3676,8 @@ void facet_rows(const dfx_form& f, T* b)
   else
     facet_load_k<T, false><<<blocks_for(f.nfacet), kBlock, 0, stream()>>>(
         f.nfacet, f.facets, x, f.x_dofmap, f.d.nodes_per_cell, co, f.d.cstride, fv);
-  const std::int32_t np = f.max_dof + 1;
-  facet_rows_k<T><<<blocks_for(np), kBlock, 0, stream()>>>(np, f.finc_off, f.finc, fv, b);
+  facet_rows_k<T><<<blocks_for(f.nfrows), kBlock, 0, stream()>>>(f.nfrows, f.frows, f.finc_off,
+                                                                  f.finc, fv, b);
   check_launch("facet loads");
 }
 
@@ -4172,8 +4174,7 @@ dfx_status dfx_form_create_facets(const dfx_form_desc* desc, int32_t npairs,
         f->finc = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * finc.size()));
         h2d(f->finc, finc.data(), sizeof(std::int32_t) * finc.size());
         f->fvals = dmalloc(dtype_size(desc->dtype) * (matrix ? 16 : 4) * std::max(npairs, 1));
-        if (matrix)
-          set_facet_rows(*f, off, 4 * std::int64_t(npairs) * nfv);
+        set_facet_rows(*f, off, matrix ? 4 * std::int64_t(npairs) * nfv : 0);
         *out = f.release();
       });
 }
