@@ -180,7 +180,8 @@ def facet_integrals(steps, V_hex, A_hex, do_cpu):
     same facets beside it for the matrices: beta u v ds over the 98,304
     boundary facets of the 128^3 hexahedra (into the headline matrix, add
     mode), g v ds on the same facets (vector; per facet its pair, 4 values
-    of g, 4 vertices and 4 entries read and written: 200 B), and
+    of g, 4 vertices and 4 entries read and written: 200 B), caller-supplied
+    element matrices on 64^3 hexahedra (the custom-kernel path), and
     gamma [grad u.n][grad v.n] dS over the 387,072 interior facets of the
     32^3 tetrahedra. Algorithmic bytes per facet: the entity (8 / 16 B), its
     coefficient (8 B), the vertex coordinates the facet matrix uses (4 x 24
@@ -265,6 +266,39 @@ def facet_integrals(steps, V_hex, A_hex, do_cpu):
                                    V.dofmap.list, q, None, None, A.indptr, A.indices, [0.5], vals)
         rec["cpu_baseline_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
     out["interior_gradjump_tet_32"] = rec
+    del aJ, A
+
+    # caller-supplied element matrices (the custom-kernel path): 64^3 Q1
+    # hexahedra, 8 x 8 tensors resident on the device; per cell its 64
+    # entries, 8 dofs, and the CSR values once (stored)
+    mh = dmesh.create_box([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], (64, 64, 64), CellType.hexahedron)
+    Vh = fem.functionspace(mh, ("Lagrange", 1))
+    ncell = mh.topology.cells.shape[0]
+    E = np.random.default_rng(1).standard_normal(ncell * 64)
+    from dolfinx_amd import la as dla
+
+    Ed = dla.Vector.from_array(E)
+    aE = fem.element_form(Vh, 2)
+    Ah = fem.create_matrix(aE)
+    # store mode each time (zeroing only marks the values; the assembly
+    # then overwrites every entry)
+    ms = kernel_ms(lambda: (Ah.set_value(0.0),
+                            _lib.call("dfx_assemble_matrix_elements", Ah.handle, aE.handle, None,
+                                      Ed.handle)))
+    nbytes = ncell * (64 * 8 + 8 * 4) + Ah.nnz * 8
+    rec = {"workload": "caller-supplied 8x8 element matrices on 64^3 hexahedra (device-resident)",
+           "cells": int(ncell), "scalar_nnz": int(Ah.nnz), "kernel_ms": round(ms, 4),
+           "gnnz_s": round(Ah.nnz / ms / 1e6, 2),
+           "roofline": {"bound": "hbm", "achieved": round(nbytes / ms / 1e6, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4)}}
+    if do_cpu:
+        import oracle as O
+
+        t0 = time.perf_counter()
+        O.add_element_matrices(Vh.dofmap.list, 1, E, None, None, Ah.indptr, Ah.indices)
+        rec["cpu_baseline_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    out["element_matrices_hex_64"] = rec
     return out
 
 
