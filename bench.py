@@ -174,7 +174,7 @@ def other_configs(steps):
     return out
 
 
-def facet_integrals(steps, V_hex, A_hex, do_cpu):
+def facet_and_element_paths(steps, V_hex, A_hex, do_cpu):
     """Facet integrals on the GPU, each one assembly call of a facet form
     timed with events around the call, with the oracle's serial loop on the
     same facets beside it for the matrices: beta u v ds over the 98,304
@@ -503,8 +503,8 @@ def main():
 
     facets_out = None
     if not args.no_configs:
-        facets_out = facet_integrals(args.steps, p64.V, A64, not args.no_cpu)
-        print(f"# facet integrals: {facets_out}", file=sys.stderr, flush=True)
+        facets_out = facet_and_element_paths(args.steps, p64.V, A64, not args.no_cpu)
+        print(f"# facet and element paths: {facets_out}", file=sys.stderr, flush=True)
 
     configs_out = None
     if not args.no_configs:
@@ -559,7 +559,7 @@ def main():
         **out_solve,
         "serve": serve_out,
         "other_configs": configs_out,
-        "facet_integrals": facets_out,
+        "facet_and_element_paths": facets_out,
         "cpu_baseline": cpu,
         "setup_s": round(setup_s, 2),
     }
