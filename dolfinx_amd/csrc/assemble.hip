@@ -1862,14 +1862,18 @@ __device__ inline void p1_gradients(const T X[4][3], T g[4][3])
 /// grad phi.n on the first cell and -grad phi.n on the second (n the unit
 /// normal of the facet).
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(64)
     interior_matrix_k(std::int32_t nfacets, const std::int32_t* __restrict__ quads,
                       const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
                       T gamma, T* __restrict__ out)
 {
-  const std::int32_t e = blockIdx.x * kBlock + threadIdx.x;
-  if (e >= nfacets)
-    return;
+  // the 64 facets of a workgroup stage their matrices in LDS (row padded
+  // to 65 entries against bank conflicts) and store them as one
+  // contiguous, coalesced block
+  __shared__ T tile[64 * 65];
+  const int t = threadIdx.x;
+  const std::int32_t e0 = blockIdx.x * 64;
+  const std::int32_t e = min(e0 + t, nfacets - 1);
   T X[2][4][3];
 #pragma unroll
   for (int s = 0; s < 2; ++s)
@@ -1913,12 +1917,20 @@ __global__ void __launch_bounds__(kBlock)
       jv[4 * s + a] = s == 0 ? dn : -dn;
     }
   }
-  T* o = out + 64 * std::int64_t(e);
 #pragma unroll
   for (int a = 0; a < 8; ++a)
 #pragma unroll
     for (int b = 0; b < 8; ++b)
-      o[8 * a + b] = w * jv[a] * jv[b];
+      tile[t * 65 + 8 * a + b] = w * jv[a] * jv[b];
+  __syncthreads();
+  const std::int64_t nv = std::int64_t(min(64, nfacets - e0)) * 64;
+  T* o = out + 64 * std::int64_t(e0);
+  for (int k = 0; k < 64; ++k)
+  {
+    const int idx = k * 64 + t;
+    if (idx < nv)
+      o[idx] = tile[(idx >> 6) * 65 + (idx & 63)];
+  }
 }
 
 /// A[r, :] += the macro-element rows at row r, facet by facet and macro
@@ -3587,7 +3599,7 @@ void facet_matrix_rows(const dfx_form& f, dfx_csr& A, const dfx_bc* bc)
 template <typename T>
 void interior_matrices(const dfx_form& f)
 {
-  interior_matrix_k<T><<<blocks_for(f.nfacet), kBlock, 0, stream()>>>(
+  interior_matrix_k<T><<<(f.nfacet + 63) / 64, 64, 0, stream()>>>(
       f.nfacet, f.facets, static_cast<const T*>(f.x), f.x_dofmap, static_cast<T>(f.constants[0]),
       static_cast<T*>(f.fvals));
   check_launch("interior facet matrices");
