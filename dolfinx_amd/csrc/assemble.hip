@@ -1720,13 +1720,14 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 /// A[r, :] += the facet matrices' rows at row r, facet by facet in facet
-/// order (one thread per row over its facet incidences finc = 4 e + facet
-/// vertex); entries in a BC row or column add zero. The column's slot is
+/// order (one thread per row with facet incidences, rows[i], over its
+/// incidences finc = 4 e + facet vertex); entries in a BC row or column add zero. The column's slot is
 /// found by binary search in the row; an absent one sets *err (err may be
 /// null once the pattern is known to hold every entry).
 template <typename T, bool HEX>
 __global__ void __launch_bounds__(kBlock)
-    facet_matrix_rows_k(std::int32_t nplan, const std::int32_t* __restrict__ off,
+    facet_matrix_rows_k(std::int32_t nrows, const std::int32_t* __restrict__ rows,
+                        const std::int32_t* __restrict__ off,
                         const std::int32_t* __restrict__ finc,
                         const std::int32_t* __restrict__ pairs,
                         const std::int32_t* __restrict__ dofmap, int k,
@@ -1735,9 +1736,10 @@ __global__ void __launch_bounds__(kBlock)
                         const std::int32_t* __restrict__ cols, T* __restrict__ values,
                         int* __restrict__ err, std::uint8_t* __restrict__ fslot)
 {
-  const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= nplan)
+  const std::int32_t ir = blockIdx.x * kBlock + threadIdx.x;
+  if (ir >= nrows)
     return;
+  const std::int32_t r = rows[ir];
   constexpr int NF = HEX ? 4 : 3;
   const std::int32_t p1 = off[r + 1];
   if (off[r] == p1)
@@ -3574,16 +3576,16 @@ void facet_matrix_rows(const dfx_form& f, dfx_csr& A, const dfx_bc* bc)
   int* err = check ? static_cast<int*>(dmalloc(sizeof(int))) : nullptr;
   if (check)
     DFX_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
-  const std::int32_t np = f.max_dof + 1;
+  const std::int32_t np = f.nfrows;
   std::uint8_t* fslot = check && A.max_row <= 256 ? f.fslot : nullptr;
   if (hex)
     facet_matrix_rows_k<T, true><<<blocks_for(np), kBlock, 0, stream()>>>(
-        np, f.finc_off, f.finc, f.facets, f.dofmap, f.d.dofs_per_cell, fm, mark, A.row_ptr,
-        A.cols, vals, err, fslot);
+        np, f.frows, f.finc_off, f.finc, f.facets, f.dofmap, f.d.dofs_per_cell, fm, mark,
+        A.row_ptr, A.cols, vals, err, fslot);
   else
     facet_matrix_rows_k<T, false><<<blocks_for(np), kBlock, 0, stream()>>>(
-        np, f.finc_off, f.finc, f.facets, f.dofmap, f.d.dofs_per_cell, fm, mark, A.row_ptr,
-        A.cols, vals, err, fslot);
+        np, f.frows, f.finc_off, f.finc, f.facets, f.dofmap, f.d.dofs_per_cell, fm, mark,
+        A.row_ptr, A.cols, vals, err, fslot);
   check_launch("facet matrix rows");
   if (!check)
     return;
