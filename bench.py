@@ -68,9 +68,28 @@ def sync():
     _lib.call("dfx_synchronize")
 
 
-def cpu_baseline(n, kappa_input, seed):
-    """Oracle (CPU restatement of the reference loops) on the host: one full
-    FP64 matrix assembly of the same mesh with the binary-search insert."""
+def host_info():
+    """nproc, usable cores, CPU model and clock of this host (/proc/cpuinfo)."""
+    model, mhz = None, []
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name") and model is None:
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("cpu MHz"):
+                    mhz.append(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable_cores": len(os.sched_getaffinity(0)),
+            "cpu_model": model, "cpu_mhz_median": round(float(np.median(mhz)), 1) if mhz else None}
+
+
+def cpu_baseline(n, kappa_input, solutions, march):
+    """Oracle (CPU restatement of the reference loops, rebuilt here with -O3
+    -march=native) on the host, one thread: the FP64 Q1 matrix assembly of
+    the same mesh with the binary-search insert, one warm-up then the
+    median of 5 (steady clock), and its FP64 Jacobi-PCG to 1e-8, whose
+    solution is the reference for the GPU solutions' errors."""
     import oracle as O
 
     om = O.Mesh((n, n, n), O.HEX)
@@ -80,29 +99,41 @@ def cpu_baseline(n, kappa_input, seed):
     bn = om.boundary_nodes(1, dm)
     mark = np.zeros(nd, np.int8)
     mark[bn] = 1
-    t0 = time.perf_counter()
-    A = O.assemble_matrix(O.K_Q1HEX_A, om.x, om.cells, dm, 1, kap, np.zeros(1), mark, mark,
-                          rp, cols)
-    O.set_diagonal(A, bn, 1, 1.0, rp, cols)
-    t1 = time.perf_counter()
-    # a few CG iterations to price the CPU solve per iteration
+    times = []
+    for rep in range(6):
+        t0 = time.perf_counter()
+        A = O.assemble_matrix(O.K_Q1HEX_A, om.x, om.cells, dm, 1, kap, np.zeros(1), mark, mark,
+                              rp, cols)
+        O.set_diagonal(A, bn, 1, 1.0, rp, cols)
+        t1 = time.perf_counter()
+        if rep > 0:
+            times.append(t1 - t0)
+    t_asm = float(np.median(times))
     b = O.assemble_vector(O.K_Q1HEX_L, om.x, om.cells, dm, 1, np.ones((om.num_cells, 8)),
                           None, nd)
+    O.lift_bc(b, O.K_Q1HEX_A, om.x, om.cells, dm, 1, kap, np.zeros(1), np.zeros(nd), mark)
     O.bc_set(b, bn, np.zeros(nd))
-    k = 10
     t2 = time.perf_counter()
-    O.cg(rp, cols, A, 1, b, kmax=k, rtol=1e-30)
+    xr, its = O.cg(rp, cols, A, 1, b, kmax=20000, rtol=1e-8, jacobi=True)
     t3 = time.perf_counter()
-    return {
-        "value": round(len(cols) / (t1 - t0) / 1e9, 4),
+    out = {
+        "value": round(len(cols) / t_asm / 1e9, 4),
         "unit": "GNNZ/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"one FP64 Q1 matrix assembly of the {n}^3 hex mesh ({om.num_cells} cells, "
-                  f"binary-search CSR insert) = {t1 - t0:.2f} s; Jacobi-PCG "
-                  f"{(t3 - t2) / k * 1e3:.1f} ms/iteration ({k} iterations timed)",
-        "cpu_pcg_ms_per_iter": round((t3 - t2) / k * 1e3, 2),
+        "sample": f"FP64 Q1 matrix assembly of the {n}^3 hex mesh ({om.num_cells} cells, "
+                  f"binary-search CSR insert, BC rows/cols zeroed, diagonal set): median of 5 "
+                  f"after 1 warm-up = {t_asm:.3f} s (runs {', '.join(f'{t:.3f}' for t in times)}); "
+                  f"Jacobi-PCG to 1e-8: {its} iterations in {t3 - t2:.1f} s",
+        "march": march,
+        "cpu_pcg_ms_per_iter": round((t3 - t2) / max(its, 1) * 1e3, 2),
+        "cpu_pcg_iterations": int(its),
+        **host_info(),
     }
+    nr = np.linalg.norm(xr)
+    for name, x in solutions.items():
+        out[f"err_{name}_vs_oracle"] = float(np.linalg.norm(x - xr) / nr)
+    return out
 
 
 def timed_assembly(assemble, steps):
@@ -330,6 +361,13 @@ def main():
     from dolfinx_amd import _lib, fem, la, problems, solvers
     from dolfinx_amd.mesh import CellType
 
+    march = None
+    if not args.no_cpu:
+        # the oracle timed as the CPU baseline is compiled for this host
+        import oracle as O
+
+        march = O.use_native()
+
     if args.only_configs:
         print(json.dumps({"other_configs": other_configs(args.steps)}))
         return
@@ -388,6 +426,24 @@ def main():
     asm_bytes = C * 8 * 4 + Nv * 3 * 8 + C * 8 + nnz * 8
     asm_gbs = asm_bytes / (asm_ms * 1e-3) / 1e9
 
+    # ---- apply_lifting with nonzero Dirichlet data g = 1 + x on the
+    # boundary (the timed step's lifting has g = 0 and returns at once) ----
+    gvals = np.zeros(N)
+    bdofs = np.asarray(p64.bc.dofs)
+    gvals[bdofs] = 1.0 + p64.mesh.geometry.x[bdofs, 0]
+    bcg = fem.dirichletbc(p64.V, bdofs, gvals)
+    bl = la.Vector(N)
+    _lib.call("dfx_apply_lifting", bl.handle, p64.a.handle, bcg.handle, None, 1.0)
+    sync()
+    lts = []
+    for _ in range(5):
+        record(8)
+        _lib.call("dfx_apply_lifting", bl.handle, p64.a.handle, bcg.handle, None, 1.0)
+        record(9)
+        lts.append(event_ms(8, 9))
+    lift_ms = float(np.median(lts))
+    del bl, bcg
+
     # ---- FP32 assembly ----
     for _ in range(2):
         problems.assemble_system(p32, A32, b32)
@@ -441,6 +497,7 @@ def main():
     del src, dst
 
     out_solve = {}
+    solutions = {}
     if not args.no_solve:
         problems.assemble_system(p64, A64, b64)
         problems.assemble_system(p32, A32, b32)
@@ -460,6 +517,7 @@ def main():
         solvers.cg(A64, b64, xt, rtol=1e-11, max_it=40000, jacobi=True, check_every=128)
         xref = xt.array
         nref = np.linalg.norm(xref)
+        solutions = {"fp64": x64, "mixed": xmix}
         out_solve = {
             "cg_fp64_s": round(r64.seconds, 4),
             "cg_fp64_iterations": r64.iterations,
@@ -513,7 +571,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(n, p64.kappa, args.seed)
+        cpu = cpu_baseline(n, p64.kappa, solutions, march)
 
     line = {
         "metric": "assembly GNNZ/s + CG time-to-1e-8, mixed vs FP64 on GPU; serve p99 latency",
@@ -533,7 +591,8 @@ def main():
                         "f=1, u=0 on the boundary (precision configuration)",
             "cells": C, "dofs": N, "nnz": nnz,
             "step": "FP64 assemble_matrix (BC rows/cols zeroed, diag 1) + assemble_vector "
-                    "+ apply_lifting + set_bc",
+                    "+ apply_lifting (g = 0, x0 = 0: returns at once; the reference's lift_bc "
+                    "visits the boundary cells and adds exactly zero) + set_bc",
         },
         "roofline": {
             "bound": "hbm",
@@ -546,6 +605,7 @@ def main():
             "algorithmic_bytes": asm_bytes,
             "kernel_ms": round(asm_ms, 4),
         },
+        "apply_lifting_nonzero_g_ms": round(lift_ms, 4),
         "copy_gbs": round(copy_gbs, 1),
         "assembly_f32_gnnz_s": round(nnz / asm32_s / 1e9, 4),
         "assembly_variant": {"f64": variant64[0], "f32": variant32[0]},

@@ -65,6 +65,7 @@ SIGNATURES = {
     "dfx_mesh_create": (C.c_int, [C.c_int, I32, P, I32, P, PP]),
     "dfx_facet_domain": (C.c_int, [P, C.c_int, C.c_int, C.c_double, P, P]),
     "dfx_form_create_facets": (C.c_int, [P, I32, P, PP]),
+    "dfx_form_create_kernel": (C.c_int, [P, P, P, PP]),
     "dfx_form_create_interior_facets": (C.c_int, [P, I32, P, PP]),
     "dfx_interior_facet_domain": (C.c_int, [P, P, P]),
     "dfx_sparsity_create_facets": (C.c_int, [P, P, I32, P, PP]),

@@ -1048,6 +1048,212 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---------------------------------------------------------------------
+// Element tensors of any size and block size (the reference's
+// la::MatrixCSR::mat_add_values<BS, BS> for the element matrices of a
+// kernel, la/MatrixCSR.h:157-174): one thread per row dof (node r,
+// component a) walks the row's incidences (c, i) in cell order and adds
+// element row BS i + a into the blocks of its row (the thread owns
+// component row a of every block of block row r: no atomics, each entry
+// summed in cell order as MatrixCSR::add sums the cells' contributions).
+// ---------------------------------------------------------------------
+
+template <typename T, typename O, int BS>
+__global__ void __launch_bounds__(kBlock)
+    element_rows_bs_k(std::int32_t nplan, int k, const std::int32_t* __restrict__ inc_off,
+                      const std::int32_t* __restrict__ inc, const O* __restrict__ islots,
+                      const std::int32_t* __restrict__ dm, const T* __restrict__ E,
+                      const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
+                      T* __restrict__ vals)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t >= std::int64_t(nplan) * BS)
+    return;
+  if (bc && bc[t])
+    return; // BC row: no contribution
+  const std::int32_t r = static_cast<std::int32_t>(t / BS);
+  const int a = static_cast<int>(t - std::int64_t(r) * BS);
+  const std::int64_t nd = std::int64_t(BS) * k;
+  T* row = vals + std::int64_t(rp[r]) * BS * BS + BS * a;
+  for (std::int32_t p = inc_off[r]; p < inc_off[r + 1]; ++p)
+  {
+    const std::int32_t cik = inc[p];
+    const std::int32_t c = cik / k;
+    const int i = cik - c * k;
+    const T* Ei = E + std::int64_t(c) * nd * nd + (std::int64_t(BS) * i + a) * nd;
+    for (int j = 0; j < k; ++j)
+    {
+      const std::int64_t node = dm[std::int64_t(c) * k + j];
+      T* blk = row + std::int64_t(islots[std::int64_t(p) * k + j]) * BS * BS;
+#pragma unroll
+      for (int b = 0; b < BS; ++b)
+        if (!(bc && bc[BS * node + b]))
+          blk[b] += Ei[BS * j + b];
+    }
+  }
+}
+
+/// b[(r, a)] += sum over the row's incidences of e[c][BS i + a].
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock)
+    element_vector_bs_k(std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
+                        const std::int32_t* __restrict__ inc, const T* __restrict__ e,
+                        T* __restrict__ b)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t >= std::int64_t(nplan) * BS)
+    return;
+  const std::int32_t r = static_cast<std::int32_t>(t / BS);
+  const int a = static_cast<int>(t - std::int64_t(r) * BS);
+  const std::int32_t p0 = inc_off[r], p1 = inc_off[r + 1];
+  if (p1 == p0)
+    return;
+  T s = b[t];
+  for (std::int32_t p = p0; p < p1; ++p)
+    s += e[std::int64_t(inc[p]) * BS + a];
+  b[t] = s;
+}
+
+/// Lifting from element matrices: b[(r, a)] -= alpha sum over the row's
+/// incidences (c, i), in cell order, of sum over the cell's Dirichlet
+/// columns (j, d) of E_c[BS i + a][BS j + d] (g - x0)_(j, d).
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock)
+    element_lift_bs_k(std::int32_t nplan, int k, const std::int32_t* __restrict__ inc_off,
+                      const std::int32_t* __restrict__ inc, const std::int32_t* __restrict__ dm,
+                      const T* __restrict__ E, const std::int8_t* __restrict__ bc,
+                      const double* __restrict__ g, const T* __restrict__ x0, double alpha,
+                      T* __restrict__ b)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t >= std::int64_t(nplan) * BS)
+    return;
+  const std::int32_t r = static_cast<std::int32_t>(t / BS);
+  const int a = static_cast<int>(t - std::int64_t(r) * BS);
+  const std::int64_t nd = std::int64_t(BS) * k;
+  double s = 0.0;
+  for (std::int32_t p = inc_off[r]; p < inc_off[r + 1]; ++p)
+  {
+    const std::int32_t cik = inc[p];
+    const std::int32_t c = cik / k;
+    const int i = cik - c * k;
+    const T* Ei = E + std::int64_t(c) * nd * nd + (std::int64_t(BS) * i + a) * nd;
+    for (int j = 0; j < k; ++j)
+    {
+      const std::int64_t node = dm[std::int64_t(c) * k + j];
+      for (int d = 0; d < BS; ++d)
+      {
+        const std::int64_t col = BS * node + d;
+        if (!bc[col])
+          continue;
+        const double gv = g ? g[col] : 0.0;
+        const double xv = x0 ? double(x0[col]) : 0.0;
+        s += double(Ei[BS * j + d]) * (gv - xv);
+      }
+    }
+  }
+  if (s != 0.0)
+    b[t] = static_cast<T>(double(b[t]) - alpha * s);
+}
+
+typedef int dfx_i32x4 __attribute__((ext_vector_type(4)));
+
+/// Variant 12-14 (Q1 hexahedra whose vertices lie in at most 8 cells): one
+/// thread per row with closed-form entries as variants 4-6, but the row's
+/// plan sits at fixed positions -- its incidences c * 8 + i in cell order,
+/// padded with -1 to 8 (inc8), and their 64 slot offsets (islot8) -- so it
+/// is loaded together with the row bounds and the cell records of all 8
+/// incidences are the second and last dependent loads, all in flight at
+/// once. The plan streams are read and the slice is written with
+/// non-temporal accesses (each byte is used once), which leaves the L2 to
+/// the cell records that neighbouring rows share. Entries are summed in
+/// cell order as in the other variants.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    row_fixed_k(std::int32_t nrows, std::int32_t nplan, const std::int32_t* __restrict__ inc8,
+                const std::uint8_t* __restrict__ islot8, const T* __restrict__ cd,
+                const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
+                T* __restrict__ vals, int accumulate)
+{
+  extern __shared__ unsigned char smem_raw[];
+  T* acc = reinterpret_cast<T*>(smem_raw);
+  const int R = blockDim.x;
+  const std::int32_t r0 = xcd_contiguous(blockIdx.x, gridDim.x) * R;
+  const std::int32_t r1 = min(r0 + R, nrows);
+  const std::int32_t base = rp[r0];
+  const std::int32_t len = rp[r1] - base;
+  const std::int32_t r = r0 + threadIdx.x;
+  const bool live = r < r1 && r < nplan;
+  const std::int32_t rr = live ? r : 0;
+  // the row's plan and bounds: independent loads
+  const dfx_i32x4* ip = reinterpret_cast<const dfx_i32x4*>(inc8 + 8 * std::int64_t(rr));
+  const dfx_i32x4 ia = __builtin_nontemporal_load(ip);
+  const dfx_i32x4 ib = __builtin_nontemporal_load(ip + 1);
+  const dfx_i32x4* sp = reinterpret_cast<const dfx_i32x4*>(islot8 + 64 * std::int64_t(rr));
+  dfx_i32x4 sw[4];
+#pragma unroll
+  for (int h = 0; h < 4; ++h)
+    sw[h] = __builtin_nontemporal_load(sp + h);
+  const std::int32_t roff = (r < r1 ? rp[r] : base) - base;
+  const bool skip = !live || (bc && bc[rr]);
+  const std::int32_t cik[8] = {ia.x, ia.y, ia.z, ia.w, ib.x, ib.y, ib.z, ib.w};
+  T g[8][7];
+  std::uint32_t m[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u)
+  {
+    load_factors<T>(cd, cik[u] < 0 ? 0 : cik[u] >> 3, g[u], m[u]);
+    if (cik[u] < 0 || skip)
+      m[u] = 0xffu; // padding (or a skipped row): every column masked
+  }
+  for (int t = threadIdx.x; t < len; t += R)
+    acc[t] = T(0);
+  __syncthreads();
+  T* a = acc + roff;
+#pragma unroll
+  for (int u = 0; u < 8; ++u)
+  {
+    const int i = cik[u] & 7;
+    T gs[3];
+    q1_signed<T>(g[u], i, gs);
+    const std::uint32_t w0 = static_cast<std::uint32_t>(sw[u / 2][2 * (u % 2)]);
+    const std::uint32_t w1 = static_cast<std::uint32_t>(sw[u / 2][2 * (u % 2) + 1]);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+    {
+      const std::uint32_t sl = ((j < 4 ? w0 : w1) >> (8 * (j % 4))) & 0xffu;
+      if (!((m[u] >> j) & 1u))
+        a[sl] += q1_entry<T>(g[u], gs, i, j);
+    }
+  }
+  __syncthreads();
+  if (accumulate)
+    for (int t = threadIdx.x; t < len; t += R)
+      vals[base + t] += acc[t];
+  else
+    for (int t = threadIdx.x; t < len; t += R)
+      __builtin_nontemporal_store(acc[t], vals + base + t);
+}
+
+/// The fixed-width plan from the row-owner plan: for each row its
+/// incidences (padded with -1 to 8) and their 8-byte slot offsets.
+__global__ void __launch_bounds__(kBlock)
+    fixed_plan_k(std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
+                 const std::int32_t* __restrict__ inc, const std::uint8_t* __restrict__ islots,
+                 std::int32_t* __restrict__ inc8, std::uint8_t* __restrict__ islot8)
+{
+  const std::int32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= nplan)
+    return;
+  const std::int32_t p0 = inc_off[r], n = inc_off[r + 1] - p0;
+  for (int u = 0; u < 8; ++u)
+  {
+    inc8[8 * std::int64_t(r) + u] = u < n ? inc[p0 + u] : -1;
+    for (int j = 0; j < 8; ++j)
+      islot8[64 * std::int64_t(r) + 8 * u + j] = u < n ? islots[8 * std::int64_t(p0 + u) + j] : 0;
+  }
+}
+
+// ---------------------------------------------------------------------
 // Batched assembly of many independent systems of one affine kernel (no
 // counterpart in the reference): one launch forms the cell factors of all
 // systems and one assembles the rows of all systems, from device tables of
@@ -2868,6 +3074,80 @@ __global__ void __launch_bounds__(64)
   store(cur);
 }
 
+/// Lifting of the P2 elasticity form, b[(r, c)] -= alpha sum over the
+/// row's incidences (cell, i), in cell order, of sum over the cell's
+/// Dirichlet columns (j, d) of A_(i,c)(j,d) (g - x0)_(j,d); one thread per
+/// row dof (node r, component c). The 3 x 3 blocks are formed from the
+/// cell records and reference tensors as in the row kernel, for the cells
+/// whose mask marks a Dirichlet column only.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    elast_lift_k(std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
+                 const std::int32_t* __restrict__ inc, const std::int32_t* __restrict__ dm,
+                 const T* __restrict__ cd, const std::uint32_t* __restrict__ mask,
+                 const T* __restrict__ ref, const double* __restrict__ g,
+                 const T* __restrict__ x0, double alpha, T* __restrict__ b)
+{
+  constexpr int K = 10;
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t >= std::int64_t(nplan) * 3)
+    return;
+  const std::int32_t r = static_cast<std::int32_t>(t / 3);
+  const int comp = static_cast<int>(t - 3 * std::int64_t(r));
+  double s = 0.0;
+  for (std::int32_t p = inc_off[r]; p < inc_off[r + 1]; ++p)
+  {
+    const std::int32_t cik = inc[p];
+    const std::int32_t c = cik / K;
+    const std::uint32_t m = mask[c];
+    if (!m)
+      continue;
+    const int i = cik - c * K;
+    const T* rec = cd + 16 * std::int64_t(c);
+    T Ki[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+      Ki[q] = rec[q];
+    const T lam = rec[10], mu = rec[11];
+    for (int j = 0; j < K; ++j)
+    {
+      const std::uint32_t colb = (m >> (3 * j)) & 7u;
+      if (!colb)
+        continue;
+      const T* Tm = ref + (i * K + j) * 10;
+      T W[9], P[9];
+#pragma unroll
+      for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          W[aa * 3 + d] = Tm[aa * 3 + 0] * Ki[0 * 3 + d] + Tm[aa * 3 + 1] * Ki[1 * 3 + d]
+                          + Tm[aa * 3 + 2] * Ki[2 * 3 + d];
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          P[cc * 3 + d] = Ki[0 * 3 + cc] * W[0 * 3 + d] + Ki[1 * 3 + cc] * W[1 * 3 + d]
+                          + Ki[2 * 3 + cc] * W[2 * 3 + d];
+      const T tr = P[0] + P[4] + P[8];
+      const std::int64_t node = dm[std::int64_t(c) * K + j];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+      {
+        if (!((colb >> d) & 1u))
+          continue;
+        const T v = comp == d ? mu * (tr + P[comp * 4]) + lam * P[comp * 4]
+                              : mu * P[d * 3 + comp] + lam * P[comp * 3 + d];
+        const std::int64_t col = 3 * node + d;
+        const double gv = g ? g[col] : 0.0;
+        const double xv = x0 ? double(x0[col]) : 0.0;
+        s += double(v) * (gv - xv);
+      }
+    }
+  }
+  if (s != 0.0)
+    b[t] = static_cast<T>(double(b[t]) - alpha * s);
+}
+
 /// Element vectors of the tetrahedral source forms with constant data,
 /// one thread per cell, stored corner-major (be[(i BS + comp) nc + c]):
 /// P2 body force b_(i,c) = |det J| w_i f_c; P3 source b_i = |det J| w_i f
@@ -3053,7 +3333,7 @@ int incidence_lanes(const dfx_form& f)
 /// with 64/128/256 rows per workgroup; 3 "lanes"; 4..6 closed-form Q1 rows;
 /// 7, 8 incidence-parallel gathers; 9 structured Q1; 10 fused geometry;
 /// 11 as 1 with eight incidences in flight.
-constexpr int kRowVariants = 12;
+constexpr int kRowVariants = 15;
 
 template <typename T, typename O, int K, int L>
 void launch_lanes(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
@@ -3359,10 +3639,50 @@ void assemble_elements(dfx_form& f, dfx_csr& A, const dfx_bc* bc, const T* E)
     go(std::uint16_t{});
 }
 
+/// Fixed-width plan of a Q1 form (built once per pattern); false if a row
+/// has more than 8 incidences or the slot offsets are wider than a byte.
+bool ensure_fixed(dfx_form& f, const dfx_csr& A)
+{
+  if (f.max_inc > 8 || f.islot_bytes != 1)
+    return false;
+  if (f.inc8 && f.fixed_pattern == A.pattern_id)
+    return true;
+  dfree(f.inc8);
+  dfree(f.islot8);
+  f.inc8 = nullptr;
+  f.islot8 = nullptr;
+  const std::int32_t nplan = f.max_dof + 1;
+  f.inc8 = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * 8 * std::max(nplan, 1)));
+  f.islot8 = static_cast<std::uint8_t*>(dmalloc(64 * std::size_t(std::max(nplan, 1))));
+  if (nplan > 0)
+    fixed_plan_k<<<blocks_for(nplan), kBlock, 0, stream()>>>(
+        nplan, f.inc_off, f.inc, static_cast<const std::uint8_t*>(f.islots), f.inc8, f.islot8);
+  check_launch("fixed plan");
+  f.fixed_pattern = A.pattern_id;
+  return true;
+}
+
 template <typename T, typename O, int K>
 void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
 {
   const int variant = f.row_variant;
+  if (variant >= 12 && variant <= 14)
+  {
+    // fixed-width rows: 64, 128 or 256 rows per workgroup
+    require(K == 8 && f.d.kernel == DFX_K_Q1HEX_A && f.affine && ensure_fixed(f, A),
+            "variant not available for this element");
+    launch_cell_factors<T>(f, bc);
+    const int R = 64 << (variant - 12);
+    const std::size_t lds = std::size_t(R) * A.max_row * sizeof(T);
+    require(lds <= 64 * 1024, "row too long for the LDS slice");
+    const int nb = (A.nrows + R - 1) / R;
+    if (nb > 0)
+      row_fixed_k<T><<<nb, R, lds, stream()>>>(
+          A.nrows, f.max_dof + 1, f.inc8, f.islot8, static_cast<const T*>(f.cell_data),
+          bc ? bc->marker : nullptr, A.row_ptr, static_cast<T*>(A.values), A.zeroed ? 0 : 1);
+    check_launch("row_assemble");
+    return;
+  }
   if (variant == 10)
   {
     // fused geometry: no cell pass; 64 rows per workgroup
@@ -3529,6 +3849,24 @@ void lift_rows(dfx_form& f, const dfx_bc& bc, const T* x0, double alpha, T* b)
     row_lift_k<T, 20><<<blocks_for(nplan), kBlock, 0, stream()>>>(
         nplan, f.inc_off, f.inc, f.dofmap, cd, f.cell_mask, ref, bc.g, x0, alpha, b);
   check_launch("row_lift");
+}
+
+/// apply_lifting of the P2 elasticity form: cell records and the cells'
+/// Dirichlet column masks for this BC, then one thread per row dof.
+template <typename T>
+void elast_lift_rows(dfx_form& f, const dfx_bc& bc, const T* x0, double alpha, T* b)
+{
+  const std::int32_t nc = f.d.num_cells;
+  if (nc > 0)
+    elast_factors_k<T><<<blocks_for(nc), kBlock, 0, stream()>>>(
+        nc, static_cast<const T*>(f.x), f.x_dofmap, f.dofmap, static_cast<const T*>(f.coeffs),
+        f.d.cstride, bc.marker, static_cast<T*>(f.cell_data), f.cell_mask);
+  const std::int32_t nplan = f.max_dof + 1;
+  if (nplan > 0)
+    elast_lift_k<T><<<blocks_for(std::int64_t(nplan) * 3), kBlock, 0, stream()>>>(
+        nplan, f.inc_off, f.inc, f.dofmap, static_cast<const T*>(f.cell_data), f.cell_mask,
+        static_cast<const T*>(f.ref_tensors), bc.g, x0, alpha, b);
+  check_launch("elasticity lifting");
 }
 
 /// Exterior facet loads: the facets' vertex contributions, then one
@@ -3846,10 +4184,9 @@ void build_incidences(dfx_form& f, const std::int32_t* dofmap)
 /// row-owner plan only.
 std::unique_ptr<dfx_form> create_element_form(const dfx_form_desc& d, const KernelInfo& ki)
 {
-  require(d.bs == 1, "element tensors are supported for block size 1");
+  (void)ki;
+  require(d.bs == 1 || d.bs == 3, "element tensors are supported for block sizes 1 and 3");
   require(d.dofs_per_cell >= 1 && d.dofs_per_cell <= 32, "dofs per cell must be 1..32");
-  require(!ki.bilinear || element_matrix_size(d.dofs_per_cell),
-          "element matrices need k in {4, 8, 10, 20}");
   require(d.dofmap, "null dofmap");
   const std::int64_t nc = d.num_cells;
   std::int32_t max_dof = -1;
@@ -4016,6 +4353,140 @@ void batch_assemble(int n, dfx_csr* const* A, const dfx_form* const* a, const df
 } // namespace dfx
 
 using namespace dfx;
+
+/// Tensors of the fast element path (block size 1, k in {4, 8, 10, 20});
+/// other sizes take the generic row-dof kernels.
+bool fast_elements(const dfx_form& f)
+{
+  return f.d.bs == 1 && element_matrix_size(f.d.dofs_per_cell);
+}
+
+/// A += E (element matrices on the device) with BC rows/columns skipped.
+template <typename T>
+void elements_into_matrix(dfx_form& f, dfx_csr& A, const dfx_bc* bc, const T* E)
+{
+  if (fast_elements(f))
+  {
+    assemble_elements<T>(f, A, bc, E);
+    return;
+  }
+  ensure_plan(f, A);
+  materialize_zero(A); // the generic kernels add into stored values
+  const std::int32_t nplan = f.max_dof + 1;
+  const std::int64_t nt = std::int64_t(nplan) * f.d.bs;
+  if (nt > 0)
+  {
+    auto go = [&](auto o)
+    {
+      using O = decltype(o);
+      const O* sl = static_cast<const O*>(f.islots);
+      const std::int8_t* mk = bc ? bc->marker : nullptr;
+      if (f.d.bs == 3)
+        element_rows_bs_k<T, O, 3><<<blocks_for(nt), kBlock, 0, stream()>>>(
+            nplan, f.d.dofs_per_cell, f.inc_off, f.inc, sl, f.dofmap, E, mk, A.row_ptr,
+            static_cast<T*>(A.values));
+      else
+        element_rows_bs_k<T, O, 1><<<blocks_for(nt), kBlock, 0, stream()>>>(
+            nplan, f.d.dofs_per_cell, f.inc_off, f.inc, sl, f.dofmap, E, mk, A.row_ptr,
+            static_cast<T*>(A.values));
+    };
+    if (f.islot_bytes == 1)
+      go(std::uint8_t{});
+    else
+      go(std::uint16_t{});
+  }
+  check_launch("element matrix assembly");
+}
+
+/// b += e (element vectors on the device).
+template <typename T>
+void elements_into_vector(const dfx_form& L, T* b, const T* e)
+{
+  const std::int32_t nplan = L.max_dof + 1;
+  if (nplan <= 0)
+    return;
+  if (L.d.bs == 3)
+    element_vector_bs_k<T, 3><<<blocks_for(std::int64_t(nplan) * 3), kBlock, 0, stream()>>>(
+        nplan, L.inc_off, L.inc, e, b);
+  else
+    element_vector_k<T><<<blocks_for(nplan), kBlock, 0, stream()>>>(nplan, L.inc_off, L.inc, e,
+                                                                      b);
+  check_launch("element vector assembly");
+}
+
+/// b -= alpha E[:, bc] (g - x0) from element matrices on the device.
+template <typename T>
+void elements_lift(const dfx_form& a, T* b, const dfx_bc& bc, const T* E, const T* x0,
+                   double alpha)
+{
+  const std::int32_t nc = a.d.num_cells;
+  const std::int32_t nplan = a.max_dof + 1;
+  if (nplan <= 0)
+    return;
+  if (!fast_elements(a))
+  {
+    const std::int64_t nt = std::int64_t(nplan) * a.d.bs;
+    if (a.d.bs == 3)
+      element_lift_bs_k<T, 3><<<blocks_for(nt), kBlock, 0, stream()>>>(
+          nplan, a.d.dofs_per_cell, a.inc_off, a.inc, a.dofmap, E, bc.marker, bc.g, x0, alpha,
+          b);
+    else
+      element_lift_bs_k<T, 1><<<blocks_for(nt), kBlock, 0, stream()>>>(
+          nplan, a.d.dofs_per_cell, a.inc_off, a.inc, a.dofmap, E, bc.marker, bc.g, x0, alpha,
+          b);
+    check_launch("element lifting");
+    return;
+  }
+  if (nc > 0)
+    element_mask_k<<<blocks_for(nc), kBlock, 0, stream()>>>(nc, a.d.dofs_per_cell, a.dofmap,
+                                                            bc.marker, a.cell_mask);
+  element_lift_k<T><<<blocks_for(nplan), kBlock, 0, stream()>>>(
+      nplan, a.d.dofs_per_cell, a.inc_off, a.inc, a.dofmap, E, a.cell_mask, bc.g, x0, alpha, b);
+  check_launch("element lifting");
+}
+
+/// Runs the form's host element kernel (the reference's kernelptr_t,
+/// fem/kernel.h:18-20) on every cell in cell order, as
+/// assemble_cells_matrix / assemble_cells call it
+/// (fem/assemble_matrix_impl.h:144-152, fem/assemble_vector_impl.h:99-107):
+/// A zeroed, w = the cell's packed coefficients, c = the constants,
+/// coordinate_dofs = the cell's geometry nodes (3 values each), no entity
+/// index or permutation. The tensors go to the device in one copy; returns
+/// their device array.
+template <typename T>
+const T* run_host_kernel(dfx_form& f)
+{
+  using Fn = void (*)(T*, const T*, const T*, const T*, const int*, const std::uint8_t*, void*);
+  const Fn fn = reinterpret_cast<Fn>(f.host_kernel);
+  const std::int64_t nc = f.d.num_cells;
+  const int npc = f.d.nodes_per_cell;
+  const std::int64_t nd = std::int64_t(f.d.bs) * f.d.dofs_per_cell;
+  const std::int64_t sz = f.d.kernel == DFX_K_ELEMENT_A ? nd * nd : nd;
+  const int cs = f.d.cstride;
+  std::vector<T> E(std::max<std::int64_t>(nc * sz, 1), T(0));
+  std::vector<T> cdofs(3 * std::size_t(std::max(npc, 1)));
+  std::vector<T> w(std::max(cs, 1), T(0));
+  std::vector<T> cst(std::max<std::size_t>(f.h_constants.size(), 1), T(0));
+  for (std::size_t q = 0; q < f.h_constants.size(); ++q)
+    cst[q] = static_cast<T>(f.h_constants[q]);
+  for (std::int64_t c = 0; c < nc; ++c)
+  {
+    for (int i = 0; i < npc; ++i)
+    {
+      const double* xi = f.h_x.data() + 3 * std::int64_t(f.h_x_dofmap[c * npc + i]);
+      for (int r = 0; r < 3; ++r)
+        cdofs[3 * i + r] = static_cast<T>(xi[r]);
+    }
+    for (int q = 0; q < cs; ++q)
+      w[q] = static_cast<T>(f.h_coeffs[c * cs + q]);
+    fn(E.data() + c * sz, w.data(), cst.data(), cdofs.data(), nullptr, nullptr,
+       f.host_kernel_data);
+  }
+  if (!f.elem_tensors)
+    f.elem_tensors = dmalloc(sizeof(T) * E.size());
+  h2d(f.elem_tensors, E.data(), sizeof(T) * E.size());
+  return static_cast<const T*>(f.elem_tensors);
+}
 
 extern "C" {
 
@@ -4240,6 +4711,46 @@ dfx_status dfx_form_create_interior_facets(const dfx_form_desc* desc, int32_t nf
       });
 }
 
+dfx_status dfx_form_create_kernel(const dfx_form_desc* desc, void* kernel, void* custom_data,
+                                  dfx_form** out)
+{
+  return guard(
+      [&]
+      {
+        require(desc && kernel && out, "null argument");
+        require(is_element_kernel(desc->kernel),
+                "a kernel form is DFX_K_ELEMENT_A (matrix) or DFX_K_ELEMENT_L (vector)");
+        require(desc->dtype == DFX_F64 || desc->dtype == DFX_F32, "bad dtype");
+        require(desc->num_cells >= 0 && desc->num_nodes >= 0, "negative size");
+        require(desc->nodes_per_cell >= 1 && desc->x && desc->x_dofmap,
+                "the kernel needs the cell geometry (x, x_dofmap)");
+        require(desc->cstride >= 0 && (desc->cstride == 0 || desc->coeffs),
+                "null coefficients");
+        require(desc->num_constants >= 0 && (desc->num_constants == 0 || desc->constants),
+                "null constants");
+        const std::int64_t nc = desc->num_cells;
+        for (std::int64_t i = 0; i < nc * desc->nodes_per_cell; ++i)
+          require(desc->x_dofmap[i] >= 0 && desc->x_dofmap[i] < desc->num_nodes,
+                  "geometry dofmap entry out of range");
+        KernelInfo ki;
+        kernel_info(desc->kernel, ki);
+        auto f = create_element_form(*desc, ki);
+        f->d.nodes_per_cell = desc->nodes_per_cell;
+        f->d.num_nodes = desc->num_nodes;
+        f->d.cstride = desc->cstride;
+        f->d.num_constants = desc->num_constants;
+        f->h_x.assign(desc->x, desc->x + 3 * std::int64_t(desc->num_nodes));
+        f->h_x_dofmap.assign(desc->x_dofmap, desc->x_dofmap + nc * desc->nodes_per_cell);
+        if (desc->cstride > 0)
+          f->h_coeffs.assign(desc->coeffs, desc->coeffs + nc * desc->cstride);
+        if (desc->num_constants > 0)
+          f->h_constants.assign(desc->constants, desc->constants + desc->num_constants);
+        f->host_kernel = kernel;
+        f->host_kernel_data = custom_data;
+        *out = f.release();
+      });
+}
+
 dfx_status dfx_form_destroy(dfx_form* f)
 {
   if (f)
@@ -4269,6 +4780,9 @@ dfx_status dfx_form_destroy(dfx_form* f)
     dfree(f->cell_elem);
     dfree(f->corner_cell);
     dfree(f->slotmap);
+    dfree(f->inc8);
+    dfree(f->islot8);
+    dfree(f->elem_tensors);
     dfree(f->pack_map);
     dfree(f->pack_stage);
     delete f;
@@ -4283,6 +4797,15 @@ dfx_status dfx_form_set_coefficients(dfx_form* f, const double* coeffs, const do
       {
         require(f, "null form");
         const std::int64_t n = std::int64_t(f->d.num_cells) * f->d.cstride;
+        if (f->host_kernel)
+        {
+          // a host kernel reads host copies
+          if (coeffs && n > 0)
+            f->h_coeffs.assign(coeffs, coeffs + n);
+          if (constants)
+            f->h_constants.assign(constants, constants + f->d.num_constants);
+          return;
+        }
         if (coeffs && n > 0)
         {
           if (f->d.dtype == DFX_F64 && f->coeffs)
@@ -4417,13 +4940,24 @@ dfx_status dfx_assemble_matrix(dfx_csr* A, const dfx_form* a, const dfx_bc* bc)
         KernelInfo ki;
         kernel_info(a->d.kernel, ki);
         require(ki.bilinear, "assemble_matrix needs a bilinear form");
-        require(!is_element_kernel(a->d.kernel),
+        require(!is_element_kernel(a->d.kernel) || a->host_kernel,
                 "element-tensor forms are assembled with dfx_assemble_matrix_elements");
         require(A->dtype == a->d.dtype, "matrix and form dtypes differ");
         require(A->bs == a->d.bs, "matrix and form block sizes differ");
         require(!bc || bc->n_total == A->nrows * A->bs, "BC size does not match the matrix");
         require(a->max_dof < A->nrows && a->max_dof < A->ncols, "dofmap exceeds the matrix size");
         dfx_form& f = const_cast<dfx_form&>(*a);
+        if (f.host_kernel)
+        {
+          // element matrices from the host kernel, inserted on the device
+          if (A->dtype == DFX_F64)
+            elements_into_matrix<double>(f, *A, bc, run_host_kernel<double>(f));
+          else
+            elements_into_matrix<float>(f, *A, bc, run_host_kernel<float>(f));
+          A->zeroed = false;
+          A->zero_pending = false;
+          return;
+        }
         if (is_facet_matrix_kernel(f.d.kernel) || is_interior_kernel(f.d.kernel))
         {
           materialize_zero(*A);
@@ -4518,10 +5052,14 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t*
             float t = 0;
             try
             {
+              // timed in store mode (A zeroed), the mode of an assembly
+              // after MatrixCSR::set(0)
               for (int rep = 0; rep < 4; ++rep)
               {
                 if (rep == 1)
                   DFX_HIP(hipEventRecord(e0, stream()));
+                A->zeroed = true;
+                A->zero_pending = true;
                 if (A->dtype == DFX_F64)
                   assemble_rows<double>(f, *A, bc);
                 else
@@ -4563,6 +5101,14 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t*
           f.inc_row = nullptr;
           f.ngblk = 0;
           f.gather_pattern = 0;
+        }
+        if (best < 12 || best > 14)
+        {
+          dfree(f.inc8);
+          dfree(f.islot8);
+          f.inc8 = nullptr;
+          f.islot8 = nullptr;
+          f.fixed_pattern = 0;
         }
         if (best != 9)
         {
@@ -4642,10 +5188,21 @@ dfx_status dfx_assemble_vector(dfx_vec* b, const dfx_form* L)
         KernelInfo ki;
         kernel_info(L->d.kernel, ki);
         require(!ki.bilinear, "assemble_vector needs a linear form");
-        require(!is_element_kernel(L->d.kernel),
+        require(!is_element_kernel(L->d.kernel) || L->host_kernel,
                 "element-tensor forms are assembled with dfx_assemble_vector_elements");
         require(b->dtype == L->d.dtype, "vector and form dtypes differ");
         require(std::int64_t(L->max_dof + 1) * L->d.bs <= b->n, "dofmap exceeds the vector size");
+        if (L->host_kernel)
+        {
+          dfx_form& f = const_cast<dfx_form&>(*L);
+          if (b->dtype == DFX_F64)
+            elements_into_vector<double>(f, static_cast<double*>(b->data),
+                                         run_host_kernel<double>(f));
+          else
+            elements_into_vector<float>(f, static_cast<float*>(b->data),
+                                        run_host_kernel<float>(f));
+          return;
+        }
         if (is_facet_kernel(L->d.kernel))
         {
           if (b->dtype == DFX_F64)
@@ -4680,18 +5237,18 @@ dfx_status dfx_assemble_matrix_elements(dfx_csr* A, const dfx_form* a, const dfx
         require(a->d.kernel == DFX_K_ELEMENT_A, "the form does not take element matrices");
         require(A->dtype == a->d.dtype && E->dtype == a->d.dtype,
                 "matrix, form and element matrices dtypes differ");
-        require(A->bs == 1, "element matrices need a block size 1 matrix");
-        const std::int64_t k = a->d.dofs_per_cell;
-        require(std::int64_t(E->n) == std::int64_t(a->d.num_cells) * k * k,
-                "element matrices: num_cells x k x k values expected");
-        require(!bc || bc->n_total == A->nrows, "BC size does not match the matrix");
+        require(A->bs == a->d.bs, "matrix and form block sizes differ");
+        const std::int64_t nd = std::int64_t(a->d.dofs_per_cell) * a->d.bs;
+        require(std::int64_t(E->n) == std::int64_t(a->d.num_cells) * nd * nd,
+                "element matrices: num_cells x (bs k) x (bs k) values expected");
+        require(!bc || bc->n_total == A->nrows * A->bs, "BC size does not match the matrix");
         require(a->max_dof < A->nrows && a->max_dof < A->ncols, "dofmap exceeds the matrix size");
         dfx_form& f = const_cast<dfx_form&>(*a);
         // store mode (A zeroed) overwrites every entry
         if (A->dtype == DFX_F64)
-          assemble_elements<double>(f, *A, bc, static_cast<const double*>(E->data));
+          elements_into_matrix<double>(f, *A, bc, static_cast<const double*>(E->data));
         else
-          assemble_elements<float>(f, *A, bc, static_cast<const float*>(E->data));
+          elements_into_matrix<float>(f, *A, bc, static_cast<const float*>(E->data));
         A->zeroed = false;
         A->zero_pending = false;
       });
@@ -4706,22 +5263,16 @@ dfx_status dfx_assemble_vector_elements(dfx_vec* b, const dfx_form* L, const dfx
         require(L->d.kernel == DFX_K_ELEMENT_L, "the form does not take element vectors");
         require(b->dtype == L->d.dtype && e->dtype == L->d.dtype,
                 "vector, form and element vectors dtypes differ");
-        require(std::int64_t(e->n) == std::int64_t(L->d.num_cells) * L->d.dofs_per_cell,
-                "element vectors: num_cells x k values expected");
-        require(L->max_dof < b->n, "dofmap exceeds the vector size");
-        const std::int32_t nplan = L->max_dof + 1;
-        if (nplan <= 0)
-          return;
-        const int nb = (nplan + kBlock - 1) / kBlock;
+        require(std::int64_t(e->n)
+                    == std::int64_t(L->d.num_cells) * L->d.dofs_per_cell * L->d.bs,
+                "element vectors: num_cells x (bs k) values expected");
+        require(std::int64_t(L->max_dof + 1) * L->d.bs <= b->n, "dofmap exceeds the vector size");
         if (b->dtype == DFX_F64)
-          element_vector_k<double><<<nb, kBlock, 0, stream()>>>(
-              nplan, L->inc_off, L->inc, static_cast<const double*>(e->data),
-              static_cast<double*>(b->data));
+          elements_into_vector<double>(*L, static_cast<double*>(b->data),
+                                       static_cast<const double*>(e->data));
         else
-          element_vector_k<float><<<nb, kBlock, 0, stream()>>>(
-              nplan, L->inc_off, L->inc, static_cast<const float*>(e->data),
-              static_cast<float*>(b->data));
-        check_launch("element vector assembly");
+          elements_into_vector<float>(*L, static_cast<float*>(b->data),
+                                      static_cast<const float*>(e->data));
       });
 }
 
@@ -4735,39 +5286,25 @@ dfx_status dfx_apply_lifting_elements(dfx_vec* b, const dfx_form* a, const dfx_b
         require(a->d.kernel == DFX_K_ELEMENT_A, "the form does not take element matrices");
         require(b->dtype == a->d.dtype && E->dtype == a->d.dtype,
                 "vector, form and element matrices dtypes differ");
-        const std::int64_t k = a->d.dofs_per_cell;
-        require(std::int64_t(E->n) == std::int64_t(a->d.num_cells) * k * k,
-                "element matrices: num_cells x k x k values expected");
+        const std::int64_t nd = std::int64_t(a->d.dofs_per_cell) * a->d.bs;
+        require(std::int64_t(E->n) == std::int64_t(a->d.num_cells) * nd * nd,
+                "element matrices: num_cells x (bs k) x (bs k) values expected");
         if (!bc || bc->ndofs == 0)
           return;
         require(bc->n_total == b->n, "BC size does not match the vector");
         require(!x0 || (x0->n == b->n && x0->dtype == b->dtype), "x0 does not match b");
-        require(a->max_dof < b->n, "dofmap exceeds the vector size");
+        require(std::int64_t(a->max_dof + 1) * a->d.bs <= b->n, "dofmap exceeds the vector size");
         // g = 0 and x0 = 0: every term alpha E_ij (g_j - x0_j) is zero
         if (!bc->g && !x0)
           return;
-        const std::int32_t nc = a->d.num_cells;
-        const std::int32_t nplan = a->max_dof + 1;
-        if (nc > 0)
-          element_mask_k<<<blocks_for(nc), kBlock, 0, stream()>>>(nc, a->d.dofs_per_cell,
-                                                                  a->dofmap, bc->marker,
-                                                                  a->cell_mask);
-        if (nplan > 0)
-        {
-          if (b->dtype == DFX_F64)
-            element_lift_k<double><<<blocks_for(nplan), kBlock, 0, stream()>>>(
-                nplan, a->d.dofs_per_cell, a->inc_off, a->inc, a->dofmap,
-                static_cast<const double*>(E->data), a->cell_mask, bc->g,
-                x0 ? static_cast<const double*>(x0->data) : nullptr, alpha,
-                static_cast<double*>(b->data));
-          else
-            element_lift_k<float><<<blocks_for(nplan), kBlock, 0, stream()>>>(
-                nplan, a->d.dofs_per_cell, a->inc_off, a->inc, a->dofmap,
-                static_cast<const float*>(E->data), a->cell_mask, bc->g,
-                x0 ? static_cast<const float*>(x0->data) : nullptr, alpha,
-                static_cast<float*>(b->data));
-        }
-        check_launch("element lifting");
+        if (b->dtype == DFX_F64)
+          elements_lift<double>(*a, static_cast<double*>(b->data), *bc,
+                                static_cast<const double*>(E->data),
+                                x0 ? static_cast<const double*>(x0->data) : nullptr, alpha);
+        else
+          elements_lift<float>(*a, static_cast<float*>(b->data), *bc,
+                               static_cast<const float*>(E->data),
+                               x0 ? static_cast<const float*>(x0->data) : nullptr, alpha);
       });
 }
 
@@ -4781,7 +5318,7 @@ dfx_status dfx_apply_lifting(dfx_vec* b, const dfx_form* a, const dfx_bc* bc, co
         KernelInfo ki;
         kernel_info(a->d.kernel, ki);
         require(ki.bilinear, "apply_lifting needs a bilinear form");
-        require(!is_element_kernel(a->d.kernel),
+        require(!is_element_kernel(a->d.kernel) || a->host_kernel,
                 "apply_lifting needs a form with a kernel (element-tensor forms have none)");
         require(b->dtype == a->d.dtype, "vector and form dtypes differ");
         if (!bc || bc->ndofs == 0)
@@ -4793,9 +5330,29 @@ dfx_status dfx_apply_lifting(dfx_vec* b, const dfx_form* a, const dfx_bc* bc, co
         if (!bc->g && !x0)
           return;
         dfx_form& f = const_cast<dfx_form&>(*a);
-        require(f.d.kernel != DFX_K_P2TET_ELAST_A,
-                "apply_lifting with nonzero Dirichlet values is not implemented for the "
-                "P2 elasticity form (g = 0 and x0 = 0 are)");
+        if (f.host_kernel)
+        {
+          // the kernel's element matrices (as lift_bc recomputes them)
+          if (b->dtype == DFX_F64)
+            elements_lift<double>(f, static_cast<double*>(b->data), *bc,
+                                  run_host_kernel<double>(f),
+                                  x0 ? static_cast<const double*>(x0->data) : nullptr, alpha);
+          else
+            elements_lift<float>(f, static_cast<float*>(b->data), *bc,
+                                 run_host_kernel<float>(f),
+                                 x0 ? static_cast<const float*>(x0->data) : nullptr, alpha);
+          return;
+        }
+        if (f.d.kernel == DFX_K_P2TET_ELAST_A)
+        {
+          if (b->dtype == DFX_F64)
+            elast_lift_rows<double>(f, *bc, x0 ? static_cast<const double*>(x0->data) : nullptr,
+                                    alpha, static_cast<double*>(b->data));
+          else
+            elast_lift_rows<float>(f, *bc, x0 ? static_cast<const float*>(x0->data) : nullptr,
+                                   alpha, static_cast<float*>(b->data));
+          return;
+        }
         if (is_interior_kernel(f.d.kernel))
         {
           if (b->dtype == DFX_F64)

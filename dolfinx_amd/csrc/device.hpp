@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 #include <memory>
 #include <string>
 
@@ -266,6 +267,24 @@ struct dfx_form
   std::uint8_t* slotmap = nullptr;
   std::uint64_t struct_pattern = 0;
   int struct_ok = -1;
+  // Fixed-width rows (hexahedra whose vertices lie in at most 8 cells):
+  // per row its incidences c * 8 + i in cell order, padded with -1 to 8
+  // (inc8, rows x 8), and their slot offsets (islot8, rows x 8 x 8 bytes),
+  // so a row's plan is read without a first load of its offsets.
+  std::int32_t* inc8 = nullptr;
+  std::uint8_t* islot8 = nullptr;
+  std::uint64_t fixed_pattern = 0;
+  // Forms whose cell tensors come from a host element kernel with the
+  // reference's kernel ABI (kernelptr_t): the kernel and its custom data,
+  // host copies of what it reads per cell (geometry, packed coefficients,
+  // constants) and the device array of the tensors of its last run.
+  void* host_kernel = nullptr;
+  void* host_kernel_data = nullptr;
+  std::vector<double> h_x;
+  std::vector<std::int32_t> h_x_dofmap;
+  std::vector<double> h_coeffs;
+  std::vector<double> h_constants;
+  void* elem_tensors = nullptr;
   // Device packing of coefficients: for each packed position the index of
   // its value in the caller's array (per input cell or input vertex), the
   // mesh and layout it was built for, and a staging buffer for the values.

@@ -360,6 +360,65 @@ def element_form(V: FunctionSpace, rank: int, dtype=np.float64) -> Form:
     return Form("element_matrix" if rank == 2 else "element_vector", V, dtype=dtype)
 
 
+# The reference's element kernel signature (kernelptr_t, fem/kernel.h:18-20)
+# for FP64 and FP32 forms: A, w, c, coordinate_dofs, entity_local_index,
+# quadrature_permutation, custom_data.
+KERNEL_F64 = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                         C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int),
+                         C.POINTER(C.c_uint8), C.c_void_p)
+KERNEL_F32 = C.CFUNCTYPE(None, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                         C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int),
+                         C.POINTER(C.c_uint8), C.c_void_p)
+
+
+class KernelForm(Form):
+    """fem::Form with a cell integral given as integral_data{kernel, cells}
+    (fem/Form.h:52-87; the shape of cpp/demo/custom_kernel/main.cpp:77-81):
+    `kernel` is a function with the reference's element kernel ABI -- the
+    address of a compiled tabulate_tensor (an int) or a KERNEL_F64 /
+    KERNEL_F32 ctypes function -- called on the host for every cell in cell
+    order; its tensors are inserted on the device. rank 2: (bs k)^2
+    row-major element matrices; rank 1: bs k element vectors."""
+
+    def __init__(self, V: FunctionSpace, rank: int, kernel, coeffs=None, constants=None,
+                 dtype=np.float64, custom_data=None):
+        self.name, self.V, self.rank, self.dtype = "kernel", V, rank, np.dtype(dtype)
+        self.integral = "cell"
+        self.facets = None
+        mesh = V.mesh
+        nc = mesh.topology.cells.shape[0]
+        self.coeffs = (None if coeffs is None
+                       else np.ascontiguousarray(coeffs, np.float64).reshape(nc, -1))
+        self.constants = (np.zeros(0) if constants is None
+                          else np.ascontiguousarray(np.atleast_1d(constants), np.float64))
+        self._kernel = kernel  # keeps a ctypes callback alive
+        addr = kernel if isinstance(kernel, int) else C.cast(kernel, C.c_void_p).value
+        x = np.ascontiguousarray(mesh.geometry.x)
+        xd = np.ascontiguousarray(mesh.geometry.dofmap)
+        dm = np.ascontiguousarray(V.dofmap.list)
+        self._dm = dm
+        d = _lib.FormDesc()
+        d.kernel = _lib.K_ELEMENT_A if rank == 2 else _lib.K_ELEMENT_L
+        d.dtype = _lib.code_of(dtype)
+        d.num_cells, d.num_nodes, d.nodes_per_cell = nc, x.shape[0], xd.shape[1]
+        d.x, d.x_dofmap = _lib.ptr(x), _lib.ptr(xd)
+        d.dofs_per_cell, d.bs, d.dofmap = dm.shape[1], V.dofmap.bs, _lib.ptr(dm)
+        d.cstride = 0 if self.coeffs is None else self.coeffs.shape[1]
+        d.coeffs = _lib.ptr(self.coeffs)
+        d.num_constants = len(self.constants)
+        d.constants = _lib.ptr(self.constants) if len(self.constants) else None
+        self._h = C.c_void_p()
+        _lib.call("dfx_form_create_kernel", C.byref(d), C.c_void_p(addr),
+                  None if custom_data is None else C.c_void_p(custom_data), C.byref(self._h))
+
+
+def kernel_form(V: FunctionSpace, rank: int, kernel, coeffs=None, constants=None,
+                dtype=np.float64, custom_data=None) -> KernelForm:
+    """A form whose cell tensors come from an element kernel with the
+    reference's ABI (see KernelForm)."""
+    return KernelForm(V, rank, kernel, coeffs, constants, dtype, custom_data)
+
+
 def _device_tensors(x, n, dtype):
     if isinstance(x, Vector):
         if x.size != n:

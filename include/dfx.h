@@ -259,6 +259,34 @@ dfx_status dfx_form_create_facets(const dfx_form_desc* desc, int32_t npairs,
  * dfx_sparsity_create_facets. */
 dfx_status dfx_form_create_interior_facets(const dfx_form_desc* desc, int32_t nfacets,
                                            const int32_t* quads, dfx_form** out);
+/* The element kernel ABI of the reference (kernelptr_t, fem/kernel.h:18-20;
+ * the tabulate_tensor_float64 / _float32 fields of a ufcx_integral,
+ * extracted by fem/kernel.h:34-60): A (zeroed by the caller), w = the
+ * cell's packed coefficients, c = the constants, coordinate_dofs = the
+ * cell's geometry nodes (3 values each), entity_local_index,
+ * quadrature_permutation, custom_data. */
+typedef void (*dfx_kernel_f64)(double* A, const double* w, const double* c,
+                               const double* coordinate_dofs, const int* entity_local_index,
+                               const uint8_t* quadrature_permutation, void* custom_data);
+typedef void (*dfx_kernel_f32)(float* A, const float* w, const float* c,
+                               const float* coordinate_dofs, const int* entity_local_index,
+                               const uint8_t* quadrature_permutation, void* custom_data);
+/* fem::Form with a cell integral given as integral_data{kernel, cells}
+ * (fem/Form.h:52-87; cpp/demo/custom_kernel/main.cpp:77-81): kernel is a
+ * dfx_kernel_f64 (dtype DFX_F64) or dfx_kernel_f32 (DFX_F32, geometry
+ * given to it in FP32, as U = scalar_value_t<T>, fem/kernel.h:37-38).
+ * desc->kernel is DFX_K_ELEMENT_A (bilinear: (bs k)^2 row-major tensors)
+ * or DFX_K_ELEMENT_L (linear: bs k); block size 1 or 3, any k <= 32.
+ * dfx_assemble_matrix / dfx_assemble_vector / dfx_apply_lifting on this
+ * form call the kernel on the host for every cell in cell order, exactly
+ * as assemble_cells_matrix / assemble_cells do
+ * (fem/assemble_matrix_impl.h:144-152, fem/assemble_vector_impl.h:99-107,
+ * entity_local_index and quadrature_permutation NULL, custom_data as
+ * given), copy the tensors to the device once and insert them there as
+ * dfx_assemble_matrix_elements / _vector_elements / dfx_apply_lifting_elements
+ * do. The kernel must be callable from the calling thread. */
+dfx_status dfx_form_create_kernel(const dfx_form_desc* desc, void* kernel, void* custom_data,
+                                  dfx_form** out);
 dfx_status dfx_form_destroy(dfx_form* f);
 /* Replace packed coefficients / constants (same sizes as at creation). */
 dfx_status dfx_form_set_coefficients(dfx_form* f, const double* coeffs,
@@ -310,16 +338,18 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc,
 dfx_status dfx_form_set_variant(dfx_form* a, int32_t variant);
 /* fem::assemble_matrix / fem::assemble_vector with a custom kernel
  * (cpp/demo/custom_kernel/main.cpp:73-118): the form (kernel
- * DFX_K_ELEMENT_A / DFX_K_ELEMENT_L) holds only a dofmap (bs = 1; no
- * geometry or coefficients needed), and the kernel's outputs are given on
- * the device: E = num_cells row-major k x k element matrices (a vector of
- * num_cells k^2 values of A's dtype), e = num_cells x k element vectors.
+ * DFX_K_ELEMENT_A / DFX_K_ELEMENT_L) holds only a dofmap (block size 1 or
+ * 3; no geometry or coefficients needed), and the kernel's outputs are
+ * given on the device: E = num_cells row-major (bs k) x (bs k) element
+ * matrices (dofs interleaved per node, as the reference's blocked
+ * kernels order them), e = num_cells x (bs k) element vectors.
  * A[dofs_c, dofs_c] += E_c and b[dofs_c] += e_c as la::MatrixCSR::add
- * (la/matrix_csr_impl.h:67-109) and assemble_cells
+ * (la/matrix_csr_impl.h:67-109; mat_add_values<3, 3> for block size 3,
+ * la/MatrixCSR.h:157-174) and assemble_cells
  * (fem/assemble_vector_impl.h:72-116) insert them, each entry summed in
  * cell order; rows and columns of bc's dofs are skipped
  * (fem/assemble_matrix_impl.h:156-180); "Entry not in sparsity" if a pair
- * is missing from A's pattern. Matrices: k in {4, 8, 10, 20}. */
+ * is missing from A's pattern. */
 dfx_status dfx_assemble_matrix_elements(dfx_csr* A, const dfx_form* a, const dfx_bc* bc,
                                         const dfx_vec* E);
 dfx_status dfx_assemble_vector_elements(dfx_vec* b, const dfx_form* L, const dfx_vec* e);

@@ -43,6 +43,25 @@ def lib():
     return _lib
 
 
+def use_native() -> str:
+    """Rebuild the oracle with -O3 -march=native on this machine (for the
+    CPU baseline) and use that copy from now on; returns the -march used
+    ("native", or "x86-64-v3" when the native build is not possible). Must
+    run before any other call of this module."""
+    global _lib
+    if _lib is not None:
+        raise RuntimeError("oracle.use_native() must run before the oracle is first used")
+    try:
+        subprocess.run(["make", "-s", "-B", "-C", _HERE, "native"], check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        _lib = C.CDLL(os.path.join(_HERE, "_native", "liboracle.so"))
+        _declare(_lib)
+        return "native"
+    except (OSError, subprocess.CalledProcessError):
+        lib()
+        return "x86-64-v3"
+
+
 P = C.c_void_p
 I32, I64, D = C.c_int32, C.c_int64, C.c_double
 

@@ -152,3 +152,41 @@ def test_p3_lifting_nonzero_g():
     O.lift_bc(br, O.K_P3TET_A, m.x, m.cells, dm, 1, kap, np.array([1.0]), g, mark)
     O.bc_set(br, dofs, g)
     assert rel_err(b.array, br) <= 1e-12
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 5e-6)])
+def test_elasticity_lifting_nonzero_g_matches_oracle(dtype, tol):
+    """apply_lifting of the P2 vector form (block size 3) with nonzero
+    Dirichlet values on the clamped face and a nonzero x0, alpha != 1,
+    then bc.set, against the oracle's lift_bc (assemble_vector_impl.h
+    lift_bc over the cells with a Dirichlet column dof)."""
+    n = (3, 3, 2)
+    p = problems.elasticity_problem(n, seed=12, dtype=dtype)
+    N = p.V.num_dofs
+    dofs = np.asarray(p.bc.dofs)
+    g = np.zeros(N)
+    g[dofs] = 0.1 + np.sin(np.arange(len(dofs)))
+    x0 = np.random.default_rng(13).standard_normal(N)
+    alpha = -0.7
+    bcg = fem.dirichletbc(p.V, dofs, g)
+    b = fem.assemble_vector(p.L)
+    x0v = la.Vector.from_array(x0.astype(dtype))
+    fem.apply_lifting(b, [p.a], [[bcg]], x0=[x0v], alpha=alpha)
+    fem.set_bc(b, [bcg], x0v, alpha)
+    m = O.Mesh(n, O.TET)
+    dm, nn = m.dofmap(2)
+    lam, mu = problems.lame_from_E(p.E)
+    co = np.stack([lam, mu], 1)[m.original_cell_index]
+    mark = np.zeros(3 * nn, np.int8)
+    mark[dofs] = 1
+    br = O.assemble_vector(O.K_P2TET_ELAST_L, m.x, m.cells, dm, 3, None, np.array([0, 0, -1.0]),
+                           3 * nn)
+    x0r = x0.astype(dtype).astype(np.float64)
+    O.lift_bc(br, O.K_P2TET_ELAST_A, m.x, m.cells, dm, 3, co, None, g, mark, x0r, alpha)
+    O.bc_set(br, dofs, g, x0r, alpha)
+    assert rel_err(b.array, br) <= tol
+    # the lifting changed the rows next to the face (not only the BC rows)
+    b0 = fem.assemble_vector(p.L)
+    fem.set_bc(b0, [bcg], x0v, alpha)
+    touched = np.flatnonzero(np.abs(b.array - b0.array) > 0)
+    assert np.setdiff1d(touched, dofs).size > 0
