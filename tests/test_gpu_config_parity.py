@@ -111,10 +111,10 @@ def test_q1_hex_128_spmv_matches_oracle(q1_128):
     v = np.random.default_rng(3).standard_normal(p.V.num_dofs)
     yr = O.spmv(ref["rp"], ref["cols"], ref["A"], 1, v)
     x = la.Vector.from_array(v)
-    y = la.Vector(p.V.num_dofs)
     for lanes in (1, 2, 8):
         A.set_lanes(lanes)
-        A.mult(x, y)
+        y = la.Vector(p.V.num_dofs)
+        A.mult(x, y)  # y += A x
         assert rel_err(y.array, yr) <= 1e-14, lanes
 
 
@@ -211,10 +211,10 @@ def test_p2_elasticity_48_rigid_modes_and_block_spmv():
     v = np.random.default_rng(4).standard_normal(3 * nn)
     yr = O.spmv(rp, ci, A.data, 3, v)
     x = la.Vector.from_array(v)
-    y = la.Vector(3 * nn)
     for lanes in (1, 2):
         A.set_lanes(lanes)
-        A.mult(x, y)
+        y = la.Vector(3 * nn)
+        A.mult(x, y)  # y += A x
         assert rel_err(y.array, yr) <= 1e-14, lanes
 
 
