@@ -121,6 +121,7 @@ SIGNATURES = {
     "dfx_apply_lifting": (C.c_int, [P, P, P, P, D]),
     "dfx_bc_set": (C.c_int, [P, P, P, D]),
     "dfx_cg_solve": (C.c_int, [P, P, P, C.POINTER(CGOpts), C.POINTER(CGResult)]),
+    "dfx_cg_solve_noalloc": (C.c_int, [P, P, P, C.POINTER(CGOpts), C.POINTER(CGResult)]),
     "dfx_mixed_solve": (C.c_int, [P, P, P, P, C.POINTER(MixedOpts), C.POINTER(MixedResult)]),
     "dfx_batch_create": (C.c_int, [I32, I64, PP]),
     "dfx_batch_destroy": (C.c_int, [P]),

@@ -26,6 +26,8 @@ std::atomic<std::uint64_t> g_pattern_counter{1};
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
+void init_noalloc_scratch(int device);
+
 Context& ctx() { return g_ctx; }
 
 void ensure_init()
@@ -45,6 +47,7 @@ void ensure_init()
   DFX_HIP(hipEventCreateWithFlags(&g_ctx.join, hipEventDisableTiming));
   for (auto& e : g_ctx.events)
     DFX_HIP(hipEventCreate(&e));
+  init_noalloc_scratch(0);
   g_ctx.ready = true;
 }
 
@@ -1085,6 +1088,7 @@ struct CGWork
 {
   std::int64_t n = 0;
   T *r = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr;
+  bool own = true; // false: the vectors live in caller-provided scratch
   void alloc(std::int64_t nn, bool jacobi)
   {
     n = nn;
@@ -1096,6 +1100,8 @@ struct CGWork
   }
   ~CGWork()
   {
+    if (!own)
+      return;
     dfree(r);
     dfree(p);
     dfree(q);
@@ -1277,12 +1283,14 @@ int launch_cg_spmv(const dfx_csr& A, const T* p, T* q, CGState* st, bool sell)
 template <typename T>
 int cg_run(const dfx_csr& A, const T* b, T* x, int max_it, double rtol, bool jacobi,
            int check_every, CGWork<T>& w, CGState* st, bool* converged, double* rr0,
-           double* rr, bool x_is_zero = false)
+           double* rr, bool x_is_zero = false, bool allow_sell = true)
 {
   hipStream_t s = stream();
   const std::int64_t n = std::int64_t(A.nrows) * A.bs;
   const int nbv = partial_blocks(n);
-  const bool sell = refresh_sell<T>(A);
+  // the sliced-ELL copy belongs to the matrix handle and is allocated on
+  // its first use; allow_sell = false keeps the CSR kernels
+  const bool sell = allow_sell && refresh_sell<T>(A);
   if (jacobi)
   {
     const int nb = static_cast<int>((n + kBlock - 1) / kBlock);
@@ -1331,6 +1339,20 @@ int cg_run(const dfx_csr& A, const T* b, T* x, int max_it, double rtol, bool jac
   return fin.iter;
 }
 
+/// a . b summed in FP64 through the device buffers partials
+/// (kMaxPartials values) and out (one value); returns it on the host.
+template <typename T>
+double dot_with(double* partials, double* out, const T* a, const T* b, std::int64_t n)
+{
+  const int nb = partial_blocks(n);
+  dot_k<T><<<nb, kBlock, 0, stream()>>>(a, b, n, partials);
+  finish_sums_k<<<1, kBlock, 0, stream()>>>(partials, nb, 1, out);
+  check_launch("dot");
+  double h = 0;
+  d2h(&h, out, sizeof(double));
+  return h;
+}
+
 /// Scratch for host-visible reductions.
 struct Reducer
 {
@@ -1344,13 +1366,7 @@ struct Reducer
   template <typename T>
   double dot(const T* a, const T* b, std::int64_t n)
   {
-    const int nb = partial_blocks(n);
-    dot_k<T><<<nb, kBlock, 0, stream()>>>(a, b, n, partials);
-    finish_sums_k<<<1, kBlock, 0, stream()>>>(partials, nb, 1, out);
-    check_launch("dot");
-    double h = 0;
-    d2h(&h, out, sizeof(double));
-    return h;
+    return dot_with<T>(partials, out, a, b, n);
   }
 };
 
@@ -1360,16 +1376,19 @@ Reducer& reducer()
   return *r;
 }
 
-/// FP64 true residual norm ||b - A x||_2, with y scratch of T.
+/// FP64 true residual norm ||b - A x||_2, with y scratch of T; the
+/// reduction goes through partials/out when given (else the library's
+/// reduction buffers).
 template <typename T>
-double true_residual(const dfx_csr& A, const T* b, const T* x, T* y)
+double true_residual(const dfx_csr& A, const T* b, const T* x, T* y,
+                     double* partials = nullptr, double* out = nullptr)
 {
   hipStream_t s = stream();
   const std::int64_t n = std::int64_t(A.nrows) * A.bs;
   launch_spmv<T>(A, x, y, false);
   axpy_k<T><<<reduce_blocks(n), kBlock, 0, s>>>(y, -1.0, y, b, n);
   check_launch("residual");
-  return std::sqrt(reducer().dot(y, y, n));
+  return std::sqrt(partials ? dot_with<T>(partials, out, y, y, n) : reducer().dot(y, y, n));
 }
 
 } // namespace dfx
@@ -1402,6 +1421,99 @@ void cg_solve_t(const dfx_csr& A, const dfx_vec& b, dfx_vec& x,
   const double bn = std::sqrt(red.dot(static_cast<const T*>(b.data), static_cast<const T*>(b.data), n));
   const double tr = true_residual<T>(A, static_cast<const T*>(b.data), static_cast<const T*>(x.data),
                                      w.q);
+  res.true_rel_residual = bn > 0 ? tr / bn : tr;
+}
+
+// ---------------------------------------------------------------------
+// CG without allocation. The work vectors (r, p, q, diag^-1), the solver
+// state and the reduction partials live in one device array, `scratch`,
+// of fixed size: kNoAllocMaxN scalar rows at FP64 (32 MiB of vectors plus
+// the state). Each device holds its own copy of the array, whose address
+// is looked up once per device at library initialisation
+// (hipGetSymbolAddress) and used wherever host code hands the array to a
+// kernel or a copy. Every call on a device shares that device's array:
+// calls that overlap on one device race.
+// ---------------------------------------------------------------------
+
+constexpr std::int64_t kNoAllocMaxN = std::int64_t(1) << 20;
+constexpr std::size_t kStateBytes = (sizeof(CGState) + 255) / 256 * 256;
+constexpr std::size_t kScratchBytes = kStateBytes + 4 * kNoAllocMaxN * sizeof(double);
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace
+{
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch_ptr; // device -> address of its scratch
+} // namespace
+
+/// Looks up the device address of `scratch` on `device` (made current by
+/// the caller) once.
+void init_noalloc_scratch(int device)
+{
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr.count(device))
+    return;
+  void* p = nullptr;
+  DFX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p)
+    throw Error(DFX_ERR_HIP, "no device address for the CG scratch array");
+  g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+}
+
+unsigned char* noalloc_scratch()
+{
+  ensure_init();
+  int dev = 0;
+  DFX_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end())
+    throw Error(DFX_ERR_HIP, "the CG scratch array was not set up on device "
+                                 + std::to_string(dev));
+  return it->second;
+}
+
+/// cg_solve_t with every device buffer in the scratch array: the CSR SpMV
+/// kernels (the sliced-ELL copy would be allocated in the matrix handle)
+/// and the reductions through the state's partials.
+template <typename T>
+void cg_solve_noalloc_t(const dfx_csr& A, const dfx_vec& b, dfx_vec& x, const dfx_cg_opts& o,
+                        dfx_cg_result& res)
+{
+  const std::int64_t n = std::int64_t(A.nrows) * A.bs;
+  unsigned char* base = noalloc_scratch();
+  CGState* st = reinterpret_cast<CGState*>(base);
+  T* v = reinterpret_cast<T*>(base + kStateBytes);
+  CGWork<T> w;
+  w.own = false;
+  w.n = n;
+  w.r = v;
+  w.p = v + n;
+  w.q = v + 2 * n;
+  w.dinv = o.jacobi ? v + 3 * n : nullptr;
+  // the state starts at zero (the previous call's contents remain)
+  DFX_HIP(hipMemsetAsync(st, 0, sizeof(CGState), stream()));
+  DFX_HIP(hipStreamSynchronize(stream()));
+  const auto t0 = std::chrono::steady_clock::now();
+  bool conv = false;
+  double rr0 = 0, rr = 0;
+  const int it = cg_run<T>(A, static_cast<const T*>(b.data), static_cast<T*>(x.data), o.max_it,
+                           o.rtol, o.jacobi != 0, o.check_every, w, st, &conv, &rr0, &rr,
+                           false, false);
+  DFX_HIP(hipStreamSynchronize(stream()));
+  const auto t1 = std::chrono::steady_clock::now();
+  res.iterations = it;
+  res.converged = conv ? 1 : 0;
+  res.rnorm0 = std::sqrt(rr0);
+  res.rnorm = std::sqrt(rr);
+  res.seconds = std::chrono::duration<double>(t1 - t0).count();
+  double* partials = st->partials;
+  double* out = st->partials + 2 * kMaxPartials;
+  const double bn = std::sqrt(dot_with<T>(partials, out, static_cast<const T*>(b.data),
+                                          static_cast<const T*>(b.data), n));
+  const double tr = true_residual<T>(A, static_cast<const T*>(b.data),
+                                     static_cast<const T*>(x.data), w.q, partials, out);
   res.true_rel_residual = bn > 0 ? tr / bn : tr;
 }
 
@@ -1927,6 +2039,35 @@ dfx_status dfx_cg_solve(const dfx_csr* A, const dfx_vec* b, dfx_vec* x,
           cg_solve_t<double>(*A, *b, *x, *opts, *res);
         else
           cg_solve_t<float>(*A, *b, *x, *opts, *res);
+        if (!res->converged)
+          throw Error(DFX_ERR_NOCONV, "CG exceeded max iterations ("
+                                          + std::to_string(opts->max_it)
+                                          + "). Relative residual="
+                                          + std::to_string(res->rnorm / std::max(res->rnorm0, 1e-300)));
+      });
+}
+
+dfx_status dfx_cg_solve_noalloc(const dfx_csr* A, const dfx_vec* b, dfx_vec* x,
+                                const dfx_cg_opts* opts, dfx_cg_result* res)
+{
+  return guard(
+      [&]
+      {
+        require(A && b && x && opts && res, "null argument");
+        require(b->dtype == A->dtype && x->dtype == A->dtype, "dtype mismatch");
+        require(A->nrows == A->ncols, "matrix not square");
+        require(b->n == A->nrows * A->bs && x->n == b->n, "size mismatch");
+        require(opts->max_it >= 0 && opts->rtol >= 0, "bad options");
+        require(std::int64_t(A->nrows) * A->bs <= kNoAllocMaxN,
+                "dfx_cg_solve_noalloc: " + std::to_string(std::int64_t(A->nrows) * A->bs)
+                    + " rows exceed the scratch maximum of " + std::to_string(kNoAllocMaxN)
+                    + " (use dfx_cg_solve)");
+        materialize_zero(*A);
+        *res = dfx_cg_result{};
+        if (A->dtype == DFX_F64)
+          cg_solve_noalloc_t<double>(*A, *b, *x, *opts, *res);
+        else
+          cg_solve_noalloc_t<float>(*A, *b, *x, *opts, *res);
         if (!res->converged)
           throw Error(DFX_ERR_NOCONV, "CG exceeded max iterations ("
                                           + std::to_string(opts->max_it)

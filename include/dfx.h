@@ -409,6 +409,18 @@ typedef struct
 dfx_status dfx_cg_solve(const dfx_csr* A, const dfx_vec* b, dfx_vec* x,
                         const dfx_cg_opts* opts, dfx_cg_result* res);
 
+/* dfx_cg_solve without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The work vectors, solver state and reduction partials live
+ * in a fixed device array of the library (one per GPU, held until the
+ * process ends): at most 1,048,576 scalar rows (nrows x bs); larger
+ * systems return DFX_ERR_ARG. The SpMV is the matrix's CSR kernel (no
+ * sliced-ELL copy, which would be allocated). Calls that overlap on one
+ * GPU share the array and race. */
+dfx_status dfx_cg_solve_noalloc(const dfx_csr* A, const dfx_vec* b, dfx_vec* x,
+                                const dfx_cg_opts* opts, dfx_cg_result* res);
+
 typedef struct
 {
   int32_t max_outer;   /* refinement steps */
