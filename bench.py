@@ -136,6 +136,50 @@ def cpu_baseline(n, kappa_input, solutions, march):
     return out
 
 
+def sell_bytes(indptr, indices, s):
+    """Bytes one SpMV through the sliced-ELL copy moves: every slot of its
+    64-row slices padded to the slice's longest row (value + a 2-byte column
+    offset, 4 bytes where the slice's columns span 2^16 or more), the row
+    pointers, x, and y read and written."""
+    n = len(indptr) - 1
+    ns = (n + 63) // 64
+    lens = np.diff(indptr)
+    padded = np.zeros(ns * 64, np.int64)
+    padded[:n] = lens
+    width = padded.reshape(ns, 64).max(axis=1)
+    first = indptr[np.minimum(np.arange(ns) * 64, n)]
+    last = indptr[np.minimum(np.arange(ns) * 64 + 64, n)]
+    cb = np.full(ns, 2)
+    for sl in range(ns):
+        if last[sl] > first[sl]:
+            c = indices[first[sl]:last[sl]]
+            if int(c.max()) - int(c.min()) >= 65536:
+                cb[sl] = 4
+    entries = int(np.sum(width * 64))
+    return int(np.sum(width * 64 * (s + cb))) + (n + 1) * 4 + 3 * n * s, entries
+
+
+def sell_spmv(A, x, y, N, nnz, s, reps):
+    """Event-timed SpMV through the sliced-ELL copy (lanes -1), with its
+    own bytes and the CSR-equivalent bytes nnz (s + 4) + (rows + 1) 4 +
+    3 N s over the same time."""
+    A.set_lanes(-1)
+    for _ in range(3):
+        A.mult(x, y)
+    record(10)
+    for _ in range(reps):
+        A.mult(x, y)
+    record(11)
+    ms = event_ms(10, 11) / reps
+    own, entries = sell_bytes(A.indptr, A.indices, s)
+    csr = nnz * (s + 4) + (N + 1) * 4 + 3 * N * s
+    return {"ms": round(ms, 4), "padded_entries": entries,
+            "own_bytes": own, "own_gbs": round(own / ms / 1e6, 1),
+            "own_frac": round(own / ms / 1e6 / HBM_PEAK_GBS, 4),
+            "csr_equivalent_gbs": round(csr / ms / 1e6, 1),
+            "csr_equivalent_frac": round(csr / ms / 1e6 / HBM_PEAK_GBS, 4)}
+
+
 def timed_assembly(assemble, steps):
     """Seconds per call of `assemble` (device-synchronised, after one warm call)."""
     assemble()
@@ -482,6 +526,13 @@ def main():
     spmv32_ms = event_ms(4, 5) / nsp
     spmv32_gbs = (nnz * 8 + (N + 1) * 4 + 3 * N * 4) / (spmv32_ms * 1e-3) / 1e9
 
+    # ---- SpMV through the solvers' sliced-ELL copy (the kernel the CG
+    # iterations run), FP64 and FP32 ----
+    sell = sell_spmv(A64, x, y, N, nnz, 8, nsp)
+    sell32 = sell_spmv(A32, x32, y32, N, nnz, 4, nsp)
+    A64.set_lanes(lanes64)
+    A32.set_lanes(lanes32)
+
     # ---- practical streaming bandwidth: device copy of 2^27 doubles ----
     ncopy = 1 << 27
     src = la.Vector(ncopy)
@@ -615,6 +666,8 @@ def main():
             "f64_frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
             "f32_ms": round(spmv32_ms, 4), "f32_gbs": round(spmv32_gbs, 1),
             "f32_frac": round(spmv32_gbs / HBM_PEAK_GBS, 4),
+            "sliced_ell_f64": sell,
+            "sliced_ell_f32": sell32,
         },
         **out_solve,
         "serve": serve_out,

@@ -36,7 +36,8 @@ class FormDesc(C.Structure):
 
 
 class CGOpts(C.Structure):
-    _fields_ = [("max_it", I32), ("rtol", D), ("jacobi", I32), ("check_every", I32)]
+    _fields_ = [("max_it", I32), ("rtol", D), ("jacobi", I32), ("check_every", I32),
+                ("graph", I32)]
 
 
 class CGResult(C.Structure):
@@ -46,7 +47,7 @@ class CGResult(C.Structure):
 
 class MixedOpts(C.Structure):
     _fields_ = [("max_outer", I32), ("max_inner", I32), ("rtol", D), ("inner_rtol", D),
-                ("check_every", I32), ("restart", I32)]
+                ("check_every", I32), ("restart", I32), ("graph", I32)]
 
 
 class MixedResult(C.Structure):

@@ -1155,6 +1155,128 @@ __global__ void __launch_bounds__(kBlock)
     b[t] = static_cast<T>(double(b[t]) - alpha * s);
 }
 
+// ---------------------------------------------------------------------
+// Entry-parallel assembly. Every CSR entry (r, col) owns the list of the
+// element contributions that add into it -- (cell c, local row i, local
+// column j) with dof(c, i) = r and dof(c, j) = col, coded (c k + i) k + j
+// and kept in cell order -- so each entry is summed by one lane in the
+// order of the reference's cell loop, and the entries of a row are written
+// by consecutive lanes. A contribution is formed from the cell factors and
+// the reference tensors (7 products), or read from the element matrices of
+// all cells (E, whose flat index is the code itself).
+// ---------------------------------------------------------------------
+
+/// Contributions per entry (the plan's counting pass).
+template <typename O>
+__global__ void __launch_bounds__(kBlock)
+    contrib_count_k(std::int64_t ninc, int k, const std::int32_t* __restrict__ inc,
+                    const O* __restrict__ islots, const std::int32_t* __restrict__ dm,
+                    const std::int32_t* __restrict__ rp, std::int32_t* __restrict__ cnt)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t >= ninc * k)
+    return;
+  const std::int32_t cik = inc[t / k];
+  const std::int64_t e = std::int64_t(rp[dm[cik]]) + islots[t];
+  atomicAdd(cnt + e, 1);
+}
+
+/// Contribution codes into their entries' lists (any order; sorted next).
+template <typename O>
+__global__ void __launch_bounds__(kBlock)
+    contrib_fill_k(std::int64_t ninc, int k, const std::int32_t* __restrict__ inc,
+                   const O* __restrict__ islots, const std::int32_t* __restrict__ dm,
+                   const std::int32_t* __restrict__ rp, std::int32_t* __restrict__ cursor,
+                   std::int32_t* __restrict__ code)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t >= ninc * k)
+    return;
+  const std::int32_t cik = inc[t / k];
+  const int j = static_cast<int>(t % k);
+  const std::int64_t e = std::int64_t(rp[dm[cik]]) + islots[t];
+  const std::int32_t pos = atomicAdd(cursor + e, 1);
+  code[pos] = cik * k + j;
+}
+
+/// Each entry's list sorted ascending (= cell order: one contribution per
+/// cell and entry).
+__global__ void __launch_bounds__(kBlock)
+    contrib_sort_k(std::int64_t nnz, const std::int32_t* __restrict__ off,
+                   std::int32_t* __restrict__ code)
+{
+  const std::int64_t e = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (e >= nnz)
+    return;
+  const std::int32_t q0 = off[e], q1 = off[e + 1];
+  for (std::int32_t q = q0 + 1; q < q1; ++q)
+  {
+    const std::int32_t v = code[q];
+    std::int32_t w = q - 1;
+    while (w >= q0 && code[w] > v)
+    {
+      code[w + 1] = code[w];
+      --w;
+    }
+    code[w + 1] = v;
+  }
+}
+
+/// G lanes per row: lane l sums entries rp[r] + l, + G, ... over their
+/// contributions; rows and columns of Dirichlet dofs store zero (every
+/// contribution to such an entry has its row or column masked).
+template <typename T, int K, int G, bool MAT>
+__global__ void __launch_bounds__(kBlock)
+    entry_assemble_k(std::int32_t nrows, std::int32_t chunk, const std::int32_t* __restrict__ rp,
+                     const std::int32_t* __restrict__ ci, const std::int32_t* __restrict__ off,
+                     const std::int32_t* __restrict__ code, const T* __restrict__ cd,
+                     const T* __restrict__ elem, const T* __restrict__ ref,
+                     const std::int8_t* __restrict__ bc, T* __restrict__ vals, int accumulate)
+{
+  constexpr int KK = K * K;
+  extern __shared__ unsigned char smem_raw[];
+  T* S = reinterpret_cast<T*>(smem_raw); // reference tensors [7][K][K] unless MAT
+  if constexpr (!MAT)
+  {
+    for (int t = threadIdx.x; t < 7 * KK; t += kBlock)
+      S[t] = ref[t];
+    __syncthreads();
+  }
+  const int lane = threadIdx.x % G;
+  // the workgroup's contiguous range of chunk rows, kBlock / G at a time
+  const std::int32_t rb = xcd_contiguous(blockIdx.x, gridDim.x) * chunk;
+  const std::int32_t re = min(rb + chunk, nrows);
+  for (std::int32_t r = rb + threadIdx.x / G; r < re; r += kBlock / G)
+  {
+    const bool rbc = bc && bc[r];
+    const std::int32_t e1 = rp[r + 1];
+    for (std::int32_t e = rp[r] + lane; e < e1; e += G)
+    {
+      T v = T(0);
+      if (!rbc && !(bc && bc[ci[e]]))
+      {
+        const std::int32_t q1 = off[e + 1];
+        for (std::int32_t q = off[e]; q < q1; ++q)
+        {
+          const std::int32_t cq = code[q];
+          if constexpr (MAT)
+            v += elem[cq];
+          else
+          {
+            const std::int32_t c = cq / KK;
+            const int ij = cq - c * KK;
+            const T* F = cd + 8 * std::int64_t(c);
+            const T* Sij = S + ij;
+            v += F[0] * Sij[0] + F[1] * Sij[KK] + F[2] * Sij[2 * KK] + F[3] * Sij[3 * KK]
+                 + F[4] * Sij[4 * KK] + F[5] * Sij[5 * KK] + F[6] * Sij[6 * KK];
+          }
+        }
+      }
+      vals[e] = accumulate ? vals[e] + v : v;
+    }
+  }
+}
+
 typedef int dfx_i32x4 __attribute__((ext_vector_type(4)));
 
 /// Variant 12-14 (Q1 hexahedra whose vertices lie in at most 8 cells): one
@@ -3332,8 +3454,10 @@ int incidence_lanes(const dfx_form& f)
 /// Row-kernel variants, picked per form by dfx_form_autotune: 0..2 "rows"
 /// with 64/128/256 rows per workgroup; 3 "lanes"; 4..6 closed-form Q1 rows;
 /// 7, 8 incidence-parallel gathers; 9 structured Q1; 10 fused geometry;
-/// 11 as 1 with eight incidences in flight.
-constexpr int kRowVariants = 15;
+/// 11 as 1 with eight incidences in flight; 12..14 fixed-width Q1 rows;
+/// 15, 16 entry-parallel (contributions from the factors / from element
+/// matrices formed on the matrix cores).
+constexpr int kRowVariants = 17;
 
 template <typename T, typename O, int K, int L>
 void launch_lanes(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
@@ -3662,10 +3786,119 @@ bool ensure_fixed(dfx_form& f, const dfx_csr& A)
   return true;
 }
 
+/// Entry-parallel plan of a form for the pattern of A (built once): count
+/// the contributions of every entry, offsets by a prefix sum, fill, then
+/// sort each entry's list into cell order.
+template <typename O>
+void ensure_contrib(dfx_form& f, const dfx_csr& A)
+{
+  if (f.contrib && f.contrib_pattern == A.pattern_id)
+    return;
+  dfree(f.contrib);
+  dfree(f.contrib_off);
+  f.contrib = nullptr;
+  f.contrib_off = nullptr;
+  const int k = f.d.dofs_per_cell;
+  const std::int64_t ninc = std::int64_t(f.d.num_cells) * k;
+  require(ninc * k < INT32_MAX, "too many element contributions for the entry-parallel plan");
+  const std::int64_t nnz = A.nnz;
+  std::int32_t* cnt = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * (nnz + 1)));
+  DFX_HIP(hipMemsetAsync(cnt, 0, sizeof(std::int32_t) * (nnz + 1), stream()));
+  const O* sl = static_cast<const O*>(f.islots);
+  if (ninc > 0)
+    contrib_count_k<O><<<blocks_for(ninc * k), kBlock, 0, stream()>>>(ninc, k, f.inc, sl,
+                                                                       f.dofmap, A.row_ptr, cnt);
+  check_launch("contribution count");
+  std::vector<std::int32_t> h(nnz + 1);
+  d2h(h.data(), cnt, sizeof(std::int32_t) * nnz);
+  std::int32_t run = 0;
+  for (std::int64_t e = 0; e < nnz; ++e)
+  {
+    const std::int32_t c = h[e];
+    h[e] = run;
+    run += c;
+  }
+  h[nnz] = run;
+  f.contrib_off = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * (nnz + 1)));
+  h2d(f.contrib_off, h.data(), sizeof(std::int32_t) * (nnz + 1));
+  h2d(cnt, h.data(), sizeof(std::int32_t) * (nnz + 1)); // fill cursors
+  f.contrib = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * std::max(run, 1)));
+  if (ninc > 0)
+  {
+    contrib_fill_k<O><<<blocks_for(ninc * k), kBlock, 0, stream()>>>(ninc, k, f.inc, sl, f.dofmap,
+                                                                      A.row_ptr, cnt, f.contrib);
+    contrib_sort_k<<<blocks_for(nnz), kBlock, 0, stream()>>>(nnz, f.contrib_off, f.contrib);
+  }
+  check_launch("contribution plan");
+  DFX_HIP(hipStreamSynchronize(stream()));
+  dfree(cnt);
+  // lanes per row: the power of two among 16, 32, 64 nearest above the
+  // mean row length
+  const double mean = A.nrows ? double(nnz) / A.nrows : 1.0;
+  f.entry_lanes = mean <= 20 ? 16 : (mean <= 40 ? 32 : 64);
+  f.contrib_pattern = A.pattern_id;
+}
+
+/// Variants 15 (contributions from the cell factors) and 16 (from the
+/// element matrices of all cells, formed on the matrix cores).
+template <typename T, typename O, int K>
+void launch_entries(dfx_form& f, dfx_csr& A, const dfx_bc* bc, bool mat)
+{
+  ensure_contrib<O>(f, A);
+  launch_cell_factors<T>(f, bc);
+  const std::int32_t nc = f.d.num_cells;
+  if (mat)
+  {
+    if (!f.cell_elem)
+      f.cell_elem = dmalloc(sizeof(T) * std::max<std::int64_t>(std::int64_t(nc) * K * K, 1));
+    const int nb = static_cast<int>((std::int64_t(nc) + 63) / 64);
+    if (nb > 0)
+      elem_mfma_k<T, K><<<nb, kBlock, 0, stream()>>>(
+          nc, static_cast<const T*>(f.cell_data), static_cast<const T*>(f.ref_tensors),
+          static_cast<T*>(f.cell_elem));
+  }
+  const std::size_t lds = mat ? 0 : sizeof(T) * 7 * K * K;
+  const int G = f.entry_lanes;
+  // a grid of 8 workgroups per CU, each over a contiguous range of rows
+  // (the reference tensors are staged once per workgroup)
+  const std::int64_t rows_per_pass = kBlock / G;
+  const std::int64_t nb64 = std::min<std::int64_t>(
+      std::int64_t(ctx().num_cu) * 8, (A.nrows + rows_per_pass - 1) / rows_per_pass);
+  if (nb64 <= 0)
+    return;
+  const int nb = static_cast<int>(nb64);
+  const std::int32_t chunk = static_cast<std::int32_t>((A.nrows + nb64 - 1) / nb64);
+  auto go = [&](auto kern)
+  {
+    kern<<<nb, kBlock, lds, stream()>>>(A.nrows, chunk, A.row_ptr, A.cols, f.contrib_off, f.contrib,
+                                        static_cast<const T*>(f.cell_data),
+                                        static_cast<const T*>(f.cell_elem),
+                                        static_cast<const T*>(f.ref_tensors),
+                                        bc ? bc->marker : nullptr, static_cast<T*>(A.values),
+                                        A.zeroed ? 0 : 1);
+  };
+  if (mat)
+    G == 16 ? go(entry_assemble_k<T, K, 16, true>)
+            : (G == 32 ? go(entry_assemble_k<T, K, 32, true>)
+                       : go(entry_assemble_k<T, K, 64, true>));
+  else
+    G == 16 ? go(entry_assemble_k<T, K, 16, false>)
+            : (G == 32 ? go(entry_assemble_k<T, K, 32, false>)
+                       : go(entry_assemble_k<T, K, 64, false>));
+  check_launch("entry assembly");
+}
+
 template <typename T, typename O, int K>
 void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
 {
   const int variant = f.row_variant;
+  if (variant == 15 || variant == 16)
+  {
+    // the entry kernel reads the Dirichlet marker at column indices
+    require(!bc || A.ncols == A.nrows, "variant not available for a non-square matrix");
+    launch_entries<T, O, K>(f, A, bc, variant == 16);
+    return;
+  }
   if (variant >= 12 && variant <= 14)
   {
     // fixed-width rows: 64, 128 or 256 rows per workgroup
@@ -4616,6 +4849,8 @@ dfx_status dfx_form_create_facets(const dfx_form_desc* desc, int32_t npairs,
         require(desc && out && (npairs == 0 || pairs), "null argument");
         require(is_facet_kernel(desc->kernel), "the kernel has no exterior facet integral");
         require(npairs >= 0, "negative size");
+        // incidences are packed as 4 e + facet vertex in an int32
+        require(npairs < (1 << 29), "too many facets (at most 2^29 - 1)");
         const bool hex = desc->kernel == DFX_K_Q1HEX_NEUMANN || desc->kernel == DFX_K_Q1HEX_ROBIN_A;
         const bool matrix = is_facet_matrix_kernel(desc->kernel);
         const int nfc = hex ? 6 : 4, nfv = hex ? 4 : 3, k = desc->dofs_per_cell;
@@ -4782,6 +5017,8 @@ dfx_status dfx_form_destroy(dfx_form* f)
     dfree(f->slotmap);
     dfree(f->inc8);
     dfree(f->islot8);
+    dfree(f->contrib);
+    dfree(f->contrib_off);
     dfree(f->elem_tensors);
     dfree(f->pack_map);
     dfree(f->pack_stage);
@@ -5088,7 +5325,7 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t*
         f.row_variant = best;
         // release the plans of the variants not kept (the element matrices
         // of variant 7 alone are k^2 values per cell)
-        if (best != 7)
+        if (best != 7 && best != 16)
         {
           dfree(f.cell_elem);
           f.cell_elem = nullptr;
@@ -5101,6 +5338,14 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t*
           f.inc_row = nullptr;
           f.ngblk = 0;
           f.gather_pattern = 0;
+        }
+        if (best != 15 && best != 16)
+        {
+          dfree(f.contrib);
+          dfree(f.contrib_off);
+          f.contrib = nullptr;
+          f.contrib_off = nullptr;
+          f.contrib_pattern = 0;
         }
         if (best < 12 || best > 14)
         {
@@ -5435,10 +5680,32 @@ dfx_status dfx_assemble_system(dfx_csr* A, const dfx_form* a, const dfx_bc* bc, 
         // assembled (below ~1M cells the fork and join cost more than the
         // overlap gains)
         const bool fork = L->d.num_cells >= (1 << 20);
+        // once forked, the library stream waits for the second stream on
+        // every way out (an error in either part included), so no later
+        // work on the library stream runs beside writes to b
+        struct JoinGuard
+        {
+          Context& c;
+          bool pending = false;
+          void join()
+          {
+            pending = false;
+            DFX_HIP(hipStreamWaitEvent(c.stream, c.join, 0));
+          }
+          ~JoinGuard()
+          {
+            if (pending)
+            {
+              (void)hipEventRecord(c.join, c.aux);
+              (void)hipStreamWaitEvent(c.stream, c.join, 0);
+            }
+          }
+        } jg{c};
         if (fork)
         {
           DFX_HIP(hipEventRecord(c.fork, c.stream));
           DFX_HIP(hipStreamWaitEvent(c.aux, c.fork, 0));
+          jg.pending = true;
         }
         {
           StreamScope on(fork ? c.aux : c.stream);
@@ -5452,7 +5719,7 @@ dfx_status dfx_assemble_system(dfx_csr* A, const dfx_form* a, const dfx_bc* bc, 
         if (bc)
           ok(dfx_set_diagonal(A, bc, diag));
         if (fork)
-          DFX_HIP(hipStreamWaitEvent(c.stream, c.join, 0));
+          jg.join();
         if (bc)
         {
           ok(dfx_apply_lifting(b, a, bc, nullptr, 1.0));

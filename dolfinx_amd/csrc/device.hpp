@@ -274,6 +274,13 @@ struct dfx_form
   std::int32_t* inc8 = nullptr;
   std::uint8_t* islot8 = nullptr;
   std::uint64_t fixed_pattern = 0;
+  // Entry-parallel plan: for every CSR entry the element contributions
+  // (c k + i) k + j that add into it, in cell order (contrib, contrib_off:
+  // nnz + 1 offsets); lanes per row of the entry kernel.
+  std::int32_t* contrib = nullptr;
+  std::int32_t* contrib_off = nullptr;
+  std::uint64_t contrib_pattern = 0;
+  int entry_lanes = 32;
   // Forms whose cell tensors come from a host element kernel with the
   // reference's kernel ABI (kernelptr_t): the kernel and its custom data,
   // host copies of what it reads per cell (geometry, packed coefficients,

@@ -447,8 +447,9 @@ __global__ void __launch_bounds__(kBlock)
   const std::int32_t w = static_cast<std::int32_t>((off[sl + 1] - base) / kSlice);
   const std::int32_t j0 = r < nrows ? rp[r] : 0;
   const std::int32_t len = r < nrows ? rp[r + 1] - j0 : 0;
-  for (std::int32_t k = 0; k < w; ++k)
-    sval[base + std::int64_t(k) * kSlice + lane] = k < len ? v[j0 + k] : T(0);
+  if (sval)
+    for (std::int32_t k = 0; k < w; ++k)
+      sval[base + std::int64_t(k) * kSlice + lane] = k < len ? v[j0 + k] : T(0);
   if (!cols)
     return;
   const std::int32_t b = cbase[sl];
@@ -743,6 +744,8 @@ __global__ void __launch_bounds__(kBlock)
 /// Lanes per row of the scalar CSR kernels for a mean row length.
 int lanes_per_row(const dfx_csr& A)
 {
+  if (A.lanes == -1)
+    return 1; // sliced-ELL snapshot: one lane per row
   if (A.lanes)
     return A.lanes;
   if (A.nblk > 0)
@@ -790,11 +793,54 @@ void prime_stream_grids()
                  });
 }
 
+constexpr int kMaxPartials = 2048;
+
+/// y (+)= A x through the sliced-ELL copy of A (its values as last
+/// copied), with the partial sums of y . dotv into partials when given.
+/// Returns the number of workgroups.
+template <typename T>
+int launch_sell(const dfx_csr& A, const T* x, T* y, int accumulate, const T* dotv,
+                double* partials, const int* stop)
+{
+  // 1024 workgroups (16 wavefronts per CU) keep the stream fed and halve
+  // the partials a reduction sums
+  const int ngroups = (A.nslices + 3) / 4;
+  const int nb = std::max(1, std::min(ngroups, kMaxPartials / 2));
+  // entries per step: the width among 16, 12, 9, 8 that pads the longest
+  // row least (27-entry hexahedral rows: 3 steps of 9, no clamped loads)
+  int G = 16;
+  for (int g : {12, 9, 8})
+    if ((A.max_row + g - 1) / g * g < (A.max_row + G - 1) / G * G)
+      G = g;
+  auto go = [&](auto kern)
+  {
+    kern<<<nb, kBlock, 0, stream()>>>(A.nrows, A.nslices, A.sell_off, A.sell_base, A.row_ptr,
+                                      static_cast<const unsigned char*>(A.sell_cols),
+                                      static_cast<const T*>(A.sell_vals), x, y, accumulate, dotv,
+                                      partials, stop);
+  };
+  if (G == 9)
+    go(sell_spmv_k<T, 9>);
+  else if (G == 12)
+    go(sell_spmv_k<T, 12>);
+  else if (G == 8)
+    go(sell_spmv_k<T, 8>);
+  else
+    go(sell_spmv_k<T, 16>);
+  return nb;
+}
+
 template <typename T>
 void launch_spmv(const dfx_csr& A, const T* x, T* y, bool accumulate)
 {
   materialize_zero(A);
   hipStream_t s = stream();
+  if (A.lanes == -1)
+  {
+    // the sliced-ELL snapshot taken by dfx_csr_set_lanes(A, -1)
+    launch_sell<T>(A, x, y, accumulate ? 1 : 0, nullptr, nullptr, nullptr);
+    return;
+  }
   if (A.bs == 1 && lanes_per_row(A) == 1)
   {
     const int nb = std::max(
@@ -869,13 +915,13 @@ void launch_spmv(const dfx_csr& A, const T* x, T* y, bool accumulate)
 // stop test fires every later kernel returns at once.
 // ---------------------------------------------------------------------
 
-constexpr int kMaxPartials = 2048;
 
 struct CGState
 {
   double rz[2];
   double rr0, rr, tol2;
-  std::int32_t iter, done, max_it, pad;
+  std::int32_t iter, done, max_it;
+  std::int32_t kd; // iteration the direction kernel completes (set by the update kernel)
   double partials[3 * kMaxPartials];
 };
 
@@ -998,11 +1044,16 @@ __global__ void __launch_bounds__(kBlock)
 /// alpha = rz_k / p.q; x += alpha p; r -= alpha q; partials of r.z, r.r.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-    cg_update_k(std::int64_t n, int k, int nb_spmv, T* x, T* r, const T* p, const T* q,
+    cg_update_k(std::int64_t n, int nb_spmv, T* x, T* r, const T* p, const T* q,
                 const T* dinv, CGState* st)
 {
   if (block_flag(&st->done))
     return;
+  // the iteration index lives in the state (written by the previous
+  // direction kernel), so a captured batch of iterations can be replayed
+  const int k = st->iter;
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    st->kd = k;
   const double pq = sum_partials(st->partials, nb_spmv);
   const T alpha = static_cast<T>(st->rz[k & 1] / pq);
   double s[2] = {0.0, 0.0};
@@ -1022,11 +1073,12 @@ __global__ void __launch_bounds__(kBlock)
 /// beta = rz_{k+1} / rz_k; stop test; p = dinv r + beta p.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-    cg_direction_k(std::int64_t n, int k, int nb_upd, const T* r, const T* dinv, T* p,
+    cg_direction_k(std::int64_t n, int nb_upd, const T* r, const T* dinv, T* p,
                    CGState* st)
 {
   if (block_flag(&st->done))
     return;
+  const int k = st->kd;
   const double rz = sum_partials(st->partials + kMaxPartials, nb_upd);
   const double rr = sum_partials(st->partials + kMaxPartials + nb_upd, nb_upd);
   const bool stop = rr / st->rr0 < st->tol2 || k + 1 >= st->max_it;
@@ -1065,13 +1117,15 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 /// Re-arms a stopped CG after a residual replacement.
-__global__ void cg_resume_k(CGState* st, double rr0, double tol2)
+__global__ void cg_resume_k(CGState* st, double rr0, double tol2, int kd)
 {
   if (threadIdx.x == 0 && blockIdx.x == 0)
   {
     st->rr0 = rr0;
     st->tol2 = tol2;
     st->done = 0;
+    if (kd >= 0)
+      st->kd = kd;
   }
 }
 
@@ -1123,7 +1177,18 @@ struct StateHolder
 /// and shorter rows mixed in one slice, as in P3 patterns, stay on the
 /// stream kernel).
 template <typename T>
-bool refresh_sell(const dfx_csr& A)
+bool refresh_sell(const dfx_csr& A, bool values = true);
+
+/// The sliced-ELL layout of A (pattern work, allocation) without copying
+/// the values; true if the solvers will use the copy.
+template <typename T>
+bool sell_layout(const dfx_csr& A)
+{
+  return refresh_sell<T>(A, false);
+}
+
+template <typename T>
+bool refresh_sell(const dfx_csr& A, bool values)
 {
   if (A.bs != 1 || A.nrows == 0 || lanes_per_row(A) != 1)
     return false;
@@ -1173,14 +1238,18 @@ bool refresh_sell(const dfx_csr& A)
     A.nslices = ns;
     A.sell_use = true;
     const int nb = static_cast<int>((nt + kBlock - 1) / kBlock);
+    // columns (and values unless only the layout is asked for)
     sell_build_k<T><<<nb, kBlock, 0, stream()>>>(
         A.nrows, ns, A.sell_off, A.sell_base, A.row_ptr, A.cols, static_cast<const T*>(A.values),
-        static_cast<unsigned char*>(A.sell_cols), static_cast<T*>(A.sell_vals));
+        static_cast<unsigned char*>(A.sell_cols),
+        values ? static_cast<T*>(A.sell_vals) : nullptr);
     check_launch("sell_build");
     return true;
   }
   if (!A.sell_use)
     return false;
+  if (!values)
+    return true;
   const std::int64_t nt = std::int64_t(ns) * kSlice;
   sell_build_k<T><<<static_cast<int>((nt + kBlock - 1) / kBlock), kBlock, 0, stream()>>>(
       A.nrows, ns, A.sell_off, A.sell_base, A.row_ptr, A.cols, static_cast<const T*>(A.values),
@@ -1195,34 +1264,7 @@ int launch_cg_spmv(const dfx_csr& A, const T* p, T* q, CGState* st, bool sell)
   hipStream_t s = stream();
   const T* v = static_cast<const T*>(A.values);
   if (sell)
-  {
-    // 1024 workgroups (16 wavefronts per CU) keep the stream fed and halve
-    // the partials every update kernel sums
-    const int ngroups = (A.nslices + 3) / 4;
-    const int nb = std::max(1, std::min(ngroups, kMaxPartials / 2));
-    // entries per step: the width among 16, 12, 9, 8 that pads the longest
-    // row least (27-entry hexahedral rows: 3 steps of 9, no clamped loads)
-    int G = 16;
-    for (int g : {12, 9, 8})
-      if ((A.max_row + g - 1) / g * g < (A.max_row + G - 1) / G * G)
-        G = g;
-    auto go = [&](auto kern)
-    {
-      kern<<<nb, kBlock, 0, s>>>(A.nrows, A.nslices, A.sell_off, A.sell_base, A.row_ptr,
-                                 static_cast<const unsigned char*>(A.sell_cols),
-                                 static_cast<const T*>(A.sell_vals), p, q, 0, p, st->partials,
-                                 &st->done);
-    };
-    if (G == 9)
-      go(sell_spmv_k<T, 9>);
-    else if (G == 12)
-      go(sell_spmv_k<T, 12>);
-    else if (G == 8)
-      go(sell_spmv_k<T, 8>);
-    else
-      go(sell_spmv_k<T, 16>);
-    return nb;
-  }
+    return launch_sell<T>(A, p, q, 0, p, st->partials, &st->done);
   if (A.bs == 1 && lanes_per_row(A) == 1)
   {
     const int nb = std::max(
@@ -1278,12 +1320,49 @@ int launch_cg_spmv(const dfx_csr& A, const T* p, T* q, CGState* st, bool sell)
   return nb;
 }
 
+/// A captured sequence of launches on a stream, replayed with
+/// hipGraphLaunch (no memset or copy nodes: kernels only).
+struct BatchGraph
+{
+  hipGraphExec_t exec = nullptr;
+  template <typename F>
+  void capture(hipStream_t s, F&& launches)
+  {
+    hipGraph_t g = nullptr;
+    DFX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try
+    {
+      launches();
+    }
+    catch (...)
+    {
+      (void)hipStreamEndCapture(s, &g);
+      if (g)
+        (void)hipGraphDestroy(g);
+      throw;
+    }
+    DFX_HIP(hipStreamEndCapture(s, &g));
+    const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess)
+    {
+      exec = nullptr;
+      throw Error(DFX_ERR_HIP, std::string("graph instantiation: ") + hipGetErrorString(e));
+    }
+  }
+  ~BatchGraph()
+  {
+    if (exec)
+      (void)hipGraphExecDestroy(exec);
+  }
+};
+
 /// Runs (preconditioned) CG on A x = b. All arrays are device arrays of T.
 /// Returns iterations; sets *converged.
 template <typename T>
 int cg_run(const dfx_csr& A, const T* b, T* x, int max_it, double rtol, bool jacobi,
            int check_every, CGWork<T>& w, CGState* st, bool* converged, double* rr0,
-           double* rr, bool x_is_zero = false, bool allow_sell = true)
+           double* rr, bool x_is_zero = false, bool allow_sell = true, bool use_graph = false)
 {
   hipStream_t s = stream();
   const std::int64_t n = std::int64_t(A.nrows) * A.bs;
@@ -1314,17 +1393,29 @@ int cg_run(const dfx_csr& A, const T* b, T* x, int max_it, double rtol, bool jac
     std::int32_t iter, done;
   } h{0, 0};
   const int batch = std::max(1, check_every);
+  auto iteration = [&]()
+  {
+    const int nbs = launch_cg_spmv<T>(A, w.p, w.q, st, sell);
+    cg_update_k<T><<<nbv, kBlock, 0, s>>>(n, nbs, x, w.r, w.p, w.q, jacobi ? w.dinv : nullptr,
+                                          st);
+    cg_direction_k<T><<<nbv, kBlock, 0, s>>>(n, nbv, w.r, jacobi ? w.dinv : nullptr, w.p, st);
+  };
+  // a batch of iterations captured once and replayed (the kernels read
+  // the iteration index from the state and do nothing once it stopped)
+  BatchGraph graph;
+  if (use_graph)
+    graph.capture(s, [&]() { for (int j = 0; j < batch; ++j) iteration(); });
   int k = 0;
   while (true)
   {
-    for (int j = 0; j < batch && k < max_it; ++j, ++k)
+    if (graph.exec)
     {
-      const int nbs = launch_cg_spmv<T>(A, w.p, w.q, st, sell);
-      cg_update_k<T><<<nbv, kBlock, 0, s>>>(n, k, nbs, x, w.r, w.p, w.q,
-                                            jacobi ? w.dinv : nullptr, st);
-      cg_direction_k<T><<<nbv, kBlock, 0, s>>>(n, k, nbv, w.r, jacobi ? w.dinv : nullptr,
-                                               w.p, st);
+      DFX_HIP(hipGraphLaunch(graph.exec, s));
+      k += batch;
     }
+    else
+      for (int j = 0; j < batch && k < max_it; ++j, ++k)
+        iteration();
     check_launch("cg iteration");
     DFX_HIP(hipMemcpyAsync(&h, &st->iter, sizeof(h), hipMemcpyDeviceToHost, s));
     DFX_HIP(hipStreamSynchronize(s));
@@ -1403,13 +1494,22 @@ void cg_solve_t(const dfx_csr& A, const dfx_vec& b, dfx_vec& x,
   CGWork<T> w;
   w.alloc(n, o.jacobi != 0);
   StateHolder sh;
+  Reducer& red = reducer();
   DFX_HIP(hipMemsetAsync(sh.st, 0, sizeof(CGState), stream()));
+  // pattern work of the sliced-ELL copy (as the sparsity pattern, outside
+  // the timed window); its values are copied inside
+  sell_layout<T>(A);
   DFX_HIP(hipStreamSynchronize(stream()));
   const auto t0 = std::chrono::steady_clock::now();
   bool conv = false;
   double rr0 = 0, rr = 0;
   const int it = cg_run<T>(A, static_cast<const T*>(b.data), static_cast<T*>(x.data), o.max_it,
-                           o.rtol, o.jacobi != 0, o.check_every, w, sh.st, &conv, &rr0, &rr);
+                           o.rtol, o.jacobi != 0, o.check_every, w, sh.st, &conv, &rr0, &rr,
+                           false, true, o.graph != 0);
+  // the true residual ||b - A x|| / ||b|| that ends the solve
+  const double bn = std::sqrt(red.dot(static_cast<const T*>(b.data), static_cast<const T*>(b.data), n));
+  const double tr = true_residual<T>(A, static_cast<const T*>(b.data), static_cast<const T*>(x.data),
+                                     w.q);
   DFX_HIP(hipStreamSynchronize(stream()));
   const auto t1 = std::chrono::steady_clock::now();
   res.iterations = it;
@@ -1417,10 +1517,6 @@ void cg_solve_t(const dfx_csr& A, const dfx_vec& b, dfx_vec& x,
   res.rnorm0 = std::sqrt(rr0);
   res.rnorm = std::sqrt(rr);
   res.seconds = std::chrono::duration<double>(t1 - t0).count();
-  Reducer& red = reducer();
-  const double bn = std::sqrt(red.dot(static_cast<const T*>(b.data), static_cast<const T*>(b.data), n));
-  const double tr = true_residual<T>(A, static_cast<const T*>(b.data), static_cast<const T*>(x.data),
-                                     w.q);
   res.true_rel_residual = bn > 0 ? tr / bn : tr;
 }
 
@@ -1661,8 +1757,17 @@ dfx_status dfx_csr_create(const int64_t* row_ptr, const int32_t* cols, int32_t n
             maxrow = std::max<std::int32_t>(maxrow, rp32[i] - rp32[i - 1]);
           }
         }
-        for (std::int64_t j = 0; j < nnz; ++j)
-          require(cols[j] >= 0 && cols[j] < ncols, "column index out of range");
+        // every row's columns strictly increasing (the finalised rows of
+        // SparsityPattern::finalize): the device kernels find entries by
+        // binary search in a row
+        for (std::int32_t i = 0; i < nrows; ++i)
+          for (std::int64_t j = row_ptr[i]; j < row_ptr[i + 1]; ++j)
+          {
+            require(cols[j] >= 0 && cols[j] < ncols, "column index out of range");
+            require(j == row_ptr[i] || cols[j] > cols[j - 1],
+                    "columns of row " + std::to_string(i)
+                        + " are not strictly increasing (sorted, no duplicates)");
+          }
         auto A = std::make_unique<dfx_csr>();
         A->nrows = nrows;
         A->ncols = ncols;
@@ -1979,9 +2084,20 @@ dfx_status dfx_csr_set_lanes(dfx_csr* A, int32_t lanes)
       [&]
       {
         require(A, "null matrix");
-        require(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8
-                    || lanes == 16 || lanes == 32 || lanes == 64,
-                "lanes must be 0 (default), 1 (stream) or a power of two in [2, 64]");
+        require(lanes == -1 || lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4
+                    || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64,
+                "lanes must be -1 (sliced-ELL snapshot), 0 (default), 1 (stream) or a power "
+                "of two in [2, 64]");
+        if (lanes == -1)
+        {
+          require(A->bs == 1, "the sliced-ELL copy is for block size 1");
+          materialize_zero(*A);
+          const int saved = A->lanes;
+          A->lanes = 1;
+          const bool ok = A->dtype == DFX_F64 ? refresh_sell<double>(*A) : refresh_sell<float>(*A);
+          A->lanes = saved;
+          require(ok, "no sliced-ELL copy for this matrix (padding above 15 %)");
+        }
         A->lanes = lanes;
       });
 }
@@ -2091,7 +2207,9 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
         require(b->n == A64->nrows * A64->bs && x->n == b->n, "size mismatch");
         materialize_zero(*A64);
         materialize_zero(*A32);
-        const bool sell = refresh_sell<float>(*A32);
+        // pattern work of the FP32 sliced-ELL copy outside the timed window
+        // (as for the FP64 solve); its values are copied inside
+        sell_layout<float>(*A32);
         *res = dfx_mixed_result{};
         hipStream_t s = stream();
         const std::int64_t n = b->n;
@@ -2109,6 +2227,7 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
         const int nb = reduce_blocks(n);
         const int nbv = partial_blocks(n);
         const auto t0 = std::chrono::steady_clock::now();
+        const bool sell = refresh_sell<float>(*A32);
         const double bn2 = red.dot(bd, bd, n);
         const double tol2 = o->rtol * o->rtol * bn2;
         // r64 = b - A64 x, r32 = (float) r64 (written into the CG residual)
@@ -2148,7 +2267,7 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
           // r = r32 - A32 * 0, p = z: reuse cg_start with y = 0
           cg_start_k<float><<<nbv, kBlock, 0, s>>>(n, w.r, w.q, w.dinv, w.r, w.p, st);
           cg_start_finish_k<<<1, kBlock, 0, s>>>(nbv, st);
-          cg_resume_k<<<1, 1, 0, s>>>(st, rr, inner_tol2(rr));
+          cg_resume_k<<<1, 1, 0, s>>>(st, rr, inner_tol2(rr), -1);
           check_launch("mixed start");
         }
         int k = 0;
@@ -2156,14 +2275,28 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
         {
           std::int32_t iter, done;
         } h{0, 0};
+        const int batch = std::max(1, o->check_every);
+        auto iteration = [&]()
+        {
+          const int nbs = launch_cg_spmv<float>(*A32, w.p, w.q, st, sell);
+          cg_update_k<float><<<nbv, kBlock, 0, s>>>(n, nbs, d32, w.r, w.p, w.q, w.dinv, st);
+          cg_direction_k<float><<<nbv, kBlock, 0, s>>>(n, nbv, w.r, w.dinv, w.p, st);
+        };
+        // inner iterations replayed from a captured batch (the kernels read
+        // the iteration index from the state)
+        BatchGraph graph;
+        if (o->graph && !ok)
+          graph.capture(s, [&]() { for (int j = 0; j < batch; ++j) iteration(); });
         while (!ok && k < o->max_inner)
         {
-          for (int j = 0; j < std::max(1, o->check_every) && k < o->max_inner; ++j, ++k)
+          if (graph.exec)
           {
-            const int nbs = launch_cg_spmv<float>(*A32, w.p, w.q, st, sell);
-            cg_update_k<float><<<nbv, kBlock, 0, s>>>(n, k, nbs, d32, w.r, w.p, w.q, w.dinv, st);
-            cg_direction_k<float><<<nbv, kBlock, 0, s>>>(n, k, nbv, w.r, w.dinv, w.p, st);
+            DFX_HIP(hipGraphLaunch(graph.exec, s));
+            k += batch;
           }
+          else
+            for (int j = 0; j < batch && k < o->max_inner; ++j, ++k)
+              iteration();
           check_launch("mixed iteration");
           DFX_HIP(hipMemcpyAsync(&h, &st->iter, sizeof(h), hipMemcpyDeviceToHost, s));
           DFX_HIP(hipStreamSynchronize(s));
@@ -2190,7 +2323,7 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
             fill_k<float><<<nb, kBlock, 0, s>>>(w.q, n, 0.0f);
             cg_start_k<float><<<nbv, kBlock, 0, s>>>(n, w.r, w.q, w.dinv, w.r, w.p, st);
             cg_start_finish_k<<<1, kBlock, 0, s>>>(nbv, st);
-            cg_resume_k<<<1, 1, 0, s>>>(st, rr, inner_tol2(rr));
+            cg_resume_k<<<1, 1, 0, s>>>(st, rr, inner_tol2(rr), -1);
             k = 0;
           }
           else
@@ -2198,9 +2331,9 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
             // reliable update: keep the search direction; beta is formed
             // from the replaced residual, rz_new / rz_old
             const int kk = h.iter - 1;
-            cg_resume_k<<<1, 1, 0, s>>>(st, rr, inner_tol2(rr));
+            cg_resume_k<<<1, 1, 0, s>>>(st, rr, inner_tol2(rr), kk);
             cg_rz_partials_k<float><<<nbv, kBlock, 0, s>>>(n, w.r, w.dinv, st);
-            cg_direction_k<float><<<nbv, kBlock, 0, s>>>(n, kk, nbv, w.r, w.dinv, w.p, st);
+            cg_direction_k<float><<<nbv, kBlock, 0, s>>>(n, nbv, w.r, w.dinv, w.p, st);
             k = kk + 1;
           }
           check_launch("mixed refinement");

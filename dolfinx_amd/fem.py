@@ -279,7 +279,10 @@ class Form:
         closed-form Q1 entries, 7 element matrices on the matrix cores then
         gathered by rows, 8 one incidence per thread, 9 structured-box Q1
         rows in registers, 10 factors formed from the geometry per
-        incidence, 11 as 1 with eight incidences in flight)."""
+        incidence, 11 as 1 with eight incidences in flight, 12-14 Q1 rows
+        with their plan at fixed positions, 15/16 entry-parallel sums of
+        each entry's contributions from the factors / from the element
+        matrices)."""
         _lib.call("dfx_form_set_variant", self._h, int(variant))
 
     @property

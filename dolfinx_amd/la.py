@@ -117,6 +117,9 @@ class MatrixCSR:
         return lanes.value, ms.value
 
     def set_lanes(self, lanes):
+        """SpMV kernel of mult: 0 default, 1 row-block stream, 2..64 lanes
+        per row, -1 the solvers' sliced-ELL copy of the values as they are
+        now (set again after the values change)."""
         _lib.call("dfx_csr_set_lanes", self._h, int(lanes))
 
     def mult(self, x, y):

@@ -210,23 +210,31 @@ class SolveService:
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
                 queue.append(pending.popleft())
-            # admission: FIFO while a slot is free and the footprint fits;
-            # the admitted group (one request per mesh size, as a template's
-            # forms hold one request's coefficients) is assembled by one
-            # batched launch pair
-            group = []
-            while queue and len(active) + len(group) < self.max_slots:
+            # admission: in arrival order while a slot is free and the
+            # footprint fits the HBM budget (a request that does not fit
+            # waits, and so do the requests behind it, so memory is not
+            # handed past it); the admitted requests are assembled in waves
+            # of one request per mesh size (a template's forms hold one
+            # request's coefficients), each wave one batched launch pair
+            admitted = []
+            while queue and len(active) + len(admitted) < self.max_slots:
                 req = queue[0]
                 tpl = self.templates[req.n]
                 fp = tpl.footprint()
-                if used + fp > self.budget and (active or group):
-                    break
-                if any(g[1] is tpl for g in group):
+                if used + fp > self.budget and (active or admitted):
                     break
                 queue.popleft()
-                group.append((req, tpl, fp))
+                admitted.append((req, tpl, fp))
                 used += fp
-            if group:
+            while admitted:
+                group, rest, seen = [], [], set()
+                for item in admitted:
+                    if id(item[1]) in seen:
+                        rest.append(item)
+                    else:
+                        seen.add(id(item[1]))
+                        group.append(item)
+                admitted = rest
                 mats = []
                 for req, tpl, _ in group:
                     tpl.load(req)

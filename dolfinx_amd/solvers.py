@@ -15,9 +15,11 @@ from .la import MatrixCSR, Vector
 
 
 def cg(A: MatrixCSR, b: Vector, x: Vector, rtol=1e-8, max_it=10000, jacobi=True,
-       check_every=32):
+       check_every=32, graph=True):
+    """CG (Jacobi-preconditioned by default); graph=True replays each batch
+    of check_every iterations from a captured HIP graph."""
     opts = _lib.CGOpts(max_it=int(max_it), rtol=float(rtol), jacobi=1 if jacobi else 0,
-                       check_every=int(check_every))
+                       check_every=int(check_every), graph=1 if graph else 0)
     res = _lib.CGResult()
     _lib.call("dfx_cg_solve", A.handle, b.handle, x.handle, C.byref(opts), C.byref(res))
     return res
@@ -25,14 +27,14 @@ def cg(A: MatrixCSR, b: Vector, x: Vector, rtol=1e-8, max_it=10000, jacobi=True,
 
 def mixed_refinement(A64: MatrixCSR, A32: MatrixCSR, b: Vector, x: Vector, rtol=1e-8,
                      inner_rtol=1e-4, max_outer=50, max_inner=20000, check_every=32,
-                     restart=True):
+                     restart=True, graph=True):
     """FP32 Jacobi-PCG inside FP64 iterative refinement: each inner solve
     reduces the residual by inner_rtol, then x += d in FP64 and the FP64
     true residual restarts the next inner solve. restart=False keeps the FP32
     search direction across refinements instead (reliable updates)."""
     opts = _lib.MixedOpts(max_outer=int(max_outer), max_inner=int(max_inner), rtol=float(rtol),
                           inner_rtol=float(inner_rtol), check_every=int(check_every),
-                          restart=1 if restart else 0)
+                          restart=1 if restart else 0, graph=1 if graph else 0)
     res = _lib.MixedResult()
     _lib.call("dfx_mixed_solve", A64.handle, A32.handle, b.handle, x.handle, C.byref(opts),
               C.byref(res))

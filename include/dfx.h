@@ -333,8 +333,13 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc,
  * structured box of hexahedra: one thread per row accumulating its 27
  * entries in registers (error if the mesh is not such a box); 10 as 4 with
  * the cell factors formed from the geometry by every incidence (no cell
- * pass; affine P1/Q1); 11 as 1 with eight incidences in flight. The P2
- * elasticity form has a single row kernel and ignores the variant. */
+ * pass; affine P1/Q1); 11 as 1 with eight incidences in flight; 12-14 Q1
+ * hexahedra whose vertices lie in at most 8 cells: one thread per row with
+ * its plan at fixed positions, 64/128/256 rows per workgroup; 15 and 16
+ * entry-parallel: each CSR entry sums its element contributions in cell
+ * order (15 formed from the cell factors, 16 read from the element
+ * matrices formed on the matrix cores). The P2 elasticity form has a
+ * single row kernel and ignores the variant. */
 dfx_status dfx_form_set_variant(dfx_form* a, int32_t variant);
 /* fem::assemble_matrix / fem::assemble_vector with a custom kernel
  * (cpp/demo/custom_kernel/main.cpp:73-118): the form (kernel
@@ -390,6 +395,9 @@ typedef struct
   double rtol;         /* stop when ||r||^2 / ||r0||^2 < rtol^2 */
   int32_t jacobi;      /* 1: Jacobi preconditioner, 0: plain CG */
   int32_t check_every; /* host convergence poll interval (iterations) */
+  int32_t graph;       /* 1: capture check_every iterations once as a HIP graph
+                          and replay it (same kernels, same results; fewer
+                          launch gaps); 0: launch every kernel */
 } dfx_cg_opts;
 
 typedef struct
@@ -409,7 +417,8 @@ typedef struct
 dfx_status dfx_cg_solve(const dfx_csr* A, const dfx_vec* b, dfx_vec* x,
                         const dfx_cg_opts* opts, dfx_cg_result* res);
 
-/* dfx_cg_solve without any device allocation. No counterpart in the
+/* dfx_cg_solve without any device allocation (opts->graph is ignored:
+ * kernels are launched one by one). No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc). The work vectors, solver state and reduction partials live
@@ -430,6 +439,7 @@ typedef struct
   int32_t check_every;
   int32_t restart;     /* 1: restart the FP32 CG at every refinement;
                           0: keep its search direction (reliable updates) */
+  int32_t graph;       /* 1: inner iterations replayed from a captured HIP graph */
 } dfx_mixed_opts;
 
 typedef struct

@@ -44,3 +44,21 @@ def test_host_setup_errors_are_reported():
     with pytest.raises(_lib.DfxError, match="p\\[0\\] < p\\[1\\]"):
         mesh.create_box([[0, 0, 0], [1, -1, 1]], [2, 2, 2])
     assert np.isfinite(1.0)
+
+
+def test_csr_pattern_rows_must_be_sorted():
+    """dfx_csr_create checks each row's columns are strictly increasing (the
+    device kernels binary-search rows); the check runs before any device
+    work, so it is reported without a GPU."""
+    import numpy as np
+    import pytest
+    from dolfinx_amd import la
+
+    rp = np.array([0, 2, 4], np.int64)
+    with pytest.raises(_lib.DfxError, match="not strictly increasing") as e:
+        la.MatrixCSR(rp, np.array([1, 0, 0, 1], np.int32))
+    assert e.value.code == _lib.DFX_ERR_ARG
+    with pytest.raises(_lib.DfxError, match="not strictly increasing"):
+        la.MatrixCSR(rp, np.array([0, 0, 0, 1], np.int32))   # duplicate column
+    with pytest.raises(_lib.DfxError, match="column index out of range"):
+        la.MatrixCSR(rp, np.array([0, 2, 0, 1], np.int32), ncols=2)

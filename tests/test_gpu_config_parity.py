@@ -111,7 +111,7 @@ def test_q1_hex_128_spmv_matches_oracle(q1_128):
     v = np.random.default_rng(3).standard_normal(p.V.num_dofs)
     yr = O.spmv(ref["rp"], ref["cols"], ref["A"], 1, v)
     x = la.Vector.from_array(v)
-    for lanes in (1, 2, 8):
+    for lanes in (1, 2, 8, -1):   # -1: the solvers' sliced-ELL copy
         A.set_lanes(lanes)
         y = la.Vector(p.V.num_dofs)
         A.mult(x, y)  # y += A x
@@ -129,7 +129,7 @@ def p3_24():
     gc.collect()
 
 
-@pytest.mark.parametrize("variant", [3, 7, 8])
+@pytest.mark.parametrize("variant", [3, 7, 8, 15, 16])
 def test_p3_tet_24_matches_oracle(p3_24, variant):
     p, ref = p3_24
     p.a.set_variant(variant)

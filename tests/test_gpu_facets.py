@@ -164,3 +164,51 @@ def test_robin_forms_reject_misuse():
                         np.arange(V.num_dofs, dtype=np.int32))
     with pytest.raises(DfxError, match="Entry not in sparsity"):
         fem.assemble_matrix(aR, [], 1.0, diag)
+
+
+def _padded(rp, cols, nd, width):
+    """The pattern with every row's columns united with 0..width-1 (rows
+    wider than the element couplings: the slot search paths for rows of
+    more than 64 and more than 256 entries)."""
+    rows = []
+    for r in range(nd):
+        rows.append(np.union1d(cols[rp[r]:rp[r + 1]], np.arange(min(width, nd))))
+    prp = np.zeros(nd + 1, np.int64)
+    prp[1:] = np.cumsum([len(x) for x in rows])
+    return prp, np.concatenate(rows).astype(np.int32)
+
+
+@pytest.mark.parametrize("width", [100, 300])
+def test_facet_forms_twice_into_a_superset_pattern(width):
+    """Robin and gradient-jump matrices assembled twice into a pattern whose
+    rows hold more than 64 (width 100) or more than 256 (width 300) entries:
+    the first assembly checks and caches the slots, the second reuses them;
+    both against the oracle."""
+    from test_gpu_interior import setup as interior_setup
+
+    n = (8, 8, 8)
+    om, pm, V, dm, nd, q, rp0, cols0 = interior_setup(n)
+    rp, cols = _padded(rp0, cols0, nd, width)
+    assert np.diff(rp).max() > (256 if width == 300 else 64)
+    bn = om.boundary_nodes(1, dm, marker=1, axis=0, value=0.0)
+    mark = np.zeros(nd, np.int8)
+    mark[bn] = 1
+    bc = fem.dirichletbc(V, bn)
+    beta_in = np.random.default_rng(8).uniform(0.5, 2.0, om.num_cells)
+    co = beta_in[om.original_cell_index].reshape(-1, 1)
+    pairs = fem.facet_domain(pm)
+    aR = fem.form("p1_robin", V, fem.pack_cellwise(pm, beta_in), facets=pairs)
+    aJ = fem.form("p1_gradient_jump", V, None, [0.6], facets=q)
+    for form, kind in ((aR, "robin"), (aJ, "jump")):
+        A = la.MatrixCSR(rp, cols)
+        ref = np.zeros(len(cols))
+        for _ in range(2):
+            fem.assemble_matrix(form, [bc], 1.0, A)
+            if kind == "robin":
+                O.assemble_matrix_facets(O.K_P1TET_ROBIN_A, om.x, om.cells, dm, co, pairs, mark,
+                                         mark, rp, cols, ref)
+            else:
+                O.assemble_matrix_interior(O.K_P1TET_GRADJUMP_A, om.x, om.cells, dm, q, mark,
+                                           mark, rp, cols, [0.6], ref)
+            O.set_diagonal(ref, bn, 1, 1.0, rp, cols)
+            assert rel_err(A.data, ref) <= 1e-13, kind

@@ -79,3 +79,44 @@ def test_service_continuous_batching_small():
     O.bc_set(b, bn, np.zeros(nd))
     xr, _ = O.cg(rp, cols, A, 1, b, rtol=1e-12)
     assert rel_err(r.x, xr) <= 1e-7
+
+
+def test_service_hbm_budget_binds():
+    """A budget that holds about one request of the largest size: requests
+    wait for memory (their latency includes the wait), resident bytes never
+    exceed the budget, and results are bitwise those of the request alone."""
+    sizes = range(6, 11)
+    probe = serve.SolveService(hbm_budget_bytes=1 << 30, max_slots=8, sizes=sizes)
+    biggest = max(t.footprint() for t in probe.templates.values())
+    smallest = min(t.footprint() for t in probe.templates.values())
+    del probe
+    budget = biggest + smallest // 2
+    svc = serve.SolveService(hbm_budget_bytes=budget, max_slots=8, rtol=1e-8, iters_per_step=8,
+                             sizes=sizes)
+    reqs = [serve.make_request(i, seed=31, n_range=(6, 10)) for i in range(20)]
+    keep = [2, 9, 17]
+    stats = svc.run(reqs, rate=5000.0, keep_solutions=keep)
+    assert stats.completed == len(reqs) and all(r.converged for r in reqs)
+    assert stats.max_bytes <= budget
+    waits = [r.admitted - r.arrival for r in reqs]
+    assert all(r.arrival <= r.admitted <= r.completed for r in reqs)
+    assert max(waits) > 1e-3            # some requests waited for memory
+    lat = np.array(stats.latencies)
+    assert np.all(lat >= 0) and lat.max() >= max(waits)
+    for r in reqs:
+        if r.rid in keep:
+            assert np.array_equal(svc.solve_alone(r), r.x)
+
+
+def test_service_same_size_requests_share_the_batch():
+    """Requests of one mesh size arriving together are all admitted (in
+    waves of one batched assembly each) instead of one per step."""
+    svc = serve.SolveService(hbm_budget_bytes=256 << 20, max_slots=8, rtol=1e-8,
+                             iters_per_step=8, sizes=[8])
+    reqs = [serve.make_request(i, seed=41, n_range=(8, 8)) for i in range(8)]
+    stats = svc.run(reqs, rate=1e6, keep_solutions=[0, 7])
+    assert stats.completed == 8 and all(r.converged for r in reqs)
+    assert stats.max_resident >= 4
+    for r in reqs:
+        if r.rid in (0, 7):
+            assert np.array_equal(svc.solve_alone(r), r.x)

@@ -98,3 +98,65 @@ def test_poisson_tet_32_end_to_end():
     assert rel_err(x.array, xr) <= 1e-8
     # maximum of the solution of -lap u = 1 on the unit cube (P1, h = 1/32)
     assert 0.0555 < x.array.max() < 0.0565
+
+
+@pytest.mark.parametrize("cell,n,jacobi", [(O.TET, (12, 10, 9), True), (O.HEX, (16, 12, 10), False)])
+def test_cg_graph_replay_equals_eager(cell, n, jacobi):
+    """Batches of iterations replayed from a captured HIP graph run the same
+    kernels in the same order as the eager launches: the iterates are
+    bitwise equal, over several replays per solve (host polls and copies in
+    between) and over repeated solves."""
+    p, A, b = system(cell, n, kappa="exp_normal", c=0.3, f="normal")
+    xe = la.Vector(p.V.num_dofs)
+    re = solvers.cg(A, b, xe, rtol=1e-11, max_it=20000, jacobi=jacobi, check_every=8, graph=False)
+    for _ in range(2):
+        xg = la.Vector(p.V.num_dofs)
+        rg = solvers.cg(A, b, xg, rtol=1e-11, max_it=20000, jacobi=jacobi, check_every=8,
+                        graph=True)
+        assert rg.iterations == re.iterations and rg.iterations > 3 * 8
+        assert np.array_equal(xg.array, xe.array)
+        assert rg.true_rel_residual == re.true_rel_residual
+    # max_it inside a replayed batch stops at max_it exactly
+    xm = la.Vector(p.V.num_dofs)
+    with pytest.raises(DfxError, match="exceeded max iterations"):
+        solvers.cg(A, b, xm, rtol=1e-14, max_it=13, check_every=8, graph=True)
+
+
+@pytest.mark.parametrize("restart", [True, False])
+def test_mixed_graph_replay_equals_eager(restart):
+    kw = dict(kappa="exp_normal", c=0.0, f="one")
+    n = (14, 12, 10)
+    p64 = problems.scalar_problem(n, O.HEX, dtype=np.float64, seed=5, **kw)
+    p32 = problems.scalar_problem(n, O.HEX, dtype=np.float32, seed=5, msh=p64.mesh, V=p64.V, **kw)
+    A64, b = problems.assemble_system(p64)
+    A32, _ = problems.assemble_system(p32)
+    out = []
+    for graph in (False, True):
+        x = la.Vector(p64.V.num_dofs)
+        r = solvers.mixed_refinement(A64, A32, b, x, rtol=1e-10, inner_rtol=1e-4, check_every=8,
+                                     restart=restart, graph=graph)
+        assert r.converged
+        out.append((x.array, r.outer, r.inner_total))
+    assert np.array_equal(out[0][0], out[1][0]) and out[0][1:] == out[1][1:]
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-14), (np.float32, 1e-6)])
+def test_sliced_ell_snapshot_spmv(dtype, tol, rng):
+    """lanes = -1: the SpMV through the solvers' sliced-ELL copy, taken when
+    the mode is set, against the oracle."""
+    p, A, b = system(O.HEX, (20, 18, 16), dtype, kappa="exp_normal")
+    A.set_lanes(-1)
+    x = rng.standard_normal(p.V.num_dofs)
+    xv = la.Vector.from_array(x.astype(dtype))
+    yv = la.Vector(p.V.num_dofs, dtype)
+    A.mult(xv, yv)
+    ref = O.spmv(A.indptr, A.indices, A.data.astype(np.float64), 1,
+                 x.astype(dtype).astype(np.float64))
+    assert rel_err(yv.array, ref) <= tol
+    # the snapshot is what the solvers use: CG gives the same iterates
+    xa = la.Vector(p.V.num_dofs, dtype)
+    solvers.cg(A, b, xa, rtol=1e-5 if dtype == np.float32 else 1e-10)
+    A.set_lanes(1)
+    xb = la.Vector(p.V.num_dofs, dtype)
+    solvers.cg(A, b, xb, rtol=1e-5 if dtype == np.float32 else 1e-10)
+    assert np.array_equal(xa.array, xb.array)
