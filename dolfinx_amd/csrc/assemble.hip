@@ -708,6 +708,16 @@ __device__ inline T mask_bits(std::uint32_t m)
     return __uint_as_float(m);
 }
 
+/// The 32-bit mask carried by mask_bits.
+template <typename T>
+__device__ inline std::uint32_t bits_mask(T v)
+{
+  if constexpr (sizeof(T) == 8)
+    return static_cast<std::uint32_t>(__double_as_longlong(v));
+  else
+    return __float_as_uint(v);
+}
+
 /// The affine factors of one cell from its Jacobian vertices q0..q3
 /// (x1-x0, x2-x0 and x3-x0 for a tetrahedron, x4-x0 for a hexahedron with
 /// lexicographic vertices): G = kappa |det J| J^-1 J^-T (6 values, through
@@ -1337,14 +1347,18 @@ __global__ void __launch_bounds__(kBlock)
     const int i = cik[u] & 7;
     T gs[3];
     q1_signed<T>(g[u], i, gs);
-    const std::uint32_t w0 = static_cast<std::uint32_t>(sw[u / 2][2 * (u % 2)]);
-    const std::uint32_t w1 = static_cast<std::uint32_t>(sw[u / 2][2 * (u % 2) + 1]);
+    const std::uint64_t w = static_cast<std::uint64_t>(static_cast<std::uint32_t>(sw[u / 2][2 * (u % 2)]))
+                            | (static_cast<std::uint64_t>(static_cast<std::uint32_t>(sw[u / 2][2 * (u % 2) + 1])) << 32);
+    // column j = i ^ e: the 1D factors of q1_entry depend on e only, so
+    // with e fixed at compile time an entry is seven products; the column
+    // order differs from j order only in which slot is added first
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
+    for (int e = 0; e < 8; ++e)
     {
-      const std::uint32_t sl = ((j < 4 ? w0 : w1) >> (8 * (j % 4))) & 0xffu;
+      const int j = i ^ e;
+      const std::uint32_t sl = static_cast<std::uint32_t>(w >> (8 * j)) & 0xffu;
       if (!((m[u] >> j) & 1u))
-        a[sl] += q1_entry<T>(g[u], gs, i, j);
+        a[sl] += q1_entry<T>(g[u], gs, 0, e);
     }
   }
   __syncthreads();
@@ -2581,6 +2595,205 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// ---------------------------------------------------------------------
+// Box tiles (variant 17): Q1 on an axis-aligned box of hexahedra, without a
+// cell pass or cell records in HBM. A workgroup owns a tile of 8 x 8 x 4
+// grid vertices (its matrix rows): it stages the coordinates and Dirichlet
+// markers of the tile's vertices and their halo (10 x 10 x 6) in LDS,
+// forms the factors of the 9 x 9 x 5 cells around the tile there (the
+// affine factors of every cell, from LDS-staged coordinates), and each
+// thread then sums its row's 27 entries in registers over the row
+// vertex's 8 corner cells (corner order) and stores them at the row's slot
+// positions. The grid maps are checked against the dofmap when the plan is
+// built; any other mesh makes the variant unavailable.
+// ---------------------------------------------------------------------
+
+constexpr int kBoxTX = 8, kBoxTY = 8, kBoxTZ = 4;   // tile of row vertices
+constexpr int kBoxHX = kBoxTX + 2, kBoxHY = kBoxTY + 2, kBoxHZ = kBoxTZ + 2; // vertex halo
+constexpr int kBoxCX = kBoxTX + 1, kBoxCY = kBoxTY + 1, kBoxCZ = kBoxTZ + 1; // cells
+constexpr int kBoxNV = kBoxHX * kBoxHY * kBoxHZ, kBoxNC = kBoxCX * kBoxCY * kBoxCZ;
+static_assert(kBoxTX * kBoxTY * kBoxTZ == kBlock, "one thread per tile vertex");
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    box_q1_k(std::int32_t nx, std::int32_t ny, std::int32_t nz,
+             const std::int32_t* __restrict__ box_dof, const std::int32_t* __restrict__ box_cell,
+             const T* __restrict__ x, const T* __restrict__ coeffs, int cstride, T creac,
+             const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
+             const std::uint8_t* __restrict__ slotmap, T* __restrict__ vals, int accumulate)
+{
+  __shared__ T Xs[kBoxNV][3];
+  __shared__ std::int8_t Bs[kBoxNV];
+  __shared__ T Rs[kBoxNC][8];
+  const int ntx = (nx + kBoxTX) / kBoxTX, nty = (ny + kBoxTY) / kBoxTY;
+  const int tile = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+  const int vx0 = tx * kBoxTX, vy0 = ty * kBoxTY, vz0 = tz * kBoxTZ;
+  const std::int64_t sx = nx + 1, sxy = std::int64_t(nx + 1) * (ny + 1);
+  // halo vertices: coordinates and Dirichlet markers
+  for (int li = threadIdx.x; li < kBoxNV; li += kBlock)
+  {
+    const int gx = vx0 - 1 + li % kBoxHX, gy = vy0 - 1 + (li / kBoxHX) % kBoxHY,
+              gz = vz0 - 1 + li / (kBoxHX * kBoxHY);
+    std::int8_t mk = 0;
+    if (gx >= 0 && gx <= nx && gy >= 0 && gy <= ny && gz >= 0 && gz <= nz)
+    {
+      const std::int64_t d = box_dof[gx + sx * gy + sxy * gz];
+      Xs[li][0] = x[3 * d];
+      Xs[li][1] = x[3 * d + 1];
+      Xs[li][2] = x[3 * d + 2];
+      mk = bc ? bc[d] : std::int8_t(0);
+    }
+    Bs[li] = mk;
+  }
+  __syncthreads();
+  // cells around the tile: affine factors and Dirichlet column masks
+  for (int cl = threadIdx.x; cl < kBoxNC; cl += kBlock)
+  {
+    const int hx = cl % kBoxCX, hy = (cl / kBoxCX) % kBoxCY, hz = cl / (kBoxCX * kBoxCY);
+    const int cx = vx0 - 1 + hx, cy = vy0 - 1 + hy, cz = vz0 - 1 + hz;
+    T g[7] = {0, 0, 0, 0, 0, 0, 0};
+    std::uint32_t m = 0xffu; // absent cell: every column masked
+    if (cx >= 0 && cx < nx && cy >= 0 && cy < ny && cz >= 0 && cz < nz)
+    {
+      const std::int64_t c = box_cell[cx + std::int64_t(nx) * (cy + std::int64_t(ny) * cz)];
+      const int v0 = hx + kBoxHX * (hy + kBoxHY * hz);
+      affine_factors<T>(Xs[v0], Xs[v0 + 1], Xs[v0 + kBoxHX], Xs[v0 + kBoxHX * kBoxHY],
+                        coeffs[c * cstride], creac, g);
+      m = 0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (Bs[v0 + (j & 1) + kBoxHX * ((j >> 1) & 1) + kBoxHX * kBoxHY * ((j >> 2) & 1)])
+          m |= 1u << j;
+    }
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+      Rs[cl][q] = g[q];
+    Rs[cl][7] = mask_bits<T>(m);
+  }
+  __syncthreads();
+  const int lx = threadIdx.x % kBoxTX, ly = (threadIdx.x / kBoxTX) % kBoxTY,
+            lz = threadIdx.x / (kBoxTX * kBoxTY);
+  const int vx = vx0 + lx, vy = vy0 + ly, vz = vz0 + lz;
+  if (vx > nx || vy > ny || vz > nz)
+    return;
+  const std::int32_t r = box_dof[vx + sx * vy + sxy * vz];
+  const bool rowbc = Bs[(lx + 1) + kBoxHX * ((ly + 1) + kBoxHY * (lz + 1))] != 0;
+  T acc[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q)
+    acc[q] = T(0);
+  if (!rowbc)
+  {
+#pragma unroll
+    for (int I = 0; I < 8; ++I)
+    {
+      const int cl = (lx + 1 - (I & 1)) + kBoxCX * ((ly + 1 - ((I >> 1) & 1))
+                                                    + kBoxCY * (lz + 1 - ((I >> 2) & 1)));
+      T g[7];
+#pragma unroll
+      for (int q = 0; q < 7; ++q)
+        g[q] = Rs[cl][q];
+      const std::uint32_t m = bits_mask<T>(Rs[cl][7]);
+      T gs[3];
+      q1_signed<T>(g, I, gs);
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (!((m >> j) & 1u))
+          acc[q1_slot(I, j)] += q1_entry<T>(g, gs, I, j);
+    }
+  }
+  const std::uint32_t* sm = reinterpret_cast<const std::uint32_t*>(slotmap + 28 * std::int64_t(r));
+  std::uint32_t w[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k)
+    w[k] = sm[k];
+  T* out = vals + rp[r];
+#pragma unroll
+  for (int q = 0; q < 27; ++q)
+  {
+    const std::uint32_t s = (w[q / 4] >> (8 * (q % 4))) & 0xffu;
+    if (s != 0xffu)
+      out[s] = accumulate ? out[s] + acc[q] : acc[q];
+  }
+}
+
+/// Variant 18 (Q1 on a structured box of hexahedra): one thread per row as
+/// variant 9 -- the row vertex's 8 corner cells, its 27 entries summed in
+/// registers with compile-time reference values, corner order -- but the
+/// rows of a workgroup are consecutive and their entries go through an LDS
+/// copy of the workgroup's CSR slice, so the slice is stored with coalesced
+/// non-temporal writes. Every entry of a row is one of its 27 slots
+/// (checked when the plan is built), so the slice needs no zero fill.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    row_struct_lds_k(std::int32_t nrows, const std::int32_t* __restrict__ corner,
+                     const std::uint8_t* __restrict__ slotmap, const T* __restrict__ cd,
+                     const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
+                     T* __restrict__ vals, int accumulate)
+{
+  extern __shared__ unsigned char smem_raw[];
+  T* slice = reinterpret_cast<T*>(smem_raw);
+  const int R = blockDim.x;
+  const std::int32_t r0 = xcd_contiguous(blockIdx.x, gridDim.x) * R;
+  const std::int32_t r1 = min(r0 + R, nrows);
+  const std::int32_t base = rp[r0];
+  const std::int32_t len = rp[r1] - base;
+  const std::int32_t r = r0 + threadIdx.x;
+  if (r < r1)
+  {
+    const dfx_i32x4* cp = reinterpret_cast<const dfx_i32x4*>(corner + 8 * std::int64_t(r));
+    const dfx_i32x4 ca = __builtin_nontemporal_load(cp);
+    const dfx_i32x4 cb = __builtin_nontemporal_load(cp + 1);
+    const std::int32_t cc[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+    const std::uint32_t* smp = reinterpret_cast<const std::uint32_t*>(slotmap + 28 * std::int64_t(r));
+    std::uint32_t w[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+      w[k] = __builtin_nontemporal_load(smp + k);
+    const bool rowbc = bc && bc[r];
+    const std::int32_t roff = rp[r] - base;
+    T acc[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q)
+      acc[q] = T(0);
+    T g[8][7];
+    std::uint32_t m[8];
+#pragma unroll
+    for (int I = 0; I < 8; ++I)
+    {
+      load_factors<T>(cd, cc[I] < 0 ? 0 : cc[I], g[I], m[I]);
+      if (cc[I] < 0 || rowbc)
+        m[I] = 0xffu; // absent corner (or a Dirichlet row): nothing added
+    }
+#pragma unroll
+    for (int I = 0; I < 8; ++I)
+    {
+      T gs[3];
+      q1_signed<T>(g[I], I, gs);
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (!((m[I] >> j) & 1u))
+          acc[q1_slot(I, j)] += q1_entry<T>(g[I], gs, I, j);
+    }
+    T* a = slice + roff;
+#pragma unroll
+    for (int q = 0; q < 27; ++q)
+    {
+      const std::uint32_t sl = (w[q / 4] >> (8 * (q % 4))) & 0xffu;
+      if (sl != 0xffu)
+        a[sl] = acc[q];
+    }
+  }
+  __syncthreads();
+  if (accumulate)
+    for (int t = threadIdx.x; t < len; t += R)
+      vals[base + t] += slice[t];
+  else
+    for (int t = threadIdx.x; t < len; t += R)
+      __builtin_nontemporal_store(slice[t], vals + base + t);
+}
+
 /// Element vectors of the source forms, one thread per cell:
 /// P1 tetrahedra b_i = |K|/20 (f_i + sum f); affine Q1 hexahedra
 /// b_i = |det J| sum_j M_ij f_j; other hexahedra 2x2x2 Gauss.
@@ -3456,8 +3669,9 @@ int incidence_lanes(const dfx_form& f)
 /// 7, 8 incidence-parallel gathers; 9 structured Q1; 10 fused geometry;
 /// 11 as 1 with eight incidences in flight; 12..14 fixed-width Q1 rows;
 /// 15, 16 entry-parallel (contributions from the factors / from element
-/// matrices formed on the matrix cores).
-constexpr int kRowVariants = 17;
+/// matrices formed on the matrix cores); 17 Q1 box tiles; 18 structured Q1
+/// rows in registers stored through an LDS slice.
+constexpr int kRowVariants = 19;
 
 template <typename T, typename O, int K, int L>
 void launch_lanes(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
@@ -3616,6 +3830,97 @@ bool ensure_struct(dfx_form& f, const dfx_csr& A)
     return false;
   }
   f.struct_ok = 1;
+  return true;
+}
+
+/// Box plan of a Q1 form (built once per pattern): the structured row plan
+/// (slot positions of the 27 neighbour offsets, ensure_struct) and the grid
+/// maps, found from the geometry (grid index = (x - min) / h per axis, h
+/// the edge lengths of cell 0) and checked against the dofmap: every grid
+/// point holds exactly one vertex, every grid cell exactly one cell, and
+/// each cell's local vertex j sits at the cell's grid point + the bits of j.
+template <typename T>
+bool ensure_box(dfx_form& f, const dfx_csr& A)
+{
+  if (f.box_pattern == A.pattern_id && f.box_ok >= 0)
+    return f.box_ok == 1;
+  dfree(f.box_dof);
+  dfree(f.box_cell);
+  f.box_dof = f.box_cell = nullptr;
+  f.box_pattern = A.pattern_id;
+  f.box_ok = 0;
+  if (f.dofmap != f.x_dofmap || f.d.nodes_per_cell != 8 || !ensure_struct(f, A))
+    return false;
+  const std::int64_t nc = f.d.num_cells, nv = f.d.num_nodes;
+  if (nc == 0 || nv != A.nrows)
+    return false;
+  std::vector<T> xt(3 * nv);
+  std::vector<std::int32_t> dm(8 * nc);
+  d2h(xt.data(), f.x, sizeof(T) * xt.size());
+  d2h(dm.data(), f.dofmap, sizeof(std::int32_t) * dm.size());
+  double lo[3], h[3];
+  for (int a = 0; a < 3; ++a)
+  {
+    lo[a] = double(xt[a]);
+    for (std::int64_t v = 1; v < nv; ++v)
+      lo[a] = std::min(lo[a], double(xt[3 * v + a]));
+  }
+  // edge lengths of cell 0 along its local x, y, z (vertices 1, 2, 4)
+  const int axis_vertex[3] = {1, 2, 4};
+  for (int a = 0; a < 3; ++a)
+  {
+    const std::int64_t v0 = dm[0], va = dm[axis_vertex[a]];
+    h[a] = double(xt[3 * va + a]) - double(xt[3 * v0 + a]);
+    if (!(h[a] > 0))
+      return false;
+  }
+  std::vector<std::int32_t> gi(3 * nv);
+  std::int64_t n[3] = {0, 0, 0};
+  for (std::int64_t v = 0; v < nv; ++v)
+    for (int a = 0; a < 3; ++a)
+    {
+      const double t = (double(xt[3 * v + a]) - lo[a]) / h[a];
+      const double k = std::nearbyint(t);
+      if (std::abs(t - k) > 1e-3 || k < 0 || k > 1e7)
+        return false;
+      gi[3 * v + a] = static_cast<std::int32_t>(k);
+      n[a] = std::max<std::int64_t>(n[a], gi[3 * v + a]);
+    }
+  if ((n[0] + 1) * (n[1] + 1) * (n[2] + 1) != nv || n[0] * n[1] * n[2] != nc)
+    return false;
+  std::vector<std::int32_t> bdof(nv, -1), bcell(nc, -1);
+  for (std::int64_t v = 0; v < nv; ++v)
+  {
+    const std::int64_t g = gi[3 * v] + (n[0] + 1) * (gi[3 * v + 1] + (n[1] + 1) * gi[3 * v + 2]);
+    if (bdof[g] != -1)
+      return false;
+    bdof[g] = static_cast<std::int32_t>(v);
+  }
+  for (std::int64_t c = 0; c < nc; ++c)
+  {
+    const std::int32_t* cv = dm.data() + 8 * c;
+    const std::int32_t* g0 = gi.data() + 3 * std::int64_t(cv[0]);
+    if (g0[0] >= n[0] || g0[1] >= n[1] || g0[2] >= n[2])
+      return false;
+    for (int j = 0; j < 8; ++j)
+    {
+      const std::int32_t* gj = gi.data() + 3 * std::int64_t(cv[j]);
+      if (gj[0] != g0[0] + (j & 1) || gj[1] != g0[1] + ((j >> 1) & 1)
+          || gj[2] != g0[2] + ((j >> 2) & 1))
+        return false;
+    }
+    const std::int64_t gc = g0[0] + n[0] * (g0[1] + n[1] * g0[2]);
+    if (bcell[gc] != -1)
+      return false;
+    bcell[gc] = static_cast<std::int32_t>(c);
+  }
+  f.box_dof = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * nv));
+  f.box_cell = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * nc));
+  h2d(f.box_dof, bdof.data(), sizeof(std::int32_t) * nv);
+  h2d(f.box_cell, bcell.data(), sizeof(std::int32_t) * nc);
+  for (int a = 0; a < 3; ++a)
+    f.box_n[a] = static_cast<std::int32_t>(n[a]);
+  f.box_ok = 1;
   return true;
 }
 
@@ -3892,6 +4197,46 @@ template <typename T, typename O, int K>
 void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
 {
   const int variant = f.row_variant;
+  if (variant == 17)
+  {
+    if constexpr (K == 8)
+    {
+      require(ensure_box<T>(f, A), "variant not available for this mesh");
+      const std::int64_t ntx = (f.box_n[0] + kBoxTX) / kBoxTX,
+                         nty = (f.box_n[1] + kBoxTY) / kBoxTY,
+                         ntz = (f.box_n[2] + kBoxTZ) / kBoxTZ;
+      const std::int64_t nb = ntx * nty * ntz;
+      require(nb < INT32_MAX, "box too large");
+      box_q1_k<T><<<static_cast<int>(nb), kBlock, 0, stream()>>>(
+          f.box_n[0], f.box_n[1], f.box_n[2], f.box_dof, f.box_cell,
+          static_cast<const T*>(f.x), static_cast<const T*>(f.coeffs), f.d.cstride,
+          static_cast<T>(f.constants[0]), bc ? bc->marker : nullptr, A.row_ptr, f.slotmap,
+          static_cast<T*>(A.values), A.zeroed ? 0 : 1);
+      check_launch("row_assemble");
+      return;
+    }
+    else
+      throw Error(DFX_ERR_ARG, "variant not available for this element");
+  }
+  if (variant == 18)
+  {
+    if constexpr (K == 8)
+    {
+      require(ensure_struct(f, A), "variant not available for this mesh");
+      launch_cell_factors<T>(f, bc);
+      const int R = 128;
+      const std::size_t lds = std::size_t(R) * A.max_row * sizeof(T);
+      const int nb = (A.nrows + R - 1) / R;
+      if (nb > 0)
+        row_struct_lds_k<T><<<nb, R, lds, stream()>>>(
+            A.nrows, f.corner_cell, f.slotmap, static_cast<const T*>(f.cell_data),
+            bc ? bc->marker : nullptr, A.row_ptr, static_cast<T*>(A.values), A.zeroed ? 0 : 1);
+      check_launch("row_assemble");
+      return;
+    }
+    else
+      throw Error(DFX_ERR_ARG, "variant not available for this element");
+  }
   if (variant == 15 || variant == 16)
   {
     // the entry kernel reads the Dirichlet marker at column indices
@@ -5019,6 +5364,8 @@ dfx_status dfx_form_destroy(dfx_form* f)
     dfree(f->islot8);
     dfree(f->contrib);
     dfree(f->contrib_off);
+    dfree(f->box_dof);
+    dfree(f->box_cell);
     dfree(f->elem_tensors);
     dfree(f->pack_map);
     dfree(f->pack_stage);
@@ -5355,7 +5702,15 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t*
           f.islot8 = nullptr;
           f.fixed_pattern = 0;
         }
-        if (best != 9)
+        if (best != 17)
+        {
+          dfree(f.box_dof);
+          dfree(f.box_cell);
+          f.box_dof = f.box_cell = nullptr;
+          f.box_pattern = 0;
+          f.box_ok = -1;
+        }
+        if (best != 9 && best != 17 && best != 18)
         {
           dfree(f.corner_cell);
           dfree(f.slotmap);

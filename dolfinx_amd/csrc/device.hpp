@@ -281,6 +281,14 @@ struct dfx_form
   std::int32_t* contrib_off = nullptr;
   std::uint64_t contrib_pattern = 0;
   int entry_lanes = 32;
+  // Box tiles (Q1 on an axis-aligned box of hexahedra): the vertex (= dof)
+  // at each grid point and the cell at each grid cell, grid sizes in cells;
+  // box_ok: -1 not checked for box_pattern, 0 not a box, 1 usable.
+  std::int32_t* box_dof = nullptr;
+  std::int32_t* box_cell = nullptr;
+  std::int32_t box_n[3] = {0, 0, 0};
+  std::uint64_t box_pattern = 0;
+  int box_ok = -1;
   // Forms whose cell tensors come from a host element kernel with the
   // reference's kernel ABI (kernelptr_t): the kernel and its custom data,
   // host copies of what it reads per cell (geometry, packed coefficients,

@@ -15,9 +15,10 @@ from .la import MatrixCSR, Vector
 
 
 def cg(A: MatrixCSR, b: Vector, x: Vector, rtol=1e-8, max_it=10000, jacobi=True,
-       check_every=32, graph=True):
-    """CG (Jacobi-preconditioned by default); graph=True replays each batch
-    of check_every iterations from a captured HIP graph."""
+       check_every=32, graph=False):
+    """CG (Jacobi-preconditioned by default). graph=True replays each batch
+    of check_every iterations from a captured HIP graph (same kernels and
+    results; the capture costs about a millisecond per solve)."""
     opts = _lib.CGOpts(max_it=int(max_it), rtol=float(rtol), jacobi=1 if jacobi else 0,
                        check_every=int(check_every), graph=1 if graph else 0)
     res = _lib.CGResult()
@@ -27,7 +28,7 @@ def cg(A: MatrixCSR, b: Vector, x: Vector, rtol=1e-8, max_it=10000, jacobi=True,
 
 def mixed_refinement(A64: MatrixCSR, A32: MatrixCSR, b: Vector, x: Vector, rtol=1e-8,
                      inner_rtol=1e-4, max_outer=50, max_inner=20000, check_every=32,
-                     restart=True, graph=True):
+                     restart=True, graph=False):
     """FP32 Jacobi-PCG inside FP64 iterative refinement: each inner solve
     reduces the residual by inner_rtol, then x += d in FP64 and the FP64
     true residual restarts the next inner solve. restart=False keeps the FP32

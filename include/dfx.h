@@ -338,8 +338,12 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc,
  * its plan at fixed positions, 64/128/256 rows per workgroup; 15 and 16
  * entry-parallel: each CSR entry sums its element contributions in cell
  * order (15 formed from the cell factors, 16 read from the element
- * matrices formed on the matrix cores). The P2 elasticity form has a
- * single row kernel and ignores the variant. */
+ * matrices formed on the matrix cores); 17 Q1 on an axis-aligned box of
+ * hexahedra: tiles of 8 x 8 x 4 row vertices form their cells' factors in
+ * LDS from the staged coordinates (no cell pass) and sum each row's 27
+ * entries in registers (error if the mesh is not such a box); 18 as 9 with
+ * the rows of a workgroup stored through an LDS slice. The P2
+ * elasticity form has a single row kernel and ignores the variant. */
 dfx_status dfx_form_set_variant(dfx_form* a, int32_t variant);
 /* fem::assemble_matrix / fem::assemble_vector with a custom kernel
  * (cpp/demo/custom_kernel/main.cpp:73-118): the form (kernel
@@ -396,8 +400,8 @@ typedef struct
   int32_t jacobi;      /* 1: Jacobi preconditioner, 0: plain CG */
   int32_t check_every; /* host convergence poll interval (iterations) */
   int32_t graph;       /* 1: capture check_every iterations once as a HIP graph
-                          and replay it (same kernels, same results; fewer
-                          launch gaps); 0: launch every kernel */
+                          and replay it (same kernels, same results);
+                          0: launch every kernel */
 } dfx_cg_opts;
 
 typedef struct

@@ -78,7 +78,7 @@ def q1_128():
     gc.collect()
 
 
-@pytest.mark.parametrize("variant", [-1, 1, 4, 9, 10, 12, 13])
+@pytest.mark.parametrize("variant", [-1, 1, 4, 9, 10, 12, 13, 17, 18])
 def test_q1_hex_128_fp64_matches_oracle(q1_128, variant):
     """variant -1: the autotuner's choice; 1 reference tensors in LDS, 4
     closed-form Q1 entries, 9 structured-box rows, 10 factors formed per
