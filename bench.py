@@ -212,9 +212,6 @@ def other_configs(steps):
         scalar_nnz = A.nnz * A.bs * A.bs
         x = la.Vector(p.V.num_dofs)
         r = solvers.cg(A, b, x, rtol=rtol, max_it=50000, jacobi=True, check_every=64)
-        xg = la.Vector(p.V.num_dofs)
-        rg_ = solvers.cg(A, b, xg, rtol=rtol, max_it=50000, jacobi=True, check_every=64,
-                         graph=True)
         rec = {
             "workload": desc,
             "cells": int(p.mesh.topology.cells.shape[0]),
@@ -226,7 +223,6 @@ def other_configs(steps):
             "cg_rtol": rtol,
             "cg_s": round(r.seconds, 4),
             "cg_us_per_iteration": round(r.seconds / max(r.iterations, 1) * 1e6, 1),
-            "cg_graph_s": round(rg_.seconds, 4),
             "cg_iterations": r.iterations,
             "cg_rel_residual": r.true_rel_residual,
             "first_assembly_s": round(setup, 3),

@@ -708,16 +708,6 @@ __device__ inline T mask_bits(std::uint32_t m)
     return __uint_as_float(m);
 }
 
-/// The 32-bit mask carried by mask_bits.
-template <typename T>
-__device__ inline std::uint32_t bits_mask(T v)
-{
-  if constexpr (sizeof(T) == 8)
-    return static_cast<std::uint32_t>(__double_as_longlong(v));
-  else
-    return __float_as_uint(v);
-}
-
 /// The affine factors of one cell from its Jacobian vertices q0..q3
 /// (x1-x0, x2-x0 and x3-x0 for a tetrahedron, x4-x0 for a hexahedron with
 /// lexicographic vertices): G = kappa |det J| J^-1 J^-T (6 values, through
@@ -1163,128 +1153,6 @@ __global__ void __launch_bounds__(kBlock)
   }
   if (s != 0.0)
     b[t] = static_cast<T>(double(b[t]) - alpha * s);
-}
-
-// ---------------------------------------------------------------------
-// Entry-parallel assembly. Every CSR entry (r, col) owns the list of the
-// element contributions that add into it -- (cell c, local row i, local
-// column j) with dof(c, i) = r and dof(c, j) = col, coded (c k + i) k + j
-// and kept in cell order -- so each entry is summed by one lane in the
-// order of the reference's cell loop, and the entries of a row are written
-// by consecutive lanes. A contribution is formed from the cell factors and
-// the reference tensors (7 products), or read from the element matrices of
-// all cells (E, whose flat index is the code itself).
-// ---------------------------------------------------------------------
-
-/// Contributions per entry (the plan's counting pass).
-template <typename O>
-__global__ void __launch_bounds__(kBlock)
-    contrib_count_k(std::int64_t ninc, int k, const std::int32_t* __restrict__ inc,
-                    const O* __restrict__ islots, const std::int32_t* __restrict__ dm,
-                    const std::int32_t* __restrict__ rp, std::int32_t* __restrict__ cnt)
-{
-  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
-  if (t >= ninc * k)
-    return;
-  const std::int32_t cik = inc[t / k];
-  const std::int64_t e = std::int64_t(rp[dm[cik]]) + islots[t];
-  atomicAdd(cnt + e, 1);
-}
-
-/// Contribution codes into their entries' lists (any order; sorted next).
-template <typename O>
-__global__ void __launch_bounds__(kBlock)
-    contrib_fill_k(std::int64_t ninc, int k, const std::int32_t* __restrict__ inc,
-                   const O* __restrict__ islots, const std::int32_t* __restrict__ dm,
-                   const std::int32_t* __restrict__ rp, std::int32_t* __restrict__ cursor,
-                   std::int32_t* __restrict__ code)
-{
-  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
-  if (t >= ninc * k)
-    return;
-  const std::int32_t cik = inc[t / k];
-  const int j = static_cast<int>(t % k);
-  const std::int64_t e = std::int64_t(rp[dm[cik]]) + islots[t];
-  const std::int32_t pos = atomicAdd(cursor + e, 1);
-  code[pos] = cik * k + j;
-}
-
-/// Each entry's list sorted ascending (= cell order: one contribution per
-/// cell and entry).
-__global__ void __launch_bounds__(kBlock)
-    contrib_sort_k(std::int64_t nnz, const std::int32_t* __restrict__ off,
-                   std::int32_t* __restrict__ code)
-{
-  const std::int64_t e = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
-  if (e >= nnz)
-    return;
-  const std::int32_t q0 = off[e], q1 = off[e + 1];
-  for (std::int32_t q = q0 + 1; q < q1; ++q)
-  {
-    const std::int32_t v = code[q];
-    std::int32_t w = q - 1;
-    while (w >= q0 && code[w] > v)
-    {
-      code[w + 1] = code[w];
-      --w;
-    }
-    code[w + 1] = v;
-  }
-}
-
-/// G lanes per row: lane l sums entries rp[r] + l, + G, ... over their
-/// contributions; rows and columns of Dirichlet dofs store zero (every
-/// contribution to such an entry has its row or column masked).
-template <typename T, int K, int G, bool MAT>
-__global__ void __launch_bounds__(kBlock)
-    entry_assemble_k(std::int32_t nrows, std::int32_t chunk, const std::int32_t* __restrict__ rp,
-                     const std::int32_t* __restrict__ ci, const std::int32_t* __restrict__ off,
-                     const std::int32_t* __restrict__ code, const T* __restrict__ cd,
-                     const T* __restrict__ elem, const T* __restrict__ ref,
-                     const std::int8_t* __restrict__ bc, T* __restrict__ vals, int accumulate)
-{
-  constexpr int KK = K * K;
-  extern __shared__ unsigned char smem_raw[];
-  T* S = reinterpret_cast<T*>(smem_raw); // reference tensors [7][K][K] unless MAT
-  if constexpr (!MAT)
-  {
-    for (int t = threadIdx.x; t < 7 * KK; t += kBlock)
-      S[t] = ref[t];
-    __syncthreads();
-  }
-  const int lane = threadIdx.x % G;
-  // the workgroup's contiguous range of chunk rows, kBlock / G at a time
-  const std::int32_t rb = xcd_contiguous(blockIdx.x, gridDim.x) * chunk;
-  const std::int32_t re = min(rb + chunk, nrows);
-  for (std::int32_t r = rb + threadIdx.x / G; r < re; r += kBlock / G)
-  {
-    const bool rbc = bc && bc[r];
-    const std::int32_t e1 = rp[r + 1];
-    for (std::int32_t e = rp[r] + lane; e < e1; e += G)
-    {
-      T v = T(0);
-      if (!rbc && !(bc && bc[ci[e]]))
-      {
-        const std::int32_t q1 = off[e + 1];
-        for (std::int32_t q = off[e]; q < q1; ++q)
-        {
-          const std::int32_t cq = code[q];
-          if constexpr (MAT)
-            v += elem[cq];
-          else
-          {
-            const std::int32_t c = cq / KK;
-            const int ij = cq - c * KK;
-            const T* F = cd + 8 * std::int64_t(c);
-            const T* Sij = S + ij;
-            v += F[0] * Sij[0] + F[1] * Sij[KK] + F[2] * Sij[2 * KK] + F[3] * Sij[3 * KK]
-                 + F[4] * Sij[4 * KK] + F[5] * Sij[5 * KK] + F[6] * Sij[6 * KK];
-          }
-        }
-      }
-      vals[e] = accumulate ? vals[e] + v : v;
-    }
-  }
 }
 
 typedef int dfx_i32x4 __attribute__((ext_vector_type(4)));
@@ -2595,130 +2463,7 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// ---------------------------------------------------------------------
-// Box tiles (variant 17): Q1 on an axis-aligned box of hexahedra, without a
-// cell pass or cell records in HBM. A workgroup owns a tile of 8 x 8 x 4
-// grid vertices (its matrix rows): it stages the coordinates and Dirichlet
-// markers of the tile's vertices and their halo (10 x 10 x 6) in LDS,
-// forms the factors of the 9 x 9 x 5 cells around the tile there (the
-// affine factors of every cell, from LDS-staged coordinates), and each
-// thread then sums its row's 27 entries in registers over the row
-// vertex's 8 corner cells (corner order) and stores them at the row's slot
-// positions. The grid maps are checked against the dofmap when the plan is
-// built; any other mesh makes the variant unavailable.
-// ---------------------------------------------------------------------
-
-constexpr int kBoxTX = 8, kBoxTY = 8, kBoxTZ = 4;   // tile of row vertices
-constexpr int kBoxHX = kBoxTX + 2, kBoxHY = kBoxTY + 2, kBoxHZ = kBoxTZ + 2; // vertex halo
-constexpr int kBoxCX = kBoxTX + 1, kBoxCY = kBoxTY + 1, kBoxCZ = kBoxTZ + 1; // cells
-constexpr int kBoxNV = kBoxHX * kBoxHY * kBoxHZ, kBoxNC = kBoxCX * kBoxCY * kBoxCZ;
-static_assert(kBoxTX * kBoxTY * kBoxTZ == kBlock, "one thread per tile vertex");
-
-template <typename T>
-__global__ void __launch_bounds__(kBlock)
-    box_q1_k(std::int32_t nx, std::int32_t ny, std::int32_t nz,
-             const std::int32_t* __restrict__ box_dof, const std::int32_t* __restrict__ box_cell,
-             const T* __restrict__ x, const T* __restrict__ coeffs, int cstride, T creac,
-             const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
-             const std::uint8_t* __restrict__ slotmap, T* __restrict__ vals, int accumulate)
-{
-  __shared__ T Xs[kBoxNV][3];
-  __shared__ std::int8_t Bs[kBoxNV];
-  __shared__ T Rs[kBoxNC][8];
-  const int ntx = (nx + kBoxTX) / kBoxTX, nty = (ny + kBoxTY) / kBoxTY;
-  const int tile = xcd_contiguous(blockIdx.x, gridDim.x);
-  const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-  const int vx0 = tx * kBoxTX, vy0 = ty * kBoxTY, vz0 = tz * kBoxTZ;
-  const std::int64_t sx = nx + 1, sxy = std::int64_t(nx + 1) * (ny + 1);
-  // halo vertices: coordinates and Dirichlet markers
-  for (int li = threadIdx.x; li < kBoxNV; li += kBlock)
-  {
-    const int gx = vx0 - 1 + li % kBoxHX, gy = vy0 - 1 + (li / kBoxHX) % kBoxHY,
-              gz = vz0 - 1 + li / (kBoxHX * kBoxHY);
-    std::int8_t mk = 0;
-    if (gx >= 0 && gx <= nx && gy >= 0 && gy <= ny && gz >= 0 && gz <= nz)
-    {
-      const std::int64_t d = box_dof[gx + sx * gy + sxy * gz];
-      Xs[li][0] = x[3 * d];
-      Xs[li][1] = x[3 * d + 1];
-      Xs[li][2] = x[3 * d + 2];
-      mk = bc ? bc[d] : std::int8_t(0);
-    }
-    Bs[li] = mk;
-  }
-  __syncthreads();
-  // cells around the tile: affine factors and Dirichlet column masks
-  for (int cl = threadIdx.x; cl < kBoxNC; cl += kBlock)
-  {
-    const int hx = cl % kBoxCX, hy = (cl / kBoxCX) % kBoxCY, hz = cl / (kBoxCX * kBoxCY);
-    const int cx = vx0 - 1 + hx, cy = vy0 - 1 + hy, cz = vz0 - 1 + hz;
-    T g[7] = {0, 0, 0, 0, 0, 0, 0};
-    std::uint32_t m = 0xffu; // absent cell: every column masked
-    if (cx >= 0 && cx < nx && cy >= 0 && cy < ny && cz >= 0 && cz < nz)
-    {
-      const std::int64_t c = box_cell[cx + std::int64_t(nx) * (cy + std::int64_t(ny) * cz)];
-      const int v0 = hx + kBoxHX * (hy + kBoxHY * hz);
-      affine_factors<T>(Xs[v0], Xs[v0 + 1], Xs[v0 + kBoxHX], Xs[v0 + kBoxHX * kBoxHY],
-                        coeffs[c * cstride], creac, g);
-      m = 0;
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        if (Bs[v0 + (j & 1) + kBoxHX * ((j >> 1) & 1) + kBoxHX * kBoxHY * ((j >> 2) & 1)])
-          m |= 1u << j;
-    }
-#pragma unroll
-    for (int q = 0; q < 7; ++q)
-      Rs[cl][q] = g[q];
-    Rs[cl][7] = mask_bits<T>(m);
-  }
-  __syncthreads();
-  const int lx = threadIdx.x % kBoxTX, ly = (threadIdx.x / kBoxTX) % kBoxTY,
-            lz = threadIdx.x / (kBoxTX * kBoxTY);
-  const int vx = vx0 + lx, vy = vy0 + ly, vz = vz0 + lz;
-  if (vx > nx || vy > ny || vz > nz)
-    return;
-  const std::int32_t r = box_dof[vx + sx * vy + sxy * vz];
-  const bool rowbc = Bs[(lx + 1) + kBoxHX * ((ly + 1) + kBoxHY * (lz + 1))] != 0;
-  T acc[27];
-#pragma unroll
-  for (int q = 0; q < 27; ++q)
-    acc[q] = T(0);
-  if (!rowbc)
-  {
-#pragma unroll
-    for (int I = 0; I < 8; ++I)
-    {
-      const int cl = (lx + 1 - (I & 1)) + kBoxCX * ((ly + 1 - ((I >> 1) & 1))
-                                                    + kBoxCY * (lz + 1 - ((I >> 2) & 1)));
-      T g[7];
-#pragma unroll
-      for (int q = 0; q < 7; ++q)
-        g[q] = Rs[cl][q];
-      const std::uint32_t m = bits_mask<T>(Rs[cl][7]);
-      T gs[3];
-      q1_signed<T>(g, I, gs);
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        if (!((m >> j) & 1u))
-          acc[q1_slot(I, j)] += q1_entry<T>(g, gs, I, j);
-    }
-  }
-  const std::uint32_t* sm = reinterpret_cast<const std::uint32_t*>(slotmap + 28 * std::int64_t(r));
-  std::uint32_t w[7];
-#pragma unroll
-  for (int k = 0; k < 7; ++k)
-    w[k] = sm[k];
-  T* out = vals + rp[r];
-#pragma unroll
-  for (int q = 0; q < 27; ++q)
-  {
-    const std::uint32_t s = (w[q / 4] >> (8 * (q % 4))) & 0xffu;
-    if (s != 0xffu)
-      out[s] = accumulate ? out[s] + acc[q] : acc[q];
-  }
-}
-
-/// Variant 18 (Q1 on a structured box of hexahedra): one thread per row as
+/// Variant 15 (Q1 on a structured box of hexahedra): one thread per row as
 /// variant 9 -- the row vertex's 8 corner cells, its 27 entries summed in
 /// registers with compile-time reference values, corner order -- but the
 /// rows of a workgroup are consecutive and their entries go through an LDS
@@ -3668,10 +3413,8 @@ int incidence_lanes(const dfx_form& f)
 /// with 64/128/256 rows per workgroup; 3 "lanes"; 4..6 closed-form Q1 rows;
 /// 7, 8 incidence-parallel gathers; 9 structured Q1; 10 fused geometry;
 /// 11 as 1 with eight incidences in flight; 12..14 fixed-width Q1 rows;
-/// 15, 16 entry-parallel (contributions from the factors / from element
-/// matrices formed on the matrix cores); 17 Q1 box tiles; 18 structured Q1
-/// rows in registers stored through an LDS slice.
-constexpr int kRowVariants = 19;
+/// 15 structured Q1 rows in registers stored through an LDS slice.
+constexpr int kRowVariants = 16;
 
 template <typename T, typename O, int K, int L>
 void launch_lanes(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
@@ -3830,97 +3573,6 @@ bool ensure_struct(dfx_form& f, const dfx_csr& A)
     return false;
   }
   f.struct_ok = 1;
-  return true;
-}
-
-/// Box plan of a Q1 form (built once per pattern): the structured row plan
-/// (slot positions of the 27 neighbour offsets, ensure_struct) and the grid
-/// maps, found from the geometry (grid index = (x - min) / h per axis, h
-/// the edge lengths of cell 0) and checked against the dofmap: every grid
-/// point holds exactly one vertex, every grid cell exactly one cell, and
-/// each cell's local vertex j sits at the cell's grid point + the bits of j.
-template <typename T>
-bool ensure_box(dfx_form& f, const dfx_csr& A)
-{
-  if (f.box_pattern == A.pattern_id && f.box_ok >= 0)
-    return f.box_ok == 1;
-  dfree(f.box_dof);
-  dfree(f.box_cell);
-  f.box_dof = f.box_cell = nullptr;
-  f.box_pattern = A.pattern_id;
-  f.box_ok = 0;
-  if (f.dofmap != f.x_dofmap || f.d.nodes_per_cell != 8 || !ensure_struct(f, A))
-    return false;
-  const std::int64_t nc = f.d.num_cells, nv = f.d.num_nodes;
-  if (nc == 0 || nv != A.nrows)
-    return false;
-  std::vector<T> xt(3 * nv);
-  std::vector<std::int32_t> dm(8 * nc);
-  d2h(xt.data(), f.x, sizeof(T) * xt.size());
-  d2h(dm.data(), f.dofmap, sizeof(std::int32_t) * dm.size());
-  double lo[3], h[3];
-  for (int a = 0; a < 3; ++a)
-  {
-    lo[a] = double(xt[a]);
-    for (std::int64_t v = 1; v < nv; ++v)
-      lo[a] = std::min(lo[a], double(xt[3 * v + a]));
-  }
-  // edge lengths of cell 0 along its local x, y, z (vertices 1, 2, 4)
-  const int axis_vertex[3] = {1, 2, 4};
-  for (int a = 0; a < 3; ++a)
-  {
-    const std::int64_t v0 = dm[0], va = dm[axis_vertex[a]];
-    h[a] = double(xt[3 * va + a]) - double(xt[3 * v0 + a]);
-    if (!(h[a] > 0))
-      return false;
-  }
-  std::vector<std::int32_t> gi(3 * nv);
-  std::int64_t n[3] = {0, 0, 0};
-  for (std::int64_t v = 0; v < nv; ++v)
-    for (int a = 0; a < 3; ++a)
-    {
-      const double t = (double(xt[3 * v + a]) - lo[a]) / h[a];
-      const double k = std::nearbyint(t);
-      if (std::abs(t - k) > 1e-3 || k < 0 || k > 1e7)
-        return false;
-      gi[3 * v + a] = static_cast<std::int32_t>(k);
-      n[a] = std::max<std::int64_t>(n[a], gi[3 * v + a]);
-    }
-  if ((n[0] + 1) * (n[1] + 1) * (n[2] + 1) != nv || n[0] * n[1] * n[2] != nc)
-    return false;
-  std::vector<std::int32_t> bdof(nv, -1), bcell(nc, -1);
-  for (std::int64_t v = 0; v < nv; ++v)
-  {
-    const std::int64_t g = gi[3 * v] + (n[0] + 1) * (gi[3 * v + 1] + (n[1] + 1) * gi[3 * v + 2]);
-    if (bdof[g] != -1)
-      return false;
-    bdof[g] = static_cast<std::int32_t>(v);
-  }
-  for (std::int64_t c = 0; c < nc; ++c)
-  {
-    const std::int32_t* cv = dm.data() + 8 * c;
-    const std::int32_t* g0 = gi.data() + 3 * std::int64_t(cv[0]);
-    if (g0[0] >= n[0] || g0[1] >= n[1] || g0[2] >= n[2])
-      return false;
-    for (int j = 0; j < 8; ++j)
-    {
-      const std::int32_t* gj = gi.data() + 3 * std::int64_t(cv[j]);
-      if (gj[0] != g0[0] + (j & 1) || gj[1] != g0[1] + ((j >> 1) & 1)
-          || gj[2] != g0[2] + ((j >> 2) & 1))
-        return false;
-    }
-    const std::int64_t gc = g0[0] + n[0] * (g0[1] + n[1] * g0[2]);
-    if (bcell[gc] != -1)
-      return false;
-    bcell[gc] = static_cast<std::int32_t>(c);
-  }
-  f.box_dof = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * nv));
-  f.box_cell = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * nc));
-  h2d(f.box_dof, bdof.data(), sizeof(std::int32_t) * nv);
-  h2d(f.box_cell, bcell.data(), sizeof(std::int32_t) * nc);
-  for (int a = 0; a < 3; ++a)
-    f.box_n[a] = static_cast<std::int32_t>(n[a]);
-  f.box_ok = 1;
   return true;
 }
 
@@ -4091,134 +3743,11 @@ bool ensure_fixed(dfx_form& f, const dfx_csr& A)
   return true;
 }
 
-/// Entry-parallel plan of a form for the pattern of A (built once): count
-/// the contributions of every entry, offsets by a prefix sum, fill, then
-/// sort each entry's list into cell order.
-template <typename O>
-void ensure_contrib(dfx_form& f, const dfx_csr& A)
-{
-  if (f.contrib && f.contrib_pattern == A.pattern_id)
-    return;
-  dfree(f.contrib);
-  dfree(f.contrib_off);
-  f.contrib = nullptr;
-  f.contrib_off = nullptr;
-  const int k = f.d.dofs_per_cell;
-  const std::int64_t ninc = std::int64_t(f.d.num_cells) * k;
-  require(ninc * k < INT32_MAX, "too many element contributions for the entry-parallel plan");
-  const std::int64_t nnz = A.nnz;
-  std::int32_t* cnt = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * (nnz + 1)));
-  DFX_HIP(hipMemsetAsync(cnt, 0, sizeof(std::int32_t) * (nnz + 1), stream()));
-  const O* sl = static_cast<const O*>(f.islots);
-  if (ninc > 0)
-    contrib_count_k<O><<<blocks_for(ninc * k), kBlock, 0, stream()>>>(ninc, k, f.inc, sl,
-                                                                       f.dofmap, A.row_ptr, cnt);
-  check_launch("contribution count");
-  std::vector<std::int32_t> h(nnz + 1);
-  d2h(h.data(), cnt, sizeof(std::int32_t) * nnz);
-  std::int32_t run = 0;
-  for (std::int64_t e = 0; e < nnz; ++e)
-  {
-    const std::int32_t c = h[e];
-    h[e] = run;
-    run += c;
-  }
-  h[nnz] = run;
-  f.contrib_off = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * (nnz + 1)));
-  h2d(f.contrib_off, h.data(), sizeof(std::int32_t) * (nnz + 1));
-  h2d(cnt, h.data(), sizeof(std::int32_t) * (nnz + 1)); // fill cursors
-  f.contrib = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * std::max(run, 1)));
-  if (ninc > 0)
-  {
-    contrib_fill_k<O><<<blocks_for(ninc * k), kBlock, 0, stream()>>>(ninc, k, f.inc, sl, f.dofmap,
-                                                                      A.row_ptr, cnt, f.contrib);
-    contrib_sort_k<<<blocks_for(nnz), kBlock, 0, stream()>>>(nnz, f.contrib_off, f.contrib);
-  }
-  check_launch("contribution plan");
-  DFX_HIP(hipStreamSynchronize(stream()));
-  dfree(cnt);
-  // lanes per row: the power of two among 16, 32, 64 nearest above the
-  // mean row length
-  const double mean = A.nrows ? double(nnz) / A.nrows : 1.0;
-  f.entry_lanes = mean <= 20 ? 16 : (mean <= 40 ? 32 : 64);
-  f.contrib_pattern = A.pattern_id;
-}
-
-/// Variants 15 (contributions from the cell factors) and 16 (from the
-/// element matrices of all cells, formed on the matrix cores).
-template <typename T, typename O, int K>
-void launch_entries(dfx_form& f, dfx_csr& A, const dfx_bc* bc, bool mat)
-{
-  ensure_contrib<O>(f, A);
-  launch_cell_factors<T>(f, bc);
-  const std::int32_t nc = f.d.num_cells;
-  if (mat)
-  {
-    if (!f.cell_elem)
-      f.cell_elem = dmalloc(sizeof(T) * std::max<std::int64_t>(std::int64_t(nc) * K * K, 1));
-    const int nb = static_cast<int>((std::int64_t(nc) + 63) / 64);
-    if (nb > 0)
-      elem_mfma_k<T, K><<<nb, kBlock, 0, stream()>>>(
-          nc, static_cast<const T*>(f.cell_data), static_cast<const T*>(f.ref_tensors),
-          static_cast<T*>(f.cell_elem));
-  }
-  const std::size_t lds = mat ? 0 : sizeof(T) * 7 * K * K;
-  const int G = f.entry_lanes;
-  // a grid of 8 workgroups per CU, each over a contiguous range of rows
-  // (the reference tensors are staged once per workgroup)
-  const std::int64_t rows_per_pass = kBlock / G;
-  const std::int64_t nb64 = std::min<std::int64_t>(
-      std::int64_t(ctx().num_cu) * 8, (A.nrows + rows_per_pass - 1) / rows_per_pass);
-  if (nb64 <= 0)
-    return;
-  const int nb = static_cast<int>(nb64);
-  const std::int32_t chunk = static_cast<std::int32_t>((A.nrows + nb64 - 1) / nb64);
-  auto go = [&](auto kern)
-  {
-    kern<<<nb, kBlock, lds, stream()>>>(A.nrows, chunk, A.row_ptr, A.cols, f.contrib_off, f.contrib,
-                                        static_cast<const T*>(f.cell_data),
-                                        static_cast<const T*>(f.cell_elem),
-                                        static_cast<const T*>(f.ref_tensors),
-                                        bc ? bc->marker : nullptr, static_cast<T*>(A.values),
-                                        A.zeroed ? 0 : 1);
-  };
-  if (mat)
-    G == 16 ? go(entry_assemble_k<T, K, 16, true>)
-            : (G == 32 ? go(entry_assemble_k<T, K, 32, true>)
-                       : go(entry_assemble_k<T, K, 64, true>));
-  else
-    G == 16 ? go(entry_assemble_k<T, K, 16, false>)
-            : (G == 32 ? go(entry_assemble_k<T, K, 32, false>)
-                       : go(entry_assemble_k<T, K, 64, false>));
-  check_launch("entry assembly");
-}
-
 template <typename T, typename O, int K>
 void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
 {
   const int variant = f.row_variant;
-  if (variant == 17)
-  {
-    if constexpr (K == 8)
-    {
-      require(ensure_box<T>(f, A), "variant not available for this mesh");
-      const std::int64_t ntx = (f.box_n[0] + kBoxTX) / kBoxTX,
-                         nty = (f.box_n[1] + kBoxTY) / kBoxTY,
-                         ntz = (f.box_n[2] + kBoxTZ) / kBoxTZ;
-      const std::int64_t nb = ntx * nty * ntz;
-      require(nb < INT32_MAX, "box too large");
-      box_q1_k<T><<<static_cast<int>(nb), kBlock, 0, stream()>>>(
-          f.box_n[0], f.box_n[1], f.box_n[2], f.box_dof, f.box_cell,
-          static_cast<const T*>(f.x), static_cast<const T*>(f.coeffs), f.d.cstride,
-          static_cast<T>(f.constants[0]), bc ? bc->marker : nullptr, A.row_ptr, f.slotmap,
-          static_cast<T*>(A.values), A.zeroed ? 0 : 1);
-      check_launch("row_assemble");
-      return;
-    }
-    else
-      throw Error(DFX_ERR_ARG, "variant not available for this element");
-  }
-  if (variant == 18)
+  if (variant == 15)
   {
     if constexpr (K == 8)
     {
@@ -4236,13 +3765,6 @@ void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
     }
     else
       throw Error(DFX_ERR_ARG, "variant not available for this element");
-  }
-  if (variant == 15 || variant == 16)
-  {
-    // the entry kernel reads the Dirichlet marker at column indices
-    require(!bc || A.ncols == A.nrows, "variant not available for a non-square matrix");
-    launch_entries<T, O, K>(f, A, bc, variant == 16);
-    return;
   }
   if (variant >= 12 && variant <= 14)
   {
@@ -5362,10 +4884,6 @@ dfx_status dfx_form_destroy(dfx_form* f)
     dfree(f->slotmap);
     dfree(f->inc8);
     dfree(f->islot8);
-    dfree(f->contrib);
-    dfree(f->contrib_off);
-    dfree(f->box_dof);
-    dfree(f->box_cell);
     dfree(f->elem_tensors);
     dfree(f->pack_map);
     dfree(f->pack_stage);
@@ -5672,7 +5190,7 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t*
         f.row_variant = best;
         // release the plans of the variants not kept (the element matrices
         // of variant 7 alone are k^2 values per cell)
-        if (best != 7 && best != 16)
+        if (best != 7)
         {
           dfree(f.cell_elem);
           f.cell_elem = nullptr;
@@ -5686,14 +5204,6 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t*
           f.ngblk = 0;
           f.gather_pattern = 0;
         }
-        if (best != 15 && best != 16)
-        {
-          dfree(f.contrib);
-          dfree(f.contrib_off);
-          f.contrib = nullptr;
-          f.contrib_off = nullptr;
-          f.contrib_pattern = 0;
-        }
         if (best < 12 || best > 14)
         {
           dfree(f.inc8);
@@ -5702,15 +5212,7 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t*
           f.islot8 = nullptr;
           f.fixed_pattern = 0;
         }
-        if (best != 17)
-        {
-          dfree(f.box_dof);
-          dfree(f.box_cell);
-          f.box_dof = f.box_cell = nullptr;
-          f.box_pattern = 0;
-          f.box_ok = -1;
-        }
-        if (best != 9 && best != 17 && best != 18)
+        if (best != 9 && best != 15)
         {
           dfree(f.corner_cell);
           dfree(f.slotmap);

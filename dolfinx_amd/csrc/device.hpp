@@ -274,21 +274,6 @@ struct dfx_form
   std::int32_t* inc8 = nullptr;
   std::uint8_t* islot8 = nullptr;
   std::uint64_t fixed_pattern = 0;
-  // Entry-parallel plan: for every CSR entry the element contributions
-  // (c k + i) k + j that add into it, in cell order (contrib, contrib_off:
-  // nnz + 1 offsets); lanes per row of the entry kernel.
-  std::int32_t* contrib = nullptr;
-  std::int32_t* contrib_off = nullptr;
-  std::uint64_t contrib_pattern = 0;
-  int entry_lanes = 32;
-  // Box tiles (Q1 on an axis-aligned box of hexahedra): the vertex (= dof)
-  // at each grid point and the cell at each grid cell, grid sizes in cells;
-  // box_ok: -1 not checked for box_pattern, 0 not a box, 1 usable.
-  std::int32_t* box_dof = nullptr;
-  std::int32_t* box_cell = nullptr;
-  std::int32_t box_n[3] = {0, 0, 0};
-  std::uint64_t box_pattern = 0;
-  int box_ok = -1;
   // Forms whose cell tensors come from a host element kernel with the
   // reference's kernel ABI (kernelptr_t): the kernel and its custom data,
   // host copies of what it reads per cell (geometry, packed coefficients,

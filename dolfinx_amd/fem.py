@@ -280,9 +280,8 @@ class Form:
         gathered by rows, 8 one incidence per thread, 9 structured-box Q1
         rows in registers, 10 factors formed from the geometry per
         incidence, 11 as 1 with eight incidences in flight, 12-14 Q1 rows
-        with their plan at fixed positions, 15/16 entry-parallel sums of
-        each entry's contributions from the factors / from the element
-        matrices)."""
+        with their plan at fixed positions, 15 structured Q1 rows in
+        registers stored through an LDS slice)."""
         _lib.call("dfx_form_set_variant", self._h, int(variant))
 
     @property

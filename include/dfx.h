@@ -335,13 +335,7 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc,
  * the cell factors formed from the geometry by every incidence (no cell
  * pass; affine P1/Q1); 11 as 1 with eight incidences in flight; 12-14 Q1
  * hexahedra whose vertices lie in at most 8 cells: one thread per row with
- * its plan at fixed positions, 64/128/256 rows per workgroup; 15 and 16
- * entry-parallel: each CSR entry sums its element contributions in cell
- * order (15 formed from the cell factors, 16 read from the element
- * matrices formed on the matrix cores); 17 Q1 on an axis-aligned box of
- * hexahedra: tiles of 8 x 8 x 4 row vertices form their cells' factors in
- * LDS from the staged coordinates (no cell pass) and sum each row's 27
- * entries in registers (error if the mesh is not such a box); 18 as 9 with
+ * its plan at fixed positions, 64/128/256 rows per workgroup; 15 as 9 with
  * the rows of a workgroup stored through an LDS slice. The P2
  * elasticity form has a single row kernel and ignores the variant. */
 dfx_status dfx_form_set_variant(dfx_form* a, int32_t variant);

@@ -61,7 +61,7 @@ def test_assemble_system_matches_oracle(cell, n, dtype):
 
 
 @pytest.mark.parametrize("cell,n", CASES)
-@pytest.mark.parametrize("variant", list(range(19)))
+@pytest.mark.parametrize("variant", list(range(16)))
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_every_row_kernel_variant_matches_oracle(cell, n, variant, dtype):
     """Variants 0-3 use the reference tensors, 4-6 closed-form entries for
@@ -69,13 +69,11 @@ def test_every_row_kernel_variant_matches_oracle(cell, n, variant, dtype):
     matrices on the matrix cores gathered by rows, 8 incidence-parallel
     gather, 9 structured-box Q1 rows in registers (hexahedra only), 10 rows
     forming the cell factors from the geometry (no cell pass), 12-14
-    Q1 rows with their plan at fixed positions (hexahedra only), 15/16
-    entry-parallel sums of each entry's contributions (from the factors /
-    from the element matrices), 17 box tiles, 18 structured rows in
+    Q1 rows with their plan at fixed positions, 15 structured rows in
     registers through an LDS slice (hexahedra only)."""
     p = build(cell, n, dtype)
     p.a.set_variant(variant)
-    if variant in (9, 12, 13, 14, 17, 18) and cell == O.TET:
+    if variant in (9, 12, 13, 14, 15) and cell == O.TET:
         with pytest.raises(DfxError, match="not available"):
             problems.assemble_system(p)
         return
@@ -91,7 +89,7 @@ def test_autotune_keeps_results():
     p = build(O.HEX, (20, 20, 20), np.float64)
     A = fem.create_matrix(p.a)
     variant, ms = p.a.autotune(A, p.bc)
-    assert 0 <= variant <= 18 and ms > 0
+    assert 0 <= variant <= 15 and ms > 0
     assert np.all(A.data == 0)
     fem.assemble_matrix(p.a, [p.bc], 1.0, A)
     ref = oracle_system(p, (20, 20, 20))
@@ -273,45 +271,41 @@ def test_single_cell_box_and_zero_rhs(cell):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_box_tiles_on_shuffled_input_and_refused_off_a_box(dtype):
-    """Variant 17 needs an axis-aligned box of hexahedra: a box given in a
-    shuffled input order (other cell and vertex numbers) is one, an L-shaped
-    block of hexahedra is not (the variant raises; the default assembles)."""
+@pytest.mark.parametrize("variant", [9, 12, 15])
+def test_structured_q1_variants_on_shuffled_and_l_shaped_input(dtype, variant):
+    """The structured Q1 row variants (corner cells / fixed-width plans) on
+    a box given in a shuffled input order (other cell and vertex numbers)
+    and on an L-shaped block of hexahedra (rows with absent corners),
+    against the oracle."""
     from dolfinx_amd import mesh as dmesh
     from test_mesh_create import box_input
 
-    n = (6, 5, 7)
-    cells, x = box_input(n, O.HEX, 3)
-    m = dmesh.create_mesh(dmesh.CellType.hexahedron, cells, x)
-    p = problems.scalar_problem(n, O.HEX, kappa="exp_normal", c=0.4, f="normal", seed=9,
-                                dtype=dtype, msh=m)
-    p.a.set_variant(17)
-    A, b = problems.assemble_system(p)
-    om = O.Mesh.from_cells(O.HEX, cells, x)
-    dm, nd = om.dofmap(1)
-    rp, cols = O.sparsity(dm, dm, nd)
-    bn = om.boundary_nodes(1, dm)
-    mark = np.zeros(nd, np.int8)
-    mark[bn] = 1
-    kap = p.kappa[om.original_cell_index].reshape(-1, 1)
-    Ar = O.assemble_matrix(O.K_Q1HEX_A, om.x, om.cells, dm, 1, kap, np.array([0.4]), mark, mark,
-                           rp, cols)
-    O.set_diagonal(Ar, bn, 1, 1.0, rp, cols)
-    assert rel_err(A.data.astype(np.float64), Ar) <= TOL[dtype]
-    # an L-shaped block: the 2 x 2 x 1 box without one of its hexahedra
-    keep = [c for c in range(4)]
     om2 = O.Mesh((2, 2, 1), O.HEX)
     cin = np.empty_like(om2.cells, dtype=np.int64)
     cin[om2.original_cell_index] = om2.input_vertex_index[om2.cells]
     xin = np.empty_like(om2.x)
     xin[om2.input_vertex_index] = om2.x
-    cl = cin[keep[:3]]
+    cl = cin[:3]
     used = np.unique(cl)
     relabel = np.full(len(xin), -1)
     relabel[used] = np.arange(len(used))
-    mL = dmesh.create_mesh(dmesh.CellType.hexahedron, relabel[cl], xin[used])
-    pL = problems.scalar_problem(None, O.HEX, kappa=np.ones(3), c=0.0, f="one", dtype=dtype,
-                                 msh=mL)
-    pL.a.set_variant(17)
-    with pytest.raises(DfxError, match="not available"):
-        problems.assemble_system(pL)
+    for cells, x in (box_input((6, 5, 7), O.HEX, 3), (relabel[cl], xin[used])):
+        m = dmesh.create_mesh(dmesh.CellType.hexahedron, cells, x)
+        nc = m.topology.cells.shape[0]
+        kappa = np.exp(np.random.default_rng(9).standard_normal(nc))
+        p = problems.scalar_problem(None, O.HEX, kappa=kappa, c=0.4, f="normal", seed=9,
+                                    dtype=dtype, msh=m)
+        p.a.set_variant(variant)
+        A, b = problems.assemble_system(p)
+        om = O.Mesh.from_cells(O.HEX, cells, x)
+        dm, nd = om.dofmap(1)
+        rp, cols = O.sparsity(dm, dm, nd)
+        bn = om.boundary_nodes(1, dm)
+        mark = np.zeros(nd, np.int8)
+        mark[bn] = 1
+        kap = kappa[om.original_cell_index].reshape(-1, 1)
+        Ar = O.assemble_matrix(O.K_Q1HEX_A, om.x, om.cells, dm, 1, kap, np.array([0.4]), mark,
+                               mark, rp, cols)
+        O.set_diagonal(Ar, bn, 1, 1.0, rp, cols)
+        assert np.array_equal(A.indptr, rp) and np.array_equal(A.indices, cols)
+        assert rel_err(A.data.astype(np.float64), Ar) <= TOL[dtype]

@@ -78,12 +78,12 @@ def q1_128():
     gc.collect()
 
 
-@pytest.mark.parametrize("variant", [-1, 1, 4, 9, 10, 12, 13, 17, 18])
+@pytest.mark.parametrize("variant", [-1, 1, 4, 9, 10, 12, 13, 15])
 def test_q1_hex_128_fp64_matches_oracle(q1_128, variant):
     """variant -1: the autotuner's choice; 1 reference tensors in LDS, 4
     closed-form Q1 entries, 9 structured-box rows, 10 factors formed per
     incidence (no cell pass), 12/13 rows with their plan at fixed
-    positions."""
+    positions, 15 structured rows through an LDS slice."""
     p, ref = q1_128
     if variant >= 0:
         p.a.set_variant(variant)
@@ -129,7 +129,7 @@ def p3_24():
     gc.collect()
 
 
-@pytest.mark.parametrize("variant", [3, 7, 8, 15, 16])
+@pytest.mark.parametrize("variant", [3, 7, 8])
 def test_p3_tet_24_matches_oracle(p3_24, variant):
     p, ref = p3_24
     p.a.set_variant(variant)
