@@ -76,7 +76,7 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
-EXAMPLES = {"poisson_demo": "poisson.cpp"}
+EXAMPLES = {"poisson_demo": "poisson.cpp", "custom_kernel_demo": "custom_kernel.cpp"}
 
 
 def build_examples() -> None:

@@ -1678,9 +1678,25 @@ __global__ void __launch_bounds__(kBlock)
     if constexpr (MAT)
     {
       const T* er = elem + std::int64_t(cik) * K;
+      if constexpr ((K * sizeof(T)) % 16 == 0)
+      {
+        // element rows start 16-byte aligned: two values per load
+        using T2 = vec2_t<T>;
+        const T2* er2 = reinterpret_cast<const T2*>(er);
 #pragma unroll
-      for (int j = 0; j < K; ++j)
-        e[j] = er[j];
+        for (int j = 0; j < K / 2; ++j)
+        {
+          const T2 v2 = er2[j];
+          e[2 * j] = v2.x;
+          e[2 * j + 1] = v2.y;
+        }
+      }
+      else
+      {
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+          e[j] = er[j];
+      }
     }
     else
     {

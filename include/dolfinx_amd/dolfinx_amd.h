@@ -478,6 +478,43 @@ public:
       _rank = 2;
     }
   }
+  /// The element kernel signature of the reference
+  /// (fem::impl::kernelptr_t<T>, fem/kernel.h:18-20).
+  using kernel_ptr = void (*)(T*, const T*, const T*, const T*, const int*, const std::uint8_t*,
+                              void*);
+  /// A form whose cell integral is an element kernel with the reference's
+  /// ABI -- a compiled tabulate_tensor or a hand-written kernel, as in
+  /// integral_data{kernel, cells} (fem/Form.h:52-87,
+  /// cpp/demo/custom_kernel/main.cpp:77-81): rank 2 (bilinear, (bs k)^2
+  /// row-major tensors) or 1. The kernel runs per cell on the host; its
+  /// tensors are inserted on the device.
+  Form(kernel_ptr kernel, int rank, std::shared_ptr<const FunctionSpace> V,
+       std::vector<double> coeffs, int cstride, std::vector<double> constants)
+      : _V(std::move(V)), _rank(rank)
+  {
+    auto m = _V->mesh();
+    const std::vector<double> x = m->x();
+    const std::vector<std::int32_t> cells = m->cells();
+    const std::vector<std::int32_t> dm = _V->dofmap();
+    dfx_form_desc d{};
+    d.kernel = rank == 2 ? DFX_K_ELEMENT_A : DFX_K_ELEMENT_L;
+    d.dtype = dtype_code<T>();
+    d.num_cells = m->num_cells();
+    d.num_nodes = m->num_vertices();
+    d.nodes_per_cell = m->nodes_per_cell();
+    d.x = x.data();
+    d.x_dofmap = cells.data();
+    d.dofs_per_cell = _V->dofs_per_cell();
+    d.bs = _V->bs();
+    d.dofmap = dm.data();
+    d.cstride = cstride;
+    d.coeffs = coeffs.empty() ? nullptr : coeffs.data();
+    d.num_constants = static_cast<std::int32_t>(constants.size());
+    d.constants = constants.empty() ? nullptr : constants.data();
+    dfx_form* h = nullptr;
+    check(dfx_form_create_kernel(&d, reinterpret_cast<void*>(kernel), nullptr, &h));
+    _h.reset(h, dfx_form_destroy);
+  }
   /// The (cell0, lf0, cell1, lf1) quads of an interior facet integral.
   const std::vector<std::int32_t>& interior_facets() const { return _interior; }
   dfx_form* handle() const { return _h.get(); }

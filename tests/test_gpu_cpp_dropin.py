@@ -52,3 +52,36 @@ def test_cpp_poisson_pipeline(tmp_path, cell, name, n):
     assert rel_err(b, br) <= 1e-12
     assert rel_err(x, xr) <= 1e-8
     assert info["iterations"] > 0
+
+
+def test_cpp_custom_kernel_forms(tmp_path):
+    """Forms built from hand-written C++ element kernels with the
+    reference's kernel ABI (fem::Form(kernel_ptr, rank, ...)), assembled
+    through the drop-in, equal the library's device kernels for the same
+    forms and the oracle."""
+    exe = os.path.join(ROOT, "dolfinx_amd", "_build", "custom_kernel_demo")
+    assert os.path.exists(exe), "build the examples first (dolfinx_amd.build)"
+    n = 6
+    out = tmp_path / "out.bin"
+    r = subprocess.run([exe, str(n), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    assert info["max_diff_rel"] <= 1e-13
+    with open(out, "rb") as fh:
+        arrs = []
+        for _ in range(4):
+            k = int(np.frombuffer(fh.read(8), np.int64)[0])
+            arrs.append(np.frombuffer(fh.read(8 * k), np.float64))
+    Ak, bk, Ad, bd = arrs
+    assert rel_err(bk, bd) <= 1e-13
+    m = O.Mesh((n, n, n), O.TET)
+    dm, nd = m.dofmap(1)
+    rp, cols = O.sparsity(dm, dm, nd)
+    kap = (1.0 + 0.5 * np.sin(m.original_cell_index.astype(float))).reshape(-1, 1)
+    bn = m.boundary_nodes(1, dm)
+    mark = np.zeros(nd, np.int8)
+    mark[bn] = 1
+    Ar = O.assemble_matrix(O.K_P1TET_A, m.x, m.cells, dm, 1, kap, np.array([0.75]), mark, mark,
+                           rp, cols)
+    O.set_diagonal(Ar, bn, 1, 1.0, rp, cols)
+    assert rel_err(Ak, Ar) <= 1e-12
