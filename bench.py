@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this bench with --variant V,
 # scaled by the counters' reading for a 1 GiB device copy (profiles/
 # q1hex128_assembly_spmv_cg.md gives the commands and the per-kernel figures).
-ASSEMBLY_TRAFFIC_BYTES = {1: 1335891297, 4: 1331244534}
+ASSEMBLY_TRAFFIC_BYTES = {1: 1335891297, 4: 1331244534, 15: 1066465075}
 
 
 def ensure_built():
@@ -577,10 +577,12 @@ def main():
             "mixed_s": round(rm.seconds, 4),
             "mixed_outer": rm.outer,
             "mixed_inner_iterations": rm.inner_total,
+            "mixed_fp64_fallback_iterations": rm.fallback_iterations,
             "mixed_rel_residual": rm.true_rel_residual,
             "mixed_speedup_vs_fp64": round(r64.seconds / rm.seconds, 3),
             "mixed_reliable_update_s": round(rr_.seconds, 4),
             "mixed_reliable_update_inner_iterations": rr_.inner_total,
+            "mixed_reliable_update_fp64_fallback_iterations": rr_.fallback_iterations,
             "err_fp64_vs_tight": float(np.linalg.norm(x64 - xref) / nref),
             "err_mixed_vs_tight": float(np.linalg.norm(xmix - xref) / nref),
             "err_mixed_vs_fp64": float(np.linalg.norm(xmix - x64) / np.linalg.norm(x64)),

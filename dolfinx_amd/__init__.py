@@ -6,6 +6,6 @@ include/dfx.h through libdfx.so, built in-tree by dolfinx_amd.build.
 """
 
 from . import _lib  # noqa: F401
-from . import fem, la, mesh, solvers  # noqa: F401
+from . import fem, graph, la, mesh, solvers  # noqa: F401
 
-__all__ = ["fem", "la", "mesh", "solvers"]
+__all__ = ["fem", "graph", "la", "mesh", "solvers"]

@@ -14,7 +14,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libdfx.so")
 
-DFX_OK, DFX_ERR_ARG, DFX_ERR_PATTERN, DFX_ERR_HIP, DFX_ERR_NOCONV, DFX_ERR_NODEV = range(6)
+(DFX_OK, DFX_ERR_ARG, DFX_ERR_PATTERN, DFX_ERR_HIP, DFX_ERR_NOCONV, DFX_ERR_NODEV,
+ DFX_ERR_CAPTURE) = range(7)
 F32, F64 = 0, 1
 TET, HEX = 0, 1
 (K_P1TET_A, K_P1TET_L, K_Q1HEX_A, K_Q1HEX_L, K_P2TET_ELAST_A, K_P2TET_ELAST_L,
@@ -47,12 +48,12 @@ class CGResult(C.Structure):
 
 class MixedOpts(C.Structure):
     _fields_ = [("max_outer", I32), ("max_inner", I32), ("rtol", D), ("inner_rtol", D),
-                ("check_every", I32), ("restart", I32), ("graph", I32)]
+                ("check_every", I32), ("restart", I32), ("graph", I32), ("fallback", I32)]
 
 
 class MixedResult(C.Structure):
     _fields_ = [("outer", I32), ("inner_total", I32), ("converged", I32),
-                ("true_rel_residual", D), ("seconds", D)]
+                ("true_rel_residual", D), ("seconds", D), ("fallback_iterations", I32)]
 
 
 # name -> (restype, argtypes); every function returns dfx_status unless noted
@@ -130,6 +131,11 @@ SIGNATURES = {
     "dfx_batch_remove": (C.c_int, [P, I32]),
     "dfx_batch_step": (C.c_int, [P, I32, I32]),
     "dfx_batch_status": (C.c_int, [P, I32, P, P, P, P, P]),
+    "dfx_batch_restart": (C.c_int, [P]),
+    "dfx_graph_begin": (C.c_int, []),
+    "dfx_graph_end": (C.c_int, [PP]),
+    "dfx_graph_launch": (C.c_int, [P]),
+    "dfx_graph_destroy": (C.c_int, [P]),
     "dfx_event_record": (C.c_int, [C.c_int]),
     "dfx_event_elapsed": (C.c_int, [C.c_int, C.c_int, C.POINTER(F)]),
 }

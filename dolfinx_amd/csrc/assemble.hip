@@ -863,6 +863,24 @@ __device__ inline void load_factors(const T* __restrict__ cd, std::int64_t c, T 
   }
 }
 
+/// a[sl[j]] += v[j] for the unmasked j of one incidence. The K slots of an
+/// incidence are distinct columns of its row, so every value is read
+/// before any is written: the K LDS reads are in flight together instead
+/// of K dependent read-add-write round trips, and each entry still gets
+/// one addition, in the same order as element by element.
+template <typename T, typename O, int K>
+__device__ inline void add_slots(T* a, const O (&sl)[K], const T (&v)[K], std::uint32_t m)
+{
+  T cur[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+    cur[j] = a[sl[j]];
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+    if (!((m >> j) & 1u))
+      a[sl[j]] = cur[j] + v[j];
+}
+
 /// The K slot offsets of incidence p with one vector load where the width
 /// allows (K bytes or 2K bytes, naturally aligned).
 template <typename O, int K>
@@ -1008,17 +1026,19 @@ __device__ inline void row_block(std::int32_t r0, std::int32_t nrows, std::int32
         {
           T gs[3];
           q1_signed<T>(g[u], i, gs);
+          T v[8];
 #pragma unroll
           for (int j = 0; j < 8; ++j)
-            if (!((m[u] >> j) & 1u))
-              a[sl[u][j]] += q1_entry<T>(g[u], gs, i, j);
+            v[j] = q1_entry<T>(g[u], gs, i, j);
+          add_slots<T, O, 8>(a, sl[u], v, m[u]);
         }
         else
         {
+          T v[K];
 #pragma unroll
           for (int j = 0; j < K; ++j)
-            if (!((m[u] >> j) & 1u))
-              a[sl[u][j]] += element_entry<T, K>(g[u], S, i, j);
+            v[j] = element_entry<T, K>(g[u], S, i, j);
+          add_slots<T, O, K>(a, sl[u], v, m[u]);
         }
       }
     }
@@ -1220,14 +1240,18 @@ __global__ void __launch_bounds__(kBlock)
     // column j = i ^ e: the 1D factors of q1_entry depend on e only, so
     // with e fixed at compile time an entry is seven products; the column
     // order differs from j order only in which slot is added first
+    std::uint32_t sle[8];
+    T v[8];
+    std::uint32_t me = 0;
 #pragma unroll
     for (int e = 0; e < 8; ++e)
     {
       const int j = i ^ e;
-      const std::uint32_t sl = static_cast<std::uint32_t>(w >> (8 * j)) & 0xffu;
-      if (!((m[u] >> j) & 1u))
-        a[sl] += q1_entry<T>(g[u], gs, 0, e);
+      sle[e] = static_cast<std::uint32_t>(w >> (8 * j)) & 0xffu;
+      me |= ((m[u] >> j) & 1u) << e;
+      v[e] = q1_entry<T>(g[u], gs, 0, e);
     }
+    add_slots<T, std::uint32_t, 8>(a, sle, v, me);
   }
   __syncthreads();
   if (accumulate)
@@ -1431,17 +1455,19 @@ __global__ void __launch_bounds__(kBlock)
           {
             T gs[3];
             q1_signed<T>(g, i, gs);
+            T v[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j)
-              if (!((m[uu] >> j) & 1u))
-                a[sl[uu][j]] += q1_entry<T>(g, gs, i, j);
+              v[j] = q1_entry<T>(g, gs, i, j);
+            add_slots<T, O, 8>(a, sl[uu], v, m[uu]);
           }
           else
           {
+            T v[K];
 #pragma unroll
             for (int j = 0; j < K; ++j)
-              if (!((m[uu] >> j) & 1u))
-                a[sl[uu][j]] += element_entry<T, K>(g, S, i, j);
+              v[j] = element_entry<T, K>(g, S, i, j);
+            add_slots<T, O, K>(a, sl[uu], v, m[uu]);
           }
         }
       }
@@ -1720,12 +1746,7 @@ __global__ void __launch_bounds__(kBlock)
   for (int k = 0; k < rounds; ++k)
   {
     if (live && rank == k)
-    {
-#pragma unroll
-      for (int j = 0; j < K; ++j)
-        if (!((m >> j) & 1u))
-          a[sl[j]] += e[j];
-    }
+      add_slots<T, O, K>(a, sl, e, m);
     __syncthreads();
   }
   if (accumulate)
@@ -1734,6 +1755,123 @@ __global__ void __launch_bounds__(kBlock)
   else
     for (int t = threadIdx.x; t < len; t += kBlock)
       vals[base + t] = acc[t];
+}
+
+/// Variant 16 (forms with reference tensors, K <= 32 nodes per cell): a
+/// team of K lanes per row, 64 / K teams per wavefront, no element matrix
+/// stored. The team walks its row's incidences (cell c, local row i) in
+/// cell order; lane j forms the entry of column j from the cell record
+/// (one record for the whole team) and the reference tensors in LDS, and
+/// adds it into the row's LDS slice at the slot of column j. The K slots
+/// of an incidence are distinct, so the team's adds need no atomics and
+/// every entry is summed in cell order, with the additions of variant 8.
+/// Each step's record and slot are loaded one step ahead; the first 2K
+/// incidences of a row are loaded together by the team and handed out by
+/// shuffles. Rows come ordered by incidence count inside chunks of 4096
+/// consecutive rows (trow), so the teams of a wavefront take similar
+/// numbers of steps.
+template <typename T, typename O, int K>
+__global__ void __launch_bounds__(kBlock)
+    row_team_k(std::int32_t ntrow, const int4* __restrict__ trow,
+               const std::int32_t* __restrict__ inc, const O* __restrict__ islots,
+               const T* __restrict__ cd, const std::uint32_t* __restrict__ mask,
+               const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
+               const T* __restrict__ ref, T* __restrict__ vals, int accumulate,
+               std::int32_t maxrow)
+{
+  constexpr int G = 64 / K;
+  extern __shared__ unsigned char smem_raw[];
+  T* S = reinterpret_cast<T*>(smem_raw); // reference tensors [7][K][K]
+  for (int t = threadIdx.x; t < 7 * K * K; t += kBlock)
+    S[t] = ref[t];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int team = lane / K, j = lane - team * K;
+  const bool member = team < G;
+  const std::int64_t tq
+      = (std::int64_t(xcd_contiguous(blockIdx.x, gridDim.x)) * (kBlock / 64) + wave) * G + team;
+  int4 ro = make_int4(-1, 0, 0, 0);
+  if (member && tq < ntrow)
+    ro = trow[tq];
+  const std::int32_t r = ro.x, p0 = ro.y, n = ro.z;
+  T* a = S + 7 * K * K + (std::size_t(wave) * G + (member ? team : 0)) * maxrow;
+  std::int32_t rbase = 0, rlen = 0;
+  bool skip = true;
+  std::int32_t pre0 = 0, pre1 = 0;
+  if (r >= 0)
+  {
+    rbase = rp[r];
+    rlen = rp[r + 1] - rbase;
+    skip = (bc && bc[r]) || n == 0;
+    if (n > 0)
+    {
+      pre0 = inc[p0 + min(j, n - 1)];
+      pre1 = inc[p0 + min(K + j, n - 1)];
+    }
+  }
+  for (int t = j; t < rlen; t += K)
+    a[t] = T(0);
+  __syncthreads();
+  if (!skip)
+  {
+    const int tb = team * K;
+    auto cik_at = [&](int st) -> std::int32_t
+    {
+      if (st < 2 * K)
+      {
+        // both shuffles run on every lane of the team (same st)
+        const std::int32_t v0 = __shfl(pre0, tb + (st < K ? st : st - K), 64);
+        const std::int32_t v1 = __shfl(pre1, tb + (st < K ? st : st - K), 64);
+        return st < K ? v0 : v1;
+      }
+      return inc[p0 + st];
+    };
+    struct Stage
+    {
+      T g[7];
+      std::uint32_t m;
+      O sl;
+      int i;
+    };
+    auto fetch = [&](Stage& sg, int st)
+    {
+      const std::int32_t cik = cik_at(st);
+      const std::int32_t c = cik / K;
+      sg.i = cik - c * K;
+      std::uint32_t mf;
+      load_factors<T>(cd, c, sg.g, mf);
+      sg.m = mask ? mask[c] : 0u;
+      sg.sl = islots[std::int64_t(p0 + st) * K + j];
+    };
+    auto add = [&](const Stage& sg)
+    {
+      const T e = element_entry<T, K>(sg.g, S, sg.i, j);
+      if (!((sg.m >> j) & 1u))
+        a[sg.sl] += e;
+    };
+    Stage s0, s1;
+    fetch(s0, 0);
+    for (int st = 0; st < n; st += 2)
+    {
+      if (st + 1 < n)
+        fetch(s1, st + 1);
+      add(s0);
+      if (st + 1 >= n)
+        break;
+      if (st + 2 < n)
+        fetch(s0, st + 2);
+      add(s1);
+    }
+  }
+  // the team stores its own row (the whole slice: zeros for skipped rows)
+  if (r >= 0)
+  {
+    if (accumulate)
+      for (int t = j; t < rlen; t += K)
+        vals[rbase + t] += a[t];
+    else
+      for (int t = j; t < rlen; t += K)
+        __builtin_nontemporal_store(a[t], vals + rbase + t);
+  }
 }
 
 /// BC column masks of element-tensor forms: bit j of mask[c] is set when
@@ -3430,7 +3568,7 @@ int incidence_lanes(const dfx_form& f)
 /// 7, 8 incidence-parallel gathers; 9 structured Q1; 10 fused geometry;
 /// 11 as 1 with eight incidences in flight; 12..14 fixed-width Q1 rows;
 /// 15 structured Q1 rows in registers stored through an LDS slice.
-constexpr int kRowVariants = 16;
+constexpr int kRowVariants = 17;
 
 template <typename T, typename O, int K, int L>
 void launch_lanes(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
@@ -3518,6 +3656,47 @@ void ensure_teams(dfx_form& f, const dfx_csr& A, std::size_t tsize)
   f.team_rows_at = static_cast<std::int32_t>(at);
   f.team_pattern = A.pattern_id;
   f.team_cap = cap;
+}
+
+/// Row order of variant 16 (built once per pattern): rows sorted by their
+/// incidence count, longest first, inside chunks of 4096 consecutive rows
+/// (so a chunk's cells stay together in the L2), each as (row, first
+/// incidence, incidences, 0).
+void ensure_trow(dfx_form& f, const dfx_csr& A)
+{
+  if (f.trow && f.trow_pattern == A.pattern_id)
+    return;
+  dfree(f.trow);
+  f.trow = nullptr;
+  f.ntrow = 0;
+  const std::int32_t nplan = f.max_dof + 1;
+  std::vector<std::int32_t> off(nplan + 1);
+  d2h(off.data(), f.inc_off, sizeof(std::int32_t) * off.size());
+  auto count = [&](std::int32_t r) { return r < nplan ? off[r + 1] - off[r] : 0; };
+  std::vector<std::int32_t> order(A.nrows);
+  for (std::int32_t r = 0; r < A.nrows; ++r)
+    order[r] = r;
+  std::sort(order.begin(), order.end(),
+            [&](std::int32_t x, std::int32_t y)
+            {
+              if ((x >> 12) != (y >> 12))
+                return x < y;
+              const std::int32_t cx = count(x), cy = count(y);
+              return cx != cy ? cx > cy : x < y;
+            });
+  std::vector<std::int32_t> rows(4 * std::size_t(A.nrows));
+  for (std::int32_t q = 0; q < A.nrows; ++q)
+  {
+    const std::int32_t r = order[q];
+    rows[4 * std::size_t(q)] = r;
+    rows[4 * std::size_t(q) + 1] = r < nplan ? off[r] : 0;
+    rows[4 * std::size_t(q) + 2] = count(r);
+    rows[4 * std::size_t(q) + 3] = 0;
+  }
+  f.trow = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * std::max<std::size_t>(rows.size(), 4)));
+  h2d(f.trow, rows.data(), sizeof(std::int32_t) * rows.size());
+  f.ntrow = A.nrows;
+  f.trow_pattern = A.pattern_id;
 }
 
 template <typename T, typename O>
@@ -3850,6 +4029,27 @@ void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
   if (variant == 7 || variant == 8)
   {
     launch_gather<T, O, K>(f, A, bc, variant == 7);
+    check_launch("row_assemble");
+    return;
+  }
+  if (variant == 16)
+  {
+    // teams of K lanes per row; 4 wavefronts per workgroup share the
+    // reference tensors
+    ensure_trow(f, A);
+    constexpr int G = 64 / K;
+    const std::size_t lds
+        = (7 * std::size_t(K) * K + std::size_t(kBlock / 64) * G * A.max_row) * sizeof(T);
+    require(lds <= 64 * 1024, "row too long for the LDS slice");
+    launch_cell_factors<T>(f, bc);
+    const std::int64_t per_block = std::int64_t(kBlock / 64) * G;
+    const int nb = static_cast<int>((std::int64_t(f.ntrow) + per_block - 1) / per_block);
+    if (nb > 0)
+      row_team_k<T, O, K><<<nb, kBlock, lds, stream()>>>(
+          f.ntrow, reinterpret_cast<const int4*>(f.trow), f.inc, static_cast<const O*>(f.islots),
+          static_cast<const T*>(f.cell_data), bc ? f.cell_mask : nullptr,
+          bc ? bc->marker : nullptr, A.row_ptr, static_cast<const T*>(f.ref_tensors),
+          static_cast<T*>(A.values), A.zeroed ? 0 : 1, A.max_row);
     check_launch("row_assemble");
     return;
   }
@@ -4364,6 +4564,10 @@ struct BatchTables
   std::size_t cwork_n = 0;
   int2* rwork = nullptr;
   std::size_t rwork_n = 0;
+  // host copies of what the device tables hold: an assembly whose tables
+  // are unchanged uploads nothing (and so can be recorded into a graph)
+  std::vector<unsigned char> h_sys;
+  std::vector<int2> h_cw, h_rw;
 };
 BatchTables& batch_tables()
 {
@@ -4430,17 +4634,35 @@ void batch_assemble(int n, dfx_csr* const* A, const dfx_form* const* a, const df
   const std::size_t lds = ((cf ? 0 : 7 * K * K) + std::size_t(R) * max_row) * sizeof(T);
   require(lds <= 64 * 1024, "row too long for the LDS slice");
   BatchTables& t = batch_tables();
+  const auto* sb = reinterpret_cast<const unsigned char*>(sys.data());
+  const std::size_t sn = sizeof(BatchSys<T>) * sys.size();
+  if (t.h_sys.size() != sn || std::memcmp(t.h_sys.data(), sb, sn) != 0)
   {
     std::size_t cap = t.sys_bytes / sizeof(BatchSys<T>);
     BatchSys<T>* p = static_cast<BatchSys<T>*>(t.sys);
+    t.h_sys.clear();
     upload_grow(p, cap, sys.data(), sys.size());
     t.sys = p;
     t.sys_bytes = cap * sizeof(BatchSys<T>);
+    t.h_sys.assign(sb, sb + sn);
   }
-  if (!cw.empty())
+  auto same = [](const std::vector<int2>& h, const std::vector<int2>& v)
+  {
+    return h.size() == v.size()
+           && std::memcmp(h.data(), v.data(), sizeof(int2) * v.size()) == 0;
+  };
+  if (!cw.empty() && !same(t.h_cw, cw))
+  {
+    t.h_cw.clear();
     upload_grow(t.cwork, t.cwork_n, cw.data(), cw.size());
-  if (!rw.empty())
+    t.h_cw = cw;
+  }
+  if (!rw.empty() && !same(t.h_rw, rw))
+  {
+    t.h_rw.clear();
     upload_grow(t.rwork, t.rwork_n, rw.data(), rw.size());
+    t.h_rw = rw;
+  }
   const BatchSys<T>* dsys = static_cast<const BatchSys<T>*>(t.sys);
   const T* ref = static_cast<const T*>(a[0]->ref_tensors);
   if (!cw.empty())
@@ -4889,6 +5111,7 @@ dfx_status dfx_form_destroy(dfx_form* f)
     dfree(f->inc_row);
     dfree(f->gblk);
     dfree(f->team_grp);
+    dfree(f->trow);
     dfree(f->facets);
     dfree(f->finc_off);
     dfree(f->finc);
@@ -5236,6 +5459,13 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t*
           f.slotmap = nullptr;
           f.struct_pattern = 0;
           f.struct_ok = -1;
+        }
+        if (best != 16)
+        {
+          dfree(f.trow);
+          f.trow = nullptr;
+          f.ntrow = 0;
+          f.trow_pattern = 0;
         }
         // leave A zeroed (stored when next read)
         A->zeroed = true;

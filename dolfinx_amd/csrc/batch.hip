@@ -36,6 +36,7 @@ struct SysDesc
   double* p;
   double* q;
   double* dinv;
+  const std::int32_t* diag; // CSR position of each row's diagonal (-1: none)
   std::int32_t nrows;
   std::int32_t chunk0; // first chunk of the system
   std::int32_t nchunks;
@@ -247,6 +248,67 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+/// Restart of every active system from x = 0, chunk by chunk over the
+/// batch's tables (the values of bcg_start_k for each chunk).
+__global__ void __launch_bounds__(kBlock)
+    bcg_restart_k(const int* nchunk, const ChunkDesc* chunks, const SysDesc* sys,
+                  const SysState* st, double* p_rz, double* p_rr)
+{
+  const int nch = *nchunk;
+  for (int t = blockIdx.x; t < nch; t += gridDim.x)
+  {
+    const ChunkDesc ch = chunks[t];
+    if (st[ch.sys].active == 0)
+      continue;
+    const SysDesc& D = sys[ch.sys];
+    double srz = 0, srr = 0;
+    for (std::int32_t i = ch.row0 + threadIdx.x; i < ch.row1; i += kBlock)
+    {
+      const std::int32_t pd = D.diag[i];
+      const double a = pd >= 0 ? D.v[pd] : 0.0;
+      const double di = a != 0.0 ? 1.0 / a : 1.0;
+      D.dinv[i] = di;
+      const double bi = D.b[i];
+      D.x[i] = 0.0;
+      D.r[i] = bi;
+      D.p[i] = di * bi;
+      srz += bi * di * bi;
+      srr += bi * bi;
+    }
+    store_block_sum(srz, p_rz, t);
+    store_block_sum(srr, p_rr, t);
+  }
+}
+
+/// Per system (one workgroup per slot): the restart's reductions and
+/// counters, as bcg_start_finish_k with the system's rtol and max_it kept.
+__global__ void __launch_bounds__(kBlock)
+    bcg_restart_finish_k(int nsys, const SysDesc* sys, SysState* st, const double* p_rz,
+                         const double* p_rr)
+{
+  for (int s = blockIdx.x; s < nsys; s += gridDim.x)
+  {
+    SysState& S = st[s];
+    if (S.active == 0)
+      continue;
+    const SysDesc& D = sys[s];
+    const double rz = sum_range(p_rz, D.chunk0, D.nchunks);
+    const double rr = sum_range(p_rr, D.chunk0, D.nchunks);
+    if (threadIdx.x == 0)
+    {
+      S.rz[0] = rz;
+      S.rz[1] = 0;
+      S.rr0 = rr;
+      S.rr = rr;
+      S.iter = 0;
+      S.iter_next = 0;
+      S.converged = rr == 0.0 ? 1 : 0;
+      S.done = (rr == 0.0 || S.max_it <= 0) ? 1 : 0;
+    }
+    __syncthreads();
+  }
+}
+
 } // namespace
 } // namespace dfx
 
@@ -286,6 +348,7 @@ namespace
 
 void batch_rebuild_chunks(dfx_batch& B)
 {
+  no_capture("a change of batch membership");
   std::vector<ChunkDesc> ch;
   for (int s = 0; s < B.max_sys; ++s)
   {
@@ -412,6 +475,7 @@ dfx_status dfx_batch_add(dfx_batch* B, const dfx_csr* A, const dfx_vec* b, dfx_v
         d.v = static_cast<const double*>(A->values);
         d.b = static_cast<const double*>(b->data);
         d.x = static_cast<double*>(x->data);
+        d.diag = A->diag;
         d.nrows = A->nrows;
         const std::int64_t n = std::max(A->nrows, 1);
         auto it = std::find_if(B->pool.begin(), B->pool.end(),
@@ -481,7 +545,8 @@ dfx_status dfx_batch_step(dfx_batch* B, int32_t iterations, int32_t use_graph)
       [&]
       {
         require(B && iterations > 0, "bad argument");
-        if (!use_graph)
+        // inside an open capture the iterations are recorded into it
+        if (!use_graph || capturing())
         {
           launch_iterations(*B, iterations);
           check_launch("batch_step");
@@ -502,6 +567,21 @@ dfx_status dfx_batch_step(dfx_batch* B, int32_t iterations, int32_t use_graph)
           B->graph_iters = iterations;
         }
         DFX_HIP(hipGraphLaunch(B->exec, stream()));
+      });
+}
+
+dfx_status dfx_batch_restart(dfx_batch* B)
+{
+  return guard(
+      [&]
+      {
+        require(B, "null batch");
+        hipStream_t s = stream();
+        bcg_restart_k<<<phase_grid(*B), kBlock, 0, s>>>(B->nchunk, B->chunks, B->sys, B->st,
+                                                        B->p_rz, B->p_rr);
+        bcg_restart_finish_k<<<std::min(B->max_sys, 1024), kBlock, 0, s>>>(
+            B->max_sys, B->sys, B->st, B->p_rz, B->p_rr);
+        check_launch("batch_restart");
       });
 }
 

@@ -58,6 +58,12 @@ int resident_blocks(const void* kernel, int threads, std::size_t lds);
 
 inline std::size_t dtype_size(int dtype) { return dtype == DFX_F64 ? 8 : 4; }
 
+/// Whether the calling thread is recording a graph (dfx_graph_begin).
+bool capturing();
+/// Raises DFX_ERR_CAPTURE when the calling thread is recording a graph:
+/// for steps that need the host part-way (allocation, uploads, waits).
+void no_capture(const char* what);
+
 /// Device allocation helpers (hipMalloc on the library's device).
 void* dmalloc(std::size_t bytes);
 void dfree(void* p);
@@ -293,6 +299,12 @@ struct dfx_form
   int pack_layout = -1;
   double* pack_stage = nullptr;
   std::int64_t pack_stage_n = 0;
+  // Row teams of variant 16: per row, in team order, (row, first
+  // incidence, incidences, 0); rows ordered by incidence count inside
+  // chunks of 4096 consecutive rows; built for trow_pattern.
+  std::int32_t* trow = nullptr;
+  std::int32_t ntrow = 0;
+  std::uint64_t trow_pattern = 0;
   // Elasticity row groups: nteam + 1 (first row position, slice length)
   // pairs, then from team_rows_at the (row, LDS slice offset, first
   // incidence, incidences) of all rows in group order; built for

@@ -24,7 +24,30 @@ Context g_ctx;
 std::atomic<std::uint64_t> g_pattern_counter{1};
 } // namespace
 
-void set_last_error(const std::string& msg) { g_last_error = msg; }
+namespace
+{
+// graph capture on this thread (dfx_graph_begin .. dfx_graph_end) and the
+// first error raised inside it
+thread_local bool g_capturing = false;
+thread_local std::string g_capture_error;
+} // namespace
+
+void set_last_error(const std::string& msg)
+{
+  g_last_error = msg;
+  if (g_capturing && g_capture_error.empty())
+    g_capture_error = msg;
+}
+
+bool capturing() { return g_capturing; }
+
+void no_capture(const char* what)
+{
+  if (g_capturing)
+    throw Error(DFX_ERR_CAPTURE, std::string(what)
+                                     + " cannot be recorded into a graph capture (run the sequence "
+                                       "once before capturing it, so that its set-up is done)");
+}
 
 void init_noalloc_scratch(int device);
 
@@ -84,6 +107,7 @@ void check_launch(const char* what)
 void* dmalloc(std::size_t bytes)
 {
   ensure_init();
+  no_capture("device allocation");
   void* p = nullptr;
   if (bytes == 0)
     bytes = 256;
@@ -99,6 +123,7 @@ void dfree(void* p)
 
 void h2d(void* dst, const void* src, std::size_t bytes)
 {
+  no_capture("host-to-device copy");
   if (bytes)
     DFX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
   DFX_HIP(hipStreamSynchronize(stream()));
@@ -106,6 +131,7 @@ void h2d(void* dst, const void* src, std::size_t bytes)
 
 void d2h(void* dst, const void* src, std::size_t bytes)
 {
+  no_capture("device-to-host copy");
   if (bytes)
     DFX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
   DFX_HIP(hipStreamSynchronize(stream()));
@@ -1687,6 +1713,12 @@ int autotune_lanes(dfx_csr& A, int reps, float* best_ms)
 
 using namespace dfx;
 
+/// A recorded sequence of library launches, replayed as one graph.
+struct dfx_graph
+{
+  hipGraphExec_t exec = nullptr;
+};
+
 extern "C" {
 
 const char* dfx_last_error(void) { return dfx::g_last_error.c_str(); }
@@ -2244,6 +2276,8 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
         };
         int outer = 0, inner = 0, inner_base = 0;
         double rr = true_residual_split();
+        // FP64 residual before the current refinement step (stall test)
+        double rr_prev = rr;
         // an inner solve stops when its residual fell by inner_rtol, or
         // earlier once it meets half the outer target (the last refinement
         // is not solved past what the outer test needs)
@@ -2314,8 +2348,11 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
             ok = true;
             break;
           }
-          if (outer >= o->max_outer || h.iter >= o->max_inner)
+          // stalled: the step did not halve the FP64 residual norm, or the
+          // budget ran out
+          if (rr > 0.25 * rr_prev || outer >= o->max_outer || h.iter >= o->max_inner)
             break;
+          rr_prev = rr;
           if (restart)
           {
             // classic refinement: a new inner CG from d = 0 on the new r
@@ -2338,18 +2375,107 @@ dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32, const dfx_vec
           }
           check_launch("mixed refinement");
         }
+        // FP64 fallback after a stall: Jacobi-PCG on A64 from the current
+        // x, stopping when its residual norm reaches rtol ||b||
+        int fb_it = 0;
+        if (!ok && o->fallback && o->max_inner > 0)
+        {
+          CGWork<double> w64;
+          w64.alloc(n, true);
+          bool conv = false;
+          double rr0f = 0, rrf = 0;
+          const double rt = std::sqrt(tol2 / std::max(rr, 1e-300));
+          fb_it = cg_run<double>(*A64, bd, xd, o->max_inner, rt, true, batch, w64, st, &conv,
+                                 &rr0f, &rrf);
+          rr = true_residual_split();
+          ok = rr <= tol2;
+        }
         DFX_HIP(hipStreamSynchronize(s));
         const auto t1 = std::chrono::steady_clock::now();
         dfree(d32);
         dfree(y);
         res->outer = outer;
         res->inner_total = std::max(inner, inner_base + h.iter);
+        res->fallback_iterations = fb_it;
         res->converged = ok ? 1 : 0;
         res->true_rel_residual = bn2 > 0 ? std::sqrt(rr / bn2) : std::sqrt(rr);
         res->seconds = std::chrono::duration<double>(t1 - t0).count();
         if (!ok)
           throw Error(DFX_ERR_NOCONV, "mixed-precision refinement did not converge");
       });
+}
+
+// ---------------------------------------------------------------------
+// Graph capture of library calls: the library stream records instead of
+// running between dfx_graph_begin and dfx_graph_end (thread-local mode, so
+// other threads keep launching), and the recorded work replays as one
+// graph launch.
+// ---------------------------------------------------------------------
+
+dfx_status dfx_graph_begin(void)
+{
+  return guard(
+      [&]
+      {
+        ensure_init();
+        require(!capturing(), "a graph capture is already open on this thread");
+        DFX_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+        g_capturing = true;
+        g_capture_error.clear();
+      });
+}
+
+dfx_status dfx_graph_end(dfx_graph** out)
+{
+  return guard(
+      [&]
+      {
+        require(capturing(), "no graph capture is open on this thread");
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(stream(), &g);
+        g_capturing = false;
+        const std::string first = g_capture_error;
+        g_capture_error.clear();
+        if (!first.empty() || e != hipSuccess || !out)
+        {
+          if (g)
+            (void)hipGraphDestroy(g);
+          (void)hipGetLastError();
+          if (!out)
+            throw Error(DFX_ERR_ARG, "null output (the capture was discarded)");
+          throw Error(DFX_ERR_CAPTURE,
+                      "graph capture discarded: "
+                          + (first.empty() ? std::string(hipGetErrorString(e)) : first));
+        }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess)
+          throw Error(DFX_ERR_HIP, std::string("graph instantiation: ") + hipGetErrorString(ei));
+        *out = new dfx_graph{exec};
+      });
+}
+
+dfx_status dfx_graph_launch(const dfx_graph* g)
+{
+  return guard(
+      [&]
+      {
+        require(g && g->exec, "null graph");
+        no_capture("dfx_graph_launch");
+        DFX_HIP(hipGraphLaunch(g->exec, stream()));
+      });
+}
+
+dfx_status dfx_graph_destroy(dfx_graph* g)
+{
+  if (g)
+  {
+    if (g->exec)
+      (void)hipGraphExecDestroy(g->exec);
+    delete g;
+  }
+  return DFX_OK;
 }
 
 } // extern "C"

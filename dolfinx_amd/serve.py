@@ -49,6 +49,11 @@ class BatchSolver:
     def step(self, iterations: int, graph: bool = True) -> None:
         _lib.call("dfx_batch_step", self._h, int(iterations), 1 if graph else 0)
 
+    def restart(self) -> None:
+        """Every system from x = 0 with the current values of its A and b
+        (kernels only: can be recorded into a graph.Graph)."""
+        _lib.call("dfx_batch_restart", self._h)
+
     def status(self, slots):
         s = np.ascontiguousarray(slots, np.int32)
         n = len(s)

@@ -28,14 +28,18 @@ def cg(A: MatrixCSR, b: Vector, x: Vector, rtol=1e-8, max_it=10000, jacobi=True,
 
 def mixed_refinement(A64: MatrixCSR, A32: MatrixCSR, b: Vector, x: Vector, rtol=1e-8,
                      inner_rtol=1e-4, max_outer=50, max_inner=20000, check_every=32,
-                     restart=True, graph=False):
+                     restart=True, graph=False, fallback=True):
     """FP32 Jacobi-PCG inside FP64 iterative refinement: each inner solve
     reduces the residual by inner_rtol, then x += d in FP64 and the FP64
     true residual restarts the next inner solve. restart=False keeps the FP32
-    search direction across refinements instead (reliable updates)."""
+    search direction across refinements instead (reliable updates).
+    If a refinement step fails to halve the FP64 residual norm (or the
+    budgets run out), fallback=True finishes with FP64 Jacobi-PCG from the
+    current x (res.fallback_iterations > 0); fallback=False raises."""
     opts = _lib.MixedOpts(max_outer=int(max_outer), max_inner=int(max_inner), rtol=float(rtol),
                           inner_rtol=float(inner_rtol), check_every=int(check_every),
-                          restart=1 if restart else 0, graph=1 if graph else 0)
+                          restart=1 if restart else 0, graph=1 if graph else 0,
+                          fallback=1 if fallback else 0)
     res = _lib.MixedResult()
     _lib.call("dfx_mixed_solve", A64.handle, A32.handle, b.handle, x.handle, C.byref(opts),
               C.byref(res))

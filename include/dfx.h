@@ -34,7 +34,8 @@ enum
   DFX_ERR_PATTERN = 2, /* entry not in sparsity (la/matrix_csr_impl.h:93) */
   DFX_ERR_HIP = 3,     /* HIP runtime failure */
   DFX_ERR_NOCONV = 4,  /* iterative solver did not converge */
-  DFX_ERR_NODEV = 5    /* no GPU device available */
+  DFX_ERR_NODEV = 5,   /* no GPU device available */
+  DFX_ERR_CAPTURE = 6  /* the call cannot be recorded into a graph capture */
 };
 
 enum { DFX_F32 = 0, DFX_F64 = 1 };
@@ -438,6 +439,8 @@ typedef struct
   int32_t restart;     /* 1: restart the FP32 CG at every refinement;
                           0: keep its search direction (reliable updates) */
   int32_t graph;       /* 1: inner iterations replayed from a captured HIP graph */
+  int32_t fallback;    /* 1: when the refinement stalls, finish with FP64
+                          Jacobi-PCG on A64 from the current x */
 } dfx_mixed_opts;
 
 typedef struct
@@ -447,6 +450,7 @@ typedef struct
   int32_t converged;
   double true_rel_residual;
   double seconds;
+  int32_t fallback_iterations; /* FP64 CG iterations after a stall (0: none) */
 } dfx_mixed_result;
 
 /* Mixed-precision iterative refinement (no counterpart in the reference;
@@ -456,7 +460,12 @@ typedef struct
  * is not solved further than the outer test needs), x += d in FP64 and the
  * residual is replaced by the FP64 true residual b - A64 x. With restart = 0 the FP32 CG keeps its search
  * direction across replacements (reliable updates, van der Vorst & Ye
- * 2000); with restart = 1 every refinement starts a new inner CG. */
+ * 2000); with restart = 1 every refinement starts a new inner CG.
+ * The refinement stalls when a refinement step reduces the FP64 residual
+ * norm by less than half, or when max_outer or max_inner runs out first;
+ * with fallback = 1 the solve then continues with FP64 Jacobi-PCG on A64
+ * from the current x down to rtol (relative to ||b||), otherwise it
+ * returns DFX_ERR_NOCONV. */
 dfx_status dfx_mixed_solve(const dfx_csr* A64, const dfx_csr* A32,
                            const dfx_vec* b, dfx_vec* x,
                            const dfx_mixed_opts* opts, dfx_mixed_result* res);
@@ -485,6 +494,39 @@ dfx_status dfx_batch_step(dfx_batch* B, int32_t iterations, int32_t use_graph);
 dfx_status dfx_batch_status(dfx_batch* B, int32_t n, const int32_t* slots, int32_t* done,
                             int32_t* converged, int32_t* iterations,
                             double* rel_residual);
+/* Restart every system of the batch from x = 0 with the current values of
+ * its A and b (Jacobi diagonal, r = b, p = z, reductions and counters as
+ * dfx_batch_add; rtol and max_it as given there). Kernels only, driven by
+ * the batch's device tables, so it can be recorded into a graph and stays
+ * valid while systems join and leave. Asynchronous. */
+dfx_status dfx_batch_restart(dfx_batch* B);
+
+/* ------------------------------------------------------------------ */
+/* Graph capture of library calls (no counterpart in the reference).   */
+/* ------------------------------------------------------------------ */
+/* Everything the library launches on its stream between dfx_graph_begin
+ * and dfx_graph_end, from the calling thread, is recorded into one HIP
+ * graph instead of running; dfx_graph_launch replays it as one launch on
+ * the library stream. It is for fixed sequences replayed many times, such
+ * as many meshes assembled and solved at once:
+ *   zero the matrices, dfx_assemble_matrix_batched, the vectors and BCs,
+ *   dfx_batch_restart, dfx_batch_step(K iterations)
+ * with new coefficients copied in (dfx_form_set_coefficients) before each
+ * replay. A replay reads the device buffers the recorded calls used, so
+ * they must stay allocated and keep their addresses. Calls that need the
+ * host part-way (allocation, uploads of plans or tables, synchronisation:
+ * first-use set-up, solves that poll, status queries) fail with
+ * DFX_ERR_CAPTURE inside a capture: run the sequence once before
+ * capturing it, so that its plans exist. After a failed call the capture
+ * stays open until dfx_graph_end, which then discards it and reports the
+ * first failure. A replay repeats the recorded work as recorded: an
+ * assembly recorded after MatrixCSR::set(0) stores (A = a), one recorded
+ * without it adds (A += a) on every replay. */
+typedef struct dfx_graph dfx_graph;
+dfx_status dfx_graph_begin(void);
+dfx_status dfx_graph_end(dfx_graph** out);
+dfx_status dfx_graph_launch(const dfx_graph* g);
+dfx_status dfx_graph_destroy(dfx_graph* g);
 
 /* ------------------------------------------------------------------ */
 /* Timing on the library stream (hipEvent).                            */

@@ -78,12 +78,13 @@ def q1_128():
     gc.collect()
 
 
-@pytest.mark.parametrize("variant", [-1, 1, 4, 9, 10, 12, 13, 15])
+@pytest.mark.parametrize("variant", [-1, 1, 4, 9, 10, 12, 13, 15, 16])
 def test_q1_hex_128_fp64_matches_oracle(q1_128, variant):
     """variant -1: the autotuner's choice; 1 reference tensors in LDS, 4
     closed-form Q1 entries, 9 structured-box rows, 10 factors formed per
     incidence (no cell pass), 12/13 rows with their plan at fixed
-    positions, 15 structured rows through an LDS slice."""
+    positions, 15 structured rows through an LDS slice, 16 a team of 8
+    lanes per row."""
     p, ref = q1_128
     if variant >= 0:
         p.a.set_variant(variant)
@@ -129,7 +130,7 @@ def p3_24():
     gc.collect()
 
 
-@pytest.mark.parametrize("variant", [3, 7, 8])
+@pytest.mark.parametrize("variant", [3, 7, 8, 16])
 def test_p3_tet_24_matches_oracle(p3_24, variant):
     p, ref = p3_24
     p.a.set_variant(variant)

@@ -7,6 +7,7 @@ relative L2 of the oracle's FP64 Jacobi-PCG solved to 1e-12.
 
 import numpy as np
 import pytest
+import scipy.sparse as sps
 
 import oracle as O
 from dolfinx_amd import la, problems, solvers
@@ -84,6 +85,36 @@ def test_mixed_refinement_matches_fp64(cell, n, restart):
     assert res.converged and res.true_rel_residual <= 1e-10
     xr, _ = O.cg(A64.indptr, A64.indices, A64.data, 1, b.array, kmax=50000, rtol=1e-13)
     assert rel_err(x.array, xr) <= 1e-8
+
+
+def test_mixed_refinement_falls_back_to_fp64_when_it_stalls():
+    """A refinement step that does not halve the FP64 residual (inner_rtol
+    0.9 here) is a stall: with fallback the solve finishes in FP64
+    Jacobi-PCG from the current x and meets rtol against the FP64 matrix;
+    without it the solve raises."""
+    kw = dict(kappa="exp_normal", c=0.0, f="one")
+    n = (16, 14, 12)
+    p64 = problems.scalar_problem(n, O.HEX, dtype=np.float64, seed=7, **kw)
+    p32 = problems.scalar_problem(n, O.HEX, dtype=np.float32, seed=7, msh=p64.mesh, V=p64.V, **kw)
+    A64, b = problems.assemble_system(p64)
+    A32, _ = problems.assemble_system(p32)
+    x = la.Vector(p64.V.num_dofs)
+    res = solvers.mixed_refinement(A64, A32, b, x, rtol=1e-10, inner_rtol=0.9)
+    assert res.converged and res.true_rel_residual <= 1e-10
+    assert res.outer == 1 and res.fallback_iterations > 0
+    xr, _ = O.cg(A64.indptr, A64.indices, A64.data, 1, b.array, kmax=50000, rtol=1e-13)
+    assert rel_err(x.array, xr) <= 1e-8
+    # the FP64 residual of the returned x, recomputed on the host
+    Ah = sps.csr_matrix((A64.data, A64.indices, A64.indptr), shape=A64.shape)
+    r = b.array - Ah @ x.array
+    assert np.linalg.norm(r) <= 1.01e-10 * np.linalg.norm(b.array)
+    # the budget running out is a stall too
+    x2 = la.Vector(p64.V.num_dofs)
+    res2 = solvers.mixed_refinement(A64, A32, b, x2, rtol=1e-10, inner_rtol=1e-4, max_outer=1)
+    assert res2.converged and res2.outer == 1 and res2.fallback_iterations > 0
+    x3 = la.Vector(p64.V.num_dofs)
+    with pytest.raises(DfxError, match="did not converge"):
+        solvers.mixed_refinement(A64, A32, b, x3, rtol=1e-10, inner_rtol=0.9, fallback=False)
 
 
 def test_poisson_tet_32_end_to_end():
