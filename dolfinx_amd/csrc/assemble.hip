@@ -1760,18 +1760,19 @@ __global__ void __launch_bounds__(kBlock)
 /// Variant 16 (forms with reference tensors, K <= 32 nodes per cell): a
 /// team of K lanes per row, 64 / K teams per wavefront, no element matrix
 /// stored. The team walks its row's incidences (cell c, local row i) in
-/// cell order; lane j forms the entry of column j from the cell record
-/// (one record for the whole team) and the reference tensors in LDS, and
-/// adds it into the row's LDS slice at the slot of column j. The K slots
-/// of an incidence are distinct, so the team's adds need no atomics and
-/// every entry is summed in cell order, with the additions of variant 8.
-/// Each step's record and slot are loaded one step ahead; the first 2K
-/// incidences of a row are loaded together by the team and handed out by
-/// shuffles. Rows come ordered by incidence count inside chunks of 4096
-/// consecutive rows (trow), so the teams of a wavefront take similar
-/// numbers of steps.
+/// cell order; lane j forms the entry of column j from the incidence's
+/// cell factors and the reference tensors in LDS and adds it into the
+/// row's LDS slice at the slot of column j. The K slots of an incidence are
+/// distinct, so the team's adds need no atomics and every entry is summed
+/// in cell order, with the additions of variant 8. The cell factors of the
+/// row's incidences are loaded lane-parallel, K incidences at a time (lane
+/// j holds incidence bK + j of batch b), and handed to the team by
+/// shuffles; each step's slots are loaded one step ahead. The wavefronts of a
+/// workgroup (blockDim.x / 64 of them) share one copy of the tensors. Rows
+/// come ordered by incidence count inside chunks of 4096 consecutive rows
+/// (trow), so the teams of a wavefront take similar numbers of steps.
 template <typename T, typename O, int K>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(1024, 8)
     row_team_k(std::int32_t ntrow, const int4* __restrict__ trow,
                const std::int32_t* __restrict__ inc, const O* __restrict__ islots,
                const T* __restrict__ cd, const std::uint32_t* __restrict__ mask,
@@ -1780,15 +1781,17 @@ __global__ void __launch_bounds__(kBlock)
                std::int32_t maxrow)
 {
   constexpr int G = 64 / K;
+  static_assert(K <= 24, "the BC mask bits share a word with the local row");
   extern __shared__ unsigned char smem_raw[];
   T* S = reinterpret_cast<T*>(smem_raw); // reference tensors [7][K][K]
-  for (int t = threadIdx.x; t < 7 * K * K; t += kBlock)
+  for (int t = threadIdx.x; t < 7 * K * K; t += blockDim.x)
     S[t] = ref[t];
+  const int nw = blockDim.x >> 6;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int team = lane / K, j = lane - team * K;
   const bool member = team < G;
   const std::int64_t tq
-      = (std::int64_t(xcd_contiguous(blockIdx.x, gridDim.x)) * (kBlock / 64) + wave) * G + team;
+      = (std::int64_t(xcd_contiguous(blockIdx.x, gridDim.x)) * nw + wave) * G + team;
   int4 ro = make_int4(-1, 0, 0, 0);
   if (member && tq < ntrow)
     ro = trow[tq];
@@ -1796,17 +1799,33 @@ __global__ void __launch_bounds__(kBlock)
   T* a = S + 7 * K * K + (std::size_t(wave) * G + (member ? team : 0)) * maxrow;
   std::int32_t rbase = 0, rlen = 0;
   bool skip = true;
-  std::int32_t pre0 = 0, pre1 = 0;
   if (r >= 0)
   {
     rbase = rp[r];
     rlen = rp[r + 1] - rbase;
     skip = (bc && bc[r]) || n == 0;
-    if (n > 0)
+  }
+  // one batch of the row's incidences: lane j holds incidence q = b K + j
+  // (its cell factors, and its local row i above the BC mask bits)
+  T bg[7] = {};
+  std::uint32_t bmi = 0;
+  auto load_batch = [&](int b)
+  {
+    const int q = b * K + j;
+    if (q < n)
     {
-      pre0 = inc[p0 + min(j, n - 1)];
-      pre1 = inc[p0 + min(K + j, n - 1)];
+      const std::int32_t cik = inc[p0 + q];
+      const std::int32_t c = cik / K;
+      std::uint32_t mf;
+      load_factors<T>(cd, c, bg, mf);
+      bmi = (mask ? mask[c] : 0u) | (static_cast<std::uint32_t>(cik - c * K) << 24);
     }
+  };
+  O sl_next = 0;
+  if (!skip)
+  {
+    load_batch(0);
+    sl_next = islots[std::int64_t(p0) * K + j];
   }
   for (int t = j; t < rlen; t += K)
     a[t] = T(0);
@@ -1814,52 +1833,22 @@ __global__ void __launch_bounds__(kBlock)
   if (!skip)
   {
     const int tb = team * K;
-    auto cik_at = [&](int st) -> std::int32_t
+    for (int st = 0; st < n; ++st)
     {
-      if (st < 2 * K)
-      {
-        // both shuffles run on every lane of the team (same st)
-        const std::int32_t v0 = __shfl(pre0, tb + (st < K ? st : st - K), 64);
-        const std::int32_t v1 = __shfl(pre1, tb + (st < K ? st : st - K), 64);
-        return st < K ? v0 : v1;
-      }
-      return inc[p0 + st];
-    };
-    struct Stage
-    {
-      T g[7];
-      std::uint32_t m;
-      O sl;
-      int i;
-    };
-    auto fetch = [&](Stage& sg, int st)
-    {
-      const std::int32_t cik = cik_at(st);
-      const std::int32_t c = cik / K;
-      sg.i = cik - c * K;
-      std::uint32_t mf;
-      load_factors<T>(cd, c, sg.g, mf);
-      sg.m = mask ? mask[c] : 0u;
-      sg.sl = islots[std::int64_t(p0 + st) * K + j];
-    };
-    auto add = [&](const Stage& sg)
-    {
-      const T e = element_entry<T, K>(sg.g, S, sg.i, j);
-      if (!((sg.m >> j) & 1u))
-        a[sg.sl] += e;
-    };
-    Stage s0, s1;
-    fetch(s0, 0);
-    for (int st = 0; st < n; st += 2)
-    {
+      const int bq = st / K, src = tb + (st - bq * K);
+      if (st > 0 && st == bq * K)
+        load_batch(bq);
+      const O sl = sl_next;
       if (st + 1 < n)
-        fetch(s1, st + 1);
-      add(s0);
-      if (st + 1 >= n)
-        break;
-      if (st + 2 < n)
-        fetch(s0, st + 2);
-      add(s1);
+        sl_next = islots[std::int64_t(p0 + st + 1) * K + j];
+      T g[7];
+#pragma unroll
+      for (int u = 0; u < 7; ++u)
+        g[u] = __shfl(bg[u], src, 64);
+      const std::uint32_t mi = __shfl(bmi, src, 64);
+      const T e = element_entry<T, K>(g, S, static_cast<int>(mi >> 24), j);
+      if (!((mi >> j) & 1u))
+        a[sl] += e;
     }
   }
   // the team stores its own row (the whole slice: zeros for skipped rows)
@@ -4034,18 +4023,22 @@ void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
   }
   if (variant == 16)
   {
-    // teams of K lanes per row; 4 wavefronts per workgroup share the
-    // reference tensors
+    // teams of K lanes per row; the wavefronts of a workgroup (16, 8 or 4,
+    // the most whose slices fit 64 KB of LDS) share the reference tensors
     ensure_trow(f, A);
     constexpr int G = 64 / K;
-    const std::size_t lds
-        = (7 * std::size_t(K) * K + std::size_t(kBlock / 64) * G * A.max_row) * sizeof(T);
+    int nw = 16;
+    auto lds_for = [&](int w)
+    { return (7 * std::size_t(K) * K + std::size_t(w) * G * A.max_row) * sizeof(T); };
+    while (nw > 4 && lds_for(nw) > 64 * 1024)
+      nw /= 2;
+    const std::size_t lds = lds_for(nw);
     require(lds <= 64 * 1024, "row too long for the LDS slice");
     launch_cell_factors<T>(f, bc);
-    const std::int64_t per_block = std::int64_t(kBlock / 64) * G;
+    const std::int64_t per_block = std::int64_t(nw) * G;
     const int nb = static_cast<int>((std::int64_t(f.ntrow) + per_block - 1) / per_block);
     if (nb > 0)
-      row_team_k<T, O, K><<<nb, kBlock, lds, stream()>>>(
+      row_team_k<T, O, K><<<nb, 64 * nw, lds, stream()>>>(
           f.ntrow, reinterpret_cast<const int4*>(f.trow), f.inc, static_cast<const O*>(f.islots),
           static_cast<const T*>(f.cell_data), bc ? f.cell_mask : nullptr,
           bc ? bc->marker : nullptr, A.row_ptr, static_cast<const T*>(f.ref_tensors),
