@@ -14,13 +14,20 @@ Encoding::ASCII:
 - read_mesh: the Topology and Geometry of the named Grid, VTK node order
   back to DOLFINx order (io/cells.cpp:402-420, 534-569), then
   mesh::create_mesh.
-HDF5 data items need the HDF5 library, which this package does not use:
-Encoding.HDF5 raises.
+With Encoding.HDF5 (io/XDMFFile.cpp:39-56) the arrays go to an HDF5 file
+beside the XDMF file (same name, extension .h5, xdmf_utils::
+get_hdf5_filename) and the DataItems hold "file.h5:/path" with Format "HDF"
+(io/xdmf_utils.h:117-140): topology and geometry at /Mesh/<name>/topology
+and /Mesh/<name>/geometry (io/xdmf_mesh.cpp:152, 209, 235), function values
+at /Function/<name>/<t> with '.' in t replaced by '_'
+(io/xdmf_function.cpp:173-174, 209); topology as 64-bit integers, the rest
+as doubles. The HDF5 C library is loaded with ctypes (dolfinx_amd._hdf5).
 """
 
 from __future__ import annotations
 
 import enum
+import os
 import xml.etree.ElementTree as ET
 
 import numpy as np
@@ -51,13 +58,20 @@ def _fmt_time(t) -> str:
     return "%.17g" % float(t)
 
 
-def _data_item(parent, a, number_type=None):
+def _data_item(parent, a, number_type=None, h5=None, h5_path=None):
+    """A DataItem holding a (rows x columns): its values as text (XML), or,
+    with h5, a reference to the dataset h5_path written into that file."""
     a = np.asarray(a)
     a2 = a.reshape(a.shape[0], -1)
     item = ET.SubElement(parent, "DataItem")
     item.set("Dimensions", f"{a2.shape[0]} {a2.shape[1]}")
     if number_type:
         item.set("NumberType", number_type)
+    if h5 is not None:
+        item.set("Format", "HDF")
+        h5.write(h5_path, a2)
+        item.text = f"{os.path.basename(h5.path)}:{h5_path}"
+        return item
     item.set("Format", "XML")
     if np.issubdtype(a2.dtype, np.floating):
         rows = (" ".join(_fmt_float(v) for v in r) for r in a2)
@@ -67,27 +81,22 @@ def _data_item(parent, a, number_type=None):
     return item
 
 
-def _read_data_item(item, dtype):
-    fmt = item.get("Format", "XML")
-    if fmt != "XML":
-        raise NotImplementedError(f"DataItem Format {fmt}: only XML (ASCII) data is read")
-    dims = [int(d) for d in item.get("Dimensions").split()]
-    vals = np.array(item.text.split(), dtype=dtype)
-    return vals.reshape(dims if len(dims) > 1 else (dims[0], 1))
-
-
 class XDMFFile:
     """io::XDMFFile over one file, ASCII encoding; modes "r", "w", "a"."""
 
     Encoding = Encoding
 
     def __init__(self, filename, mode="r", encoding=Encoding.ASCII):
-        if encoding != Encoding.ASCII:
-            raise NotImplementedError(
-                "HDF5 data items need the HDF5 library; use XDMFFile.Encoding.ASCII")
         if mode not in ("r", "w", "a"):
             raise ValueError(f"file mode {mode!r}")
         self.filename, self.mode = str(filename), mode
+        self.encoding = Encoding(encoding)
+        self._h5 = None
+        if self.encoding == Encoding.HDF5:
+            from . import _hdf5
+
+            # xdmf_utils::get_hdf5_filename: the XDMF name with extension .h5
+            self._h5 = _hdf5.File(os.path.splitext(self.filename)[0] + ".h5", mode)
         if mode == "w":
             self._root = ET.Element("Xdmf", {"Version": "3.0", "xmlns:xi": _XI})
             ET.SubElement(self._root, "Domain")
@@ -106,6 +115,9 @@ class XDMFFile:
 
     def close(self):
         self._root = None
+        if self._h5 is not None:
+            self._h5.close()
+            self._h5 = None
 
     # -- writing --------------------------------------------------------
     def _domain(self):
@@ -133,9 +145,11 @@ class XDMFFile:
         topo = ET.SubElement(grid, "Topology", {"TopologyType": _TOPOLOGY[ct]})
         topo.set("NumberOfElements", str(dm.shape[0]))
         topo.set("NodesPerElement", str(dm.shape[1]))
-        _data_item(topo, dm[:, _VTK[ct]].astype(np.int64), "Int")
+        _data_item(topo, dm[:, _VTK[ct]].astype(np.int64), "Int", self._h5,
+                   f"/Mesh/{name}/topology")
         geo = ET.SubElement(grid, "Geometry", {"GeometryType": "XYZ"})
-        _data_item(geo, np.asarray(mesh.geometry.x, np.float64))
+        _data_item(geo, np.asarray(mesh.geometry.x, np.float64), None, self._h5,
+                   f"/Mesh/{name}/geometry")
         self._save()
 
     def write_function(self, u, t: float = 0.0, mesh_name: str | None = None) -> None:
@@ -161,10 +175,35 @@ class XDMFFile:
         attr = ET.SubElement(grid, "Attribute", {
             "Name": u.name, "AttributeType": {1: "Scalar", 3: "Vector"}.get(ncomp, "Matrix"),
             "Center": centre})
-        _data_item(attr, vals.reshape(-1, ncomp))
+        # io/xdmf_function.cpp:173-174: the time in the dataset path with
+        # '.' replaced by '_'
+        _data_item(attr, vals.reshape(-1, ncomp), None, self._h5,
+                   f"/Function/{u.name}/{_fmt_time(t).replace('.', '_')}")
         self._save()
 
     # -- reading --------------------------------------------------------
+    def _read_data_item(self, item, dtype):
+        fmt = item.get("Format", "XML")
+        dims = [int(d) for d in item.get("Dimensions").split()]
+        shape = dims if len(dims) > 1 else (dims[0], 1)
+        if fmt == "XML":
+            return np.array(item.text.split(), dtype=dtype).reshape(shape)
+        if fmt != "HDF":
+            raise NotImplementedError(f"DataItem Format {fmt}: XML and HDF data are read")
+        fname, path = item.text.strip().rsplit(":", 1)
+        h5name = os.path.join(os.path.dirname(os.path.abspath(self.filename)), fname)
+        if self._h5 is not None and os.path.abspath(self._h5.path) == h5name:
+            data = self._h5.read(path)
+        else:
+            from . import _hdf5
+
+            f = _hdf5.File(h5name, "r")
+            try:
+                data = f.read(path)
+            finally:
+                f.close()
+        return data.astype(dtype, copy=False).reshape(shape)
+
     def _grid(self, name):
         dom = self._root.find("Domain")
         g = None if dom is None else dom.find(f"Grid[@Name='{name}']")
@@ -180,7 +219,7 @@ class XDMFFile:
         if tt is None or tt.lower() not in types:
             raise NotImplementedError(f"TopologyType {tt}: tetrahedra and hexahedra are read")
         ct = types[tt.lower()]
-        cells = _read_data_item(topo.find("DataItem"), np.int64)
+        cells = self._read_data_item(topo.find("DataItem"), np.int64)
         return ct, np.ascontiguousarray(cells[:, _VTK[ct]])
 
     def read_geometry_data(self, name="mesh"):
@@ -188,7 +227,7 @@ class XDMFFile:
         gt = geo.get("GeometryType") or geo.get("Type")
         if gt != "XYZ":
             raise NotImplementedError(f"GeometryType {gt}: XYZ is read")
-        return _read_data_item(geo.find("DataItem"), np.float64).reshape(-1, 3)
+        return self._read_data_item(geo.find("DataItem"), np.float64).reshape(-1, 3)
 
     def read_mesh(self, name="mesh") -> Mesh:
         """mesh::create_mesh on the named Grid's cells and points."""

@@ -1,5 +1,6 @@
-"""XDMF (ASCII) input and output, io::XDMFFile with Encoding::ASCII
-(cpp/dolfinx/io/XDMFFile.cpp, io/xdmf_mesh.cpp, io/xdmf_function.cpp):
+"""XDMF input and output, io::XDMFFile with Encoding::ASCII and
+Encoding::HDF5 (cpp/dolfinx/io/XDMFFile.cpp, io/xdmf_mesh.cpp,
+io/xdmf_function.cpp, io/xdmf_utils.h):
 the reference's own test meshes (python/test/data/create_unit_cube_*.xdmf,
 read by python/test/unit/fem/test_fem_pipeline.py:457-470) read into the
 same host setup as the oracle's, written meshes and functions read back.
@@ -70,11 +71,15 @@ def test_hexahedron_fixture_is_read_in_dolfinx_order():
         assert np.all(x[c[:4], 2] < x[c[4:], 2])
 
 
+ENCODINGS = [io.XDMFFile.Encoding.ASCII, io.XDMFFile.Encoding.HDF5]
+
+
+@pytest.mark.parametrize("enc", ENCODINGS)
 @pytest.mark.parametrize("ct", [mesh.CellType.tetrahedron, mesh.CellType.hexahedron])
-def test_write_mesh_and_read_back(ct, tmp_path):
+def test_write_mesh_and_read_back(ct, enc, tmp_path):
     m = mesh.create_box([[0.0, 0.0, 0.0], [1.0, 2.0, 0.5]], (3, 2, 2), ct)
     fn = tmp_path / "m.xdmf"
-    with io.XDMFFile(fn, "w") as f:
+    with io.XDMFFile(fn, "w", enc) as f:
         f.write_mesh(m)
     root = ET.parse(fn).getroot()
     assert root.tag == "Xdmf" and root.get("Version") == "3.0"
@@ -97,13 +102,15 @@ def test_write_mesh_and_read_back(ct, tmp_path):
 
 
 @pytest.mark.gpu
-def test_write_function_time_series(tmp_path):
-    """Function values live in device vectors: this test needs the GPU."""
+@pytest.mark.parametrize("enc", ENCODINGS)
+def test_write_function_time_series(enc, tmp_path):
+    """Function values live in device vectors: this test needs the GPU.
+    With Encoding::HDF5 the values sit at /Function/u/<t> ('.' -> '_')."""
     m = mesh.create_box([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], (2, 3, 2), mesh.CellType.tetrahedron)
     V = fem.functionspace(m, ("Lagrange", 1))
     u = fem.Function(V, name="u")
     fn = tmp_path / "u.xdmf"
-    with io.XDMFFile(fn, "w") as f:
+    with io.XDMFFile(fn, "w", enc) as f:
         f.write_mesh(m)
         for t in (0.0, 0.5):
             u.interpolate(lambda X: 1.0 + X[0] + t * X[1] * X[2])
@@ -119,19 +126,80 @@ def test_write_function_time_series(tmp_path):
     for s, t in zip(steps, (0.0, 0.5)):
         a = s.find("Attribute")
         assert a.get("Center") == "Node" and a.get("AttributeType") == "Scalar"
-        vals = np.array(a.find("DataItem").text.split(), float)
+        item = a.find("DataItem")
+        if enc == io.XDMFFile.Encoding.HDF5:
+            from dolfinx_amd import _hdf5
+
+            path = "/Function/u/" + ("0" if t == 0.0 else "0_5")
+            assert item.get("Format") == "HDF" and item.text == "u.h5:" + path
+            h5 = _hdf5.File(tmp_path / "u.h5", "r")
+            vals = h5.read(path).ravel()
+            h5.close()
+        else:
+            vals = np.array(item.text.split(), float)
         assert np.allclose(vals, 1.0 + x[:, 0] + t * x[:, 1] * x[:, 2], rtol=1e-15, atol=1e-15)
     # appending a step keeps the file valid
-    with io.XDMFFile(fn, "a") as f:
+    with io.XDMFFile(fn, "a", enc) as f:
         f.write_function(u, 1.0)
     assert len(ET.parse(fn).getroot().findall("Domain/Grid[@Name='u']/Grid")) == 3
     P2 = fem.functionspace(m, ("Lagrange", 2))
-    with io.XDMFFile(tmp_path / "p2.xdmf", "w") as f:
+    with io.XDMFFile(tmp_path / "p2.xdmf", "w", enc) as f:
         f.write_mesh(m)
         with pytest.raises(RuntimeError, match="Degree of output Function"):
             f.write_function(fem.Function(P2, name="w"))
 
 
-def test_hdf5_encoding_raises(tmp_path):
-    with pytest.raises(NotImplementedError, match="ASCII"):
-        io.XDMFFile(tmp_path / "h.xdmf", "w", io.XDMFFile.Encoding.HDF5)
+@pytest.mark.parametrize("ct", [mesh.CellType.tetrahedron, mesh.CellType.hexahedron])
+def test_hdf5_items_and_datasets(ct, tmp_path):
+    """Encoding::HDF5: the DataItems reference "m.h5:/Mesh/mesh/topology"
+    and ".../geometry" with Format "HDF" (io/xdmf_utils.h:117-128,
+    io/xdmf_mesh.cpp:152, 209, 235); the datasets hold the same arrays as
+    the ASCII items, topology as 64-bit integers; a second mesh goes under
+    its own name in the same file, and reading either file gives the same
+    topology and geometry."""
+    from dolfinx_amd import _hdf5
+
+    m = mesh.create_box([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], (2, 3, 4), ct)
+    m2 = mesh.create_box([[0.0, 0.0, 0.0], [2.0, 1.0, 1.0]], (1, 1, 1), ct)
+    m2.name = "coarse"
+    with io.XDMFFile(tmp_path / "m.xdmf", "w", io.XDMFFile.Encoding.HDF5) as f:
+        f.write_mesh(m)
+        f.write_mesh(m2)
+    with io.XDMFFile(tmp_path / "a.xdmf", "w") as f:
+        f.write_mesh(m)
+    root = ET.parse(tmp_path / "m.xdmf").getroot()
+    items = root.findall("Domain/Grid[@Name='mesh']/*/DataItem")
+    assert [i.get("Format") for i in items] == ["HDF", "HDF"]
+    assert [i.text for i in items] == ["m.h5:/Mesh/mesh/topology", "m.h5:/Mesh/mesh/geometry"]
+    assert items[0].get("NumberType") == "Int"
+    nn = 4 if ct == mesh.CellType.tetrahedron else 8
+    assert items[0].get("Dimensions") == f"{m.topology.cells.shape[0]} {nn}"
+    h5 = _hdf5.File(tmp_path / "m.h5", "r")
+    topo, geo = h5.read("/Mesh/mesh/topology"), h5.read("/Mesh/mesh/geometry")
+    assert topo.dtype == np.int64 and geo.dtype == np.float64
+    assert h5.exists("/Mesh/coarse/topology") and h5.read("/Mesh/coarse/geometry").shape == (8, 3)
+    h5.close()
+    ascii_items = ET.parse(tmp_path / "a.xdmf").getroot().findall(
+        "Domain/Grid[@Name='mesh']/*/DataItem")
+    assert np.array_equal(topo.ravel(), np.array(ascii_items[0].text.split(), np.int64))
+    assert np.array_equal(geo.ravel(), np.array(ascii_items[1].text.split(), np.float64))
+    with io.XDMFFile(tmp_path / "m.xdmf", "r") as f, io.XDMFFile(tmp_path / "a.xdmf", "r") as g:
+        assert np.array_equal(f.read_topology_data()[1], g.read_topology_data()[1])
+        assert np.array_equal(f.read_geometry_data(), g.read_geometry_data())
+        c = f.read_mesh(name="coarse")
+    assert c.topology.cells.shape[0] == m2.topology.cells.shape[0]
+
+
+def test_hdf5_reference_fixture_round_trip(tmp_path):
+    """The reference's hexahedron fixture written with Encoding::HDF5 and
+    read back gives the setup of the ASCII fixture, bit for bit."""
+    pm, cells, x = read_fixture(O.HEX)
+    with io.XDMFFile(tmp_path / "f.xdmf", "w", io.XDMFFile.Encoding.HDF5) as f:
+        f.write_mesh(pm)
+    with io.XDMFFile(tmp_path / "f.xdmf", "r") as f:
+        m2 = f.read_mesh(name="Grid")  # the fixture's grid name, kept by write_mesh
+    # the written mesh lists cells in pm's numbering: reading it is
+    # create_mesh on the same cells and points
+    om = O.Mesh.from_cells(O.HEX, np.asarray(pm.geometry.dofmap, np.int64), pm.geometry.x)
+    assert np.array_equal(m2.geometry.x, om.x)
+    assert np.array_equal(m2.topology.cells, om.cells)
