@@ -1863,6 +1863,122 @@ __global__ void __launch_bounds__(1024, 8)
   }
 }
 
+/// Variant 17: the rows of variant 16 (a team of K lanes per row, rows in
+/// trow order) gathering the element matrices E of variant 7 instead of
+/// forming entries: lane j of the team adds E[c][i][j] of each incidence
+/// (c, i) into the row's slice at the slot of column j, in cell order (the
+/// additions of variant 7). A step's element value, slot and mask bit are
+/// loaded four steps ahead; the first 2K incidences of the row are loaded
+/// lane-parallel and handed out by shuffles.
+template <typename T, typename O, int K>
+__global__ void __launch_bounds__(1024, 8)
+    row_team_gather_k(std::int32_t ntrow, const int4* __restrict__ trow,
+                      const std::int32_t* __restrict__ inc, const O* __restrict__ islots,
+                      const T* __restrict__ elem, const std::uint32_t* __restrict__ mask,
+                      const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
+                      T* __restrict__ vals, int accumulate, std::int32_t maxrow)
+{
+  constexpr int G = 64 / K;
+  extern __shared__ unsigned char smem_raw[];
+  const int nw = blockDim.x >> 6;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int team = lane / K, j = lane - team * K;
+  const bool member = team < G;
+  const std::int64_t tq
+      = (std::int64_t(xcd_contiguous(blockIdx.x, gridDim.x)) * nw + wave) * G + team;
+  int4 ro = make_int4(-1, 0, 0, 0);
+  if (member && tq < ntrow)
+    ro = trow[tq];
+  const std::int32_t r = ro.x, p0 = ro.y, n = ro.z;
+  T* a = reinterpret_cast<T*>(smem_raw) + (std::size_t(wave) * G + (member ? team : 0)) * maxrow;
+  std::int32_t rbase = 0, rlen = 0;
+  bool skip = true;
+  std::int32_t pre0 = 0, pre1 = 0;
+  if (r >= 0)
+  {
+    rbase = rp[r];
+    rlen = rp[r + 1] - rbase;
+    skip = (bc && bc[r]) || n == 0;
+    if (!skip)
+    {
+      pre0 = inc[p0 + min(j, n - 1)];
+      pre1 = inc[p0 + min(K + j, n - 1)];
+    }
+  }
+  for (int t = j; t < rlen; t += K)
+    a[t] = T(0);
+  __syncthreads();
+  if (!skip)
+  {
+    const int tb = team * K;
+    // incidence st of the row (every lane of the team calls it with the
+    // same st, so the shuffles read lanes of the team only)
+    auto cik_at = [&](int st) -> std::int32_t
+    {
+      if (st < 2 * K)
+      {
+        const int src = tb + (st < K ? st : st - K);
+        const std::int32_t v0 = __shfl(pre0, src, 64);
+        const std::int32_t v1 = __shfl(pre1, src, 64);
+        return st < K ? v0 : v1;
+      }
+      return inc[p0 + st];
+    };
+    struct Step
+    {
+      T e;
+      O sl;
+      std::uint32_t off; // 1: column j of this incidence is a Dirichlet dof
+    };
+    auto fetch = [&](Step& q, int st)
+    {
+      if (st < n)
+      {
+        const std::int32_t cik = cik_at(st);
+        q.e = elem[std::int64_t(cik) * K + j];
+        q.sl = islots[std::int64_t(p0 + st) * K + j];
+        q.off = mask ? (mask[cik / K] >> j) & 1u : 0u;
+      }
+    };
+    auto add = [&](const Step& q)
+    {
+      if (!q.off)
+        a[q.sl] += q.e;
+    };
+    Step q0{}, q1{}, q2{}, q3{};
+    fetch(q0, 0);
+    fetch(q1, 1);
+    fetch(q2, 2);
+    fetch(q3, 3);
+    for (int st = 0; st < n; st += 4)
+    {
+      add(q0);
+      fetch(q0, st + 4);
+      if (st + 1 >= n)
+        break;
+      add(q1);
+      fetch(q1, st + 5);
+      if (st + 2 >= n)
+        break;
+      add(q2);
+      fetch(q2, st + 6);
+      if (st + 3 >= n)
+        break;
+      add(q3);
+      fetch(q3, st + 7);
+    }
+  }
+  if (r >= 0)
+  {
+    if (accumulate)
+      for (int t = j; t < rlen; t += K)
+        vals[rbase + t] += a[t];
+    else
+      for (int t = j; t < rlen; t += K)
+        __builtin_nontemporal_store(a[t], vals + rbase + t);
+  }
+}
+
 /// BC column masks of element-tensor forms: bit j of mask[c] is set when
 /// node j of cell c is a Dirichlet dof.
 __global__ void __launch_bounds__(kBlock)
@@ -3557,7 +3673,7 @@ int incidence_lanes(const dfx_form& f)
 /// 7, 8 incidence-parallel gathers; 9 structured Q1; 10 fused geometry;
 /// 11 as 1 with eight incidences in flight; 12..14 fixed-width Q1 rows;
 /// 15 structured Q1 rows in registers stored through an LDS slice.
-constexpr int kRowVariants = 17;
+constexpr int kRowVariants = 18;
 
 template <typename T, typename O, int K, int L>
 void launch_lanes(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
@@ -4018,6 +4134,38 @@ void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
   if (variant == 7 || variant == 8)
   {
     launch_gather<T, O, K>(f, A, bc, variant == 7);
+    check_launch("row_assemble");
+    return;
+  }
+  if (variant == 17)
+  {
+    // teams of K lanes per row gathering the element matrices of the
+    // matrix-core product
+    ensure_trow(f, A);
+    constexpr int G = 64 / K;
+    int nw = 16;
+    auto lds_for = [&](int w) { return std::size_t(w) * G * A.max_row * sizeof(T); };
+    while (nw > 4 && lds_for(nw) > 64 * 1024)
+      nw /= 2;
+    const std::size_t lds = lds_for(nw);
+    require(lds <= 64 * 1024, "row too long for the LDS slice");
+    launch_cell_factors<T>(f, bc);
+    const std::int32_t nc = f.d.num_cells;
+    if (!f.cell_elem)
+      f.cell_elem = dmalloc(sizeof(T) * std::max<std::int64_t>(std::int64_t(nc) * K * K, 1));
+    const int nbe = static_cast<int>((std::int64_t(nc) + 63) / 64);
+    if (nbe > 0)
+      elem_mfma_k<T, K><<<nbe, kBlock, 0, stream()>>>(
+          nc, static_cast<const T*>(f.cell_data), static_cast<const T*>(f.ref_tensors),
+          static_cast<T*>(f.cell_elem));
+    const std::int64_t per_block = std::int64_t(nw) * G;
+    const int nb = static_cast<int>((std::int64_t(f.ntrow) + per_block - 1) / per_block);
+    if (nb > 0)
+      row_team_gather_k<T, O, K><<<nb, 64 * nw, lds, stream()>>>(
+          f.ntrow, reinterpret_cast<const int4*>(f.trow), f.inc, static_cast<const O*>(f.islots),
+          static_cast<const T*>(f.cell_elem), bc ? f.cell_mask : nullptr,
+          bc ? bc->marker : nullptr, A.row_ptr, static_cast<T*>(A.values), A.zeroed ? 0 : 1,
+          A.max_row);
     check_launch("row_assemble");
     return;
   }
@@ -5422,7 +5570,7 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t*
         f.row_variant = best;
         // release the plans of the variants not kept (the element matrices
         // of variant 7 alone are k^2 values per cell)
-        if (best != 7)
+        if (best != 7 && best != 17)
         {
           dfree(f.cell_elem);
           f.cell_elem = nullptr;
@@ -5453,7 +5601,7 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc, int32_t*
           f.struct_pattern = 0;
           f.struct_ok = -1;
         }
-        if (best != 16)
+        if (best != 16 && best != 17)
         {
           dfree(f.trow);
           f.trow = nullptr;

@@ -62,3 +62,17 @@ def test_csr_pattern_rows_must_be_sorted():
         la.MatrixCSR(rp, np.array([0, 0, 0, 1], np.int32))   # duplicate column
     with pytest.raises(_lib.DfxError, match="column index out of range"):
         la.MatrixCSR(rp, np.array([0, 2, 0, 1], np.int32), ncols=2)
+
+
+def test_graph_calls_outside_a_capture_are_refused():
+    """dfx_graph_end without dfx_graph_begin and a launch of no graph fail
+    before touching a device; destroying no graph is a no-op."""
+    import ctypes as C
+    import pytest
+
+    h = C.c_void_p()
+    with pytest.raises(_lib.DfxError, match="no graph capture is open"):
+        _lib.call("dfx_graph_end", C.byref(h))
+    with pytest.raises(_lib.DfxError, match="null graph"):
+        _lib.call("dfx_graph_launch", None)
+    assert _lib.lib().dfx_graph_destroy(None) == _lib.DFX_OK

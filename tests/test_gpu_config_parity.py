@@ -130,7 +130,7 @@ def p3_24():
     gc.collect()
 
 
-@pytest.mark.parametrize("variant", [3, 7, 8, 16])
+@pytest.mark.parametrize("variant", [3, 7, 8, 16, 17])
 def test_p3_tet_24_matches_oracle(p3_24, variant):
     p, ref = p3_24
     p.a.set_variant(variant)

@@ -102,12 +102,13 @@ def oracle_p3(p, n, c):
 
 
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 5e-6)])
-@pytest.mark.parametrize("variant", [3, 7, 8, 16])
+@pytest.mark.parametrize("variant", [3, 7, 8, 16, 17])
 def test_p3_matches_oracle(dtype, tol, variant):
     """P3 variants: 3 one incidence per lane, 7 element matrices as one
     batched product on the matrix cores then gathered by rows, 8
     incidence-parallel gather from the cell factors, 16 a team of 20 lanes
-    per row forming the entries from the cell factors."""
+    per row forming the entries from the cell factors, 17 such teams
+    gathering the element matrices of variant 7."""
     n = (3, 2, 2)
     p = problems.p3_problem(n, seed=8, dtype=dtype)
     p.a.set_variant(variant)
@@ -119,7 +120,7 @@ def test_p3_matches_oracle(dtype, tol, variant):
     assert rel_err(b.array, br) <= tol
 
 
-@pytest.mark.parametrize("variant", [3, 7, 8, 16])
+@pytest.mark.parametrize("variant", [3, 7, 8, 16, 17])
 def test_p3_solve_matches_oracle(variant):
     n = (4, 4, 3)
     p = problems.p3_problem(n, seed=9)
