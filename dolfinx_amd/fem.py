@@ -281,7 +281,9 @@ class Form:
         rows in registers, 10 factors formed from the geometry per
         incidence, 11 as 1 with eight incidences in flight, 12-14 Q1 rows
         with their plan at fixed positions, 15 structured Q1 rows in
-        registers stored through an LDS slice)."""
+        registers stored through an LDS slice, 16 a team of lanes per row
+        forming the entries, 17 such teams gathering the element matrices
+        of 7)."""
         _lib.call("dfx_form_set_variant", self._h, int(variant))
 
     @property

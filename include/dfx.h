@@ -337,8 +337,11 @@ dfx_status dfx_form_autotune(dfx_form* a, dfx_csr* A, const dfx_bc* bc,
  * pass; affine P1/Q1); 11 as 1 with eight incidences in flight; 12-14 Q1
  * hexahedra whose vertices lie in at most 8 cells: one thread per row with
  * its plan at fixed positions, 64/128/256 rows per workgroup; 15 as 9 with
- * the rows of a workgroup stored through an LDS slice. The P2
- * elasticity form has a single row kernel and ignores the variant. */
+ * the rows of a workgroup stored through an LDS slice; 16 a team of lanes
+ * per row (lane j: column j of each incidence) forming the entries from the
+ * cell factors; 17 such teams gathering the matrix-core element matrices
+ * of 7. The P2 elasticity form has a single row kernel and ignores the
+ * variant. */
 dfx_status dfx_form_set_variant(dfx_form* a, int32_t variant);
 /* fem::assemble_matrix / fem::assemble_vector with a custom kernel
  * (cpp/demo/custom_kernel/main.cpp:73-118): the form (kernel
