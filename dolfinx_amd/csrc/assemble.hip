@@ -2801,6 +2801,77 @@ __global__ void __launch_bounds__(kBlock)
 /// Element vectors of the source forms, one thread per cell:
 /// P1 tetrahedra b_i = |K|/20 (f_i + sum f); affine Q1 hexahedra
 /// b_i = |det J| sum_j M_ij f_j; other hexahedra 2x2x2 Gauss.
+/// Element source vector of an affine cell from its vertices q0..q3 (P1:
+/// the four vertices; Q1: vertices 0, 1, 2, 4) and the source values at its
+/// NV nodes: P1 b_i = |K|/20 (f_i + sum f), Q1 b_i = |det J| sum_j M_ij f_j.
+template <typename T, int NV>
+__device__ inline void affine_source(const T* q0, const T* q1, const T* q2, const T* q3,
+                                     const T (&f)[NV], T (&out)[NV])
+{
+  T J[9], adj[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+  {
+    J[3 * r + 0] = q1[r] - q0[r];
+    J[3 * r + 1] = q2[r] - q0[r];
+    J[3 * r + 2] = q3[r] - q0[r];
+  }
+  const T det = adjugate(J, adj);
+  const T adet = det < T(0) ? -det : det;
+  if constexpr (NV == 4)
+  {
+    const T sum = f[0] + f[1] + f[2] + f[3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      out[i] = adet / T(120) * (sum + f[i]);
+  }
+  else
+  {
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+    {
+      T a = 0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+      {
+        T m = 1;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          m *= (((i >> d) & 1) == ((j >> d) & 1)) ? T(1) / T(3) : T(1) / T(6);
+        a += m * f[j];
+      }
+      out[i] = adet * a;
+    }
+  }
+}
+
+/// The NV source values of cell c, with vector loads when the coefficient
+/// rows are 16-byte aligned.
+template <typename T, int NV>
+__device__ inline void load_source(const T* __restrict__ coeffs, int cstride, std::int64_t c,
+                                   T (&f)[NV])
+{
+  const T* fp = coeffs + c * cstride;
+  if ((cstride * sizeof(T)) % 16 == 0)
+  {
+    using T2 = vec2_t<T>;
+    const T2* f2 = reinterpret_cast<const T2*>(fp);
+#pragma unroll
+    for (int h = 0; h < NV / 2; ++h)
+    {
+      const T2 w = f2[h];
+      f[2 * h] = w.x;
+      f[2 * h + 1] = w.y;
+    }
+  }
+  else
+  {
+#pragma unroll
+    for (int h = 0; h < NV; ++h)
+      f[h] = fp[h];
+  }
+}
+
 template <typename T, int NV>
 __global__ void __launch_bounds__(kBlock)
     cell_source_k(std::int32_t nc, const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
@@ -2809,51 +2880,25 @@ __global__ void __launch_bounds__(kBlock)
   const std::int64_t c = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
   if (c >= nc)
     return;
-  const std::int32_t* v = xdm + NV * c;
-  const T* f = coeffs + c * cstride;
-  T out[NV];
-  if (affine)
+  // the cell's vertex row (NV ints, 16-byte aligned) and its NV source
+  // values with vector loads when the coefficient rows are 16-byte aligned
+  std::int32_t v[NV];
   {
-    const T* q0 = x + 3 * std::int64_t(v[0]);
-    const T* q1 = x + 3 * std::int64_t(v[1]);
-    const T* q2 = x + 3 * std::int64_t(v[2]);
-    const T* q3 = x + 3 * std::int64_t(v[NV == 4 ? 3 : 4]);
-    T J[9], adj[9];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
+    const int4 a4 = *reinterpret_cast<const int4*>(xdm + NV * c);
+    v[0] = a4.x; v[1] = a4.y; v[2] = a4.z; v[3] = a4.w;
+    if constexpr (NV == 8)
     {
-      J[3 * r + 0] = q1[r] - q0[r];
-      J[3 * r + 1] = q2[r] - q0[r];
-      J[3 * r + 2] = q3[r] - q0[r];
-    }
-    const T det = adjugate(J, adj);
-    const T adet = det < T(0) ? -det : det;
-    if constexpr (NV == 4)
-    {
-      const T sum = f[0] + f[1] + f[2] + f[3];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        out[i] = adet / T(120) * (sum + f[i]);
-    }
-    else
-    {
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-      {
-        T a = 0;
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-        {
-          T m = 1;
-#pragma unroll
-          for (int d = 0; d < 3; ++d)
-            m *= (((i >> d) & 1) == ((j >> d) & 1)) ? T(1) / T(3) : T(1) / T(6);
-          a += m * f[j];
-        }
-        out[i] = adet * a;
-      }
+      const int4 b4 = *reinterpret_cast<const int4*>(xdm + NV * c + 4);
+      v[4] = b4.x; v[5] = b4.y; v[6] = b4.z; v[7] = b4.w;
     }
   }
+  T f[NV];
+  load_source<T, NV>(coeffs, cstride, c, f);
+  T out[NV];
+  if (affine)
+    affine_source<T, NV>(x + 3 * std::int64_t(v[0]), x + 3 * std::int64_t(v[1]),
+                         x + 3 * std::int64_t(v[2]), x + 3 * std::int64_t(v[NV == 4 ? 3 : 4]),
+                         f, out);
   else
   {
     T cdv[8][3];
@@ -2887,6 +2932,88 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int i = 0; i < NV; ++i)
     be[std::int64_t(i) * nc + c] = out[i];
+}
+
+/// One pass over the cells of a system whose bilinear and linear forms
+/// share their geometry (dfx_assemble_system with an affine P1/Q1
+/// stiffness form and the P1/Q1 source form on the same mesh): the cell
+/// factor record and BC mask of the matrix, as cell_factors_k, and the
+/// element source vector, as cell_source_k, from one load of the cell's
+/// vertex row and coordinates.
+template <typename T, int NV>
+__global__ void __launch_bounds__(kBlock)
+    cell_system_k(std::int32_t nc, const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
+                  const T* __restrict__ coeffs, int cstride, T creac,
+                  const std::int8_t* __restrict__ bc, T* __restrict__ cd,
+                  std::uint32_t* __restrict__ mask, const T* __restrict__ src, int sstride,
+                  T* __restrict__ be)
+{
+  const std::int64_t c
+      = xcd_contiguous(blockIdx.x, gridDim.x) * std::int64_t(kBlock) + threadIdx.x;
+  if (c >= nc)
+    return;
+  std::int32_t v[NV];
+  {
+    const int4 a = *reinterpret_cast<const int4*>(xdm + NV * c);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    if constexpr (NV == 8)
+    {
+      const int4 b = *reinterpret_cast<const int4*>(xdm + NV * c + 4);
+      v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    }
+  }
+  T f[NV];
+  load_source<T, NV>(src, sstride, c, f);
+  std::uint32_t m = 0;
+  if (mask)
+  {
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+      if (bc[v[j]])
+        m |= 1u << j;
+  }
+  const T* q0 = x + 3 * std::int64_t(v[0]);
+  const T* q1 = x + 3 * std::int64_t(v[1]);
+  const T* q2 = x + 3 * std::int64_t(v[2]);
+  const T* q3 = x + 3 * std::int64_t(v[NV == 4 ? 3 : 4]);
+  T g[7];
+  affine_factors<T>(q0, q1, q2, q3, coeffs[c * cstride], creac, g);
+  T out[NV];
+  affine_source<T, NV>(q0, q1, q2, q3, f, out);
+  if (mask)
+    mask[c] = m;
+  const T g7 = mask_bits<T>(m);
+  if constexpr (sizeof(T) == 8)
+  {
+    double2* G = reinterpret_cast<double2*>(cd + 8 * c);
+    G[0] = make_double2(g[0], g[1]);
+    G[1] = make_double2(g[2], g[3]);
+    G[2] = make_double2(g[4], g[5]);
+    G[3] = make_double2(g[6], g7);
+  }
+  else
+  {
+    float4* G = reinterpret_cast<float4*>(cd + 8 * c);
+    G[0] = make_float4(g[0], g[1], g[2], g[3]);
+    G[1] = make_float4(g[4], g[5], g[6], g7);
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+    be[std::int64_t(i) * nc + c] = out[i];
+}
+
+/// *diff = 1 if a[0..n) and b[0..n) differ in any byte (32-bit words).
+__global__ void __launch_bounds__(kBlock)
+    words_differ_k(std::int64_t n, const std::uint32_t* __restrict__ a,
+                   const std::uint32_t* __restrict__ b, int* diff)
+{
+  for (std::int64_t i = blockIdx.x * std::int64_t(kBlock) + threadIdx.x; i < n;
+       i += std::int64_t(gridDim.x) * kBlock)
+    if (a[i] != b[i])
+    {
+      *diff = 1;
+      return;
+    }
 }
 
 /// b[BS r + comp] += sum of the element vector entries of the row's
@@ -3636,6 +3763,12 @@ void ensure_plan(dfx_form& f, const dfx_csr& A)
 template <typename T>
 void launch_cell_factors(const dfx_form& f, const dfx_bc* bc)
 {
+  // done by the system cell pass of this assembly
+  if (f.cell_data_ready)
+  {
+    f.cell_data_ready = false;
+    return;
+  }
   const int nv = f.d.nodes_per_cell;
   const std::int32_t nc = f.d.num_cells;
   const T creac = static_cast<T>(f.constants[0]);
@@ -4517,7 +4650,9 @@ void vector_rows(const dfx_form& f, T* b)
     check_launch("assemble_vector");
     return;
   }
-  if (nv == 4)
+  if (f.cell_vec_ready)
+    f.cell_vec_ready = false; // element vectors from the system cell pass
+  else if (nv == 4)
     cell_source_k<T, 4><<<blocks_for(nc), kBlock, 0, stream()>>>(
         nc, static_cast<const T*>(f.x), f.x_dofmap, static_cast<const T*>(f.coeffs),
         f.d.cstride, 1, be);
@@ -5069,6 +5204,7 @@ dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out)
               f->ref_tensors = upload_as(w.data(), static_cast<std::int64_t>(w.size()), d.dtype);
           }
         }
+        f->id = next_pattern_id();
         *out = f.release();
       });
 }
@@ -5905,6 +6041,83 @@ dfx_status dfx_bc_set(dfx_vec* b, const dfx_bc* bc, const dfx_vec* x0, double al
       });
 }
 
+} // extern "C"
+
+namespace dfx
+{
+namespace
+{
+/// The system cell pass (cell_system_k) for a stiffness form a and source
+/// form L that share their geometry: returns false (nothing launched) when
+/// the pair does not qualify. Whether the device geometry of the two forms
+/// is the same is checked once per pair (one comparison pass and a host
+/// round trip), then remembered by form id.
+template <typename T>
+void launch_cell_system(const dfx_form& a, const dfx_form& L, const dfx_bc* bc)
+{
+  const std::int32_t nc = a.d.num_cells;
+  auto go = [&](auto kern)
+  {
+    kern<<<blocks_for(nc), kBlock, 0, stream()>>>(
+        nc, static_cast<const T*>(a.x), a.x_dofmap, static_cast<const T*>(a.coeffs),
+        a.d.cstride, static_cast<T>(a.constants[0]), bc ? bc->marker : nullptr,
+        static_cast<T*>(a.cell_data), bc ? a.cell_mask : nullptr,
+        static_cast<const T*>(L.coeffs), L.d.cstride, static_cast<T*>(L.cell_vec));
+  };
+  if (a.d.nodes_per_cell == 4)
+    go(cell_system_k<T, 4>);
+  else
+    go(cell_system_k<T, 8>);
+  check_launch("cell_system");
+}
+
+bool system_cell_pass(dfx_form& a, dfx_form& L, const dfx_bc* bc)
+{
+  const int ka = a.d.kernel, kl = L.d.kernel;
+  const bool pair = (ka == DFX_K_P1TET_A && kl == DFX_K_P1TET_L)
+                    || (ka == DFX_K_Q1HEX_A && kl == DFX_K_Q1HEX_L);
+  if (!pair || !a.affine || !L.affine || a.d.dtype != L.d.dtype || a.d.bs != 1 || L.d.bs != 1
+      || a.d.num_cells != L.d.num_cells || a.d.num_nodes != L.d.num_nodes
+      || a.d.nodes_per_cell != L.d.nodes_per_cell || a.dofmap != a.x_dofmap
+      || !row_path(a) || a.row_variant == 10 || a.d.num_cells == 0 || !a.cell_data
+      || !L.cell_vec || (bc && !a.cell_mask))
+    return false;
+  if (a.same_geometry_as != L.id)
+  {
+    if (capturing())
+      return false;
+    const std::size_t ts = dtype_size(a.d.dtype);
+    int* diff = static_cast<int*>(dmalloc(sizeof(int)));
+    fill_i32_k<<<1, kBlock, 0, stream()>>>(diff, 1, 0);
+    const std::int64_t nx = std::int64_t(a.d.num_nodes) * 3 * ts / 4;
+    const std::int64_t nd = std::int64_t(a.d.num_cells) * a.d.nodes_per_cell;
+    if (nx > 0)
+      words_differ_k<<<blocks_for(nx), kBlock, 0, stream()>>>(
+          nx, static_cast<const std::uint32_t*>(a.x), static_cast<const std::uint32_t*>(L.x), diff);
+    words_differ_k<<<blocks_for(nd), kBlock, 0, stream()>>>(
+        nd, reinterpret_cast<const std::uint32_t*>(a.x_dofmap),
+        reinterpret_cast<const std::uint32_t*>(L.x_dofmap), diff);
+    check_launch("geometry comparison");
+    int h = 1;
+    d2h(&h, diff, sizeof(int));
+    dfree(diff);
+    if (h)
+      return false;
+    a.same_geometry_as = L.id;
+  }
+  if (a.d.dtype == DFX_F64)
+    launch_cell_system<double>(a, L, bc);
+  else
+    launch_cell_system<float>(a, L, bc);
+  a.cell_data_ready = true;
+  L.cell_vec_ready = true;
+  return true;
+}
+} // namespace
+} // namespace dfx
+
+extern "C" {
+
 dfx_status dfx_assemble_system(dfx_csr* A, const dfx_form* a, const dfx_bc* bc, double diag,
                                dfx_vec* b, const dfx_form* L, int32_t zero_b)
 {
@@ -5924,6 +6137,23 @@ dfx_status dfx_assemble_system(dfx_csr* A, const dfx_form* a, const dfx_bc* bc, 
         // assembled (below ~1M cells the fork and join cost more than the
         // overlap gains)
         const bool fork = L->d.num_cells >= (1 << 20);
+        // a stiffness and a source form on the same geometry share one
+        // pass over the cells (records for the matrix, element vectors for
+        // b); the flags that tell the two assemblies so are cleared on
+        // every way out
+        dfx_form& af = const_cast<dfx_form&>(*a);
+        dfx_form& Lf = const_cast<dfx_form&>(*L);
+        struct ReadyGuard
+        {
+          dfx_form& a;
+          dfx_form& L;
+          ~ReadyGuard()
+          {
+            a.cell_data_ready = false;
+            L.cell_vec_ready = false;
+          }
+        } rg{af, Lf};
+        system_cell_pass(af, Lf, bc);
         // once forked, the library stream waits for the second stream on
         // every way out (an error in either part included), so no later
         // work on the library stream runs beside writes to b

@@ -229,6 +229,13 @@ struct dfx_bc
 struct dfx_form
 {
   dfx_form_desc d{};          // sizes and kernel id (host pointers cleared)
+  std::uint64_t id = 0;       // unique per form created in this process
+  // id of a form found to hold the same geometry (x and x_dofmap) on the
+  // device, and whether the cell pass of the next assembly was already
+  // done by a system cell pass (cell records / element vectors current)
+  std::uint64_t same_geometry_as = 0;
+  mutable bool cell_data_ready = false;
+  mutable bool cell_vec_ready = false;
   double constants[8] = {0};
   bool affine = true;         // every cell an affine image of the reference
   std::int32_t max_dof = -1;  // largest node index in the dofmap

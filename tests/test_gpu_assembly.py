@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from dolfinx_amd import _lib, fem, la, problems
+from dolfinx_amd import _lib, fem, la, mesh, problems
 from dolfinx_amd._lib import DfxError
 
 from conftest import rel_err
@@ -115,8 +115,10 @@ def test_lifting_nonzero_g_matches_oracle(cell, n):
 @pytest.mark.parametrize("cell,n", CASES + [(O.HEX, (104, 104, 100))])
 def test_fused_system_equals_separate_calls(cell, n):
     """assemble_system gives the bits of the separate calls, with nonzero
-    Dirichlet data, repeated into the same A and b; the 1,081,600-cell box
-    takes the path with the vector assembly on a second stream."""
+    Dirichlet data, repeated into the same A and b (the stiffness and source
+    forms share their mesh, so one cell pass serves both); the
+    1,081,600-cell box takes the path with the vector assembly on a second
+    stream."""
     p = build(cell, n, np.float64)
     x = p.mesh.geometry.x
     g = 1.0 + x[:, 0] ** 2 + 0.5 * x[:, 1] * x[:, 2]
@@ -130,6 +132,32 @@ def test_fused_system_equals_separate_calls(cell, n):
     for _ in range(2):
         fem.assemble_system(p.a, p.L, [bcg], 1.0, A2, b2)
         assert np.array_equal(A2.data, A1.data) and np.array_equal(b2.array, b1.array)
+
+
+@pytest.mark.parametrize("cell", [O.TET, O.HEX])
+def test_system_with_forms_on_different_geometry(cell):
+    """A source form on another mesh of the same size (same cells, moved
+    vertices) does not share the cell pass: assemble_system still gives the
+    separate calls' bits, before and after a system call with matching
+    forms."""
+    n = (6, 5, 4)
+    p = build(cell, n, np.float64)
+    ct = mesh.CellType.tetrahedron if cell == O.TET else mesh.CellType.hexahedron
+    m2 = mesh.create_box([[0.0, 0.0, 0.0], [2.0, 1.5, 1.0]], n, ct)
+    V2 = fem.functionspace(m2, ("Lagrange", 1))
+    nc = m2.topology.cells.shape[0]
+    nv = 4 if cell == O.TET else 8
+    f2 = np.random.default_rng(3).standard_normal((nc, nv))
+    L2 = fem.form("p1_source" if cell == O.TET else "q1_source", V2, f2, None)
+    A1 = fem.assemble_matrix(p.a, [p.bc])
+    b1 = fem.assemble_vector(L2)
+    fem.apply_lifting(b1, [p.a], [[p.bc]])
+    fem.set_bc(b1, [p.bc])
+    A2, b2 = fem.assemble_system(p.a, L2, [p.bc])
+    assert np.array_equal(A2.data, A1.data) and np.array_equal(b2.array, b1.array)
+    fem.assemble_system(p.a, p.L, [p.bc], 1.0, A2, b2)
+    fem.assemble_system(p.a, L2, [p.bc], 1.0, A2, b2)
+    assert np.array_equal(A2.data, A1.data) and np.array_equal(b2.array, b1.array)
 
 
 def test_entry_not_in_sparsity_raises():
