@@ -6132,11 +6132,6 @@ dfx_status dfx_assemble_system(dfx_csr* A, const dfx_form* a, const dfx_bc* bc, 
         };
         Context& c = ctx();
         ensure_init();
-        // the vector assembly reads only L's data and writes only b: on
-        // large meshes it runs on the second stream while the matrix is
-        // assembled (below ~1M cells the fork and join cost more than the
-        // overlap gains)
-        const bool fork = L->d.num_cells >= (1 << 20);
         // a stiffness and a source form on the same geometry share one
         // pass over the cells (records for the matrix, element vectors for
         // b); the flags that tell the two assemblies so are cleared on
@@ -6153,7 +6148,14 @@ dfx_status dfx_assemble_system(dfx_csr* A, const dfx_form* a, const dfx_bc* bc, 
             L.cell_vec_ready = false;
           }
         } rg{af, Lf};
-        system_cell_pass(af, Lf, bc);
+        const bool shared = system_cell_pass(af, Lf, bc);
+        // the vector assembly reads only L's data and writes only b: on
+        // large meshes it runs on the second stream while the matrix is
+        // assembled (below ~1M cells the fork and join cost more than the
+        // overlap gains). After a shared cell pass only the vector's row
+        // pass is left, and beside the matrix's row kernel both slow down
+        // more than they overlap, so it runs after it on one stream.
+        const bool fork = !shared && L->d.num_cells >= (1 << 20);
         // once forked, the library stream waits for the second stream on
         // every way out (an error in either part included), so no later
         // work on the library stream runs beside writes to b

@@ -1923,6 +1923,138 @@ dfx_status dfx_csr_sizes(const dfx_csr* A, int32_t* nrows, int64_t* nnz, int32_t
       });
 }
 
+} // extern "C"
+
+namespace dfx
+{
+namespace
+{
+/// MatrixCSR::add for staged dense blocks (la/MatrixCSR.h:157-174 via
+/// impl::insert_blocked_csr, la/matrix_csr_impl.h:67-109), one thread per
+/// (row, component): the row's contributions (block b, local row i) come in
+/// the order they were staged (off/item, built on the host by a counting
+/// sort over rows), and each adds its n column blocks at the slots found by
+/// binary search in the row; a column outside the pattern sets *err.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    add_blocks_k(std::int32_t nrows, int bs, int m, int n, const std::int32_t* __restrict__ off,
+                 const std::int64_t* __restrict__ item, const std::int32_t* __restrict__ cols_in,
+                 const T* __restrict__ vals_in, const std::int32_t* __restrict__ rp,
+                 const std::int32_t* __restrict__ ci, T* __restrict__ values, int* err)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t >= std::int64_t(nrows) * bs)
+    return;
+  const std::int32_t r = static_cast<std::int32_t>(t / bs);
+  const int cr = static_cast<int>(t - std::int64_t(r) * bs);
+  const std::int32_t lo = rp[r], hi = rp[r + 1];
+  const std::int64_t ld = std::int64_t(n) * bs; // values per row of a block
+  for (std::int32_t p = off[r]; p < off[r + 1]; ++p)
+  {
+    const std::int64_t bi = item[p]; // b * m + i
+    const std::int64_t b = bi / m;
+    const T* v = vals_in + (bi * bs + cr) * ld;
+    for (int j = 0; j < n; ++j)
+    {
+      const std::int32_t c = cols_in[b * n + j];
+      std::int32_t a = lo, z = hi;
+      while (a < z)
+      {
+        const std::int32_t mid = (a + z) >> 1;
+        if (ci[mid] < c)
+          a = mid + 1;
+        else
+          z = mid;
+      }
+      if (a >= hi || ci[a] != c)
+      {
+        *err = 1;
+        continue;
+      }
+      T* out = values + (std::int64_t(a) * bs + cr) * bs;
+      for (int cc = 0; cc < bs; ++cc)
+        out[cc] += v[std::int64_t(j) * bs + cc];
+    }
+  }
+}
+} // namespace
+} // namespace dfx
+
+extern "C" {
+
+dfx_status dfx_csr_add_blocks(dfx_csr* A, int64_t nblocks, int32_t m, int32_t n,
+                              const int32_t* rows, const int32_t* cols, const void* values)
+{
+  return guard(
+      [&]
+      {
+        require(A && (nblocks == 0 || (rows && cols && values)), "null argument");
+        require(nblocks >= 0 && m > 0 && n > 0, "bad block shape");
+        if (nblocks == 0)
+          return;
+        const std::int64_t nitems = nblocks * m;
+        require(nitems < (std::int64_t(1) << 40), "too many blocks");
+        // the staged rows grouped by matrix row, in staging order
+        std::vector<std::int32_t> off(std::size_t(A->nrows) + 1, 0);
+        for (std::int64_t q = 0; q < nitems; ++q)
+        {
+          require(rows[q] >= 0 && rows[q] < A->nrows, "row index out of range");
+          ++off[std::size_t(rows[q]) + 1];
+        }
+        for (std::int64_t q = 0; q < std::int64_t(nblocks) * n; ++q)
+          require(cols[q] >= 0 && cols[q] < A->ncols, "column index out of range");
+        for (std::int32_t r = 0; r < A->nrows; ++r)
+          off[std::size_t(r) + 1] += off[r];
+        std::vector<std::int64_t> item(nitems);
+        {
+          std::vector<std::int32_t> pos(off.begin(), off.end() - 1);
+          for (std::int64_t q = 0; q < nitems; ++q)
+            item[pos[rows[q]]++] = q;
+        }
+        const std::size_t ts = dtype_size(A->dtype);
+        const std::size_t nval = std::size_t(nblocks) * m * n * A->bs * A->bs;
+        std::int32_t* d_off = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * off.size()));
+        std::int64_t* d_item = static_cast<std::int64_t*>(dmalloc(sizeof(std::int64_t) * item.size()));
+        std::int32_t* d_cols = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * nblocks * n));
+        void* d_vals = dmalloc(ts * nval);
+        int* err = static_cast<int*>(dmalloc(sizeof(int)));
+        struct Free
+        {
+          std::vector<void*> p;
+          ~Free()
+          {
+            for (void* q : p)
+              dfree(q);
+          }
+        } fr{{d_off, d_item, d_cols, d_vals, err}};
+        h2d(d_off, off.data(), sizeof(std::int32_t) * off.size());
+        h2d(d_item, item.data(), sizeof(std::int64_t) * item.size());
+        h2d(d_cols, cols, sizeof(std::int32_t) * nblocks * n);
+        h2d(d_vals, values, ts * nval);
+        materialize_zero(*A);
+        fill_k<int><<<1, kBlock, 0, stream()>>>(err, 1, 0);
+        const std::int64_t nt = std::int64_t(A->nrows) * A->bs;
+        const int nb = static_cast<int>((nt + kBlock - 1) / kBlock);
+        if (nb > 0)
+        {
+          if (A->dtype == DFX_F64)
+            add_blocks_k<double><<<nb, kBlock, 0, stream()>>>(
+                A->nrows, A->bs, m, n, d_off, d_item, d_cols, static_cast<const double*>(d_vals),
+                A->row_ptr, A->cols, static_cast<double*>(A->values), err);
+          else
+            add_blocks_k<float><<<nb, kBlock, 0, stream()>>>(
+                A->nrows, A->bs, m, n, d_off, d_item, d_cols, static_cast<const float*>(d_vals),
+                A->row_ptr, A->cols, static_cast<float*>(A->values), err);
+        }
+        check_launch("add_blocks");
+        A->zeroed = false;
+        int h = 0;
+        d2h(&h, err, sizeof(int));
+        if (h)
+          throw Error(DFX_ERR_PATTERN, "Entry not in sparsity");
+      });
+}
+
 dfx_status dfx_csr_zero(dfx_csr* A)
 {
   return guard(

@@ -4,11 +4,17 @@
 // kappa grad u . grad v + c u v (DG0 kappa, constant c) and the load f v (f
 // at the cell's vertices) -- assembled with the drop-in's
 // fem::assemble_matrix / assemble_vector on the GPU, beside the library's
-// own device kernels for the same forms.
+// own device kernels for the same forms; and the same stiffness kernel
+// called in a host loop over the cells whose element matrices go through
+// the matrix's MatSet functor (A.mat_add_values(), then scatter_rev), as
+// the reference's custom assembly loop does.
 // Usage: custom_kernel_demo <n> <out.bin>
-// Writes the matrix and vector values of both (FP64) to out.bin.
+// Writes the matrix and vector values (FP64) to out.bin: kernel-ABI form,
+// its vector, device kernels, their vector, then the host-loop matrix and
+// the kernel-ABI form without BCs.
 #include <dolfinx_amd/dolfinx_amd.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -116,17 +122,44 @@ int main(int argc, char* argv[])
     fem::assemble_vector(bk, Lk);
     fem::assemble_vector(bd, Ld);
 
+    // the host loop: tabulate per cell, add through the MatSet functor
+    la::MatrixCSR<double> Ah(fem::create_sparsity_pattern(ak));
+    {
+      const std::vector<double> x = msh->x();
+      const std::vector<std::int32_t> cells = msh->cells();
+      const std::vector<std::int32_t> dofs = V->dofmap();
+      auto mat_add = Ah.mat_add_values();
+      std::vector<double> X(12), Ae(16);
+      for (std::int32_t c = 0; c < nc; ++c)
+      {
+        for (int v = 0; v < 4; ++v)
+          for (int d = 0; d < 3; ++d)
+            X[3 * v + d] = x[3 * std::size_t(cells[4 * std::size_t(c) + v]) + d];
+        std::fill(Ae.begin(), Ae.end(), 0.0);
+        stiffness_kernel(Ae.data(), &kappa[c], &creac, X.data(), nullptr, nullptr, nullptr);
+        std::span<const std::int32_t> dc(dofs.data() + 4 * std::size_t(c), 4);
+        mat_add(dc, dc, Ae);
+      }
+      Ah.scatter_rev();
+    }
+    la::MatrixCSR<double> An(fem::create_sparsity_pattern(ak));
+    fem::assemble_matrix(An.mat_add_values(), ak, {});
+
     const std::vector<double> vk = Ak.values(), vd = Ad.values();
+    const std::vector<double> vh = Ah.values(), vn = An.values();
     double dmax = 0, amax = 0;
     for (std::size_t e = 0; e < vk.size(); ++e)
     {
       dmax = std::max(dmax, std::abs(vk[e] - vd[e]));
       amax = std::max(amax, std::abs(vd[e]));
     }
-    std::cout << "{\"nnz\": " << vk.size() << ", \"max_diff_rel\": " << dmax / amax << "}"
-              << std::endl;
+    double hmax = 0;
+    for (std::size_t e = 0; e < vh.size(); ++e)
+      hmax = std::max(hmax, std::abs(vh[e] - vn[e]));
+    std::cout << "{\"nnz\": " << vk.size() << ", \"max_diff_rel\": " << dmax / amax
+              << ", \"host_loop_max_diff_rel\": " << hmax / amax << "}" << std::endl;
     std::ofstream out(argv[2], std::ios::binary);
-    for (const auto& v : {vk, bk.array(), vd, bd.array()})
+    for (const auto& v : {vk, bk.array(), vd, bd.array(), vh, vn})
     {
       const std::int64_t m = static_cast<std::int64_t>(v.size());
       out.write(reinterpret_cast<const char*>(&m), sizeof(m));

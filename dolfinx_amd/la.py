@@ -116,6 +116,21 @@ class MatrixCSR:
         _lib.call("dfx_csr_autotune", self._h, C.byref(lanes), C.byref(ms))
         return lanes.value, ms.value
 
+    def add(self, rows, cols, values):
+        """MatrixCSR::add of dense blocks (la/MatrixCSR.h:157-174): rows is
+        (nblocks, m), cols (nblocks, n) block indices, values (nblocks,
+        m bs, n bs); entries are added in block order, and a column outside
+        the pattern raises "Entry not in sparsity"."""
+        rows = np.ascontiguousarray(rows, np.int32)
+        cols = np.ascontiguousarray(cols, np.int32)
+        if rows.ndim == 1:
+            rows, cols = rows[None, :], cols[None, :]
+        nb, m = rows.shape
+        n = cols.shape[1]
+        v = np.ascontiguousarray(values, self.dtype).reshape(nb, m * self.bs, n * self.bs)
+        _lib.call("dfx_csr_add_blocks", self._h, int(nb), int(m), int(n), _lib.ptr(rows),
+                  _lib.ptr(cols), _lib.ptr(v))
+
     def set_lanes(self, lanes):
         """SpMV kernel of mult: 0 default, 1 row-block stream, 2..64 lanes
         per row, -1 the solvers' sliced-ELL copy of the values as they are

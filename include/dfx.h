@@ -172,6 +172,15 @@ dfx_status dfx_csr_sizes(const dfx_csr* A, int32_t* nrows, int64_t* nnz,
 dfx_status dfx_csr_zero(dfx_csr* A);
 /* Copy values (nnz*bs*bs of the matrix dtype) to / from the host. */
 dfx_status dfx_csr_get_values(const dfx_csr* A, void* values);
+/* MatrixCSR::add / mat_add_values<bs, bs> (la/MatrixCSR.h:157-174,
+ * la/matrix_csr_impl.h:67-109) for nblocks dense blocks staged on the host:
+ * block q adds its (m bs) x (n bs) row-major values (host, dtype of A) at
+ * block rows rows[q m .. q m + m) and block columns cols[q n .. q n + n).
+ * Every entry receives its additions in staging order (block, then row,
+ * then column), as the reference's call-by-call insertion; an entry
+ * outside the pattern returns DFX_ERR_PATTERN ("Entry not in sparsity"). */
+dfx_status dfx_csr_add_blocks(dfx_csr* A, int64_t nblocks, int32_t m, int32_t n,
+                              const int32_t* rows, const int32_t* cols, const void* values);
 dfx_status dfx_csr_set_values(dfx_csr* A, const void* values);
 /* Copy the device pattern back (row_ptr as int64 like the reference). */
 dfx_status dfx_csr_get_pattern(const dfx_csr* A, int64_t* row_ptr,

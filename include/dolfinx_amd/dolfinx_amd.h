@@ -229,17 +229,79 @@ private:
   std::int32_t _n = 0;
 };
 
-/// Tag returned by mat_add_values(): routes fem::assemble_matrix to the
-/// device scatter (the reference's MatSet functor, la/utils.h:26-29).
+/// Host staging of MatrixCSR::add calls (la/MatrixCSR.h:157-174): blocks
+/// of one shape are collected in call order and inserted on the device
+/// (dfx_csr_add_blocks, each entry summed in call order) when the shape
+/// changes, at scatter_rev(), or before the matrix is next used on the
+/// device.
+template <typename T>
+struct AddStage
+{
+  dfx_csr* A = nullptr;
+  int bs = 1;
+  int m = 0, n = 0;
+  std::int64_t nblocks = 0;
+  std::vector<std::int32_t> rows, cols;
+  std::vector<T> vals;
+  void add(std::span<const std::int32_t> r, std::span<const std::int32_t> c, std::span<const T> x)
+  {
+    if (x.size() != r.size() * c.size() * std::size_t(bs * bs))
+      throw std::runtime_error("MatrixCSR::add: data size does not match the block");
+    if (nblocks > 0 && (int(r.size()) != m || int(c.size()) != n))
+      flush();
+    m = int(r.size());
+    n = int(c.size());
+    rows.insert(rows.end(), r.begin(), r.end());
+    cols.insert(cols.end(), c.begin(), c.end());
+    vals.insert(vals.end(), x.begin(), x.end());
+    ++nblocks;
+  }
+  void flush()
+  {
+    if (nblocks == 0)
+      return;
+    const std::int64_t nb = nblocks;
+    nblocks = 0;
+    const dfx_status st
+        = dfx_csr_add_blocks(A, nb, m, n, rows.data(), cols.data(), vals.data());
+    rows.clear();
+    cols.clear();
+    vals.clear();
+    check(st);
+  }
+};
+
+/// mat_add_values(): the reference's MatSet functor (la/utils.h:26-29,
+/// la/MatrixCSR.h:224-237) -- mat_add(rows, cols, data) adds the dense
+/// (rows x bs) x (cols x bs) block, staged on the host as above -- and the
+/// handle through which fem::assemble_matrix routes to the device kernels.
 template <typename T>
 struct DeviceMatAdd
 {
   dfx_csr* A;
+  std::shared_ptr<AddStage<T>> stage;
+  int operator()(std::span<const std::int32_t> rows, std::span<const std::int32_t> cols,
+                 std::span<const T> data) const
+  {
+    stage->add(rows, cols, data);
+    return 0;
+  }
+  void flush() const
+  {
+    if (stage)
+      stage->flush();
+  }
 };
 template <typename T>
 struct DeviceMatSet
 {
   dfx_csr* A;
+  std::shared_ptr<AddStage<T>> stage;
+  void flush() const
+  {
+    if (stage)
+      stage->flush();
+  }
 };
 
 /// la::MatrixCSR<T, DeviceContainer> in compact block mode.
@@ -257,24 +319,40 @@ public:
     _bs = p.block_size();
     _nrows = p.num_rows();
     _nnz = static_cast<std::int64_t>(p.graph_columns().size());
+    _stage = std::make_shared<AddStage<T>>();
+    _stage->A = h;
+    _stage->bs = _bs;
   }
-  dfx_csr* handle() const { return _h.get(); }
-  DeviceMatAdd<T> mat_add_values() { return {_h.get()}; }
-  DeviceMatSet<T> mat_set_values() { return {_h.get()}; }
-  /// MatrixCSR::set(x): only zero is supported on the device
+  /// the device matrix, with any staged MatrixCSR::add calls inserted
+  dfx_csr* handle() const
+  {
+    _stage->flush();
+    return _h.get();
+  }
+  DeviceMatAdd<T> mat_add_values() { return {_h.get(), _stage}; }
+  DeviceMatSet<T> mat_set_values() { return {_h.get(), _stage}; }
+  /// MatrixCSR::scatter_rev (la/MatrixCSR.h:589-626): one rank, so only the
+  /// staged additions are inserted
+  void scatter_rev() { _stage->flush(); }
+  /// MatrixCSR::set(x): only zero is supported on the device (staged
+  /// additions made before it are dropped, as set(0) overwrites them)
   void set(T x)
   {
     if (x != T(0))
       throw std::runtime_error("MatrixCSR::set: only zero is supported");
+    _stage->rows.clear();
+    _stage->cols.clear();
+    _stage->vals.clear();
+    _stage->nblocks = 0;
     check(dfx_csr_zero(_h.get()));
   }
   /// y += A x (la/MatrixCSR.h:877-946)
-  void mult(Vector<T>& x, Vector<T>& y) const { check(dfx_spmv(_h.get(), x.handle(), y.handle())); }
+  void mult(Vector<T>& x, Vector<T>& y) const { check(dfx_spmv(handle(), x.handle(), y.handle())); }
   /// Host copy of the values (row-major blocks)
   std::vector<T> values() const
   {
     std::vector<T> v(_nnz * _bs * _bs);
-    check(dfx_csr_get_values(_h.get(), v.data()));
+    check(dfx_csr_get_values(handle(), v.data()));
     return v;
   }
   std::array<int, 2> block_size() const { return {_bs, _bs}; }
@@ -292,12 +370,13 @@ public:
   {
     std::int32_t l = 0;
     float ms = 0;
-    check(dfx_csr_autotune(_h.get(), &l, &ms));
+    check(dfx_csr_autotune(handle(), &l, &ms));
     return {l, ms};
   }
 
 private:
   std::shared_ptr<dfx_csr> _h;
+  std::shared_ptr<AddStage<T>> _stage;
   int _bs = 1;
   std::int32_t _nrows = 0;
   std::int64_t _nnz = 0;
@@ -569,6 +648,7 @@ void assemble_matrix(la::DeviceMatAdd<T> mat_add, const Form<T>& a, const BCs<T>
 {
   if (bcs.size() > 1)
     throw std::runtime_error("pass the union of Dirichlet dofs as one DirichletBC");
+  mat_add.flush();
   check(dfx_assemble_matrix(mat_add.A, a.handle(), bcs.empty() ? nullptr : bcs[0].get().handle()));
 }
 
@@ -577,6 +657,7 @@ template <typename T>
 void set_diagonal(la::DeviceMatSet<T> mat_set, const FunctionSpace&, const BCs<T>& bcs,
                   T diagonal = 1)
 {
+  mat_set.flush();
   for (auto& bc : bcs)
     check(dfx_set_diagonal(mat_set.A, bc.get().handle(), diagonal));
 }
@@ -613,6 +694,7 @@ void assemble_matrix_elements(la::DeviceMatAdd<T> mat_add, const Form<T>& a,
 {
   if (bcs.size() > 1)
     throw std::runtime_error("pass the union of Dirichlet dofs as one DirichletBC");
+  mat_add.flush();
   check(dfx_assemble_matrix_elements(mat_add.A, a.handle(),
                                      bcs.empty() ? nullptr : bcs[0].get().handle(), E.handle()));
 }
