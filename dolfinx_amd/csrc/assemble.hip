@@ -2798,9 +2798,6 @@ __global__ void __launch_bounds__(kBlock)
       __builtin_nontemporal_store(slice[t], vals + base + t);
 }
 
-/// Element vectors of the source forms, one thread per cell:
-/// P1 tetrahedra b_i = |K|/20 (f_i + sum f); affine Q1 hexahedra
-/// b_i = |det J| sum_j M_ij f_j; other hexahedra 2x2x2 Gauss.
 /// Element source vector of an affine cell from its vertices q0..q3 (P1:
 /// the four vertices; Q1: vertices 0, 1, 2, 4) and the source values at its
 /// NV nodes: P1 b_i = |K|/20 (f_i + sum f), Q1 b_i = |det J| sum_j M_ij f_j.
@@ -2872,6 +2869,9 @@ __device__ inline void load_source(const T* __restrict__ coeffs, int cstride, st
   }
 }
 
+/// Element vectors of the source forms, one thread per cell:
+/// P1 tetrahedra b_i = |K|/20 (f_i + sum f); affine Q1 hexahedra
+/// b_i = |det J| sum_j M_ij f_j; other hexahedra 2x2x2 Gauss.
 template <typename T, int NV>
 __global__ void __launch_bounds__(kBlock)
     cell_source_k(std::int32_t nc, const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
@@ -3805,7 +3805,8 @@ int incidence_lanes(const dfx_form& f)
 /// with 64/128/256 rows per workgroup; 3 "lanes"; 4..6 closed-form Q1 rows;
 /// 7, 8 incidence-parallel gathers; 9 structured Q1; 10 fused geometry;
 /// 11 as 1 with eight incidences in flight; 12..14 fixed-width Q1 rows;
-/// 15 structured Q1 rows in registers stored through an LDS slice.
+/// 15 structured Q1 rows in registers stored through an LDS slice; 16 row
+/// teams forming the entries; 17 row teams gathering the element matrices.
 constexpr int kRowVariants = 18;
 
 template <typename T, typename O, int K, int L>
