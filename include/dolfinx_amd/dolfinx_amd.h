@@ -745,4 +745,39 @@ int cg(la::Vector<T>& x, const la::Vector<T>& b, const la::MatrixCSR<T>& A, int 
 }
 } // namespace linalg
 
+/// A sequence of library calls recorded into one HIP graph (no counterpart
+/// in the reference): every launch the library makes from this thread
+/// inside capture(f) is recorded instead of run, and launch() replays the
+/// sequence as one graph launch. Run the sequence once before recording it
+/// (calls that need the host part-way -- allocation, plan uploads, waits --
+/// throw inside a capture).
+class Graph
+{
+public:
+  template <typename F>
+  static Graph capture(F&& f)
+  {
+    check(dfx_graph_begin());
+    try
+    {
+      f();
+    }
+    catch (...)
+    {
+      dfx_graph* g = nullptr;
+      (void)dfx_graph_end(&g); // closes (and discards) the capture
+      dfx_graph_destroy(g);
+      throw;
+    }
+    dfx_graph* g = nullptr;
+    check(dfx_graph_end(&g));
+    return Graph(g);
+  }
+  void launch() const { check(dfx_graph_launch(_g.get())); }
+
+private:
+  explicit Graph(dfx_graph* g) : _g(g, dfx_graph_destroy) {}
+  std::shared_ptr<dfx_graph> _g;
+};
+
 } // namespace dolfinx_amd
