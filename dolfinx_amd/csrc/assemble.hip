@@ -863,6 +863,16 @@ __device__ inline void load_factors(const T* __restrict__ cd, std::int64_t c, T 
   }
 }
 
+/// Orders the LDS accesses of the lanes of one wavefront around it
+/// (release and acquire fences at wavefront scope and a wave barrier): a
+/// value one lane writes before it is seen by the others reading after it.
+__device__ inline void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 /// a[sl[j]] += v[j] for the unmasked j of one incidence. The K slots of an
 /// incidence are distinct columns of its row, so every value is read
 /// before any is written: the K LDS reads are in flight together instead
@@ -1766,8 +1776,8 @@ __global__ void __launch_bounds__(kBlock)
 /// distinct, so the team's adds need no atomics and every entry is summed
 /// in cell order, with the additions of variant 8. The cell factors of the
 /// row's incidences are loaded lane-parallel, K incidences at a time (lane
-/// j holds incidence bK + j of batch b), and handed to the team by
-/// shuffles; each step's slots are loaded one step ahead. The wavefronts of a
+/// j holds incidence bK + j of batch b), and handed to the team through an
+/// 8-value LDS slot per team; each step's slots are loaded one step ahead. The wavefronts of a
 /// workgroup (blockDim.x / 64 of them) share one copy of the tensors. Rows
 /// come ordered by incidence count inside chunks of 4096 consecutive rows
 /// (trow), so the teams of a wavefront take similar numbers of steps.
@@ -1796,7 +1806,11 @@ __global__ void __launch_bounds__(1024, 8)
   if (member && tq < ntrow)
     ro = trow[tq];
   const std::int32_t r = ro.x, p0 = ro.y, n = ro.z;
-  T* a = S + 7 * K * K + (std::size_t(wave) * G + (member ? team : 0)) * maxrow;
+  const std::size_t slot = std::size_t(wave) * G + (member ? team : 0);
+  T* a = S + 7 * K * K + slot * maxrow;
+  // the team's broadcast slot (8 values, 16-byte aligned) after the slices
+  const std::size_t slices = (std::size_t(nw) * G * maxrow + 3) & ~std::size_t(3);
+  T* bs = S + 7 * K * K + slices + slot * 8;
   std::int32_t rbase = 0, rlen = 0;
   bool skip = true;
   if (r >= 0)
@@ -1841,17 +1855,58 @@ __global__ void __launch_bounds__(1024, 8)
       const O sl = sl_next;
       if (st + 1 < n)
         sl_next = islots[std::int64_t(p0 + st + 1) * K + j];
-      T g[7];
+      // the lane holding incidence st writes its factors into the team's
+      // slot and the team reads them back (LDS accesses of one wavefront
+      // complete in order): four 16-byte writes and four broadcast reads
+      // instead of fifteen shuffles
+      (void)src;
+      using T2 = vec2_t<T>;
+      T2* b2 = reinterpret_cast<T2*>(bs);
+      // the slot passes values between lanes: wavefront-scope fences and a
+      // wave barrier keep every lane's read of the previous step before
+      // this write, and this write before every lane's read below (without
+      // them the compiler may move a lane's read across another lane's
+      // write)
+      wave_sync();
+      if (j == st - bq * K)
+      {
+        T w8[8] = {bg[0], bg[1], bg[2], bg[3], bg[4], bg[5], bg[6], T(0)};
+        if constexpr (sizeof(T) == 8)
+          w8[7] = __longlong_as_double(static_cast<long long>(bmi));
+        else
+          w8[7] = __uint_as_float(bmi);
 #pragma unroll
-      for (int u = 0; u < 7; ++u)
-        g[u] = __shfl(bg[u], src, 64);
-      const std::uint32_t mi = __shfl(bmi, src, 64);
+        for (int h = 0; h < 4; ++h)
+          b2[h] = T2{w8[2 * h], w8[2 * h + 1]};
+      }
+      wave_sync();
+      T g[7];
+      std::uint32_t mi;
+      {
+        T r8[8];
+#pragma unroll
+        for (int h = 0; h < 4; ++h)
+        {
+          const T2 v = b2[h];
+          r8[2 * h] = v.x;
+          r8[2 * h + 1] = v.y;
+        }
+#pragma unroll
+        for (int u = 0; u < 7; ++u)
+          g[u] = r8[u];
+        if constexpr (sizeof(T) == 8)
+          mi = static_cast<std::uint32_t>(__double_as_longlong(r8[7]));
+        else
+          mi = __float_as_uint(r8[7]);
+      }
       const T e = element_entry<T, K>(g, S, static_cast<int>(mi >> 24), j);
       if (!((mi >> j) & 1u))
         a[sl] += e;
     }
   }
-  // the team stores its own row (the whole slice: zeros for skipped rows)
+  // the team stores its own row (the whole slice: zeros for skipped rows),
+  // whose entries other lanes of the team added
+  wave_sync();
   if (r >= 0)
   {
     if (accumulate)
@@ -1968,6 +2023,8 @@ __global__ void __launch_bounds__(1024, 8)
       fetch(q3, st + 7);
     }
   }
+  // the row's entries were added by other lanes of the team
+  wave_sync();
   if (r >= 0)
   {
     if (accumulate)
@@ -4311,7 +4368,10 @@ void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
     constexpr int G = 64 / K;
     int nw = 16;
     auto lds_for = [&](int w)
-    { return (7 * std::size_t(K) * K + std::size_t(w) * G * A.max_row) * sizeof(T); };
+    {
+      const std::size_t slices = (std::size_t(w) * G * A.max_row + 3) & ~std::size_t(3);
+      return (7 * std::size_t(K) * K + slices + std::size_t(w) * G * 8) * sizeof(T);
+    };
     while (nw > 4 && lds_for(nw) > 64 * 1024)
       nw /= 2;
     const std::size_t lds = lds_for(nw);
