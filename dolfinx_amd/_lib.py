@@ -133,6 +133,7 @@ SIGNATURES = {
     "dfx_batch_status": (C.c_int, [P, I32, P, P, P, P, P]),
     "dfx_batch_restart": (C.c_int, [P]),
     "dfx_csr_add_blocks": (C.c_int, [P, I64, I32, I32, P, P, P]),
+    "dfx_form_create_kernel_facets": (C.c_int, [P, P, P, I32, I32, P, PP]),
     "dfx_graph_begin": (C.c_int, []),
     "dfx_graph_end": (C.c_int, [PP]),
     "dfx_graph_launch": (C.c_int, [P]),

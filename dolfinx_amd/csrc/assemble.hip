@@ -5163,6 +5163,111 @@ const T* run_host_kernel(dfx_form& f)
   return static_cast<const T*>(f.elem_tensors);
 }
 
+/// Facet integrals with a host kernel of the reference's ABI, as
+/// assemble_exterior_facets / assemble_interior_facets call theirs
+/// (fem/assemble_matrix_impl.h:264-383, :442-667;
+/// fem/assemble_vector_impl.h:118-174): per facet, in the given order, the
+/// kernel gets the cell's (interior: both cells') geometry nodes, packed
+/// coefficients and local facet indices (quadrature permutations 0); BC
+/// rows and columns of the tensor are zeroed as there; the tensors of all
+/// facets are then added on the device, every entry in facet order
+/// (mat_set(dmapjoint0, dmapjoint1, Ae)).
+template <typename T>
+void host_facet_tensors(const dfx_form& f, const std::int8_t* mark, std::vector<std::int32_t>& dofs,
+                        std::vector<T>& vals)
+{
+  using Fn = void (*)(T*, const T*, const T*, const T*, const int*, const std::uint8_t*, void*);
+  const Fn fn = reinterpret_cast<Fn>(f.host_kernel);
+  const bool interior = f.host_facets == 2;
+  const int sides = interior ? 2 : 1;
+  const int k = f.d.dofs_per_cell, bs = f.d.bs, npc = f.d.nodes_per_cell;
+  const int m = sides * k;          // block rows of a facet tensor
+  const int nd = m * bs;            // scalar rows
+  const bool mat = f.d.kernel == DFX_K_ELEMENT_A;
+  const std::int64_t sz = mat ? std::int64_t(nd) * nd : nd;
+  const int per = interior ? 4 : 2;
+  const std::int64_t ne = std::int64_t(f.h_entities.size()) / per;
+  const int cs = f.d.cstride;
+  dofs.resize(std::size_t(ne) * m);
+  vals.assign(std::size_t(std::max<std::int64_t>(ne * sz, 1)), T(0));
+  std::vector<T> cdofs(3 * std::size_t(sides) * npc);
+  std::vector<T> w(std::max(1, sides * cs), T(0));
+  std::vector<T> cst(std::max<std::size_t>(f.h_constants.size(), 1), T(0));
+  for (std::size_t q = 0; q < f.h_constants.size(); ++q)
+    cst[q] = static_cast<T>(f.h_constants[q]);
+  for (std::int64_t e = 0; e < ne; ++e)
+  {
+    int lf[2] = {0, 0};
+    const std::uint8_t perm[2] = {0, 0};
+    std::int32_t* de = dofs.data() + e * m;
+    for (int s = 0; s < sides; ++s)
+    {
+      const std::int32_t c = f.h_entities[per * e + 2 * s];
+      lf[s] = f.h_entities[per * e + 2 * s + 1];
+      for (int i = 0; i < npc; ++i)
+      {
+        const double* xi = f.h_x.data() + 3 * std::int64_t(f.h_x_dofmap[std::int64_t(c) * npc + i]);
+        for (int r = 0; r < 3; ++r)
+          cdofs[3 * (std::size_t(s) * npc + i) + r] = static_cast<T>(xi[r]);
+      }
+      for (int q = 0; q < cs; ++q)
+        w[std::size_t(s) * cs + q] = static_cast<T>(f.h_coeffs[std::int64_t(c) * cs + q]);
+      for (int i = 0; i < k; ++i)
+        de[s * k + i] = f.h_dofmap[std::int64_t(c) * k + i];
+    }
+    T* Ae = vals.data() + e * sz;
+    fn(Ae, w.data(), cst.data(), cdofs.data(), lf, perm, f.host_kernel_data);
+    if (mat && mark)
+    {
+      // zero the rows and columns of Dirichlet dofs
+      for (int i = 0; i < m; ++i)
+        for (int c2 = 0; c2 < bs; ++c2)
+          if (mark[std::int64_t(bs) * de[i] + c2])
+          {
+            for (int j = 0; j < nd; ++j)
+              Ae[std::int64_t(bs * i + c2) * nd + j] = T(0);
+            for (int j = 0; j < nd; ++j)
+              Ae[std::int64_t(j) * nd + bs * i + c2] = T(0);
+          }
+    }
+  }
+}
+
+/// Host copy of a BC marker (nullptr without BCs).
+std::vector<std::int8_t> host_marker(const dfx_bc* bc)
+{
+  std::vector<std::int8_t> h;
+  if (bc)
+  {
+    h.resize(bc->n_total);
+    d2h(h.data(), bc->marker, bc->n_total);
+  }
+  return h;
+}
+
+template <typename T>
+void host_facets_matrix(dfx_csr& A, const dfx_form& f, const dfx_bc* bc)
+{
+  const std::vector<std::int8_t> mark = host_marker(bc);
+  std::vector<std::int32_t> dofs;
+  std::vector<T> vals;
+  host_facet_tensors<T>(f, bc ? mark.data() : nullptr, dofs, vals);
+  const int m = (f.host_facets == 2 ? 2 : 1) * f.d.dofs_per_cell;
+  const std::int64_t ne = std::int64_t(dofs.size()) / m;
+  csr_add_blocks(A, ne, m, m, dofs.data(), dofs.data(), vals.data());
+}
+
+template <typename T>
+void host_facets_vector(dfx_vec& b, const dfx_form& f)
+{
+  std::vector<std::int32_t> dofs;
+  std::vector<T> vals;
+  host_facet_tensors<T>(f, nullptr, dofs, vals);
+  const int m = (f.host_facets == 2 ? 2 : 1) * f.d.dofs_per_cell;
+  const std::int64_t ne = std::int64_t(dofs.size()) / m;
+  vec_add_blocks(b, ne, m, f.d.bs, dofs.data(), vals.data());
+}
+
 extern "C" {
 
 dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out)
@@ -5429,6 +5534,72 @@ dfx_status dfx_form_create_kernel(const dfx_form_desc* desc, void* kernel, void*
       });
 }
 
+dfx_status dfx_form_create_kernel_facets(const dfx_form_desc* desc, void* kernel,
+                                         void* custom_data, int32_t type, int32_t nentities,
+                                         const int32_t* entities, dfx_form** out)
+{
+  return guard(
+      [&]
+      {
+        require(desc && kernel && out && (nentities == 0 || entities), "null argument");
+        require(type == 1 || type == 2, "type is 1 (exterior facets) or 2 (interior facets)");
+        require(is_element_kernel(desc->kernel),
+                "a kernel form is DFX_K_ELEMENT_A (matrix) or DFX_K_ELEMENT_L (vector)");
+        require(type == 1 || desc->kernel == DFX_K_ELEMENT_A,
+                "interior facet integrals are bilinear here");
+        require(desc->dtype == DFX_F64 || desc->dtype == DFX_F32, "bad dtype");
+        require(desc->bs == 1 || desc->bs == 3, "block size 1 or 3");
+        require(desc->num_cells >= 0 && desc->num_nodes >= 0 && nentities >= 0, "negative size");
+        require(desc->nodes_per_cell == 4 || desc->nodes_per_cell == 8,
+                "tetrahedra (4 nodes) or hexahedra (8 nodes)");
+        require(desc->x && desc->x_dofmap && desc->dofmap, "null mesh data");
+        require(desc->dofs_per_cell >= 1 && desc->dofs_per_cell <= 32, "1 to 32 dofs per cell");
+        require(desc->cstride >= 0 && (desc->cstride == 0 || desc->coeffs), "null coefficients");
+        require(desc->num_constants >= 0 && (desc->num_constants == 0 || desc->constants),
+                "null constants");
+        const std::int64_t nc = desc->num_cells;
+        const int nfac = desc->nodes_per_cell == 4 ? 4 : 6;
+        const int per = type == 2 ? 4 : 2;
+        for (std::int64_t e = 0; e < std::int64_t(nentities); ++e)
+          for (int s = 0; s < per / 2; ++s)
+          {
+            const std::int32_t c = entities[per * e + 2 * s];
+            const std::int32_t l = entities[per * e + 2 * s + 1];
+            require(c >= 0 && c < nc && l >= 0 && l < nfac, "facet entity out of range");
+          }
+        for (std::int64_t i = 0; i < nc * desc->nodes_per_cell; ++i)
+          require(desc->x_dofmap[i] >= 0 && desc->x_dofmap[i] < desc->num_nodes,
+                  "geometry dofmap entry out of range");
+        auto f = std::make_unique<dfx_form>();
+        f->d = *desc;
+        f->d.x = nullptr;
+        f->d.x_dofmap = nullptr;
+        f->d.dofmap = nullptr;
+        f->d.coeffs = nullptr;
+        f->d.constants = nullptr;
+        std::int32_t maxd = -1;
+        f->h_dofmap.assign(desc->dofmap, desc->dofmap + nc * desc->dofs_per_cell);
+        for (std::int32_t v : f->h_dofmap)
+        {
+          require(v >= 0, "negative dof");
+          maxd = std::max(maxd, v);
+        }
+        f->max_dof = maxd;
+        f->h_x.assign(desc->x, desc->x + 3 * std::int64_t(desc->num_nodes));
+        f->h_x_dofmap.assign(desc->x_dofmap, desc->x_dofmap + nc * desc->nodes_per_cell);
+        if (desc->cstride > 0)
+          f->h_coeffs.assign(desc->coeffs, desc->coeffs + nc * desc->cstride);
+        if (desc->num_constants > 0)
+          f->h_constants.assign(desc->constants, desc->constants + desc->num_constants);
+        f->h_entities.assign(entities, entities + std::int64_t(nentities) * per);
+        f->host_kernel = kernel;
+        f->host_kernel_data = custom_data;
+        f->host_facets = type;
+        f->id = next_pattern_id();
+        *out = f.release();
+      });
+}
+
 dfx_status dfx_form_destroy(dfx_form* f)
 {
   if (f)
@@ -5626,6 +5797,15 @@ dfx_status dfx_assemble_matrix(dfx_csr* A, const dfx_form* a, const dfx_bc* bc)
         require(!bc || bc->n_total == A->nrows * A->bs, "BC size does not match the matrix");
         require(a->max_dof < A->nrows && a->max_dof < A->ncols, "dofmap exceeds the matrix size");
         dfx_form& f = const_cast<dfx_form&>(*a);
+        if (f.host_facets)
+        {
+          // facet tensors from the host kernel, added on the device
+          if (A->dtype == DFX_F64)
+            host_facets_matrix<double>(*A, f, bc);
+          else
+            host_facets_matrix<float>(*A, f, bc);
+          return;
+        }
         if (f.host_kernel)
         {
           // element matrices from the host kernel, inserted on the device
@@ -5878,6 +6058,14 @@ dfx_status dfx_assemble_vector(dfx_vec* b, const dfx_form* L)
                 "element-tensor forms are assembled with dfx_assemble_vector_elements");
         require(b->dtype == L->d.dtype, "vector and form dtypes differ");
         require(std::int64_t(L->max_dof + 1) * L->d.bs <= b->n, "dofmap exceeds the vector size");
+        if (L->host_facets)
+        {
+          if (b->dtype == DFX_F64)
+            host_facets_vector<double>(*b, *L);
+          else
+            host_facets_vector<float>(*b, *L);
+          return;
+        }
         if (L->host_kernel)
         {
           dfx_form& f = const_cast<dfx_form&>(*L);
@@ -6016,6 +6204,9 @@ dfx_status dfx_apply_lifting(dfx_vec* b, const dfx_form* a, const dfx_bc* bc, co
         if (!bc->g && !x0)
           return;
         dfx_form& f = const_cast<dfx_form&>(*a);
+        require(!f.host_facets,
+                "apply_lifting with nonzero Dirichlet data through a facet integral given by "
+                "a host kernel is not supported");
         if (f.host_kernel)
         {
           // the kernel's element matrices (as lift_bc recomputes them)

@@ -236,6 +236,13 @@ struct dfx_form
   std::uint64_t same_geometry_as = 0;
   mutable bool cell_data_ready = false;
   mutable bool cell_vec_ready = false;
+  // Facet integrals whose kernel has the reference's ABI (run on the
+  // host): 1 exterior, given as (cell, local facet) pairs; 2 interior,
+  // given as (cell0, local facet0, cell1, local facet1) quads; 0 none. The
+  // host copy of the dofmap gives the rows of each facet's tensor.
+  int host_facets = 0;
+  std::vector<std::int32_t> h_entities;
+  std::vector<std::int32_t> h_dofmap;
   double constants[8] = {0};
   bool affine = true;         // every cell an affine image of the reference
   std::int32_t max_dof = -1;  // largest node index in the dofmap
@@ -349,4 +356,13 @@ namespace dfx
 /// Store the pending zeros of A's values (no-op unless zero_pending).
 /// Every reader of the values calls this first.
 void materialize_zero(const dfx_csr& A);
+/// MatrixCSR::add of nblocks host blocks (m x n block entries each, values
+/// (m bs) x (n bs) row-major in A's dtype), every entry summed in staging
+/// order on the device; "Entry not in sparsity" for an absent entry.
+void csr_add_blocks(dfx_csr& A, std::int64_t nblocks, int m, int n, const std::int32_t* rows,
+                    const std::int32_t* cols, const void* values);
+/// b[bs rows[q] + comp] += values[q bs + comp] for nblocks blocks of m rows
+/// (host values in b's dtype), summed in staging order on the device.
+void vec_add_blocks(dfx_vec& b, std::int64_t nblocks, int m, int bs, const std::int32_t* rows,
+                    const void* values);
 } // namespace dfx

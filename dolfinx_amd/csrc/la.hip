@@ -1977,7 +1977,142 @@ __global__ void __launch_bounds__(kBlock)
     }
   }
 }
+/// b[bs r + comp] += the staged block vectors at row r, in staging order,
+/// one thread per (row, component) (off/item as for add_blocks_k).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    add_vec_blocks_k(std::int32_t nrows, int bs, const std::int32_t* __restrict__ off,
+                     const std::int64_t* __restrict__ item, const T* __restrict__ vals_in,
+                     T* __restrict__ b)
+{
+  const std::int64_t t = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
+  if (t >= std::int64_t(nrows) * bs)
+    return;
+  const std::int32_t r = static_cast<std::int32_t>(t / bs);
+  const int cr = static_cast<int>(t - std::int64_t(r) * bs);
+  T s = b[t];
+  for (std::int32_t p = off[r]; p < off[r + 1]; ++p)
+    s += vals_in[item[p] * bs + cr];
+  b[t] = s;
+}
+
+/// The staged rows grouped by matrix row in staging order: off (nrows + 1)
+/// and the staged row index of each (a counting sort on the host).
+void group_rows(std::int32_t nrows, std::int64_t nitems, const std::int32_t* rows,
+                std::vector<std::int32_t>& off, std::vector<std::int64_t>& item)
+{
+  off.assign(std::size_t(nrows) + 1, 0);
+  for (std::int64_t q = 0; q < nitems; ++q)
+  {
+    require(rows[q] >= 0 && rows[q] < nrows, "row index out of range");
+    ++off[std::size_t(rows[q]) + 1];
+  }
+  for (std::int32_t r = 0; r < nrows; ++r)
+    off[std::size_t(r) + 1] += off[r];
+  item.resize(nitems);
+  std::vector<std::int32_t> pos(off.begin(), off.end() - 1);
+  for (std::int64_t q = 0; q < nitems; ++q)
+    item[pos[rows[q]]++] = q;
+}
 } // namespace
+
+void csr_add_blocks(dfx_csr& A, std::int64_t nblocks, int m, int n, const std::int32_t* rows,
+                    const std::int32_t* cols, const void* values)
+{
+  require(nblocks >= 0 && m > 0 && n > 0, "bad block shape");
+  if (nblocks == 0)
+    return;
+  const std::int64_t nitems = nblocks * m;
+  require(nitems < (std::int64_t(1) << 40), "too many blocks");
+  for (std::int64_t q = 0; q < nblocks * n; ++q)
+    require(cols[q] >= 0 && cols[q] < A.ncols, "column index out of range");
+  std::vector<std::int32_t> off;
+  std::vector<std::int64_t> item;
+  group_rows(A.nrows, nitems, rows, off, item);
+  const std::size_t ts = dtype_size(A.dtype);
+  const std::size_t nval = std::size_t(nblocks) * m * n * A.bs * A.bs;
+  std::int32_t* d_off = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * off.size()));
+  std::int64_t* d_item = static_cast<std::int64_t*>(dmalloc(sizeof(std::int64_t) * item.size()));
+  std::int32_t* d_cols = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * nblocks * n));
+  void* d_vals = dmalloc(ts * nval);
+  int* err = static_cast<int*>(dmalloc(sizeof(int)));
+  struct Free
+  {
+    std::vector<void*> p;
+    ~Free()
+    {
+      for (void* q : p)
+        dfree(q);
+    }
+  } fr{{d_off, d_item, d_cols, d_vals, err}};
+  h2d(d_off, off.data(), sizeof(std::int32_t) * off.size());
+  h2d(d_item, item.data(), sizeof(std::int64_t) * item.size());
+  h2d(d_cols, cols, sizeof(std::int32_t) * nblocks * n);
+  h2d(d_vals, values, ts * nval);
+  materialize_zero(A);
+  fill_k<int><<<1, kBlock, 0, stream()>>>(err, 1, 0);
+  const std::int64_t nt = std::int64_t(A.nrows) * A.bs;
+  const int nb = static_cast<int>((nt + kBlock - 1) / kBlock);
+  if (nb > 0)
+  {
+    if (A.dtype == DFX_F64)
+      add_blocks_k<double><<<nb, kBlock, 0, stream()>>>(
+          A.nrows, A.bs, m, n, d_off, d_item, d_cols, static_cast<const double*>(d_vals),
+          A.row_ptr, A.cols, static_cast<double*>(A.values), err);
+    else
+      add_blocks_k<float><<<nb, kBlock, 0, stream()>>>(
+          A.nrows, A.bs, m, n, d_off, d_item, d_cols, static_cast<const float*>(d_vals),
+          A.row_ptr, A.cols, static_cast<float*>(A.values), err);
+  }
+  check_launch("add_blocks");
+  A.zeroed = false;
+  int h = 0;
+  d2h(&h, err, sizeof(int));
+  if (h)
+    throw Error(DFX_ERR_PATTERN, "Entry not in sparsity");
+}
+
+void vec_add_blocks(dfx_vec& b, std::int64_t nblocks, int m, int bs, const std::int32_t* rows,
+                    const void* values)
+{
+  require(nblocks >= 0 && m > 0 && bs > 0, "bad block shape");
+  if (nblocks == 0)
+    return;
+  const std::int32_t nrows = static_cast<std::int32_t>(b.n / bs);
+  require(std::int64_t(nrows) * bs == b.n, "vector size is not a multiple of the block size");
+  std::vector<std::int32_t> off;
+  std::vector<std::int64_t> item;
+  group_rows(nrows, nblocks * m, rows, off, item);
+  const std::size_t ts = dtype_size(b.dtype);
+  const std::size_t nval = std::size_t(nblocks) * m * bs;
+  std::int32_t* d_off = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * off.size()));
+  std::int64_t* d_item = static_cast<std::int64_t*>(dmalloc(sizeof(std::int64_t) * item.size()));
+  void* d_vals = dmalloc(ts * nval);
+  struct Free
+  {
+    std::vector<void*> p;
+    ~Free()
+    {
+      for (void* q : p)
+        dfree(q);
+    }
+  } fr{{d_off, d_item, d_vals}};
+  h2d(d_off, off.data(), sizeof(std::int32_t) * off.size());
+  h2d(d_item, item.data(), sizeof(std::int64_t) * item.size());
+  h2d(d_vals, values, ts * nval);
+  const int nb = static_cast<int>((b.n + kBlock - 1) / kBlock);
+  if (nb > 0)
+  {
+    if (b.dtype == DFX_F64)
+      add_vec_blocks_k<double><<<nb, kBlock, 0, stream()>>>(
+          nrows, bs, d_off, d_item, static_cast<const double*>(d_vals), static_cast<double*>(b.data));
+    else
+      add_vec_blocks_k<float><<<nb, kBlock, 0, stream()>>>(
+          nrows, bs, d_off, d_item, static_cast<const float*>(d_vals), static_cast<float*>(b.data));
+  }
+  check_launch("add_vec_blocks");
+  DFX_HIP(hipStreamSynchronize(stream()));
+}
 } // namespace dfx
 
 extern "C" {
@@ -1989,69 +2124,7 @@ dfx_status dfx_csr_add_blocks(dfx_csr* A, int64_t nblocks, int32_t m, int32_t n,
       [&]
       {
         require(A && (nblocks == 0 || (rows && cols && values)), "null argument");
-        require(nblocks >= 0 && m > 0 && n > 0, "bad block shape");
-        if (nblocks == 0)
-          return;
-        const std::int64_t nitems = nblocks * m;
-        require(nitems < (std::int64_t(1) << 40), "too many blocks");
-        // the staged rows grouped by matrix row, in staging order
-        std::vector<std::int32_t> off(std::size_t(A->nrows) + 1, 0);
-        for (std::int64_t q = 0; q < nitems; ++q)
-        {
-          require(rows[q] >= 0 && rows[q] < A->nrows, "row index out of range");
-          ++off[std::size_t(rows[q]) + 1];
-        }
-        for (std::int64_t q = 0; q < std::int64_t(nblocks) * n; ++q)
-          require(cols[q] >= 0 && cols[q] < A->ncols, "column index out of range");
-        for (std::int32_t r = 0; r < A->nrows; ++r)
-          off[std::size_t(r) + 1] += off[r];
-        std::vector<std::int64_t> item(nitems);
-        {
-          std::vector<std::int32_t> pos(off.begin(), off.end() - 1);
-          for (std::int64_t q = 0; q < nitems; ++q)
-            item[pos[rows[q]]++] = q;
-        }
-        const std::size_t ts = dtype_size(A->dtype);
-        const std::size_t nval = std::size_t(nblocks) * m * n * A->bs * A->bs;
-        std::int32_t* d_off = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * off.size()));
-        std::int64_t* d_item = static_cast<std::int64_t*>(dmalloc(sizeof(std::int64_t) * item.size()));
-        std::int32_t* d_cols = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * nblocks * n));
-        void* d_vals = dmalloc(ts * nval);
-        int* err = static_cast<int*>(dmalloc(sizeof(int)));
-        struct Free
-        {
-          std::vector<void*> p;
-          ~Free()
-          {
-            for (void* q : p)
-              dfree(q);
-          }
-        } fr{{d_off, d_item, d_cols, d_vals, err}};
-        h2d(d_off, off.data(), sizeof(std::int32_t) * off.size());
-        h2d(d_item, item.data(), sizeof(std::int64_t) * item.size());
-        h2d(d_cols, cols, sizeof(std::int32_t) * nblocks * n);
-        h2d(d_vals, values, ts * nval);
-        materialize_zero(*A);
-        fill_k<int><<<1, kBlock, 0, stream()>>>(err, 1, 0);
-        const std::int64_t nt = std::int64_t(A->nrows) * A->bs;
-        const int nb = static_cast<int>((nt + kBlock - 1) / kBlock);
-        if (nb > 0)
-        {
-          if (A->dtype == DFX_F64)
-            add_blocks_k<double><<<nb, kBlock, 0, stream()>>>(
-                A->nrows, A->bs, m, n, d_off, d_item, d_cols, static_cast<const double*>(d_vals),
-                A->row_ptr, A->cols, static_cast<double*>(A->values), err);
-          else
-            add_blocks_k<float><<<nb, kBlock, 0, stream()>>>(
-                A->nrows, A->bs, m, n, d_off, d_item, d_cols, static_cast<const float*>(d_vals),
-                A->row_ptr, A->cols, static_cast<float*>(A->values), err);
-        }
-        check_launch("add_blocks");
-        A->zeroed = false;
-        int h = 0;
-        d2h(&h, err, sizeof(int));
-        if (h)
-          throw Error(DFX_ERR_PATTERN, "Entry not in sparsity");
+        csr_add_blocks(*A, nblocks, m, n, rows, cols, values);
       });
 }
 

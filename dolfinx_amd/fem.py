@@ -416,6 +416,63 @@ class KernelForm(Form):
                   None if custom_data is None else C.c_void_p(custom_data), C.byref(self._h))
 
 
+class KernelFacetForm(Form):
+    """fem::Form with an exterior or interior facet integral given as
+    integral_data{kernel, entities} (fem/Form.h:52-87): `entities` are
+    (cell, local facet) pairs ("exterior_facet") or (cell0, local facet0,
+    cell1, local facet1) quads ("interior_facet", e.g. from
+    interior_facet_domain); the kernel has the reference's element kernel
+    ABI (an address or a KERNEL_F64 / KERNEL_F32 function) and is called on
+    the host per entity, as assemble_exterior_facets /
+    assemble_interior_facets do; the tensors are added on the device in
+    entity order. coeffs are packed per cell (cells x cstride); an
+    interior kernel gets both cells' coefficients."""
+
+    def __init__(self, V: FunctionSpace, rank: int, kernel, integral: str, entities,
+                 coeffs=None, constants=None, dtype=np.float64, custom_data=None):
+        if integral not in ("exterior_facet", "interior_facet"):
+            raise ValueError(f"integral {integral!r}")
+        self.name, self.V, self.rank, self.dtype = "kernel", V, rank, np.dtype(dtype)
+        self.integral = integral
+        per = 4 if integral == "interior_facet" else 2
+        ent = np.ascontiguousarray(entities, np.int32).reshape(-1, per)
+        self.facets = ent
+        mesh = V.mesh
+        nc = mesh.topology.cells.shape[0]
+        self.coeffs = (None if coeffs is None
+                       else np.ascontiguousarray(coeffs, np.float64).reshape(nc, -1))
+        self.constants = (np.zeros(0) if constants is None
+                          else np.ascontiguousarray(np.atleast_1d(constants), np.float64))
+        self._kernel = kernel
+        addr = kernel if isinstance(kernel, int) else C.cast(kernel, C.c_void_p).value
+        x = np.ascontiguousarray(mesh.geometry.x)
+        xd = np.ascontiguousarray(mesh.geometry.dofmap)
+        dm = np.ascontiguousarray(V.dofmap.list)
+        self._keep = (x, xd, dm)
+        d = _lib.FormDesc()
+        d.kernel = _lib.K_ELEMENT_A if rank == 2 else _lib.K_ELEMENT_L
+        d.dtype = _lib.code_of(dtype)
+        d.num_cells, d.num_nodes, d.nodes_per_cell = nc, x.shape[0], xd.shape[1]
+        d.x, d.x_dofmap = _lib.ptr(x), _lib.ptr(xd)
+        d.dofs_per_cell, d.bs, d.dofmap = dm.shape[1], V.dofmap.bs, _lib.ptr(dm)
+        d.cstride = 0 if self.coeffs is None else self.coeffs.shape[1]
+        d.coeffs = _lib.ptr(self.coeffs)
+        d.num_constants = len(self.constants)
+        d.constants = _lib.ptr(self.constants) if len(self.constants) else None
+        self._h = C.c_void_p()
+        _lib.call("dfx_form_create_kernel_facets", C.byref(d), C.c_void_p(addr),
+                  None if custom_data is None else C.c_void_p(custom_data),
+                  1 if per == 2 else 2, ent.shape[0], _lib.ptr(ent), C.byref(self._h))
+
+
+def kernel_facet_form(V: FunctionSpace, rank: int, kernel, integral: str, entities, coeffs=None,
+                      constants=None, dtype=np.float64, custom_data=None) -> KernelFacetForm:
+    """A facet integral whose tensors come from a kernel with the reference's
+    ABI (see KernelFacetForm)."""
+    return KernelFacetForm(V, rank, kernel, integral, entities, coeffs, constants, dtype,
+                           custom_data)
+
+
 def kernel_form(V: FunctionSpace, rank: int, kernel, coeffs=None, constants=None,
                 dtype=np.float64, custom_data=None) -> KernelForm:
     """A form whose cell tensors come from an element kernel with the

@@ -297,6 +297,24 @@ typedef void (*dfx_kernel_f32)(float* A, const float* w, const float* c,
  * do. The kernel must be callable from the calling thread. */
 dfx_status dfx_form_create_kernel(const dfx_form_desc* desc, void* kernel, void* custom_data,
                                   dfx_form** out);
+/* fem::Form with an exterior (type 1) or interior (type 2) facet integral
+ * given as integral_data{kernel, entities} (fem/Form.h:52-87): entities
+ * are (cell, local facet) pairs (type 1) or (cell0, local facet0, cell1,
+ * local facet1) quads (type 2) in integration order. The kernel has the
+ * reference's ABI and is called on the host per entity in that order as
+ * assemble_exterior_facets / assemble_interior_facets call it
+ * (fem/assemble_matrix_impl.h:264-383, :442-667;
+ * fem/assemble_vector_impl.h:118-174): coordinate_dofs of the cell (type 2:
+ * both cells, cell0 first), w the cell's packed coefficients (type 2: both
+ * cells'), entity_local_index the local facet(s), quadrature_permutation
+ * zeros. Matrices ((bs k)^2, type 2: (2 bs k)^2 over the joint dofmap of
+ * the two cells) get the rows and columns of Dirichlet dofs zeroed and are
+ * added on the device with every entry summed in entity order, as
+ * mat_set(dmapjoint0, dmapjoint1, Ae); vectors (type 1 only) likewise.
+ * dfx_apply_lifting through such a form is supported only for zero data. */
+dfx_status dfx_form_create_kernel_facets(const dfx_form_desc* desc, void* kernel,
+                                         void* custom_data, int32_t type, int32_t nentities,
+                                         const int32_t* entities, dfx_form** out);
 dfx_status dfx_form_destroy(dfx_form* f);
 /* Replace packed coefficients / constants (same sizes as at creation). */
 dfx_status dfx_form_set_coefficients(dfx_form* f, const double* coeffs,
