@@ -7,11 +7,14 @@
 // own device kernels for the same forms; and the same stiffness kernel
 // called in a host loop over the cells whose element matrices go through
 // the matrix's MatSet functor (A.mat_add_values(), then scatter_rev), as
-// the reference's custom assembly loop does.
+// the reference's custom assembly loop does. An exterior facet integral,
+// beta u v ds on every boundary facet, is given the same way: a
+// hand-written facet kernel called per (cell, local facet) pair.
 // Usage: custom_kernel_demo <n> <out.bin>
 // Writes the matrix and vector values (FP64) to out.bin: kernel-ABI form,
-// its vector, device kernels, their vector, then the host-loop matrix and
-// the kernel-ABI form without BCs.
+// its vector, device kernels, their vector, the host-loop matrix, the
+// kernel-ABI form without BCs, then the facet matrix from the facet kernel
+// and from the library's device Robin kernel.
 #include <dolfinx_amd/dolfinx_amd.h>
 
 #include <algorithm>
@@ -77,6 +80,29 @@ void load_kernel(double* b, const double* w, const double*, const double* X, con
   const double sum = w[0] + w[1] + w[2] + w[3];
   for (int i = 0; i < 4; ++i)
     b[i] += vol / 20.0 * (w[i] + sum);
+}
+/// beta u v ds on local facet lf of a P1 tetrahedron (the facet opposite
+/// vertex lf): |F|/12 (1 + delta_ij) beta for i, j on the facet; w = [beta].
+void robin_kernel(double* A, const double* w, const double*, const double* X, const int* lf,
+                  const std::uint8_t*, void*)
+{
+  int v[3], k = 0;
+  for (int i = 0; i < 4; ++i)
+    if (i != lf[0])
+      v[k++] = i;
+  double e1[3], e2[3];
+  for (int d = 0; d < 3; ++d)
+  {
+    e1[d] = X[3 * v[1] + d] - X[3 * v[0] + d];
+    e2[d] = X[3 * v[2] + d] - X[3 * v[0] + d];
+  }
+  const double n0 = e1[1] * e2[2] - e1[2] * e2[1];
+  const double n1 = e1[2] * e2[0] - e1[0] * e2[2];
+  const double n2 = e1[0] * e2[1] - e1[1] * e2[0];
+  const double area = 0.5 * std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      A[4 * v[i] + v[j]] += w[0] * area / 12.0 * (i == j ? 2.0 : 1.0);
 }
 } // namespace
 
@@ -145,8 +171,22 @@ int main(int argc, char* argv[])
     la::MatrixCSR<double> An(fem::create_sparsity_pattern(ak));
     fem::assemble_matrix(An.mat_add_values(), ak, {});
 
+    // beta u v ds on the boundary: facet kernel and device kernel
+    std::vector<double> beta(nc);
+    for (std::int32_t c = 0; c < nc; ++c)
+      beta[c] = 2.0 + std::cos(double(orig[c]));
+    const std::vector<std::int32_t> pairs = mesh::facet_domain(*msh);
+    fem::Form<double> rk(&robin_kernel, 2, V, beta, 1, {}, fem::IntegralType::exterior_facet,
+                         pairs);
+    fem::Form<double> rd(DFX_K_P1TET_ROBIN_A, V, beta, 1, {}, pairs, true);
+    la::MatrixCSR<double> Rk(fem::create_sparsity_pattern(rk));
+    fem::assemble_matrix(Rk.mat_add_values(), rk, {});
+    la::MatrixCSR<double> Rd(fem::create_sparsity_pattern(rd));
+    fem::assemble_matrix(Rd.mat_add_values(), rd, {});
+
     const std::vector<double> vk = Ak.values(), vd = Ad.values();
     const std::vector<double> vh = Ah.values(), vn = An.values();
+    const std::vector<double> rvk = Rk.values(), rvd = Rd.values();
     double dmax = 0, amax = 0;
     for (std::size_t e = 0; e < vk.size(); ++e)
     {
@@ -159,7 +199,7 @@ int main(int argc, char* argv[])
     std::cout << "{\"nnz\": " << vk.size() << ", \"max_diff_rel\": " << dmax / amax
               << ", \"host_loop_max_diff_rel\": " << hmax / amax << "}" << std::endl;
     std::ofstream out(argv[2], std::ios::binary);
-    for (const auto& v : {vk, bk.array(), vd, bd.array(), vh, vn})
+    for (const auto& v : {vk, bk.array(), vd, bd.array(), vh, vn, rvk, rvd})
     {
       const std::int64_t m = static_cast<std::int64_t>(v.size());
       out.write(reinterpret_cast<const char*>(&m), sizeof(m));

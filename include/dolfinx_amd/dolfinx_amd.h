@@ -494,6 +494,14 @@ private:
   std::shared_ptr<dfx_bc> _h;
 };
 
+/// fem::IntegralType (fem/Form.h:33-39): where an integral is taken
+enum class IntegralType : std::int32_t
+{
+  cell = 0,
+  exterior_facet = 1,
+  interior_facet = 2
+};
+
 /// fem::Form whose single cell integral is one of the hand-written device
 /// kernels (DFX_K_*), with packed coefficients (cells x cstride, in the
 /// mesh's cell order) and constants.
@@ -566,9 +574,16 @@ public:
   /// integral_data{kernel, cells} (fem/Form.h:52-87,
   /// cpp/demo/custom_kernel/main.cpp:77-81): rank 2 (bilinear, (bs k)^2
   /// row-major tensors) or 1. The kernel runs per cell on the host; its
-  /// tensors are inserted on the device.
+  /// tensors are inserted on the device. With type exterior_facet or
+  /// interior_facet the integral is taken over `entities` instead
+  /// (integral_data{kernel, entities}: flattened (cell, local facet) pairs,
+  /// or (cell0, lf0, cell1, lf1) quads for a bilinear interior facet
+  /// integral), the kernel called per facet with the local facet
+  /// index (indices) as assemble_exterior_facets / assemble_interior_facets
+  /// call it; coefficients stay packed per cell.
   Form(kernel_ptr kernel, int rank, std::shared_ptr<const FunctionSpace> V,
-       std::vector<double> coeffs, int cstride, std::vector<double> constants)
+       std::vector<double> coeffs, int cstride, std::vector<double> constants,
+       IntegralType type = IntegralType::cell, std::span<const std::int32_t> entities = {})
       : _V(std::move(V)), _rank(rank)
   {
     auto m = _V->mesh();
@@ -591,7 +606,18 @@ public:
     d.num_constants = static_cast<std::int32_t>(constants.size());
     d.constants = constants.empty() ? nullptr : constants.data();
     dfx_form* h = nullptr;
-    check(dfx_form_create_kernel(&d, reinterpret_cast<void*>(kernel), nullptr, &h));
+    if (type == IntegralType::cell)
+      check(dfx_form_create_kernel(&d, reinterpret_cast<void*>(kernel), nullptr, &h));
+    else
+    {
+      const std::size_t per = type == IntegralType::interior_facet ? 4 : 2;
+      if (per == 4)
+        _interior.assign(entities.begin(), entities.end());
+      check(dfx_form_create_kernel_facets(&d, reinterpret_cast<void*>(kernel), nullptr,
+                                          static_cast<std::int32_t>(type),
+                                          static_cast<std::int32_t>(entities.size() / per),
+                                          entities.data(), &h));
+    }
     _h.reset(h, dfx_form_destroy);
   }
   /// The (cell0, lf0, cell1, lf1) quads of an interior facet integral.

@@ -61,7 +61,10 @@ def test_cpp_custom_kernel_forms(tmp_path):
     forms and the oracle; the same kernel in a host loop over the cells,
     its element matrices added through A.mat_add_values() (the MatSet
     functor) and scatter_rev, gives the bits of the form without BCs, and
-    the oracle's cell-by-cell insertion."""
+    the oracle's cell-by-cell insertion. A hand-written exterior facet
+    kernel (beta u v ds, fem::Form(kernel_ptr, ..., IntegralType::
+    exterior_facet, pairs)) equals the device Robin kernel and the
+    oracle's facet assembly."""
     exe = os.path.join(ROOT, "dolfinx_amd", "_build", "custom_kernel_demo")
     assert os.path.exists(exe), "build the examples first (dolfinx_amd.build)"
     n = 6
@@ -73,10 +76,10 @@ def test_cpp_custom_kernel_forms(tmp_path):
     assert info["host_loop_max_diff_rel"] == 0.0
     with open(out, "rb") as fh:
         arrs = []
-        for _ in range(6):
+        for _ in range(8):
             k = int(np.frombuffer(fh.read(8), np.int64)[0])
             arrs.append(np.frombuffer(fh.read(8 * k), np.float64))
-    Ak, bk, Ad, bd, Ah, An = arrs
+    Ak, bk, Ad, bd, Ah, An, Rk, Rd = arrs
     assert np.array_equal(Ah, An)
     assert rel_err(bk, bd) <= 1e-13
     m = O.Mesh((n, n, n), O.TET)
@@ -94,3 +97,8 @@ def test_cpp_custom_kernel_forms(tmp_path):
     Arn = O.assemble_matrix(O.K_P1TET_A, m.x, m.cells, dm, 1, kap, np.array([0.75]), none, none,
                             rp, cols)
     assert rel_err(Ah, Arn) <= 1e-12
+    assert rel_err(Rk, Rd) <= 1e-13
+    beta = (2.0 + np.cos(m.original_cell_index.astype(float))).reshape(-1, 1)
+    Rr = O.assemble_matrix_facets(O.K_P1TET_ROBIN_A, m.x, m.cells, dm, beta,
+                                  m.facet_domain(), None, None, rp, cols)
+    assert rel_err(Rk, Rr) <= 1e-12
