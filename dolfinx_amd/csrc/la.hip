@@ -1155,10 +1155,12 @@ __global__ void cg_resume_k(CGState* st, double rr0, double tol2, int kd)
   }
 }
 
-/// Blocks of a reduction kernel: at most kMaxPartials.
+/// Blocks of a reduction kernel: one element per thread up to kMaxPartials
+/// blocks (a small system spread over many CUs: each element's loads are
+/// one dependent round trip, so per-thread loops would serialise them).
 int partial_blocks(std::int64_t n)
 {
-  std::int64_t b = (n + kBlock * 8 - 1) / (kBlock * 8);
+  std::int64_t b = (n + kBlock - 1) / kBlock;
   return static_cast<int>(std::clamp<std::int64_t>(b, 1, 1024));
 }
 
