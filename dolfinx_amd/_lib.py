@@ -38,12 +38,12 @@ class FormDesc(C.Structure):
 
 class CGOpts(C.Structure):
     _fields_ = [("max_it", I32), ("rtol", D), ("jacobi", I32), ("check_every", I32),
-                ("graph", I32)]
+                ("graph", I32), ("classic", I32)]
 
 
 class CGResult(C.Structure):
     _fields_ = [("iterations", I32), ("converged", I32), ("rnorm0", D), ("rnorm", D),
-                ("true_rel_residual", D), ("seconds", D)]
+                ("true_rel_residual", D), ("seconds", D), ("cooperative", I32)]
 
 
 class MixedOpts(C.Structure):

@@ -31,6 +31,7 @@ struct Context
   bool ready = false;
   int device = 0;
   int num_cu = 256;
+  bool cooperative = false; // cooperative launches (co-resident grids) supported
   hipStream_t stream = nullptr;
   hipStream_t aux = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;

@@ -64,6 +64,7 @@ void ensure_init()
   hipDeviceProp_t prop;
   DFX_HIP(hipGetDeviceProperties(&prop, 0));
   g_ctx.num_cu = prop.multiProcessorCount;
+  g_ctx.cooperative = prop.cooperativeLaunch != 0;
   DFX_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
   DFX_HIP(hipStreamCreateWithFlags(&g_ctx.aux, hipStreamNonBlocking));
   DFX_HIP(hipEventCreateWithFlags(&g_ctx.fork, hipEventDisableTiming));
@@ -1164,6 +1165,319 @@ int partial_blocks(std::int64_t n)
   return static_cast<int>(std::clamp<std::int64_t>(b, 1, 1024));
 }
 
+// ---------------------------------------------------------------------
+// CG of a small scalar system in one cooperative launch. A system of a few
+// ten thousand rows lives in the L2 caches, and each kernel of the
+// three-kernel iteration is a short chain of dependent round trips plus a
+// launch; here every block stays resident for the whole solve and the
+// phases are separated by grid barriers, two per iteration:
+//   phase A  p = z + beta p_old on the block's rows (z = dinv r), written
+//            to the other of two p buffers; q = A p for its rows, with p
+//            of the neighbour columns formed the same way from z and
+//            p_old; partial sums of p.q                         | barrier
+//   phase B  alpha = rz / p.q (every block sums the partials in the same
+//            order); x += alpha p, r -= alpha q, z = dinv r (written);
+//            partials of r.z and r.r                            | barrier
+// then every block sums rz, rr identically and takes the same stop
+// decision (||r||^2/||r0||^2 < rtol^2 or max_it), so all blocks leave the
+// loop at the same iteration. x, r, dinv and the block's own p and q stay
+// in registers (at most RPT rows per thread). The barrier counters only
+// grow (barrier e completes when the top counter reaches e x groups)
+// under agent-scope release/acquire; a wait gives up after a bounded
+// number of polls and raises a flag that every block sees, so the grid
+// always drains. Same stop test and FP64 reductions as the
+// three-kernel path; sums are grouped differently, so the last bits (and
+// at most an iteration) may differ from it.
+// ---------------------------------------------------------------------
+
+/// Barrier state and results of the cooperative CG (zeroed before each
+/// launch): blocks arrive on one of 8 group counters (block b on group
+/// b mod 8, each on its own 128-byte line, so arrivals on different
+/// counters proceed in parallel); the last arrival of a group advances
+/// the top counter, which every block polls.
+struct CoopSync
+{
+  unsigned int top;
+  int failed;
+  std::int32_t iter;
+  std::int32_t pad0;
+  double rr0, rr;
+  unsigned int pad1[24];
+  unsigned int group[8 * 32];
+};
+
+constexpr unsigned int kCoopSpinLimit = 1u << 22;
+constexpr int kCoopMaxRPT = 4;
+constexpr std::int32_t kCoopMaxRows = 65536;
+
+/// Grid barrier of the cooperative CG: false if it timed out (or another
+/// block's did), in which case the caller returns. Every wavefront waits
+/// for its own memory operations before the block barrier, so thread 0's
+/// release (an L2 write-back at agent scope) covers the block's stores;
+/// its acquire (an invalidate of the CU's caches) precedes every load of
+/// the block after the barrier.
+__device__ inline bool coop_barrier(CoopSync* g, unsigned int nb, unsigned int& epoch)
+{
+  __shared__ int ok;
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  epoch += 1;
+  if (threadIdx.x == 0)
+  {
+    const unsigned int grp = blockIdx.x & 7u;
+    const unsigned int gsize = (nb - grp + 7u) / 8u;
+    const unsigned int ngroups = nb < 8u ? nb : 8u;
+    const unsigned int old = __hip_atomic_fetch_add(&g->group[32 * grp], 1u, __ATOMIC_ACQ_REL,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+    if (old == epoch * gsize - 1u)
+      __hip_atomic_fetch_add(&g->top, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int target = epoch * ngroups;
+    int good = 1;
+    unsigned int spins = 0;
+    while (__hip_atomic_load(&g->top, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
+    {
+      if (__hip_atomic_load(&g->failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+          || ++spins > kCoopSpinLimit)
+      {
+        __hip_atomic_store(&g->failed, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        good = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    ok = good;
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+/// p of row j in iteration k: z_j (k = 0) or z_j + beta p_old_j, one
+/// rounding pattern wherever it is formed.
+template <typename T>
+__device__ inline T coop_dir(T z, T pold, T beta, bool first)
+{
+  return first ? z : __builtin_fma(beta, pold, z);
+}
+
+template <typename T, int RPT>
+__global__ void __launch_bounds__(kBlock)
+    cg_coop_k(std::int32_t n, const std::int32_t* __restrict__ rp,
+              const std::int32_t* __restrict__ ci, const T* __restrict__ v,
+              const std::int32_t* __restrict__ diag, int jacobi, const T* __restrict__ b,
+              T* __restrict__ x, T* z, T* p0, T* p1, double* parts, CoopSync* g, int max_it,
+              double tol2)
+{
+  const unsigned int nb = gridDim.x;
+  const std::int32_t nt = static_cast<std::int32_t>(nb) * kBlock;
+  const std::int32_t t0 = static_cast<std::int32_t>(blockIdx.x) * kBlock + threadIdx.x;
+  unsigned int epoch = 0;
+  T xr[RPT], rr_[RPT], dr[RPT], pr[RPT], qr[RPT];
+  // r = b - A x, z = dinv r; partials of r.z, r.r
+  double s[2] = {0.0, 0.0};
+#pragma unroll
+  for (int u = 0; u < RPT; ++u)
+  {
+    const std::int32_t i = t0 + u * nt;
+    xr[u] = rr_[u] = dr[u] = pr[u] = qr[u] = T(0);
+    if (i < n)
+    {
+      T ax = 0;
+      for (std::int32_t e = rp[i]; e < rp[i + 1]; ++e)
+        ax += v[e] * x[ci[e]];
+      xr[u] = x[i];
+      rr_[u] = b[i] - ax;
+      T d = T(1);
+      if (jacobi)
+      {
+        const std::int32_t pd = diag[i];
+        const T a = pd >= 0 ? v[pd] : T(0);
+        d = a != T(0) ? T(1) / a : T(1);
+      }
+      dr[u] = d;
+      const T zi = d * rr_[u];
+      z[i] = zi;
+      s[0] += double(rr_[u]) * double(zi);
+      s[1] += double(rr_[u]) * double(rr_[u]);
+    }
+  }
+  block_partials<2>(s, parts + nb);
+  if (!coop_barrier(g, nb, epoch))
+    return;
+  double rz = sum_partials(parts + nb, nb);
+  const double rr0 = sum_partials(parts + 2 * nb, nb);
+  double rr = rr0;
+  int k = 0;
+  bool done = rr0 == 0.0 || max_it <= 0;
+  T beta = T(0);
+  while (!done)
+  {
+    const bool first = k == 0;
+    T* pn = (k & 1) ? p1 : p0;
+    const T* po = (k & 1) ? p0 : p1;
+    // phase A
+    double d = 0.0;
+#pragma unroll
+    for (int u = 0; u < RPT; ++u)
+    {
+      const std::int32_t i = t0 + u * nt;
+      if (i < n)
+      {
+        const T pi = coop_dir<T>(dr[u] * rr_[u], pr[u], beta, first);
+        pr[u] = pi;
+        pn[i] = pi;
+        // the row in chunks of 8 entries: the chunk's columns and values,
+        // then all its gathers in flight together, then the sums in order
+        T q = 0;
+        const std::int32_t re = rp[i + 1];
+        for (std::int32_t e0 = rp[i]; e0 < re; e0 += 8)
+        {
+          std::int32_t jj[8];
+          T vv[8], zz[8], pp[8];
+#pragma unroll
+          for (int h = 0; h < 8; ++h)
+          {
+            const bool in = e0 + h < re;
+            jj[h] = in ? ci[e0 + h] : i;
+            vv[h] = in ? v[e0 + h] : T(0);
+          }
+#pragma unroll
+          for (int h = 0; h < 8; ++h)
+          {
+            zz[h] = z[jj[h]];
+            pp[h] = first ? T(0) : po[jj[h]];
+          }
+#pragma unroll
+          for (int h = 0; h < 8; ++h)
+            if (e0 + h < re)
+              q += vv[h] * coop_dir<T>(zz[h], pp[h], beta, first);
+        }
+        qr[u] = q;
+        d += double(q) * double(pi);
+      }
+    }
+    {
+      double dd[1] = {d};
+      block_partials<1>(dd, parts);
+    }
+    if (!coop_barrier(g, nb, epoch))
+      return;
+    // phase B
+    const double pq = sum_partials(parts, nb);
+    const T alpha = static_cast<T>(rz / pq);
+    s[0] = s[1] = 0.0;
+#pragma unroll
+    for (int u = 0; u < RPT; ++u)
+    {
+      const std::int32_t i = t0 + u * nt;
+      if (i < n)
+      {
+        xr[u] += alpha * pr[u];
+        rr_[u] = rr_[u] - alpha * qr[u];
+        const T zi = dr[u] * rr_[u];
+        z[i] = zi;
+        s[0] += double(rr_[u]) * double(zi);
+        s[1] += double(rr_[u]) * double(rr_[u]);
+      }
+    }
+    block_partials<2>(s, parts + nb);
+    if (!coop_barrier(g, nb, epoch))
+      return;
+    const double rzn = sum_partials(parts + nb, nb);
+    rr = sum_partials(parts + 2 * nb, nb);
+    done = rr / rr0 < tol2 || k + 1 >= max_it;
+    beta = static_cast<T>(rzn / rz);
+    rz = rzn;
+    ++k;
+  }
+#pragma unroll
+  for (int u = 0; u < RPT; ++u)
+  {
+    const std::int32_t i = t0 + u * nt;
+    if (i < n)
+      x[i] = xr[u];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+  {
+    g->iter = k;
+    g->rr0 = rr0;
+    g->rr = rr;
+  }
+}
+
+/// Rows per thread and blocks of the cooperative CG for A, or 0 when the
+/// system is not one for it (block size > 1, too large for a co-resident
+/// grid, no cooperative launch, or a barrier counter that could wrap).
+template <typename T>
+std::pair<int, int> coop_shape(const dfx_csr& A, int max_it)
+{
+  if (A.bs != 1 || A.nrows == 0 || A.nrows > kCoopMaxRows || !ctx().cooperative)
+    return {0, 0};
+  const int need = (A.nrows + kBlock - 1) / kBlock;
+  // a barrier costs more the more blocks arrive (measured on MI355X: 5.8 us
+  // at 141 blocks, 3.4 us at 64, 2.5 us at 32): two rows per thread
+  // once a system needs more than 64 blocks of one row per thread
+  int rpt = need > 64 ? 2 : 1;
+  int cap = 0;
+  for (; rpt <= kCoopMaxRPT; rpt *= 2)
+  {
+    const void* kern = rpt == 1   ? reinterpret_cast<const void*>(cg_coop_k<T, 1>)
+                       : rpt == 2 ? reinterpret_cast<const void*>(cg_coop_k<T, 2>)
+                                  : reinterpret_cast<const void*>(cg_coop_k<T, 4>);
+    cap = std::min(resident_blocks(kern, kBlock, 0), kMaxPartials);
+    if ((need + rpt - 1) / rpt <= cap)
+      break;
+  }
+  if (rpt > kCoopMaxRPT)
+    return {0, 0};
+  const int nb = (need + rpt - 1) / rpt;
+  // 2 barriers per iteration plus 2: the counter reaches (2 max_it + 2) nb
+  if ((2.0 * max_it + 4.0) * nb >= 4294967295.0)
+    return {0, 0};
+  return {rpt, nb};
+}
+
+/// The cooperative CG (coop_shape gave rpt, nb); work vectors from w.
+template <typename T>
+int cg_coop_run(const dfx_csr& A, const T* b, T* x, int max_it, double rtol, bool jacobi,
+                int rpt, int nb, T* z, T* p0, T* p1, double* parts, CoopSync* g,
+                bool* converged, double* rr0, double* rr)
+{
+  hipStream_t s = stream();
+  fill_k<unsigned int><<<1, kBlock, 0, s>>>(reinterpret_cast<unsigned int*>(g),
+                                            sizeof(CoopSync) / sizeof(unsigned int), 0u);
+  check_launch("cg_coop set-up");
+  std::int32_t n = A.nrows;
+  const std::int32_t* rpp = A.row_ptr;
+  const std::int32_t* cip = A.cols;
+  const T* vp = static_cast<const T*>(A.values);
+  const std::int32_t* dg = A.diag;
+  int jac = jacobi ? 1 : 0;
+  double tol2 = rtol * rtol;
+  void* args[] = {&n, &rpp, &cip, &vp, &dg, &jac, &b, &x, &z, &p0, &p1, &parts, &g, &max_it,
+                  &tol2};
+  const void* kern = rpt == 1   ? reinterpret_cast<const void*>(cg_coop_k<T, 1>)
+                     : rpt == 2 ? reinterpret_cast<const void*>(cg_coop_k<T, 2>)
+                                : reinterpret_cast<const void*>(cg_coop_k<T, 4>);
+  const hipError_t le = hipLaunchCooperativeKernel(kern, dim3(nb), dim3(kBlock), args, 0, s);
+  if (le != hipSuccess)
+  {
+    // refused (the process cannot launch co-resident grids): the caller
+    // runs the three-kernel iteration, here and for later solves
+    (void)hipGetLastError();
+    ctx().cooperative = false;
+    return -1;
+  }
+  CoopSync h{};
+  DFX_HIP(hipMemcpyAsync(&h, g, sizeof(CoopSync), hipMemcpyDeviceToHost, s));
+  DFX_HIP(hipStreamSynchronize(s));
+  if (h.failed)
+    throw Error(DFX_ERR_HIP, "cooperative CG: a grid barrier timed out");
+  *rr0 = h.rr0;
+  *rr = h.rr;
+  *converged = h.rr0 == 0.0 || h.rr / h.rr0 < rtol * rtol;
+  return h.iter;
+}
+
 /// Per-dtype scratch of a CG solve (device).
 template <typename T>
 struct CGWork
@@ -1528,12 +1842,32 @@ void cg_solve_t(const dfx_csr& A, const dfx_vec& b, dfx_vec& x,
   // the timed window); its values are copied inside
   sell_layout<T>(A);
   DFX_HIP(hipStreamSynchronize(stream()));
+  const auto [rpt, nbc] = o.classic ? std::pair<int, int>{0, 0} : coop_shape<T>(A, o.max_it);
+  T* p1 = nullptr;
+  CoopSync* cs = nullptr;
+  if (rpt)
+  {
+    p1 = static_cast<T*>(dmalloc(sizeof(T) * std::max<std::int64_t>(n, 1)));
+    cs = static_cast<CoopSync*>(dmalloc(sizeof(CoopSync)));
+  }
+  struct Free
+  {
+    void* p;
+    ~Free() { dfree(p); }
+  } free_p1{p1}, free_cs{cs};
+  DFX_HIP(hipStreamSynchronize(stream()));
   const auto t0 = std::chrono::steady_clock::now();
   bool conv = false;
   double rr0 = 0, rr = 0;
-  const int it = cg_run<T>(A, static_cast<const T*>(b.data), static_cast<T*>(x.data), o.max_it,
-                           o.rtol, o.jacobi != 0, o.check_every, w, sh.st, &conv, &rr0, &rr,
-                           false, true, o.graph != 0);
+  int it = rpt ? cg_coop_run<T>(A, static_cast<const T*>(b.data), static_cast<T*>(x.data),
+                                o.max_it, o.rtol, o.jacobi != 0, rpt, nbc, w.r, w.p, p1,
+                                sh.st->partials, cs, &conv, &rr0, &rr)
+               : -1;
+  res.cooperative = it >= 0 ? 1 : 0;
+  if (it < 0)
+    it = cg_run<T>(A, static_cast<const T*>(b.data), static_cast<T*>(x.data), o.max_it, o.rtol,
+                   o.jacobi != 0, o.check_every, w, sh.st, &conv, &rr0, &rr, false, true,
+                   o.graph != 0);
   // the true residual ||b - A x|| / ||b|| that ends the solve
   const double bn = std::sqrt(red.dot(static_cast<const T*>(b.data), static_cast<const T*>(b.data), n));
   const double tr = true_residual<T>(A, static_cast<const T*>(b.data), static_cast<const T*>(x.data),

@@ -15,12 +15,15 @@ from .la import MatrixCSR, Vector
 
 
 def cg(A: MatrixCSR, b: Vector, x: Vector, rtol=1e-8, max_it=10000, jacobi=True,
-       check_every=32, graph=False):
-    """CG (Jacobi-preconditioned by default). graph=True replays each batch
-    of check_every iterations from a captured HIP graph (same kernels and
-    results; the capture costs about a millisecond per solve)."""
+       check_every=32, graph=False, classic=False):
+    """CG (Jacobi-preconditioned by default). A scalar system of at most
+    65,536 rows is solved in one cooperative launch unless classic=True;
+    otherwise three kernels per iteration, and graph=True replays each
+    batch of check_every iterations from a captured HIP graph (same kernels
+    and results; the capture costs about a millisecond per solve)."""
     opts = _lib.CGOpts(max_it=int(max_it), rtol=float(rtol), jacobi=1 if jacobi else 0,
-                       check_every=int(check_every), graph=1 if graph else 0)
+                       check_every=int(check_every), graph=1 if graph else 0,
+                       classic=1 if classic else 0)
     res = _lib.CGResult()
     _lib.call("dfx_cg_solve", A.handle, b.handle, x.handle, C.byref(opts), C.byref(res))
     return res

@@ -427,6 +427,10 @@ typedef struct
   int32_t graph;       /* 1: capture check_every iterations once as a HIP graph
                           and replay it (same kernels, same results);
                           0: launch every kernel */
+  int32_t classic;     /* 0: a scalar system of at most 65,536 rows is solved
+                          in one cooperative launch (resident blocks, grid
+                          barriers; check_every and graph do not apply);
+                          1: always the three-kernel iteration */
 } dfx_cg_opts;
 
 typedef struct
@@ -437,6 +441,7 @@ typedef struct
   double rnorm;        /* ||r_k||_2 (recursive, FP64) */
   double true_rel_residual; /* ||b - A x||_2 / ||b||_2 evaluated in FP64 */
   double seconds;      /* wall time of the solve */
+  int32_t cooperative; /* 1: solved in one cooperative launch */
 } dfx_cg_result;
 
 /* CG of the reference (cpp/demo/poisson_matrix_free/main.cpp:84-132,

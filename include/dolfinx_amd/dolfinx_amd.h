@@ -764,7 +764,7 @@ template <typename T>
 int cg(la::Vector<T>& x, const la::Vector<T>& b, const la::MatrixCSR<T>& A, int kmax = 50,
        double rtol = 1e-8, bool jacobi = false)
 {
-  dfx_cg_opts o{kmax, rtol, jacobi ? 1 : 0, 32, 0};
+  dfx_cg_opts o{kmax, rtol, jacobi ? 1 : 0, 32, 0, 0};
   dfx_cg_result r{};
   check(dfx_cg_solve(A.handle(), b.handle(), x.handle(), &o, &r));
   return r.iterations;

@@ -139,18 +139,62 @@ def test_cg_graph_replay_equals_eager(cell, n, jacobi):
     between) and over repeated solves."""
     p, A, b = system(cell, n, kappa="exp_normal", c=0.3, f="normal")
     xe = la.Vector(p.V.num_dofs)
-    re = solvers.cg(A, b, xe, rtol=1e-11, max_it=20000, jacobi=jacobi, check_every=8, graph=False)
+    re = solvers.cg(A, b, xe, rtol=1e-11, max_it=20000, jacobi=jacobi, check_every=8, graph=False,
+                    classic=True)
     for _ in range(2):
         xg = la.Vector(p.V.num_dofs)
         rg = solvers.cg(A, b, xg, rtol=1e-11, max_it=20000, jacobi=jacobi, check_every=8,
-                        graph=True)
+                        graph=True, classic=True)
         assert rg.iterations == re.iterations and rg.iterations > 3 * 8
         assert np.array_equal(xg.array, xe.array)
         assert rg.true_rel_residual == re.true_rel_residual
     # max_it inside a replayed batch stops at max_it exactly
     xm = la.Vector(p.V.num_dofs)
     with pytest.raises(DfxError, match="exceeded max iterations"):
-        solvers.cg(A, b, xm, rtol=1e-14, max_it=13, check_every=8, graph=True)
+        solvers.cg(A, b, xm, rtol=1e-14, max_it=13, check_every=8, graph=True, classic=True)
+
+
+@pytest.mark.parametrize("cell,n,jacobi", [(O.TET, (16, 14, 12), True), (O.HEX, (24, 20, 18), True),
+                                           (O.TET, (10, 9, 8), False)])
+@pytest.mark.parametrize("dtype,rtol,tol", [(np.float64, 1e-11, 1e-9), (np.float32, 1e-5, 2e-4)])
+def test_cooperative_cg_matches_classic_and_oracle(cell, n, jacobi, dtype, rtol, tol):
+    """The one-launch cooperative CG of small scalar systems (grid barriers
+    between the phases) against the three-kernel iteration and the
+    oracle's PCG, from x = 0 and from a nonzero initial guess; repeated
+    solves are bitwise equal."""
+    p = problems.scalar_problem(n, cell, kappa="exp_normal", c=0.3, f="normal", seed=9,
+                                dtype=dtype)
+    A, b = problems.assemble_system(p)
+    N = p.V.num_dofs
+    assert N <= 65536
+    xc = la.Vector(N, dtype)
+    rc = solvers.cg(A, b, xc, rtol=rtol, max_it=20000, jacobi=jacobi, classic=True)
+    xs = []
+    for _ in range(2):
+        x = la.Vector(N, dtype)
+        r = solvers.cg(A, b, x, rtol=rtol, max_it=20000, jacobi=jacobi)
+        assert r.cooperative == 1 and rc.cooperative == 0
+        assert r.converged and abs(r.iterations - rc.iterations) <= 2
+        xs.append(x.array.copy())
+    assert np.array_equal(xs[0], xs[1])
+    assert rel_err(xs[0], xc.array) <= tol
+    if dtype == np.float64:
+        xr, it = O.cg(A.indptr, A.indices, A.data, 1, b.array, kmax=20000, rtol=1e-12,
+                      jacobi=jacobi)
+        assert rel_err(xs[0], xr) <= 1e-8
+    # a nonzero initial guess
+    x0 = np.random.default_rng(3).standard_normal(N).astype(dtype)
+    xa, xb = la.Vector.from_array(x0.copy()), la.Vector.from_array(x0.copy())
+    ra = solvers.cg(A, b, xa, rtol=rtol, max_it=20000, jacobi=jacobi)
+    rb = solvers.cg(A, b, xb, rtol=rtol, max_it=20000, jacobi=jacobi, classic=True)
+    assert abs(ra.iterations - rb.iterations) <= 2
+    # from a random guess the initial residual is large: FP32 iterates
+    # stop at rtol 1e-5 of it, so the two solutions differ more
+    assert rel_err(xa.array, xb.array) <= (tol if dtype == np.float64 else 10 * tol)
+    assert ra.rnorm0 == pytest.approx(rb.rnorm0, rel=1e-6)
+    # max_it reached inside the launch
+    with pytest.raises(DfxError, match="exceeded max iterations"):
+        solvers.cg(A, b, la.Vector(N, dtype), rtol=1e-30, max_it=7, jacobi=jacobi)
 
 
 @pytest.mark.parametrize("restart", [True, False])

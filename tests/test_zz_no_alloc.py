@@ -6,8 +6,10 @@ Two calls in a row, the second on a smaller system (the array keeps the
 first call's contents, so the state must be cleared by the call). With the
 SpMV set to two lanes per row both entry points run the same kernels:
 iterates are bitwise equal. With the default SpMV, dfx_cg_solve uses its
-sliced-ELL copy (other reduction order): solutions agree within 1e-8
-relative L2 and the iteration counts within 2. Sizes above the array's
+sliced-ELL copy, or for these small systems its one-launch cooperative
+CG (other reduction orders): solutions agree within 1e-8 relative L2 and
+the iteration counts within 2; the bitwise comparisons ask dfx_cg_solve
+for its three-kernel iteration (classic = 1). Sizes above the array's
 maximum (1,048,576 scalar rows) are refused before any device work.
 """
 
@@ -24,9 +26,10 @@ from conftest import rel_err
 pytestmark = pytest.mark.gpu
 
 
-def _solve(name, A, b, rtol, jacobi=True):
+def _solve(name, A, b, rtol, jacobi=True, classic=0):
     x = la.Vector(b.array.shape[0], A.dtype)
-    opts = _lib.CGOpts(max_it=20000, rtol=rtol, jacobi=1 if jacobi else 0, check_every=16)
+    opts = _lib.CGOpts(max_it=20000, rtol=rtol, jacobi=1 if jacobi else 0, check_every=16,
+                       classic=classic)
     res = _lib.CGResult()
     _lib.call(name, A.handle, b.handle, x.handle, C.byref(opts), C.byref(res))
     return x.array, res
@@ -50,7 +53,7 @@ def test_noalloc_cg_matches_cg_two_calls(dtype, rtol, tol):
         assert abs(rn.iterations - ra.iterations) <= 2
         # same kernels: bitwise equal
         A.set_lanes(2)
-        xa2, ra2 = _solve("dfx_cg_solve", A, b, rtol)
+        xa2, ra2 = _solve("dfx_cg_solve", A, b, rtol, classic=1)
         xn2, rn2 = _solve("dfx_cg_solve_noalloc", A, b, rtol)
         assert rn2.iterations == ra2.iterations
         assert np.array_equal(xn2, xa2)
@@ -60,7 +63,7 @@ def test_noalloc_cg_matches_cg_two_calls(dtype, rtol, tol):
 def test_noalloc_cg_plain_cg_repeated():
     A, b = _system((10, 9, 8), 0, np.float64, seed=3)
     A.set_lanes(2)
-    xa, ra = _solve("dfx_cg_solve", A, b, 1e-10, jacobi=False)
+    xa, ra = _solve("dfx_cg_solve", A, b, 1e-10, jacobi=False, classic=1)
     for _ in range(2):
         xn, rn = _solve("dfx_cg_solve_noalloc", A, b, 1e-10, jacobi=False)
         assert rn.iterations == ra.iterations and np.array_equal(xn, xa)
