@@ -425,17 +425,16 @@ __device__ inline void q1_tab(const T xi[3], T phi[8], T dphi[8][3])
   }
 }
 
-/// General (non-affine) hexahedra: one thread per cell, 2x2x2 Gauss, the
-/// full 8x8 element matrix in registers, then the scatter.
-template <typename T, typename O, int MODE>
+/// Element matrices of the Q1 stiffness + reaction form on general
+/// (non-affine) hexahedra: one thread per cell, 2x2x2 Gauss, the 8x8
+/// matrix summed in registers over the quadrature points and stored
+/// (E: cells x 8 x 8, row-major); the row-owner element path then adds
+/// them in cell order, so the result does not depend on the schedule.
+template <typename T>
 __global__ void __launch_bounds__(kBlock)
-    q1hex_a_general_k(std::int32_t nc, const T* __restrict__ x,
-                      const std::int32_t* __restrict__ xdm, const std::int32_t* __restrict__ dm,
-                      const T* __restrict__ coeffs, int cstride, T creac,
-                      const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
-                      const O* __restrict__ slots, T* __restrict__ vals,
-                      const double* __restrict__ g, const T* __restrict__ x0, double alpha,
-                      T* __restrict__ b)
+    q1hex_elem_general_k(std::int32_t nc, const T* __restrict__ x,
+                         const std::int32_t* __restrict__ xdm, const T* __restrict__ coeffs,
+                         int cstride, T creac, T* __restrict__ E)
 {
   const std::int64_t c = blockIdx.x * std::int64_t(kBlock) + threadIdx.x;
   if (c >= nc)
@@ -471,27 +470,10 @@ __global__ void __launch_bounds__(kBlock)
         A[p][q] += ws * (h[p][0] * h[q][0] + h[p][1] * h[q][1] + h[p][2] * h[q][2])
                    + wm * phi[p] * phi[q];
   }
-  std::int32_t d[8];
-  for (int v = 0; v < 8; ++v)
-    d[v] = dm[8 * c + v];
+  T* out = E + 64 * c;
   for (int p = 0; p < 8; ++p)
     for (int q = 0; q < 8; ++q)
-    {
-      if constexpr (MODE == 0)
-      {
-        if (bc && (bc[d[p]] | bc[d[q]]))
-          continue;
-        atomicAdd(vals + rp[d[p]] + slots[c * 64 + 8 * p + q], A[p][q]);
-      }
-      else
-      {
-        if (!bc[d[q]])
-          continue;
-        const double xb = g ? g[d[q]] : 0.0;
-        const double xo = x0 ? double(x0[d[q]]) : 0.0;
-        atomicAdd(b + d[p], static_cast<T>(-double(A[p][q]) * alpha * (xb - xo)));
-      }
-    }
+      out[8 * p + q] = A[p][q];
 }
 
 /// Q1 hexahedra: L = f v with a Q1 coefficient (w = f at the vertices).
@@ -4731,6 +4713,28 @@ void vector_rows(const dfx_form& f, T* b)
   check_launch("assemble_vector");
 }
 
+/// The element matrices of a Q1 form on general hexahedra, computed into
+/// the form's element-matrix array (device).
+template <typename T>
+const T* general_hex_elements(dfx_form& f)
+{
+  const std::int32_t nc = f.d.num_cells;
+  if (!f.cell_elem)
+    f.cell_elem = dmalloc(sizeof(T) * std::max<std::int64_t>(std::int64_t(nc) * 64, 1));
+  if (nc > 0)
+    q1hex_elem_general_k<T><<<blocks_for(nc), kBlock, 0, stream()>>>(
+        nc, static_cast<const T*>(f.x), f.x_dofmap, static_cast<const T*>(f.coeffs),
+        f.d.cstride, static_cast<T>(f.constants[0]), static_cast<T*>(f.cell_elem));
+  check_launch("general hexahedra");
+  return static_cast<const T*>(f.cell_elem);
+}
+
+/// Q1 stiffness + reaction on general hexahedra.
+bool general_hex(const dfx_form& f)
+{
+  return f.d.kernel == DFX_K_Q1HEX_A && !f.affine && !f.host_kernel;
+}
+
 template <typename T, typename O, int MODE>
 void launch_bilinear(const dfx_form& f, const dfx_csr* A, const dfx_bc* bc, const T* x0,
                      double alpha, T* b)
@@ -4754,12 +4758,10 @@ void launch_bilinear(const dfx_form& f, const dfx_csr* A, const dfx_bc* bc, cons
         nc, x, f.x_dofmap, f.dofmap, co, f.d.cstride, creac, mark, rp, sl, vals, g, x0, alpha, b);
     break;
   case DFX_K_Q1HEX_A:
-    if (f.affine)
-      q1hex_a_affine_k<T, O, MODE><<<static_cast<int>((nc + 255) / 256), kBlock, 0, s>>>(
-          nc, x, f.x_dofmap, f.dofmap, co, f.d.cstride, creac, mark, rp, sl, vals, g, x0, alpha, b);
-    else
-      q1hex_a_general_k<T, O, MODE><<<blocks_for(nc), kBlock, 0, s>>>(
-          nc, x, f.x_dofmap, f.dofmap, co, f.d.cstride, creac, mark, rp, sl, vals, g, x0, alpha, b);
+    // general hexahedra go through their element matrices (general_hex_elements)
+    require(f.affine, "general hexahedra are assembled from their element matrices");
+    q1hex_a_affine_k<T, O, MODE><<<static_cast<int>((nc + 255) / 256), kBlock, 0, s>>>(
+        nc, x, f.x_dofmap, f.dofmap, co, f.d.cstride, creac, mark, rp, sl, vals, g, x0, alpha, b);
     break;
   default:
     throw Error(DFX_ERR_ARG, "no device kernel for this bilinear form yet");
@@ -5828,6 +5830,17 @@ dfx_status dfx_assemble_matrix(dfx_csr* A, const dfx_form* a, const dfx_bc* bc)
             in ? interior_rows<float>(f, *A, bc) : facet_matrix_rows<float>(f, *A, bc);
           return;
         }
+        if (general_hex(f))
+        {
+          // element matrices by quadrature, added by rows in cell order
+          if (A->dtype == DFX_F64)
+            elements_into_matrix<double>(f, *A, bc, general_hex_elements<double>(f));
+          else
+            elements_into_matrix<float>(f, *A, bc, general_hex_elements<float>(f));
+          A->zeroed = false;
+          A->zero_pending = false;
+          return;
+        }
         if (row_path(f))
         {
           // store mode (A zeroed) overwrites every entry; add mode needs
@@ -6248,6 +6261,18 @@ dfx_status dfx_apply_lifting(dfx_vec* b, const dfx_form* a, const dfx_bc* bc, co
           else
             facet_lift_rows<float>(f, *bc, x0 ? static_cast<const float*>(x0->data) : nullptr,
                                    alpha, static_cast<float*>(b->data));
+          return;
+        }
+        if (general_hex(f))
+        {
+          if (b->dtype == DFX_F64)
+            elements_lift<double>(f, static_cast<double*>(b->data), *bc,
+                                  general_hex_elements<double>(f),
+                                  x0 ? static_cast<const double*>(x0->data) : nullptr, alpha);
+          else
+            elements_lift<float>(f, static_cast<float*>(b->data), *bc,
+                                 general_hex_elements<float>(f),
+                                 x0 ? static_cast<const float*>(x0->data) : nullptr, alpha);
           return;
         }
         if (row_path(f))
