@@ -11,7 +11,9 @@ exterior facet, the gradient-jump penalty on an interior facet) through
 their C function pointers, and, for FP32, a ctypes kernel that calls the
 same oracle kernel in double. Checked against the oracle's serial facet
 assembly (FP64 within 1e-12, FP32 within 2e-6, patterns bit-exact) and
-against the device forms of the same integrals."""
+against the device forms of the same integrals; an interior facet kernel
+on Q1 hexahedra (written here) against the same kernel assembled on the
+host in facet order."""
 
 import ctypes as C
 
@@ -197,3 +199,69 @@ def test_kernel_facet_forms_reject_misuse():
     b = la.Vector(nd)
     with pytest.raises(DfxError, match="host kernel"):
         fem.apply_lifting(b, [aF], [[fem.dirichletbc(V, bn, g)]])
+
+
+# DOLFINx's hexahedron facets as local vertex numbers (facet f of the
+# reference cell), the first three spanning the face
+_HEX_FACETS = [(0, 1, 2, 3), (0, 1, 4, 5), (0, 2, 4, 6), (1, 3, 5, 7), (2, 3, 6, 7), (4, 5, 6, 7)]
+
+
+def _hex_face_tensor(X, lf0, lf1, gamma):
+    """A 16 x 16 interior facet tensor on the joint dofs (cell 0's 8, then
+    cell 1's 8): gamma |F| times a pattern coupling the face nodes of both
+    sides, different per local facet pair, so every index of the joint
+    layout is exercised."""
+    f0, f1 = _HEX_FACETS[lf0], _HEX_FACETS[lf1]
+    p = X[f0[0]]
+    area = np.linalg.norm(np.cross(X[f0[1]] - p, X[f0[2]] - p))
+    M = np.zeros((16, 16))
+    for u, a in enumerate(f0):
+        for v, b in enumerate(f1):
+            w = gamma * area * (1.0 + 0.1 * u + 0.01 * v + 0.001 * lf0 + 0.0001 * lf1)
+            M[a, 8 + b] += w
+            M[8 + b, a] += w
+            M[a, a] += 0.5 * w
+            M[8 + b, 8 + b] += 0.25 * w
+    return M
+
+
+def test_interior_kernel_on_hexahedra_matches_host_assembly():
+    """An interior facet integral on Q1 hexahedra given as a ctypes kernel:
+    the device path (both cells' geometry, local facets, joint dofmap, BC
+    rows and columns, insertion in facet order) against the same kernel
+    assembled on the host in facet order."""
+    n = (3, 3, 2)
+    pm = mesh.create_box([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], n, mesh.CellType.hexahedron)
+    V = fem.functionspace(pm, ("Lagrange", 1))
+    q = fem.interior_facet_domain(pm)
+    assert len(q) == 2 * 3 * 2 + 3 * 2 * 2 + 3 * 3 * 1   # faces between neighbours
+    gamma = 1.3
+
+    def k(A, w, c, xc, ent, perm, data):
+        X = np.ctypeslib.as_array(xc, (48,)).reshape(16, 3)
+        out = np.ctypeslib.as_array(A, (256,))
+        out[:] += _hex_face_tensor(X, ent[0], ent[1], c[0]).ravel()
+
+    kern = fem.KERNEL_F64(k)
+    aF = fem.kernel_facet_form(V, 2, kern, "interior_facet", q, constants=[gamma])
+    bn = fem.locate_boundary_dofs(V, 0, 0.0)
+    A = fem.assemble_matrix(aF, [fem.dirichletbc(V, bn)])
+    x = pm.geometry.x
+    xd = pm.geometry.dofmap
+    dm = V.dofmap.list
+    N = V.num_dofs
+    ref = np.zeros((N, N))
+    mark = np.zeros(N, bool)
+    mark[bn] = True
+    for c0, l0, c1, l1 in q:
+        X = np.concatenate([x[xd[c0]], x[xd[c1]]])
+        M = _hex_face_tensor(X, l0, l1, gamma)
+        joint = np.concatenate([dm[c0], dm[c1]])
+        M[mark[joint], :] = 0.0
+        M[:, mark[joint]] = 0.0
+        for i in range(16):
+            for jj in range(16):
+                ref[joint[i], joint[jj]] += M[i, jj]
+    ref[bn, bn] = 1.0
+    D = A.to_scipy().toarray()
+    assert np.abs(D - ref).max() <= 1e-14 * np.abs(ref).max()
