@@ -718,11 +718,44 @@ __device__ inline void affine_factors(const T* q0, const T* q1, const T* q2, con
   G[6] = creac * adet;
 }
 
+/// Compact record of a cell whose Jacobian is diagonal (G is then
+/// diagonal): G00, G11, G22 and c |det J|, 16-byte aligned.
+template <typename T>
+__device__ inline void store_compact(T* __restrict__ cd, std::int64_t c, const T (&g)[7])
+{
+  if constexpr (sizeof(T) == 8)
+  {
+    double2* G = reinterpret_cast<double2*>(cd + 4 * c);
+    G[0] = make_double2(g[0], g[1]);
+    G[1] = make_double2(g[2], g[6]);
+  }
+  else
+    *reinterpret_cast<float4*>(cd + 4 * c) = make_float4(g[0], g[1], g[2], g[6]);
+}
+
+/// The factors of a compact record (off-diagonal factors zero).
+template <typename T>
+__device__ inline void load_compact(const T* __restrict__ cd, std::int64_t c, T (&g)[7])
+{
+  if constexpr (sizeof(T) == 8)
+  {
+    const double2* p = reinterpret_cast<const double2*>(cd + 4 * c);
+    const double2 a = p[0], b = p[1];
+    g[0] = a.x; g[1] = a.y; g[2] = b.x; g[6] = b.y;
+  }
+  else
+  {
+    const float4 a = *reinterpret_cast<const float4*>(cd + 4 * c);
+    g[0] = a.x; g[1] = a.y; g[2] = a.z; g[6] = a.w;
+  }
+  g[3] = g[4] = g[5] = T(0);
+}
+
 /// Pass 1: the factor record of every cell (one thread per cell). With SAME
 /// the dofmap is the vertex map (P1/Q1 on the mesh's numbering): the
 /// vertex row, loaded with vector loads, also addresses the BC markers, so
 /// the marker and coordinate gathers are in flight together.
-template <typename T, int NV, bool SAME>
+template <typename T, int NV, bool SAME, bool COMPACT = false>
 __global__ void __launch_bounds__(kBlock)
     cell_factors_k(std::int32_t nc, const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
                    const std::int32_t* __restrict__ dm, int k, const T* __restrict__ coeffs,
@@ -767,6 +800,11 @@ __global__ void __launch_bounds__(kBlock)
   }
   if (mask)
     mask[c] = m;
+  if constexpr (COMPACT)
+  {
+    store_compact<T>(cd, c, g);
+    return;
+  }
   // the eighth slot carries the BC column mask bits (moved, never computed
   // with), so a row kernel reads factors and mask with one load; the record
   // is stored with full-width vector stores
@@ -2768,12 +2806,12 @@ __global__ void __launch_bounds__(kBlock)
 /// copy of the workgroup's CSR slice, so the slice is stored with coalesced
 /// non-temporal writes. Every entry of a row is one of its 27 slots
 /// (checked when the plan is built), so the slice needs no zero fill.
-template <typename T>
+template <typename T, bool COMPACT = false>
 __global__ void __launch_bounds__(kBlock)
     row_struct_lds_k(std::int32_t nrows, const std::int32_t* __restrict__ corner,
                      const std::uint8_t* __restrict__ slotmap, const T* __restrict__ cd,
-                     const std::int8_t* __restrict__ bc, const std::int32_t* __restrict__ rp,
-                     T* __restrict__ vals, int accumulate)
+                     const std::uint32_t* __restrict__ mask, const std::int8_t* __restrict__ bc,
+                     const std::int32_t* __restrict__ rp, T* __restrict__ vals, int accumulate)
 {
   extern __shared__ unsigned char smem_raw[];
   T* slice = reinterpret_cast<T*>(smem_raw);
@@ -2805,7 +2843,14 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int I = 0; I < 8; ++I)
     {
-      load_factors<T>(cd, cc[I] < 0 ? 0 : cc[I], g[I], m[I]);
+      const std::int32_t c = cc[I] < 0 ? 0 : cc[I];
+      if constexpr (COMPACT)
+      {
+        load_compact<T>(cd, c, g[I]);
+        m[I] = mask ? mask[c] : 0u;
+      }
+      else
+        load_factors<T>(cd, c, g[I], m[I]);
       if (cc[I] < 0 || rowbc)
         m[I] = 0xffu; // absent corner (or a Dirichlet row): nothing added
     }
@@ -2979,7 +3024,7 @@ __global__ void __launch_bounds__(kBlock)
 /// factor record and BC mask of the matrix, as cell_factors_k, and the
 /// element source vector, as cell_source_k, from one load of the cell's
 /// vertex row and coordinates.
-template <typename T, int NV>
+template <typename T, int NV, bool COMPACT = false>
 __global__ void __launch_bounds__(kBlock)
     cell_system_k(std::int32_t nc, const T* __restrict__ x, const std::int32_t* __restrict__ xdm,
                   const T* __restrict__ coeffs, int cstride, T creac,
@@ -3021,20 +3066,25 @@ __global__ void __launch_bounds__(kBlock)
   affine_source<T, NV>(q0, q1, q2, q3, f, out);
   if (mask)
     mask[c] = m;
-  const T g7 = mask_bits<T>(m);
-  if constexpr (sizeof(T) == 8)
-  {
-    double2* G = reinterpret_cast<double2*>(cd + 8 * c);
-    G[0] = make_double2(g[0], g[1]);
-    G[1] = make_double2(g[2], g[3]);
-    G[2] = make_double2(g[4], g[5]);
-    G[3] = make_double2(g[6], g7);
-  }
+  if constexpr (COMPACT)
+    store_compact<T>(cd, c, g);
   else
   {
-    float4* G = reinterpret_cast<float4*>(cd + 8 * c);
-    G[0] = make_float4(g[0], g[1], g[2], g[3]);
-    G[1] = make_float4(g[4], g[5], g[6], g7);
+    const T g7 = mask_bits<T>(m);
+    if constexpr (sizeof(T) == 8)
+    {
+      double2* G = reinterpret_cast<double2*>(cd + 8 * c);
+      G[0] = make_double2(g[0], g[1]);
+      G[1] = make_double2(g[2], g[3]);
+      G[2] = make_double2(g[4], g[5]);
+      G[3] = make_double2(g[6], g7);
+    }
+    else
+    {
+      float4* G = reinterpret_cast<float4*>(cd + 8 * c);
+      G[0] = make_float4(g[0], g[1], g[2], g[3]);
+      G[1] = make_float4(g[4], g[5], g[6], g7);
+    }
   }
 #pragma unroll
   for (int i = 0; i < NV; ++i)
@@ -3800,7 +3850,7 @@ void ensure_plan(dfx_form& f, const dfx_csr& A)
 }
 
 template <typename T>
-void launch_cell_factors(const dfx_form& f, const dfx_bc* bc)
+void launch_cell_factors(const dfx_form& f, const dfx_bc* bc, bool compact = false)
 {
   // done by the system cell pass of this assembly
   if (f.cell_data_ready)
@@ -3822,7 +3872,12 @@ void launch_cell_factors(const dfx_form& f, const dfx_bc* bc)
   };
   if (nc > 0)
   {
-    if (nv == 4)
+    if (compact)
+    {
+      require(nv == 8 && same, "compact records need Q1 on the vertex map");
+      go(cell_factors_k<T, 8, true, true>);
+    }
+    else if (nv == 4)
       same ? go(cell_factors_k<T, 4, true>) : go(cell_factors_k<T, 4, false>);
     else
       same ? go(cell_factors_k<T, 8, true>) : go(cell_factors_k<T, 8, false>);
@@ -4225,13 +4280,19 @@ void launch_row_assemble(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
     if constexpr (K == 8)
     {
       require(ensure_struct(f, A), "variant not available for this mesh");
-      launch_cell_factors<T>(f, bc);
+      const bool compact = f.compact_records;
+      launch_cell_factors<T>(f, bc, compact);
       const int R = 128;
       const std::size_t lds = std::size_t(R) * A.max_row * sizeof(T);
       const int nb = (A.nrows + R - 1) / R;
-      if (nb > 0)
+      const std::uint32_t* mk = bc ? f.cell_mask : nullptr;
+      if (nb > 0 && compact)
+        row_struct_lds_k<T, true><<<nb, R, lds, stream()>>>(
+            A.nrows, f.corner_cell, f.slotmap, static_cast<const T*>(f.cell_data), mk,
+            bc ? bc->marker : nullptr, A.row_ptr, static_cast<T*>(A.values), A.zeroed ? 0 : 1);
+      else if (nb > 0)
         row_struct_lds_k<T><<<nb, R, lds, stream()>>>(
-            A.nrows, f.corner_cell, f.slotmap, static_cast<const T*>(f.cell_data),
+            A.nrows, f.corner_cell, f.slotmap, static_cast<const T*>(f.cell_data), mk,
             bc ? bc->marker : nullptr, A.row_ptr, static_cast<T*>(A.values), A.zeroed ? 0 : 1);
       check_launch("row_assemble");
       return;
@@ -4894,6 +4955,27 @@ bool hex_all_affine(const dfx_form_desc& d)
   return true;
 }
 
+/// Whether every hexahedron's Jacobian (x1 - x0, x2 - x0, x4 - x0 as
+/// columns) is exactly diagonal: each edge from vertex 0 moves along one
+/// axis only.
+bool hex_all_axis_aligned(const dfx_form_desc& d)
+{
+  for (std::int64_t c = 0; c < d.num_cells; ++c)
+  {
+    const std::int32_t* v = d.x_dofmap + 8 * c;
+    const double* X0 = d.x + 3 * std::int64_t(v[0]);
+    const int col[3] = {1, 2, 4};
+    for (int a = 0; a < 3; ++a)
+    {
+      const double* Xa = d.x + 3 * std::int64_t(v[col[a]]);
+      for (int r = 0; r < 3; ++r)
+        if (r != a && Xa[r] - X0[r] != 0.0)
+          return false;
+    }
+  }
+  return true;
+}
+
 /// Device copies of the batch tables, grown as needed and reused.
 struct BatchTables
 {
@@ -5330,6 +5412,12 @@ dfx_status dfx_form_create(const dfx_form_desc* desc, dfx_form** out)
         for (int i = 0; i < d.num_constants; ++i)
           f->constants[i] = d.constants[i];
         f->affine = ki.cell == DFX_TET ? true : hex_all_affine(d);
+        f->compact_records = ki.cell != DFX_TET && f->affine && d.kernel == DFX_K_Q1HEX_A
+                             && d.dofs_per_cell == 8
+                             && (d.dofmap == d.x_dofmap
+                                 || std::memcmp(d.dofmap, d.x_dofmap,
+                                                sizeof(std::int32_t) * 8 * d.num_cells) == 0)
+                             && hex_all_axis_aligned(d);
         f->x = upload_as(d.x, std::int64_t(d.num_nodes) * 3, d.dtype);
         f->x_dofmap = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * nc * d.nodes_per_cell));
         h2d(f->x_dofmap, d.x_dofmap, sizeof(std::int32_t) * nc * d.nodes_per_cell);
@@ -6341,7 +6429,10 @@ void launch_cell_system(const dfx_form& a, const dfx_form& L, const dfx_bc* bc)
         static_cast<T*>(a.cell_data), bc ? a.cell_mask : nullptr,
         static_cast<const T*>(L.coeffs), L.d.cstride, static_cast<T*>(L.cell_vec));
   };
-  if (a.d.nodes_per_cell == 4)
+  // the records variant 15 reads when the form keeps compact ones
+  if (a.d.nodes_per_cell == 8 && a.row_variant == 15 && a.compact_records)
+    go(cell_system_k<T, 8, true>);
+  else if (a.d.nodes_per_cell == 4)
     go(cell_system_k<T, 4>);
   else
     go(cell_system_k<T, 8>);

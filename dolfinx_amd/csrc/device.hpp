@@ -246,6 +246,11 @@ struct dfx_form
   std::vector<std::int32_t> h_dofmap;
   double constants[8] = {0};
   bool affine = true;         // every cell an affine image of the reference
+  // Q1 hexahedra whose Jacobians are all exactly diagonal (edges along the
+  // axes) and whose dofmap is the vertex map: variant 15 then keeps 4
+  // values per cell record (the diagonal of G and c |det J|), the BC
+  // column masks in cell_mask
+  bool compact_records = false;
   std::int32_t max_dof = -1;  // largest node index in the dofmap
   void* x = nullptr;          // device, num_nodes x 3 of dtype
   std::int32_t* x_dofmap = nullptr;
