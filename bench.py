@@ -35,8 +35,9 @@ HBM_PEAK_GBS = 8000.0
 # kernel, store mode) on the 128^3 hexahedra, per row-kernel variant, from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this bench with --variant V,
 # scaled by the counters' reading for a 1 GiB device copy (profiles/
-# q1hex128_assembly_spmv_cg.md gives the commands and the per-kernel figures).
-ASSEMBLY_TRAFFIC_BYTES = {1: 1335891297, 4: 1331244534, 15: 1066465075}
+# q1hex128_assembly_spmv_cg.md gives the commands and the per-kernel figures;
+# variant 15 with the 4-value records of an axis-aligned box).
+ASSEMBLY_TRAFFIC_BYTES = {1: 1335891297, 4: 1331244534, 15: 920051360}
 
 
 def ensure_built():
@@ -673,7 +674,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "cell_factors_k + row_assemble_k (FP64 Q1 matrix assembly, one dfx_assemble_matrix call)",
+            "kernel": "cell_factors_k + row kernel of the kept variant (FP64 Q1 matrix assembly, one dfx_assemble_matrix call)",
             "achieved": round(asm_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
