@@ -1570,7 +1570,7 @@ __global__ void __launch_bounds__(kBlock)
 
 /// Lifting b[r] -= alpha sum_{(c,i) at r} sum_{j in BC} A_ij (g_j - x0_j),
 /// one thread per row, over cells whose mask marks a BC column.
-template <typename T, int K>
+template <typename T, int K, bool COMPACT = false>
 __global__ void __launch_bounds__(kBlock)
     row_lift_k(std::int32_t nplan, const std::int32_t* __restrict__ inc_off,
                const std::int32_t* __restrict__ inc, const std::int32_t* __restrict__ dm,
@@ -1582,27 +1582,54 @@ __global__ void __launch_bounds__(kBlock)
   if (r >= nplan)
     return;
   double s = 0.0;
-  for (std::int32_t p = inc_off[r]; p < inc_off[r + 1]; ++p)
+  const std::int32_t p0 = inc_off[r], p1 = inc_off[r + 1];
+  // eight incidences at a time: their cells and masks loaded together
+  // (clamped positions), then the cells with a Dirichlet column in order
+  for (std::int32_t pb = p0; pb < p1; pb += 8)
   {
-    const std::int32_t cik = inc[p];
-    const std::int32_t c = cik / K;
-    const std::uint32_t m = mask[c];
-    if (!m)
-      continue;
-    const int i = cik - c * K;
-    const T* G = cd + 8 * std::int64_t(c);
-    const T* Si = ref + i * K;
-    for (int j = 0; j < K; ++j)
+    std::int32_t cq[8];
+    std::uint32_t mq[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      cq[u] = inc[min(pb + u, p1 - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      mq[u] = mask[cq[u] / K];
+    for (int u = 0; u < 8 && pb + u < p1; ++u)
     {
-      if (!((m >> j) & 1u))
+      const std::uint32_t m = mq[u];
+      if (!m)
         continue;
-      const T v = G[0] * Si[j] + G[1] * Si[K * K + j] + G[2] * Si[2 * K * K + j]
-                  + G[3] * Si[3 * K * K + j] + G[4] * Si[4 * K * K + j]
-                  + G[5] * Si[5 * K * K + j] + G[6] * Si[6 * K * K + j];
-      const std::int32_t col = dm[std::int64_t(c) * K + j];
-      const double gv = g ? g[col] : 0.0;
-      const double xv = x0 ? double(x0[col]) : 0.0;
-      s += double(v) * (gv - xv);
+      const std::int32_t cik = cq[u];
+      const std::int32_t c = cik / K;
+      const int i = cik - c * K;
+      T G[7];
+      if constexpr (COMPACT)
+      {
+        const T* rc = cd + 4 * std::int64_t(c);
+        G[0] = rc[0]; G[1] = rc[1]; G[2] = rc[2]; G[6] = rc[3];
+        G[3] = G[4] = G[5] = T(0);
+      }
+      else
+      {
+        const T* rc = cd + 8 * std::int64_t(c);
+#pragma unroll
+        for (int q = 0; q < 7; ++q)
+          G[q] = rc[q];
+      }
+      const T* Si = ref + i * K;
+      for (int j = 0; j < K; ++j)
+      {
+        if (!((m >> j) & 1u))
+          continue;
+        const T v = G[0] * Si[j] + G[1] * Si[K * K + j] + G[2] * Si[2 * K * K + j]
+                    + G[3] * Si[3 * K * K + j] + G[4] * Si[4 * K * K + j]
+                    + G[5] * Si[5 * K * K + j] + G[6] * Si[6 * K * K + j];
+        const std::int32_t col = dm[std::int64_t(c) * K + j];
+        const double gv = g ? g[col] : 0.0;
+        const double xv = x0 ? double(x0[col]) : 0.0;
+        s += double(v) * (gv - xv);
+      }
     }
   }
   if (s != 0.0)
@@ -4529,12 +4556,18 @@ void assemble_rows(dfx_form& f, dfx_csr& A, const dfx_bc* bc)
 template <typename T>
 void lift_rows(dfx_form& f, const dfx_bc& bc, const T* x0, double alpha, T* b)
 {
-  launch_cell_factors<T>(f, &bc);
+  // axis-aligned Q1 boxes: the 4-value records (the masks are always
+  // written here, a BC being given)
+  const bool compact = f.compact_records && f.d.dofs_per_cell == 8;
+  launch_cell_factors<T>(f, &bc, compact);
   const std::int32_t nplan = f.max_dof + 1;
   const T* cd = static_cast<const T*>(f.cell_data);
   const T* ref = static_cast<const T*>(f.ref_tensors);
   if (f.d.dofs_per_cell == 4)
     row_lift_k<T, 4><<<blocks_for(nplan), kBlock, 0, stream()>>>(
+        nplan, f.inc_off, f.inc, f.dofmap, cd, f.cell_mask, ref, bc.g, x0, alpha, b);
+  else if (compact)
+    row_lift_k<T, 8, true><<<blocks_for(nplan), kBlock, 0, stream()>>>(
         nplan, f.inc_off, f.inc, f.dofmap, cd, f.cell_mask, ref, bc.g, x0, alpha, b);
   else if (f.d.dofs_per_cell == 8)
     row_lift_k<T, 8><<<blocks_for(nplan), kBlock, 0, stream()>>>(
