@@ -3677,52 +3677,66 @@ __global__ void __launch_bounds__(kBlock)
   const std::int32_t r = static_cast<std::int32_t>(t / 3);
   const int comp = static_cast<int>(t - 3 * std::int64_t(r));
   double s = 0.0;
-  for (std::int32_t p = inc_off[r]; p < inc_off[r + 1]; ++p)
+  const std::int32_t p0 = inc_off[r], p1 = inc_off[r + 1];
+  // eight incidences at a time: their cells and masks loaded together
+  // (clamped positions), then the cells with a Dirichlet column in order
+  for (std::int32_t pb = p0; pb < p1; pb += 8)
   {
-    const std::int32_t cik = inc[p];
-    const std::int32_t c = cik / K;
-    const std::uint32_t m = mask[c];
-    if (!m)
-      continue;
-    const int i = cik - c * K;
-    const T* rec = cd + 16 * std::int64_t(c);
-    T Ki[9];
+    std::int32_t cq[8];
+    std::uint32_t mq[8];
 #pragma unroll
-    for (int q = 0; q < 9; ++q)
-      Ki[q] = rec[q];
-    const T lam = rec[10], mu = rec[11];
-    for (int j = 0; j < K; ++j)
+    for (int u = 0; u < 8; ++u)
+      cq[u] = inc[min(pb + u, p1 - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      mq[u] = mask[cq[u] / K];
+    for (int u = 0; u < 8 && pb + u < p1; ++u)
     {
-      const std::uint32_t colb = (m >> (3 * j)) & 7u;
-      if (!colb)
+      const std::int32_t cik = cq[u];
+      const std::int32_t c = cik / K;
+      const std::uint32_t m = mq[u];
+      if (!m)
         continue;
-      const T* Tm = ref + (i * K + j) * 10;
-      T W[9], P[9];
+      const int i = cik - c * K;
+      const T* rec = cd + 16 * std::int64_t(c);
+      T Ki[9];
 #pragma unroll
-      for (int aa = 0; aa < 3; ++aa)
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-          W[aa * 3 + d] = Tm[aa * 3 + 0] * Ki[0 * 3 + d] + Tm[aa * 3 + 1] * Ki[1 * 3 + d]
-                          + Tm[aa * 3 + 2] * Ki[2 * 3 + d];
-#pragma unroll
-      for (int cc = 0; cc < 3; ++cc)
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-          P[cc * 3 + d] = Ki[0 * 3 + cc] * W[0 * 3 + d] + Ki[1 * 3 + cc] * W[1 * 3 + d]
-                          + Ki[2 * 3 + cc] * W[2 * 3 + d];
-      const T tr = P[0] + P[4] + P[8];
-      const std::int64_t node = dm[std::int64_t(c) * K + j];
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
+      for (int q = 0; q < 9; ++q)
+        Ki[q] = rec[q];
+      const T lam = rec[10], mu = rec[11];
+      for (int j = 0; j < K; ++j)
       {
-        if (!((colb >> d) & 1u))
+        const std::uint32_t colb = (m >> (3 * j)) & 7u;
+        if (!colb)
           continue;
-        const T v = comp == d ? mu * (tr + P[comp * 4]) + lam * P[comp * 4]
-                              : mu * P[d * 3 + comp] + lam * P[comp * 3 + d];
-        const std::int64_t col = 3 * node + d;
-        const double gv = g ? g[col] : 0.0;
-        const double xv = x0 ? double(x0[col]) : 0.0;
-        s += double(v) * (gv - xv);
+        const T* Tm = ref + (i * K + j) * 10;
+        T W[9], P[9];
+#pragma unroll
+        for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            W[aa * 3 + d] = Tm[aa * 3 + 0] * Ki[0 * 3 + d] + Tm[aa * 3 + 1] * Ki[1 * 3 + d]
+                            + Tm[aa * 3 + 2] * Ki[2 * 3 + d];
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            P[cc * 3 + d] = Ki[0 * 3 + cc] * W[0 * 3 + d] + Ki[1 * 3 + cc] * W[1 * 3 + d]
+                            + Ki[2 * 3 + cc] * W[2 * 3 + d];
+        const T tr = P[0] + P[4] + P[8];
+        const std::int64_t node = dm[std::int64_t(c) * K + j];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+        {
+          if (!((colb >> d) & 1u))
+            continue;
+          const T v = comp == d ? mu * (tr + P[comp * 4]) + lam * P[comp * 4]
+                                : mu * P[d * 3 + comp] + lam * P[comp * 3 + d];
+          const std::int64_t col = 3 * node + d;
+          const double gv = g ? g[col] : 0.0;
+          const double xv = x0 ? double(x0[col]) : 0.0;
+          s += double(v) * (gv - xv);
+        }
       }
     }
   }
