@@ -2756,9 +2756,10 @@ __global__ void __launch_bounds__(kBlock)
         seen |= 1u << v;
       ++count;
     }
-    slotmap[std::int64_t(r) * 28 + q] = v < 0 ? std::uint8_t(255) : static_cast<std::uint8_t>(v);
+    slotmap[std::int64_t(r) * 32 + q] = v < 0 ? std::uint8_t(255) : static_cast<std::uint8_t>(v);
   }
-  slotmap[std::int64_t(r) * 28 + 27] = 255;
+  for (int q = 27; q < 32; ++q)
+    slotmap[std::int64_t(r) * 32 + q] = 255;
   if (count != len)
     atomicOr(err, 8);
 }
@@ -2811,7 +2812,7 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
   }
-  const std::uint32_t* sm = reinterpret_cast<const std::uint32_t*>(slotmap + 28 * std::int64_t(r));
+  const std::uint32_t* sm = reinterpret_cast<const std::uint32_t*>(slotmap + 32 * std::int64_t(r));
   std::uint32_t w[7];
 #pragma unroll
   for (int k = 0; k < 7; ++k)
@@ -2854,11 +2855,13 @@ __global__ void __launch_bounds__(kBlock)
     const dfx_i32x4 ca = __builtin_nontemporal_load(cp);
     const dfx_i32x4 cb = __builtin_nontemporal_load(cp + 1);
     const std::int32_t cc[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
-    const std::uint32_t* smp = reinterpret_cast<const std::uint32_t*>(slotmap + 28 * std::int64_t(r));
-    std::uint32_t w[7];
-#pragma unroll
-    for (int k = 0; k < 7; ++k)
-      w[k] = __builtin_nontemporal_load(smp + k);
+    // the row's 27 slot offsets, padded to 32 bytes: two 16-byte loads
+    const dfx_i32x4* smp = reinterpret_cast<const dfx_i32x4*>(slotmap + 32 * std::int64_t(r));
+    const dfx_i32x4 sa = __builtin_nontemporal_load(smp);
+    const dfx_i32x4 sb = __builtin_nontemporal_load(smp + 1);
+    const std::uint32_t w[7] = {std::uint32_t(sa.x), std::uint32_t(sa.y), std::uint32_t(sa.z),
+                                std::uint32_t(sa.w), std::uint32_t(sb.x), std::uint32_t(sb.y),
+                                std::uint32_t(sb.z)};
     const bool rowbc = bc && bc[r];
     const std::int32_t roff = rp[r] - base;
     T acc[27];
@@ -4115,7 +4118,7 @@ bool ensure_struct(dfx_form& f, const dfx_csr& A)
   if (f.d.kernel != DFX_K_Q1HEX_A || !f.affine || f.max_dof + 1 != nrows || A.max_row > 27)
     return false;
   f.corner_cell = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * 8 * std::max(nrows, 1)));
-  f.slotmap = static_cast<std::uint8_t*>(dmalloc(28 * std::size_t(std::max(nrows, 1))));
+  f.slotmap = static_cast<std::uint8_t*>(dmalloc(32 * std::size_t(std::max(nrows, 1))));
   std::int32_t* tmp = static_cast<std::int32_t*>(dmalloc(sizeof(std::int32_t) * 27 * std::size_t(std::max(nrows, 1))));
   int* err = static_cast<int*>(dmalloc(sizeof(int)));
   DFX_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
